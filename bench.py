@@ -1,0 +1,213 @@
+"""Benchmark of the Zebraflow LBM time step on one MI355X.
+
+One "step" is one LBM iteration (stream + collide + boundary) over the whole
+fluid-node list of the workload, inputs resident in HBM.  Prints ONE JSON
+line (fluid MLUPS of the FP32-deviation form) with:
+  - fp64_form: the plain FP64-on-GPU form on the same workload and ratio;
+  - error: relative L2 velocity error of the FP32-deviation form against the
+    FP64 form after `--err-steps` steps (the FP64 form agrees with the FP64
+    oracle to 1e-13 in tests/test_gpu_lbm.py);
+  - roofline: the stream-collide kernel's algorithmic bytes per launch over
+    its average launch time (HIP events on the launch stream);
+  - cpu_baseline: the reference-structured OpenMP CPU step (oracle FP32) on a
+    bounded sample of the same workload, rank 0 only.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+STORE_NAMES = {"f64": 0, "f32dev": 1, "f16dev": 2}
+STORE_BYTES = {"f64": 8, "f32dev": 4, "f16dev": 2}
+
+# Workloads (box, generator args).  vessel_pair_15: the vessel-pair topology
+# (DA + PCV, sinusoidal centrelines of amplitude 8, 6 connectors) with radii
+# scaled so the fluid fraction is 15 % of the 512x128x128 box (about 1.26 M
+# fluid nodes); vessel_pair_r10: the same with radius 10 / 4 (3.9 % fluid).
+WORKLOADS = {
+    "vessel_pair_15": dict(box=(512, 128, 128), kind="pair", r_main=19.8, r_conn=7.9, amp=8.0,
+                           n_conn=6, seed=20261205),
+    "vessel_pair_r10": dict(box=(512, 128, 128), kind="pair", r_main=10.0, r_conn=4.0, amp=8.0,
+                            n_conn=6, seed=20261205),
+    "tube_cfg1": dict(box=(32, 32, 128), kind="tube", r=14.0),
+    "tree_cfg5": dict(box=(1024, 256, 256), kind="tree", r_root=24.0, ratio=0.79, gens=4,
+                      a_lo=30.0, a_hi=45.0, seed=20261207),
+}
+
+
+def make_geometry(name):
+    from tests import geometry as g  # the product's C generators (libzfhost.so)
+    w = WORKLOADS[name]
+    if w["kind"] == "pair":
+        return g.vessel_pair(w["box"], w["r_main"], w["r_conn"], w["amp"], w["n_conn"], w["seed"])
+    if w["kind"] == "tube":
+        return g.tube(w["box"], axis=2, r=w["r"], periodic=True)
+    return g.tree(w["box"], w["r_root"], w["ratio"], w["gens"], w["a_lo"], w["a_hi"], w["seed"])
+
+
+def engine_for(name, b, store):
+    from zebraflow_amd.engine import Engine, lattice_config
+    w = WORKLOADS[name]
+    if w["kind"] == "tube":
+        # cfg1: tau 0.8, body force giving a peak velocity of 0.02 (g = 4 nu u / R^2)
+        nu = 0.3 / 3
+        cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], periodic=(0, 0, 1),
+                             body_force=(0, 0, 4 * nu * 0.02 / w["r"] ** 2))
+    else:
+        # pulsatile inlet u(t) = 0.03 (1 + 0.5 sin(2 pi t / 4000)) along +i
+        cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], inlet_u0=0.03,
+                             inlet_amp=0.5, inlet_period=4000.0, inlet_dir=(1, 0, 0))
+    eng = Engine(cfg)
+    eng.upload_lattice(b)
+    return eng
+
+
+def time_form(name, b, store, steps, warmup):
+    eng = engine_for(name, b, store)
+    eng.run(0, warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.record(0)
+    eng.run(warmup, steps)
+    eng.record(1)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    ms = eng.elapsed_ms(0, 1)
+    return eng, ms, wall
+
+
+def kernel_roofline(eng, store, n_int, steps):
+    t = eng.time_kernels(10 ** 6, steps)
+    avg_ms = t[0] / t[2]
+    algo = 2 * 19 * STORE_BYTES[store] * n_int
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_stream_collide", avg_us=round(avg_ms * 1e3, 3),
+                algorithmic_bytes_per_launch=algo,
+                boundary_kernel_avg_us=round(t[1] / steps * 1e3, 3))
+
+
+def velocity_error(name, b, steps):
+    """Both forms from the same equilibrium start; rel. L2 velocity error."""
+    out = {}
+    for store in ("f64", "f32dev"):
+        eng = engine_for(name, b, store)
+        eng.run(0, steps)
+        out[store] = eng.macro()
+        eng.close()
+    u64, u32 = out["f64"][1:4], out["f32dev"][1:4]
+    return float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+
+
+def cpu_baseline(name, b, seconds):
+    """Reference-structured OpenMP step (oracle, FP32 storage) on the same
+    workload; as many whole steps as fit in about `seconds`."""
+    from oracle import pyoracle as po
+    w = WORKLOADS[name]
+    if w["kind"] == "tube":
+        params = dict(wall_mode=1, visc_scale=0.3, force_dt=1.0, periodic=(0, 0, 1),
+                      body_force=(0, 0, 4 * 0.1 * 0.02 / w["r"] ** 2))
+    else:
+        params = dict(wall_mode=1, visc_scale=0.3, force_dt=1.0, inlet_u0=0.03, inlet_amp=0.5,
+                      inlet_period=4000.0, inlet_dir=(1, 0, 0), outlet_rho=1.0)
+    o = po.LbmOracle(b, params, dtype=np.float32)
+    n_int = int(((o.bnode() > 0) & (o.bnode() < 10)).sum())
+    threads = po.max_threads()
+    o.step(1)  # warm-up (first touch)
+    t0 = po.wtime()
+    n = 0
+    while True:
+        o.step(2 + n)
+        n += 1
+        el = po.wtime() - t0
+        if el >= seconds or n >= 2000:
+            break
+    return dict(value=round(n_int * n / el / 1e6, 3), unit="MLUPS", cores=threads,
+                kind="port",
+                sample=f"{n} steps of the full {name} lattice ({n_int} fluid nodes), FP32 "
+                       f"reference loop structure (F/F_OLD/FEQ arrays, DOMID lookups, "
+                       f"schedule(static,5000)), {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="vessel_pair_15", choices=sorted(WORKLOADS))
+    ap.add_argument("--store", default="f32dev", choices=sorted(STORE_NAMES))
+    ap.add_argument("--err-steps", type=int, default=10000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip FP64 form and error legs")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this engine runs the time step on a single MI355X "
+              "(multi-GPU slab decomposition is a design note, not built); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    b = make_geometry(args.workload)
+    n_fluid = int(((b > 0) & (b < 10)).sum())
+    eng, ms, wall = time_form(args.workload, b, args.store, args.steps, args.warmup)
+    n_int = eng.n_interior
+    assert n_int == n_fluid
+    mlups = n_int * args.steps / (ms * 1e-3) / 1e6
+    roof = kernel_roofline(eng, args.store, n_int, min(args.steps, 500))
+    info = eng.info()
+    eng.close()
+
+    extra = {}
+    if not args.no_extra:
+        e64, ms64, _ = time_form(args.workload, b, "f64", args.steps, args.warmup)
+        mlups64 = n_int * args.steps / (ms64 * 1e-3) / 1e6
+        roof64 = kernel_roofline(e64, "f64", n_int, min(args.steps, 500))
+        e64.close()
+        extra["fp64_form"] = dict(value=round(mlups64, 1), unit="MLUPS",
+                                  ms_per_step=round(ms64 / args.steps, 5),
+                                  kernel_avg_us=roof64["avg_us"], kernel_frac=roof64["frac"])
+        extra["speedup_vs_fp64_form"] = round(mlups / mlups64, 3)
+        extra["error"] = dict(vel_rel_l2_vs_fp64=velocity_error(args.workload, b, args.err_steps),
+                              steps=args.err_steps)
+
+    line = {
+        "metric": "fluid MLUPS (FP32-deviation D3Q19 step) + % HBM roofline vs FP64 form",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.store != "f64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "box": list(b.shape), "fluid_nodes": n_int,
+                   "listed_nodes": info["dommax"], "storage": args.store,
+                   "fluid_fraction": round(n_int / b.size, 4), "tau": 0.8,
+                   "wall": "halfway bounce-back",
+                   "inlet": "u(t)=0.03(1+0.5 sin 2 pi t/4000)" if "pair" in args.workload else None,
+                   "chords": info["n_chords"]},
+        "wall_clock_s": round(wall, 4),
+        "roofline": roof,
+    }
+    line.update(extra)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.workload, b, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

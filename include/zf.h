@@ -1,0 +1,151 @@
+/*
+ * zf.h -- C ABI of libzfgpu.so, the MI355X engine for the Zebraflow time step.
+ *
+ * The reference (ZebraflowWT.c) has no operator API: its time step is a
+ * sequence of inline OpenMP regions in main() over static, 1-based,
+ * structure-of-arrays globals (ZebraflowWT.c:5646-12412).  The entry points
+ * below replace those regions one for one; each comment names the region.
+ *
+ * Conventions
+ *  - Every call returns ZF_OK (0) or a negative ZF_ERR_* code; the message of
+ *    the last failure is returned by zf_last_error().  Nothing aborts.
+ *  - All host arrays are caller-owned; the library never keeps a host pointer.
+ *  - The box array b_index is dense, element (i,j,k) (1-based lattice
+ *    indices) at ((i-1)*jm + (j-1))*km + (k-1): the reference's
+ *    B_index[i][j][k] without its padding row (ZebraflowWT.c:180).
+ *  - List outputs are 1-based, as the reference stores them.
+ *  - Per-node host arrays are indexed by p = DOMID-1 (the reference's list
+ *    order); populations are SoA f[c*dommax + p], c = 0..18 for the
+ *    reference's directions 1..19 (ZebraflowWT.c:2471-2504).
+ *  - One host thread per context.  All device work is ordered on the
+ *    context's single HIP stream; calls that return host data synchronise it.
+ */
+#ifndef ZF_H
+#define ZF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ZF_OK = 0,
+  ZF_ERR_ARG = -1,      /* bad argument or size */
+  ZF_ERR_HIP = -2,      /* a HIP runtime call failed */
+  ZF_ERR_STATE = -3,    /* call out of order (e.g. step before lattice) */
+  ZF_ERR_GEOMETRY = -4, /* lattice rejected (interior node next to B<0) */
+  ZF_ERR_NOMEM = -5     /* device allocation failed */
+};
+
+/* Population storage forms. */
+enum {
+  ZF_STORE_F64 = 0,    /* FP64 populations f_i (the plain comparator form) */
+  ZF_STORE_F32DEV = 1, /* FP32 deviations f_i - w_i, FP32 arithmetic */
+  ZF_STORE_F16DEV = 2  /* FP16 (f_i - w_i)/w_i, FP32 arithmetic */
+};
+
+/* Wall treatment for B==0 nodes. */
+enum {
+  ZF_WALL_EQUILIBRIUM = 0, /* reference: wall node = feq(D[nbr], 0), ZebraflowWT.c:6309-6315 */
+  ZF_WALL_HALFWAY_BB = 1   /* halfway bounce-back on links from B==0 nodes */
+};
+
+/* Boundary codes in b_index beyond the reference's (0 wall, 1..9 fluid with
+ * tau = 0.5 + visc_scale*B, 1001..1009 pressure waveforms). */
+#define ZF_B_VELOCITY_INLET 2001 /* feq(D[nbr], u_in(t)*inlet_dir) */
+#define ZF_B_PRESSURE_OUTLET 3001 /* feq(outlet_rho, U[nbr]) */
+
+typedef struct zf_config {
+  int32_t im, jm, km;     /* box (ZebraflowWT.c:81-83 IM, JM, KM) */
+  int32_t periodic[3];    /* wrap the pull along i, j, k */
+  int32_t storage;        /* ZF_STORE_* */
+  int32_t wall_mode;      /* ZF_WALL_* */
+  int32_t device;         /* HIP device ordinal */
+  int32_t reserved;
+  double visc_scale;      /* tau = 0.5 + visc_scale*B; reference MU_SCALE*MU = 0.72 */
+  double force_dt;        /* DT in the Guo force terms; reference 0.5 */
+  double body_force[3];   /* uniform force added to the per-node field */
+  double inlet_u0, inlet_amp, inlet_period; /* u_in = u0*(1+amp*sin(2 pi iter/period)) */
+  double inlet_dir[3];
+  double outlet_rho;
+  double t_scale;         /* seconds per step for codes 1001-1009 (T_SCALE) */
+  double pressure_scale;  /* LBM_PRESSURE_SCALE */
+} zf_config;
+
+/* Reference units and constants (ZebraflowWT.c:34-74): visc_scale 0.72,
+ * force_dt 0.5, equilibrium walls, T_SCALE 5e-7 s. */
+void zf_config_reference(zf_config *cfg, int32_t im, int32_t jm, int32_t km);
+/* Lattice units: tau for B==1, force_dt 1, halfway bounce-back. */
+void zf_config_lattice(zf_config *cfg, int32_t im, int32_t jm, int32_t km, double tau);
+
+typedef struct zf_ctx zf_ctx;
+
+int zf_create(const zf_config *cfg, zf_ctx **out);
+int zf_destroy(zf_ctx *ctx);
+const char *zf_last_error(const zf_ctx *ctx);
+const char *zf_version(void);
+
+/* ---- index lists ---------------------------------------------------------
+ * Builds, on the device, the fluid-node list (LBM_DOM_SETUP,
+ * ZebraflowWT.c:1481-1502), the BC list and BC-neighbour list
+ * (LBM_BC_SETUP, ZebraflowWT.c:1506-1589) and the pull-neighbour list
+ * (ZebraflowWT.c:6113-6125), then the kernel's run-length form of it.
+ * Counts are returned through the (nullable) out-params. */
+int zf_upload_lattice(zf_ctx *ctx, const int32_t *b_index, int32_t *dommax,
+                      int32_t *bcmax, int32_t *n_interior);
+/* DOMLIST dommax*3, BCLIST bcmax*4, BCNLIST bcmax*3 (any may be NULL). */
+int zf_download_lists(zf_ctx *ctx, int32_t *domlist, int32_t *bclist, int32_t *bcnlist);
+/* DOMID over the box (1-based l or -1). */
+int zf_download_domid(zf_ctx *ctx, int32_t *domid);
+/* nbr[c*dommax + p] = DOMID of (i,j,k) - e_c, c = 0..17 (1-based, -1 none). */
+int zf_download_neighbours(zf_ctx *ctx, int32_t *nbr);
+
+typedef struct zf_lattice_info {
+  int32_t dommax, bcmax, n_interior, n_chords, n_bc_active, n_bb_links;
+  int32_t n_macro_nodes, stride;
+  int64_t device_bytes;
+} zf_lattice_info;
+int zf_get_lattice_info(zf_ctx *ctx, zf_lattice_info *info);
+
+/* ---- LBM state -------------------------------------------------------------
+ * Populations: f = feq(rho, u) everywhere, as the reference initialises them
+ * (ZebraflowWT.c:5253-5262). */
+int zf_init_equilibrium(zf_ctx *ctx, double rho, const double u[3]);
+int zf_upload_populations(zf_ctx *ctx, const double *f);
+int zf_download_populations(zf_ctx *ctx, double *f);
+/* Per-node force field BODYFORCE (ZebraflowWT.c:183), 3*dommax SoA; NULL
+ * clears it. */
+int zf_upload_force(zf_ctx *ctx, const double *force);
+int zf_download_force(zf_ctx *ctx, double *force);
+
+/* ---- the time step -------------------------------------------------------
+ * One LBM iteration: stream + macro + BGK-Guo collide (ZebraflowWT.c:6101-6145),
+ * boundary conditions (:6292-6433) and the copy (:6601-6626, a buffer swap
+ * here).  rbc_step != 0 clears the force field afterwards, as :6622-6624. */
+int zf_lbm_step(zf_ctx *ctx, int32_t iter, int32_t rbc_step);
+/* nsteps iterations iter0+1 .. iter0+nsteps with rbc_step = 0. */
+int zf_lbm_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps);
+/* D, U, V, W of every listed node from the current populations without the
+ * force shift (ZebraflowWT.c:6606-6614); each array dommax (any may be NULL). */
+int zf_download_macro(zf_ctx *ctx, double *d, double *u, double *v, double *w);
+/* Mass sum_p sum_c f and momentum over the interior nodes in FP64, reduced
+ * on the device; out[0..3] = mass, jx, jy, jz. */
+int zf_conservation_sums(zf_ctx *ctx, double out[4]);
+
+/* Per-kernel timing pass: runs nsteps iterations iter0+1.. like zf_lbm_run,
+ * bracketing every launch with HIP events on the context stream.
+ * out_ms[0] = total ms in the stream-collide kernel, out_ms[1] = total ms in
+ * the boundary kernel, out_ms[2] = launches of the stream-collide kernel. */
+int zf_lbm_time_kernels(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double out_ms[3]);
+
+/* ---- timing and synchronisation ------------------------------------------ */
+int zf_sync(zf_ctx *ctx);
+/* Record HIP event `slot` (0..15) on the context stream. */
+int zf_event_record(zf_ctx *ctx, int32_t slot);
+int zf_event_elapsed(zf_ctx *ctx, int32_t slot_a, int32_t slot_b, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+"""ctypes bindings of liboracle.so, the CPU restatement (TEST INFRASTRUCTURE).
+
+Only tests/, the smoke entry point and bench.py's CPU-baseline leg import
+this module.  See zf_oracle.h for the functions and the reference lines they
+restate.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+class LbmParams(C.Structure):
+    _fields_ = [
+        ("im", C.c_int), ("jm", C.c_int), ("km", C.c_int), ("periodic", C.c_int * 3),
+        ("wall_mode", C.c_int), ("visc_scale", C.c_double), ("force_dt", C.c_double),
+        ("body_force", C.c_double * 3), ("inlet_u0", C.c_double), ("inlet_amp", C.c_double),
+        ("inlet_period", C.c_double), ("inlet_dir", C.c_double * 3), ("outlet_rho", C.c_double),
+        ("t_scale", C.c_double), ("pressure_scale", C.c_double),
+    ]
+
+
+class Lattice(C.Structure):
+    _fields_ = [
+        ("im", C.c_int), ("jm", C.c_int), ("km", C.c_int), ("dommax", C.c_int), ("bcmax", C.c_int),
+        ("b_index", C.c_void_p), ("domid", C.c_void_p), ("domlist", C.c_void_p),
+        ("bclist", C.c_void_p), ("bcnlist", C.c_void_p),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: make -C oracle")
+        _lib = C.CDLL(LIB)
+        _lib.zfo_dom_setup.restype = C.c_int
+        _lib.zfo_bc_setup.restype = C.c_int
+        _lib.zfo_feq.restype = C.c_double
+        _lib.zfo_feq.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        _lib.zfo_bc_pressure.restype = C.c_double
+        _lib.zfo_bc_pressure.argtypes = [C.c_int, C.c_double]
+        _lib.zfo_wtime.restype = C.c_double
+        _lib.zfo_max_threads.restype = C.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+E = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+              [1, 1, 0], [-1, -1, 0], [-1, 1, 0], [1, -1, 0], [1, 0, 1], [-1, 0, -1],
+              [-1, 0, 1], [1, 0, -1], [0, 1, 1], [0, -1, -1], [0, -1, 1], [0, 1, -1],
+              [0, 0, 0]])
+W = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+
+
+def dom_setup(b):
+    b = np.ascontiguousarray(b, np.int32)
+    im, jm, km = b.shape
+    domid = np.zeros(b.shape, np.int32)
+    cap = int((b >= 0).sum())
+    domlist = np.zeros((max(cap, 1), 3), np.int32)
+    n = lib().zfo_dom_setup(_p(b), im, jm, km, _p(domid), _p(domlist), cap)
+    assert n == cap
+    return domid, domlist[:n]
+
+
+def bc_setup(b, periodic=(0, 0, 0)):
+    b = np.ascontiguousarray(b, np.int32)
+    im, jm, km = b.shape
+    cap = int(((b == 0) | (b >= 1000)).sum())
+    bl = np.zeros((max(cap, 1), 4), np.int32)
+    bn = np.zeros((max(cap, 1), 3), np.int32)
+    per = (C.c_int * 3)(*periodic)
+    nf = C.c_int()
+    n = lib().zfo_bc_setup(_p(b), im, jm, km, per, _p(bl), _p(bn), cap, C.byref(nf))
+    assert n == cap
+    return bl[:n], bn[:n], nf.value
+
+
+def neighbours(domid, domlist, periodic=(0, 0, 0)):
+    im, jm, km = domid.shape
+    n = len(domlist)
+    nbr = np.zeros((18, n), np.int32)
+    per = (C.c_int * 3)(*periodic)
+    lib().zfo_neighbours(_p(np.ascontiguousarray(domid)), _p(np.ascontiguousarray(domlist)), n,
+                         im, jm, km, per, _p(nbr))
+    return nbr
+
+
+def feq(rho, u):
+    """feq_c(rho, u) for rho[n], u[3, n] -> [19, n] (ZebraflowWT.c:6139, C_LBM = 1)."""
+    rho = np.asarray(rho, np.float64)
+    u = np.asarray(u, np.float64)
+    cu = E.astype(np.float64) @ u
+    usq = (u * u).sum(axis=0)
+    return W[:, None] * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
+
+
+class LbmOracle:
+    """Reference-structured LBM (F, F_OLD, FEQ, D, U, V, W, BODYFORCE)."""
+
+    def __init__(self, b, params, dtype=np.float64, periodic=None):
+        self.b = np.ascontiguousarray(b, np.int32)
+        self.im, self.jm, self.km = self.b.shape
+        per = periodic if periodic is not None else tuple(params.get("periodic", (0, 0, 0)))
+        self.domid, self.domlist = dom_setup(self.b)
+        self.bclist, self.bcnlist, self.n_fail = bc_setup(self.b, per)
+        self.n = len(self.domlist)
+        self.dtype = np.dtype(dtype)
+        p = LbmParams()
+        p.im, p.jm, p.km = self.im, self.jm, self.km
+        for i, v in enumerate(per):
+            p.periodic[i] = v
+        for k, v in params.items():
+            if k == "periodic":
+                continue
+            if k in ("body_force", "inlet_dir"):
+                arr = getattr(p, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            else:
+                setattr(p, k, v)
+        self.params = p
+        lat = Lattice()
+        lat.im, lat.jm, lat.km, lat.dommax, lat.bcmax = self.im, self.jm, self.km, self.n, len(self.bclist)
+        self._keep = [np.ascontiguousarray(self.domid), np.ascontiguousarray(self.domlist),
+                      np.ascontiguousarray(self.bclist), np.ascontiguousarray(self.bcnlist)]
+        lat.b_index = _p(self.b)
+        lat.domid, lat.domlist, lat.bclist, lat.bcnlist = [_p(a) for a in self._keep]
+        self.lat = lat
+        n = self.n
+        self.F = np.zeros((19, n), self.dtype)
+        self.F_OLD = np.zeros((19, n), self.dtype)
+        self.FEQ = np.zeros((19, n), self.dtype)
+        self.D = np.ones(n, self.dtype)
+        self.U = np.zeros(n, self.dtype)
+        self.V = np.zeros(n, self.dtype)
+        self.W = np.zeros(n, self.dtype)
+        self.force = np.zeros((3, n), self.dtype)
+        self.set_populations(feq(np.ones(n), np.zeros((3, n))))
+
+    def set_populations(self, f):
+        self.F[...] = f
+        self.F_OLD[...] = f
+        d = self.F_OLD.astype(np.float64).sum(0)
+        self.D[...] = d
+        self.U[...] = (E[:, 0] @ self.F_OLD) / d
+        self.V[...] = (E[:, 1] @ self.F_OLD) / d
+        self.W[...] = (E[:, 2] @ self.F_OLD) / d
+
+    def step(self, it, rbc_step=0, nthreads=0):
+        fn = lib().zfo_lbm_step_f64 if self.dtype == np.float64 else lib().zfo_lbm_step_f32
+        fn(C.byref(self.lat), C.byref(self.params), _p(self.F), _p(self.F_OLD), _p(self.FEQ),
+           _p(self.D), _p(self.U), _p(self.V), _p(self.W), _p(self.force), int(it), int(rbc_step),
+           int(nthreads))
+
+    @property
+    def populations(self):
+        return self.F_OLD
+
+    def bnode(self):
+        d = self.domlist
+        return self.b[d[:, 0] - 1, d[:, 1] - 1, d[:, 2] - 1]
+
+
+def wtime():
+    return lib().zfo_wtime()
+
+
+def max_threads():
+    return lib().zfo_max_threads()

@@ -1,0 +1,145 @@
+/*
+ * ref_harness.c -- drives the reference's own setup functions to write
+ * golden vectors (TEST INFRASTRUCTURE; runs only in the development
+ * container, where the reference source exists).
+ *
+ * Linked against ZebraflowWT.c compiled from its original location with
+ * -Dmain=zf_main (see oracle/Makefile).  The reference keeps all state in
+ * static globals sized by compile-time macros (ZebraflowWT.c:81-122,
+ * :171-277); this file only declares the ones it fills or reads.
+ *
+ * Modes:
+ *   ref_harness lists <in.bin> <out.bin>
+ *     in.bin  : int32 im,jm,km then im*jm*km int32 B values (i outer, k
+ *               inner).  The block is placed at lattice (1,1,1) of the
+ *               reference's 249x547x723 box; the rest is B=-1.
+ *     out.bin : int32 DOMMAX, DOMLIST (DOMMAX*3), BCMAX, BCLIST (BCMAX*4),
+ *               BCNLIST (BCMAX*3); all as the reference filled them.
+ *     Calls LBM_DOM_SETUP (ZebraflowWT.c:1481) and LBM_BC_SETUP (:1506).
+ *
+ *   ref_harness mesh <workdir> <out.bin>
+ *     Runs CGSM_ZFISHRBC_DATA_GEN (:1592) in <workdir> (which must hold
+ *     InputData/ZFISHRBC*.dat, InputData/Nucleus*.dat and an empty
+ *     CGSMDataStructure/), then the 51-pass CGSM_FILAMENT_PARAMETERS /
+ *     CGSM_EQUILIBRIUM_METRICS loop exactly as main() does (:2339-2346).
+ *     out.bin : int32 EdgeM, EDGENODE_LIST rows 1..EdgeM (6 ints each),
+ *               NodeM, NEIGHNODE_LIST rows (10 ints), NEIGHTRI_LIST rows
+ *               (10 ints), then float32 LMAX, Plength, kp, Theta0,
+ *               SINETHETA0, COSINETHETA0 (EdgeM each).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#define R_IM 249
+#define R_JM 547
+#define R_KM 723
+#define R_NODEM 504
+#define R_EDGEM 1506
+#define R_TRIM 1004
+#define R_NLISTM 9
+
+extern int B_index[R_IM + 1][R_JM + 1][R_KM + 1];
+extern int DOMLIST[8500000][3], DOMMAX;
+extern int BCLIST[1200000][4], BCNLIST[1200000][3], BCMAX;
+extern float Xlattice[3][19 + 1];
+extern int iter, iter_local;
+
+extern int EDGENODE_LIST[R_EDGEM + 1][6 + 1];
+extern int NEIGHNODE_LIST[R_NODEM + 1][R_NLISTM + 1];
+extern int NEIGHTRI_LIST[R_NODEM + 1][R_NLISTM + 1];
+extern float LMAX[R_EDGEM + 1], Plength[R_EDGEM + 1], kp[R_EDGEM + 1],
+    Theta0[R_EDGEM + 1], SINETHETA0[R_EDGEM + 1], COSINETHETA0[R_EDGEM + 1];
+
+void LBM_DOM_SETUP(void);
+void LBM_BC_SETUP(void);
+void CGSM_ZFISHRBC_DATA_GEN(void);
+void CGSM_FILAMENT_PARAMETERS(void);
+void CGSM_EQUILIBRIUM_METRICS(void);
+
+/* The lattice table main() fills at ZebraflowWT.c:2471-2504. */
+static const int E[19][3] = {
+    {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},
+    {0, 0, -1}, {1, 1, 0},  {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0},
+    {1, 0, 1},  {-1, 0, -1}, {-1, 0, 1}, {1, 0, -1}, {0, 1, 1},
+    {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 0, 0}};
+
+static void wr(FILE *f, const void *p, size_t sz, size_t n) {
+  if (fwrite(p, sz, n, f) != n) { perror("fwrite"); exit(2); }
+}
+
+static int run_lists(const char *in, const char *out) {
+  FILE *f = fopen(in, "rb");
+  if (!f) { perror(in); return 2; }
+  int dims[3];
+  if (fread(dims, 4, 3, f) != 3) return 2;
+  const int im = dims[0], jm = dims[1], km = dims[2];
+  if (im > R_IM || jm > R_JM || km > R_KM) { fprintf(stderr, "box too big\n"); return 2; }
+  int *blk = malloc(sizeof(int) * (size_t)im * jm * km);
+  if (fread(blk, 4, (size_t)im * jm * km, f) != (size_t)im * jm * km) return 2;
+  fclose(f);
+  for (int i = 1; i <= R_IM; i++)
+    for (int j = 1; j <= R_JM; j++)
+      for (int k = 1; k <= R_KM; k++) B_index[i][j][k] = -1;
+  for (int i = 1; i <= im; i++)
+    for (int j = 1; j <= jm; j++)
+      for (int k = 1; k <= km; k++)
+        B_index[i][j][k] = blk[((size_t)(i - 1) * jm + (j - 1)) * km + (k - 1)];
+  for (int c = 1; c <= 19; c++)
+    for (int d = 0; d < 3; d++) Xlattice[d][c] = (float)E[c - 1][d];
+  iter = 0;
+  LBM_DOM_SETUP();
+  LBM_BC_SETUP();
+  FILE *o = fopen(out, "wb");
+  if (!o) { perror(out); return 2; }
+  wr(o, &DOMMAX, 4, 1);
+  wr(o, &DOMLIST[1][0], 4, (size_t)3 * DOMMAX);
+  wr(o, &BCMAX, 4, 1);
+  wr(o, &BCLIST[1][0], 4, (size_t)4 * BCMAX);
+  wr(o, &BCNLIST[1][0], 4, (size_t)3 * BCMAX);
+  fclose(o);
+  free(blk);
+  return 0;
+}
+
+static int run_mesh(const char *dir, const char *out) {
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd)) return 2;
+  char outabs[4096];
+  if (out[0] == '/') snprintf(outabs, sizeof outabs, "%s", out);
+  else snprintf(outabs, sizeof outabs, "%s/%s", cwd, out);
+  if (chdir(dir) != 0) { perror(dir); return 2; }
+  CGSM_ZFISHRBC_DATA_GEN();
+  /* main(): iter_local runs 1..51 around the two parameter routines
+   * (ZebraflowWT.c:2338-2346). */
+  iter_local = 0;
+  do {
+    iter_local += 1;
+    CGSM_FILAMENT_PARAMETERS();
+    CGSM_EQUILIBRIUM_METRICS();
+  } while (iter_local <= 50);
+  FILE *o = fopen(outabs, "wb");
+  if (!o) { perror(outabs); return 2; }
+  int em = R_EDGEM, nm = R_NODEM;
+  wr(o, &em, 4, 1);
+  for (int e = 1; e <= R_EDGEM; e++) wr(o, &EDGENODE_LIST[e][1], 4, 6);
+  wr(o, &nm, 4, 1);
+  for (int n = 1; n <= R_NODEM; n++) wr(o, &NEIGHNODE_LIST[n][0], 4, 10);
+  for (int n = 1; n <= R_NODEM; n++) wr(o, &NEIGHTRI_LIST[n][0], 4, 10);
+  wr(o, &LMAX[1], 4, R_EDGEM);
+  wr(o, &Plength[1], 4, R_EDGEM);
+  wr(o, &kp[1], 4, R_EDGEM);
+  wr(o, &Theta0[1], 4, R_EDGEM);
+  wr(o, &SINETHETA0[1], 4, R_EDGEM);
+  wr(o, &COSINETHETA0[1], 4, R_EDGEM);
+  fclose(o);
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc == 4 && strcmp(argv[1], "lists") == 0) return run_lists(argv[2], argv[3]);
+  if (argc == 4 && strcmp(argv[1], "mesh") == 0) return run_mesh(argv[2], argv[3]);
+  fprintf(stderr, "usage: ref_harness lists <in.bin> <out.bin> | mesh <dir> <out.bin>\n");
+  return 1;
+}

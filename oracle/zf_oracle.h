@@ -1,0 +1,117 @@
+/*
+ * zf_oracle.h -- CPU restatement of the Zebraflow time step (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker for the MI355X engine.  Only tests/, the smoke
+ * entry point and bench.py's CPU-baseline leg may load it.  The product path
+ * (libzfgpu.so, zf_host) never links or calls it.
+ *
+ * Every function restates the algorithm of ZebraflowWT.c (the reference, a
+ * single C99/OpenMP file) and cites the lines it follows.  Conventions kept
+ * from the reference:
+ *   - lattice indices i,j,k are 1-based; the box array b_index is dense and
+ *     0-based, element (i,j,k) at ((i-1)*jm + (j-1))*km + (k-1)  (i outer,
+ *     k inner, as B_index[i][j][k] in ZebraflowWT.c:180);
+ *   - list entries (DOMLIST, BCLIST, BCNLIST, DOMID values) are 1-based;
+ *   - populations are SoA f[c*n + p] with c = 0..18 for the reference's
+ *     directions 1..19 and p = DOMID-1.
+ *
+ * Pinning status of each function is stated in its comment and in the
+ * oracle test files (tests/test_oracle_*.py).
+ */
+#ifndef ZF_ORACLE_H
+#define ZF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* D3Q19 table, ZebraflowWT.c:2471-2504 (Xlattice) and :2329-2333 (Wcoeff). */
+extern const int ZFO_E[19][3];
+extern const double ZFO_W[19];
+
+/* ---- a1: fluid-node list, LBM_DOM_SETUP ZebraflowWT.c:1481-1502 ----------
+ * domid: box-sized, receives 1-based l or -1.  domlist: 3*cap ints.
+ * Returns DOMMAX, or -1 if cap is exceeded.  Pinned by reference goldens. */
+int zfo_dom_setup(const int32_t *b_index, int im, int jm, int km,
+                  int32_t *domid, int32_t *domlist, int cap);
+
+/* ---- a3: BC list, LBM_BC_SETUP ZebraflowWT.c:1506-1589 -------------------
+ * bclist: 4*cap (i,j,k,B); bcnlist: 3*cap (i,j,k of the chosen neighbour).
+ * periodic[3] extends the reference's out-of-box skip (ZebraflowWT.c:1541)
+ * with a wrap on periodic axes; all-zero reproduces the reference exactly.
+ * A node with no neighbour gets (0,0,0) and is counted in *n_fail (the
+ * reference leaves stale values, ZebraflowWT.c:1575-1581).
+ * Returns BCMAX or -1 on overflow.  Pinned by reference goldens. */
+int zfo_bc_setup(const int32_t *b_index, int im, int jm, int km,
+                 const int periodic[3], int32_t *bclist, int32_t *bcnlist,
+                 int cap, int *n_fail);
+
+/* ---- a2: pull-neighbour list, implicit in ZebraflowWT.c:6113-6125 --------
+ * nbr[c*dommax + (l-1)] = DOMID[(i,j,k) - e_c] for c = 0..17 (1-based, or
+ * -1 if unlisted/outside the box; periodic axes wrap).  Pinned by brute
+ * force and the pairing invariant nbr[opp c][nbr[c][l]] == l. */
+void zfo_neighbours(const int32_t *domid, const int32_t *domlist, int dommax,
+                    int im, int jm, int km, const int periodic[3],
+                    int32_t *nbr);
+
+/* ---- a4-a6: one LBM time step ---------------------------------------------
+ * Follows the reference loop structure: stream+collide for interior nodes
+ * (0<B<10, ZebraflowWT.c:6101-6145), boundary conditions (:6292-6433),
+ * macro recompute + copy (:6601-6626).  Arrays F, F_OLD, FEQ as in the
+ * reference (:178), DOMID lookups per direction.
+ *
+ * Extensions (not in the reference, all default-off):
+ *   wall_mode 1   : halfway bounce-back on links whose source has B==0
+ *                   instead of the equilibrium wall (B==0 nodes untouched);
+ *   body_force[3] : uniform force added to the per-node force field;
+ *   B==2001       : velocity inlet feq(D[m], u_in(iter)*inlet_dir);
+ *   B==3001       : pressure outlet feq(outlet_rho, U[m]);
+ *   periodic[3]   : wrap on the pull.
+ */
+typedef struct zfo_lbm_params {
+  int im, jm, km;
+  int periodic[3];
+  int wall_mode;            /* 0 = equilibrium wall (reference), 1 = halfway BB */
+  double visc_scale;        /* tau = 0.5 + visc_scale*B; reference MU_SCALE*MU = 0.72 */
+  double force_dt;          /* DT in the Guo terms; reference 0.5 (ZebraflowWT.c:41) */
+  double body_force[3];
+  double inlet_u0, inlet_amp, inlet_period, inlet_dir[3];
+  double outlet_rho;
+  double t_scale;           /* seconds per step for the 1001-1009 waveforms (5e-7) */
+  double pressure_scale;    /* LBM_PRESSURE_SCALE (3333.33..) */
+} zfo_lbm_params;
+
+typedef struct zfo_lattice {
+  int im, jm, km, dommax, bcmax;
+  const int32_t *b_index, *domid, *domlist, *bclist, *bcnlist;
+} zfo_lattice;
+
+/* F, F_OLD: 19*dommax.  D,U,V,W: dommax.  force: 3*dommax (may be NULL).
+ * feq: 19*dommax scratch (the reference's FEQ store) or NULL.
+ * On return F_OLD holds the new populations (the reference's copy). */
+void zfo_lbm_step_f64(const zfo_lattice *lat, const zfo_lbm_params *p,
+                      double *F, double *F_OLD, double *FEQ, double *D,
+                      double *U, double *V, double *W, double *force,
+                      int iter, int rbc_step, int nthreads);
+void zfo_lbm_step_f32(const zfo_lattice *lat, const zfo_lbm_params *p,
+                      float *F, float *F_OLD, float *FEQ, float *D, float *U,
+                      float *V, float *W, float *force, int iter,
+                      int rbc_step, int nthreads);
+
+/* Equilibrium, ZebraflowWT.c:2568 with C_LBM = 1 (EQ_A=3, EQ_B=4.5, EQ_C=-1.5). */
+double zfo_feq(int c, double rho, double u, double v, double w);
+
+/* Pressure waveform of the BC codes 1001-1009, ZebraflowWT.c:6318-6421
+ * (pressure in the reference's units, before /LBM_PRESSURE_SCALE). */
+double zfo_bc_pressure(int code, double simtime);
+
+/* Wall-clock helper for the CPU baseline. */
+double zfo_wtime(void);
+int zfo_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+"""ctypes declarations mirroring include/zf.h (keep in the same order)."""
+import ctypes as C
+
+i32 = C.c_int32
+i64 = C.c_int64
+f64 = C.c_double
+P = C.c_void_p
+
+
+class ZfConfig(C.Structure):
+    _fields_ = [
+        ("im", i32), ("jm", i32), ("km", i32),
+        ("periodic", i32 * 3),
+        ("storage", i32), ("wall_mode", i32), ("device", i32), ("reserved", i32),
+        ("visc_scale", f64), ("force_dt", f64), ("body_force", f64 * 3),
+        ("inlet_u0", f64), ("inlet_amp", f64), ("inlet_period", f64), ("inlet_dir", f64 * 3),
+        ("outlet_rho", f64), ("t_scale", f64), ("pressure_scale", f64),
+    ]
+
+
+class ZfLatticeInfo(C.Structure):
+    _fields_ = [
+        ("dommax", i32), ("bcmax", i32), ("n_interior", i32), ("n_chords", i32),
+        ("n_bc_active", i32), ("n_bb_links", i32), ("n_macro_nodes", i32), ("stride", i32),
+        ("device_bytes", i64),
+    ]
+
+
+# name -> (restype, argtypes)
+FUNCS = {
+    "zf_version": (C.c_char_p, []),
+    "zf_config_reference": (None, [C.POINTER(ZfConfig), i32, i32, i32]),
+    "zf_config_lattice": (None, [C.POINTER(ZfConfig), i32, i32, i32, f64]),
+    "zf_create": (C.c_int, [C.POINTER(ZfConfig), C.POINTER(P)]),
+    "zf_destroy": (C.c_int, [P]),
+    "zf_last_error": (C.c_char_p, [P]),
+    "zf_upload_lattice": (C.c_int, [P, P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "zf_download_lists": (C.c_int, [P, P, P, P]),
+    "zf_download_domid": (C.c_int, [P, P]),
+    "zf_download_neighbours": (C.c_int, [P, P]),
+    "zf_get_lattice_info": (C.c_int, [P, C.POINTER(ZfLatticeInfo)]),
+    "zf_init_equilibrium": (C.c_int, [P, f64, P]),
+    "zf_upload_populations": (C.c_int, [P, P]),
+    "zf_download_populations": (C.c_int, [P, P]),
+    "zf_upload_force": (C.c_int, [P, P]),
+    "zf_download_force": (C.c_int, [P, P]),
+    "zf_lbm_step": (C.c_int, [P, i32, i32]),
+    "zf_lbm_run": (C.c_int, [P, i32, i32]),
+    "zf_download_macro": (C.c_int, [P, P, P, P, P]),
+    "zf_conservation_sums": (C.c_int, [P, P]),
+    "zf_lbm_time_kernels": (C.c_int, [P, i32, i32, P]),
+    "zf_sync": (C.c_int, [P]),
+    "zf_event_record": (C.c_int, [P, i32]),
+    "zf_event_elapsed": (C.c_int, [P, i32, i32, C.POINTER(C.c_float)]),
+}
+
+HOST_FUNCS = {
+    "zfh_geom_tube": (i64, [P, i32, i32, i32, i32, f64, i32, i32, i32]),
+    "zfh_geom_vessel_pair": (i64, [P, i32, i32, i32, f64, f64, f64, i32, C.c_uint64]),
+    "zfh_geom_tree": (i64, [P, i32, i32, i32, f64, f64, i32, f64, f64, C.c_uint64]),
+    "zfh_write_bindex_csv": (C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64]),
+    "zfh_read_bindex_csv": (i64, [P, P, i32, i32, i32, f64, f64, f64, f64]),
+    "zfh_write_fluid_csv": (C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64, P, P, P, P, P, i64]),
+}
+
+
+def declare(lib):
+    for name, (res, args) in FUNCS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def declare_host(lib):
+    for name, (res, args) in HOST_FUNCS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args

@@ -1,0 +1,113 @@
+// zf_internal.h -- shared state and helpers of libzfgpu.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/zf.h"
+
+namespace zf {
+
+// D3Q19 in the reference's order, ZebraflowWT.c:2471-2504: c and c^1 are
+// opposites for c < 18, c = 18 is the rest direction.
+constexpr int Q = 19;
+constexpr int kE[Q][3] = {
+    {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0}, {0, 0, 1},
+    {0, 0, -1}, {1, 1, 0},  {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0},
+    {1, 0, 1},  {-1, 0, -1}, {-1, 0, 1}, {1, 0, -1}, {0, 1, 1},
+    {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 0, 0}};
+// Wcoeff, ZebraflowWT.c:2329-2333.
+constexpr double kW[Q] = {
+    1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36,
+    1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+    1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 3};
+
+// Node word of the stream-collide kernel (one per interior node):
+//   bits  0..17  halfway bounce-back mask (link c pulls f_{c^1} of the node)
+//   bit   18     write rho,u of this node for the boundary kernel
+//   bits 20..31  chord index relative to the first chord of its 64-node block
+constexpr uint32_t kWordMacro = 1u << 18;
+constexpr int kWordChordShift = 20;
+constexpr int kBlockNodes = 64;
+// Chord record: 18 pull deltas (storage index of the source minus the
+// node's own), the tau class B, and padding to 80 bytes.
+constexpr int kChordInts = 20;
+
+// One active boundary node for the boundary kernel.
+struct BcEntry {
+  int32_t s;      // storage index of the BC node
+  int32_t sm;     // storage index of its neighbour (BCNLIST)
+  int32_t code;   // B value of the BC node
+  int32_t m_int;  // neighbour is updated by stream-collide (0<B<10): its rho,u
+                  // come from this step's macro output, else from moments
+};
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace zf
+
+struct zf_ctx {
+  zf_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t events[16] = {};
+  std::string err;
+
+  // lattice
+  bool have_lists = false;    // index lists built (downloadable)
+  bool have_lattice = false;  // stream tables built and state allocated
+  int32_t dommax = 0, bcmax = 0, n_int = 0, n_chords = 0, n_bc = 0;
+  int32_t n_bb_links = 0, n_macro = 0;
+  size_t stride = 0;          // elements per direction in the population arrays
+  size_t box = 0;
+  int32_t *d_b = nullptr;     // box B_index
+  int32_t *d_domid = nullptr; // box DOMID (1-based or -1)
+  int32_t *d_domlist = nullptr, *d_bclist = nullptr, *d_bcnlist = nullptr;
+  int32_t *d_nbr = nullptr;   // 18 x dommax
+  int32_t *d_s_of_p = nullptr;  // list position -> storage index
+  int32_t *d_p_of_s = nullptr;  // storage index -> list position
+  uint32_t *d_word = nullptr;   // n_int
+  int32_t *d_block_chord0 = nullptr;
+  int32_t *d_chords = nullptr;  // n_chords x kChordInts
+  zf::BcEntry *d_bc = nullptr;  // n_bc
+  std::vector<int32_t> h_bnode;  // B value per list position
+
+  // state
+  void *d_f[2] = {nullptr, nullptr};  // population buffers (storage-typed)
+  int cur = 0;                        // buffer holding the current state
+  void *d_macro = nullptr;            // 4 x n_int (rho,u,v,w), real-typed
+  float *d_force = nullptr;           // 3 x stride, storage-indexed; null = none
+  double *d_sums = nullptr;           // reduction scratch
+  int64_t device_bytes = 0;
+};
+
+namespace zf {
+
+int fail(zf_ctx *ctx, int code, const char *fmt, ...);
+// HIP error check that records the failing call in the context.
+#define ZF_HIP(ctx, call)                                                     \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::zf::fail((ctx), ZF_ERR_HIP, "%s failed: %s (%s:%d)", #call,    \
+                        hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)
+
+int dev_alloc(zf_ctx *ctx, void **p, size_t bytes);
+void dev_free(zf_ctx *ctx, void **p);
+int upload_lattice_impl(zf_ctx *ctx, const int32_t *b);
+void free_lattice(zf_ctx *ctx);
+void free_state(zf_ctx *ctx);
+int alloc_state(zf_ctx *ctx);
+size_t store_bytes(int storage);
+
+}  // namespace zf

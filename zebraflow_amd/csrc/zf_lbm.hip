@@ -1,0 +1,685 @@
+// zf_lbm.hip -- D3Q19 BGK-Guo time step on the sparse fluid-node list.
+//
+// Replaces three OpenMP regions of the reference time loop:
+//   stream + macro + collide  ZebraflowWT.c:6101-6145
+//   boundary conditions       ZebraflowWT.c:6292-6433
+//   macro recompute + copy    ZebraflowWT.c:6601-6626
+// with two kernels per step (k_stream_collide, k_boundary) and a buffer swap.
+//
+// HBM layout: populations are SoA, one array per direction of `stride`
+// elements, indexed by a storage index s: the interior nodes (0<B<10) first,
+// in list order, then every other listed node.  The interior block is what
+// the stream kernel reads and writes, so its stores are dense and its pulls
+// are unit-stride runs shifted by a per-chord delta.
+//
+// Storage forms (the arithmetic is always on deviations g = f - w):
+//   ZF_STORE_F64     double f                      double arithmetic
+//   ZF_STORE_F32DEV  float  f - w                  float arithmetic
+//   ZF_STORE_F16DEV  half   (f - w)/w              float arithmetic
+#include "zf_internal.h"
+
+#include <cmath>
+#include <type_traits>
+
+namespace zf {
+namespace {
+
+template <int S>
+struct Store;
+template <>
+struct Store<ZF_STORE_F64> {
+  using T = double;
+  using R = double;
+  __device__ static R ld(const T *a, size_t i, double w) { return a[i] - w; }
+  __device__ static T st(R g, double w) { return g + w; }
+};
+template <>
+struct Store<ZF_STORE_F32DEV> {
+  using T = float;
+  using R = float;
+  __device__ static R ld(const T *a, size_t i, double) { return a[i]; }
+  __device__ static T st(R g, double) { return g; }
+};
+template <>
+struct Store<ZF_STORE_F16DEV> {
+  using T = _Float16;
+  using R = float;
+  __device__ static R ld(const T *a, size_t i, double w) { return (float)a[i] * (float)w; }
+  __device__ static T st(R g, double w) { return (T)(g * (float)(1.0 / w)); }
+};
+
+// Compile-time lattice tables for fully unrolled loops.
+__device__ constexpr int EX[Q] = {1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0, 0};
+__device__ constexpr int EY[Q] = {0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0};
+__device__ constexpr int EZ[Q] = {0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0};
+__device__ constexpr double WQ[Q] = {
+    1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36,
+    1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+    1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 3};
+
+struct StepParams {
+  double visc_scale;  // tau = 0.5 + visc_scale*B
+  double dt;          // DT of the Guo terms
+  double g[3];        // uniform body force
+};
+
+// Equilibrium deviation feq_c - w_c for rho = 1 + drho (ZebraflowWT.c:6139).
+template <class R>
+__device__ __forceinline__ R geq(int c, R drho, R rho, R ux, R uy, R uz, R usq) {
+  const R cu = (R)EX[c] * ux + (R)EY[c] * uy + (R)EZ[c] * uz;
+  return (R)WQ[c] * (drho + rho * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq));
+}
+
+// ---------------------------------------------------------------------------
+// Stream + collide, one interior node per lane.  The pull source of link c
+// is s + delta_c(chord) or, for a bounce-back link, f_{c^1} of the node
+// itself.  HBM traffic per node: 19 loads + 19 stores of the storage type,
+// the 4-byte node word and an amortised share of the 80-byte chord record.
+template <int S, bool FIELD>
+__global__ __launch_bounds__(256) void k_stream_collide(
+    const typename Store<S>::T *__restrict__ src, typename Store<S>::T *__restrict__ dst,
+    size_t stride, int n_int, const uint32_t *__restrict__ word,
+    const int32_t *__restrict__ block0, const int32_t *__restrict__ chords,
+    const float *__restrict__ force, StepParams prm, typename Store<S>::R *__restrict__ macro) {
+  using R = typename Store<S>::R;
+  using Sx = Store<S>;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= n_int) return;
+  const uint32_t w = word[s];
+  const int ch = block0[s >> 6] + (int)(w >> kWordChordShift);
+  const int4 *cr = reinterpret_cast<const int4 *>(chords + (size_t)ch * kChordInts);
+  int d[20];
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    const int4 x = cr[v];
+    d[4 * v] = x.x; d[4 * v + 1] = x.y; d[4 * v + 2] = x.z; d[4 * v + 3] = x.w;
+  }
+  R g[Q];
+#pragma unroll
+  for (int c = 0; c < 18; c++) {
+    const bool bb = (w >> c) & 1u;
+    const size_t idx = bb ? (size_t)(c ^ 1) * stride + s : (size_t)c * stride + (s + d[c]);
+    g[c] = Sx::ld(src, idx, WQ[c]);
+  }
+  g[18] = Sx::ld(src, (size_t)18 * stride + s, WQ[18]);
+
+  R drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    drho += g[c];
+    if (EX[c]) jx += EX[c] > 0 ? g[c] : -g[c];
+    if (EY[c]) jy += EY[c] > 0 ? g[c] : -g[c];
+    if (EZ[c]) jz += EZ[c] > 0 ? g[c] : -g[c];
+  }
+  R fx = (R)prm.g[0], fy = (R)prm.g[1], fz = (R)prm.g[2];
+  if (FIELD) {
+    fx += (R)force[s];
+    fy += (R)force[stride + s];
+    fz += (R)force[2 * stride + s];
+  }
+  const R dt = (R)prm.dt;
+  const R rho = (R)1 + drho;
+  const R inv_rho = (R)1 / rho;
+  // half-force velocity shift, ZebraflowWT.c:6132-6135
+  const R ux = (jx + (R)0.5 * dt * fx) * inv_rho;
+  const R uy = (jy + (R)0.5 * dt * fy) * inv_rho;
+  const R uz = (jz + (R)0.5 * dt * fz) * inv_rho;
+  if (w & kWordMacro) {
+    macro[s] = rho;
+    macro[(size_t)n_int + s] = ux;
+    macro[2 * (size_t)n_int + s] = uy;
+    macro[3 * (size_t)n_int + s] = uz;
+  }
+  const R usq = ux * ux + uy * uy + uz * uz;
+  const R omega = (R)1 / ((R)0.5 + (R)prm.visc_scale * (R)d[18]);
+  const R fpref = dt * ((R)1 - (R)0.5 * omega);  // DT*(1 - 1/(2 tau))
+  const R uf = ux * fx + uy * fy + uz * fz;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const R cu = (R)EX[c] * ux + (R)EY[c] * uy + (R)EZ[c] * uz;
+    const R cf = (R)EX[c] * fx + (R)EY[c] * fy + (R)EZ[c] * fz;
+    const R ge = geq<R>(c, drho, rho, ux, uy, uz, usq);
+    // Guo source, ZebraflowWT.c:6140: ((c-u).F)/cs^2 + (c.u)(c.F)/cs^4
+    const R src_c = (R)WQ[c] * ((R)3 * (cf - uf) + (R)9 * cu * cf);
+    const R out = g[c] - omega * (g[c] - ge) + fpref * src_c;
+    dst[(size_t)c * stride + s] = Sx::st(out, WQ[c]);
+  }
+}
+
+struct BcParams {
+  double inlet[3];        // inlet velocity vector at this iteration
+  double outlet_rho;
+  double prho[9];         // density of codes 1001..1009 at this iteration
+  int n_int;
+};
+
+// Boundary nodes: all 19 populations set to feq(D_bc, U_bc)
+// (ZebraflowWT.c:6294-6433).  D[m], U[m] of the neighbour come from this
+// step's stream kernel when m is an interior node, otherwise from the
+// moments of m's populations before this step (the reference's previous
+// macro pass, ZebraflowWT.c:6606-6614).
+template <int S>
+__global__ __launch_bounds__(256) void k_boundary(const typename Store<S>::T *__restrict__ src,
+                                                  typename Store<S>::T *__restrict__ dst,
+                                                  size_t stride, int n_bc,
+                                                  const BcEntry *__restrict__ bc,
+                                                  const typename Store<S>::R *__restrict__ macro,
+                                                  BcParams bp) {
+  using R = typename Store<S>::R;
+  using Sx = Store<S>;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n_bc) return;
+  const BcEntry e = bc[t];
+  R rm, um, vm, wm;
+  if (e.m_int) {
+    const size_t n = (size_t)bp.n_int;
+    rm = macro[e.sm];
+    um = macro[n + e.sm];
+    vm = macro[2 * n + e.sm];
+    wm = macro[3 * n + e.sm];
+  } else {
+    R drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int c = 0; c < Q; c++) {
+      const R gv = Sx::ld(src, (size_t)c * stride + e.sm, WQ[c]);
+      drho += gv;
+      jx += (R)EX[c] * gv;
+      jy += (R)EY[c] * gv;
+      jz += (R)EZ[c] * gv;
+    }
+    rm = (R)1 + drho;
+    um = jx / rm;
+    vm = jy / rm;
+    wm = jz / rm;
+  }
+  R dbc, ub, vb, wb;
+  if (e.code == 0 || e.code == 4998) {
+    dbc = rm; ub = vb = wb = 0;
+  } else if (e.code >= 1001 && e.code <= 1009) {
+    dbc = (R)bp.prho[e.code - 1001]; ub = um; vb = vm; wb = wm;
+  } else if (e.code == ZF_B_VELOCITY_INLET) {
+    dbc = rm; ub = (R)bp.inlet[0]; vb = (R)bp.inlet[1]; wb = (R)bp.inlet[2];
+  } else {  // ZF_B_PRESSURE_OUTLET
+    dbc = (R)bp.outlet_rho; ub = um; vb = vm; wb = wm;
+  }
+  const R usq = ub * ub + vb * vb + wb * wb;
+  const R drb = dbc - (R)1;
+#pragma unroll
+  for (int c = 0; c < Q; c++)
+    dst[(size_t)c * stride + e.s] = Sx::st(geq<R>(c, drb, dbc, ub, vb, wb, usq), WQ[c]);
+}
+
+// Moments without the force shift for every listed node, written in list
+// order p (ZebraflowWT.c:6606-6614).
+template <int S>
+__global__ void k_macro(const typename Store<S>::T *__restrict__ f, size_t stride, int n,
+                        const int32_t *__restrict__ p_of_s, double *__restrict__ out) {
+  using R = typename Store<S>::R;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const double gv = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]);
+    drho += gv;
+    jx += EX[c] * gv;
+    jy += EY[c] * gv;
+    jz += EZ[c] * gv;
+  }
+  const double rho = 1.0 + drho;
+  const int p = p_of_s[s];
+  out[p] = rho;
+  out[(size_t)n + p] = jx / rho;
+  out[2 * (size_t)n + p] = jy / rho;
+  out[3 * (size_t)n + p] = jz / rho;
+}
+
+template <int S>
+__global__ void k_import(const double *__restrict__ fp, int n, const int32_t *__restrict__ s_of_p,
+                         typename Store<S>::T *__restrict__ f0, typename Store<S>::T *__restrict__ f1,
+                         size_t stride) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const double v = fp[(size_t)c * n + p];
+    const auto t = Store<S>::st((typename Store<S>::R)(v - WQ[c]), WQ[c]);
+    f0[(size_t)c * stride + s] = t;
+    if (f1) f1[(size_t)c * stride + s] = t;
+  }
+}
+
+template <int S>
+__global__ void k_export(const typename Store<S>::T *__restrict__ f, size_t stride, int n,
+                         const int32_t *__restrict__ s_of_p, double *__restrict__ fp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+#pragma unroll
+  for (int c = 0; c < Q; c++)
+    fp[(size_t)c * n + p] = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]) + WQ[c];
+}
+
+template <int S>
+__global__ void k_init_eq(typename Store<S>::T *__restrict__ f0, typename Store<S>::T *__restrict__ f1,
+                          size_t stride, int n, double rho, double ux, double uy, double uz) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const double gv = geq<double>(c, rho - 1.0, rho, ux, uy, uz, usq);
+    const auto t = Store<S>::st((typename Store<S>::R)gv, WQ[c]);
+    f0[(size_t)c * stride + s] = t;
+    f1[(size_t)c * stride + s] = t;
+  }
+}
+
+__global__ void k_force_import(const double *__restrict__ fp, int n, const int32_t *__restrict__ s_of_p,
+                               float *__restrict__ force, size_t stride) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+  for (int a = 0; a < 3; a++) force[(size_t)a * stride + s] = (float)fp[(size_t)a * n + p];
+}
+
+__global__ void k_force_export(const float *__restrict__ force, size_t stride, int n,
+                               const int32_t *__restrict__ s_of_p, double *__restrict__ fp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+  for (int a = 0; a < 3; a++) fp[(size_t)a * n + p] = force[(size_t)a * stride + s];
+}
+
+__global__ void k_fill_zero(float *__restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0f;
+}
+
+// Conservation sums over interior nodes in FP64: per-block partials, then a
+// fixed-order final pass (deterministic).
+template <int S>
+__global__ __launch_bounds__(256) void k_sums_partial(const typename Store<S>::T *__restrict__ f,
+                                                      size_t stride, int n, double *__restrict__ part) {
+  __shared__ double sh[4][256];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  double v[4] = {0, 0, 0, 0};
+  if (s < n) {
+#pragma unroll
+    for (int c = 0; c < Q; c++) {
+      const double gv = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]);
+      v[0] += gv;
+      v[1] += EX[c] * gv;
+      v[2] += EY[c] * gv;
+      v[3] += EZ[c] * gv;
+    }
+  }
+  for (int a = 0; a < 4; a++) sh[a][threadIdx.x] = v[a];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int a = 0; a < 4; a++) sh[a][threadIdx.x] += sh[a][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 4; a++) part[4 * (size_t)blockIdx.x + a] = sh[a][0];
+}
+
+__global__ void k_sums_final(const double *__restrict__ part, int nblk, double *__restrict__ out) {
+  __shared__ double sh[4][256];
+  double v[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int a = 0; a < 4; a++) v[a] += part[4 * (size_t)b + a];
+  for (int a = 0; a < 4; a++) sh[a][threadIdx.x] = v[a];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int a = 0; a < 4; a++) sh[a][threadIdx.x] += sh[a][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 4; a++) out[a] = sh[a][0];
+}
+
+inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+double bc_pressure(int code, double simtime);
+
+// ev (optional): events recorded before the stream kernel, between the two
+// kernels and after the boundary kernel.
+template <int S>
+int step_impl(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
+  using T = typename Store<S>::T;
+  using R = typename Store<S>::R;
+  const zf_config &c = ctx->cfg;
+  T *src = static_cast<T *>(ctx->d_f[ctx->cur]);
+  T *dst = static_cast<T *>(ctx->d_f[ctx->cur ^ 1]);
+  StepParams prm{c.visc_scale, c.force_dt, {c.body_force[0], c.body_force[1], c.body_force[2]}};
+  R *macro = static_cast<R *>(ctx->d_macro);
+  if (ev) ZF_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+  if (ctx->n_int > 0) {
+    if (ctx->d_force)
+      hipLaunchKernelGGL((k_stream_collide<S, true>), dim3(nblocks(ctx->n_int, 256)), dim3(256), 0,
+                         ctx->stream, src, dst, ctx->stride, ctx->n_int, ctx->d_word,
+                         ctx->d_block_chord0, ctx->d_chords, ctx->d_force, prm, macro);
+    else
+      hipLaunchKernelGGL((k_stream_collide<S, false>), dim3(nblocks(ctx->n_int, 256)), dim3(256), 0,
+                         ctx->stream, src, dst, ctx->stride, ctx->n_int, ctx->d_word,
+                         ctx->d_block_chord0, ctx->d_chords, (const float *)nullptr, prm, macro);
+  }
+  if (ev) ZF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+  if (ctx->n_bc > 0) {
+    BcParams bp{};
+    const double ph = 2.0 * 3.14159265358979323846 * iter / (c.inlet_period > 0 ? c.inlet_period : 1.0);
+    const double uin = c.inlet_u0 * (1.0 + c.inlet_amp * std::sin(ph));
+    for (int a = 0; a < 3; a++) bp.inlet[a] = uin * c.inlet_dir[a];
+    bp.outlet_rho = c.outlet_rho;
+    for (int k = 0; k < 9; k++)
+      bp.prho[k] = bc_pressure(1001 + k, iter * c.t_scale) / c.pressure_scale + 1.0;
+    bp.n_int = ctx->n_int;
+    hipLaunchKernelGGL((k_boundary<S>), dim3(nblocks(ctx->n_bc, 256)), dim3(256), 0, ctx->stream,
+                       src, dst, ctx->stride, ctx->n_bc, ctx->d_bc, macro, bp);
+  }
+  if (ev) ZF_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+  if (rbc_step && ctx->d_force)
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_force, 3 * ctx->stride);
+  ZF_HIP(ctx, hipGetLastError());
+  ctx->cur ^= 1;
+  return ZF_OK;
+}
+
+int step(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
+  switch (ctx->cfg.storage) {
+    case ZF_STORE_F64: return step_impl<ZF_STORE_F64>(ctx, iter, rbc_step, ev);
+    case ZF_STORE_F32DEV: return step_impl<ZF_STORE_F32DEV>(ctx, iter, rbc_step, ev);
+    default: return step_impl<ZF_STORE_F16DEV>(ctx, iter, rbc_step, ev);
+  }
+}
+
+// Cardiac pressure waveforms of codes 1001-1009 (ZebraflowWT.c:6318-6421).
+double bc_pressure(int code, double simtime) {
+  const double P = 3.14159265359, T = 0.36;
+  double shift = 0.2 * T, base = 0, amp = 0;
+  bool sin2 = true;
+  switch (code) {
+    case 1001: shift = 0.0; base = 54.0; amp = 82.0; break;
+    case 1002: shift = 0.06 * T; base = 48.0; amp = 68.0; break;
+    case 1003: base = 11.0; amp = 23.0; sin2 = false; break;
+    case 1004: base = 18.0; amp = 27.0; sin2 = false; break;
+    case 1005: base = 23.0; amp = 27.0; sin2 = false; break;
+    case 1006: case 1008: base = 29.0; amp = 66.0; break;
+    case 1007: case 1009: base = 28.5; amp = 60.0; break;
+    default: return 0.0;
+  }
+  double cyc = std::fmod(simtime - shift, T);
+  if (cyc < 0) cyc += T;
+  const double tc = cyc / T;
+  if (sin2) {
+    if (tc <= 0.23) {
+      const double sv = std::sin(P * tc / 0.46);
+      return 120.0 + base + amp * sv * sv;
+    }
+    return 120.0 + base + amp * (1.0 + std::sin(2 * P * (tc + 0.54) / 1.54 - P / 2.0)) / 2.0;
+  }
+  if (tc <= 0.3) return 120.0 + base + amp * std::sin(P * tc / 0.6);
+  return 120.0 + base + amp * (1.0 + std::sin(2 * P * (tc + 0.4) / 1.4 - P / 2.0)) / 2.0;
+}
+
+template <class F>
+int dispatch(int storage, F &&f) {
+  switch (storage) {
+    case ZF_STORE_F64: return f(std::integral_constant<int, ZF_STORE_F64>());
+    case ZF_STORE_F32DEV: return f(std::integral_constant<int, ZF_STORE_F32DEV>());
+    default: return f(std::integral_constant<int, ZF_STORE_F16DEV>());
+  }
+}
+
+int require_state(zf_ctx *ctx) {
+  if (!ctx->have_lattice || !ctx->d_f[0]) return fail(ctx, ZF_ERR_STATE, "no lattice uploaded");
+  return ZF_OK;
+}
+
+}  // namespace
+
+size_t store_bytes(int storage) {
+  return storage == ZF_STORE_F64 ? 8 : storage == ZF_STORE_F32DEV ? 4 : 2;
+}
+
+void free_state(zf_ctx *ctx) {
+  dev_free(ctx, &ctx->d_f[0]);
+  dev_free(ctx, &ctx->d_f[1]);
+  dev_free(ctx, &ctx->d_macro);
+  dev_free(ctx, (void **)&ctx->d_force);
+  dev_free(ctx, (void **)&ctx->d_sums);
+  ctx->cur = 0;
+}
+
+int alloc_state(zf_ctx *ctx) {
+  const size_t eb = store_bytes(ctx->cfg.storage);
+  const size_t rb = ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4;
+  int rc;
+  for (int b = 0; b < 2; b++)
+    if ((rc = dev_alloc(ctx, &ctx->d_f[b], eb * Q * ctx->stride))) return rc;
+  if ((rc = dev_alloc(ctx, &ctx->d_macro, rb * 4 * (size_t)std::max(ctx->n_int, 1)))) return rc;
+  const size_t nblk = nblocks(std::max(ctx->n_int, 1), 256);
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_sums, sizeof(double) * (4 * nblk + 4)))) return rc;
+  const double u0[3] = {0, 0, 0};
+  return zf_init_equilibrium(ctx, 1.0, u0);
+}
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+int zf_init_equilibrium(zf_ctx *ctx, double rho, const double u[3]) {
+  if (!ctx || !u) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  rc = dispatch(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using T = typename Store<S>::T;
+    hipLaunchKernelGGL((k_init_eq<S>), dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<T *>(ctx->d_f[0]), static_cast<T *>(ctx->d_f[1]), ctx->stride,
+                       ctx->dommax, rho, u[0], u[1], u[2]);
+    return ZF_OK;
+  });
+  ZF_HIP(ctx, hipGetLastError());
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->cur = 0;
+  return rc;
+}
+
+int zf_upload_populations(zf_ctx *ctx, const double *f) {
+  if (!ctx || !f) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  double *tmp = nullptr;
+  const size_t nbytes = sizeof(double) * Q * (size_t)ctx->dommax;
+  if ((rc = dev_alloc(ctx, (void **)&tmp, nbytes))) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, f, nbytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    dispatch(ctx->cfg.storage, [&](auto tag) {
+      constexpr int S = decltype(tag)::value;
+      using T = typename Store<S>::T;
+      hipLaunchKernelGGL((k_import<S>), dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                         tmp, ctx->dommax, ctx->d_s_of_p, static_cast<T *>(ctx->d_f[0]),
+                         static_cast<T *>(ctx->d_f[1]), ctx->stride);
+      return ZF_OK;
+    });
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)nbytes;
+  dev_free(ctx, (void **)&tmp);
+  ctx->cur = 0;
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "population upload: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_download_populations(zf_ctx *ctx, double *f) {
+  if (!ctx || !f) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  double *tmp = nullptr;
+  const size_t nbytes = sizeof(double) * Q * (size_t)ctx->dommax;
+  if ((rc = dev_alloc(ctx, (void **)&tmp, nbytes))) return rc;
+  dispatch(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using T = typename Store<S>::T;
+    hipLaunchKernelGGL((k_export<S>), dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<const T *>(ctx->d_f[ctx->cur]), ctx->stride, ctx->dommax,
+                       ctx->d_s_of_p, tmp);
+    return ZF_OK;
+  });
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, nbytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)nbytes;
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "population download: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_upload_force(zf_ctx *ctx, const double *force) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  if (!force) {
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->d_force) ctx->device_bytes -= (int64_t)(sizeof(float) * 3 * ctx->stride);
+    dev_free(ctx, (void **)&ctx->d_force);
+    return ZF_OK;
+  }
+  if (!ctx->d_force) {
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_force, sizeof(float) * 3 * ctx->stride))) return rc;
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_force, 3 * ctx->stride);
+  }
+  double *tmp = nullptr;
+  const size_t nbytes = sizeof(double) * 3 * (size_t)ctx->dommax;
+  if ((rc = dev_alloc(ctx, (void **)&tmp, nbytes))) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, force, nbytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_force_import, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                       tmp, ctx->dommax, ctx->d_s_of_p, ctx->d_force, ctx->stride);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)nbytes;
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "force upload: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_download_force(zf_ctx *ctx, double *force) {
+  if (!ctx || !force) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  if (!ctx->d_force) {
+    for (size_t i = 0; i < 3 * (size_t)ctx->dommax; i++) force[i] = 0.0;
+    return ZF_OK;
+  }
+  double *tmp = nullptr;
+  const size_t nbytes = sizeof(double) * 3 * (size_t)ctx->dommax;
+  if ((rc = dev_alloc(ctx, (void **)&tmp, nbytes))) return rc;
+  hipLaunchKernelGGL(k_force_export, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_force, ctx->stride, ctx->dommax, ctx->d_s_of_p, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(force, tmp, nbytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)nbytes;
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "force download: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_lbm_step(zf_ctx *ctx, int32_t iter, int32_t rbc_step) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  return step(ctx, iter, rbc_step);
+}
+
+int zf_lbm_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps) {
+  if (!ctx || nsteps < 0) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  for (int t = 1; t <= nsteps; t++)
+    if ((rc = step(ctx, iter0 + t, 0))) return rc;
+  return ZF_OK;
+}
+
+int zf_lbm_time_kernels(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double out_ms[3]) {
+  if (!ctx || !out_ms || nsteps < 0) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  double t_sc = 0, t_bc = 0;
+  hipEvent_t *ev = ctx->events;  // slots 13..15
+  for (int t = 1; t <= nsteps; t++) {
+    if ((rc = step(ctx, iter0 + t, 0, ev + 13))) return rc;
+    ZF_HIP(ctx, hipEventSynchronize(ev[15]));
+    float a = 0, b = 0;
+    ZF_HIP(ctx, hipEventElapsedTime(&a, ev[13], ev[14]));
+    ZF_HIP(ctx, hipEventElapsedTime(&b, ev[14], ev[15]));
+    t_sc += a;
+    t_bc += b;
+  }
+  out_ms[0] = t_sc;
+  out_ms[1] = t_bc;
+  out_ms[2] = ctx->n_int > 0 ? nsteps : 0;
+  return ZF_OK;
+}
+
+int zf_download_macro(zf_ctx *ctx, double *d, double *u, double *v, double *w) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  const size_t n = (size_t)ctx->dommax;
+  double *tmp = nullptr;
+  if ((rc = dev_alloc(ctx, (void **)&tmp, sizeof(double) * 4 * n))) return rc;
+  dispatch(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using T = typename Store<S>::T;
+    hipLaunchKernelGGL((k_macro<S>), dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<const T *>(ctx->d_f[ctx->cur]), ctx->stride, (int)n,
+                       ctx->d_p_of_s, tmp);
+    return ZF_OK;
+  });
+  hipError_t e = hipGetLastError();
+  double *outs[4] = {d, u, v, w};
+  for (int a = 0; a < 4 && e == hipSuccess; a++)
+    if (outs[a]) e = hipMemcpyAsync(outs[a], tmp + a * n, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)(sizeof(double) * 4 * n);
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "macro download: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_conservation_sums(zf_ctx *ctx, double out[4]) {
+  if (!ctx || !out) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  const int nblk = (int)nblocks(std::max(ctx->n_int, 1), 256);
+  dispatch(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using T = typename Store<S>::T;
+    hipLaunchKernelGGL((k_sums_partial<S>), dim3(nblk), dim3(256), 0, ctx->stream,
+                       static_cast<const T *>(ctx->d_f[ctx->cur]), ctx->stride, ctx->n_int,
+                       ctx->d_sums + 4);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(256), 0, ctx->stream, ctx->d_sums + 4, nblk,
+                       ctx->d_sums);
+    return ZF_OK;
+  });
+  ZF_HIP(ctx, hipGetLastError());
+  ZF_HIP(ctx, hipMemcpyAsync(out, ctx->d_sums, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  out[0] += (double)ctx->n_int;  // sum of the weights, the deviations' offset
+  return ZF_OK;
+}
+
+}  // extern "C"

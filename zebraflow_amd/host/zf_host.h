@@ -1,0 +1,65 @@
+/*
+ * zf_host.h -- host side of the engine (C): synthetic geometry, the
+ * reference's input/output file formats and its step/output loop.
+ * Built into libzfhost.so and the zf_host program; calls libzfgpu.so only
+ * through include/zf.h.
+ */
+#ifndef ZF_HOST_H
+#define ZF_HOST_H
+
+#include <stdint.h>
+
+#include "zf.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- synthetic geometry -----------------------------------------------------
+ * All generators fill a dense box b[(i-1)*jm*km + (j-1)*km + (k-1)] with
+ * -1 (outside), 0 (wall layer: non-fluid nodes with a D3Q19 fluid
+ * neighbour), 1 (fluid) and the inlet/outlet codes of zf.h.  They return the
+ * number of fluid (B==1) nodes, or a negative value on bad arguments. */
+
+/* Straight tube of radius r along `axis` (0=i, 1=j, 2=k), centred in the
+ * cross-section.  periodic: the tube runs through both end faces; otherwise
+ * its end faces carry inlet_code / outlet_code (0 = closed wall). */
+int64_t zfh_geom_tube(int32_t *b, int im, int jm, int km, int axis, double r, int periodic,
+                      int inlet_code, int outlet_code);
+
+/* Zebrafish-trunk vessel pair along i: a dorsal-aorta tube (j centre
+ * 0.25*jm) and a posterior-cardinal-vein tube (j centre 0.75*jm), both of
+ * radius r_main with seeded sinusoidal centrelines of amplitude `amp`,
+ * joined by n_conn segmental connector tubes of radius r_conn at seeded i
+ * positions.  DA: velocity inlet at i=1, pressure outlet at i=im.  PCV:
+ * pressure outlet at i=1, closed (capped) before i=im. */
+int64_t zfh_geom_vessel_pair(int32_t *b, int im, int jm, int km, double r_main, double r_conn,
+                             double amp, int n_conn, uint64_t seed);
+
+/* Branching tree along i: root of radius r_root from i=1 (velocity inlet),
+ * `generations` levels of bifurcations, child radius = ratio * parent,
+ * branch half-angles drawn in [angle_lo, angle_hi] degrees; every leaf that
+ * reaches the i=im face is a pressure outlet, others are capped. */
+int64_t zfh_geom_tree(int32_t *b, int im, int jm, int km, double r_root, double ratio,
+                      int generations, double angle_lo, double angle_hi, uint64_t seed);
+
+/* ---- reference file formats ----------------------------------------------- */
+/* FLUIDdomainBindex.csv (ZebraflowWT.c:2574-2619): header, then
+ * "bvalue,x,y,z" for every node with B >= 0; x = x0 + (i-1)*dx. */
+int zfh_write_bindex_csv(const char *path, const int32_t *b, int im, int jm, int km,
+                         double x0, double y0, double z0, double dx);
+/* Reads it into b (prefilled with -1), index (int)((x-x0)/dx + 1.01)
+ * exactly as ZebraflowWT.c:2611-2613.  Returns rows read or -1. */
+int64_t zfh_read_bindex_csv(const char *path, int32_t *b, int im, int jm, int km, double x0,
+                            double y0, double z0, double dx);
+/* FLUIDdomain-%09d.csv in the OUTPUT() layout (ZebraflowWT.c:362-368):
+ * X,Y,Z,Bindex,U,V,W,D,Bx,By,Bz,EulNeigh,Windex for every B >= 0 node.
+ * u,v,w,d,force are in list order p (force may be NULL). */
+int zfh_write_fluid_csv(const char *path, const int32_t *b, int im, int jm, int km, double x0,
+                        double y0, double z0, double dx, const double *d, const double *u,
+                        const double *v, const double *w, const double *force, int64_t dommax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
