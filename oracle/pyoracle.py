@@ -177,3 +177,76 @@ def wtime():
 
 def max_threads():
     return lib().zfo_max_threads()
+
+
+# ---- membrane (zf_oracle_mesh.c) ------------------------------------------
+class MembParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "len_scale", "kT", "x0", "kbend", "kd", "ka", "kv", "area0", "vol0", "x_max", "x_floor",
+        "cos_cut", "area_lo", "area_hi", "force_limit", "vol_limit", "sliver_len",
+        "sliver_len_min", "sliver_f1", "sliver_f2", "relax", "relax2")]
+
+
+def reference_membrane_params():
+    """SI constants of ZebraflowWT.c:34-36, :124-134, :5654-5655."""
+    return dict(len_scale=1e-6, kT=300 * 1.3806488e-23, x0=0.3154574132, kbend=1.3856406e-19,
+                kd=3.6e-5, ka=7.7e-5, kv=55.0, area0=144.916e-12, vol0=97.5671e-18, x_max=0.95,
+                x_floor=0.02435, cos_cut=0.99, area_lo=0.1, area_hi=10.0, force_limit=1e-10,
+                vol_limit=1e-10, sliver_len=0.05, sliver_len_min=0.001, sliver_f1=1e-14,
+                sliver_f2=1e-11, relax=1.0, relax2=1.0)
+
+
+def memb_params(d):
+    p = MembParams()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+def mesh_topology(nn, tris, nlist=10):
+    tris = np.ascontiguousarray(tris, np.int32)
+    nt = len(tris)
+    cap = 3 * nt
+    edge = np.zeros((cap, 6), np.int32)
+    nbn = np.zeros((nn, nlist), np.int32)
+    nbt = np.zeros((nn, nlist), np.int32)
+    ne = lib().zfo_mesh_topology(nn, nt, _p(tris), _p(edge), cap, _p(nbn), _p(nbt), nlist)
+    assert ne > 0
+    return edge[:ne].copy(), nbn, nbt
+
+
+def mesh_equilibrium(xyz, edge, mp, es, passes=51):
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    edge = np.ascontiguousarray(edge, np.int32)
+    ne = len(edge)
+    out = [np.zeros(ne) for _ in range(6)]
+    lib().zfo_mesh_equilibrium(len(xyz), ne, _p(xyz), _p(edge), C.byref(memb_params(mp)),
+                               C.c_double(es), passes, *[_p(a) for a in out])
+    return dict(zip(("lmax", "plength", "kp", "theta0", "sin0", "cos0"), out))
+
+
+def tri_areas(xyz, tris, len_scale):
+    p = xyz[tris - 1]
+    a = 0.5 * np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]) * len_scale ** 2
+    return np.linalg.norm(a, axis=1)
+
+
+def membrane_forces(xyz_cells, tris, edge, eq, tri_area_eq, mp):
+    """xyz_cells: [ncell, nn, 3].  Returns a dict of per-cell outputs."""
+    x = np.ascontiguousarray(xyz_cells, np.float64)
+    nc, nn, _ = x.shape
+    nt, ne = len(tris), len(edge)
+    out = dict(force=np.zeros((nc, nn, 3)), node_normal=np.zeros((nc, nn, 3)),
+               tri_area=np.zeros((nc, nt)), tri_normal=np.zeros((nc, nt, 3)),
+               tri_centroid=np.zeros((nc, nt, 3)), cell_area=np.zeros(nc), cell_vol=np.zeros(nc),
+               cell_centroid=np.zeros((nc, 3)), guard_fail=np.zeros(nc, np.int32))
+    t = np.ascontiguousarray(tris, np.int32)
+    e = np.ascontiguousarray(edge, np.int32)
+    arrs = [np.ascontiguousarray(eq[k], np.float64) for k in ("lmax", "plength", "kp", "sin0", "cos0")]
+    tae = np.ascontiguousarray(tri_area_eq, np.float64)
+    lib().zfo_membrane_forces(nc, nn, nt, ne, _p(t), _p(e), *[_p(a) for a in arrs], _p(tae),
+                              C.byref(memb_params(mp)), _p(x),
+                              *[_p(out[k]) for k in ("force", "node_normal", "tri_area",
+                                                     "tri_normal", "tri_centroid", "cell_area",
+                                                     "cell_vol", "cell_centroid", "guard_fail")])
+    return out

@@ -107,6 +107,41 @@ double zfo_feq(int c, double rho, double u, double v, double w);
  * (pressure in the reference's units, before /LBM_PRESSURE_SCALE). */
 double zfo_bc_pressure(int code, double simtime);
 
+/* ---- RBC membrane (zf_oracle_mesh.c) -------------------------------------- */
+typedef struct zfo_memb_params {
+  double len_scale;          /* force-formula length per coordinate unit (ref 1e-6) */
+  double kT;                 /* BOLTZ*TEMPERATURE */
+  double x0;                 /* e_ratio_eq, rest length / max length */
+  double kbend, kd, ka, kv;  /* bending, local area, global area, volume */
+  double area0, vol0;        /* global area and volume targets (RBC_SURF, MCV) */
+  double x_max, x_floor;     /* e_ratio clamp 0.95, length floor 0.02435 */
+  double cos_cut;            /* |cos theta| above which bending is off (0.99) */
+  double area_lo, area_hi;   /* triangle area clamp factors (0.1, 10) */
+  double force_limit;        /* per-component guard of force terms (1e-10) */
+  double vol_limit;          /* guard of the volume-gradient terms (1e-10) */
+  double sliver_len, sliver_len_min, sliver_f1, sliver_f2; /* 0.05, 0.001, 1e-14, 1e-11 */
+  double relax, relax2;      /* relaxfactor, relaxfactor2 (1, 1) */
+} zfo_memb_params;
+
+/* a7.  tri: 3*nt (1-based).  edge: 6*ecap rows n1 n2 n3 n4 t1 t2.
+ * nbr_node / nbr_tri: nn rows of nlist ints (count, entries).  Returns the
+ * edge count or -1.  Pinned by reference goldens (zebrafish mesh). */
+int zfo_mesh_topology(int nn, int nt, const int32_t *tri, int32_t *edge, int ecap,
+                      int32_t *nbr_node, int32_t *nbr_tri, int nlist);
+/* a8.  xyz: 3*nn node-major rest coordinates. */
+void zfo_mesh_equilibrium(int nn, int ne, const double *xyz, const int32_t *edge,
+                          const zfo_memb_params *mp, double es, int passes, double *lmax,
+                          double *plength, double *kp, double *theta0, double *sin0,
+                          double *cos0);
+/* a9-a12.  xyz: ncell*nn*3 (cell, node, axis).  Outputs per cell. */
+void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
+                         const int32_t *edge, const double *lmax, const double *plength,
+                         const double *kp, const double *sin0, const double *cos0,
+                         const double *tri_area_eq, const zfo_memb_params *mp, const double *xyz,
+                         double *force, double *node_normal, double *tri_area, double *tri_normal,
+                         double *tri_centroid, double *cell_area, double *cell_vol,
+                         double *cell_centroid, int *guard_fail);
+
 /* Wall-clock helper for the CPU baseline. */
 double zfo_wtime(void);
 int zfo_max_threads(void);

@@ -29,3 +29,26 @@ def tree(shape, r_root, ratio, generations, a_lo, a_hi, seed):
                                           a_lo, a_hi, seed)
     assert n > 0
     return b
+
+
+def rbc_mesh(radius=4.0, level=3, biconcave=True):
+    """Evans-Fung RBC mesh: (xyz [nn,3], tris [nt,3] 1-based)."""
+    import ctypes as C
+    cap_v, cap_t = 10 * 4 ** level + 2, 20 * 4 ** level
+    xyz = np.zeros((cap_v, 3))
+    tri = np.zeros((cap_t, 3), np.int32)
+    nt = C.c_int32()
+    nv = load_host_library().zfh_mesh_icosphere(level, radius, int(biconcave), xyz.ctypes.data,
+                                                tri.ctypes.data, cap_v, cap_t, C.byref(nt))
+    assert nv == cap_v and nt.value == cap_t
+    return xyz, tri
+
+
+def place_cells(b, ref, r_cell, gap, ncell, seed, max_tries=200000):
+    im, jm, km = b.shape
+    b = np.ascontiguousarray(b, np.int32)
+    ref = np.ascontiguousarray(ref, np.float64)
+    out = np.zeros((ncell, len(ref), 3))
+    n = load_host_library().zfh_place_cells(b.ctypes.data, im, jm, km, ref.ctypes.data, len(ref),
+                                            r_cell, gap, ncell, seed, max_tries, out.ctypes.data)
+    return out[:n]

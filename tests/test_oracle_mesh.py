@@ -1,0 +1,175 @@
+"""Oracle membrane (a7-a12) pinned against the reference and the mathematics.
+
+a7 (topology) is bit-exact with the reference's CGSM_ZFISHRBC_DATA_GEN run on
+the zebrafish mesh it ships (tests/golden/mesh_zfish.npz, written by the
+reference's own code).  a8 (equilibrium parameters) matches the reference's
+FP32 values within FP32 rounding.  a9-a12 (forces) have no reference output;
+they are pinned by: zero force at the rest shape, zero net force and torque
+per cell, and forces equal to minus the gradient of the WLC/POW, bending and
+area energies (central finite differences), plus closed-form area/volume of
+a sphere.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import geometry
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "mesh_zfish.npz")
+
+
+def test_topology_matches_reference_zebrafish_mesh():
+    g = np.load(GOLD)
+    edge, nbn, nbt = po.mesh_topology(len(g["nodes"]), g["tris"])
+    assert len(edge) == 1506
+    np.testing.assert_array_equal(edge, g["edge"])
+    np.testing.assert_array_equal(nbn, g["nbr_node"])
+    np.testing.assert_array_equal(nbt, g["nbr_tri"])
+
+
+def test_equilibrium_parameters_match_reference():
+    g = np.load(GOLD)
+    mp = po.reference_membrane_params()
+    eq = po.mesh_equilibrium(g["nodes"], g["edge"], mp, es=3e-6)
+    np.testing.assert_allclose(eq["lmax"], g["lmax"], rtol=2e-6)
+    np.testing.assert_allclose(eq["kp"], g["kp"], rtol=5e-4)
+    np.testing.assert_allclose(eq["plength"], g["plength"], rtol=5e-4)
+    np.testing.assert_allclose(eq["cos0"], g["cos0"], rtol=0, atol=2e-5)
+    np.testing.assert_allclose(eq["sin0"], g["sin0"], rtol=0, atol=2e-3)
+    np.testing.assert_allclose(eq["theta0"], g["theta0"], rtol=0, atol=2e-3)
+
+
+def test_topology_closed_mesh_invariants():
+    xyz, tris = geometry.rbc_mesh()
+    edge, nbn, nbt = po.mesh_topology(len(xyz), tris)
+    V, E, F = len(xyz), len(edge), len(tris)
+    assert (V, E, F) == (642, 1920, 1280)
+    assert V - E + F == 2 and 2 * E == 3 * F
+    assert (edge[:, 2:] > 0).all()                     # every edge has two triangles
+    assert (nbn[:, 0] == nbt[:, 0]).all()              # closed fans: degree = triangles
+
+
+def lattice_params(xyz, tris, es=5e-3, kb=2e-3, **over):
+    """Lattice-unit membrane constants: the reference's ratios to the shear
+    modulus (kd = 12 Es, ka = 25.67 Es, kv = 9.17 Es per lu, kT = 0.0036 Es
+    l0^2), rest area and volume as the global targets."""
+    edge, _, _ = po.mesh_topology(len(xyz), tris)
+    l0 = np.linalg.norm(xyz[edge[:, 0] - 1] - xyz[edge[:, 1] - 1], axis=1).mean()
+    p = xyz[tris - 1]
+    av = 0.5 * np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    mp = dict(po.reference_membrane_params())
+    mp.update(len_scale=1.0, kT=0.0036 * es * l0 * l0, kbend=2 / np.sqrt(3) * kb, kd=12 * es,
+              ka=25.67 * es, kv=9.17 * es, area0=np.linalg.norm(av, axis=1).sum(),
+              vol0=(av * p.mean(1)).sum() / 3, force_limit=60 * es * l0, vol_limit=1e30,
+              sliver_len=0.1, sliver_len_min=0.002, sliver_f1=0.01 * es * l0,
+              sliver_f2=6 * es * l0)
+    mp.update(over)
+    eq = po.mesh_equilibrium(xyz, edge, mp, es)
+    tae = po.tri_areas(xyz, tris, 1.0)
+    return mp, edge, eq, tae
+
+
+def test_sphere_area_and_volume_closed_form():
+    xyz, tris = geometry.rbc_mesh(radius=3.0, level=4, biconcave=False)
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    out = po.membrane_forces(xyz[None], tris, edge, eq, tae, mp)
+    # inscribed polyhedron converges to the sphere from below (O(h^2))
+    assert abs(out["cell_area"][0] / (4 * np.pi * 9) - 1) < 3e-3
+    assert abs(out["cell_vol"][0] / (4 / 3 * np.pi * 27) - 1) < 5e-3
+    np.testing.assert_allclose(out["cell_centroid"][0], 0, atol=1e-12)
+
+
+def test_rest_shape_is_force_free():
+    xyz, tris = geometry.rbc_mesh()
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    out = po.membrane_forces(xyz[None], tris, edge, eq, tae, mp)
+    fmax = np.abs(out["force"]).max()
+    assert out["guard_fail"][0] == 0
+    assert fmax < 1e-9 * mp["kd"], fmax
+
+
+def deformed(seed=5, amp=0.15):
+    xyz, tris = geometry.rbc_mesh()
+    rng = np.random.default_rng(seed)
+    stretch = xyz * np.array([1.15, 0.9, 1.05])
+    return xyz, stretch + amp * rng.standard_normal(xyz.shape) * 0.3, tris
+
+
+def test_net_force_and_torque_vanish():
+    xyz, x1, tris = deformed()
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    shift = np.array([10.0, -3.0, 7.0])
+    out = po.membrane_forces((x1 + shift)[None], tris, edge, eq, tae, mp)
+    F = out["force"][0]
+    scale = np.abs(F).max()
+    assert scale > 1e-6
+    np.testing.assert_allclose(F.sum(0), 0, atol=1e-10 * scale * len(F))
+    # torque about the centroid; the volume term is not a pure gradient in the
+    # reference (see test below), so torque is checked with it switched off
+    mp0 = dict(mp, kv=0.0)
+    F0 = po.membrane_forces((x1 + shift)[None], tris, edge, eq, tae, mp0)["force"][0]
+    r = x1 - x1.mean(0)
+    np.testing.assert_allclose(np.cross(r, F0).sum(0), 0, atol=1e-9 * np.abs(F0).max() * len(F0))
+
+
+def energy(x, tris, edge, eq, tae, mp):
+    """Energies whose negative gradients the reference's force terms are:
+    WLC + POW (m=2) springs, bending 2 kb(1-cos(theta-theta0)), local area
+    kd(A-A0)^2/(4A0) and global area ka(At-A0)^2/(4A0).  The factor 2 on
+    bending (and the 1/4 on the area terms) is the reference's: its
+    cross-product forms use the triangle area A where the normal's
+    derivative has |(x2-x1)x(x3-x1)| = 2A (ZebraflowWT.c:7001-7058)."""
+    kT, L = mp["kT"], mp["len_scale"]
+    d = x[edge[:, 0] - 1] - x[edge[:, 1] - 1]
+    l = np.linalg.norm(d, axis=1) * L
+    xr = l / eq["lmax"]
+    u_wlc = kT * eq["lmax"] / (4 * eq["plength"]) * (3 * xr ** 2 - 2 * xr ** 3) / (1 - xr)
+    u_pow = eq["kp"] / l
+    p = x[tris - 1]
+    av = 0.5 * np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]) * L * L
+    A = np.linalg.norm(av, axis=1)
+    n = av / A[:, None]
+    c = p.mean(1)
+    t1, t2 = edge[:, 4] - 1, edge[:, 5] - 1
+    ct = np.clip((n[t1] * n[t2]).sum(1), -1, 1)
+    chk = ((n[t1] - n[t2]) * (c[t1] - c[t2])).sum(1)
+    st = np.where(chk >= 0, 1, -1) * np.sqrt(1 - ct * ct)
+    u_bend = 2 * mp["kbend"] * (1 - (ct * eq["cos0"] + st * eq["sin0"]))
+    u_area = mp["kd"] * (A - tae) ** 2 / (4 * tae)
+    u_glob = mp["ka"] * (A.sum() - mp["area0"]) ** 2 / (4 * mp["area0"])
+    return u_wlc.sum() + u_pow.sum() + u_bend.sum() + u_area.sum() + u_glob
+
+
+def test_forces_are_energy_gradients():
+    xyz, x1, tris = deformed(amp=0.1)
+    # cos_cut = 1 removes the bending cut-off so the energy is smooth
+    mp, edge, eq, tae = lattice_params(xyz, tris, kv=0.0, cos_cut=1.0)
+    F = po.membrane_forces(x1[None], tris, edge, eq, tae, mp)["force"][0]
+    rng = np.random.default_rng(11)
+    for node in rng.choice(len(x1), 12, replace=False):
+        for ax in range(3):
+            h = 1e-5
+            xp, xm = x1.copy(), x1.copy()
+            xp[node, ax] += h
+            xm[node, ax] -= h
+            g = (energy(xp, tris, edge, eq, tae, mp) - energy(xm, tris, edge, eq, tae, mp)) / (2 * h)
+            assert abs(F[node, ax] + g) <= 1e-5 * np.abs(F).max() + 1e-4 * abs(g), (node, ax, F[node, ax], -g)
+
+
+def test_volume_force_restores_volume():
+    """The volume term pushes outward when compressed (sign) and scales with
+    kv (V-V0)/V0 (linearity)."""
+    xyz, tris = geometry.rbc_mesh()
+    mp, edge, eq, tae = lattice_params(xyz, tris, kd=0, ka=0, kbend=0)
+    x = xyz * 0.98
+    eq0 = dict(eq)
+    eq0["kp"] = np.zeros_like(eq["kp"])
+    mp2 = dict(mp, kT=0.0)
+    out = po.membrane_forces(x[None], tris, edge, eq0, tae, mp2)
+    F = out["force"][0]
+    outward = (F * x).sum(1)
+    assert (outward > 0).mean() > 0.95
+    out2 = po.membrane_forces(x[None], tris, edge, eq0, tae, dict(mp2, kv=2 * mp2["kv"]))
+    np.testing.assert_allclose(out2["force"][0], 2 * F, rtol=1e-12, atol=1e-18)

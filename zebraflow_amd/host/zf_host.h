@@ -43,6 +43,21 @@ int64_t zfh_geom_vessel_pair(int32_t *b, int im, int jm, int km, double r_main, 
 int64_t zfh_geom_tree(int32_t *b, int im, int jm, int km, double r_root, double ratio,
                       int generations, double angle_lo, double angle_hi, uint64_t seed);
 
+/* ---- membrane meshes ---------------------------------------------------------
+ * Icosphere of `level` subdivisions (level 3: 642 nodes, 1280 triangles),
+ * optionally mapped to the Evans-Fung biconcave disc, scaled to `radius`.
+ * xyz: 3*node_cap node-major; tri: 3*tri_cap, 1-based, outward winding.
+ * Returns the node count (triangles in *n_tri) or -1. */
+int zfh_mesh_icosphere(int level, double radius, int biconcave, double *xyz, int32_t *tri,
+                       int node_cap, int tri_cap, int32_t *n_tri);
+/* Seeded non-overlapping placement of up to ncell copies of the reference
+ * shape `ref` (nn nodes, centred at the origin) inside interior fluid of b.
+ * Positions are lattice coordinates, node (i,j,k) at (i-1,j-1,k-1).
+ * xyz_out: ncell*nn*3.  Returns the number placed. */
+int zfh_place_cells(const int32_t *b, int im, int jm, int km, const double *ref, int nn,
+                    double r_cell, double gap, int ncell, uint64_t seed, int max_tries,
+                    double *xyz_out);
+
 /* ---- reference file formats ----------------------------------------------- */
 /* FLUIDdomainBindex.csv (ZebraflowWT.c:2574-2619): header, then
  * "bvalue,x,y,z" for every node with B >= 0; x = x0 + (i-1)*dx. */

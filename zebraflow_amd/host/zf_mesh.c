@@ -1,0 +1,149 @@
+/*
+ * zf_mesh.c -- RBC membrane meshes and cell placement.
+ *
+ * The reference reads one fixed membrane mesh (ZFISHRBCNODES/TRIANGLES.dat,
+ * ZebraflowWT.c:1601-1629).  The benchmark cells are Evans-Fung biconcave
+ * discs meshed from a subdivided icosahedron: level 3 gives 642 nodes, 1280
+ * triangles and 1920 edges.  Triangles are wound counter-clockwise seen
+ * from outside, so (n2-n1) x (n3-n1) is the outward normal, the orientation
+ * CGSM_AREA relies on (ZebraflowWT.c:5808-5831).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "zf_host.h"
+
+typedef struct { int a, b, mid; } emid_t;
+
+static int find_or_add_mid(emid_t *tab, int cap, int a, int b, double *xyz, int *nv) {
+  if (a > b) { int t = a; a = b; b = t; }
+  unsigned h = ((unsigned)a * 73856093u ^ (unsigned)b * 19349663u) % (unsigned)cap;
+  while (tab[h].mid >= 0) {
+    if (tab[h].a == a && tab[h].b == b) return tab[h].mid;
+    h = (h + 1) % (unsigned)cap;
+  }
+  const int m = (*nv)++;
+  double p[3], n = 0;
+  for (int d = 0; d < 3; d++) { p[d] = 0.5 * (xyz[3 * a + d] + xyz[3 * b + d]); n += p[d] * p[d]; }
+  n = sqrt(n);
+  for (int d = 0; d < 3; d++) xyz[3 * m + d] = p[d] / n;
+  tab[h].a = a; tab[h].b = b; tab[h].mid = m;
+  return m;
+}
+
+int zfh_mesh_icosphere(int level, double radius, int biconcave, double *xyz, int32_t *tri,
+                       int node_cap, int tri_cap, int32_t *n_tri) {
+  if (level < 0 || level > 6 || !xyz || !tri || !n_tri) return -1;
+  int nv = 12, nt = 20;
+  int need_v = 10 * (1 << (2 * level)) + 2, need_t = 20 * (1 << (2 * level));
+  if (node_cap < need_v || tri_cap < need_t) return -1;
+  const double t = (1.0 + sqrt(5.0)) / 2.0;
+  const double v0[12][3] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0},
+                            {0, -1, t}, {0, 1, t}, {0, -1, -t}, {0, 1, -t},
+                            {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+  const int f0[20][3] = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11},
+                         {1, 5, 9},  {5, 11, 4}, {11, 10, 2}, {10, 7, 6}, {7, 1, 8},
+                         {3, 9, 4},  {3, 4, 2},  {3, 2, 6},   {3, 6, 8},  {3, 8, 9},
+                         {4, 9, 5},  {2, 4, 11}, {6, 2, 10},  {8, 6, 7},  {9, 8, 1}};
+  for (int i = 0; i < 12; i++) {
+    const double n = sqrt(v0[i][0] * v0[i][0] + v0[i][1] * v0[i][1] + v0[i][2] * v0[i][2]);
+    for (int d = 0; d < 3; d++) xyz[3 * i + d] = v0[i][d] / n;
+  }
+  int32_t *cur = malloc(sizeof(int32_t) * 3 * need_t), *nxt = malloc(sizeof(int32_t) * 3 * need_t);
+  for (int f = 0; f < 20; f++)
+    for (int s = 0; s < 3; s++) cur[3 * f + s] = f0[f][s];
+  for (int l = 0; l < level; l++) {
+    const int cap = 8 * nt + 1;
+    emid_t *tab = malloc(sizeof(emid_t) * cap);
+    for (int i = 0; i < cap; i++) tab[i].mid = -1;
+    for (int f = 0; f < nt; f++) {
+      const int a = cur[3 * f], b = cur[3 * f + 1], c = cur[3 * f + 2];
+      const int ab = find_or_add_mid(tab, cap, a, b, xyz, &nv);
+      const int bc = find_or_add_mid(tab, cap, b, c, xyz, &nv);
+      const int ca = find_or_add_mid(tab, cap, c, a, xyz, &nv);
+      const int q[4][3] = {{a, ab, ca}, {b, bc, ab}, {c, ca, bc}, {ab, bc, ca}};
+      for (int k = 0; k < 4; k++)
+        for (int s = 0; s < 3; s++) nxt[3 * (4 * f + k) + s] = q[k][s];
+    }
+    free(tab);
+    nt *= 4;
+    int32_t *sw = cur; cur = nxt; nxt = sw;
+  }
+  /* Evans-Fung biconcave profile on the unit sphere (z = +-sqrt(1-r^2)):
+   * z' = 0.5 z (c0 + c1 r^2 + c2 r^4), then scale by the radius. */
+  const double c0 = 0.2072, c1 = 2.0026, c2 = -1.1228;
+  for (int i = 0; i < nv; i++) {
+    double *p = &xyz[3 * i];
+    if (biconcave) {
+      const double r2 = p[0] * p[0] + p[1] * p[1];
+      p[2] = 0.5 * p[2] * (c0 + c1 * r2 + c2 * r2 * r2);
+    }
+    for (int d = 0; d < 3; d++) p[d] *= radius;
+  }
+  for (int f = 0; f < nt; f++)
+    for (int s = 0; s < 3; s++) tri[3 * f + s] = cur[3 * f + s] + 1; /* 1-based */
+  free(cur);
+  free(nxt);
+  *n_tri = nt;
+  return nv;
+}
+
+/* Rotation from three seeded angles, then translate. */
+static void place(const double *ref, int nn, const double c[3], double a, double b, double g,
+                  double *out) {
+  const double ca = cos(a), sa = sin(a), cb = cos(b), sb = sin(b), cg = cos(g), sg = sin(g);
+  const double R[3][3] = {{ca * cb, ca * sb * sg - sa * cg, ca * sb * cg + sa * sg},
+                          {sa * cb, sa * sb * sg + ca * cg, sa * sb * cg - ca * sg},
+                          {-sb, cb * sg, cb * cg}};
+  for (int n = 0; n < nn; n++)
+    for (int d = 0; d < 3; d++)
+      out[3 * n + d] = c[d] + R[d][0] * ref[3 * n] + R[d][1] * ref[3 * n + 1] + R[d][2] * ref[3 * n + 2];
+}
+
+static uint64_t sm(uint64_t *s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static double ur(uint64_t *s) { return (sm(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+/* Random non-overlapping placement: each cell's bounding sphere (radius
+ * r_cell + gap) lies inside interior fluid (every lattice node within it has
+ * 0<B<10) and does not meet another cell's.  Returns cells placed. */
+int zfh_place_cells(const int32_t *b, int im, int jm, int km, const double *ref, int nn,
+                    double r_cell, double gap, int ncell, uint64_t seed, int max_tries,
+                    double *xyz_out) {
+  uint64_t s = seed;
+  double *cen = malloc(sizeof(double) * 3 * (ncell > 0 ? ncell : 1));
+  int placed = 0;
+  const double rr = r_cell + gap;
+  const int ri = (int)ceil(rr);
+  for (int tries = 0; tries < max_tries && placed < ncell; tries++) {
+    const double c[3] = {rr + 1 + ur(&s) * (im - 3 - 2 * rr), rr + 1 + ur(&s) * (jm - 3 - 2 * rr),
+                         rr + 1 + ur(&s) * (km - 3 - 2 * rr)};
+    int ok = 1;
+    for (int q = 0; q < placed && ok; q++) {
+      const double dx = c[0] - cen[3 * q], dy = c[1] - cen[3 * q + 1], dz = c[2] - cen[3 * q + 2];
+      if (dx * dx + dy * dy + dz * dz < 4 * rr * rr) ok = 0;
+    }
+    for (int i = (int)floor(c[0]) - ri; i <= (int)ceil(c[0]) + ri && ok; i++)
+      for (int j = (int)floor(c[1]) - ri; j <= (int)ceil(c[1]) + ri && ok; j++)
+        for (int k = (int)floor(c[2]) - ri; k <= (int)ceil(c[2]) + ri && ok; k++) {
+          const double dx = i - c[0], dy = j - c[1], dz = k - c[2];
+          if (dx * dx + dy * dy + dz * dz > (rr + 1) * (rr + 1)) continue;
+          if (i < 0 || i >= im || j < 0 || j >= jm || k < 0 || k >= km) { ok = 0; break; }
+          const int v = b[((size_t)i * jm + j) * km + k];
+          if (!(v > 0 && v < 10)) ok = 0;
+        }
+    if (!ok) continue;
+    for (int d = 0; d < 3; d++) cen[3 * placed + d] = c[d];
+    place(ref, nn, c, 2 * M_PI * ur(&s), acos(2 * ur(&s) - 1), 2 * M_PI * ur(&s),
+          xyz_out + (size_t)3 * nn * placed);
+    placed++;
+  }
+  free(cen);
+  return placed;
+}
