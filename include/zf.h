@@ -139,6 +139,129 @@ int zf_conservation_sums(zf_ctx *ctx, double out[4]);
  * the boundary kernel, out_ms[2] = launches of the stream-collide kernel. */
 int zf_lbm_time_kernels(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double out_ms[3]);
 
+/* ---- RBC membranes ---------------------------------------------------------
+ * Constants of the membrane model (ZebraflowWT.c:34-36, :118-134, :5654).
+ * len_scale converts a coordinate difference to the force formulas' length
+ * (1e-6 m per um in the reference; 1 in lattice units). */
+typedef struct zf_membrane_config {
+  double len_scale, kT, x0, es;     /* es: shear modulus of the WLC-POW fit */
+  double kbend, kd, ka, kv;
+  double area0, vol0;               /* global targets (RBC_SURF, MCV) */
+  double x_max, x_floor, cos_cut, area_lo, area_hi;
+  double force_limit, vol_limit;    /* per-component guards (1e-10) */
+  double sliver_len, sliver_len_min, sliver_f1, sliver_f2;
+  double relax, relax2;
+  int32_t passes;                   /* fixed-point passes (51, :2338-2346) */
+  int32_t max_cells;
+} zf_membrane_config;
+
+/* Reference SI constants (the zebrafish cell).  es = 3e-6 N/m. */
+void zf_membrane_config_reference(zf_membrane_config *mc);
+
+/* Builds the mesh topology (CGSM_ZFISHRBC_DATA_GEN, ZebraflowWT.c:1632-1901),
+ * the equilibrium parameters from the rest shape (CGSM_FILAMENT_PARAMETERS /
+ * CGSM_EQUILIBRIUM_METRICS, :1930-2223, :2338-2346) and rest triangle areas
+ * (:2387-2460) on the host, and uploads them.  tri: 3*nt 1-based node ids
+ * wound outward; rest_xyz: 3*nn. */
+int zf_membrane_setup(zf_ctx *ctx, const zf_membrane_config *mc, int32_t nn, int32_t nt,
+                      const int32_t *tri, const double *rest_xyz);
+/* edge: 6*ne (n1 n2 n3 n4 t1 t2, 1-based); nbr_node / nbr_tri: nn rows of
+ * 10 ints (count, up to 9 entries) as NEIGHNODE_LIST / NEIGHTRI_LIST. */
+int zf_download_mesh_topology(zf_ctx *ctx, int32_t *ne, int32_t *edge, int32_t *nbr_node,
+                              int32_t *nbr_tri);
+/* Per edge: LMAX, Plength, kp, Theta0, sin(Theta0), cos(Theta0). */
+int zf_download_mesh_equilibrium(zf_ctx *ctx, double *lmax, double *plength, double *kp,
+                                 double *theta0, double *sin0, double *cos0);
+/* Cell node coordinates, ncell*nn*3 (cell, node, axis); resets velocities. */
+int zf_upload_cells(zf_ctx *ctx, int32_t ncell, const double *xyz);
+int zf_download_cells(zf_ctx *ctx, double *xyz, double *vel, int32_t *status);
+
+/* Membrane forces of every live cell: triangle geometry and node normals
+ * (CGSM_AREA, :5757-5851, :6089-6093), anti-sliver terms (:6042-6086), cell
+ * area/centroid (:6717-6796), spring, bending and area terms (:6800-7384) and
+ * the volume constraint (:7512-7535).  Cell area, volume and centroid are
+ * FP64 reductions on the device. */
+int zf_membrane_forces(zf_ctx *ctx);
+/* force, node_normal: ncell*nn*3; cell_area, cell_vol: ncell; centroid:
+ * ncell*3; guard_fail: ncell (terms dropped by the guards).  Any may be NULL. */
+int zf_download_membrane(zf_ctx *ctx, double *force, double *node_normal, double *cell_area,
+                         double *cell_vol, double *cell_centroid, int32_t *guard_fail);
+
+/* ---- immersed-boundary coupling ------------------------------------------- */
+typedef struct zf_ibm_config {
+  double x0, y0, z0, dx;  /* node (i,j,k) at (x0+(i-1)dx, ...) (X0, Y0, Z0, DX) */
+  double vel_scale;       /* VEL_SCALE */
+  double force_scale;     /* 1/UNITMASS/LBM_BFORCE_SCALE */
+  double gamma_p;         /* collision dissipation: F_fluid = gamma_p v_rep + F */
+  double dt_advance;      /* DT*SOLVERSTAGGER */
+  double vmax;            /* a cell is removed above this node speed (5e-3) */
+  int32_t rbc_interval;   /* SOLVERSTAGGER: membrane/IBM every n-th iteration */
+  int32_t reserved;
+} zf_ibm_config;
+
+/* Reference values (ZebraflowWT.c:40-74, :95-97, :124, :151). */
+void zf_ibm_config_reference(zf_ibm_config *ic);
+int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic);
+
+/* Node velocities from the fluid: the 4-point cosine kernel with the
+ * reference's spherical cut-off over fluid nodes 0 < B < 4998, using D,U of
+ * the current populations (ZebraflowWT.c:8702-8996, macro of :6606-6614).
+ * Overwrites the node velocities. */
+int zf_ibm_interpolate(zf_ctx *ctx);
+/* Node velocities += v_rep, then F_fluid = gamma_p v_rep + F_Total spread
+ * onto every B >= 0 node (ZebraflowWT.c:10746-10867).  The force field is
+ * overwritten: the reference zeroes it on the same iteration (:6622-6624).
+ * Deterministic: nodes are binned into 4^3 lattice tiles and each tile
+ * gathers from its 27 neighbours in a fixed order; no atomics. */
+int zf_ibm_spread(zf_ctx *ctx);
+/* x += v*dt_advance; a cell with a node faster than vmax (or NaN) is
+ * removed (ZebraflowWT.c:11975-11989).  *n_removed may be NULL. */
+int zf_advance_cells(zf_ctx *ctx, int32_t *n_removed);
+/* Overrides for tests and external coupling: F_Total and v_rep, ncell*nn*3. */
+int zf_upload_membrane_force(zf_ctx *ctx, const double *force);
+int zf_upload_repulsion(zf_ctx *ctx, const double *vrep);
+int zf_download_repulsion(zf_ctx *ctx, double *vrep);
+
+/* ---- cell list and repulsion ---------------------------------------------- */
+typedef struct zf_repulsion_config {
+  double bin_cutoff;   /* a triangle enters its nearest node's bin below this (1 um) */
+  double k_cell;       /* cell-cell repulsion speed scale (8.333e-7) */
+  double cutoff;       /* cell-cell range (1 um) */
+  double search;       /* triangle centroid search radius (2 um) */
+  double k_wall;       /* wall repulsion speed scale (1e-4) */
+  double wall_d0, wall_dmax, wall_search;  /* 0.25, 1.0, 1.5 */
+} zf_repulsion_config;
+
+void zf_repulsion_config_reference(zf_repulsion_config *rc);
+int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc);
+/* Lumen wall mesh (the reference's WallMesh*.dat, ZebraflowWT.c:3715-4643):
+ * node coordinates (3*nwn), triangles (3*nwt, 1-based), unit normals into
+ * the fluid (3*nwt).  Builds the node-to-triangle lists (:3900-3935) and the
+ * W_index map of each lattice node to its nearest wall node within the 9^3
+ * window (:3715-3826) on the device. */
+int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
+                  const int32_t *wall_tri, const double *wall_normal);
+/* W_index over the box (1-based wall node or 0). */
+int zf_download_windex(zf_ctx *ctx, int32_t *windex);
+/* Triangle cell list (EulNeigh, ZebraflowWT.c:5851-5900) from the triangle
+ * centroids of the last membrane evaluation: bin[c*nt + t] = DOMID of the
+ * triangle's bin (1-based) or 0.  Built as (bin, triangle) sorted keys, so
+ * each bin lists its triangles in (cell, triangle) order. */
+int zf_cell_list(zf_ctx *ctx);
+int zf_download_cell_list(zf_ctx *ctx, int32_t *bin);
+/* v_rep = cell-wall (ZebraflowWT.c:8998-9142, when a wall is set up) +
+ * cell-cell (:9913-10052) repulsion velocities of every live node. */
+int zf_repulsion(zf_ctx *ctx);
+/* The wall triangle chosen for each node (NODE_WALLNEIGH), ncell*nn,
+ * 1-based or 0. */
+int zf_download_wall_neighbour(zf_ctx *ctx, int32_t *nearest_wall);
+
+/* One iteration in the reference's order (ZebraflowWT.c:5646-12412): on an
+ * iteration divisible by rbc_interval, the LBM step (clearing the force
+ * field), membrane forces, interpolation, repulsion, spreading and the
+ * position update; otherwise the LBM step alone. */
+int zf_step(zf_ctx *ctx, int32_t iter);
+
 /* ---- timing and synchronisation ------------------------------------------ */
 int zf_sync(zf_ctx *ctx);
 /* Record HIP event `slot` (0..15) on the context stream. */

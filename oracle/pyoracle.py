@@ -250,3 +250,120 @@ def membrane_forces(xyz_cells, tris, edge, eq, tri_area_eq, mp):
                                                      "tri_normal", "tri_centroid", "cell_area",
                                                      "cell_vol", "cell_centroid", "guard_fail")])
     return out
+
+
+# ---- immersed boundary (zf_oracle_ibm.c) ----------------------------------
+class IbmParams(C.Structure):
+    _fields_ = [("im", C.c_int), ("jm", C.c_int), ("km", C.c_int)] + [
+        (n, C.c_double) for n in ("x0", "y0", "z0", "dx", "vel_scale", "force_scale",
+                                  "dt_advance", "vmax")]
+
+
+def ibm_params(shape, **kw):
+    p = IbmParams()
+    p.im, p.jm, p.km = shape
+    d = dict(x0=0.0, y0=0.0, z0=0.0, dx=1.0, vel_scale=1.0, force_scale=1.0, dt_advance=1.0,
+             vmax=1e30)
+    d.update(kw)
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+def ibm_interpolate(p, b, domid, uvw, xyz):
+    b = np.ascontiguousarray(b, np.int32)
+    domid = np.ascontiguousarray(domid, np.int32)
+    u = [np.ascontiguousarray(a, np.float64) for a in uvw]
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    v = np.zeros_like(x)
+    lib().zfo_ibm_interpolate(C.byref(p), _p(b), _p(domid), _p(u[0]), _p(u[1]), _p(u[2]),
+                              C.c_int64(len(x)), _p(x), _p(v))
+    return v.reshape(np.shape(xyz))
+
+
+def ibm_spread(p, b, domid, dommax, xyz, ffluid, force=None):
+    b = np.ascontiguousarray(b, np.int32)
+    domid = np.ascontiguousarray(domid, np.int32)
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(ffluid, np.float64).reshape(-1, 3)
+    out = np.zeros((3, dommax)) if force is None else force
+    lib().zfo_ibm_spread(C.byref(p), _p(b), _p(domid), C.c_int64(dommax), C.c_int64(len(x)), _p(x),
+                         _p(f), _p(out))
+    return out
+
+
+def advance(p, xyz, vel, status):
+    x = np.ascontiguousarray(xyz, np.float64)
+    v = np.ascontiguousarray(vel, np.float64)
+    st = np.ascontiguousarray(status, np.int32)
+    nc, nn, _ = x.shape
+    removed = lib().zfo_advance(C.byref(p), nc, nn, _p(x), _p(v), _p(st))
+    return x, st, removed
+
+
+# ---- cell list and repulsion (zf_oracle_cells.c) ---------------------------
+class RepParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("bin_cutoff", "k_cell", "cutoff", "search", "k_wall",
+                                          "wall_d0", "wall_dmax", "wall_search")]
+
+
+def rep_params(**kw):
+    d = dict(bin_cutoff=1.0, k_cell=8.333e-7, cutoff=1.0, search=2.0, k_wall=1e-4, wall_d0=0.25,
+             wall_dmax=1.0, wall_search=1.5)
+    d.update(kw)
+    r = RepParams()
+    for k, v in d.items():
+        setattr(r, k, v)
+    return r
+
+
+def cell_list(p, b, domid, tri_centroid, status, bin_cutoff):
+    tc = np.ascontiguousarray(tri_centroid, np.float64)
+    nc, nt, _ = tc.shape
+    out = np.zeros((nc, nt), np.int32)
+    lib().zfo_cell_list(C.byref(p), _p(np.ascontiguousarray(b, np.int32)),
+                        _p(np.ascontiguousarray(domid, np.int32)), nc, nt,
+                        _p(np.ascontiguousarray(status, np.int32)), _p(tc), C.c_double(bin_cutoff),
+                        _p(out))
+    return out
+
+
+def cell_repulsion(p, r, b, domid, tris, status, xyz, node_normal, tri_centroid, tri_normal, bins):
+    x = np.ascontiguousarray(xyz, np.float64)
+    nc, nn, _ = x.shape
+    nt = len(tris)
+    v = np.zeros_like(x)
+    lib().zfo_cell_repulsion(C.byref(p), C.byref(r), _p(np.ascontiguousarray(b, np.int32)),
+                             _p(np.ascontiguousarray(domid, np.int32)), nc, nn, nt,
+                             _p(np.ascontiguousarray(tris, np.int32)),
+                             _p(np.ascontiguousarray(status, np.int32)), _p(x),
+                             _p(np.ascontiguousarray(node_normal, np.float64)),
+                             _p(np.ascontiguousarray(tri_centroid, np.float64)),
+                             _p(np.ascontiguousarray(tri_normal, np.float64)),
+                             _p(np.ascontiguousarray(bins, np.int32)), _p(v))
+    return v
+
+
+def wall_tables(p, wall_xyz, wall_tri):
+    wx = np.ascontiguousarray(wall_xyz, np.float64)
+    wt = np.ascontiguousarray(wall_tri, np.int32)
+    nwt = np.zeros((len(wx), 13), np.int32)
+    lib().zfo_wall_node_tri(len(wx), len(wt), _p(wt), _p(nwt))
+    windex = np.zeros((p.im, p.jm, p.km), np.int32)
+    lib().zfo_windex(C.byref(p), len(wx), _p(wx), _p(windex))
+    return nwt, windex
+
+
+def wall_repulsion(p, r, windex, wall_xyz, wall_tri, wall_normal, wall_node_tri, status, xyz):
+    x = np.ascontiguousarray(xyz, np.float64)
+    nc, nn, _ = x.shape
+    v = np.zeros_like(x)
+    near = np.zeros((nc, nn), np.int32)
+    wx = np.ascontiguousarray(wall_xyz, np.float64)
+    wt = np.ascontiguousarray(wall_tri, np.int32)
+    lib().zfo_wall_repulsion(C.byref(p), C.byref(r), _p(np.ascontiguousarray(windex, np.int32)),
+                             len(wx), _p(wx), len(wt), _p(wt),
+                             _p(np.ascontiguousarray(wall_normal, np.float64)),
+                             _p(np.ascontiguousarray(wall_node_tri, np.int32)), nc, nn,
+                             _p(np.ascontiguousarray(status, np.int32)), _p(x), _p(v), _p(near))
+    return v, near

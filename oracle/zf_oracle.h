@@ -142,6 +142,53 @@ void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
                          double *tri_centroid, double *cell_area, double *cell_vol,
                          double *cell_centroid, int *guard_fail);
 
+/* ---- immersed boundary (zf_oracle_ibm.c) ---------------------------------- */
+typedef struct zfo_ibm_params {
+  int im, jm, km;
+  double x0, y0, z0, dx;   /* node (i,j,k) at (x0+(i-1)dx, ...); ref X0,Y0,Z0, DX */
+  double vel_scale;        /* VEL_SCALE */
+  double force_scale;      /* 1/UNITMASS/LBM_BFORCE_SCALE */
+  double dt_advance;       /* DT*SOLVERSTAGGER */
+  double vmax;             /* removal speed, 5e-3 */
+} zfo_ibm_params;
+
+/* a13.  U,V,W per listed node (list order); xyz, vel: nnode*3. */
+void zfo_ibm_interpolate(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
+                         const double *U, const double *V, const double *W, int64_t nnode,
+                         const double *xyz, double *vel);
+/* a16.  ffluid: nnode*3; force: 3*dommax, accumulated (+=). */
+void zfo_ibm_spread(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
+                    int64_t dommax, int64_t nnode, const double *xyz, const double *ffluid,
+                    double *force);
+/* a17.  xyz, vel: ncell*nn*3; status per cell. */
+int zfo_advance(const zfo_ibm_params *p, int ncell, int nn, double *xyz, const double *vel,
+                int32_t *status);
+
+/* ---- cell list and repulsion (zf_oracle_cells.c) --------------------------- */
+typedef struct zfo_rep_params {
+  double bin_cutoff;       /* triangle kept in its nearest node's bin below this (1 um) */
+  double k_cell, cutoff, search;          /* 8.333e-7, 1 um, 2 um */
+  double k_wall, wall_d0, wall_dmax, wall_search;  /* 1e-4, 0.25, 1.0, 1.5 */
+} zfo_rep_params;
+
+/* a9 (cell list).  tri_centroid: ncell*nt*3; bin: ncell*nt, 1-based DOMID or 0. */
+void zfo_cell_list(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid, int ncell,
+                   int nt, const int32_t *status, const double *tri_centroid, double bin_cutoff,
+                   int32_t *bin);
+/* a15.  Adds to vrep (ncell*nn*3). */
+void zfo_cell_repulsion(const zfo_ibm_params *p, const zfo_rep_params *r, const int32_t *b,
+                        const int32_t *domid, int ncell, int nn, int nt, const int32_t *tri,
+                        const int32_t *status, const double *xyz, const double *node_normal,
+                        const double *tri_centroid, const double *tri_normal, const int32_t *bin,
+                        double *vrep);
+void zfo_wall_node_tri(int nwn, int nwt, const int32_t *wall_tri, int32_t *wall_node_tri);
+void zfo_windex(const zfo_ibm_params *p, int nwn, const double *wall_xyz, int32_t *windex);
+/* a14.  Adds to vrep; nearest: ncell*nn wall triangle (1-based) or 0. */
+void zfo_wall_repulsion(const zfo_ibm_params *p, const zfo_rep_params *r, const int32_t *windex,
+                        int nwn, const double *wall_xyz, int nwt, const int32_t *wall_tri,
+                        const double *wall_normal, const int32_t *wall_node_tri, int ncell, int nn,
+                        const int32_t *status, const double *xyz, double *vrep, int32_t *nearest);
+
 /* Wall-clock helper for the CPU baseline. */
 double zfo_wtime(void);
 int zfo_max_threads(void);

@@ -52,3 +52,21 @@ def place_cells(b, ref, r_cell, gap, ncell, seed, max_tries=200000):
     n = load_host_library().zfh_place_cells(b.ctypes.data, im, jm, km, ref.ctypes.data, len(ref),
                                             r_cell, gap, ncell, seed, max_tries, out.ctypes.data)
     return out[:n]
+
+
+def wall_mesh(b, x0=0.0, y0=0.0, z0=0.0, dx=1.0):
+    """Surface-nets wall mesh: (xyz [nv,3], tri [nt,3] 1-based, normal [nt,3])."""
+    import ctypes as C
+    lib = load_host_library()
+    im, jm, km = b.shape
+    b = np.ascontiguousarray(b, np.int32)
+    nv, nt = C.c_int64(), C.c_int64()
+    assert lib.zfh_wall_mesh(b.ctypes.data, im, jm, km, x0, y0, z0, dx, None, 0, None, None, 0,
+                             C.byref(nv), C.byref(nt)) == 0
+    xyz = np.zeros((nv.value, 3))
+    tri = np.zeros((nt.value, 3), np.int32)
+    nrm = np.zeros((nt.value, 3))
+    assert lib.zfh_wall_mesh(b.ctypes.data, im, jm, km, x0, y0, z0, dx, xyz.ctypes.data, nv.value,
+                             tri.ctypes.data, nrm.ctypes.data, nt.value, C.byref(nv),
+                             C.byref(nt)) == 0
+    return xyz, tri, nrm

@@ -26,6 +26,25 @@ class ZfLatticeInfo(C.Structure):
     ]
 
 
+class ZfMembraneConfig(C.Structure):
+    _fields_ = [(n, f64) for n in (
+        "len_scale", "kT", "x0", "es", "kbend", "kd", "ka", "kv", "area0", "vol0", "x_max",
+        "x_floor", "cos_cut", "area_lo", "area_hi", "force_limit", "vol_limit", "sliver_len",
+        "sliver_len_min", "sliver_f1", "sliver_f2", "relax", "relax2")] + [
+        ("passes", i32), ("max_cells", i32)]
+
+
+class ZfIbmConfig(C.Structure):
+    _fields_ = [(n, f64) for n in (
+        "x0", "y0", "z0", "dx", "vel_scale", "force_scale", "gamma_p", "dt_advance", "vmax")] + [
+        ("rbc_interval", i32), ("reserved", i32)]
+
+
+class ZfRepulsionConfig(C.Structure):
+    _fields_ = [(n, f64) for n in ("bin_cutoff", "k_cell", "cutoff", "search", "k_wall", "wall_d0",
+                                   "wall_dmax", "wall_search")]
+
+
 # name -> (restype, argtypes)
 FUNCS = {
     "zf_version": (C.c_char_p, []),
@@ -49,6 +68,31 @@ FUNCS = {
     "zf_download_macro": (C.c_int, [P, P, P, P, P]),
     "zf_conservation_sums": (C.c_int, [P, P]),
     "zf_lbm_time_kernels": (C.c_int, [P, i32, i32, P]),
+    "zf_membrane_config_reference": (None, [C.POINTER(ZfMembraneConfig)]),
+    "zf_membrane_setup": (C.c_int, [P, C.POINTER(ZfMembraneConfig), i32, i32, P, P]),
+    "zf_download_mesh_topology": (C.c_int, [P, C.POINTER(i32), P, P, P]),
+    "zf_download_mesh_equilibrium": (C.c_int, [P, P, P, P, P, P, P]),
+    "zf_upload_cells": (C.c_int, [P, i32, P]),
+    "zf_download_cells": (C.c_int, [P, P, P, P]),
+    "zf_membrane_forces": (C.c_int, [P]),
+    "zf_download_membrane": (C.c_int, [P, P, P, P, P, P, P]),
+    "zf_ibm_config_reference": (None, [C.POINTER(ZfIbmConfig)]),
+    "zf_ibm_setup": (C.c_int, [P, C.POINTER(ZfIbmConfig)]),
+    "zf_ibm_interpolate": (C.c_int, [P]),
+    "zf_ibm_spread": (C.c_int, [P]),
+    "zf_advance_cells": (C.c_int, [P, C.POINTER(i32)]),
+    "zf_upload_membrane_force": (C.c_int, [P, P]),
+    "zf_upload_repulsion": (C.c_int, [P, P]),
+    "zf_download_repulsion": (C.c_int, [P, P]),
+    "zf_step": (C.c_int, [P, i32]),
+    "zf_repulsion_config_reference": (None, [C.POINTER(ZfRepulsionConfig)]),
+    "zf_repulsion_setup": (C.c_int, [P, C.POINTER(ZfRepulsionConfig)]),
+    "zf_wall_setup": (C.c_int, [P, i32, P, i32, P, P]),
+    "zf_download_windex": (C.c_int, [P, P]),
+    "zf_cell_list": (C.c_int, [P]),
+    "zf_download_cell_list": (C.c_int, [P, P]),
+    "zf_repulsion": (C.c_int, [P]),
+    "zf_download_wall_neighbour": (C.c_int, [P, P]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),
     "zf_event_elapsed": (C.c_int, [P, i32, i32, C.POINTER(C.c_float)]),
@@ -60,6 +104,8 @@ HOST_FUNCS = {
     "zfh_geom_tree": (i64, [P, i32, i32, i32, f64, f64, i32, f64, f64, C.c_uint64]),
     "zfh_mesh_icosphere": (C.c_int, [i32, f64, i32, P, P, i32, i32, C.POINTER(i32)]),
     "zfh_place_cells": (C.c_int, [P, i32, i32, i32, P, i32, f64, f64, i32, C.c_uint64, i32, P]),
+    "zfh_wall_mesh": (C.c_int, [P, i32, i32, i32, f64, f64, f64, f64, P, i64, P, P, i64,
+                                C.POINTER(i64), C.POINTER(i64)]),
     "zfh_write_bindex_csv":(C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64]),
     "zfh_read_bindex_csv": (i64, [P, P, i32, i32, i32, f64, f64, f64, f64]),
     "zfh_write_fluid_csv": (C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64, P, P, P, P, P, i64]),

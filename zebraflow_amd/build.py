@@ -17,8 +17,9 @@ CSRC = os.path.join(PKG, "csrc")
 HOST = os.path.join(PKG, "host")
 OBJ = os.path.join(PKG, "_build")
 
-HIP_SOURCES = ["zf_core.hip", "zf_lists.hip", "zf_lbm.hip"]
-HOST_SOURCES = ["zf_io.c", "zf_geom.c", "zf_mesh.c", "zf_loop.c"]  # those present are built
+HIP_SOURCES = ["zf_core.hip", "zf_lists.hip", "zf_lbm.hip", "zf_membrane.hip", "zf_ibm.hip",
+               "zf_cells.hip"]
+HOST_SOURCES = ["zf_io.c", "zf_geom.c", "zf_mesh.c", "zf_wallmesh.c", "zf_loop.c"]  # those present are built
 
 # gfx950 only; packed-FP32 instructions are disabled for all device code.
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -1,0 +1,449 @@
+// zf_cells.hip -- triangle cell list, cell-cell and cell-wall repulsion.
+//
+// Replaces, for the lattice of this context, ZebraflowWT.c:
+//   EulNeigh cell list       :5851-5900   (built by atomic capture there;
+//                                           here a radix sort of
+//                                           (bin << 32 | triangle) keys, so
+//                                           each bin's triangles come in
+//                                           (cell, triangle) order)
+//   cell-wall repulsion      :8998-9142   (nearest wall triangle through the
+//                                           W_index map, plane distance)
+//   cell-cell repulsion      :9913-10052  (16-point sub-triangle probe)
+//   W_index / wall lists     :3715-3826, :3900-3935
+// One lane per membrane node (repulsion) or per triangle (binning); all
+// sums in FP64 in a fixed order.
+#include "zf_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <vector>
+
+namespace zf {
+namespace {
+
+inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+struct Grid {
+  double x0, y0, z0, dx;
+  int im, jm, km;
+};
+
+Grid grid_of(const zf_ctx *ctx) {
+  return Grid{ctx->icfg.x0, ctx->icfg.y0, ctx->icfg.z0, ctx->icfg.dx, ctx->cfg.im, ctx->cfg.jm,
+              ctx->cfg.km};
+}
+
+// The reference's 9^3 search window, (int)((x-X0)/DX + 1.01) +- 4, clipped.
+__device__ __forceinline__ void window(const Grid &g, double x, double y, double z, int lo[3],
+                                       int hi[3]) {
+  const int i = (int)((x - g.x0) / g.dx + 1.01), j = (int)((y - g.y0) / g.dx + 1.01),
+            k = (int)((z - g.z0) / g.dx + 1.01);
+  lo[0] = max(1, i - 4); hi[0] = min(g.im, i + 4);
+  lo[1] = max(1, j - 4); hi[1] = min(g.jm, j + 4);
+  lo[2] = max(1, k - 4); hi[2] = min(g.km, k + 4);
+}
+
+__device__ __forceinline__ size_t bq(const Grid &g, int i, int j, int k) {
+  return ((size_t)(i - 1) * g.jm + (j - 1)) * g.km + (k - 1);
+}
+
+// Cell list: nearest listed node to each triangle centroid, strict < in
+// (i,j,k) order, kept below the bin cut-off (ZebraflowWT.c:5857-5887).
+__global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, int nt, int ncell,
+                          const int32_t *__restrict__ status, const int32_t *__restrict__ b,
+                          const int32_t *__restrict__ domid, Grid g, double cutoff, int dommax,
+                          int32_t *__restrict__ tbin, uint64_t *__restrict__ keys) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncell * nt) return;
+  int bin = 0;
+  if (status[q / nt] >= 0) {
+    const double x = tg[4 * TT + q], y = tg[5 * TT + q], z = tg[6 * TT + q];
+    int lo[3], hi[3];
+    window(g, x, y, z, lo, hi);
+    double mind = 5000.0;
+    int bi = 0, bj = 0, bk = 0;
+    for (int i = lo[0]; i <= hi[0]; i++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int k = lo[2]; k <= hi[2]; k++) {
+          if (b[bq(g, i, j, k)] < 0) continue;
+          const double dx = g.x0 + (i - 1) * g.dx - x, dy = g.y0 + (j - 1) * g.dx - y,
+                       dz = g.z0 + (k - 1) * g.dx - z;
+          const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
+        }
+    if (mind < cutoff) bin = domid[bq(g, bi, bj, bk)];
+  }
+  tbin[q] = bin;
+  const uint64_t key_bin = bin > 0 ? (uint64_t)(bin - 1) : (uint64_t)dommax;
+  keys[q] = (key_bin << 32) | (uint64_t)q;
+}
+
+__global__ void k_bin_ranges(const uint64_t *__restrict__ keys, int n, uint64_t nbins,
+                             int32_t *__restrict__ range) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t t = keys[i] >> 32;
+  if (t >= nbins) return;
+  if (i == 0 || (keys[i - 1] >> 32) != t) range[2 * t] = i;
+  if (i == n - 1 || (keys[i + 1] >> 32) != t) range[2 * t + 1] = i + 1;
+}
+
+__global__ void k_fill(int32_t *a, size_t n, int32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// The 16 sub-triangle centroids of a triangle refined four times
+// (ZebraflowWT.c:9990-10025), one axis at a time.
+__device__ __forceinline__ void sub16(double a, double b, double c, double tc[16]) {
+  double tp[16];
+  tp[1] = a; tp[2] = b; tp[3] = c;
+  tp[4] = 0.5 * (tp[1] + tp[2]); tp[5] = 0.5 * (tp[2] + tp[3]); tp[6] = 0.5 * (tp[1] + tp[3]);
+  tp[7] = 0.5 * (tp[1] + tp[4]); tp[8] = 0.5 * (tp[2] + tp[4]); tp[9] = 0.5 * (tp[2] + tp[5]);
+  tp[10] = 0.5 * (tp[5] + tp[3]); tp[11] = 0.5 * (tp[3] + tp[6]); tp[12] = 0.5 * (tp[6] + tp[1]);
+  tp[13] = 0.5 * (tp[4] + tp[6]); tp[14] = 0.5 * (tp[4] + tp[5]); tp[15] = 0.5 * (tp[5] + tp[6]);
+  tc[0] = (tp[1] + tp[7] + tp[12]) / 3.0;   tc[1] = (tp[7] + tp[13] + tp[12]) / 3.0;
+  tc[2] = (tp[7] + tp[4] + tp[13]) / 3.0;   tc[3] = (tp[4] + tp[14] + tp[13]) / 3.0;
+  tc[4] = (tp[4] + tp[8] + tp[14]) / 3.0;   tc[5] = (tp[8] + tp[9] + tp[14]) / 3.0;
+  tc[6] = (tp[8] + tp[2] + tp[9]) / 3.0;    tc[7] = (tp[12] + tp[13] + tp[6]) / 3.0;
+  tc[8] = (tp[13] + tp[15] + tp[6]) / 3.0;  tc[9] = (tp[13] + tp[14] + tp[15]) / 3.0;
+  tc[10] = (tp[14] + tp[5] + tp[15]) / 3.0; tc[11] = (tp[14] + tp[9] + tp[5]) / 3.0;
+  tc[12] = (tp[6] + tp[15] + tp[11]) / 3.0; tc[13] = (tp[15] + tp[10] + tp[11]) / 3.0;
+  tc[14] = (tp[15] + tp[5] + tp[10]) / 3.0; tc[15] = (tp[11] + tp[10] + tp[3]) / 3.0;
+}
+
+struct RepParams {
+  double k_cell, cutoff, search, k_wall, wall_d0, wall_dmax, wall_search;
+};
+
+// Cell-cell repulsion, one lane per node of a released cell (status > 0).
+__global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
+                                 const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
+                                 const double *__restrict__ nnormal, const double *__restrict__ tg,
+                                 size_t TT, const int32_t *__restrict__ domid,
+                                 const uint64_t *__restrict__ keys,
+                                 const int32_t *__restrict__ range, Grid g, RepParams r,
+                                 double *__restrict__ vrep) {
+  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= ncell * nn) return;
+  const int c = gq / nn, node = gq % nn;
+  if (status[c] <= 0) return;
+  const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
+  const double e1x = nnormal[gq], e1y = nnormal[NT + gq], e1z = nnormal[2 * NT + gq];
+  int lo[3], hi[3];
+  window(g, xc, yc, zc, lo, hi);
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int i = lo[0]; i <= hi[0]; i++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const int o = domid[bq(g, i, j, k)];
+        if (o <= 0) continue;
+        const int beg = range[2 * (o - 1)], end = range[2 * (o - 1) + 1];
+        for (int e = beg; e < end; e++) {
+          const int q = (int)(keys[e] & 0xffffffffu);
+          const int m = q / nt, t = q % nt;
+          const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
+          if (m == c && (n1 == node || n2 == node || n3 == node)) continue;
+          if (e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0) continue;
+          const double dx = xc - tg[4 * TT + q], dy = yc - tg[5 * TT + q], dz = zc - tg[6 * TT + q];
+          if (sqrt(dx * dx + dy * dy + dz * dz) >= r.search) continue;
+          const size_t b0 = (size_t)m * nn;
+          double tx[16], ty[16], tz[16];
+          sub16(X[b0 + n1], X[b0 + n2], X[b0 + n3], tx);
+          sub16(X[NT + b0 + n1], X[NT + b0 + n2], X[NT + b0 + n3], ty);
+          sub16(X[2 * NT + b0 + n1], X[2 * NT + b0 + n2], X[2 * NT + b0 + n3], tz);
+          for (int s = 0; s < 16; s++) {
+            const double ax = tx[s] - xc, ay = ty[s] - yc, az = tz[s] - zc;
+            const double d = sqrt(ax * ax + ay * ay + az * az);
+            if (d < r.cutoff) {
+              const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
+              v0 += f * ax / d;
+              v1 += f * ay / d;
+              v2 += f * az / d;
+            }
+          }
+        }
+      }
+  vrep[gq] += v0;
+  vrep[NT + gq] += v1;
+  vrep[2 * NT + gq] += v2;
+}
+
+// Cell-wall repulsion, one lane per node of a released cell.
+__global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn, int ncell,
+                                 const int32_t *__restrict__ status, const int32_t *__restrict__ windex,
+                                 const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
+                                 const double *__restrict__ wxyz, const double *__restrict__ wnormal,
+                                 Grid g, RepParams r, double *__restrict__ vrep,
+                                 int32_t *__restrict__ nearest) {
+  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= ncell * nn) return;
+  nearest[gq] = 0;
+  if (status[gq / nn] <= 0) return;
+  const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
+  int lo[3], hi[3];
+  window(g, x, y, z, lo, hi);
+  double mind = r.wall_search;
+  int triwall = 0;
+  for (int i = lo[0]; i <= hi[0]; i++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const int wn = windex[bq(g, i, j, k)];
+        if (wn <= 0) continue;
+        const int32_t *row = &wnode_tri[13 * (size_t)(wn - 1)];
+        const int cnt = min(row[0], 12);
+        for (int l = 1; l <= cnt; l++) {
+          const int t = row[l];
+          double cx = 0, cy = 0, cz = 0;
+          for (int s = 0; s < 3; s++) {
+            const double *w = &wxyz[3 * (size_t)(wtri[3 * (size_t)(t - 1) + s] - 1)];
+            const double d = sqrt((x - w[0]) * (x - w[0]) + (y - w[1]) * (y - w[1]) + (z - w[2]) * (z - w[2]));
+            if (d < mind) { mind = d; triwall = t; }
+            cx += w[0] / 3.0;
+            cy += w[1] / 3.0;
+            cz += w[2] / 3.0;
+          }
+          const double d = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
+          if (d < mind) { mind = d; triwall = t; }
+        }
+      }
+  if (triwall == 0) return;
+  nearest[gq] = triwall;
+  const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
+  const double *n = &wnormal[3 * (size_t)(triwall - 1)];
+  const double d = (x - w1[0]) * n[0] + (y - w1[1]) * n[1] + (z - w1[2]) * n[2];
+  if (d < r.wall_d0 && fabs(d) < r.wall_dmax) {
+    double f = (r.wall_d0 - d) / r.wall_dmax;
+    if (d < r.wall_d0 - r.wall_dmax) f = 1.0;
+    vrep[gq] += f * r.k_wall * n[0];
+    vrep[NT + gq] += f * r.k_wall * n[1];
+    vrep[2 * NT + gq] += f * r.k_wall * n[2];
+  }
+}
+
+// W_index: every lattice node in a wall node's window keeps the nearest wall
+// node, compared as (FP32 distance, node) through one 64-bit atomicMin, which
+// is the reference's serial strict-< rule over FP32 distances (ties to the
+// lower wall node).
+__global__ void k_windex(const double *__restrict__ wxyz, int nwn, Grid g,
+                         unsigned long long *__restrict__ key) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nwn) return;
+  const double wx = wxyz[3 * (size_t)l], wy = wxyz[3 * (size_t)l + 1], wz = wxyz[3 * (size_t)l + 2];
+  int lo[3], hi[3];
+  window(g, wx, wy, wz, lo, hi);
+  for (int i = lo[0]; i <= hi[0]; i++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const double dx = wx - (g.x0 + (i - 1) * g.dx), dy = wy - (g.y0 + (j - 1) * g.dx),
+                     dz = wz - (g.z0 + (k - 1) * g.dx);
+        const float d = (float)sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(d < 5000.0f)) continue;
+        const unsigned long long kv =
+            ((unsigned long long)__float_as_uint(d) << 32) | (unsigned long long)(l + 1);
+        atomicMin(&key[bq(g, i, j, k)], kv);
+      }
+}
+
+__global__ void k_windex_final(const unsigned long long *__restrict__ key, size_t box,
+                               int32_t *__restrict__ windex) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= box) return;
+  const unsigned long long kv = key[q];
+  windex[q] = kv == ~0ull ? 0 : (int32_t)(kv & 0xffffffffu);
+}
+
+__global__ void k_fill_u64(unsigned long long *a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = ~0ull;
+}
+
+int require_rep(zf_ctx *ctx) {
+  if (!ctx->have_rep) return fail(ctx, ZF_ERR_STATE, "zf_repulsion_setup not called");
+  if (!ctx->have_ibm) return fail(ctx, ZF_ERR_STATE, "zf_ibm_setup not called");
+  return ZF_OK;
+}
+
+void free_wall(zf_ctx *ctx) {
+  void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
+                   (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex};
+  for (auto b : bufs) dev_free(ctx, b);
+  ctx->have_wall = false;
+}
+
+}  // namespace
+
+void free_cells(zf_ctx *ctx) {
+  void **bufs[] = {(void **)&ctx->d_tkeys[0], (void **)&ctx->d_tkeys[1], (void **)&ctx->d_tsort_tmp,
+                   (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near};
+  for (auto b : bufs) dev_free(ctx, b);
+  free_wall(ctx);
+  ctx->have_rep = false;
+}
+
+int cell_list_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, TT = ncap * ctx->nt;
+  const int n = ctx->ncell * ctx->nt;
+  hipLaunchKernelGGL(k_fill, dim3(nb(2 * (size_t)ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_bin_range, 2 * (size_t)ctx->dommax, 0);
+  if (n == 0) return ZF_OK;
+  hipLaunchKernelGGL(k_tri_bin, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_tgeo, TT,
+                     ctx->nt, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_domid, grid_of(ctx),
+                     ctx->rcfg.bin_cutoff, ctx->dommax, ctx->d_tbin, ctx->d_tkeys[0]);
+  int bits = 1;
+  while ((1ull << bits) <= (uint64_t)ctx->dommax) bits++;
+  size_t tmp = ctx->tsort_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_tsort_tmp, tmp, ctx->d_tkeys[0],
+                                                  ctx->d_tkeys[1], n, 0, 32 + bits, ctx->stream));
+  hipLaunchKernelGGL(k_bin_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_tkeys[1], n,
+                     (uint64_t)ctx->dommax, ctx->d_bin_range);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int repulsion_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  const RepParams r{ctx->rcfg.k_cell, ctx->rcfg.cutoff, ctx->rcfg.search, ctx->rcfg.k_wall,
+                    ctx->rcfg.wall_d0, ctx->rcfg.wall_dmax, ctx->rcfg.wall_search};
+  hipLaunchKernelGGL(k_fill, dim3(nb(3 * NT * 2, 256)), dim3(256), 0, ctx->stream,
+                     (int32_t *)ctx->d_vrep, 3 * NT * 2, 0);
+  hipLaunchKernelGGL(k_fill, dim3(nb(NT, 256)), dim3(256), 0, ctx->stream, ctx->d_near, NT, 0);
+  if (ctx->have_wall)
+    hipLaunchKernelGGL(k_wall_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
+                       ctx->nn, ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wnode_tri,
+                       ctx->d_wtri, ctx->d_wxyz, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep,
+                       ctx->d_near);
+  hipLaunchKernelGGL(k_cell_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
+                     ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
+                     ctx->d_tgeo, TT, ctx->d_domid, ctx->d_tkeys[1], ctx->d_bin_range, grid_of(ctx),
+                     r, ctx->d_vrep);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+void zf_repulsion_config_reference(zf_repulsion_config *rc) {
+  rc->bin_cutoff = 1.0;
+  rc->k_cell = 8.333e-7;
+  rc->cutoff = 1.0;
+  rc->search = 2.0;
+  rc->k_wall = 1e-4;
+  rc->wall_d0 = 0.25;
+  rc->wall_dmax = 1.0;
+  rc->wall_search = 1.5;
+}
+
+int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
+  if (!ctx || !rc) return ZF_ERR_ARG;
+  if (!ctx->have_ibm) return fail(ctx, ZF_ERR_STATE, "call zf_ibm_setup first");
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  free_cells(ctx);
+  ctx->rcfg = *rc;
+  const size_t ncap = ctx->mcfg.max_cells, TT = ncap * ctx->nt, NT = ncap * ctx->nn;
+  int r;
+  if ((r = dev_alloc(ctx, (void **)&ctx->d_tkeys[0], 8 * TT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_tkeys[1], 8 * TT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_tbin, 4 * TT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_bin_range, 8 * (size_t)std::max(ctx->dommax, 1))) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_near, 4 * NT)))
+    return r;
+  size_t tmp = 0;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ctx->d_tkeys[0], ctx->d_tkeys[1],
+                                                  (int)TT, 0, 64, ctx->stream));
+  ctx->tsort_tmp_bytes = tmp;
+  if ((r = dev_alloc(ctx, &ctx->d_tsort_tmp, tmp))) return r;
+  ctx->have_rep = true;
+  return ZF_OK;
+}
+
+int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
+                  const int32_t *wall_tri, const double *wall_normal) {
+  if (!ctx || nwn <= 0 || nwt <= 0 || !wall_xyz || !wall_tri || !wall_normal) return ZF_ERR_ARG;
+  int rc = require_rep(ctx);
+  if (rc) return rc;
+  for (size_t i = 0; i < 3 * (size_t)nwt; i++)
+    if (wall_tri[i] < 1 || wall_tri[i] > nwn) return fail(ctx, ZF_ERR_ARG, "wall triangle node out of range");
+  free_wall(ctx);
+  // node -> triangle rows (count, up to 12 triangles), ZebraflowWT.c:3900-3935
+  std::vector<int32_t> rows(13 * (size_t)nwn, 0);
+  for (int m = 1; m <= nwt; m++)
+    for (int s = 0; s < 3; s++) {
+      int32_t *row = &rows[13 * (size_t)(wall_tri[3 * (size_t)(m - 1) + s] - 1)];
+      row[0]++;
+      if (row[0] <= 12) row[row[0]] = m;
+    }
+  unsigned long long *key = nullptr;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_wxyz, 24 * (size_t)nwn)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wnormal, 24 * (size_t)nwt)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wtri, 12 * (size_t)nwt)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wnode_tri, 52 * (size_t)nwn)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_windex, 4 * ctx->box)) ||
+      (rc = dev_alloc(ctx, (void **)&key, 8 * ctx->box)))
+    return rc;
+  ZF_HIP(ctx, hipMemcpy(ctx->d_wxyz, wall_xyz, 24 * (size_t)nwn, hipMemcpyHostToDevice));
+  ZF_HIP(ctx, hipMemcpy(ctx->d_wnormal, wall_normal, 24 * (size_t)nwt, hipMemcpyHostToDevice));
+  ZF_HIP(ctx, hipMemcpy(ctx->d_wtri, wall_tri, 12 * (size_t)nwt, hipMemcpyHostToDevice));
+  ZF_HIP(ctx, hipMemcpy(ctx->d_wnode_tri, rows.data(), 52 * (size_t)nwn, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_fill_u64, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key, ctx->box);
+  hipLaunchKernelGGL(k_windex, dim3(nb(nwn, 64)), dim3(64), 0, ctx->stream, ctx->d_wxyz, nwn,
+                     grid_of(ctx), key);
+  hipLaunchKernelGGL(k_windex_final, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key,
+                     ctx->box, ctx->d_windex);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= 8 * (int64_t)ctx->box;
+  dev_free(ctx, (void **)&key);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "wall setup: %s", hipGetErrorString(e));
+  ctx->nwn = nwn;
+  ctx->nwt = nwt;
+  ctx->have_wall = true;
+  return ZF_OK;
+}
+
+int zf_download_windex(zf_ctx *ctx, int32_t *windex) {
+  if (!ctx || !windex) return ZF_ERR_ARG;
+  if (!ctx->have_wall) return fail(ctx, ZF_ERR_STATE, "no wall set up");
+  ZF_HIP(ctx, hipMemcpy(windex, ctx->d_windex, 4 * ctx->box, hipMemcpyDeviceToHost));
+  return ZF_OK;
+}
+
+int zf_cell_list(zf_ctx *ctx) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_rep(ctx);
+  return rc ? rc : cell_list_impl(ctx);
+}
+
+int zf_download_cell_list(zf_ctx *ctx, int32_t *bin) {
+  if (!ctx || !bin) return ZF_ERR_ARG;
+  int rc = require_rep(ctx);
+  if (rc) return rc;
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->ncell)
+    ZF_HIP(ctx, hipMemcpy(bin, ctx->d_tbin, 4 * (size_t)ctx->ncell * ctx->nt, hipMemcpyDeviceToHost));
+  return ZF_OK;
+}
+
+int zf_repulsion(zf_ctx *ctx) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_rep(ctx);
+  return rc ? rc : repulsion_impl(ctx);
+}
+
+int zf_download_wall_neighbour(zf_ctx *ctx, int32_t *nearest) {
+  if (!ctx || !nearest) return ZF_ERR_ARG;
+  int rc = require_rep(ctx);
+  if (rc) return rc;
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->ncell)
+    ZF_HIP(ctx, hipMemcpy(nearest, ctx->d_near, 4 * (size_t)ctx->ncell * ctx->nn, hipMemcpyDeviceToHost));
+  return ZF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,539 @@
+// zf_ibm.hip -- immersed-boundary coupling and the membrane position update.
+//
+// Replaces, for the lattice of this context, ZebraflowWT.c:
+//   velocity interpolation   :8702-8996
+//   force spreading          :10746-10867
+//   position update          :11975-11989
+// and drives one whole iteration in the reference's order (zf_step).
+//
+// Spreading without atomics: membrane nodes are sorted by the 4x4x4 lattice
+// tile holding them (64-bit keys tile<<32 | node, radix sort, so the order is
+// deterministic); one 64-lane wave per tile then gathers, for its 64 lattice
+// nodes, the contributions of every membrane node in the 27 surrounding
+// tiles, staged through LDS in chunks of 64.  The kernel support (radius 2
+// lattice units) never reaches past the neighbouring tiles.
+#include "zf_internal.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <vector>
+
+namespace zf {
+
+int step(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev);
+int membrane_forces_impl(zf_ctx *ctx);
+
+namespace {
+
+constexpr int kTileW = 4;
+constexpr double kPi = 3.14159265358979;  // PI, ZebraflowWT.c:33
+
+inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+__global__ void k_sbox(const int32_t *__restrict__ domid, size_t box, const int32_t *__restrict__ s_of_p,
+                       int32_t *__restrict__ sbox) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= box) return;
+  const int d = domid[q];
+  sbox[q] = d > 0 ? s_of_p[d - 1] : -1;
+}
+
+// U,V,W of every listed node without the force shift (ZebraflowWT.c:6606-6614).
+template <class T, bool DEV16, bool FULL>
+__global__ void k_macro_vel(const T *__restrict__ f, size_t stride, int n, float *__restrict__ u) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  constexpr int EX[19] = {1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0, 0};
+  constexpr int EY[19] = {0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0};
+  constexpr int EZ[19] = {0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0};
+  double drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int c = 0; c < 19; c++) {
+    const double w = c < 6 ? 1.0 / 18 : c < 18 ? 1.0 / 36 : 1.0 / 3;
+    double g = (double)f[(size_t)c * stride + s];
+    if (FULL) g -= w;          // FP64 storage holds f
+    if (DEV16) g *= w;         // FP16 storage holds (f - w)/w
+    drho += g;
+    jx += EX[c] * g;
+    jy += EY[c] * g;
+    jz += EZ[c] * g;
+  }
+  const double rho = 1.0 + drho;
+  u[s] = (float)(jx / rho);
+  u[stride + s] = (float)(jy / rho);
+  u[2 * stride + s] = (float)(jz / rho);
+}
+
+struct IbmParams {
+  double x0, y0, z0, dx, vel_scale, force_scale, gamma_p, dt_advance, vmax;
+  int im, jm, km, ntx, nty, ntz;
+};
+
+__device__ __forceinline__ double kern1(double d, double width) {
+  return 1.0 + cos(kPi * fabs(d) / width);
+}
+
+// Interpolation, one lane per membrane node: the nodes within the spherical
+// cut-off |d| <= 2 DX all lie in the window [i-2, i+3] of the reference's
+// (int)((x-X0)/DX + 1.01) index, which is what is scanned.
+__global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int ncell,
+                         const int32_t *__restrict__ status, const int32_t *__restrict__ b,
+                         const int32_t *__restrict__ sbox, const float *__restrict__ umac,
+                         size_t stride, IbmParams p, double *__restrict__ V) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncell * nn) return;
+  if (status[g / nn] < 0) return;
+  const double xp = X[g], yp = X[NT + g], zp = X[2 * NT + g];
+  const int i = (int)((xp - p.x0) / p.dx + 1.01), j = (int)((yp - p.y0) / p.dx + 1.01),
+            k = (int)((zp - p.z0) / p.dx + 1.01);
+  const int i1 = max(1, i - 3), i2 = min(p.im, i + 3), j1 = max(1, j - 3), j2 = min(p.jm, j + 3),
+            k1 = max(1, k - 3), k2 = min(p.km, k + 3);
+  const double width = 2.0 * p.dx, w2 = width * width;
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int ii = i1; ii <= i2; ii++) {
+    const double dxv = p.x0 + (ii - 1) * p.dx - xp;
+    for (int jj = j1; jj <= j2; jj++) {
+      const double dyv = p.y0 + (jj - 1) * p.dx - yp;
+      for (int kk = k1; kk <= k2; kk++) {
+        const double dzv = p.z0 + (kk - 1) * p.dx - zp;
+        if (dxv * dxv + dyv * dyv + dzv * dzv > w2) continue;
+        const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
+        const int bv = b[q];
+        if (!(bv > 0 && bv < 4998)) continue;
+        const double w = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
+        const int s = sbox[q];
+        v0 += w * p.vel_scale * (double)umac[s];
+        v1 += w * p.vel_scale * (double)umac[stride + s];
+        v2 += w * p.vel_scale * (double)umac[2 * stride + s];
+      }
+    }
+  }
+  V[g] = v0;
+  V[NT + g] = v1;
+  V[2 * NT + g] = v2;
+}
+
+__device__ __forceinline__ int tile_of(double pos, double origin, double dx, int nt) {
+  int t = (int)floor((pos - origin) / dx / kTileW);
+  return t < 0 ? 0 : t >= nt ? nt - 1 : t;
+}
+
+// v += v_rep; F_fluid = gamma_p v_rep + F_Total (ZebraflowWT.c:10749-10754);
+// sort key = tile << 32 | node (dead cells sort past the last tile).
+__global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V,
+                            const double *__restrict__ vrep, const double *__restrict__ F,
+                            size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
+                            IbmParams p, float *__restrict__ ffl, uint64_t *__restrict__ keys) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncell * nn) return;
+  const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
+  if (status[g / nn] < 0) {
+    keys[g] = (ntiles << 32) | (uint64_t)g;
+    return;
+  }
+  double fl[3];
+  for (int a = 0; a < 3; a++) {
+    const double vr = vrep[a * NT + g];
+    V[a * NT + g] += vr;
+    fl[a] = p.gamma_p * vr + F[a * NT + g];
+  }
+  ffl[g] = (float)fl[0];
+  ffl[NT + g] = (float)fl[1];
+  ffl[2 * NT + g] = (float)fl[2];
+  const int tx = tile_of(X[g], p.x0, p.dx, p.ntx), ty = tile_of(X[NT + g], p.y0, p.dx, p.nty),
+            tz = tile_of(X[2 * NT + g], p.z0, p.dx, p.ntz);
+  const uint64_t tile = ((uint64_t)tx * p.nty + ty) * p.ntz + tz;
+  keys[g] = (tile << 32) | (uint64_t)g;
+}
+
+__global__ void k_fill_i32(int32_t *a, size_t n, int32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_tile_ranges(const uint64_t *__restrict__ keys, int n, uint64_t ntiles,
+                              int32_t *__restrict__ range) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t t = keys[i] >> 32;
+  if (t >= ntiles) return;
+  if (i == 0 || (keys[i - 1] >> 32) != t) range[2 * t] = i;
+  if (i == n - 1 || (keys[i + 1] >> 32) != t) range[2 * t + 1] = i + 1;
+}
+
+// One wave per tile; lane = lattice node (4x4x4).  Gathers every membrane
+// node of the 27 neighbouring tiles in (tile, node) order.
+__global__ __launch_bounds__(64) void k_spread_gather(
+    const uint64_t *__restrict__ keys, const int32_t *__restrict__ range,
+    const double *__restrict__ X, const float *__restrict__ ffl, size_t NT,
+    const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, IbmParams p,
+    float *__restrict__ force, size_t stride) {
+  __shared__ float sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
+  const int tile = blockIdx.x;
+  const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
+  const int lane = threadIdx.x;
+  // lattice node of this lane (0-based), and its coordinate relative to the
+  // tile origin in lattice units
+  const int li = tx * kTileW + lane / 16, lj = ty * kTileW + (lane / 4) % 4, lk = tz * kTileW + lane % 4;
+  const double ox = p.x0 + (double)(tx * kTileW) * p.dx, oy = p.y0 + (double)(ty * kTileW) * p.dx,
+               oz = p.z0 + (double)(tz * kTileW) * p.dx;
+  const float rx = (float)(lane / 16), ry = (float)((lane / 4) % 4), rz = (float)(lane % 4);
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  bool any = false;
+  for (int dxT = -1; dxT <= 1; dxT++)
+    for (int dyT = -1; dyT <= 1; dyT++)
+      for (int dzT = -1; dzT <= 1; dzT++) {
+        const int nx = tx + dxT, ny = ty + dyT, nz = tz + dzT;
+        if (nx < 0 || ny < 0 || nz < 0 || nx >= p.ntx || ny >= p.nty || nz >= p.ntz) continue;
+        const int nt = (nx * p.nty + ny) * p.ntz + nz;
+        const int beg = range[2 * nt], end = range[2 * nt + 1];
+        for (int c0 = beg; c0 < end; c0 += 64) {
+          const int cnt = min(64, end - c0);
+          if (lane < cnt) {
+            const int g = (int)(keys[c0 + lane] & 0xffffffffu);
+            sx[lane] = (float)((X[g] - ox) / p.dx);
+            sy[lane] = (float)((X[NT + g] - oy) / p.dx);
+            sz[lane] = (float)((X[2 * NT + g] - oz) / p.dx);
+            sf0[lane] = ffl[g];
+            sf1[lane] = ffl[NT + g];
+            sf2[lane] = ffl[2 * NT + g];
+          }
+          __syncthreads();
+          any = true;
+          for (int m = 0; m < cnt; m++) {
+            const float dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
+            if (dx * dx + dy * dy + dz * dz > 4.0f) continue;
+            // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
+            const float w = (1.0f + cospif(0.5f * fabsf(dx))) * (1.0f + cospif(0.5f * fabsf(dy))) *
+                            (1.0f + cospif(0.5f * fabsf(dz))) * (1.0f / 64.0f);
+            a0 += w * sf0[m];
+            a1 += w * sf1[m];
+            a2 += w * sf2[m];
+          }
+          __syncthreads();
+        }
+      }
+  if (!any || li >= p.im || lj >= p.jm || lk >= p.km) return;
+  const size_t q = ((size_t)li * p.jm + lj) * p.km + lk;
+  if (b[q] < 0) return;
+  const int s = sbox[q];
+  const float fs = (float)p.force_scale;
+  force[s] = a0 * fs;
+  force[stride + s] = a1 * fs;
+  force[2 * stride + s] = a2 * fs;
+}
+
+__global__ void k_speed(const double *__restrict__ V, size_t NT, int nn, int ncell,
+                        const int32_t *__restrict__ status, double vmax, int32_t *__restrict__ flag) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncell * nn) return;
+  if (status[g / nn] < 0) return;
+  const double v0 = V[g], v1 = V[NT + g], v2 = V[2 * NT + g];
+  const double s = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+  if (s > vmax || isnan(s) || isinf(s)) atomicOr(&flag[g / nn], 1);
+}
+
+__global__ void k_move(double *__restrict__ X, const double *__restrict__ V, size_t NT, int nn,
+                       int ncell, int32_t *__restrict__ status, const int32_t *__restrict__ flag,
+                       double dt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncell * nn) return;
+  const int c = g / nn;
+  if (status[c] < 0 || flag[c]) return;
+  for (int a = 0; a < 3; a++) X[a * NT + g] += V[a * NT + g] * dt;
+}
+
+__global__ void k_remove(int32_t *__restrict__ status, const int32_t *__restrict__ flag, int ncell,
+                         int32_t *__restrict__ count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  if (status[c] >= 0 && flag[c]) {
+    status[c] = -5000;
+    atomicAdd(count, 1);
+  }
+}
+
+__global__ void k_import_soa(const double *__restrict__ h, int n, size_t NT, double *__restrict__ A) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  for (int a = 0; a < 3; a++) A[a * NT + g] = h[3 * (size_t)g + a];
+}
+
+__global__ void k_export_soa(const double *__restrict__ A, int n, size_t NT, double *__restrict__ h) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  for (int a = 0; a < 3; a++) h[3 * (size_t)g + a] = A[a * NT + g];
+}
+
+__global__ void k_zero_f64(double *a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+IbmParams params(const zf_ctx *ctx) {
+  const zf_ibm_config &c = ctx->icfg;
+  return IbmParams{c.x0, c.y0, c.z0, c.dx, c.vel_scale, c.force_scale, c.gamma_p, c.dt_advance,
+                   c.vmax, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km, ctx->ntx, ctx->nty, ctx->ntz};
+}
+
+int require_ibm(zf_ctx *ctx) {
+  if (!ctx->have_ibm) return fail(ctx, ZF_ERR_STATE, "zf_ibm_setup not called");
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  if (!ctx->have_lattice) return fail(ctx, ZF_ERR_STATE, "no lattice uploaded");
+  return ZF_OK;
+}
+
+int ensure_force_field(zf_ctx *ctx) {
+  if (ctx->d_force) return ZF_OK;
+  int rc = dev_alloc(ctx, (void **)&ctx->d_force, sizeof(float) * 3 * ctx->stride);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
+                     (int32_t *)ctx->d_force, 3 * ctx->stride, 0);
+  return ZF_OK;
+}
+
+}  // namespace
+
+void free_ibm(zf_ctx *ctx) {
+  void **bufs[] = {(void **)&ctx->d_sbox,  (void **)&ctx->d_umac, (void **)&ctx->d_vrep,
+                   (void **)&ctx->d_ffl,   (void **)&ctx->d_keys[0], (void **)&ctx->d_keys[1],
+                   (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag};
+  for (auto b : bufs) dev_free(ctx, b);
+  ctx->have_ibm = false;
+}
+
+int ibm_interpolate_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  // macro velocities of the current populations
+  const void *f = ctx->d_f[ctx->cur];
+  const unsigned g = nb(ctx->dommax, 256);
+  switch (ctx->cfg.storage) {
+    case ZF_STORE_F64:
+      hipLaunchKernelGGL((k_macro_vel<double, false, true>), dim3(g), dim3(256), 0, ctx->stream,
+                         (const double *)f, ctx->stride, ctx->dommax, ctx->d_umac);
+      break;
+    case ZF_STORE_F32DEV:
+      hipLaunchKernelGGL((k_macro_vel<float, false, false>), dim3(g), dim3(256), 0, ctx->stream,
+                         (const float *)f, ctx->stride, ctx->dommax, ctx->d_umac);
+      break;
+    default:
+      hipLaunchKernelGGL((k_macro_vel<_Float16, true, false>), dim3(g), dim3(256), 0, ctx->stream,
+                         (const _Float16 *)f, ctx->stride, ctx->dommax, ctx->d_umac);
+  }
+  hipLaunchKernelGGL(k_interp, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
+                     ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, ctx->d_umac, ctx->stride,
+                     params(ctx), ctx->d_v);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int ibm_spread_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  int rc = ensure_force_field(ctx);
+  if (rc) return rc;
+  const IbmParams p = params(ctx);
+  const uint64_t ntiles = (uint64_t)ctx->ntx * ctx->nty * ctx->ntz;
+  // the force field is rebuilt from zero (cleared by a fill kernel)
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
+                     (int32_t *)ctx->d_force, 3 * ctx->stride, 0);
+  if (n == 0) return ZF_OK;
+  hipLaunchKernelGGL(k_node_prep, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, ctx->d_v,
+                     ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
+                     ctx->d_ffl, ctx->d_keys[0]);
+  // end bit: tile ids need ceil(log2(ntiles+1)) bits above the 32 node bits
+  int tbits = 1;
+  while ((1ull << tbits) <= ntiles) tbits++;
+  size_t tmp = ctx->sort_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
+                                                  n, 0, 32 + tbits, ctx->stream));
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * ntiles, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_tile_range, (size_t)(2 * ntiles), 0);
+  hipLaunchKernelGGL(k_tile_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1], n,
+                     ntiles, ctx->d_tile_range);
+  hipLaunchKernelGGL(k_spread_gather, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
+                     ctx->d_keys[1], ctx->d_tile_range, ctx->d_x, ctx->d_ffl, NT, ctx->d_b,
+                     ctx->d_sbox, p, ctx->d_force, ctx->stride);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int advance_impl(zf_ctx *ctx, int32_t *n_removed) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
+  const int n = ctx->ncell * ctx->nn, nc = ctx->ncell;
+  if (n == 0) {
+    if (n_removed) *n_removed = 0;
+    return ZF_OK;
+  }
+  // flag[nc] is the removal counter
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(nc + 1, 256)), dim3(256), 0, ctx->stream, ctx->d_flag,
+                     (size_t)nc + 1, 0);
+  hipLaunchKernelGGL(k_speed, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_v, NT, ctx->nn,
+                     nc, ctx->d_status, ctx->icfg.vmax, ctx->d_flag);
+  hipLaunchKernelGGL(k_move, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, ctx->d_v, NT,
+                     ctx->nn, nc, ctx->d_status, ctx->d_flag, ctx->icfg.dt_advance);
+  hipLaunchKernelGGL(k_remove, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, ctx->d_status,
+                     ctx->d_flag, nc, ctx->d_flag + nc);
+  ZF_HIP(ctx, hipGetLastError());
+  if (n_removed) {
+    ZF_HIP(ctx, hipMemcpyAsync(n_removed, ctx->d_flag + nc, 4, hipMemcpyDeviceToHost, ctx->stream));
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return ZF_OK;
+}
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+// Reference scales: X0=-1, Y0=29, Z0=-27 (ZebraflowWT.c:95-97), DX=0.5,
+// VEL_SCALE = L_SCALE/T_SCALE = 1, UNITMASS = DX^3 * 1e4 kg/m^3,
+// LBM_BFORCE_SCALE = L_SCALE/T_SCALE^2/3, gamma_p = 1e-7 (:124),
+// DT*SOLVERSTAGGER = 25 (:41, :151), removal speed 5e-3 (:11978).
+void zf_ibm_config_reference(zf_ibm_config *ic) {
+  *ic = zf_ibm_config{};
+  const double l = 0.5e-6, t = 0.5e-6;
+  ic->x0 = -1.0;
+  ic->y0 = 29.0;
+  ic->z0 = -27.0;
+  ic->dx = 0.5;
+  ic->vel_scale = l / t;
+  const double unitmass = l * l * l * 10.0 * 1000.0;
+  const double bforce = 1.0 / 3.0 * l / t / t;
+  ic->force_scale = 1.0 / unitmass / bforce;
+  ic->gamma_p = 1e-7;
+  ic->dt_advance = 0.5 * 50;
+  ic->vmax = 5e-3;
+  ic->rbc_interval = 50;
+}
+
+int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
+  if (!ctx || !ic || ic->dx <= 0 || ic->rbc_interval < 1) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "call zf_membrane_setup first");
+  if (!ctx->have_lattice) return fail(ctx, ZF_ERR_STATE, "call zf_upload_lattice first");
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  free_cells(ctx);
+  free_ibm(ctx);
+  ctx->icfg = *ic;
+  ctx->ntx = (ctx->cfg.im + kTileW - 1) / kTileW;
+  ctx->nty = (ctx->cfg.jm + kTileW - 1) / kTileW;
+  ctx->ntz = (ctx->cfg.km + kTileW - 1) / kTileW;
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
+  const size_t ntiles = (size_t)ctx->ntx * ctx->nty * ctx->ntz;
+  if (NT >= (1ull << 32) || ntiles >= (1ull << 31)) return fail(ctx, ZF_ERR_ARG, "too many nodes or tiles");
+  int rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_sbox, 4 * ctx->box)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_umac, 12 * ctx->stride)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_vrep, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_ffl, 12 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_keys[0], 8 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_keys[1], 8 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_tile_range, 8 * ntiles)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_flag, 4 * (ncap + 1))))
+    return rc;
+  size_t tmp = 0;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ctx->d_keys[0], ctx->d_keys[1],
+                                                  (int)NT, 0, 64, ctx->stream));
+  ctx->sort_tmp_bytes = tmp;
+  if ((rc = dev_alloc(ctx, &ctx->d_sort_tmp, tmp))) return rc;
+  hipLaunchKernelGGL(k_sbox, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, ctx->d_domid,
+                     ctx->box, ctx->d_s_of_p, ctx->d_sbox);
+  hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
+  ZF_HIP(ctx, hipGetLastError());
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->have_ibm = true;
+  return ZF_OK;
+}
+
+int zf_ibm_interpolate(zf_ctx *ctx) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_ibm(ctx);
+  return rc ? rc : ibm_interpolate_impl(ctx);
+}
+
+int zf_ibm_spread(zf_ctx *ctx) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_ibm(ctx);
+  return rc ? rc : ibm_spread_impl(ctx);
+}
+
+int zf_advance_cells(zf_ctx *ctx, int32_t *n_removed) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_ibm(ctx);
+  return rc ? rc : advance_impl(ctx, n_removed);
+}
+
+static int upload_node_field(zf_ctx *ctx, const double *h, double *dst) {
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  double *tmp = nullptr;
+  int rc = dev_alloc(ctx, (void **)&tmp, 24 * (size_t)n);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, h, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_import_soa, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, tmp, n, NT, dst);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= 24 * (int64_t)n;
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "node field upload: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_upload_membrane_force(zf_ctx *ctx, const double *force) {
+  if (!ctx || !force) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  return upload_node_field(ctx, force, ctx->d_mforce);
+}
+
+int zf_upload_repulsion(zf_ctx *ctx, const double *vrep) {
+  if (!ctx || !vrep) return ZF_ERR_ARG;
+  if (!ctx->have_ibm) return fail(ctx, ZF_ERR_STATE, "zf_ibm_setup not called");
+  return upload_node_field(ctx, vrep, ctx->d_vrep);
+}
+
+int zf_download_repulsion(zf_ctx *ctx, double *vrep) {
+  if (!ctx || !vrep) return ZF_ERR_ARG;
+  if (!ctx->have_ibm) return fail(ctx, ZF_ERR_STATE, "zf_ibm_setup not called");
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  double *tmp = nullptr;
+  int rc = dev_alloc(ctx, (void **)&tmp, 24 * (size_t)n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_export_soa, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, n, NT, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(vrep, tmp, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= 24 * (int64_t)n;
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "repulsion download: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_step(zf_ctx *ctx, int32_t iter) {
+  if (!ctx) return ZF_ERR_ARG;
+  const bool rbc = ctx->have_ibm && ctx->ncell > 0 && iter % ctx->icfg.rbc_interval == 0;
+  int rc = zf_lbm_step(ctx, iter, rbc ? 1 : 0);
+  if (rc || !rbc) return rc;
+  if ((rc = membrane_forces_impl(ctx))) return rc;
+  if (ctx->have_rep && (rc = cell_list_impl(ctx))) return rc;
+  if ((rc = ibm_interpolate_impl(ctx))) return rc;
+  if (ctx->have_rep) {
+    if ((rc = repulsion_impl(ctx))) return rc;
+  } else {
+    const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+    hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
+  }
+  if ((rc = ibm_spread_impl(ctx))) return rc;
+  return advance_impl(ctx, nullptr);
+}
+
+}  // extern "C"

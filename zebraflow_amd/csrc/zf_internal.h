@@ -88,6 +88,55 @@ struct zf_ctx {
   float *d_force = nullptr;           // 3 x stride, storage-indexed; null = none
   double *d_sums = nullptr;           // reduction scratch
   int64_t device_bytes = 0;
+
+  // membranes (shared mesh, per-cell state)
+  bool have_mesh = false;
+  zf_membrane_config mcfg{};
+  int32_t nn = 0, nt = 0, ne = 0, ncell = 0;
+  std::vector<int32_t> h_edge, h_nbr_node, h_nbr_tri;  // reference row formats
+  std::vector<double> h_eq;       // 6 x ne: lmax plength kp theta0 sin0 cos0
+  int32_t *d_tri = nullptr;       // 3 x nt, 0-based
+  int32_t *d_edge = nullptr;      // 8 x ne: n1 n2 n3 n4 t1 t2 own1 own2 (0-based)
+  double *d_edge_eq = nullptr;    // 5 x ne: lmax plength kp sin0 cos0
+  double *d_tri_area_eq = nullptr;
+  int32_t *d_node_tri = nullptr;  // nn x 10 (count, tris 0-based)
+  int32_t *d_inc_start = nullptr, *d_inc = nullptr;  // node -> (edge<<2|role), edge order
+  double *d_x = nullptr;          // 3 x (ncap*nn) coordinates
+  double *d_v = nullptr;          // 3 x (ncap*nn) node velocities
+  double *d_mforce = nullptr;     // 3 x (ncap*nn) membrane force F_Total
+  double *d_nnormal = nullptr;    // 3 x (ncap*nn) node normals
+  double *d_tgeo = nullptr;       // 7 x (ncap*nt): area, normal(3), centroid(3)
+  double *d_cell = nullptr;       // 8 x ncap: area, vol, cx, cy, cz, F_area, F_vol, spare
+  int32_t *d_status = nullptr;    // ncap: >=0 live, <0 removed
+  int32_t *d_fail = nullptr;      // ncap: guard failures this evaluation
+
+  // immersed boundary
+  bool have_ibm = false;
+  zf_ibm_config icfg{};
+  int32_t *d_sbox = nullptr;      // box: storage index of the listed node or -1
+  float *d_umac = nullptr;        // 3 x stride: U,V,W without force shift (s order)
+  double *d_vrep = nullptr;       // 3 x (ncap*nn) repulsion velocity v_rep
+  float *d_ffl = nullptr;         // 3 x (ncap*nn): F_fluid of each node
+  uint64_t *d_keys[2] = {nullptr, nullptr};  // (tile << 32 | node) sort keys
+  void *d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int32_t *d_tile_range = nullptr;  // 2 x ntiles: [begin, end) in the sorted keys
+  int32_t *d_flag = nullptr;        // ncap: speed check
+  int ntx = 0, nty = 0, ntz = 0;
+
+  // cell list and repulsion
+  bool have_rep = false, have_wall = false;
+  zf_repulsion_config rcfg{};
+  uint64_t *d_tkeys[2] = {nullptr, nullptr};  // (bin << 32 | triangle) sort keys
+  void *d_tsort_tmp = nullptr;
+  size_t tsort_tmp_bytes = 0;
+  int32_t *d_tbin = nullptr;      // ncap*nt: 1-based DOMID of the triangle's bin or 0
+  int32_t *d_bin_range = nullptr; // 2 x dommax: [begin, end) in the sorted keys
+  int32_t *d_near = nullptr;      // ncap*nn: chosen wall triangle (1-based) or 0
+  int32_t nwn = 0, nwt = 0;
+  double *d_wxyz = nullptr, *d_wnormal = nullptr;  // 3 x nwn, 3 x nwt (AoS)
+  int32_t *d_wtri = nullptr, *d_wnode_tri = nullptr;  // 3 x nwt (1-based), 13 x nwn
+  int32_t *d_windex = nullptr;    // box
 };
 
 namespace zf {
@@ -107,6 +156,12 @@ void dev_free(zf_ctx *ctx, void **p);
 int upload_lattice_impl(zf_ctx *ctx, const int32_t *b);
 void free_lattice(zf_ctx *ctx);
 void free_state(zf_ctx *ctx);
+void free_membrane(zf_ctx *ctx);
+void free_ibm(zf_ctx *ctx);
+void free_cells(zf_ctx *ctx);
+int membrane_forces_impl(zf_ctx *ctx);
+int cell_list_impl(zf_ctx *ctx);
+int repulsion_impl(zf_ctx *ctx);
 int alloc_state(zf_ctx *ctx);
 size_t store_bytes(int storage);
 

@@ -1,0 +1,674 @@
+// zf_membrane.hip -- RBC membrane forces (coarse-grained spectrin network).
+//
+// Replaces, for every live cell, ZebraflowWT.c:
+//   CGSM_AREA triangle geometry + node normals   :5757-5851, :6089-6093
+//   anti-sliver terms                             :6042-6086
+//   cell area / centroid                          :6717-6796
+//   CGSM_LOCAL_FORCES (spring, bending, areas)    :6800-7384
+//   CGSM_VOLUME_CALC                              :7512-7535
+// and, on the host side of this file, the set-up of the shared mesh:
+//   topology CGSM_ZFISHRBC_DATA_GEN  :1632-1901
+//   equilibrium parameters           :1930-2223, :2338-2346, :2387-2460
+//
+// Kernels (all FP64 arithmetic; the membrane is a small fraction of a step):
+//   k_tri_geom     one lane per (cell, triangle): area, outward normal, centroid
+//   k_cell_reduce  one workgroup per cell: FP64 tree sums of area, volume,
+//                  centroid, then the global area and volume force scalars
+//   k_node_force   one lane per (cell, node): gathers the node's normal, its
+//                  anti-sliver terms and every rhombus (edge) it belongs to in
+//                  edge order, so F_Total is summed in the reference's order
+//                  with no atomics and no per-edge scratch.
+#include "zf_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+namespace zf {
+namespace {
+
+struct MembParams {
+  double L, kT, kbend, kd, ka, kv, area0, vol0, x_max, x_floor, cos_cut, area_lo, area_hi;
+  double force_limit, vol_limit, sliver_len, sliver_len_min, sliver_f1, sliver_f2, relax, relax2;
+};
+
+MembParams params_of(const zf_membrane_config &m) {
+  return MembParams{m.len_scale, m.kT, m.kbend, m.kd, m.ka, m.kv, m.area0, m.vol0, m.x_max,
+                    m.x_floor, m.cos_cut, m.area_lo, m.area_hi, m.force_limit, m.vol_limit,
+                    m.sliver_len, m.sliver_len_min, m.sliver_f1, m.sliver_f2, m.relax, m.relax2};
+}
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 a, D3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ D3 ld3(const double *a, size_t stride, size_t i) {
+  return {a[i], a[stride + i], a[2 * stride + i]};
+}
+__device__ __forceinline__ int nbad(D3 f, double lim) {
+  auto b = [lim](double v) { return (isnan(v) || isinf(v) || fabs(v) > lim) ? 1 : 0; };
+  return b(f.x) + b(f.y) + b(f.z);
+}
+
+// Triangle geometry, ZebraflowWT.c:5800-5831: A = (e1 x e3)/2 with
+// e1 = n2-n1, e3 = n1-n3; the outward normal is -A/|A|.
+__global__ void k_tri_geom(const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
+                           const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
+                           double L, double *__restrict__ tg, size_t TT) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * nt) return;
+  const int c = idx / nt, t = idx % nt;
+  if (status[c] < 0) return;
+  const size_t base = (size_t)c * nn;
+  const D3 p1 = ld3(X, NT, base + tri[3 * t]), p2 = ld3(X, NT, base + tri[3 * t + 1]),
+           p3 = ld3(X, NT, base + tri[3 * t + 2]);
+  const D3 cen = (1.0 / 3.0) * (p1 + p2 + p3);
+  const D3 e1 = p2 - p1, e3 = p1 - p3;
+  const D3 A = (0.5 * L * L) * cross(e1, e3);
+  const double a = sqrt(dot(A, A));
+  const size_t q = (size_t)c * nt + t;
+  tg[q] = a;
+  tg[TT + q] = -A.x / a;
+  tg[2 * TT + q] = -A.y / a;
+  tg[3 * TT + q] = -A.z / a;
+  tg[4 * TT + q] = cen.x;
+  tg[5 * TT + q] = cen.y;
+  tg[6 * TT + q] = cen.z;
+}
+
+// Per-cell FP64 reductions (ZebraflowWT.c:6729-6791, :7517-7524).
+__global__ __launch_bounds__(256) void k_cell_reduce(const double *__restrict__ X, size_t NT,
+                                                     int nn, int nt, const double *__restrict__ tg,
+                                                     size_t TT, const int32_t *__restrict__ status,
+                                                     MembParams mp, double *__restrict__ cell) {
+  __shared__ double sh[5][256];
+  const int c = blockIdx.x;
+  if (status[c] < 0) return;
+  double s[5] = {0, 0, 0, 0, 0};
+  const size_t nb = (size_t)c * nn, tb = (size_t)c * nt;
+  for (int n = threadIdx.x; n < nn; n += 256) {
+    s[0] += X[nb + n];
+    s[1] += X[NT + nb + n];
+    s[2] += X[2 * NT + nb + n];
+  }
+  for (int t = threadIdx.x; t < nt; t += 256) {
+    const size_t q = tb + t;
+    const double a = tg[q];
+    s[3] += a;
+    s[4] += a * (tg[4 * TT + q] * tg[TT + q] + tg[5 * TT + q] * tg[2 * TT + q] +
+                 tg[6 * TT + q] * tg[3 * TT + q]);
+  }
+  for (int k = 0; k < 5; k++) sh[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int k = 0; k < 5; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double area = sh[3][0], vol = sh[4][0] * mp.L / 3.0;
+    cell[8 * c + 0] = area;
+    cell[8 * c + 1] = vol;
+    cell[8 * c + 2] = sh[0][0] / nn;
+    cell[8 * c + 3] = sh[1][0] / nn;
+    cell[8 * c + 4] = sh[2][0] / nn;
+    cell[8 * c + 5] = -mp.ka * mp.relax * (area - mp.area0) / mp.area0;  // F_AREA_GLOBAL
+    cell[8 * c + 6] = -mp.kv * (vol - mp.vol0) / mp.vol0;               // F_VOLUME_SCALAR
+  }
+}
+
+struct EdgeOut {
+  D3 f, fv;
+  int fails;
+  bool has_fv;
+};
+
+// One rhombus (edge) evaluated for one of its four nodes (role r),
+// ZebraflowWT.c:6888-7384.  Within the edge the terms are summed in the
+// reference's order: spring, bending, tri1 local+global area, tri2.
+__device__ EdgeOut edge_role(int r, const int32_t *__restrict__ E, int ne, int e,
+                             const double *__restrict__ EQ, const double *__restrict__ X, size_t NT,
+                             size_t nbase, const double *__restrict__ tg, size_t TT, size_t tbase,
+                             const double *__restrict__ tae, D3 cc, double f_ag,
+                             const MembParams &mp) {
+  const int n1 = E[e], n2 = E[ne + e], n3 = E[2 * ne + e], n4 = E[3 * ne + e];
+  const int t1 = E[4 * ne + e], t2 = E[5 * ne + e];
+  const bool own1 = E[6 * ne + e] != 0, own2 = E[7 * ne + e] != 0;
+  const double lmax = EQ[e], plen = EQ[ne + e], kp = EQ[2 * ne + e], s0 = EQ[3 * ne + e],
+               c0 = EQ[4 * ne + e];
+  const double L = mp.L;
+  const D3 p1 = ld3(X, NT, nbase + n1), p2 = ld3(X, NT, nbase + n2), p3 = ld3(X, NT, nbase + n3),
+           p4 = ld3(X, NT, nbase + n4);
+  EdgeOut o{{0, 0, 0}, {0, 0, 0}, 0, false};
+  // WLC + POW spring (:6888-6924)
+  const D3 d12 = p1 - p2;
+  double len = sqrt(dot(d12, d12));
+  const D3 e1v = (1.0 / len) * d12;
+  double ratio = len * L / lmax;
+  if (ratio > mp.x_max) ratio = mp.x_max;
+  if (ratio < mp.x_floor) len = mp.x_floor * lmax / L;
+  const double wlc = -mp.kT / plen * (ratio - 0.25 + 1.0 / (4.0 * (1 - ratio) * (1 - ratio)));
+  const double comp = kp / (len * len * L * L);
+  if (r == 0) o.f = (mp.relax2 * wlc) * e1v + (mp.relax2 * comp) * e1v;
+  if (r == 1) o.f = (-mp.relax2 * wlc) * e1v + (-mp.relax2 * comp) * e1v;
+  // clamped triangle areas (:6929-6951)
+  const size_t q1 = tbase + t1, q2 = tbase + t2;
+  double a1 = tg[q1], a2 = tg[q2];
+  const double ae1 = tae[t1], ae2 = tae[t2];
+  if (a1 < mp.area_lo * ae1) a1 = mp.area_lo * ae1;
+  else if (a1 > mp.area_hi * ae1) a1 = mp.area_hi * ae1;
+  if (a2 < mp.area_lo * ae2) a2 = mp.area_lo * ae2;
+  else if (a2 > mp.area_hi * ae2) a2 = mp.area_hi * ae2;
+  const D3 m1 = {tg[TT + q1], tg[2 * TT + q1], tg[3 * TT + q1]};
+  const D3 m2 = {tg[TT + q2], tg[2 * TT + q2], tg[3 * TT + q2]};
+  const D3 C1 = {tg[4 * TT + q1], tg[5 * TT + q1], tg[6 * TT + q1]};
+  const D3 C2 = {tg[4 * TT + q2], tg[5 * TT + q2], tg[6 * TT + q2]};
+  double ct = dot(m1, m2);
+  if (ct > 1) ct = 1;
+  else if (ct < -1) ct = -1;
+  const double chk = dot(m1 - m2, L * (C1 - C2));
+  const double st = chk >= 0 ? sqrt(1 - ct * ct) : -sqrt(1 - ct * ct);
+  // bending (:6984-7081)
+  const double bb = fabs(ct) > mp.cos_cut ? 0.0 : mp.kbend * (st * c0 - ct * s0) / st;
+  const D3 dp21 = L * (p2 - p1), dp24 = L * (p2 - p4), dp32 = L * (p3 - p2), dp41 = L * (p4 - p1),
+           dp13 = L * (p1 - p3);
+  const D3 cp1 = a1 * cross(m1, dp21), cp2 = a2 * cross(m2, dp21), cp3 = a1 * cross(m1, dp24),
+           cp4 = a2 * cross(m2, dp24), cp5 = a1 * cross(m1, dp32), cp6 = a2 * cross(m2, dp32),
+           cp7 = a1 * cross(m1, dp41), cp8 = a2 * cross(m2, dp41), cp9 = a1 * cross(m1, dp13),
+           cp10 = a2 * cross(m2, dp13);
+  const double b11 = -bb * ct / (a1 * a1), b12 = bb / (a1 * a2), b22 = -bb * ct / (a2 * a2);
+  D3 fb;
+  if (r == 0) fb = b11 * cp5 + b12 * (cp3 + cp6) + b22 * cp4;
+  else if (r == 1) fb = b11 * cp9 + b12 * (cp7 + cp10) + b22 * cp8;
+  else if (r == 2) fb = b11 * cp1 + b12 * cp2;
+  else fb = (-b12) * cp1 + (-b22) * cp2;
+  const int kb = nbad(fb, mp.force_limit);
+  o.fails += kb;
+  if (!kb) o.f = o.f + fb;
+  // area and volume terms of the triangles this edge owns (:7126-7359)
+  for (int side = 0; side < 2; side++) {
+    if (side == 0 && (!own1 || r == 3)) continue;
+    if (side == 1 && (!own2 || r == 2)) continue;
+    const double a = side ? a2 : a1, ae = side ? ae2 : ae1;
+    const D3 m = side ? m2 : m1;
+    const D3 lc = L * ((side ? C2 : C1) - cc);
+    D3 g, gv;
+    if (side == 0) {
+      g = r == 0 ? cp5 : r == 1 ? cp9 : cp1;
+      gv = r == 0 ? cross(lc, dp32) : r == 1 ? cross(lc, dp13) : cross(lc, dp21);
+    } else {
+      g = r == 0 ? cp4 : r == 1 ? cp8 : (-1.0) * cp2;
+      gv = r == 0 ? cross(lc, dp24) : r == 1 ? cross(lc, dp41) : (-1.0) * cross(lc, dp21);
+    }
+    const double alpha = -mp.kd * mp.relax * (a - ae) / (4.0 * ae * a);
+    const D3 fa = alpha * g;
+    const D3 faa = (0.25 * f_ag / a) * g;
+    const D3 fv = (1.0 / 6.0) * ((a / 3.0) * m + gv);
+    const int ka = nbad(fa, mp.force_limit);
+    o.fails += ka;
+    if (!ka) o.f = o.f + fa;
+    const int kaa = nbad(faa, mp.force_limit);
+    o.fails += kaa;
+    if (!kaa) o.f = o.f + faa;
+    if (!nbad(fv, mp.vol_limit)) {
+      o.fv = o.fv + fv;
+      o.has_fv = true;
+    }
+  }
+  return o;
+}
+
+__global__ void k_node_force(const double *__restrict__ X, size_t NT, int nn, int nt, int ne,
+                             int ncell, const int32_t *__restrict__ tri,
+                             const int32_t *__restrict__ node_tri, const int32_t *__restrict__ inc_start,
+                             const int32_t *__restrict__ inc, const int32_t *__restrict__ E,
+                             const double *__restrict__ EQ, const double *__restrict__ tae,
+                             const double *__restrict__ tg, size_t TT, const double *__restrict__ cell,
+                             const int32_t *__restrict__ status, MembParams mp,
+                             double *__restrict__ F, double *__restrict__ NN,
+                             int32_t *__restrict__ fail) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * nn) return;
+  const int c = idx / nn, n = idx % nn;
+  const size_t q = (size_t)c * nn + n;
+  if (status[c] < 0) {
+    F[q] = F[NT + q] = F[2 * NT + q] = 0;
+    return;
+  }
+  const size_t nbase = (size_t)c * nn, tbase = (size_t)c * nt;
+  const D3 xn = ld3(X, NT, q);
+  // node area and normal, anti-sliver terms, in triangle order
+  D3 f = {0, 0, 0}, nrm = {0, 0, 0};
+  double narea = 0;
+  const int ntri = node_tri[10 * n];
+  for (int k = 1; k <= ntri && k < 10; k++) {
+    const int t = node_tri[10 * n + k];
+    const size_t qt = tbase + t;
+    const double a = tg[qt];
+    const D3 mt = {tg[TT + qt], tg[2 * TT + qt], tg[3 * TT + qt]};
+    const D3 ct = {tg[4 * TT + qt], tg[5 * TT + qt], tg[6 * TT + qt]};
+    narea += a / 3.0;
+    nrm = nrm + (a / 3.0) * mt;
+    const D3 ev = xn - ct;
+    const double l = sqrt(dot(ev, ev));
+    if (l < mp.sliver_len) {
+      const double mag = l > mp.sliver_len_min ? mp.relax2 * mp.sliver_f1 / l : mp.relax2 * mp.sliver_f2;
+      f = f + (mag / l) * ev;
+    }
+  }
+  const double inv = 1.0 / narea;
+  NN[q] = nrm.x * inv;
+  NN[NT + q] = nrm.y * inv;
+  NN[2 * NT + q] = nrm.z * inv;
+  // every rhombus containing the node, in edge order
+  const D3 cc = {cell[8 * c + 2], cell[8 * c + 3], cell[8 * c + 4]};
+  const double f_ag = cell[8 * c + 5], f_vs = cell[8 * c + 6];
+  D3 fv = {0, 0, 0};
+  int fails = 0;
+  for (int k = inc_start[n]; k < inc_start[n + 1]; k++) {
+    const int code = inc[k];
+    const EdgeOut o = edge_role(code & 3, E, ne, code >> 2, EQ, X, NT, nbase, tg, TT, tbase, tae,
+                                cc, f_ag, mp);
+    f = f + o.f;
+    if (o.has_fv) fv = fv + o.fv;
+    fails += o.fails;
+  }
+  // volume constraint (:7527-7531)
+  f = f + (mp.relax * f_vs) * fv;
+  F[q] = f.x;
+  F[NT + q] = f.y;
+  F[2 * NT + q] = f.z;
+  if (fails) atomicAdd(&fail[c], fails);
+}
+
+__global__ void k_zero_i32(int32_t *a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// (cell, node, axis) host order <-> SoA device order
+__global__ void k_cells_import(const double *__restrict__ h, int ncell, int nn, size_t NT,
+                               double *__restrict__ X, double *__restrict__ V) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * nn) return;
+  for (int d = 0; d < 3; d++) {
+    X[d * NT + idx] = h[3 * (size_t)idx + d];
+    V[d * NT + idx] = 0.0;
+  }
+}
+
+__global__ void k_soa_export(const double *__restrict__ A, int n, size_t NT, double *__restrict__ h) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  for (int d = 0; d < 3; d++) h[3 * (size_t)idx + d] = A[d * NT + idx];
+}
+
+inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// ---- host: topology and equilibrium parameters ------------------------------
+
+// Edge list by first appearance of the pairs (1,2),(2,3),(3,1) over the
+// triangles; n3/t1 from the first triangle holding the edge, n4/t2 from the
+// second (ZebraflowWT.c:1655-1805).
+void build_topology(int nn, int nt, const int32_t *tri, std::vector<int32_t> &edge,
+                    std::vector<int32_t> &nbr_node, std::vector<int32_t> &nbr_tri) {
+  std::map<std::pair<int, int>, int> id;
+  edge.clear();
+  for (int t = 0; t < nt; t++)
+    for (int s = 0; s < 3; s++) {
+      const int p = tri[3 * t + s], q = tri[3 * t + (s + 1) % 3];
+      const auto key = std::make_pair(std::min(p, q), std::max(p, q));
+      if (id.count(key)) continue;
+      id[key] = (int)edge.size() / 6;
+      edge.insert(edge.end(), {p, q, 0, 0, 0, 0});
+    }
+  for (int t = 0; t < nt; t++)
+    for (int s = 0; s < 3; s++) {
+      const int p = tri[3 * t + s], q = tri[3 * t + (s + 1) % 3], w = tri[3 * t + (s + 2) % 3];
+      int32_t *r = &edge[6 * id[std::make_pair(std::min(p, q), std::max(p, q))]];
+      if (r[2] == 0) { r[2] = w; r[4] = t + 1; }
+      else if (r[3] == 0) { r[3] = w; r[5] = t + 1; }
+    }
+  const int ne = (int)edge.size() / 6;
+  nbr_node.assign((size_t)nn * 10, 0);
+  nbr_tri.assign((size_t)nn * 10, 0);
+  for (int e = 0; e < ne; e++)
+    for (int s = 0; s < 2; s++) {
+      const int n = edge[6 * e + s], other = edge[6 * e + 1 - s];
+      int32_t *row = &nbr_node[(size_t)(n - 1) * 10];
+      row[0]++;
+      if (row[0] < 10) row[row[0]] = other;
+    }
+  for (int t = 0; t < nt; t++)
+    for (int s = 0; s < 3; s++) {
+      int32_t *row = &nbr_tri[(size_t)(tri[3 * t + s] - 1) * 10];
+      row[0]++;
+      if (row[0] < 10) row[row[0]] = t + 1;
+    }
+}
+
+// WLC-POW fixed point for Plength, kp (ZebraflowWT.c:1942-1990) and the
+// rest dihedral angle (:2081-2210).
+void build_equilibrium(const zf_membrane_config &m, const double *x, const std::vector<int32_t> &edge,
+                       std::vector<double> &eq) {
+  const int ne = (int)edge.size() / 6;
+  const double L = m.len_scale, x0 = m.x0;
+  const double beta = 0.5 * x0 / std::pow(1 - x0, 3) - 0.25 / ((1 - x0) * (1 - x0)) + 0.25;
+  eq.assign((size_t)6 * ne, 0.0);
+  auto P = [&](int n, int d) { return x[3 * (n - 1) + d]; };
+  for (int e = 0; e < ne; e++) {
+    const int n1 = edge[6 * e], n2 = edge[6 * e + 1], n3 = edge[6 * e + 2], n4 = edge[6 * e + 3];
+    double l2 = 0;
+    for (int d = 0; d < 3; d++) l2 += (P(n1, d) - P(n2, d)) * (P(n1, d) - P(n2, d));
+    const double l = std::sqrt(l2);
+    const double alpha = std::sqrt(3.0) * 0.25 * m.kT / (l * L);
+    const double eta = std::sqrt(3.0) * 0.25 * 3.0 / (l * l * l * L * L * L);
+    double plen = 1e-9 * (L / 1e-6), kp = 0;
+    for (int pass = 0; pass < m.passes; pass++) {
+      kp = (m.es - alpha * beta / plen) / eta;
+      plen = l * l * L * L * m.kT * (x0 - 0.25 + 1.0 / (4.0 * (1 - x0) * (1 - x0))) / kp;
+    }
+    double a1[3], a2[3], c1[3], c2[3], u1[3], v1[3], u2[3], v2[3];
+    for (int d = 0; d < 3; d++) {
+      u1[d] = P(n2, d) - P(n1, d); v1[d] = P(n3, d) - P(n1, d);
+      u2[d] = P(n1, d) - P(n2, d); v2[d] = P(n4, d) - P(n2, d);
+      c1[d] = (P(n1, d) + P(n2, d) + P(n3, d)) / 3.0;
+      c2[d] = (P(n1, d) + P(n2, d) + P(n4, d)) / 3.0;
+    }
+    auto crs = [](const double *a, const double *b, double *o) {
+      o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    crs(u1, v1, a1);
+    crs(u2, v2, a2);
+    const double m1 = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+    const double m2 = std::sqrt(a2[0] * a2[0] + a2[1] * a2[1] + a2[2] * a2[2]);
+    double ct = 0, chk = 0;
+    for (int d = 0; d < 3; d++) {
+      ct += a1[d] / m1 * a2[d] / m2;
+      chk += (a1[d] / m1 - a2[d] / m2) * (c1[d] - c2[d]);
+    }
+    ct = std::max(-1.0, std::min(1.0, ct));
+    const double st = chk >= 0 ? std::sqrt(1 - ct * ct) : -std::sqrt(1 - ct * ct);
+    eq[e] = L * l / x0;
+    eq[ne + e] = plen;
+    eq[2 * ne + e] = kp;
+    eq[3 * ne + e] = std::asin(st);
+    eq[4 * ne + e] = st;
+    eq[5 * ne + e] = ct;
+  }
+}
+
+void free_mesh(zf_ctx *ctx) {
+  void **bufs[] = {(void **)&ctx->d_tri,       (void **)&ctx->d_edge,      (void **)&ctx->d_edge_eq,
+                   (void **)&ctx->d_tri_area_eq, (void **)&ctx->d_node_tri, (void **)&ctx->d_inc_start,
+                   (void **)&ctx->d_inc,       (void **)&ctx->d_x,         (void **)&ctx->d_v,
+                   (void **)&ctx->d_mforce,    (void **)&ctx->d_nnormal,   (void **)&ctx->d_tgeo,
+                   (void **)&ctx->d_cell,      (void **)&ctx->d_status,    (void **)&ctx->d_fail};
+  for (auto b : bufs) dev_free(ctx, b);
+  ctx->have_mesh = false;
+  ctx->ncell = 0;
+}
+
+template <class T>
+int upload(zf_ctx *ctx, T **d, const std::vector<T> &h) {
+  int rc = dev_alloc(ctx, (void **)d, sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (rc) return rc;
+  if (!h.empty()) ZF_HIP(ctx, hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return ZF_OK;
+}
+
+}  // namespace
+
+void free_membrane(zf_ctx *ctx) { free_mesh(ctx); }
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+void zf_membrane_config_reference(zf_membrane_config *m) {
+  *m = zf_membrane_config{};
+  m->len_scale = 1e-6;
+  m->kT = 300 * 1.3806488e-23;  // TEMPERATURE*BOLTZ, ZebraflowWT.c:35-36
+  m->x0 = 0.3154574132;         // e_ratio_eq, :133
+  m->es = 3e-6;                 // Es, :128
+  m->kbend = 1.3856406e-19;     // kBend, :127
+  m->kd = 3.6e-5;
+  m->ka = 7.7e-5;
+  m->kv = 55.0;
+  m->area0 = 144.916e-12;       // RBC_SURF, :126
+  m->vol0 = 97.5671e-18;        // MCV, :125
+  m->x_max = 0.95;
+  m->x_floor = 0.02435;
+  m->cos_cut = 0.99;
+  m->area_lo = 0.1;
+  m->area_hi = 10.0;
+  m->force_limit = 1e-10;
+  m->vol_limit = 1e-10;
+  m->sliver_len = 0.05;
+  m->sliver_len_min = 0.001;
+  m->sliver_f1 = 1e-14;
+  m->sliver_f2 = 1e-11;
+  m->relax = 1.0;
+  m->relax2 = 1.0;
+  m->passes = 51;
+  m->max_cells = 1;
+}
+
+int zf_membrane_setup(zf_ctx *ctx, const zf_membrane_config *mc, int32_t nn, int32_t nt,
+                      const int32_t *tri, const double *rest_xyz) {
+  if (!ctx || !mc || !tri || !rest_xyz || nn <= 3 || nt <= 3 || mc->max_cells <= 0 || mc->passes < 1)
+    return ZF_ERR_ARG;
+  for (int i = 0; i < 3 * nt; i++)
+    if (tri[i] < 1 || tri[i] > nn) return fail(ctx, ZF_ERR_ARG, "triangle node id out of range");
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  free_cells(ctx);  // sized by the old mesh
+  free_ibm(ctx);
+  free_mesh(ctx);
+  ctx->mcfg = *mc;
+  ctx->nn = nn;
+  ctx->nt = nt;
+  build_topology(nn, nt, tri, ctx->h_edge, ctx->h_nbr_node, ctx->h_nbr_tri);
+  const int ne = (int)ctx->h_edge.size() / 6;
+  ctx->ne = ne;
+  for (int e = 0; e < ne; e++)
+    if (ctx->h_edge[6 * e + 3] == 0) return fail(ctx, ZF_ERR_GEOMETRY, "mesh is not closed (edge %d)", e + 1);
+  for (int n = 0; n < nn; n++)
+    if (ctx->h_nbr_tri[10 * n] > 9) return fail(ctx, ZF_ERR_GEOMETRY, "node %d has >9 triangles", n + 1);
+  build_equilibrium(*mc, rest_xyz, ctx->h_edge, ctx->h_eq);
+  // device tables
+  std::vector<int32_t> dtri(3 * (size_t)nt), dedge(8 * (size_t)ne), ntri(10 * (size_t)nn);
+  for (int i = 0; i < 3 * nt; i++) dtri[i] = tri[i] - 1;
+  std::vector<int> owner(nt, -1);
+  for (int e = 0; e < ne; e++)
+    for (int s = 0; s < 2; s++) {
+      const int t = ctx->h_edge[6 * e + 4 + s] - 1;
+      if (owner[t] < 0) owner[t] = e;
+    }
+  for (int e = 0; e < ne; e++) {
+    for (int k = 0; k < 6; k++) dedge[(size_t)k * ne + e] = ctx->h_edge[6 * e + k] - 1;
+    dedge[6 * (size_t)ne + e] = owner[ctx->h_edge[6 * e + 4] - 1] == e;
+    dedge[7 * (size_t)ne + e] = owner[ctx->h_edge[6 * e + 5] - 1] == e;
+  }
+  for (int n = 0; n < nn; n++) {
+    ntri[10 * n] = ctx->h_nbr_tri[10 * n];
+    for (int k = 1; k < 10; k++) ntri[10 * n + k] = ctx->h_nbr_tri[10 * n + k] - 1;
+  }
+  std::vector<double> eqd(5 * (size_t)ne), tae(nt);
+  for (int e = 0; e < ne; e++) {
+    eqd[e] = ctx->h_eq[e];
+    eqd[ne + e] = ctx->h_eq[ne + e];
+    eqd[2 * ne + e] = ctx->h_eq[2 * ne + e];
+    eqd[3 * ne + e] = ctx->h_eq[4 * ne + e];
+    eqd[4 * ne + e] = ctx->h_eq[5 * ne + e];
+  }
+  const double L = mc->len_scale;
+  for (int t = 0; t < nt; t++) {
+    const double *p1 = &rest_xyz[3 * (tri[3 * t] - 1)], *p2 = &rest_xyz[3 * (tri[3 * t + 1] - 1)],
+                 *p3 = &rest_xyz[3 * (tri[3 * t + 2] - 1)];
+    const double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    const double v[3] = {p1[0] - p3[0], p1[1] - p3[1], p1[2] - p3[2]};
+    const double ax = u[1] * v[2] - u[2] * v[1], ay = u[2] * v[0] - u[0] * v[2], az = u[0] * v[1] - u[1] * v[0];
+    tae[t] = 0.5 * L * L * std::sqrt(ax * ax + ay * ay + az * az);
+  }
+  // node -> (edge, role) incidence in edge order
+  std::vector<std::vector<int32_t>> per(nn);
+  for (int e = 0; e < ne; e++)
+    for (int r = 0; r < 4; r++) per[ctx->h_edge[6 * e + r] - 1].push_back((e << 2) | r);
+  std::vector<int32_t> istart(nn + 1, 0), inc;
+  for (int n = 0; n < nn; n++) {
+    istart[n] = (int32_t)inc.size();
+    inc.insert(inc.end(), per[n].begin(), per[n].end());
+  }
+  istart[nn] = (int32_t)inc.size();
+  int rc;
+  if ((rc = upload(ctx, &ctx->d_tri, dtri)) || (rc = upload(ctx, &ctx->d_edge, dedge)) ||
+      (rc = upload(ctx, &ctx->d_edge_eq, eqd)) || (rc = upload(ctx, &ctx->d_tri_area_eq, tae)) ||
+      (rc = upload(ctx, &ctx->d_node_tri, ntri)) || (rc = upload(ctx, &ctx->d_inc_start, istart)) ||
+      (rc = upload(ctx, &ctx->d_inc, inc)))
+    return rc;
+  const size_t ncap = (size_t)mc->max_cells, NT = ncap * nn, TT = ncap * nt;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_x, 24 * NT)) || (rc = dev_alloc(ctx, (void **)&ctx->d_v, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_mforce, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_nnormal, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_tgeo, 56 * TT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_cell, 64 * ncap)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_status, 4 * ncap)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_fail, 4 * ncap)))
+    return rc;
+  ctx->have_mesh = true;
+  return ZF_OK;
+}
+
+int zf_download_mesh_topology(zf_ctx *ctx, int32_t *ne, int32_t *edge, int32_t *nbr_node,
+                              int32_t *nbr_tri) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  if (ne) *ne = ctx->ne;
+  if (edge) std::copy(ctx->h_edge.begin(), ctx->h_edge.end(), edge);
+  if (nbr_node) std::copy(ctx->h_nbr_node.begin(), ctx->h_nbr_node.end(), nbr_node);
+  if (nbr_tri) std::copy(ctx->h_nbr_tri.begin(), ctx->h_nbr_tri.end(), nbr_tri);
+  return ZF_OK;
+}
+
+int zf_download_mesh_equilibrium(zf_ctx *ctx, double *lmax, double *plength, double *kp,
+                                 double *theta0, double *sin0, double *cos0) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  double *outs[6] = {lmax, plength, kp, theta0, sin0, cos0};
+  for (int k = 0; k < 6; k++)
+    if (outs[k]) std::copy(ctx->h_eq.begin() + (size_t)k * ctx->ne, ctx->h_eq.begin() + (size_t)(k + 1) * ctx->ne, outs[k]);
+  return ZF_OK;
+}
+
+int zf_upload_cells(zf_ctx *ctx, int32_t ncell, const double *xyz) {
+  if (!ctx || (ncell > 0 && !xyz) || ncell < 0) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  if (ncell > ctx->mcfg.max_cells) return fail(ctx, ZF_ERR_ARG, "%d cells > max_cells %d", ncell, ctx->mcfg.max_cells);
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn, n = (size_t)ncell * ctx->nn;
+  ctx->ncell = ncell;
+  std::vector<int32_t> st(ctx->mcfg.max_cells, -5000);
+  for (int c = 0; c < ncell; c++) st[c] = 1;
+  ZF_HIP(ctx, hipMemcpy(ctx->d_status, st.data(), 4 * st.size(), hipMemcpyHostToDevice));
+  if (n == 0) return ZF_OK;
+  double *tmp = nullptr;
+  int rc = dev_alloc(ctx, (void **)&tmp, 24 * n);
+  if (rc) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, xyz, 24 * n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_cells_import, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, tmp, ncell,
+                       ctx->nn, NT, ctx->d_x, ctx->d_v);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)(24 * n);
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "cell upload: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+static int export_soa(zf_ctx *ctx, const double *src, double *dst) {
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn, n = (size_t)ctx->ncell * ctx->nn;
+  if (!dst || n == 0) return ZF_OK;
+  double *tmp = nullptr;
+  int rc = dev_alloc(ctx, (void **)&tmp, 24 * n);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_soa_export, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, src, (int)n, NT, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, tmp, 24 * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)(24 * n);
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "export: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_download_cells(zf_ctx *ctx, double *xyz, double *vel, int32_t *status) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  int rc;
+  if ((rc = export_soa(ctx, ctx->d_x, xyz)) || (rc = export_soa(ctx, ctx->d_v, vel))) return rc;
+  if (status && ctx->ncell)
+    ZF_HIP(ctx, hipMemcpy(status, ctx->d_status, 4 * (size_t)ctx->ncell, hipMemcpyDeviceToHost));
+  return ZF_OK;
+}
+
+int zf_membrane_forces(zf_ctx *ctx) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  return membrane_forces_impl(ctx);
+}
+
+}  // extern "C"
+
+int zf::membrane_forces_impl(zf_ctx *ctx) {
+  if (ctx->ncell == 0) return ZF_OK;
+  const MembParams mp = params_of(ctx->mcfg);
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
+  const int nc = ctx->ncell;
+  hipLaunchKernelGGL(k_zero_i32, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, ctx->d_fail, nc);
+  hipLaunchKernelGGL(k_tri_geom, dim3(nb((size_t)nc * ctx->nt, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_x, NT, ctx->nn, ctx->nt, nc, ctx->d_tri, ctx->d_status, mp.L, ctx->d_tgeo, TT);
+  hipLaunchKernelGGL(k_cell_reduce, dim3(nc), dim3(256), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
+                     ctx->nt, ctx->d_tgeo, TT, ctx->d_status, mp, ctx->d_cell);
+  hipLaunchKernelGGL(k_node_force, dim3(nb((size_t)nc * ctx->nn, 128)), dim3(128), 0, ctx->stream,
+                     ctx->d_x, NT, ctx->nn, ctx->nt, ctx->ne, nc, ctx->d_tri, ctx->d_node_tri,
+                     ctx->d_inc_start, ctx->d_inc, ctx->d_edge, ctx->d_edge_eq, ctx->d_tri_area_eq,
+                     ctx->d_tgeo, TT, ctx->d_cell, ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal,
+                     ctx->d_fail);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+extern "C" {
+
+int zf_download_membrane(zf_ctx *ctx, double *force, double *node_normal, double *cell_area,
+                         double *cell_vol, double *cell_centroid, int32_t *guard_fail) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  int rc;
+  if ((rc = export_soa(ctx, ctx->d_mforce, force)) || (rc = export_soa(ctx, ctx->d_nnormal, node_normal)))
+    return rc;
+  const int nc = ctx->ncell;
+  if (nc == 0) return ZF_OK;
+  std::vector<double> cell(8 * (size_t)nc);
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ZF_HIP(ctx, hipMemcpy(cell.data(), ctx->d_cell, 64 * (size_t)nc, hipMemcpyDeviceToHost));
+  for (int c = 0; c < nc; c++) {
+    if (cell_area) cell_area[c] = cell[8 * c];
+    if (cell_vol) cell_vol[c] = cell[8 * c + 1];
+    if (cell_centroid)
+      for (int d = 0; d < 3; d++) cell_centroid[3 * c + d] = cell[8 * c + 2 + d];
+  }
+  if (guard_fail) ZF_HIP(ctx, hipMemcpy(guard_fail, ctx->d_fail, 4 * (size_t)nc, hipMemcpyDeviceToHost));
+  return ZF_OK;
+}
+
+}  // extern "C"

@@ -159,6 +159,119 @@ class Engine:
                     "zf_lbm_time_kernels")
         return out
 
+    # -- membranes -----------------------------------------------------------
+    def membrane_setup(self, mc, tris, rest_xyz):
+        tris = np.ascontiguousarray(tris, np.int32)
+        rest = np.ascontiguousarray(rest_xyz, np.float64)
+        self._check(self.lib.zf_membrane_setup(self.h, C.byref(mc), len(rest), len(tris), _ptr(tris),
+                                               _ptr(rest)), "zf_membrane_setup")
+        self.nn, self.nt = len(rest), len(tris)
+        ne = C.c_int32()
+        self._check(self.lib.zf_download_mesh_topology(self.h, C.byref(ne), None, None, None),
+                    "zf_download_mesh_topology")
+        self.ne = ne.value
+
+    def mesh_topology(self):
+        edge = np.zeros((self.ne, 6), np.int32)
+        nbn = np.zeros((self.nn, 10), np.int32)
+        nbt = np.zeros((self.nn, 10), np.int32)
+        self._check(self.lib.zf_download_mesh_topology(self.h, None, _ptr(edge), _ptr(nbn),
+                                                       _ptr(nbt)), "zf_download_mesh_topology")
+        return edge, nbn, nbt
+
+    def mesh_equilibrium(self):
+        out = [np.zeros(self.ne) for _ in range(6)]
+        self._check(self.lib.zf_download_mesh_equilibrium(self.h, *[_ptr(a) for a in out]),
+                    "zf_download_mesh_equilibrium")
+        return dict(zip(("lmax", "plength", "kp", "theta0", "sin0", "cos0"), out))
+
+    def upload_cells(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        self.ncell = len(xyz)
+        self._check(self.lib.zf_upload_cells(self.h, self.ncell, _ptr(xyz)), "zf_upload_cells")
+
+    def cells(self):
+        x = np.zeros((self.ncell, self.nn, 3))
+        v = np.zeros((self.ncell, self.nn, 3))
+        st = np.zeros(self.ncell, np.int32)
+        self._check(self.lib.zf_download_cells(self.h, _ptr(x), _ptr(v), _ptr(st)), "zf_download_cells")
+        return x, v, st
+
+    def membrane_forces(self):
+        self._check(self.lib.zf_membrane_forces(self.h), "zf_membrane_forces")
+
+    def membrane(self):
+        nc, nn = self.ncell, self.nn
+        out = dict(force=np.zeros((nc, nn, 3)), node_normal=np.zeros((nc, nn, 3)),
+                   cell_area=np.zeros(nc), cell_vol=np.zeros(nc), cell_centroid=np.zeros((nc, 3)),
+                   guard_fail=np.zeros(nc, np.int32))
+        self._check(self.lib.zf_download_membrane(
+            self.h, *[_ptr(out[k]) for k in ("force", "node_normal", "cell_area", "cell_vol",
+                                             "cell_centroid", "guard_fail")]), "zf_download_membrane")
+        return out
+
+    # -- immersed boundary ------------------------------------------------------
+    def ibm_setup(self, ic):
+        self._check(self.lib.zf_ibm_setup(self.h, C.byref(ic)), "zf_ibm_setup")
+
+    def ibm_interpolate(self):
+        self._check(self.lib.zf_ibm_interpolate(self.h), "zf_ibm_interpolate")
+
+    def ibm_spread(self):
+        self._check(self.lib.zf_ibm_spread(self.h), "zf_ibm_spread")
+
+    def advance_cells(self):
+        n = C.c_int32()
+        self._check(self.lib.zf_advance_cells(self.h, C.byref(n)), "zf_advance_cells")
+        return n.value
+
+    def upload_membrane_force(self, f):
+        f = np.ascontiguousarray(f, np.float64)
+        self._check(self.lib.zf_upload_membrane_force(self.h, _ptr(f)), "zf_upload_membrane_force")
+
+    def upload_repulsion(self, v):
+        v = np.ascontiguousarray(v, np.float64)
+        self._check(self.lib.zf_upload_repulsion(self.h, _ptr(v)), "zf_upload_repulsion")
+
+    def repulsion(self):
+        v = np.zeros((self.ncell, self.nn, 3))
+        self._check(self.lib.zf_download_repulsion(self.h, _ptr(v)), "zf_download_repulsion")
+        return v
+
+    def full_step(self, it):
+        self._check(self.lib.zf_step(self.h, it), "zf_step")
+
+    # -- cell list and repulsion ------------------------------------------------
+    def repulsion_setup(self, rc):
+        self._check(self.lib.zf_repulsion_setup(self.h, C.byref(rc)), "zf_repulsion_setup")
+
+    def wall_setup(self, xyz, tri, normal):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        tri = np.ascontiguousarray(tri, np.int32)
+        normal = np.ascontiguousarray(normal, np.float64)
+        self._check(self.lib.zf_wall_setup(self.h, len(xyz), _ptr(xyz), len(tri), _ptr(tri),
+                                           _ptr(normal)), "zf_wall_setup")
+
+    def windex(self):
+        w = np.zeros((self.cfg.im, self.cfg.jm, self.cfg.km), np.int32)
+        self._check(self.lib.zf_download_windex(self.h, _ptr(w)), "zf_download_windex")
+        return w
+
+    def cell_list(self):
+        self._check(self.lib.zf_cell_list(self.h), "zf_cell_list")
+        out = np.zeros((self.ncell, self.nt), np.int32)
+        self._check(self.lib.zf_download_cell_list(self.h, _ptr(out)), "zf_download_cell_list")
+        return out
+
+    def repulse(self):
+        self._check(self.lib.zf_repulsion(self.h), "zf_repulsion")
+
+    def wall_neighbour(self):
+        out = np.zeros((self.ncell, self.nn), np.int32)
+        self._check(self.lib.zf_download_wall_neighbour(self.h, _ptr(out)),
+                    "zf_download_wall_neighbour")
+        return out
+
     def sync(self):
         self._check(self.lib.zf_sync(self.h), "zf_sync")
 
@@ -169,3 +282,14 @@ class Engine:
         ms = C.c_float()
         self._check(self.lib.zf_event_elapsed(self.h, a, b, C.byref(ms)), "zf_event_elapsed")
         return ms.value
+
+
+def membrane_config(d, max_cells):
+    """ZfMembraneConfig from a dict of the oracle's parameter names plus es."""
+    from ._abi import ZfMembraneConfig
+    mc = ZfMembraneConfig()
+    load_library().zf_membrane_config_reference(C.byref(mc))
+    for k, v in d.items():
+        setattr(mc, k, v)
+    mc.max_cells = max_cells
+    return mc

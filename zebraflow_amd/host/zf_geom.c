@@ -205,6 +205,12 @@ int64_t zfh_geom_tree(int32_t *b, int im, int jm, int km, double r_root, double 
       br[nb++] = (branch_t){{e[0], e[1], e[2]}, {d[0], d[1], d[2]}, p.r * ratio, seglen, p.gen + 1};
     }
   }
+  /* branches that reach a side face (j or k) are cut one node short of it,
+   * so the wall layer closes them inside the box */
+  for (int i = 1; i <= im; i++)
+    for (int j = 1; j <= jm; j++)
+      for (int k = 1; k <= km; k++)
+        if (j == 1 || j == jm || k == 1 || k == km) tag[bq(i, j, k, jm, km)] = 0;
   const int per[3] = {0, 0, 0};
   const int64_t nf = finish(b, tag, im, jm, km, per);
   for (int j = 1; j <= jm; j++)

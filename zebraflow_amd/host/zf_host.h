@@ -58,6 +58,14 @@ int zfh_place_cells(const int32_t *b, int im, int jm, int km, const double *ref,
                     double r_cell, double gap, int ncell, uint64_t seed, int max_tries,
                     double *xyz_out);
 
+/* Lumen wall mesh of a B field by surface nets on the fluid indicator B > 0:
+ * vertices at lattice-cube crossings, one quad per crossing lattice edge,
+ * normals into the fluid.  Coordinates x0 + (i-1) dx.  With xyz and tri NULL
+ * only counts.  tri: 1-based.  Returns 0, -2 if a capacity is too small. */
+int zfh_wall_mesh(const int32_t *b, int im, int jm, int km, double x0, double y0, double z0,
+                  double dx, double *xyz, int64_t node_cap, int32_t *tri, double *normal,
+                  int64_t tri_cap, int64_t *n_node, int64_t *n_tri);
+
 /* ---- reference file formats ----------------------------------------------- */
 /* FLUIDdomainBindex.csv (ZebraflowWT.c:2574-2619): header, then
  * "bvalue,x,y,z" for every node with B >= 0; x = x0 + (i-1)*dx. */
