@@ -54,6 +54,16 @@ def make_geometry(name):
     return g.tree(w["box"], w["r_root"], w["ratio"], w["gens"], w["a_lo"], w["a_hi"], w["seed"])
 
 
+FLAGS = 0  # zf_config.flags of the measured forms (--flags)
+
+# HBM bytes per stream-collide launch on vessel_pair_15 (default flags), from
+# two rocprofv3 --pmc runs of `bench.py --probe` (FETCH_SIZE, then WRITE_SIZE,
+# KiB per dispatch, averaged over the 20 launches of each form), each scaled by
+# what the counter reads for the probe's 256 MiB copy kernel: FETCH_SIZE x 2.0,
+# WRITE_SIZE x 1.0 (profiles/stream_collide_traffic_vessel_pair_15.md).
+TRAFFIC = {}
+
+
 def engine_for(name, b, store):
     from zebraflow_amd.engine import Engine, lattice_config
     w = WORKLOADS[name]
@@ -61,11 +71,11 @@ def engine_for(name, b, store):
         # cfg1: tau 0.8, body force giving a peak velocity of 0.02 (g = 4 nu u / R^2)
         nu = 0.3 / 3
         cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], periodic=(0, 0, 1),
-                             body_force=(0, 0, 4 * nu * 0.02 / w["r"] ** 2))
+                             body_force=(0, 0, 4 * nu * 0.02 / w["r"] ** 2), flags=FLAGS)
     else:
         # pulsatile inlet u(t) = 0.03 (1 + 0.5 sin(2 pi t / 4000)) along +i
         cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], inlet_u0=0.03,
-                             inlet_amp=0.5, inlet_period=4000.0, inlet_dir=(1, 0, 0))
+                             inlet_amp=0.5, inlet_period=4000.0, inlet_dir=(1, 0, 0), flags=FLAGS)
     eng = Engine(cfg)
     eng.upload_lattice(b)
     return eng
@@ -85,16 +95,22 @@ def time_form(name, b, store, steps, warmup):
     return eng, ms, wall
 
 
-def kernel_roofline(eng, store, n_int, steps):
-    t = eng.time_kernels(10 ** 6, steps)
-    avg_ms = t[0] / t[2]
+def kernel_roofline(eng, store, n_int, ms, steps):
+    """The timed region is `steps` back-to-back launches of the one fused
+    stream-collide kernel on the engine's stream, so its HIP-event time over
+    `steps` is the kernel's average launch duration (dispatch gaps included).
+    A second pass brackets every launch with its own event pair (adds the
+    event overhead; reported for comparison)."""
+    avg_ms = ms / steps
     algo = 2 * 19 * STORE_BYTES[store] * n_int
     achieved = algo / (avg_ms * 1e-3) / 1e9
+    t = eng.time_kernels(10 ** 6, min(steps, 500))
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get(store),
                 kernel="k_stream_collide", avg_us=round(avg_ms * 1e3, 3),
                 algorithmic_bytes_per_launch=algo,
-                boundary_kernel_avg_us=round(t[1] / steps * 1e3, 3))
+                per_launch_event_us=round(t[0] / t[2] * 1e3, 3),
+                note="boundary nodes are updated inside the same launch")
 
 
 def velocity_error(name, b, steps):
@@ -139,6 +155,21 @@ def cpu_baseline(name, b, seconds):
                        f"schedule(static,5000)), {el:.1f} s")
 
 
+def probe(name, b):
+    """A copy of known size (256 MiB read + 256 MiB written per launch, three
+    launches) to calibrate the HBM counters, then 20 steps of each form."""
+    eng = engine_for(name, b, "f32dev")
+    eng.copy_bandwidth(256 << 20, 3)
+    eng.close()
+    print("probe: copies done", flush=True)
+    for store in ("f32dev", "f64", "f16dev"):
+        eng = engine_for(name, b, store)
+        eng.run(0, 20)
+        eng.sync()
+        print(f"probe: {store} 20 steps, {eng.n_interior} fluid nodes", flush=True)
+        eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,10 +177,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="vessel_pair_15", choices=sorted(WORKLOADS))
     ap.add_argument("--store", default="f32dev", choices=sorted(STORE_NAMES))
+    ap.add_argument("--flags", type=int, default=0, help="zf_config.flags (ZF_FLAG_*)")
     ap.add_argument("--err-steps", type=int, default=10000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 form and error legs")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter-calibration run for rocprofv3 --pmc: three 256 MiB copies, "
+                         "then 20 steps of each storage form; prints no JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -158,13 +193,21 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    global FLAGS
+    FLAGS = args.flags
     b = make_geometry(args.workload)
+    if args.probe:
+        probe(args.workload, b)
+        return
     n_fluid = int(((b > 0) & (b < 10)).sum())
     eng, ms, wall = time_form(args.workload, b, args.store, args.steps, args.warmup)
     n_int = eng.n_interior
     assert n_int == n_fluid
     mlups = n_int * args.steps / (ms * 1e-3) / 1e6
-    roof = kernel_roofline(eng, args.store, n_int, min(args.steps, 500))
+    roof = kernel_roofline(eng, args.store, n_int, ms, args.steps)
+    # measured device-copy ceiling (2 GiB each way: far beyond the 256 MB
+    # last-level cache, which inflates copies of one launch's size)
+    roof["copy_ceiling"] = round(eng.copy_bandwidth(2 << 30, 20), 1)
     info = eng.info()
     eng.close()
 
@@ -172,7 +215,7 @@ def main():
     if not args.no_extra:
         e64, ms64, _ = time_form(args.workload, b, "f64", args.steps, args.warmup)
         mlups64 = n_int * args.steps / (ms64 * 1e-3) / 1e6
-        roof64 = kernel_roofline(e64, "f64", n_int, min(args.steps, 500))
+        roof64 = kernel_roofline(e64, "f64", n_int, ms64, args.steps)
         e64.close()
         extra["fp64_form"] = dict(value=round(mlups64, 1), unit="MLUPS",
                                   ms_per_step=round(ms64 / args.steps, 5),
@@ -199,7 +242,7 @@ def main():
                    "fluid_fraction": round(n_int / b.size, 4), "tau": 0.8,
                    "wall": "halfway bounce-back",
                    "inlet": "u(t)=0.03(1+0.5 sin 2 pi t/4000)" if "pair" in args.workload else None,
-                   "chords": info["n_chords"]},
+                   "chords": info["n_chords"], "flags": args.flags},
         "wall_clock_s": round(wall, 4),
         "roofline": roof,
     }

@@ -51,6 +51,12 @@ enum {
   ZF_WALL_HALFWAY_BB = 1   /* halfway bounce-back on links from B==0 nodes */
 };
 
+/* Kernel options (zf_config.flags); results do not depend on them except
+ * ZF_FLAG_F64_ARITH, which computes the FP32/FP16 storage forms in FP64. */
+#define ZF_FLAG_LINEAR_BLOCKS 1u    /* plain block order instead of one range per XCD */
+#define ZF_FLAG_NT_STORES 2u        /* non-temporal population stores */
+#define ZF_FLAG_F64_ARITH 4u        /* FP64 arithmetic on FP32/FP16 storage */
+
 /* Boundary codes in b_index beyond the reference's (0 wall, 1..9 fluid with
  * tau = 0.5 + visc_scale*B, 1001..1009 pressure waveforms). */
 #define ZF_B_VELOCITY_INLET 2001 /* feq(D[nbr], u_in(t)*inlet_dir) */
@@ -62,7 +68,7 @@ typedef struct zf_config {
   int32_t storage;        /* ZF_STORE_* */
   int32_t wall_mode;      /* ZF_WALL_* */
   int32_t device;         /* HIP device ordinal */
-  int32_t reserved;
+  int32_t flags;          /* ZF_FLAG_* kernel options */
   double visc_scale;      /* tau = 0.5 + visc_scale*B; reference MU_SCALE*MU = 0.72 */
   double force_dt;        /* DT in the Guo force terms; reference 0.5 */
   double body_force[3];   /* uniform force added to the per-node field */
@@ -263,6 +269,9 @@ int zf_download_wall_neighbour(zf_ctx *ctx, int32_t *nearest_wall);
 int zf_step(zf_ctx *ctx, int32_t iter);
 
 /* ---- timing and synchronisation ------------------------------------------ */
+/* Device-copy ceiling: a plain 16-byte-per-lane copy of `bytes` between two
+ * fresh buffers, `reps` times; *gbps = 2*bytes*reps / time (read + write). */
+int zf_copy_bandwidth(zf_ctx *ctx, int64_t bytes, int32_t reps, double *gbps);
 int zf_sync(zf_ctx *ctx);
 /* Record HIP event `slot` (0..15) on the context stream. */
 int zf_event_record(zf_ctx *ctx, int32_t slot);

@@ -11,7 +11,7 @@ class ZfConfig(C.Structure):
     _fields_ = [
         ("im", i32), ("jm", i32), ("km", i32),
         ("periodic", i32 * 3),
-        ("storage", i32), ("wall_mode", i32), ("device", i32), ("reserved", i32),
+        ("storage", i32), ("wall_mode", i32), ("device", i32), ("flags", i32),
         ("visc_scale", f64), ("force_dt", f64), ("body_force", f64 * 3),
         ("inlet_u0", f64), ("inlet_amp", f64), ("inlet_period", f64), ("inlet_dir", f64 * 3),
         ("outlet_rho", f64), ("t_scale", f64), ("pressure_scale", f64),
@@ -93,6 +93,7 @@ FUNCS = {
     "zf_download_cell_list": (C.c_int, [P, P]),
     "zf_repulsion": (C.c_int, [P]),
     "zf_download_wall_neighbour": (C.c_int, [P, P]),
+    "zf_copy_bandwidth": (C.c_int, [P, i64, i32, C.POINTER(f64)]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),
     "zf_event_elapsed": (C.c_int, [P, i32, i32, C.POINTER(C.c_float)]),

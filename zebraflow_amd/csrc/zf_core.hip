@@ -120,6 +120,46 @@ int zf_sync(zf_ctx *ctx) {
   return ZF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+__global__ void k_copy16(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+}  // namespace
+
+extern "C" {
+
+int zf_copy_bandwidth(zf_ctx *ctx, int64_t bytes, int32_t reps, double *gbps) {
+  if (!ctx || !gbps || bytes < 16 || reps < 1) return ZF_ERR_ARG;
+  void *a = nullptr, *b = nullptr;
+  int rc = dev_alloc(ctx, &a, (size_t)bytes);
+  if (rc) return rc;
+  if ((rc = dev_alloc(ctx, &b, (size_t)bytes))) {
+    dev_free(ctx, &a);
+    return rc;
+  }
+  const size_t n = (size_t)bytes / 16;
+  hipError_t e = hipMemsetAsync(a, 0, (size_t)bytes, ctx->stream);
+  // warm-up, then the timed copies
+  hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, ctx->stream, (const float4 *)a, (float4 *)b, n);
+  if (e == hipSuccess) e = hipEventRecord(ctx->events[14], ctx->stream);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, ctx->stream, (const float4 *)a, (float4 *)b, n);
+  if (e == hipSuccess) e = hipEventRecord(ctx->events[15], ctx->stream);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipEventSynchronize(ctx->events[15]);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->events[14], ctx->events[15]);
+  ctx->device_bytes -= 2 * bytes;
+  dev_free(ctx, &a);
+  dev_free(ctx, &b);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "copy bandwidth: %s", hipGetErrorString(e));
+  *gbps = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+  return ZF_OK;
+}
+
 int zf_event_record(zf_ctx *ctx, int32_t slot) {
   if (!ctx || slot < 0 || slot >= 16) return ZF_ERR_ARG;
   ZF_HIP(ctx, hipEventRecord(ctx->events[slot], ctx->stream));

@@ -29,7 +29,7 @@ constexpr double kW[Q] = {
 
 // Node word of the stream-collide kernel (one per interior node):
 //   bits  0..17  halfway bounce-back mask (link c pulls f_{c^1} of the node)
-//   bit   18     write rho,u of this node for the boundary kernel
+//   bit   18     the node writes the boundary nodes that take it as neighbour
 //   bits 20..31  chord index relative to the first chord of its 64-node block
 constexpr uint32_t kWordMacro = 1u << 18;
 constexpr int kWordChordShift = 20;
@@ -78,13 +78,15 @@ struct zf_ctx {
   uint32_t *d_word = nullptr;   // n_int
   int32_t *d_block_chord0 = nullptr;
   int32_t *d_chords = nullptr;  // n_chords x kChordInts
-  zf::BcEntry *d_bc = nullptr;  // n_bc
+  zf::BcEntry *d_bc = nullptr;  // n_bc: fused entries (by neighbour) then late ones
+  int32_t n_bc_fused = 0, n_fused_nodes = 0;
+  int32_t *d_fused_s = nullptr;   // sorted storage indices of nodes with fused entries
+  int32_t *d_att_start = nullptr; // n_fused_nodes+1: their ranges in d_bc
   std::vector<int32_t> h_bnode;  // B value per list position
 
   // state
   void *d_f[2] = {nullptr, nullptr};  // population buffers (storage-typed)
   int cur = 0;                        // buffer holding the current state
-  void *d_macro = nullptr;            // 4 x n_int (rho,u,v,w), real-typed
   float *d_force = nullptr;           // 3 x stride, storage-indexed; null = none
   double *d_sums = nullptr;           // reduction scratch
   int64_t device_bytes = 0;

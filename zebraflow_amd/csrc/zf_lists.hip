@@ -211,7 +211,8 @@ void free_lattice(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_b,        (void **)&ctx->d_domid,  (void **)&ctx->d_domlist,
                    (void **)&ctx->d_bclist,   (void **)&ctx->d_bcnlist, (void **)&ctx->d_nbr,
                    (void **)&ctx->d_s_of_p,   (void **)&ctx->d_p_of_s, (void **)&ctx->d_word,
-                   (void **)&ctx->d_block_chord0, (void **)&ctx->d_chords, (void **)&ctx->d_bc};
+                   (void **)&ctx->d_block_chord0, (void **)&ctx->d_chords, (void **)&ctx->d_bc,
+                   (void **)&ctx->d_fused_s, (void **)&ctx->d_att_start};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_lists = false;
   ctx->have_lattice = false;
@@ -343,7 +344,29 @@ static int build_kernel_tables(zf_ctx *ctx, const int32_t *b, const std::vector<
     }
     bce.push_back(e);
   }
+  // Boundary entries whose neighbour is an interior node are written by that
+  // node's lane in the stream kernel, right after it computes rho and u
+  // (fused); they come first, grouped by neighbour.  The rest read their
+  // neighbour's moments from the previous state and run in extra workgroups
+  // of the same launch ("late" entries), in list order.
+  std::stable_sort(bce.begin(), bce.end(), [](const BcEntry &a, const BcEntry &b) {
+    if (a.m_int != b.m_int) return a.m_int > b.m_int;
+    return a.m_int ? a.sm < b.sm : false;
+  });
+  std::vector<int32_t> fused_s, att_start;
+  int n_t1 = 0;
+  for (const BcEntry &e : bce) {
+    if (!e.m_int) break;
+    if (fused_s.empty() || fused_s.back() != e.sm) {
+      fused_s.push_back(e.sm);
+      att_start.push_back(n_t1);
+    }
+    n_t1++;
+  }
+  att_start.push_back(n_t1);
   ctx->n_bc = (int)bce.size();
+  ctx->n_bc_fused = n_t1;
+  ctx->n_fused_nodes = (int)fused_s.size();
   ctx->n_macro = nmacro;
 
   // upload
@@ -369,6 +392,14 @@ static int build_kernel_tables(zf_ctx *ctx, const int32_t *b, const std::vector<
                           hipMemcpyHostToDevice));
   if (!bce.empty())
     ZF_HIP(ctx, hipMemcpy(ctx->d_bc, bce.data(), sizeof(BcEntry) * bce.size(), hipMemcpyHostToDevice));
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_fused_s, sizeof(int32_t) * std::max<size_t>(fused_s.size(), 1))))
+    return rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_att_start, sizeof(int32_t) * att_start.size()))) return rc;
+  if (!fused_s.empty())
+    ZF_HIP(ctx, hipMemcpy(ctx->d_fused_s, fused_s.data(), sizeof(int32_t) * fused_s.size(),
+                          hipMemcpyHostToDevice));
+  ZF_HIP(ctx, hipMemcpy(ctx->d_att_start, att_start.data(), sizeof(int32_t) * att_start.size(),
+                        hipMemcpyHostToDevice));
   return ZF_OK;
 }
 

@@ -272,6 +272,12 @@ class Engine:
                     "zf_download_wall_neighbour")
         return out
 
+    def copy_bandwidth(self, nbytes, reps=20):
+        g = C.c_double()
+        self._check(self.lib.zf_copy_bandwidth(self.h, int(nbytes), reps, C.byref(g)),
+                    "zf_copy_bandwidth")
+        return g.value
+
     def sync(self):
         self._check(self.lib.zf_sync(self.h), "zf_sync")
 
