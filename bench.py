@@ -61,7 +61,7 @@ FLAGS = 0  # zf_config.flags of the measured forms (--flags)
 # KiB per dispatch, averaged over the 20 launches of each form), each scaled by
 # what the counter reads for the probe's 256 MiB copy kernel: FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0 (profiles/stream_collide_traffic_vessel_pair_15.md).
-TRAFFIC = {}
+TRAFFIC = {"f32dev": 203603931, "f64": 399333437, "f16dev": 106048845}
 
 
 def engine_for(name, b, store):
@@ -95,7 +95,7 @@ def time_form(name, b, store, steps, warmup):
     return eng, ms, wall
 
 
-def kernel_roofline(eng, store, n_int, ms, steps):
+def kernel_roofline(eng, store, n_int, ms, steps, workload):
     """The timed region is `steps` back-to-back launches of the one fused
     stream-collide kernel on the engine's stream, so its HIP-event time over
     `steps` is the kernel's average launch duration (dispatch gaps included).
@@ -106,7 +106,8 @@ def kernel_roofline(eng, store, n_int, ms, steps):
     achieved = algo / (avg_ms * 1e-3) / 1e9
     t = eng.time_kernels(10 ** 6, min(steps, 500))
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get(store),
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=TRAFFIC.get(store) if workload == "vessel_pair_15" and FLAGS == 0 else None,
                 kernel="k_stream_collide", avg_us=round(avg_ms * 1e3, 3),
                 algorithmic_bytes_per_launch=algo,
                 per_launch_event_us=round(t[0] / t[2] * 1e3, 3),
@@ -204,7 +205,7 @@ def main():
     n_int = eng.n_interior
     assert n_int == n_fluid
     mlups = n_int * args.steps / (ms * 1e-3) / 1e6
-    roof = kernel_roofline(eng, args.store, n_int, ms, args.steps)
+    roof = kernel_roofline(eng, args.store, n_int, ms, args.steps, args.workload)
     # measured device-copy ceiling (2 GiB each way: far beyond the 256 MB
     # last-level cache, which inflates copies of one launch's size)
     roof["copy_ceiling"] = round(eng.copy_bandwidth(2 << 30, 20), 1)
@@ -215,7 +216,7 @@ def main():
     if not args.no_extra:
         e64, ms64, _ = time_form(args.workload, b, "f64", args.steps, args.warmup)
         mlups64 = n_int * args.steps / (ms64 * 1e-3) / 1e6
-        roof64 = kernel_roofline(e64, "f64", n_int, ms64, args.steps)
+        roof64 = kernel_roofline(e64, "f64", n_int, ms64, args.steps, args.workload)
         e64.close()
         extra["fp64_form"] = dict(value=round(mlups64, 1), unit="MLUPS",
                                   ms_per_step=round(ms64 / args.steps, 5),

@@ -44,7 +44,11 @@ template <>
 struct Store<ZF_STORE_F16DEV> {
   using T = _Float16;
   using R = float;
-  __device__ static R ld(const T *a, size_t i, double w) { return (float)a[i] * (float)w; }
+  // w applied as hi + lo floats (the float weights alone are biased high)
+  __device__ static R ld(const T *a, size_t i, double w) {
+    const float h = (float)a[i];
+    return __builtin_fmaf((float)w, h, (float)(w - (double)(float)w) * h);
+  }
   __device__ static T st(R g, double w) { return (T)(g * (float)(1.0 / w)); }
 };
 
@@ -63,15 +67,29 @@ struct StepParams {
   double g[3];        // uniform body force
 };
 
+// w_c * y.  In FP32 the weights 1/18, 1/36 and 1/3 round up, so that the
+// float weights sum to 1 + 2.6e-8: multiplying by them makes the equilibrium
+// a mass (and momentum) source proportional to drho on every step, a bias
+// that accumulates along the flow.  The float form therefore multiplies by
+// the weight split in two floats, w = hi + lo, which rounds w * y correctly
+// up to 2^-48 and leaves only unbiased round-to-nearest errors.
+template <class R>
+__device__ __forceinline__ R wmul(int c, R y) { return (R)WQ[c] * y; }
+template <>
+__device__ __forceinline__ float wmul<float>(int c, float y) {
+  const float hi = (float)WQ[c];
+  const float lo = (float)(WQ[c] - (double)hi);
+  return __builtin_fmaf(hi, y, lo * y);
+}
+
 // Equilibrium deviation feq_c - w_c for rho = 1 + drho (ZebraflowWT.c:6139),
 // w (drho + X + drho X) with X = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2: the sum 1 +
-// drho is never formed, because rounding it in FP32 (ulp 1.2e-7) is a
-// systematic error far above that of the deviations themselves.
+// drho is never formed, so the deviations keep their own relative precision.
 template <class R>
 __device__ __forceinline__ R geq(int c, R drho, R ux, R uy, R uz, R usq) {
   const R cu = (R)EX[c] * ux + (R)EY[c] * uy + (R)EZ[c] * uz;
   const R x = (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq;
-  return (R)WQ[c] * (drho + x + drho * x);
+  return wmul<R>(c, drho + x + drho * x);
 }
 
 // ---------------------------------------------------------------------------
@@ -195,8 +213,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     if (EZ[c]) jz += EZ[c] > 0 ? g[c] : -g[c];
   }
   // half-force velocity shift, ZebraflowWT.c:6132-6135: u = v / (1 + drho)
-  // taken as v - v q with q = drho / (1 + drho), so the rounding of 1 + drho
-  // only perturbs the small correction v q
+  // taken as v - v q with q = drho / (1 + drho)
   const CT q = drho / ((CT)1 + drho);
   const CT vx = jx + (CT)0.5 * dt * fx;
   const CT vy = jy + (CT)0.5 * dt * fy;
