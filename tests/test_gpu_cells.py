@@ -1,0 +1,148 @@
+"""GPU cell list, cell-cell and cell-wall repulsion (a9, a14, a15) and the
+whole RBC iteration of zf_step against the oracle, through the C ABI.
+
+Cell lists and the W_index map are integers and must match exactly;
+repulsion velocities are FP64 on both sides and agree to 1e-12 of their
+largest value.  The composed step reads FP32 macro velocities and FP32 body
+forces on the GPU, so positions agree to 1e-8 lu and the spread force to
+2e-6 of its largest value.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import geometry
+from tests.test_gpu_ibm import ibm_config, macro_of
+from tests.test_oracle_cells import membrane_geometry, two_cells, wall_case
+from tests.test_oracle_mesh import lattice_params
+from zebraflow_amd.engine import STORE_F64, Engine, lattice_config, load_library, membrane_config
+from zebraflow_amd._abi import ZfRepulsionConfig
+
+pytestmark = pytest.mark.gpu
+
+
+def repulsion_config():
+    rc = ZfRepulsionConfig()
+    load_library().zf_repulsion_config_reference(C.byref(rc))
+    r = po.rep_params()
+    for name, _ in ZfRepulsionConfig._fields_:
+        setattr(rc, name, getattr(r, name))
+    return rc
+
+
+def engine(b, cells, tris, xyz_rest, periodic=(1, 0, 0), wall=None, store=STORE_F64):
+    cfg = lattice_config(*b.shape, 0.8, storage=store, periodic=periodic)
+    mp, edge, eq, tae = lattice_params(xyz_rest, tris)
+    eng = Engine(cfg)
+    eng.upload_lattice(b)
+    eng.membrane_setup(membrane_config(dict(mp, es=5e-3), len(cells)), tris, xyz_rest)
+    eng.upload_cells(cells)
+    eng.ibm_setup(ibm_config())
+    eng.repulsion_setup(repulsion_config())
+    if wall is not None:
+        eng.wall_setup(*wall)
+    return eng
+
+
+def test_cell_list_and_cell_repulsion_match_oracle():
+    print("case cell-cell, two cells 0.3 apart", flush=True)
+    b, cells, tris = two_cells(0.3, seed=2)
+    xyz_rest = geometry.rbc_mesh(radius=3.0, level=2)[0]
+    eng = engine(b, cells, tris, xyz_rest)
+    eng.membrane_forces()
+    bins = eng.cell_list()
+    eng.repulse()
+    v = eng.repulsion()
+    g = membrane_geometry(cells, tris, xyz_rest)
+    p = po.ibm_params(b.shape)
+    r = po.rep_params()
+    domid, _ = po.dom_setup(b)
+    status = np.ones(len(cells), np.int32)
+    ref_bins = po.cell_list(p, b, domid, g["tri_centroid"], status, r.bin_cutoff)
+    np.testing.assert_array_equal(bins, ref_bins)
+    ref = po.cell_repulsion(p, r, b, domid, tris, status, cells, g["node_normal"],
+                            g["tri_centroid"], g["tri_normal"], ref_bins)
+    scale = np.abs(ref).max()
+    assert scale > 0
+    np.testing.assert_allclose(v, ref, rtol=0, atol=1e-12 * scale)
+    eng.close()
+
+
+@pytest.mark.parametrize("offset", [0.1, -0.3])
+def test_windex_and_wall_repulsion_match_oracle(offset):
+    print(f"case cell-wall, offset {offset}", flush=True)
+    b, wall, cells, tris = wall_case(offset)
+    xyz_rest = geometry.rbc_mesh(radius=3.0, level=2)[0]
+    eng = engine(b, cells, tris, xyz_rest, wall=wall)
+    p = po.ibm_params(b.shape)
+    r = po.rep_params()
+    nwt, windex = po.wall_tables(p, wall[0], wall[1])
+    np.testing.assert_array_equal(eng.windex(), windex)
+    eng.membrane_forces()
+    eng.cell_list()
+    eng.repulse()
+    v = eng.repulsion()
+    near = eng.wall_neighbour()
+    status = np.ones(1, np.int32)
+    vw, near_ref = po.wall_repulsion(p, r, windex, *wall, nwt, status, cells)
+    g = membrane_geometry(cells, tris, xyz_rest)
+    domid, _ = po.dom_setup(b)
+    bins = po.cell_list(p, b, domid, g["tri_centroid"], status, r.bin_cutoff)
+    vc = po.cell_repulsion(p, r, b, domid, tris, status, cells, g["node_normal"],
+                           g["tri_centroid"], g["tri_normal"], bins)
+    np.testing.assert_array_equal(near, near_ref)
+    ref = vw + vc
+    scale = np.abs(ref).max()
+    assert np.abs(vw).max() > 0
+    np.testing.assert_allclose(v, ref, rtol=0, atol=1e-12 * scale)
+    eng.close()
+
+
+def test_full_step_matches_composed_oracle():
+    """Three RBC iterations of zf_step (LBM step, membrane forces, cell list,
+    interpolation, repulsion, spreading, position update) against the same
+    sequence of oracle calls."""
+    print("case zf_step, 3 cells, 3 iterations", flush=True)
+    b = geometry.tube((48, 30, 30), axis=0, r=12.0, periodic=True)
+    xyz, tris = geometry.rbc_mesh(radius=3.0)
+    cells = geometry.place_cells(b, xyz, 3.0, 0.6, 3, 11)
+    assert len(cells) == 3
+    wall = geometry.wall_mesh(b)
+    eng = engine(b, cells, tris, xyz, wall=wall)
+    rng = np.random.default_rng(4)
+    domid, dl = po.dom_setup(b)
+    n = len(dl)
+    f0 = po.feq(1 + 0.001 * rng.standard_normal(n), 0.01 * rng.standard_normal((3, n)))
+    eng.upload_populations(f0)
+    o = po.LbmOracle(b, dict(wall_mode=1, visc_scale=0.3, force_dt=1.0), periodic=(1, 0, 0))
+    o.set_populations(f0)
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    p = po.ibm_params(b.shape)
+    r = po.rep_params()
+    nwt, windex = po.wall_tables(p, wall[0], wall[1])
+    x = cells.copy()
+    status = np.ones(len(cells), np.int32)
+    for it in range(1, 4):
+        eng.full_step(it)
+        o.step(it, rbc_step=1)
+        g = po.membrane_forces(x, tris, edge, eq, tae, mp)
+        bins = po.cell_list(p, b, domid, g["tri_centroid"], status, r.bin_cutoff)
+        v = po.ibm_interpolate(p, b, domid, macro_of(o.populations), x)
+        vrep = po.cell_repulsion(p, r, b, domid, tris, status, x, g["node_normal"],
+                                 g["tri_centroid"], g["tri_normal"], bins)
+        vrep += po.wall_repulsion(p, r, windex, *wall, nwt, status, x)[0]
+        po.ibm_spread(p, b, domid, n, x, 0.5 * vrep + g["force"], force=o.force)
+        x, status, removed = po.advance(p, x, v + vrep, status)
+        assert removed == 0
+    xg, _, st = eng.cells()
+    assert (st > 0).all()
+    assert np.abs(x - cells).max() > 1e-4  # the cells moved
+    np.testing.assert_allclose(xg, x, rtol=0, atol=1e-8)
+    fg = eng.force()
+    scale = np.abs(o.force).max()
+    assert scale > 0
+    np.testing.assert_allclose(fg, o.force, rtol=0, atol=2e-6 * scale)
+    np.testing.assert_allclose(eng.populations(), o.populations, rtol=0, atol=1e-9)
+    eng.close()

@@ -1,0 +1,265 @@
+"""Oracle cell list, cell-cell and cell-wall repulsion (a9, a14, a15) pinned
+by brute force over all candidates on small inputs, written independently of
+the oracle's loops (vectorised, barycentric sub-centroids, whole-box
+searches), and by the sign of the physics: repulsion pushes a membrane away
+from the neighbouring cell and from the wall.
+
+ZebraflowWT.c:5851-5900 (cell list), :9913-10052 (cell-cell), :8998-9142 and
+:3715-3826, :3900-3935 (cell-wall, W_index, wall node -> triangle lists).
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import geometry
+from tests.test_oracle_mesh import lattice_params
+
+
+def node_xyz(shape):
+    """Lattice-node positions (x0 = 0, dx = 1): node (i,j,k) at (i-1,j-1,k-1)."""
+    g = np.indices(shape).reshape(3, -1).T.astype(np.float64)
+    return g
+
+
+def two_cells(gap, seed=0, level=2, shape=(40, 26, 26)):
+    """A tube and two small cells facing each other along the axis, `gap`
+    lattice units apart at their closest nodes."""
+    b = geometry.tube(shape, axis=0, r=11.0, periodic=True)
+    xyz, tris = geometry.rbc_mesh(radius=3.0, level=level)
+    rng = np.random.default_rng(seed)
+    c = np.array(shape, float) / 2 - 0.5 + rng.uniform(-0.3, 0.3, 3)
+    a = xyz + c
+    # second cell: the mesh rotated a little, placed so its min x is `gap`
+    # beyond the first cell's max x
+    th = rng.uniform(0, 2 * np.pi)
+    rot = np.array([[1, 0, 0], [0, np.cos(th), -np.sin(th)], [0, np.sin(th), np.cos(th)]])
+    bb = xyz @ rot.T + c
+    bb[:, 0] += a[:, 0].max() - bb[:, 0].min() + gap
+    return b, np.stack([a, bb]), tris
+
+
+def brute_cell_list(b, tri_centroid, status, cutoff):
+    pos = node_xyz(b.shape)
+    ok = b.reshape(-1) >= 0
+    domid, _ = po.dom_setup(b)
+    dflat = domid.reshape(-1)
+    out = np.zeros(tri_centroid.shape[:2], np.int32)
+    for c in range(tri_centroid.shape[0]):
+        if status[c] < 0:
+            continue
+        for t in range(tri_centroid.shape[1]):
+            d = np.sqrt(((pos - tri_centroid[c, t]) ** 2).sum(1))
+            d[~ok] = np.inf
+            q = int(np.argmin(d))  # first minimum in (i, j, k) order
+            if d[q] < cutoff:
+                out[c, t] = dflat[q]
+    return out
+
+
+# barycentric coordinates of the 16 sub-triangle centroids of a triangle
+# split four ways along each edge: 10 upward (a+1/3, b+1/3, c+1/3)/4 with
+# a+b+c = 3 and 6 downward (a+2/3, b+2/3, c+2/3)/4 with a+b+c = 2
+BARY16 = np.array([[a + 1 / 3, b + 1 / 3, 3 - a - b + 1 / 3] for a in range(4) for b in range(4 - a)]
+                  + [[a + 2 / 3, b + 2 / 3, 2 - a - b + 2 / 3] for a in range(3) for b in range(3 - a)]
+                  ) / 4.0
+
+
+def brute_cell_repulsion(b, xyz, tris, status, node_normal, tri_centroid, tri_normal, bins, r):
+    nc, nn, _ = xyz.shape
+    domid, dl = po.dom_setup(b)
+    pos_of = {int(domid[tuple(x - 1)]): x - 1 for x in dl}  # 1-based DOMID -> node index
+    v = np.zeros_like(xyz)
+    cand = [(m, t) for m in range(nc) for t in range(len(tris)) if bins[m, t] > 0]
+    binpos = np.array([pos_of[int(bins[m, t])] for m, t in cand])
+    for c in range(nc):
+        if status[c] <= 0:
+            continue
+        for node in range(nn):
+            x = xyz[c, node]
+            ijk = (x + 1.01).astype(int) - 1  # 0-based window centre
+            for (m, t), bp in zip(cand, binpos):
+                if np.any(np.abs(bp - ijk) > 4):
+                    continue
+                if m == c and node + 1 in tris[t]:
+                    continue
+                if node_normal[c, node] @ tri_normal[m, t] >= 0:
+                    continue
+                if np.linalg.norm(x - tri_centroid[m, t]) >= r.search:
+                    continue
+                p = xyz[m, tris[t] - 1]
+                sc = BARY16 @ p
+                a = sc - x
+                d = np.linalg.norm(a, axis=1)
+                near = d < r.cutoff
+                f = -r.k_cell * (r.cutoff - d[near]) / r.cutoff
+                v[c, node] += (f[:, None] * a[near] / d[near, None]).sum(0)
+    return v
+
+
+def membrane_geometry(cells, tris, xyz_rest):
+    mp, edge, eq, tae = lattice_params(xyz_rest, tris)
+    return po.membrane_forces(cells, tris, edge, eq, tae, mp)
+
+
+def test_sub_centroids_partition_the_triangle():
+    # the 16 centroids average to the triangle centroid and are distinct
+    assert np.allclose(BARY16.sum(1), 1.0)
+    assert np.allclose(BARY16.mean(0), 1 / 3)
+    assert len(np.unique(np.round(BARY16, 12), axis=0)) == 16
+
+
+@pytest.mark.parametrize("gap", [0.4, 2.5])
+def test_cell_list_matches_brute_force(gap):
+    b, cells, tris = two_cells(gap, seed=1)
+    g = membrane_geometry(cells, tris, geometry.rbc_mesh(radius=3.0, level=2)[0])
+    p = po.ibm_params(b.shape)
+    domid, _ = po.dom_setup(b)
+    status = np.ones(2, np.int32)
+    bins = po.cell_list(p, b, domid, g["tri_centroid"], status, 1.0)
+    ref = brute_cell_list(b, g["tri_centroid"], status, 1.0)
+    np.testing.assert_array_equal(bins, ref)
+    assert (bins > 0).mean() > 0.9  # cells in the bulk: nearly every centroid is binned
+    # a parked cell (status < 0) is not binned
+    bins2 = po.cell_list(p, b, domid, g["tri_centroid"], np.array([1, -5000], np.int32), 1.0)
+    assert (bins2[1] == 0).all() and (bins2[0] == bins[0]).all()
+
+
+def test_cell_repulsion_matches_brute_force_and_pushes_apart():
+    b, cells, tris = two_cells(0.3, seed=2)
+    g = membrane_geometry(cells, tris, geometry.rbc_mesh(radius=3.0, level=2)[0])
+    p = po.ibm_params(b.shape)
+    r = po.rep_params()
+    domid, _ = po.dom_setup(b)
+    status = np.ones(2, np.int32)
+    bins = po.cell_list(p, b, domid, g["tri_centroid"], status, r.bin_cutoff)
+    v = po.cell_repulsion(p, r, b, domid, tris, status, cells, g["node_normal"], g["tri_centroid"],
+                          g["tri_normal"], bins)
+    ref = brute_cell_repulsion(b, cells, tris, status, g["node_normal"], g["tri_centroid"],
+                               g["tri_normal"], bins, r)
+    scale = np.abs(ref).max()
+    assert scale > 0
+    np.testing.assert_allclose(v, ref, rtol=0, atol=1e-12 * scale)
+    # the two cells are pushed apart along the axis joining them
+    axis = cells[1].mean(0) - cells[0].mean(0)
+    assert v[0].sum(0) @ axis < 0 and v[1].sum(0) @ axis > 0
+    # far apart, each cell feels only its own non-adjacent facing triangles
+    # (the biconcave dimple): exactly what it feels alone
+    b2, cells2, tris2 = two_cells(3.0, seed=2)
+    g2 = membrane_geometry(cells2, tris2, geometry.rbc_mesh(radius=3.0, level=2)[0])
+    bins2 = po.cell_list(p, b2, domid, g2["tri_centroid"], status, r.bin_cutoff)
+    v2 = po.cell_repulsion(p, r, b2, domid, tris2, status, cells2, g2["node_normal"],
+                           g2["tri_centroid"], g2["tri_normal"], bins2)
+    one = np.ones(1, np.int32)
+    for c in range(2):
+        gc = {k: g2[k][c:c + 1] for k in ("node_normal", "tri_centroid", "tri_normal")}
+        bc = po.cell_list(p, b2, domid, gc["tri_centroid"], one, r.bin_cutoff)
+        vc = po.cell_repulsion(p, r, b2, domid, tris2, one, cells2[c:c + 1], gc["node_normal"],
+                               gc["tri_centroid"], gc["tri_normal"], bc)
+        np.testing.assert_array_equal(v2[c], vc[0])
+    # and in contact the difference from the far case is confined to nodes
+    # within reach of the other cell
+    v_self = np.stack([po.cell_repulsion(
+        p, r, b, domid, tris, one, cells[c:c + 1], g["node_normal"][c:c + 1],
+        g["tri_centroid"][c:c + 1], g["tri_normal"][c:c + 1],
+        po.cell_list(p, b, domid, g["tri_centroid"][c:c + 1], one, r.bin_cutoff))[0]
+        for c in range(2)])
+    moved = np.abs(v - v_self).sum(2) > 0
+    reach = np.array([[np.min(np.linalg.norm(cells[1 - c] - x, axis=1)) for x in cells[c]]
+                      for c in range(2)])
+    assert moved.any() and (reach[moved] < r.cutoff + 1.0).all()
+
+
+def wall_case(offset, seed=3):
+    """A tube with its wall mesh and one cell pushed towards the wall so that
+    its outermost node lies `offset` inside the wall mesh."""
+    shape = (24, 22, 22)
+    b = geometry.tube(shape, axis=0, r=8.0, periodic=True)
+    wx, wt, wn = geometry.wall_mesh(b)
+    xyz, tris = geometry.rbc_mesh(radius=3.0, level=2)
+    rng = np.random.default_rng(seed)
+    c = np.array([11.5, 10.5, 10.5]) + rng.uniform(-0.2, 0.2, 3)
+    cell = xyz + c
+    # move along +y until the top node is `offset` from the wall mesh's
+    # innermost vertex in that direction
+    top = cell[:, 1].max()
+    wall_y = wx[(np.abs(wx[:, 0] - c[0]) < 1.5) & (np.abs(wx[:, 2] - c[2]) < 1.5), 1].max()
+    cell[:, 1] += wall_y - top - offset
+    return b, (wx, wt, wn), cell[None], tris
+
+
+def brute_windex(shape, wx):
+    pos = node_xyz(shape)
+    out = np.zeros(len(pos), np.int32)
+    best = np.full(len(pos), np.float32(5000.0))
+    ijk = (wx + 1.01).astype(int) - 1
+    for l in range(len(wx)):
+        inwin = np.all(np.abs(pos - ijk[l]) <= 4, axis=1)
+        d = np.sqrt(((pos[inwin] - wx[l]) ** 2).sum(1)).astype(np.float32)
+        idx = np.nonzero(inwin)[0]
+        better = d < best[idx]
+        best[idx[better]] = d[better]
+        out[idx[better]] = l + 1
+    return out.reshape(shape)
+
+
+def test_wall_tables_match_brute_force():
+    b, (wx, wt, wn), _, _ = wall_case(0.2)
+    p = po.ibm_params(b.shape)
+    nwt, windex = po.wall_tables(p, wx, wt)
+    # node -> triangle rows
+    for l in range(len(wx)):
+        ts = list(np.nonzero((wt == l + 1).any(1))[0] + 1)
+        assert nwt[l, 0] == len(ts)
+        assert list(nwt[l, 1:1 + min(len(ts), 12)]) == ts[:12]
+    np.testing.assert_array_equal(windex, brute_windex(b.shape, wx))
+    # every fluid node within 4 of the wall has a wall node
+    assert (windex[b > 0] > 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("offset,pushes", [(0.1, True), (-0.3, True), (0.6, False)])
+def test_wall_repulsion_pushes_into_the_fluid(offset, pushes):
+    b, (wx, wt, wn), cells, tris = wall_case(offset)
+    p = po.ibm_params(b.shape)
+    r = po.rep_params()
+    nwt, windex = po.wall_tables(p, wx, wt)
+    status = np.ones(1, np.int32)
+    v, near = po.wall_repulsion(p, r, windex, wx, wt, wn, nwt, status, cells)
+    # brute force over every wall triangle reachable through W_index
+    vb = np.zeros_like(v)
+    top = cells[0][:, 1].max()
+    sel = (near[0] > 0) | (cells[0][:, 1] > top - 2.5)
+    assert (near[0, ~sel] == 0).all()
+    for n_, x in enumerate(cells[0]):
+        if not sel[n_]:
+            continue
+        ijk = (x + 1.01).astype(int)
+        lo, hi = np.maximum(ijk - 4, 1), np.minimum(ijk + 4, b.shape)
+        best, tw = r.wall_search, 0
+        for i in range(lo[0], hi[0] + 1):
+            for j in range(lo[1], hi[1] + 1):
+                for k in range(lo[2], hi[2] + 1):
+                    wnode = windex[i - 1, j - 1, k - 1]
+                    if wnode <= 0:
+                        continue
+                    for t in nwt[wnode - 1, 1:1 + min(nwt[wnode - 1, 0], 12)]:
+                        pts = wx[wt[t - 1] - 1]
+                        for d in list(np.linalg.norm(pts - x, axis=1)) + [
+                                np.linalg.norm(x - (pts[0] / 3 + pts[1] / 3 + pts[2] / 3))]:
+                            if d < best:
+                                best, tw = d, t
+        assert near[0, n_] == tw
+        if tw:
+            n = wn[tw - 1]
+            d = (x - wx[wt[tw - 1, 0] - 1]) @ n
+            if d < r.wall_d0 and abs(d) < r.wall_dmax:
+                f = 1.0 if d < r.wall_d0 - r.wall_dmax else (r.wall_d0 - d) / r.wall_dmax
+                vb[0, n_] = f * r.k_wall * n
+    np.testing.assert_allclose(v, vb, rtol=0, atol=1e-18)
+    pushed = np.abs(v[0]).sum(1) > 0
+    assert pushed.any() == pushes
+    if not pushes:
+        return
+    # the push points back into the tube (towards -y at the +y wall)
+    assert (v[0, pushed, 1] < 0).mean() > 0.9
+    # nodes far from the wall are not pushed
+    assert not pushed[cells[0][:, 1] < cells[0][:, 1].max() - 2.0].any()
