@@ -141,8 +141,8 @@ int zf_conservation_sums(zf_ctx *ctx, double out[4]);
 
 /* Per-kernel timing pass: runs nsteps iterations iter0+1.. like zf_lbm_run,
  * bracketing every launch with HIP events on the context stream.
- * out_ms[0] = total ms in the stream-collide kernel, out_ms[1] = total ms in
- * the boundary kernel, out_ms[2] = launches of the stream-collide kernel. */
+ * out_ms[0] = total ms in the stream-collide kernel, out_ms[1] = 0 (the
+ * boundary update runs inside that kernel), out_ms[2] = launches timed. */
 int zf_lbm_time_kernels(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double out_ms[3]);
 
 /* ---- RBC membranes ---------------------------------------------------------

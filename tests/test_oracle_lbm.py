@@ -111,3 +111,25 @@ def test_plane_poiseuille_profile():
     ux = ((po.E[:, 0] @ o.populations[:, fluid]) + 0.5 * g) / rho
     exact = g / (2 * nu) * y * (H - y)
     np.testing.assert_allclose(ux, exact, rtol=2e-3, atol=1e-3 * exact.max())
+
+
+@pytest.mark.parametrize("code", range(1001, 1010))
+def test_cardiac_pressure_waveforms_are_continuous_and_periodic(code):
+    """ZebraflowWT.c:6318-6421: each waveform is a rise (sin^2 to 23 % of the
+    0.36 s cycle, or sin to 30 %) joined to a cosine fall; the pieces meet at
+    the peak and at the cycle end, so the pressure is continuous, periodic
+    and spans [120 + base, 120 + base + amp] mmHg-scaled units.  A wrong rise
+    fraction or fall phase breaks one of the joins."""
+    T = 0.36
+    t = np.linspace(0.0, 3 * T, 200001)
+    p = np.array([po.lib().zfo_bc_pressure(code, x) for x in t])
+    dp = np.abs(np.diff(p))
+    assert dp.max() < 1e-2 * (p.max() - p.min())  # no jumps at 1.6e-6 s sampling
+    np.testing.assert_allclose(
+        [po.lib().zfo_bc_pressure(code, x + T) for x in t[:2000:97]],
+        p[:2000:97], rtol=0, atol=1e-9)
+    lo, hi = p.min(), p.max()
+    assert lo >= 120.0 and hi > lo
+    # the minimum is reached at the cycle boundary and the peak once per cycle
+    peaks = np.nonzero((p[1:-1] > p[:-2]) & (p[1:-1] >= p[2:]))[0]
+    assert len(peaks) == 3
