@@ -156,6 +156,47 @@ def cpu_baseline(name, b, seconds):
                        f"schedule(static,5000)), {el:.1f} s")
 
 
+def cells_leg(name, b, ncell, steps, warmup):
+    """Dense-cell load on the same lattice: `ncell` Evans-Fung cells (642
+    nodes, radius 4 lu) placed at random without overlap, membrane shear
+    modulus 5e-3 and bending modulus 2e-3 lu, cell-cell (cutoff 1 lu, 2-lu
+    cell list) and cell-wall repulsion against the lumen mesh, membrane/IBM
+    work on every step.  Times whole zf_step iterations."""
+    from tests import geometry as g  # the product's C generators (libzfhost.so)
+    from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
+                                      lattice_repulsion_config)
+    xyz, tris = g.rbc_mesh(radius=4.0)
+    cells = g.place_cells_dense(b, xyz, 0.6, ncell, 20261206)
+    wall = g.wall_mesh(b)
+    eng = engine_for(name, b, "f32dev")
+    eng.membrane_setup(lattice_membrane_config(xyz, tris, len(cells)), tris, xyz)
+    eng.upload_cells(cells)
+    eng.ibm_setup(lattice_ibm_config(rbc_interval=1))
+    eng.repulsion_setup(lattice_repulsion_config())
+    eng.wall_setup(*wall)
+    for it in range(1, warmup + 1):
+        eng.full_step(it)
+    eng.sync()
+    eng.record(0)
+    for it in range(warmup + 1, warmup + steps + 1):
+        eng.full_step(it)
+    eng.record(1)
+    eng.sync()
+    ms = eng.elapsed_ms(0, 1)
+    _, _, st = eng.cells()
+    nn = len(xyz)
+    p = xyz[tris - 1]
+    vol = (0.5 * np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]) * p.mean(1)).sum() / 3
+    n_int = eng.n_interior
+    eng.close()
+    return dict(cells=len(cells), membrane_nodes=len(cells) * nn,
+                haematocrit=round(len(cells) * vol / n_int, 4), steps=steps, warmup=warmup,
+                ms_per_step=round(ms / steps, 4),
+                rbc_node_updates_per_s=round(len(cells) * nn * steps / (ms * 1e-3), 1),
+                fluid_mlups=round(n_int * steps / (ms * 1e-3) / 1e6, 1),
+                cells_live_after=int((st > 0).sum()), rbc_interval=1)
+
+
 def probe(name, b):
     """A copy of known size (256 MiB read + 256 MiB written per launch, three
     launches) to calibrate the HBM counters, then 20 steps of each form."""
@@ -183,6 +224,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 form and error legs")
+    ap.add_argument("--cells", type=int, default=1200,
+                    help="cells of the dense-cell leg (0 skips it)")
+    ap.add_argument("--cell-steps", type=int, default=200)
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: three 256 MiB copies, "
                          "then 20 steps of each storage form; prints no JSON line")
@@ -224,6 +268,9 @@ def main():
         extra["speedup_vs_fp64_form"] = round(mlups / mlups64, 3)
         extra["error"] = dict(vel_rel_l2_vs_fp64=velocity_error(args.workload, b, args.err_steps),
                               steps=args.err_steps)
+        if args.cells > 0:
+            extra["dense_cells"] = cells_leg(args.workload, b, args.cells, args.cell_steps,
+                                             max(1, args.cell_steps // 10))
 
     line = {
         "metric": "fluid MLUPS (FP32-deviation D3Q19 step) + % HBM roofline vs FP64 form",

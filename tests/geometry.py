@@ -54,6 +54,18 @@ def place_cells(b, ref, r_cell, gap, ncell, seed, max_tries=200000):
     return out[:n]
 
 
+def place_cells_dense(b, ref, gap, ncell, seed, max_tries=2000000):
+    im, jm, km = b.shape
+    b = np.ascontiguousarray(b, np.int32)
+    ref = np.ascontiguousarray(ref, np.float64)
+    out = np.zeros((ncell, len(ref), 3))
+    n = load_host_library().zfh_place_cells_dense(b.ctypes.data, im, jm, km, ref.ctypes.data,
+                                                  len(ref), gap, ncell, seed, max_tries,
+                                                  out.ctypes.data)
+    assert n >= 0
+    return out[:n]
+
+
 def wall_mesh(b, x0=0.0, y0=0.0, z0=0.0, dx=1.0):
     """Surface-nets wall mesh: (xyz [nv,3], tri [nt,3] 1-based, normal [nt,3])."""
     import ctypes as C

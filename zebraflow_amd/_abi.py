@@ -105,6 +105,7 @@ HOST_FUNCS = {
     "zfh_geom_tree": (i64, [P, i32, i32, i32, f64, f64, i32, f64, f64, C.c_uint64]),
     "zfh_mesh_icosphere": (C.c_int, [i32, f64, i32, P, P, i32, i32, C.POINTER(i32)]),
     "zfh_place_cells": (C.c_int, [P, i32, i32, i32, P, i32, f64, f64, i32, C.c_uint64, i32, P]),
+    "zfh_place_cells_dense": (C.c_int, [P, i32, i32, i32, P, i32, f64, i32, C.c_uint64, i32, P]),
     "zfh_wall_mesh": (C.c_int, [P, i32, i32, i32, f64, f64, f64, f64, P, i64, P, P, i64,
                                 C.POINTER(i64), C.POINTER(i64)]),
     "zfh_write_bindex_csv":(C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64]),

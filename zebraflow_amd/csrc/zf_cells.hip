@@ -49,33 +49,86 @@ __device__ __forceinline__ size_t bq(const Grid &g, int i, int j, int k) {
 
 // Cell list: nearest listed node to each triangle centroid, strict < in
 // (i,j,k) order, kept below the bin cut-off (ZebraflowWT.c:5857-5887).
-__global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, int nt, int ncell,
+// The nearest lattice point of a centroid is a corner of the unit cell
+// holding it, and every other lattice point is strictly farther than one
+// of those corners; so when the nearest corner is listed (B >= 0), the
+// minimum over the 8 corners, visited in (i,j,k) order, is the window's
+// minimum with the same tie rule.  Otherwise the full window is scanned.
+// Also records, for the repulsion pass, the largest centroid-to-bin
+// distance and the largest centroid-to-vertex distance of any triangle
+// (atomicMax on the bits of non-negative doubles, which order like the
+// values).
+__global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double *__restrict__ X,
+                          size_t NT, int nn, const int32_t *__restrict__ tri, int nt, int ncell,
                           const int32_t *__restrict__ status, const int32_t *__restrict__ b,
                           const int32_t *__restrict__ domid, Grid g, double cutoff, int dommax,
-                          int32_t *__restrict__ tbin, uint64_t *__restrict__ keys) {
+                          int32_t *__restrict__ tbin, uint64_t *__restrict__ keys,
+                          unsigned long long *__restrict__ bounds) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncell * nt) return;
+  const bool live = q < ncell * nt;
   int bin = 0;
-  if (status[q / nt] >= 0) {
+  double bdist = 0, ext = 0;
+  if (live && status[q / nt] >= 0) {
     const double x = tg[4 * TT + q], y = tg[5 * TT + q], z = tg[6 * TT + q];
-    int lo[3], hi[3];
-    window(g, x, y, z, lo, hi);
     double mind = 5000.0;
     int bi = 0, bj = 0, bk = 0;
-    for (int i = lo[0]; i <= hi[0]; i++)
-      for (int j = lo[1]; j <= hi[1]; j++)
-        for (int k = lo[2]; k <= hi[2]; k++) {
-          if (b[bq(g, i, j, k)] < 0) continue;
+    // corners of the unit cell around the centroid (1-based lattice indices)
+    const int ci = (int)floor((x - g.x0) / g.dx) + 1, cj = (int)floor((y - g.y0) / g.dx) + 1,
+              ck = (int)floor((z - g.z0) / g.dx) + 1;
+    // fast path only where all 8 corners lie in the box
+    const bool inside = ci >= 1 && ci + 1 <= g.im && cj >= 1 && cj + 1 <= g.jm && ck >= 1 &&
+                        ck + 1 <= g.km;
+    double cmin = 5000.0;  // nearest corner, listed or not
+    for (int i = ci; i <= ci + 1 && inside; i++)
+      for (int j = cj; j <= cj + 1; j++)
+        for (int k = ck; k <= ck + 1; k++) {
           const double dx = g.x0 + (i - 1) * g.dx - x, dy = g.y0 + (j - 1) * g.dx - y,
                        dz = g.z0 + (k - 1) * g.dx - z;
           const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          cmin = fmin(cmin, d);
+          if (b[bq(g, i, j, k)] < 0) continue;
           if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
         }
-    if (mind < cutoff) bin = domid[bq(g, bi, bj, bk)];
+    if (!inside || !(mind <= cmin)) {
+      int lo[3], hi[3];
+      window(g, x, y, z, lo, hi);
+      mind = 5000.0;
+      for (int i = lo[0]; i <= hi[0]; i++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+          for (int k = lo[2]; k <= hi[2]; k++) {
+            if (b[bq(g, i, j, k)] < 0) continue;
+            const double dx = g.x0 + (i - 1) * g.dx - x, dy = g.y0 + (j - 1) * g.dx - y,
+                         dz = g.z0 + (k - 1) * g.dx - z;
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
+          }
+    }
+    if (mind < cutoff) {
+      bin = domid[bq(g, bi, bj, bk)];
+      bdist = mind;
+    }
+    const size_t b0 = (size_t)(q / nt) * nn;
+    const int t = q % nt;
+    for (int v = 0; v < 3; v++) {
+      const size_t n = b0 + tri[3 * t + v];
+      const double dx = X[n] - x, dy = X[NT + n] - y, dz = X[2 * NT + n] - z;
+      ext = fmax(ext, sqrt(dx * dx + dy * dy + dz * dz));
+    }
   }
-  tbin[q] = bin;
-  const uint64_t key_bin = bin > 0 ? (uint64_t)(bin - 1) : (uint64_t)dommax;
-  keys[q] = (key_bin << 32) | (uint64_t)q;
+  if (live) {
+    tbin[q] = bin;
+    const uint64_t key_bin = bin > 0 ? (uint64_t)(bin - 1) : (uint64_t)dommax;
+    keys[q] = (key_bin << 32) | (uint64_t)q;
+  }
+  // one atomic per wave for each bound (every lane of the wave takes part)
+  for (int off = 32; off > 0; off >>= 1) {
+    bdist = fmax(bdist, __shfl_xor(bdist, off));
+    ext = fmax(ext, __shfl_xor(ext, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&bounds[0], (unsigned long long)__double_as_longlong(bdist));
+    atomicMax(&bounds[1], (unsigned long long)__double_as_longlong(ext));
+  }
 }
 
 __global__ void k_bin_ranges(const uint64_t *__restrict__ keys, int n, uint64_t nbins,
@@ -114,7 +167,13 @@ __device__ __forceinline__ void sub16(double a, double b, double c, double tc[16
 
 struct RepParams {
   double k_cell, cutoff, search, k_wall, wall_d0, wall_dmax, wall_search;
+  double wall_reach2;  // (wall_search + largest wall-triangle extent)^2, see k_wall_repulsion
+  int use_bounds;      // the cell list's distance bounds match the current geometry
 };
+
+// Slack on the distance prefilters below, far above FP64 rounding of the
+// distances and far below any geometric scale.
+constexpr double kReachSlack = 1e-9;
 
 // Cell-cell repulsion, one lane per node of a released cell (status > 0).
 __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
@@ -122,8 +181,9 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
                                  const double *__restrict__ nnormal, const double *__restrict__ tg,
                                  size_t TT, const int32_t *__restrict__ domid,
                                  const uint64_t *__restrict__ keys,
-                                 const int32_t *__restrict__ range, Grid g, RepParams r,
-                                 double *__restrict__ vrep) {
+                                 const int32_t *__restrict__ range,
+                                 const unsigned long long *__restrict__ bounds, Grid g,
+                                 RepParams r, double *__restrict__ vrep) {
   const int gq = blockIdx.x * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
   const int c = gq / nn, node = gq % nn;
@@ -133,20 +193,44 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
   int lo[3], hi[3];
   window(g, xc, yc, zc, lo, hi);
   double v0 = 0, v1 = 0, v2 = 0;
+  // Exact prefilters (they skip only work that adds nothing to the sum):
+  //  - a triangle contributes only through sub-centroids closer than
+  //    `cutoff`; they lie within `ext` (largest centroid-to-vertex distance)
+  //    of its centroid, so a centroid farther than min(search, cutoff + ext)
+  //    contributes nothing;
+  //  - a triangle's bin is within `bd` (largest centroid-to-bin distance)
+  //    of its centroid, so a bin farther than that reach plus bd holds no
+  //    contributing triangle.
+  // Both bounds were recorded when the cell list was built; the three tests
+  // on a triangle are filters of one sum, so their order is free.
+  double treach = r.search + kReachSlack, breach = 1e300;
+  if (r.use_bounds) {
+    const double bd = __longlong_as_double((long long)bounds[0]);
+    const double ext = __longlong_as_double((long long)bounds[1]);
+    treach = fmin(r.search, r.cutoff + ext) + kReachSlack;
+    breach = treach + bd + kReachSlack;
+  }
+  const double treach2 = treach * treach, breach2 = breach * breach;
   for (int i = lo[0]; i <= hi[0]; i++)
-    for (int j = lo[1]; j <= hi[1]; j++)
+    for (int j = lo[1]; j <= hi[1]; j++) {
+      const double qx = g.x0 + (i - 1) * g.dx - xc, qy = g.y0 + (j - 1) * g.dx - yc;
+      const double rij = qx * qx + qy * qy;
+      if (rij >= breach2) continue;
       for (int k = lo[2]; k <= hi[2]; k++) {
+        const double qz = g.z0 + (k - 1) * g.dx - zc;
+        if (rij + qz * qz >= breach2) continue;
         const int o = domid[bq(g, i, j, k)];
         if (o <= 0) continue;
         const int beg = range[2 * (o - 1)], end = range[2 * (o - 1) + 1];
         for (int e = beg; e < end; e++) {
           const int q = (int)(keys[e] & 0xffffffffu);
+          const double dx = xc - tg[4 * TT + q], dy = yc - tg[5 * TT + q], dz = zc - tg[6 * TT + q];
+          const double dc2 = dx * dx + dy * dy + dz * dz;
+          if (dc2 >= treach2 || sqrt(dc2) >= r.search) continue;
           const int m = q / nt, t = q % nt;
           const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
           if (m == c && (n1 == node || n2 == node || n3 == node)) continue;
           if (e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0) continue;
-          const double dx = xc - tg[4 * TT + q], dy = yc - tg[5 * TT + q], dz = zc - tg[6 * TT + q];
-          if (sqrt(dx * dx + dy * dy + dz * dz) >= r.search) continue;
           const size_t b0 = (size_t)m * nn;
           double tx[16], ty[16], tz[16];
           sub16(X[b0 + n1], X[b0 + n2], X[b0 + n3], tx);
@@ -164,6 +248,7 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
           }
         }
       }
+    }
   vrep[gq] += v0;
   vrep[NT + gq] += v1;
   vrep[2 * NT + gq] += v2;
@@ -172,6 +257,7 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
 // Cell-wall repulsion, one lane per node of a released cell.
 __global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn, int ncell,
                                  const int32_t *__restrict__ status, const int32_t *__restrict__ windex,
+                                 const float *__restrict__ wdist,
                                  const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
                                  const double *__restrict__ wxyz, const double *__restrict__ wnormal,
                                  Grid g, RepParams r, double *__restrict__ vrep,
@@ -181,6 +267,24 @@ __global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn
   nearest[gq] = 0;
   if (status[gq / nn] <= 0) return;
   const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
+  // Whole-node skip: with q0 the lattice node nearest x and d0 its distance
+  // to its nearest wall node (W_index), every wall node is at least
+  // d0 - |x - q0| from x.  A lattice node without a W_index entry lies
+  // outside the 9^3 window of every wall node, at least 3.9 dx from all of
+  // them.  If that bound reaches sqrt(wall_reach2), no triangle can be
+  // chosen (see the per-node test below) and the node gets nothing.
+  {
+    const int i0 = (int)floor((x - g.x0) / g.dx + 0.5) + 1, j0 = (int)floor((y - g.y0) / g.dx + 0.5) + 1,
+              k0 = (int)floor((z - g.z0) / g.dx + 0.5) + 1;
+    if (i0 >= 1 && i0 <= g.im && j0 >= 1 && j0 <= g.jm && k0 >= 1 && k0 <= g.km) {
+      const float w0 = wdist[bq(g, i0, j0, k0)];
+      const double d0 = isinf(w0) ? 3.9 * g.dx : (double)w0 * (1.0 - 1e-6);
+      const double qx = g.x0 + (i0 - 1) * g.dx - x, qy = g.y0 + (j0 - 1) * g.dx - y,
+                   qz = g.z0 + (k0 - 1) * g.dx - z;
+      const double lb = d0 - sqrt(qx * qx + qy * qy + qz * qz);
+      if (lb > 0 && lb * lb >= r.wall_reach2) return;
+    }
+  }
   int lo[3], hi[3];
   window(g, x, y, z, lo, hi);
   double mind = r.wall_search;
@@ -190,6 +294,12 @@ __global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn
       for (int k = lo[2]; k <= hi[2]; k++) {
         const int wn = windex[bq(g, i, j, k)];
         if (wn <= 0) continue;
+        // every vertex and the centroid of a triangle of wall node wn lie
+        // within wall_lmax of it: if wn is farther than wall_search +
+        // wall_lmax, none of them can beat mindis <= wall_search
+        const double *wp = &wxyz[3 * (size_t)(wn - 1)];
+        const double ex = x - wp[0], ey = y - wp[1], ez = z - wp[2];
+        if (ex * ex + ey * ey + ez * ez >= r.wall_reach2) continue;
         const int32_t *row = &wnode_tri[13 * (size_t)(wn - 1)];
         const int cnt = min(row[0], 12);
         for (int l = 1; l <= cnt; l++) {
@@ -246,11 +356,12 @@ __global__ void k_windex(const double *__restrict__ wxyz, int nwn, Grid g,
 }
 
 __global__ void k_windex_final(const unsigned long long *__restrict__ key, size_t box,
-                               int32_t *__restrict__ windex) {
+                               int32_t *__restrict__ windex, float *__restrict__ wdist) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= box) return;
   const unsigned long long kv = key[q];
   windex[q] = kv == ~0ull ? 0 : (int32_t)(kv & 0xffffffffu);
+  wdist[q] = kv == ~0ull ? __builtin_inff() : __uint_as_float((unsigned)(kv >> 32));
 }
 
 __global__ void k_fill_u64(unsigned long long *a, size_t n) {
@@ -266,7 +377,7 @@ int require_rep(zf_ctx *ctx) {
 
 void free_wall(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
-                   (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex};
+                   (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_wall = false;
 }
@@ -275,7 +386,8 @@ void free_wall(zf_ctx *ctx) {
 
 void free_cells(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_tkeys[0], (void **)&ctx->d_tkeys[1], (void **)&ctx->d_tsort_tmp,
-                   (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near};
+                   (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near,
+                   (void **)&ctx->d_rep_bounds};
   for (auto b : bufs) dev_free(ctx, b);
   free_wall(ctx);
   ctx->have_rep = false;
@@ -287,9 +399,14 @@ int cell_list_impl(zf_ctx *ctx) {
   hipLaunchKernelGGL(k_fill, dim3(nb(2 * (size_t)ctx->dommax, 256)), dim3(256), 0, ctx->stream,
                      ctx->d_bin_range, 2 * (size_t)ctx->dommax, 0);
   if (n == 0) return ZF_OK;
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctx->d_rep_bounds,
+                     (size_t)4, 0);
+  const size_t NT = ncap * ctx->nn;
   hipLaunchKernelGGL(k_tri_bin, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_tgeo, TT,
-                     ctx->nt, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_domid, grid_of(ctx),
-                     ctx->rcfg.bin_cutoff, ctx->dommax, ctx->d_tbin, ctx->d_tkeys[0]);
+                     ctx->d_x, NT, ctx->nn, ctx->d_tri, ctx->nt, ctx->ncell, ctx->d_status,
+                     ctx->d_b, ctx->d_domid, grid_of(ctx), ctx->rcfg.bin_cutoff, ctx->dommax,
+                     ctx->d_tbin, ctx->d_tkeys[0], ctx->d_rep_bounds);
+  ctx->bins_gen = ctx->geom_gen;
   int bits = 1;
   while ((1ull << bits) <= (uint64_t)ctx->dommax) bits++;
   size_t tmp = ctx->tsort_tmp_bytes;
@@ -305,20 +422,23 @@ int repulsion_impl(zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
   const int n = ctx->ncell * ctx->nn;
   if (n == 0) return ZF_OK;
+  const double wall_reach = ctx->rcfg.wall_search + ctx->wall_lmax + kReachSlack;
   const RepParams r{ctx->rcfg.k_cell, ctx->rcfg.cutoff, ctx->rcfg.search, ctx->rcfg.k_wall,
-                    ctx->rcfg.wall_d0, ctx->rcfg.wall_dmax, ctx->rcfg.wall_search};
+                    ctx->rcfg.wall_d0, ctx->rcfg.wall_dmax, ctx->rcfg.wall_search,
+                    wall_reach * wall_reach, ctx->bins_gen == ctx->geom_gen ? 1 : 0};
   hipLaunchKernelGGL(k_fill, dim3(nb(3 * NT * 2, 256)), dim3(256), 0, ctx->stream,
                      (int32_t *)ctx->d_vrep, 3 * NT * 2, 0);
   hipLaunchKernelGGL(k_fill, dim3(nb(NT, 256)), dim3(256), 0, ctx->stream, ctx->d_near, NT, 0);
   if (ctx->have_wall)
     hipLaunchKernelGGL(k_wall_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
-                       ctx->nn, ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wnode_tri,
+                       ctx->nn, ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wdist,
+                       ctx->d_wnode_tri,
                        ctx->d_wtri, ctx->d_wxyz, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep,
                        ctx->d_near);
   hipLaunchKernelGGL(k_cell_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
-                     ctx->d_tgeo, TT, ctx->d_domid, ctx->d_tkeys[1], ctx->d_bin_range, grid_of(ctx),
-                     r, ctx->d_vrep);
+                     ctx->d_tgeo, TT, ctx->d_domid, ctx->d_tkeys[1], ctx->d_bin_range,
+                     ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_vrep);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }
@@ -352,8 +472,10 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
       (r = dev_alloc(ctx, (void **)&ctx->d_tkeys[1], 8 * TT)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_tbin, 4 * TT)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_bin_range, 8 * (size_t)std::max(ctx->dommax, 1))) ||
-      (r = dev_alloc(ctx, (void **)&ctx->d_near, 4 * NT)))
+      (r = dev_alloc(ctx, (void **)&ctx->d_near, 4 * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_rep_bounds, 2 * sizeof(unsigned long long))))
     return r;
+  ctx->bins_gen = ~0ull;
   size_t tmp = 0;
   ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ctx->d_tkeys[0], ctx->d_tkeys[1],
                                                   (int)TT, 0, 64, ctx->stream));
@@ -379,12 +501,29 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
       row[0]++;
       if (row[0] <= 12) row[row[0]] = m;
     }
+  // largest distance from a wall triangle's vertex to its other vertices
+  // and its centroid (the reach of a wall node's triangles)
+  double lmax = 0;
+  for (int m = 0; m < nwt; m++) {
+    const double *v[3];
+    for (int s = 0; s < 3; s++) v[s] = &wall_xyz[3 * (size_t)(wall_tri[3 * (size_t)m + s] - 1)];
+    double cen[3];
+    for (int a = 0; a < 3; a++) cen[a] = (v[0][a] + v[1][a] + v[2][a]) / 3.0;
+    for (int s = 0; s < 3; s++) {
+      const double *pts[3] = {v[(s + 1) % 3], v[(s + 2) % 3], cen};
+      for (const double *p : pts) {
+        const double dx = v[s][0] - p[0], dy = v[s][1] - p[1], dz = v[s][2] - p[2];
+        lmax = std::max(lmax, std::sqrt(dx * dx + dy * dy + dz * dz));
+      }
+    }
+  }
   unsigned long long *key = nullptr;
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_wxyz, 24 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnormal, 24 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wtri, 12 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnode_tri, 52 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_windex, 4 * ctx->box)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wdist, 4 * ctx->box)) ||
       (rc = dev_alloc(ctx, (void **)&key, 8 * ctx->box)))
     return rc;
   ZF_HIP(ctx, hipMemcpy(ctx->d_wxyz, wall_xyz, 24 * (size_t)nwn, hipMemcpyHostToDevice));
@@ -395,7 +534,7 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   hipLaunchKernelGGL(k_windex, dim3(nb(nwn, 64)), dim3(64), 0, ctx->stream, ctx->d_wxyz, nwn,
                      grid_of(ctx), key);
   hipLaunchKernelGGL(k_windex_final, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key,
-                     ctx->box, ctx->d_windex);
+                     ctx->box, ctx->d_windex, ctx->d_wdist);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   ctx->device_bytes -= 8 * (int64_t)ctx->box;
@@ -403,6 +542,7 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "wall setup: %s", hipGetErrorString(e));
   ctx->nwn = nwn;
   ctx->nwt = nwt;
+  ctx->wall_lmax = lmax;
   ctx->have_wall = true;
   return ZF_OK;
 }

@@ -362,6 +362,7 @@ int ibm_spread_impl(zf_ctx *ctx) {
 }
 
 int advance_impl(zf_ctx *ctx, int32_t *n_removed) {
+  ctx->geom_gen++;  // positions change: repulsion bounds of the cell list expire
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
   const int n = ctx->ncell * ctx->nn, nc = ctx->ncell;
   if (n == 0) {

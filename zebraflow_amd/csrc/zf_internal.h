@@ -135,10 +135,17 @@ struct zf_ctx {
   int32_t *d_tbin = nullptr;      // ncap*nt: 1-based DOMID of the triangle's bin or 0
   int32_t *d_bin_range = nullptr; // 2 x dommax: [begin, end) in the sorted keys
   int32_t *d_near = nullptr;      // ncap*nn: chosen wall triangle (1-based) or 0
+  // bounds recorded with the cell list: largest centroid-to-bin and
+  // centroid-to-vertex distance (FP64 bits); valid while the triangle
+  // geometry is the one the list was built from (geom_gen == bins_gen)
+  unsigned long long *d_rep_bounds = nullptr;
+  uint64_t geom_gen = 0, bins_gen = ~0ull;
   int32_t nwn = 0, nwt = 0;
+  double wall_lmax = 0;  // largest vertex-to-vertex or vertex-to-centroid distance of a wall triangle
   double *d_wxyz = nullptr, *d_wnormal = nullptr;  // 3 x nwn, 3 x nwt (AoS)
   int32_t *d_wtri = nullptr, *d_wnode_tri = nullptr;  // 3 x nwt (1-based), 13 x nwn
   int32_t *d_windex = nullptr;    // box
+  float *d_wdist = nullptr;       // box: FP32 distance to that wall node (or +inf)
 };
 
 namespace zf {

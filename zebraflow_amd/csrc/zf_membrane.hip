@@ -570,6 +570,7 @@ int zf_download_mesh_equilibrium(zf_ctx *ctx, double *lmax, double *plength, dou
 
 int zf_upload_cells(zf_ctx *ctx, int32_t ncell, const double *xyz) {
   if (!ctx || (ncell > 0 && !xyz) || ncell < 0) return ZF_ERR_ARG;
+  ctx->geom_gen++;  // positions change: repulsion bounds of the cell list expire
   if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
   if (ncell > ctx->mcfg.max_cells) return fail(ctx, ZF_ERR_ARG, "%d cells > max_cells %d", ncell, ctx->mcfg.max_cells);
   const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn, n = (size_t)ncell * ctx->nn;
@@ -629,6 +630,7 @@ int zf_membrane_forces(zf_ctx *ctx) {
 }  // extern "C"
 
 int zf::membrane_forces_impl(zf_ctx *ctx) {
+  ctx->geom_gen++;
   if (ctx->ncell == 0) return ZF_OK;
   const MembParams mp = params_of(ctx->mcfg);
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;

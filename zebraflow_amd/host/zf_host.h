@@ -57,6 +57,12 @@ int zfh_mesh_icosphere(int level, double radius, int biconcave, double *xyz, int
 int zfh_place_cells(const int32_t *b, int im, int jm, int km, const double *ref, int nn,
                     double r_cell, double gap, int ncell, uint64_t seed, int max_tries,
                     double *xyz_out);
+/* Denser placement for cell-laden runs: centres drawn from interior fluid
+ * nodes, the overlap test on the mesh nodes themselves (no node within gap
+ * of another cell's node, every node's lattice neighbourhood interior
+ * fluid).  Returns cells placed, or -1 on bad arguments. */
+int zfh_place_cells_dense(const int32_t *b, int im, int jm, int km, const double *ref, int nn,
+                          double gap, int ncell, uint64_t seed, int max_tries, double *xyz_out);
 
 /* Lumen wall mesh of a B field by surface nets on the fluid indicator B > 0:
  * vertices at lattice-cube crossings, one quad per crossing lattice edge,

@@ -147,3 +147,77 @@ int zfh_place_cells(const int32_t *b, int im, int jm, int km, const double *ref,
   free(cen);
   return placed;
 }
+
+/* Dense random placement for high haematocrit: centres drawn from the
+ * interior fluid nodes, random orientations, and the test is on the mesh
+ * itself rather than a bounding sphere: every node must sit on a lattice
+ * point neighbourhood (its nearest node and that node's 6 neighbours) of
+ * interior fluid, and no node may come within `gap` of a node of a cell
+ * already placed (uniform-grid hash of placed nodes).  Returns cells placed. */
+int zfh_place_cells_dense(const int32_t *b, int im, int jm, int km, const double *ref, int nn,
+                          double gap, int ncell, uint64_t seed, int max_tries, double *xyz_out) {
+  if (!b || !ref || nn <= 0 || ncell < 0 || gap <= 0 || !xyz_out) return -1;
+  const size_t box = (size_t)im * jm * km;
+  int64_t nf = 0;
+  for (size_t q = 0; q < box; q++) nf += b[q] > 0 && b[q] < 10;
+  if (nf == 0 || ncell == 0) return 0;
+  int64_t *fl = malloc(sizeof(int64_t) * nf);
+  nf = 0;
+  for (size_t q = 0; q < box; q++)
+    if (b[q] > 0 && b[q] < 10) fl[nf++] = (int64_t)q;
+  /* hash grid of placed nodes, cell size h >= gap */
+  const double h = gap > 1.0 ? gap : 1.0;
+  const int gi = (int)(im / h) + 2, gj = (int)(jm / h) + 2, gk = (int)(km / h) + 2;
+  const size_t ng = (size_t)gi * gj * gk;
+  int32_t *head = malloc(sizeof(int32_t) * ng);
+  int32_t *next = malloc(sizeof(int32_t) * (size_t)ncell * nn);
+  for (size_t q = 0; q < ng; q++) head[q] = -1;
+  double *cand = malloc(sizeof(double) * 3 * nn);
+  uint64_t s = seed;
+  int placed = 0;
+  const int off[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int tries = 0; tries < max_tries && placed < ncell; tries++) {
+    const int64_t q = fl[(int64_t)(ur(&s) * nf) % nf];
+    const double c[3] = {(double)(q / ((int64_t)jm * km)) + ur(&s) - 0.5,
+                         (double)((q / km) % jm) + ur(&s) - 0.5, (double)(q % km) + ur(&s) - 0.5};
+    place(ref, nn, c, 2 * M_PI * ur(&s), acos(2 * ur(&s) - 1), 2 * M_PI * ur(&s), cand);
+    int ok = 1;
+    for (int n = 0; n < nn && ok; n++) {
+      const double *x = &cand[3 * n];
+      const int i = (int)floor(x[0] + 0.5), j = (int)floor(x[1] + 0.5), k = (int)floor(x[2] + 0.5);
+      for (int o = 0; o < 7 && ok; o++) {
+        const int a = i + off[o][0], bb = j + off[o][1], cc = k + off[o][2];
+        if (a < 0 || a >= im || bb < 0 || bb >= jm || cc < 0 || cc >= km) { ok = 0; break; }
+        const int v = b[((size_t)a * jm + bb) * km + cc];
+        if (!(v > 0 && v < 10)) ok = 0;
+      }
+      if (!ok) break;
+      const int hi = (int)(x[0] / h), hj = (int)(x[1] / h), hk = (int)(x[2] / h);
+      for (int a = hi - 1; a <= hi + 1 && ok; a++)
+        for (int bb = hj - 1; bb <= hj + 1 && ok; bb++)
+          for (int cc = hk - 1; cc <= hk + 1 && ok; cc++) {
+            if (a < 0 || a >= gi || bb < 0 || bb >= gj || cc < 0 || cc >= gk) continue;
+            for (int e = head[((size_t)a * gj + bb) * gk + cc]; e >= 0; e = next[e]) {
+              const double *y = &xyz_out[3 * (size_t)e];
+              const double dx = x[0] - y[0], dy = x[1] - y[1], dz = x[2] - y[2];
+              if (dx * dx + dy * dy + dz * dz < gap * gap) { ok = 0; break; }
+            }
+          }
+    }
+    if (!ok) continue;
+    memcpy(&xyz_out[3 * (size_t)placed * nn], cand, sizeof(double) * 3 * nn);
+    for (int n = 0; n < nn; n++) {
+      const int e = placed * nn + n;
+      const double *x = &cand[3 * n];
+      const size_t g = ((size_t)(int)(x[0] / h) * gj + (size_t)(int)(x[1] / h)) * gk + (size_t)(int)(x[2] / h);
+      next[e] = head[g];
+      head[g] = e;
+    }
+    placed++;
+  }
+  free(fl);
+  free(head);
+  free(next);
+  free(cand);
+  return placed;
+}
