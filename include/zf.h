@@ -268,6 +268,19 @@ int zf_download_wall_neighbour(zf_ctx *ctx, int32_t *nearest_wall);
  * position update; otherwise the LBM step alone. */
 int zf_step(zf_ctx *ctx, int32_t iter);
 
+/* ---- wall shear stress and wall pressure ----------------------------------
+ * At n wall-mesh points (xyz: n*3, unit normals into the fluid: n*3), from
+ * the current populations (PostProWSSandPressure.c:267-369):
+ *   pressure[m] = (D - 1) * pressure_scale - pgauge at the nearest lattice
+ *                 node with B >= 0 closer than 1.42 DX (valid[m] = 1), else
+ *                 0 with valid[m] = 0;
+ *   wss[3m..]   = -mu_over_dx * (u - (u.n) n), u the unnormalised cosine-
+ *                 kernel velocity at x + DX n over nodes 0.11 < B < 4997.
+ * Positions use the IBM grid (X0, DX) when set up, lattice units otherwise.
+ * The reference's mu/DX is 0.0012 / (DX * 1e-6). */
+int zf_wss(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal, double mu_over_dx,
+           double pgauge, double *wss, double *pressure, int32_t *valid);
+
 /* ---- timing and synchronisation ------------------------------------------ */
 /* Device-copy ceiling: a plain 16-byte-per-lane copy of `bytes` between two
  * fresh buffers, `reps` times; *gbps = 2*bytes*reps / time (read + write). */

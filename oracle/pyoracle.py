@@ -292,6 +292,21 @@ def ibm_spread(p, b, domid, dommax, xyz, ffluid, force=None):
     return out
 
 
+def wss(p, b, domid, macro, xyz, nrm, mu_over_dx, pressure_scale=10000.0 / 3, pgauge=0.0):
+    """Wall shear stress [n,3], wall pressure [n], valid [n] at wall points;
+    macro = (D, U, V, W) per listed node."""
+    x = np.ascontiguousarray(xyz, np.float64)
+    nv = np.ascontiguousarray(nrm, np.float64)
+    n = len(x)
+    out, pr, va = np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.int32)
+    m = [np.ascontiguousarray(a, np.float64) for a in macro]
+    lib().zfo_wss(C.byref(p), _p(np.ascontiguousarray(b, np.int32)),
+                  _p(np.ascontiguousarray(domid, np.int32)), *[_p(a) for a in m], C.c_int64(n),
+                  _p(x), _p(nv), C.c_double(mu_over_dx), C.c_double(pressure_scale),
+                  C.c_double(pgauge), _p(out), _p(pr), _p(va))
+    return out, pr, va
+
+
 def advance(p, xyz, vel, status):
     x = np.ascontiguousarray(xyz, np.float64)
     v = np.ascontiguousarray(vel, np.float64)

@@ -160,6 +160,12 @@ void zfo_ibm_interpolate(const zfo_ibm_params *p, const int32_t *b, const int32_
 void zfo_ibm_spread(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
                     int64_t dommax, int64_t nnode, const double *xyz, const double *ffluid,
                     double *force);
+/* Wall shear stress and wall pressure at n wall-mesh points (xyz, unit
+ * normals into the fluid nrm: n*3); D,U,V,W per listed node. */
+void zfo_wss(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid, const double *D,
+             const double *U, const double *V, const double *W, int64_t n, const double *xyz,
+             const double *nrm, double mu_over_dx, double pressure_scale, double pgauge,
+             double *wss, double *press, int32_t *valid);
 /* a17.  xyz, vel: ncell*nn*3; status per cell. */
 int zfo_advance(const zfo_ibm_params *p, int ncell, int nn, double *xyz, const double *vel,
                 int32_t *status);

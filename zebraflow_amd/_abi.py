@@ -93,6 +93,7 @@ FUNCS = {
     "zf_download_cell_list": (C.c_int, [P, P]),
     "zf_repulsion": (C.c_int, [P]),
     "zf_download_wall_neighbour": (C.c_int, [P, P]),
+    "zf_wss": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
     "zf_copy_bandwidth": (C.c_int, [P, i64, i32, C.POINTER(f64)]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),

@@ -31,14 +31,6 @@ constexpr double kPi = 3.14159265358979;  // PI, ZebraflowWT.c:33
 
 inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-__global__ void k_sbox(const int32_t *__restrict__ domid, size_t box, const int32_t *__restrict__ s_of_p,
-                       int32_t *__restrict__ sbox) {
-  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= box) return;
-  const int d = domid[q];
-  sbox[q] = d > 0 ? s_of_p[d - 1] : -1;
-}
-
 // U,V,W of every listed node without the force shift (ZebraflowWT.c:6606-6614).
 template <class T, bool DEV16, bool FULL>
 __global__ void k_macro_vel(const T *__restrict__ f, size_t stride, int n, float *__restrict__ u) {
@@ -296,7 +288,7 @@ int ensure_force_field(zf_ctx *ctx) {
 }  // namespace
 
 void free_ibm(zf_ctx *ctx) {
-  void **bufs[] = {(void **)&ctx->d_sbox,  (void **)&ctx->d_umac, (void **)&ctx->d_vrep,
+  void **bufs[] = {(void **)&ctx->d_umac, (void **)&ctx->d_vrep,
                    (void **)&ctx->d_ffl,   (void **)&ctx->d_keys[0], (void **)&ctx->d_keys[1],
                    (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag};
   for (auto b : bufs) dev_free(ctx, b);
@@ -428,8 +420,7 @@ int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
   const size_t ntiles = (size_t)ctx->ntx * ctx->nty * ctx->ntz;
   if (NT >= (1ull << 32) || ntiles >= (1ull << 31)) return fail(ctx, ZF_ERR_ARG, "too many nodes or tiles");
   int rc;
-  if ((rc = dev_alloc(ctx, (void **)&ctx->d_sbox, 4 * ctx->box)) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_umac, 12 * ctx->stride)) ||
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_umac, 12 * ctx->stride)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_vrep, 24 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_ffl, 12 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_keys[0], 8 * NT)) ||
@@ -442,8 +433,6 @@ int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
                                                   (int)NT, 0, 64, ctx->stream));
   ctx->sort_tmp_bytes = tmp;
   if ((rc = dev_alloc(ctx, &ctx->d_sort_tmp, tmp))) return rc;
-  hipLaunchKernelGGL(k_sbox, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, ctx->d_domid,
-                     ctx->box, ctx->d_s_of_p, ctx->d_sbox);
   hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
   ZF_HIP(ctx, hipGetLastError());
   ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));

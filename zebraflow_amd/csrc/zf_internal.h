@@ -171,6 +171,12 @@ void free_cells(zf_ctx *ctx);
 int membrane_forces_impl(zf_ctx *ctx);
 int cell_list_impl(zf_ctx *ctx);
 int repulsion_impl(zf_ctx *ctx);
+// Physical position of lattice node (i,j,k): (x0 + (i-1) dx, ...); the IBM
+// configuration's when set up, lattice units otherwise.
+struct GridGeom {
+  double x0, y0, z0, dx;
+};
+GridGeom grid_geom(const zf_ctx *ctx);
 int alloc_state(zf_ctx *ctx);
 size_t store_bytes(int storage);
 

@@ -16,7 +16,7 @@
 //   ZF_STORE_F64     double f                      double arithmetic
 //   ZF_STORE_F32DEV  float  f - w                  float arithmetic
 //   ZF_STORE_F16DEV  half   (f - w)/w              float arithmetic
-#include "zf_internal.h"
+#include "zf_d3q19.h"
 
 #include <cmath>
 #include <type_traits>
@@ -24,73 +24,13 @@
 namespace zf {
 namespace {
 
-template <int S>
-struct Store;
-template <>
-struct Store<ZF_STORE_F64> {
-  using T = double;
-  using R = double;
-  __device__ static R ld(const T *a, size_t i, double w) { return a[i] - w; }
-  __device__ static T st(R g, double w) { return g + w; }
-};
-template <>
-struct Store<ZF_STORE_F32DEV> {
-  using T = float;
-  using R = float;
-  __device__ static R ld(const T *a, size_t i, double) { return a[i]; }
-  __device__ static T st(R g, double) { return g; }
-};
-template <>
-struct Store<ZF_STORE_F16DEV> {
-  using T = _Float16;
-  using R = float;
-  // w applied as hi + lo floats (the float weights alone are biased high)
-  __device__ static R ld(const T *a, size_t i, double w) {
-    const float h = (float)a[i];
-    return __builtin_fmaf((float)w, h, (float)(w - (double)(float)w) * h);
-  }
-  __device__ static T st(R g, double w) { return (T)(g * (float)(1.0 / w)); }
-};
-
-// Compile-time lattice tables for fully unrolled loops.
-__device__ constexpr int EX[Q] = {1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0, 0};
-__device__ constexpr int EY[Q] = {0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 0};
-__device__ constexpr int EZ[Q] = {0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0};
-__device__ constexpr double WQ[Q] = {
-    1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 36,
-    1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
-    1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 3};
+using namespace d3q19;
 
 struct StepParams {
   double visc_scale;  // tau = 0.5 + visc_scale*B
   double dt;          // DT of the Guo terms
   double g[3];        // uniform body force
 };
-
-// w_c * y.  In FP32 the weights 1/18, 1/36 and 1/3 round up, so that the
-// float weights sum to 1 + 2.6e-8: multiplying by them makes the equilibrium
-// a mass (and momentum) source proportional to drho on every step, a bias
-// that accumulates along the flow.  The float form therefore multiplies by
-// the weight split in two floats, w = hi + lo, which rounds w * y correctly
-// up to 2^-48 and leaves only unbiased round-to-nearest errors.
-template <class R>
-__device__ __forceinline__ R wmul(int c, R y) { return (R)WQ[c] * y; }
-template <>
-__device__ __forceinline__ float wmul<float>(int c, float y) {
-  const float hi = (float)WQ[c];
-  const float lo = (float)(WQ[c] - (double)hi);
-  return __builtin_fmaf(hi, y, lo * y);
-}
-
-// Equilibrium deviation feq_c - w_c for rho = 1 + drho (ZebraflowWT.c:6139),
-// w (drho + X + drho X) with X = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2: the sum 1 +
-// drho is never formed, so the deviations keep their own relative precision.
-template <class R>
-__device__ __forceinline__ R geq(int c, R drho, R ux, R uy, R uz, R usq) {
-  const R cu = (R)EX[c] * ux + (R)EY[c] * uy + (R)EZ[c] * uz;
-  const R x = (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq;
-  return wmul<R>(c, drho + x + drho * x);
-}
 
 // ---------------------------------------------------------------------------
 // Boundary values, ZebraflowWT.c:6294-6433: all 19 populations of a BC node

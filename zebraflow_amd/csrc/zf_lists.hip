@@ -176,6 +176,15 @@ __global__ void k_bc_neighbour(const int32_t *__restrict__ b, Box bx, int bcmax,
 }
 
 // Pull sources: DOMID of (i,j,k) - e_c (ZebraflowWT.c:6114-6119).
+// Storage index of the listed node at each box position, or -1.
+__global__ void k_sbox(const int32_t *__restrict__ domid, size_t box, const int32_t *__restrict__ s_of_p,
+                       int32_t *__restrict__ sbox) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= box) return;
+  const int d = domid[q];
+  sbox[q] = d > 0 ? s_of_p[d - 1] : -1;
+}
+
 __global__ void k_pull_neighbours(const int32_t *__restrict__ domid, Box bx, int dommax,
                                   const int32_t *__restrict__ domlist, int32_t *__restrict__ nbr) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,7 +221,7 @@ void free_lattice(zf_ctx *ctx) {
                    (void **)&ctx->d_bclist,   (void **)&ctx->d_bcnlist, (void **)&ctx->d_nbr,
                    (void **)&ctx->d_s_of_p,   (void **)&ctx->d_p_of_s, (void **)&ctx->d_word,
                    (void **)&ctx->d_block_chord0, (void **)&ctx->d_chords, (void **)&ctx->d_bc,
-                   (void **)&ctx->d_fused_s, (void **)&ctx->d_att_start};
+                   (void **)&ctx->d_fused_s, (void **)&ctx->d_att_start, (void **)&ctx->d_sbox};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_lists = false;
   ctx->have_lattice = false;
@@ -463,6 +472,16 @@ int upload_lattice_impl(zf_ctx *ctx, const int32_t *b) {
     }
     ctx->have_lists = true;
     if ((rc = build_kernel_tables(ctx, b, domlist, nbr, bclist, bcnlist))) goto out;
+    // box -> storage index map, for the operators that look up lattice
+    // nodes by position (IBM, wall shear stress)
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_sbox, 4 * ctx->box))) goto out;
+    hipLaunchKernelGGL(k_sbox, dim3((unsigned)((ctx->box + 255) / 256)), dim3(256), 0, ctx->stream, ctx->d_domid,
+                       ctx->box, ctx->d_s_of_p, ctx->d_sbox);
+    if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      rc = fail(ctx, ZF_ERR_HIP, "box map: %s", hipGetErrorString(e));
+      goto out;
+    }
   }
   ctx->stride = ((size_t)std::max(ctx->dommax, 1) + 255) / 256 * 256;
   ctx->have_lattice = true;

@@ -1,0 +1,198 @@
+// zf_wss.hip -- wall shear stress and wall pressure at wall-mesh points.
+//
+// Replaces the reference's post-processing operator
+// (PostProWSSandPressure.c:267-369), computed on the device from the
+// current populations instead of from a FLUIDdomain CSV:
+//   wall pressure      nearest lattice node with B >= 0 closer than 1.42 DX,
+//                      p = (D - 1) * LBM_PRESSURE_SCALE - Pgauge
+//   wall shear stress  probe x + DX n, unnormalised cosine kernel over
+//                      |d| <= 2 DX on nodes 0.11 < B < 4997,
+//                      tau = -(mu/DX) (u - (u.n) n)
+// with the post-processor's window: index (int)((x-X0)/DX + 1.01) +- 3, i
+// clipped to [3, IM] and j, k to [1, JM], [1, KM] (its "else if" pairs clip
+// one side only; indices outside the box are skipped).  One lane per point,
+// FP64 throughout.  The velocities are the unshifted macro velocities of the
+// listed nodes (ZebraflowWT.c:6606-6614), the ones the reference writes out.
+#include "zf_d3q19.h"
+
+#include <cmath>
+
+namespace zf {
+namespace {
+
+using namespace d3q19;
+
+inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+struct WGrid {
+  double x0, y0, z0, dx;
+  int im, jm, km;
+};
+
+__device__ __forceinline__ size_t wq(const WGrid &g, int i, int j, int k) {
+  return ((size_t)(i - 1) * g.jm + (j - 1)) * g.km + (k - 1);
+}
+
+__device__ __forceinline__ void wss_window(const WGrid &g, double x, double y, double z, int lo[3],
+                                           int hi[3]) {
+  const int ii = (int)((x - g.x0) / g.dx + 1.01), jj = (int)((y - g.y0) / g.dx + 1.01),
+            kk = (int)((z - g.z0) / g.dx + 1.01);
+  lo[0] = ii - 3; hi[0] = ii + 3;
+  if (hi[0] > g.im) hi[0] = g.im;
+  else if (lo[0] < 3) lo[0] = 3;
+  lo[1] = jj - 3; hi[1] = jj + 3;
+  if (hi[1] > g.jm) hi[1] = g.jm;
+  else if (lo[1] < 1) lo[1] = 1;
+  lo[2] = kk - 3; hi[2] = kk + 3;
+  if (hi[2] > g.km) hi[2] = g.km;
+  else if (lo[2] < 1) lo[2] = 1;
+  lo[0] = max(lo[0], 1); lo[1] = max(lo[1], 1); lo[2] = max(lo[2], 1);
+}
+
+// drho and momentum of one listed node from its stored populations.
+template <int S>
+__device__ __forceinline__ void moments(const typename Store<S>::T *__restrict__ f, size_t stride,
+                                        int s, double &drho, double j[3]) {
+  drho = 0;
+  j[0] = j[1] = j[2] = 0;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const double g = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]);
+    drho += g;
+    j[0] += EX[c] * g;
+    j[1] += EY[c] * g;
+    j[2] += EZ[c] * g;
+  }
+}
+
+template <int S>
+__global__ void k_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
+                      const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, WGrid g,
+                      int n, const double *__restrict__ xyz, const double *__restrict__ nrm,
+                      double mu_over_dx, double pressure_scale, double pgauge,
+                      double *__restrict__ wss, double *__restrict__ press,
+                      int32_t *__restrict__ valid) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double x = xyz[3 * m], y = xyz[3 * m + 1], z = xyz[3 * m + 2];
+  const double nx = nrm[3 * m], ny = nrm[3 * m + 1], nz = nrm[3 * m + 2];
+  int lo[3], hi[3];
+  wss_window(g, x, y, z, lo, hi);
+  double mind = 1.42 * g.dx;
+  size_t best = ~(size_t)0;
+  for (int i = lo[0]; i <= hi[0]; i++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const size_t q = wq(g, i, j, k);
+        if (b[q] < 0) continue;
+        const double dx = x - (g.x0 + (i - 1) * g.dx), dy = y - (g.y0 + (j - 1) * g.dx),
+                     dz = z - (g.z0 + (k - 1) * g.dx);
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (d < mind) { mind = d; best = q; }
+      }
+  if (best != ~(size_t)0) {
+    double drho, jm[3];
+    moments<S>(f, stride, sbox[best], drho, jm);
+    valid[m] = 1;
+    press[m] = drho * pressure_scale - pgauge;
+  } else {
+    valid[m] = 0;
+    press[m] = 0.0;
+  }
+  const double xp = x + g.dx * nx, yp = y + g.dx * ny, zp = z + g.dx * nz;
+  const double width = 2.0 * g.dx, pi = 3.14159265358979;
+  wss_window(g, xp, yp, zp, lo, hi);
+  double u0 = 0, u1 = 0, u2 = 0;
+  for (int i = lo[0]; i <= hi[0]; i++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int k = lo[2]; k <= hi[2]; k++) {
+        const double ax = xp - (g.x0 + (i - 1) * g.dx), ay = yp - (g.y0 + (j - 1) * g.dx),
+                     az = zp - (g.z0 + (k - 1) * g.dx);
+        if (sqrt(ax * ax + ay * ay + az * az) > width) continue;
+        const size_t q = wq(g, i, j, k);
+        const int bv = b[q];
+        if (!(bv >= 1 && bv <= 4996)) continue;
+        const double w = (1.0 + cos(pi * fabs(ax) / width)) * (1.0 + cos(pi * fabs(ay) / width)) *
+                         (1.0 + cos(pi * fabs(az) / width)) / 64.0;
+        double drho, jv[3];
+        moments<S>(f, stride, sbox[q], drho, jv);
+        const double r = 1.0 + drho;
+        u0 += w * (jv[0] / r);
+        u1 += w * (jv[1] / r);
+        u2 += w * (jv[2] / r);
+      }
+  const double un = u0 * nx + u1 * ny + u2 * nz;
+  wss[3 * m] = -mu_over_dx * (u0 - un * nx);
+  wss[3 * m + 1] = -mu_over_dx * (u1 - un * ny);
+  wss[3 * m + 2] = -mu_over_dx * (u2 - un * nz);
+}
+
+}  // namespace
+
+GridGeom grid_geom(const zf_ctx *ctx) {
+  if (ctx->have_ibm) return GridGeom{ctx->icfg.x0, ctx->icfg.y0, ctx->icfg.z0, ctx->icfg.dx};
+  return GridGeom{0.0, 0.0, 0.0, 1.0};
+}
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+int zf_wss(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal, double mu_over_dx,
+           double pgauge, double *wss, double *pressure, int32_t *valid) {
+  if (!ctx || n < 0 || (n > 0 && (!xyz || !normal || !wss || !pressure || !valid)))
+    return ZF_ERR_ARG;
+  if (!ctx->have_lattice || !ctx->d_f[0]) return fail(ctx, ZF_ERR_STATE, "no lattice state");
+  if (n == 0) return ZF_OK;
+  if (n > (1ll << 30)) return fail(ctx, ZF_ERR_ARG, "too many wall points");
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  double *buf = nullptr;
+  int32_t *vb = nullptr;
+  int rc = dev_alloc(ctx, (void **)&buf, sizeof(double) * 10 * (size_t)n);
+  if (rc) return rc;
+  if ((rc = dev_alloc(ctx, (void **)&vb, sizeof(int32_t) * (size_t)n))) {
+    dev_free(ctx, (void **)&buf);
+    return rc;
+  }
+  double *d_xyz = buf, *d_nrm = buf + 3 * n, *d_wss = buf + 6 * n, *d_p = buf + 9 * n;
+  const GridGeom gp = grid_geom(ctx);
+  const WGrid g{gp.x0, gp.y0, gp.z0, gp.dx, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km};
+  hipError_t e = hipMemcpyAsync(d_xyz, xyz, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_nrm, normal, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    const void *f = ctx->d_f[ctx->cur];
+    const dim3 grid(nb((size_t)n, 64)), blk(64);
+    const double ps = ctx->cfg.pressure_scale;
+    switch (ctx->cfg.storage) {
+      case ZF_STORE_F64:
+        hipLaunchKernelGGL(k_wss<ZF_STORE_F64>, grid, blk, 0, ctx->stream, (const double *)f,
+                           ctx->stride, ctx->d_b, ctx->d_sbox, g, (int)n, d_xyz, d_nrm, mu_over_dx,
+                           ps, pgauge, d_wss, d_p, vb);
+        break;
+      case ZF_STORE_F32DEV:
+        hipLaunchKernelGGL(k_wss<ZF_STORE_F32DEV>, grid, blk, 0, ctx->stream, (const float *)f,
+                           ctx->stride, ctx->d_b, ctx->d_sbox, g, (int)n, d_xyz, d_nrm, mu_over_dx,
+                           ps, pgauge, d_wss, d_p, vb);
+        break;
+      default:
+        hipLaunchKernelGGL(k_wss<ZF_STORE_F16DEV>, grid, blk, 0, ctx->stream, (const _Float16 *)f,
+                           ctx->stride, ctx->d_b, ctx->d_sbox, g, (int)n, d_xyz, d_nrm, mu_over_dx,
+                           ps, pgauge, d_wss, d_p, vb);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(wss, d_wss, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pressure, d_p, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(valid, vb, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)(84 * (size_t)n);
+  dev_free(ctx, (void **)&buf);
+  dev_free(ctx, (void **)&vb);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "wall shear stress: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+}  // extern "C"

@@ -272,6 +272,17 @@ class Engine:
                     "zf_download_wall_neighbour")
         return out
 
+    # -- wall shear stress ------------------------------------------------------
+    def wss(self, xyz, normal, mu_over_dx, pgauge=0.0):
+        """(wss [n,3], pressure [n], valid [n]) at wall points."""
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        normal = np.ascontiguousarray(normal, np.float64)
+        n = len(xyz)
+        out, pr, va = np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.int32)
+        self._check(self.lib.zf_wss(self.h, n, _ptr(xyz), _ptr(normal), mu_over_dx, pgauge,
+                                    _ptr(out), _ptr(pr), _ptr(va)), "zf_wss")
+        return out, pr, va
+
     def copy_bandwidth(self, nbytes, reps=20):
         g = C.c_double()
         self._check(self.lib.zf_copy_bandwidth(self.h, int(nbytes), reps, C.byref(g)),
