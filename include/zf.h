@@ -281,6 +281,50 @@ int zf_step(zf_ctx *ctx, int32_t iter);
 int zf_wss(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal, double mu_over_dx,
            double pgauge, double *wss, double *pressure, int32_t *valid);
 
+/* ---- adjoint --------------------------------------------------------------
+ * Reverse-mode gradients of a wall-shear-stress objective through the LBM
+ * time step (the stream-collide of ZebraflowWT.c:6101-6145 with the
+ * boundary rules of :6292-6433), written by hand: the adjoint of the pull
+ * stream is a push along the same links, the collision's transpose is
+ * applied per node, boundary values feed back to their neighbour's moments.
+ * Populations are checkpointed every checkpoint_interval iterations in the
+ * forward pass; the backward pass recomputes each segment from its
+ * checkpoint and runs the adjoint steps through it in reverse.
+ *
+ * Objective: J = 1/((t_end - t_begin + 1) npts) * sum over iterations
+ * t_begin..t_end and over the npts wall points of |tau|, with tau the wall
+ * shear stress of zf_wss (mu_over_dx) on the populations after that
+ * iteration.  Gradients: inlet amplitude and u0 of the velocity inlet
+ * (ZF_B_VELOCITY_INLET), the uniform body force, the initial populations,
+ * and the per-node force field when one is uploaded.  FP64 and FP32
+ * storage (the adjoint populations use the storage's arithmetic type);
+ * not FP16.  This entry point has no counterpart in the reference, which
+ * has no adjoint. */
+typedef struct zf_adjoint_config {
+  int32_t checkpoint_interval;  /* K */
+  int32_t max_steps;            /* longest forward pass to record */
+  int32_t t_begin, t_end;       /* objective window (iteration numbers, inclusive) */
+  double mu_over_dx;            /* WSS scale, as in zf_wss */
+} zf_adjoint_config;
+
+int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, const double *xyz,
+                     const double *normal);
+/* Runs iterations iter0+1 .. iter0+nsteps from the current state (which
+ * it advances, as zf_lbm_run does), records checkpoints and returns J. */
+int zf_adjoint_forward(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double *objective);
+/* Backward pass of the last forward pass: grad[0] = dJ/d inlet_amp,
+ * grad[1] = dJ/d inlet_u0, grad[2..4] = dJ/d body_force.  The state is
+ * left as the forward pass left it. */
+int zf_adjoint_backward(zf_ctx *ctx, double grad[5]);
+/* dJ/d(initial populations), 19*dommax in the layout of
+ * zf_upload_populations; dJ/d(force field) 3*dommax (zeros without one). */
+int zf_adjoint_download(zf_ctx *ctx, double *dJ_df0, double *dJ_dforce);
+/* Milliseconds of the last forward and backward passes (HIP events). */
+int zf_adjoint_timing(zf_ctx *ctx, double ms[2]);
+/* Inlet and body-force parameters of later iterations (the gradients'
+ * variables), without rebuilding the context. */
+int zf_set_forcing(zf_ctx *ctx, double inlet_u0, double inlet_amp, const double body_force[3]);
+
 /* ---- timing and synchronisation ------------------------------------------ */
 /* Device-copy ceiling: a plain 16-byte-per-lane copy of `bytes` between two
  * fresh buffers, `reps` times; *gbps = 2*bytes*reps / time (read + write). */

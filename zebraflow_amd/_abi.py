@@ -45,6 +45,11 @@ class ZfRepulsionConfig(C.Structure):
                                    "wall_dmax", "wall_search")]
 
 
+class ZfAdjointConfig(C.Structure):
+    _fields_ = [("checkpoint_interval", i32), ("max_steps", i32), ("t_begin", i32), ("t_end", i32),
+                ("mu_over_dx", f64)]
+
+
 # name -> (restype, argtypes)
 FUNCS = {
     "zf_version": (C.c_char_p, []),
@@ -94,6 +99,12 @@ FUNCS = {
     "zf_repulsion": (C.c_int, [P]),
     "zf_download_wall_neighbour": (C.c_int, [P, P]),
     "zf_wss": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
+    "zf_adjoint_setup": (C.c_int, [P, C.POINTER(ZfAdjointConfig), i64, P, P]),
+    "zf_adjoint_forward": (C.c_int, [P, i32, i32, C.POINTER(f64)]),
+    "zf_adjoint_backward": (C.c_int, [P, P]),
+    "zf_adjoint_download": (C.c_int, [P, P, P]),
+    "zf_adjoint_timing": (C.c_int, [P, P]),
+    "zf_set_forcing": (C.c_int, [P, f64, f64, P]),
     "zf_copy_bandwidth": (C.c_int, [P, i64, i32, C.POINTER(f64)]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),

@@ -1,0 +1,850 @@
+// zf_adjoint.hip -- hand-written adjoint of the LBM time step with a
+// wall-shear-stress objective, with checkpointing.
+//
+// Forward iteration k maps the populations S_{k-1} to S_k
+// (ZebraflowWT.c:6101-6145 stream + collide, :6292-6433 boundary values,
+// the kernel of zf_lbm.hip).  With lambda_k = dJ/dS_k:
+//   lambda_{k-1} = A_k^T lambda_k + dJ/dS_{k-1}
+// and every parameter theta of iteration k collects lambda_k . dS_k/dtheta.
+//
+// A_k^T, one lane per interior node s (k_adjoint): the forward pulls
+// g_c(s) = S(src(s,c), c) and writes S_k(s, :) = C(g(s)) (BGK + Guo); the
+// pull map is one-to-one, so the transpose is a push: the lane forms
+// mu = dC/dg^T lambda_k(s, :) and writes mu_c to lambda_{k-1}(src(s,c), c),
+// each target written by exactly one lane.  Boundary nodes whose BCNLIST
+// neighbour is s take their values from s's moments; their lambda_k folds
+// into s's moment adjoint before mu is formed.  Boundary nodes fed by a
+// non-interior neighbour ("late" entries) add to that neighbour's lambda in
+// a second, sequential pass (k_adjoint_late).  Entries that the adjoint
+// reads but no lane pulls are cleared (k_zero_idx).
+//
+// Collision transpose.  out_c = (1-w) g_c + w geq_c(drho, u) + p S_c(u, F),
+// w = 1/tau, p = DT (1 - w/2), geq_c = w_c (drho + (1+drho) X_c),
+// X_c = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2, S_c = w_c (3 (c-u).F + 9 (c.u)(c.F)),
+// u = v/(1+drho), v = j + DT F/2.  With l = lambda_k(s, :):
+//   a_drho = w sum_c l_c w_c (1 + X_c)
+//   a_u    = sum_c l_c w_c [w (1+drho) ((3 + 9 c.u) c - 3 u) + p (9 (c.F) c - 3 F)]
+//   a_F    = p sum_c l_c w_c (3 (c - u) + 9 (c.u) c)  +  DT/2 a_u/(1+drho)
+//   mu_c   = (1-w) l_c + a_drho - a_u.u/(1+drho) + a_u.c/(1+drho)
+//
+// Objective: mean over the window and the wall points of |tau|, tau the
+// zf_wss shear stress; its gradient reaches the populations of each
+// stencil node through u = j/(1+drho): dJ/dg_c = dJ/du . (c - u)/(1+drho).
+#include "zf_wss_dev.h"
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+namespace zf {
+namespace {
+
+using namespace d3q19;
+using namespace wssdev;
+
+inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// arithmetic and storage type of the adjoint populations
+template <int S>
+struct Adj {
+  using A = double;
+};
+template <>
+struct Adj<ZF_STORE_F32DEV> {
+  using A = float;
+};
+
+struct AdjParams {
+  double u0, amp, sin_phase;  // inlet velocity u0 (1 + amp sin_phase) dir
+  double dir[3];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Sums five per-lane values over the workgroup (256 lanes) into
+// part[5 * blockIdx.x ..]; k_reduce_grads adds the workgroups' sums in
+// workgroup order, so the gradients do not depend on scheduling.
+__device__ __forceinline__ void block_sum(double r[5], double *__restrict__ part) {
+  __shared__ double wp[4][5];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = 0; i < 5; i++) {
+    const double v = wave_sum(r[i]);
+    if (lane == 0) wp[wave][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5)
+    part[5 * (size_t)blockIdx.x + threadIdx.x] =
+        wp[0][threadIdx.x] + wp[1][threadIdx.x] + wp[2][threadIdx.x] + wp[3][threadIdx.x];
+}
+
+// grads[i] += sum over the n partial rows, in row order (one lane per i).
+__global__ void k_reduce_grads(const double *__restrict__ part, int n, double *__restrict__ grads) {
+  const int i = threadIdx.x;
+  if (i >= 5) return;
+  double acc = 0;
+  for (int r = 0; r < n; r++) acc += part[5 * (size_t)r + i];
+  grads[i] += acc;
+}
+
+template <int S, bool FIELD>
+__global__ __launch_bounds__(256) void k_adjoint(
+    const typename Store<S>::T *__restrict__ src, const typename Adj<S>::A *__restrict__ lin,
+    typename Adj<S>::A *__restrict__ lout, size_t stride, int n_int, int nblk, int remap,
+    const uint32_t *__restrict__ word, const int32_t *__restrict__ block0,
+    const int32_t *__restrict__ chords, const float *__restrict__ force, StepParams prm,
+    const BcEntry *__restrict__ bc, const int32_t *__restrict__ fused_s,
+    const int32_t *__restrict__ att_start, int n_fused, BcParams bp, AdjParams ap,
+    double *__restrict__ gF, double *__restrict__ part) {
+  using R = typename Store<S>::R;
+  using A = typename Adj<S>::A;
+  const int s = logical_block(blockIdx.x, nblk, remap != 0) * 256 + threadIdx.x;
+  double red[5] = {0, 0, 0, 0, 0};
+  if (s < n_int) {
+    const uint32_t w = word[s];
+    const int ch = block0[s >> 6] + (int)(w >> kWordChordShift);
+    const int4 *cr = reinterpret_cast<const int4 *>(chords + (size_t)ch * kChordInts);
+    int d[20];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const int4 x = cr[v];
+      d[4 * v] = x.x; d[4 * v + 1] = x.y; d[4 * v + 2] = x.z; d[4 * v + 3] = x.w;
+    }
+    size_t idx[Q];
+    R g[Q];
+#pragma unroll
+    for (int c = 0; c < 18; c++) {
+      const bool bb = (w >> c) & 1u;
+      idx[c] = bb ? (size_t)(c ^ 1) * stride + s : (size_t)c * stride + (s + d[c]);
+      g[c] = Store<S>::ld(src, idx[c], WQ[c]);
+    }
+    idx[18] = (size_t)18 * stride + s;
+    g[18] = Store<S>::ld(src, idx[18], WQ[18]);
+    R fx = (R)prm.g[0], fy = (R)prm.g[1], fz = (R)prm.g[2];
+    if (FIELD) {
+      fx += (R)force[s];
+      fy += (R)force[stride + s];
+      fz += (R)force[2 * stride + s];
+    }
+    const R dt = (R)prm.dt;
+    R drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int c = 0; c < Q; c++) {
+      drho += g[c];
+      jx += (R)EX[c] * g[c];
+      jy += (R)EY[c] * g[c];
+      jz += (R)EZ[c] * g[c];
+    }
+    const R rho = (R)1 + drho;
+    const R q = drho / rho;
+    const R vx = jx + (R)0.5 * dt * fx, vy = jy + (R)0.5 * dt * fy, vz = jz + (R)0.5 * dt * fz;
+    const R ux = vx - vx * q, uy = vy - vy * q, uz = vz - vz * q;
+    const R usq = ux * ux + uy * uy + uz * uz;
+    const R om = (R)1 / ((R)0.5 + (R)prm.visc_scale * (R)d[18]);
+    const R fp = dt * ((R)1 - (R)0.5 * om);
+    R lam[Q];
+#pragma unroll
+    for (int c = 0; c < Q; c++) lam[c] = (R)lin[(size_t)c * stride + s];
+    R add = 0, aux = 0, auy = 0, auz = 0, afx = 0, afy = 0, afz = 0;
+#pragma unroll
+    for (int c = 0; c < Q; c++) {
+      const R ex = (R)EX[c], ey = (R)EY[c], ez = (R)EZ[c];
+      const R cu = ex * ux + ey * uy + ez * uz;
+      const R cf = ex * fx + ey * fy + ez * fz;
+      const R X = (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq;
+      const R wl = wmul<R>(c, lam[c]);
+      add += om * wl * ((R)1 + X);
+      const R k1 = om * wl * rho, t1 = (R)3 + (R)9 * cu;
+      const R k2 = fp * wl, t2 = (R)9 * cf;
+      aux += k1 * (t1 * ex - (R)3 * ux) + k2 * (t2 * ex - (R)3 * fx);
+      auy += k1 * (t1 * ey - (R)3 * uy) + k2 * (t2 * ey - (R)3 * fy);
+      auz += k1 * (t1 * ez - (R)3 * uz) + k2 * (t2 * ez - (R)3 * fz);
+      const R t3 = (R)9 * cu;
+      afx += k2 * ((R)3 * (ex - ux) + t3 * ex);
+      afy += k2 * ((R)3 * (ey - uy) + t3 * ey);
+      afz += k2 * ((R)3 * (ez - uz) + t3 * ez);
+    }
+    R gix = 0, giy = 0, giz = 0;  // d(out)/d(inlet velocity) . lambda
+    if (w & kWordMacro) {
+      int lo = 0, hi = n_fused - 1, k = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1, v = fused_s[mid];
+        if (v == s) { k = mid; break; }
+        if (v < s) lo = mid + 1; else hi = mid - 1;
+      }
+      if (k >= 0)
+        for (int t = att_start[k]; t < att_start[k + 1]; t++) {
+          const BcEntry e = bc[t];
+          const bool wall = e.code == 0 || e.code == 4998;
+          const bool inlet = e.code == ZF_B_VELOCITY_INLET;
+          R drb = 0;
+          if (e.code >= 1001 && e.code <= 1009) drb = (R)bp.pdrho[e.code - 1001];
+          else if (e.code == ZF_B_PRESSURE_OUTLET) drb = (R)bp.outlet_drho;
+          const R ubx = inlet ? (R)bp.inlet[0] : ux, uby = inlet ? (R)bp.inlet[1] : uy,
+                  ubz = inlet ? (R)bp.inlet[2] : uz;
+          const R ubsq = ubx * ubx + uby * uby + ubz * ubz;
+#pragma unroll
+          for (int c = 0; c < Q; c++) {
+            const R wl = wmul<R>(c, (R)lin[(size_t)c * stride + e.s]);
+            const R ex = (R)EX[c], ey = (R)EY[c], ez = (R)EZ[c];
+            if (wall) {
+              add += wl;
+            } else if (inlet) {
+              const R cu = ex * ubx + ey * uby + ez * ubz;
+              add += wl * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * ubsq);
+              const R k1 = wl * rho, t1 = (R)3 + (R)9 * cu;
+              gix += k1 * (t1 * ex - (R)3 * ubx);
+              giy += k1 * (t1 * ey - (R)3 * uby);
+              giz += k1 * (t1 * ez - (R)3 * ubz);
+            } else {
+              const R cu = ex * ux + ey * uy + ez * uz;
+              const R k1 = wl * ((R)1 + drb), t1 = (R)3 + (R)9 * cu;
+              aux += k1 * (t1 * ex - (R)3 * ux);
+              auy += k1 * (t1 * ey - (R)3 * uy);
+              auz += k1 * (t1 * ez - (R)3 * uz);
+            }
+          }
+        }
+    }
+    // through u = v / (1 + drho)
+    const R avx = aux / rho, avy = auy / rho, avz = auz / rho;
+    const R adt = add - (avx * ux + avy * uy + avz * uz);
+    afx += (R)0.5 * dt * avx;
+    afy += (R)0.5 * dt * avy;
+    afz += (R)0.5 * dt * avz;
+#pragma unroll
+    for (int c = 0; c < Q; c++) {
+      const R mu = ((R)1 - om) * lam[c] + adt + (R)EX[c] * avx + (R)EY[c] * avy + (R)EZ[c] * avz;
+      lout[idx[c]] = (A)mu;
+    }
+    if (FIELD && gF) {
+      gF[s] += (double)afx;
+      gF[stride + s] += (double)afy;
+      gF[2 * stride + s] += (double)afz;
+    }
+    const double gdir = (double)gix * ap.dir[0] + (double)giy * ap.dir[1] + (double)giz * ap.dir[2];
+    red[0] = gdir * ap.u0 * ap.sin_phase;
+    red[1] = gdir * (1.0 + ap.amp * ap.sin_phase);
+    red[2] = afx;
+    red[3] = afy;
+    red[4] = afz;
+  }
+  block_sum(red, part);
+}
+
+// Boundary entries fed by a non-interior neighbour, one entry after the
+// other (entries may share a neighbour), one lane per direction.  The
+// forward took the neighbour's unshifted moments from S_{k-1}.
+template <int S>
+__global__ __launch_bounds__(64) void k_adjoint_late(
+    const typename Store<S>::T *__restrict__ src, const typename Adj<S>::A *__restrict__ lin,
+    typename Adj<S>::A *__restrict__ lout, size_t stride, const BcEntry *__restrict__ bc, int t0,
+    int t1, BcParams bp, AdjParams ap, double *__restrict__ part_row) {
+  using A = typename Adj<S>::A;
+  const int c = threadIdx.x;
+  const bool on = c < Q;
+  const double ex = on ? EX[c] : 0, ey = on ? EY[c] : 0, ez = on ? EZ[c] : 0, wc = on ? WQ[c] : 0;
+  double gacc = 0, gacc0 = 0;
+  for (int t = t0; t < t1; t++) {
+    const BcEntry e = bc[t];
+    const double gv = on ? (double)Store<S>::ld(src, (size_t)c * stride + e.sm, wc) : 0.0;
+    const double drho = wave_sum(gv);
+    const double rm = 1.0 + drho;
+    const double ux = wave_sum(ex * gv) / rm, uy = wave_sum(ey * gv) / rm, uz = wave_sum(ez * gv) / rm;
+    const double l = on ? (double)lin[(size_t)c * stride + e.s] : 0.0;
+    const double wl = wc * l;
+    double cdd = 0, cux = 0, cuy = 0, cuz = 0, cgx = 0, cgy = 0, cgz = 0;
+    if (e.code == 0 || e.code == 4998) {
+      cdd = wl;
+    } else if (e.code == ZF_B_VELOCITY_INLET) {
+      const double ubx = bp.inlet[0], uby = bp.inlet[1], ubz = bp.inlet[2];
+      const double cu = ex * ubx + ey * uby + ez * ubz;
+      cdd = wl * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * (ubx * ubx + uby * uby + ubz * ubz));
+      const double k1 = wl * rm, t1v = 3.0 + 9.0 * cu;
+      cgx = k1 * (t1v * ex - 3.0 * ubx);
+      cgy = k1 * (t1v * ey - 3.0 * uby);
+      cgz = k1 * (t1v * ez - 3.0 * ubz);
+    } else {
+      const double drb = (e.code >= 1001 && e.code <= 1009) ? bp.pdrho[e.code - 1001] : bp.outlet_drho;
+      const double cu = ex * ux + ey * uy + ez * uz;
+      const double k1 = wl * (1.0 + drb), t1v = 3.0 + 9.0 * cu;
+      cux = k1 * (t1v * ex - 3.0 * ux);
+      cuy = k1 * (t1v * ey - 3.0 * uy);
+      cuz = k1 * (t1v * ez - 3.0 * uz);
+    }
+    const double add = wave_sum(cdd);
+    const double aux = wave_sum(cux), auy = wave_sum(cuy), auz = wave_sum(cuz);
+    const double gx = wave_sum(cgx), gy = wave_sum(cgy), gz = wave_sum(cgz);
+    if (on) {
+      const double v = add + (aux * (ex - ux) + auy * (ey - uy) + auz * (ez - uz)) / rm;
+      const size_t o = (size_t)c * stride + e.sm;
+      lout[o] = (A)((double)lout[o] + v);
+    }
+    const double gdir = gx * ap.dir[0] + gy * ap.dir[1] + gz * ap.dir[2];
+    gacc += gdir * ap.u0 * ap.sin_phase;
+    gacc0 += gdir * (1.0 + ap.amp * ap.sin_phase);
+  }
+  if (c == 0) {
+    part_row[0] = gacc;
+    part_row[1] = gacc0;
+    part_row[2] = part_row[3] = part_row[4] = 0.0;
+  }
+}
+
+template <class A>
+__global__ void k_zero_idx(A *__restrict__ lam, const int64_t *__restrict__ idx, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lam[idx[i]] = (A)0;
+}
+
+template <class A>
+__global__ void k_fill_a(A *__restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (A)0;
+}
+
+// Marks every (c, s) an interior node pulls.
+__global__ void k_mark_pulled(size_t stride, int n_int, const uint32_t *__restrict__ word,
+                              const int32_t *__restrict__ block0, const int32_t *__restrict__ chords,
+                              uint8_t *__restrict__ mark) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_int) return;
+  const uint32_t w = word[s];
+  const int ch = block0[s >> 6] + (int)(w >> kWordChordShift);
+  const int32_t *d = chords + (size_t)ch * kChordInts;
+  for (int c = 0; c < 18; c++) {
+    const bool bb = (w >> c) & 1u;
+    mark[bb ? (size_t)(c ^ 1) * stride + s : (size_t)c * stride + (s + d[c])] = 1;
+  }
+  mark[(size_t)18 * stride + s] = 1;
+}
+
+// Stencil of every probe: up to kStencilMax (storage index, weight) pairs
+// per probe, count in cnt[m].
+constexpr int kStencilMax = 64;
+__global__ void k_stencil_table(const int32_t *__restrict__ b, const int32_t *__restrict__ sbox,
+                                WGrid g, int n, const double *__restrict__ pts,
+                                int32_t *__restrict__ cnt, int32_t *__restrict__ sidx,
+                                double *__restrict__ wgt) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double *x = pts + 3 * (size_t)m, *nv = pts + 3 * (size_t)n + 3 * (size_t)m;
+  int k = 0;
+  wss_stencil(g, b, sbox, x[0] + g.dx * nv[0], x[1] + g.dx * nv[1], x[2] + g.dx * nv[2],
+              [&](int sq, double w) {
+                if (k < kStencilMax) {
+                  sidx[(size_t)m * kStencilMax + k] = sq;
+                  wgt[(size_t)m * kStencilMax + k] = w;
+                }
+                k++;
+              });
+  cnt[m] = k;
+}
+
+template <int S>
+__device__ void probe_tau(const typename Store<S>::T *__restrict__ f, size_t stride,
+                          const int32_t *__restrict__ b, const int32_t *__restrict__ sbox,
+                          const WGrid &g, const double *x, const double *nv, double mu_over_dx,
+                          double tau[3]) {
+  double u0 = 0, u1 = 0, u2 = 0;
+  wss_stencil(g, b, sbox, x[0] + g.dx * nv[0], x[1] + g.dx * nv[1], x[2] + g.dx * nv[2],
+              [&](int sq, double w) {
+                double drho, jv[3];
+                node_moments<S>(f, stride, sq, drho, jv);
+                const double r = 1.0 + drho;
+                u0 += w * (jv[0] / r);
+                u1 += w * (jv[1] / r);
+                u2 += w * (jv[2] / r);
+              });
+  const double un = u0 * nv[0] + u1 * nv[1] + u2 * nv[2];
+  tau[0] = -mu_over_dx * (u0 - un * nv[0]);
+  tau[1] = -mu_over_dx * (u1 - un * nv[1]);
+  tau[2] = -mu_over_dx * (u2 - un * nv[2]);
+}
+
+template <int S>
+__global__ void k_obj_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
+                          const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, WGrid g,
+                          int n, const double *__restrict__ pts, double mu_over_dx,
+                          double *__restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double tau[3];
+  probe_tau<S>(f, stride, b, sbox, g, pts + 3 * (size_t)m, pts + 3 * (size_t)n + 3 * (size_t)m,
+               mu_over_dx, tau);
+  out[m] = sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
+}
+
+// dJ/dtau of each probe, times -(mu/DX): J gains scale |tau|, so
+// dJ/dtau = scale tau/|tau|, and dJ/du_q = -(mu/DX) w_q (I - n n) dJ/dtau,
+// which is w_q a_m with a_m = -(mu/DX) dJ/dtau (dJ/dtau is tangential).
+template <int S>
+__global__ void k_obj_wss_adj(const typename Store<S>::T *__restrict__ f, size_t stride,
+                              const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, WGrid g,
+                              int n, const double *__restrict__ pts, double mu_over_dx, double scale,
+                              double *__restrict__ a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double tau[3];
+  probe_tau<S>(f, stride, b, sbox, g, pts + 3 * (size_t)m, pts + 3 * (size_t)n + 3 * (size_t)m,
+               mu_over_dx, tau);
+  const double mag = sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
+  const double k = mag > 0 ? -mu_over_dx * scale / mag : 0.0;
+  a[3 * m] = k * tau[0];
+  a[3 * m + 1] = k * tau[1];
+  a[3 * m + 2] = k * tau[2];
+}
+
+// At each support node s (one lane each): dJ/du_s = sum over the probes
+// whose stencil holds s of w_ms a_m (a fixed order), then
+// lambda(s, c) += dJ/du_s . (c - u_s)/(1 + drho_s).
+template <int S>
+__global__ void k_obj_apply(const typename Store<S>::T *__restrict__ f, size_t stride,
+                            const int32_t *__restrict__ list, int nlist,
+                            const int32_t *__restrict__ off, const int32_t *__restrict__ probe,
+                            const double *__restrict__ weight, const double *__restrict__ a,
+                            typename Adj<S>::A *__restrict__ lam) {
+  using A = typename Adj<S>::A;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nlist) return;
+  const int s = list[i];
+  double d0 = 0, d1 = 0, d2 = 0;
+  for (int e = off[i]; e < off[i + 1]; e++) {
+    const double w = weight[e];
+    const double *am = a + 3 * (size_t)probe[e];
+    d0 += w * am[0];
+    d1 += w * am[1];
+    d2 += w * am[2];
+  }
+  double drho, jv[3];
+  node_moments<S>(f, stride, s, drho, jv);
+  const double r = 1.0 + drho, ux = jv[0] / r, uy = jv[1] / r, uz = jv[2] / r;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const double v = (d0 * (EX[c] - ux) + d1 * (EY[c] - uy) + d2 * (EZ[c] - uz)) / r;
+    const size_t o = (size_t)c * stride + s;
+    lam[o] = (A)((double)lam[o] + v);
+  }
+}
+
+// lambda (storage order) -> host layout, in double.
+template <class A>
+__global__ void k_export_lam(const A *__restrict__ lam, size_t stride, int n,
+                             const int32_t *__restrict__ s_of_p, double *__restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+  for (int c = 0; c < Q; c++) out[(size_t)c * n + p] = (double)lam[(size_t)c * stride + s];
+}
+
+template <int S>
+const typename Store<S>::T *st(const void *p) {
+  return static_cast<const typename Store<S>::T *>(p);
+}
+
+WGrid wgrid(const zf_ctx *ctx) {
+  const GridGeom gg = grid_geom(ctx);
+  return WGrid{gg.x0, gg.y0, gg.z0, gg.dx, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km};
+}
+
+template <class F>
+int with_store(int storage, F &&f) {
+  if (storage == ZF_STORE_F64) return f(std::integral_constant<int, ZF_STORE_F64>());
+  return f(std::integral_constant<int, ZF_STORE_F32DEV>());
+}
+
+void *state_at(zf_ctx *ctx, void *base, size_t i) {
+  return static_cast<char *>(base) + i * ctx->adj_state_bytes;
+}
+
+int require_adj(zf_ctx *ctx) {
+  if (!ctx->have_adj) return fail(ctx, ZF_ERR_STATE, "zf_adjoint_setup not called");
+  return ZF_OK;
+}
+
+// Objective terms |tau| of the current state, stored for iteration `it`.
+int obj_forward(zf_ctx *ctx, const void *f, int it) {
+  const zf_adjoint_config &a = ctx->acfg;
+  double *out = ctx->d_objbuf + (size_t)(it - a.t_begin) * ctx->adj_npts;
+  with_store(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    hipLaunchKernelGGL(k_obj_wss<S>, dim3(nb(ctx->adj_npts, 64)), dim3(64), 0, ctx->stream,
+                       st<S>(f), ctx->stride, ctx->d_b, ctx->d_sbox, wgrid(ctx), ctx->adj_npts,
+                       ctx->d_adj_pts, a.mu_over_dx, out);
+    return 0;
+  });
+  return ZF_OK;
+}
+
+// lambda += dJ/dS for state f (iteration `it` in the window).
+int obj_adjoint(zf_ctx *ctx, const void *f, void *lam) {
+  const zf_adjoint_config &a = ctx->acfg;
+  const double scale = 1.0 / ((double)(a.t_end - a.t_begin + 1) * ctx->adj_npts);
+  with_store(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using A = typename Adj<S>::A;
+    hipLaunchKernelGGL(k_obj_wss_adj<S>, dim3(nb(ctx->adj_npts, 64)), dim3(64), 0, ctx->stream,
+                       st<S>(f), ctx->stride, ctx->d_b, ctx->d_sbox, wgrid(ctx), ctx->adj_npts,
+                       ctx->d_adj_pts, a.mu_over_dx, scale, ctx->d_adj_a);
+    if (ctx->n_support > 0)
+      hipLaunchKernelGGL(k_obj_apply<S>, dim3(nb(ctx->n_support, 256)), dim3(256), 0, ctx->stream,
+                         st<S>(f), ctx->stride, ctx->d_support, ctx->n_support, ctx->d_sup_off,
+                         ctx->d_sup_probe, ctx->d_sup_w, ctx->d_adj_a, static_cast<A *>(lam));
+    return 0;
+  });
+  return ZF_OK;
+}
+
+// One adjoint iteration k: lout = A_k^T lin, given S_{k-1} = src.
+int adjoint_step(zf_ctx *ctx, const void *src, const void *lin, void *lout, int k) {
+  StepParams prm;
+  BcParams bp;
+  step_params(ctx, k, prm, bp);
+  const zf_config &c = ctx->cfg;
+  const double ph = 2.0 * 3.14159265358979323846 * k / (c.inlet_period > 0 ? c.inlet_period : 1.0);
+  const AdjParams ap{c.inlet_u0, c.inlet_amp, std::sin(ph), {c.inlet_dir[0], c.inlet_dir[1], c.inlet_dir[2]}};
+  const int nblk = (int)nb(ctx->n_int, 256);
+  const int remap = (c.flags & ZF_FLAG_LINEAR_BLOCKS) ? 0 : 1;
+  with_store(c.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using A = typename Adj<S>::A;
+    if (nblk > 0) {
+      if (ctx->d_force)
+        hipLaunchKernelGGL((k_adjoint<S, true>), dim3(nblk), dim3(256), 0, ctx->stream, st<S>(src),
+                           static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride,
+                           ctx->n_int, nblk, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
+                           ctx->d_force, prm, ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
+                           ctx->n_fused_nodes, bp, ap, ctx->d_gF, ctx->d_gpart);
+      else
+        hipLaunchKernelGGL((k_adjoint<S, false>), dim3(nblk), dim3(256), 0, ctx->stream, st<S>(src),
+                           static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride,
+                           ctx->n_int, nblk, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
+                           nullptr, prm, ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
+                           ctx->n_fused_nodes, bp, ap, nullptr, ctx->d_gpart);
+    }
+    if (ctx->n_unpulled > 0)
+      hipLaunchKernelGGL(k_zero_idx<A>, dim3(nb((size_t)ctx->n_unpulled, 256)), dim3(256), 0,
+                         ctx->stream, static_cast<A *>(lout), ctx->d_unpulled, ctx->n_unpulled);
+    int rows = nblk;
+    if (ctx->n_bc > ctx->n_bc_fused) {
+      hipLaunchKernelGGL(k_adjoint_late<S>, dim3(1), dim3(64), 0, ctx->stream, st<S>(src),
+                         static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride, ctx->d_bc,
+                         ctx->n_bc_fused, ctx->n_bc, bp, ap, ctx->d_gpart + 5 * (size_t)nblk);
+      rows++;
+    }
+    if (rows > 0)
+      hipLaunchKernelGGL(k_reduce_grads, dim3(1), dim3(64), 0, ctx->stream, ctx->d_gpart, rows,
+                         ctx->d_grads);
+    return 0;
+  });
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+}  // namespace
+
+void free_adjoint(zf_ctx *ctx) {
+  void **bufs[] = {(void **)&ctx->d_adj_pts, &ctx->d_ckpt, &ctx->d_seg, &ctx->d_lam[0],
+                   &ctx->d_lam[1], (void **)&ctx->d_adj_a, (void **)&ctx->d_support,
+                   (void **)&ctx->d_sup_off, (void **)&ctx->d_sup_probe, (void **)&ctx->d_sup_w,
+                   (void **)&ctx->d_gpart,
+                   (void **)&ctx->d_unpulled, (void **)&ctx->d_grads, (void **)&ctx->d_gF,
+                   (void **)&ctx->d_objbuf};
+  for (auto b : bufs) dev_free(ctx, b);
+  ctx->have_adj = ctx->adj_ready = false;
+  ctx->n_support = 0;
+  ctx->n_unpulled = 0;
+}
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+int zf_set_forcing(zf_ctx *ctx, double inlet_u0, double inlet_amp, const double body_force[3]) {
+  if (!ctx || !body_force) return ZF_ERR_ARG;
+  ctx->cfg.inlet_u0 = inlet_u0;
+  ctx->cfg.inlet_amp = inlet_amp;
+  for (int a = 0; a < 3; a++) ctx->cfg.body_force[a] = body_force[a];
+  return ZF_OK;
+}
+
+int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, const double *xyz,
+                     const double *normal) {
+  if (!ctx || !ac || npts <= 0 || !xyz || !normal) return ZF_ERR_ARG;
+  if (ac->checkpoint_interval < 1 || ac->max_steps < 1 || ac->t_begin > ac->t_end || npts > (1 << 24))
+    return fail(ctx, ZF_ERR_ARG, "bad adjoint configuration");
+  if (!ctx->have_lattice || !ctx->d_f[0]) return fail(ctx, ZF_ERR_STATE, "no lattice uploaded");
+  if (ctx->cfg.storage == ZF_STORE_F16DEV)
+    return fail(ctx, ZF_ERR_ARG, "the adjoint supports FP64 and FP32 storage only");
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  free_adjoint(ctx);
+  ctx->acfg = *ac;
+  ctx->adj_npts = (int32_t)npts;
+  const size_t stride = ctx->stride;
+  const size_t eb = store_bytes(ctx->cfg.storage);
+  const size_t ab = ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4;
+  ctx->adj_state_bytes = eb * Q * stride;
+  const int K = ac->checkpoint_interval;
+  const size_t nck = (size_t)(ac->max_steps + K - 1) / K;
+  const size_t nobj = (size_t)(ac->t_end - ac->t_begin + 1) * npts;
+  int rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_adj_pts, 48 * (size_t)npts)) ||
+      (rc = dev_alloc(ctx, &ctx->d_ckpt, nck * ctx->adj_state_bytes)) ||
+      (rc = dev_alloc(ctx, &ctx->d_seg, (size_t)std::max(K - 1, 1) * ctx->adj_state_bytes)) ||
+      (rc = dev_alloc(ctx, &ctx->d_lam[0], ab * Q * stride)) ||
+      (rc = dev_alloc(ctx, &ctx->d_lam[1], ab * Q * stride)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_adj_a, 24 * (size_t)npts)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_gpart, 40 * ((size_t)nb(ctx->n_int, 256) + 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_grads, 8 * 8)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_objbuf, 8 * nobj))) {
+    free_adjoint(ctx);
+    return rc;
+  }
+  if (ctx->d_force && (rc = dev_alloc(ctx, (void **)&ctx->d_gF, 24 * stride))) {
+    free_adjoint(ctx);
+    return rc;
+  }
+  ZF_HIP(ctx, hipMemcpy(ctx->d_adj_pts, xyz, 24 * (size_t)npts, hipMemcpyHostToDevice));
+  ZF_HIP(ctx, hipMemcpy(ctx->d_adj_pts + 3 * npts, normal, 24 * (size_t)npts, hipMemcpyHostToDevice));
+  // pulled entries and probe support, marked on the device, listed here
+  uint8_t *mark = nullptr;
+  if ((rc = dev_alloc(ctx, (void **)&mark, Q * stride))) {
+    free_adjoint(ctx);
+    return rc;
+  }
+  ZF_HIP(ctx, hipMemsetAsync(mark, 0, Q * stride, ctx->stream));
+  if (ctx->n_int > 0)
+    hipLaunchKernelGGL(k_mark_pulled, dim3(nb(ctx->n_int, 256)), dim3(256), 0, ctx->stream, stride,
+                       ctx->n_int, ctx->d_word, ctx->d_block_chord0, ctx->d_chords, mark);
+  std::vector<uint8_t> hm(Q * stride);
+  ZF_HIP(ctx, hipGetLastError());
+  // the context stream does not synchronise with hipMemcpy's stream
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ZF_HIP(ctx, hipMemcpy(hm.data(), mark, Q * stride, hipMemcpyDeviceToHost));
+  std::vector<BcEntry> hbc(ctx->n_bc);
+  if (ctx->n_bc)
+    ZF_HIP(ctx, hipMemcpy(hbc.data(), ctx->d_bc, sizeof(BcEntry) * ctx->n_bc, hipMemcpyDeviceToHost));
+  // entries the adjoint reads: interior nodes, boundary nodes, late neighbours
+  std::vector<uint8_t> read(stride, 0);
+  for (int s = 0; s < ctx->n_int; s++) read[s] = 1;
+  for (int t = 0; t < ctx->n_bc; t++) {
+    read[hbc[t].s] = 1;
+    if (t >= ctx->n_bc_fused) read[hbc[t].sm] = 1;
+  }
+  std::vector<int64_t> unpulled;
+  for (int c = 0; c < Q; c++)
+    for (size_t s = 0; s < stride; s++)
+      if (read[s] && !hm[(size_t)c * stride + s]) unpulled.push_back((int64_t)(c * stride + s));
+  ctx->device_bytes -= (int64_t)(Q * stride);
+  dev_free(ctx, (void **)&mark);
+  // probe stencils -> per support node (probe, weight) lists, probe order
+  int32_t *tcnt = nullptr, *tidx = nullptr;
+  double *tw = nullptr;
+  const size_t tslots = (size_t)npts * kStencilMax;
+  if ((rc = dev_alloc(ctx, (void **)&tcnt, 4 * (size_t)npts)) ||
+      (rc = dev_alloc(ctx, (void **)&tidx, 4 * tslots)) || (rc = dev_alloc(ctx, (void **)&tw, 8 * tslots))) {
+    dev_free(ctx, (void **)&tcnt);
+    dev_free(ctx, (void **)&tidx);
+    free_adjoint(ctx);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_stencil_table, dim3(nb((size_t)npts, 64)), dim3(64), 0, ctx->stream, ctx->d_b,
+                     ctx->d_sbox, wgrid(ctx), (int)npts, ctx->d_adj_pts, tcnt, tidx, tw);
+  std::vector<int32_t> hcnt(npts), hidx(tslots);
+  std::vector<double> hw(tslots);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = hipMemcpy(hcnt.data(), tcnt, 4 * (size_t)npts, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(hidx.data(), tidx, 4 * tslots, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(hw.data(), tw, 8 * tslots, hipMemcpyDeviceToHost);
+  ctx->device_bytes -= (int64_t)(4 * (size_t)npts + 12 * tslots);
+  dev_free(ctx, (void **)&tcnt);
+  dev_free(ctx, (void **)&tidx);
+  dev_free(ctx, (void **)&tw);
+  if (e != hipSuccess) {
+    free_adjoint(ctx);
+    return fail(ctx, ZF_ERR_HIP, "adjoint stencil table: %s", hipGetErrorString(e));
+  }
+  std::vector<std::vector<std::pair<int32_t, double>>> per(stride);
+  for (int64_t m = 0; m < npts; m++) {
+    if (hcnt[m] > kStencilMax) {
+      free_adjoint(ctx);
+      return fail(ctx, ZF_ERR_ARG, "probe %lld has %d stencil nodes (> %d)", (long long)m, hcnt[m],
+                  kStencilMax);
+    }
+    for (int k = 0; k < hcnt[m]; k++)
+      per[hidx[(size_t)m * kStencilMax + k]].push_back({(int32_t)m, hw[(size_t)m * kStencilMax + k]});
+  }
+  std::vector<int32_t> support, off(1, 0), sprobe;
+  std::vector<double> sw;
+  for (size_t q = 0; q < stride; q++)
+    if (!per[q].empty()) {
+      support.push_back((int32_t)q);
+      for (auto &pw : per[q]) {
+        sprobe.push_back(pw.first);
+        sw.push_back(pw.second);
+      }
+      off.push_back((int32_t)sprobe.size());
+    }
+  ctx->n_unpulled = (int64_t)unpulled.size();
+  ctx->n_support = (int32_t)support.size();
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_unpulled, 8 * std::max<size_t>(unpulled.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_support, 4 * std::max<size_t>(support.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_off, 4 * off.size())) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_probe, 4 * std::max<size_t>(sprobe.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_w, 8 * std::max<size_t>(sw.size(), 1)))) {
+    free_adjoint(ctx);
+    return rc;
+  }
+  ZF_HIP(ctx, hipMemcpy(ctx->d_sup_off, off.data(), 4 * off.size(), hipMemcpyHostToDevice));
+  if (!sprobe.empty()) {
+    ZF_HIP(ctx, hipMemcpy(ctx->d_sup_probe, sprobe.data(), 4 * sprobe.size(), hipMemcpyHostToDevice));
+    ZF_HIP(ctx, hipMemcpy(ctx->d_sup_w, sw.data(), 8 * sw.size(), hipMemcpyHostToDevice));
+  }
+  if (!unpulled.empty())
+    ZF_HIP(ctx, hipMemcpy(ctx->d_unpulled, unpulled.data(), 8 * unpulled.size(), hipMemcpyHostToDevice));
+  if (!support.empty())
+    ZF_HIP(ctx, hipMemcpy(ctx->d_support, support.data(), 4 * support.size(), hipMemcpyHostToDevice));
+  ctx->have_adj = true;
+  return ZF_OK;
+}
+
+int zf_adjoint_forward(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double *objective) {
+  if (!ctx || nsteps < 1) return ZF_ERR_ARG;
+  int rc = require_adj(ctx);
+  if (rc) return rc;
+  const zf_adjoint_config &a = ctx->acfg;
+  if (nsteps > a.max_steps) return fail(ctx, ZF_ERR_ARG, "nsteps %d > max_steps %d", nsteps, a.max_steps);
+  if (a.t_begin <= iter0 || a.t_end > iter0 + nsteps)
+    return fail(ctx, ZF_ERR_ARG, "objective window [%d, %d] outside iterations %d..%d", a.t_begin,
+                a.t_end, iter0 + 1, iter0 + nsteps);
+  const int K = a.checkpoint_interval;
+  ZF_HIP(ctx, hipEventRecord(ctx->events[12], ctx->stream));
+  for (int k = iter0 + 1; k <= iter0 + nsteps; k++) {
+    const int r = k - 1 - iter0;  // state S_{k-1}
+    if (r % K == 0)
+      ZF_HIP(ctx, hipMemcpyAsync(state_at(ctx, ctx->d_ckpt, r / K), ctx->d_f[ctx->cur],
+                                 ctx->adj_state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = zf_lbm_step(ctx, k, 0))) return rc;
+    if (k >= a.t_begin && k <= a.t_end) obj_forward(ctx, ctx->d_f[ctx->cur], k);
+  }
+  ZF_HIP(ctx, hipEventRecord(ctx->events[13], ctx->stream));
+  ZF_HIP(ctx, hipGetLastError());
+  const size_t nobj = (size_t)(a.t_end - a.t_begin + 1) * ctx->adj_npts;
+  std::vector<double> h(nobj);
+  ZF_HIP(ctx, hipMemcpyAsync(h.data(), ctx->d_objbuf, 8 * nobj, hipMemcpyDeviceToHost, ctx->stream));
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ZF_HIP(ctx, hipEventElapsedTime(&ctx->adj_ms[0], ctx->events[12], ctx->events[13]));
+  double sum = 0;
+  for (double v : h) sum += v;
+  if (objective) *objective = sum / (double)nobj;
+  ctx->adj_iter0 = iter0;
+  ctx->adj_nsteps = nsteps;
+  ctx->adj_ready = true;
+  return ZF_OK;
+}
+
+int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
+  if (!ctx || !grad) return ZF_ERR_ARG;
+  int rc = require_adj(ctx);
+  if (rc) return rc;
+  if (!ctx->adj_ready) return fail(ctx, ZF_ERR_STATE, "zf_adjoint_forward not run");
+  const zf_adjoint_config &a = ctx->acfg;
+  const int K = a.checkpoint_interval, iter0 = ctx->adj_iter0, N = ctx->adj_nsteps;
+  const size_t stride = ctx->stride, ab = ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4;
+  ZF_HIP(ctx, hipEventRecord(ctx->events[12], ctx->stream));
+  hipLaunchKernelGGL(k_fill_a<double>, dim3(1), dim3(64), 0, ctx->stream, ctx->d_grads, (size_t)8);
+  if (ctx->d_gF)
+    hipLaunchKernelGGL(k_fill_a<double>, dim3(nb(3 * stride, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_gF, 3 * stride);
+  for (int b = 0; b < 2; b++)
+    with_store(ctx->cfg.storage, [&](auto tag) {
+      constexpr int S = decltype(tag)::value;
+      using A = typename Adj<S>::A;
+      hipLaunchKernelGGL(k_fill_a<A>, dim3(nb(Q * stride, 256)), dim3(256), 0, ctx->stream,
+                         static_cast<A *>(ctx->d_lam[b]), Q * stride);
+      return 0;
+    });
+  (void)ab;
+  int cur = 0;
+  // lambda_N = dJ/dS_N
+  if (iter0 + N >= a.t_begin && iter0 + N <= a.t_end) obj_adjoint(ctx, ctx->d_f[ctx->cur], ctx->d_lam[cur]);
+  const int nseg = (N + K - 1) / K;
+  for (int j = nseg - 1; j >= 0; j--) {
+    // segment j: iterations iter0 + jK + 1 .. iter0 + min((j+1)K, N), which
+    // read S_{iter0+jK} (checkpoint j) .. S_{iter0+min((j+1)K,N)-1}
+    const int len = std::min(K, N - j * K);
+    const void *ck = state_at(ctx, ctx->d_ckpt, j);
+    auto state = [&](int r) -> const void * {  // S_{iter0 + jK + r}
+      return r == 0 ? ck : state_at(ctx, ctx->d_seg, r - 1);
+    };
+    for (int r = 1; r < len; r++)
+      if ((rc = step_into(ctx, state(r - 1), state_at(ctx, ctx->d_seg, r - 1), iter0 + j * K + r)))
+        return rc;
+    for (int r = len; r >= 1; r--) {
+      const int k = iter0 + j * K + r;  // iteration k reads S_{k-1} = state(r - 1)
+      if ((rc = adjoint_step(ctx, state(r - 1), ctx->d_lam[cur], ctx->d_lam[cur ^ 1], k))) return rc;
+      cur ^= 1;
+      if (k - 1 >= a.t_begin && k - 1 <= a.t_end) obj_adjoint(ctx, state(r - 1), ctx->d_lam[cur]);
+    }
+  }
+  ctx->adj_cur = cur;
+  ZF_HIP(ctx, hipEventRecord(ctx->events[13], ctx->stream));
+  ZF_HIP(ctx, hipGetLastError());
+  double g[8];
+  ZF_HIP(ctx, hipMemcpyAsync(g, ctx->d_grads, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ZF_HIP(ctx, hipEventElapsedTime(&ctx->adj_ms[1], ctx->events[12], ctx->events[13]));
+  for (int i = 0; i < 5; i++) grad[i] = g[i];
+  return ZF_OK;
+}
+
+int zf_adjoint_download(zf_ctx *ctx, double *dJ_df0, double *dJ_dforce) {
+  if (!ctx) return ZF_ERR_ARG;
+  int rc = require_adj(ctx);
+  if (rc) return rc;
+  const int n = ctx->dommax;
+  if (dJ_df0 && n > 0) {
+    double *tmp = nullptr;
+    if ((rc = dev_alloc(ctx, (void **)&tmp, 8 * (size_t)Q * n))) return rc;
+    with_store(ctx->cfg.storage, [&](auto tag) {
+      constexpr int S = decltype(tag)::value;
+      using A = typename Adj<S>::A;
+      hipLaunchKernelGGL(k_export_lam<A>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream,
+                         static_cast<const A *>(ctx->d_lam[ctx->adj_cur]), ctx->stride, n,
+                         ctx->d_s_of_p, tmp);
+      return 0;
+    });
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(dJ_df0, tmp, 8 * (size_t)Q * n, hipMemcpyDeviceToHost);
+    ctx->device_bytes -= 8 * (int64_t)Q * n;
+    dev_free(ctx, (void **)&tmp);
+    if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "adjoint download: %s", hipGetErrorString(e));
+  }
+  if (dJ_dforce && n > 0) {
+    std::vector<double> g(3 * ctx->stride, 0.0);
+    std::vector<int32_t> sp(n);
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->d_gF)
+      ZF_HIP(ctx, hipMemcpy(g.data(), ctx->d_gF, 24 * ctx->stride, hipMemcpyDeviceToHost));
+    ZF_HIP(ctx, hipMemcpy(sp.data(), ctx->d_s_of_p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int p = 0; p < n; p++)
+      for (int a = 0; a < 3; a++) dJ_dforce[(size_t)a * n + p] = g[(size_t)a * ctx->stride + sp[p]];
+  }
+  return ZF_OK;
+}
+
+int zf_adjoint_timing(zf_ctx *ctx, double ms[2]) {
+  if (!ctx || !ms) return ZF_ERR_ARG;
+  ms[0] = ctx->adj_ms[0];
+  ms[1] = ctx->adj_ms[1];
+  return ZF_OK;
+}
+
+}  // extern "C"

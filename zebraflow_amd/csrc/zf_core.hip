@@ -100,6 +100,7 @@ int zf_destroy(zf_ctx *ctx) {
   if (!ctx) return ZF_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_adjoint(ctx);
   free_cells(ctx);
   free_ibm(ctx);
   free_membrane(ctx);

@@ -72,7 +72,34 @@ __device__ __forceinline__ R geq(int c, R drho, R ux, R uy, R uz, R usq) {
   return wmul<R>(c, drho + x + drho * x);
 }
 
+// Workgroup order: the hardware deals workgroups round-robin over the 8
+// XCDs; with `remap`, XCD x runs one contiguous range of node blocks, so the
+// chord boundaries a block shares with its neighbours stay in one L2.
+__device__ __forceinline__ int logical_block(int b, int nblk, bool remap) {
+  if (!remap) return b;
+  const int q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+struct StepParams {
+  double visc_scale;  // tau = 0.5 + visc_scale*B
+  double dt;          // DT of the Guo terms
+  double g[3];        // uniform body force
+};
+
+struct BcParams {
+  double inlet[3];        // inlet velocity vector at this iteration
+  double outlet_drho;     // outlet density - 1
+  double pdrho[9];        // density - 1 of codes 1001..1009 at this iteration
+};
+
 }  // namespace d3q19
+
+// Parameters of iteration `iter` (inlet velocity, pressure waveforms).
+void step_params(const zf_ctx *ctx, int iter, d3q19::StepParams &prm, d3q19::BcParams &bp);
+// One stream-collide launch from src to dst (no buffer swap, force kept).
+int step_into(zf_ctx *ctx, const void *src, void *dst, int iter);
+
 }  // namespace zf
 
 #endif

@@ -146,6 +146,30 @@ struct zf_ctx {
   int32_t *d_wtri = nullptr, *d_wnode_tri = nullptr;  // 3 x nwt (1-based), 13 x nwn
   int32_t *d_windex = nullptr;    // box
   float *d_wdist = nullptr;       // box: FP32 distance to that wall node (or +inf)
+
+  // adjoint of the LBM step with a wall-shear-stress objective
+  bool have_adj = false, adj_ready = false;  // set up / forward recorded
+  zf_adjoint_config acfg{};
+  int32_t adj_npts = 0, adj_iter0 = 0, adj_nsteps = 0;
+  size_t adj_state_bytes = 0;
+  double *d_adj_pts = nullptr;     // 6 x npts: xyz then unit normals
+  void *d_ckpt = nullptr;          // checkpoints, one state every K iterations
+  void *d_seg = nullptr;           // K - 1 recomputed states of one segment
+  void *d_lam[2] = {nullptr, nullptr};  // adjoint populations (ping-pong)
+  int adj_cur = 0;
+  double *d_adj_a = nullptr;       // 3 x npts: -(mu/DX) dJ/dtau of each probe
+  int32_t *d_support = nullptr;    // storage indices in any probe stencil
+  int32_t *d_sup_off = nullptr;    // n_support + 1: CSR over (probe, weight) pairs
+  int32_t *d_sup_probe = nullptr;
+  double *d_sup_w = nullptr;
+  int32_t n_support = 0;
+  double *d_gpart = nullptr;       // 5 per workgroup (+1 row): gradient partial sums
+  int64_t *d_unpulled = nullptr;   // (c*stride + s) read by the adjoint but pulled by no node
+  int64_t n_unpulled = 0;
+  double *d_grads = nullptr;       // dJ/d(inlet_amp, inlet_u0, g_x, g_y, g_z)
+  double *d_gF = nullptr;          // 3 x stride: dJ/d(force field), when a field exists
+  double *d_objbuf = nullptr;      // |tau| per objective iteration and probe
+  float adj_ms[2] = {0, 0};        // forward, backward
 };
 
 namespace zf {
@@ -168,6 +192,7 @@ void free_state(zf_ctx *ctx);
 void free_membrane(zf_ctx *ctx);
 void free_ibm(zf_ctx *ctx);
 void free_cells(zf_ctx *ctx);
+void free_adjoint(zf_ctx *ctx);
 int membrane_forces_impl(zf_ctx *ctx);
 int cell_list_impl(zf_ctx *ctx);
 int repulsion_impl(zf_ctx *ctx);

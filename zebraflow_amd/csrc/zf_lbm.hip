@@ -26,20 +26,10 @@ namespace {
 
 using namespace d3q19;
 
-struct StepParams {
-  double visc_scale;  // tau = 0.5 + visc_scale*B
-  double dt;          // DT of the Guo terms
-  double g[3];        // uniform body force
-};
 
 // ---------------------------------------------------------------------------
 // Boundary values, ZebraflowWT.c:6294-6433: all 19 populations of a BC node
 // set to feq(D_bc, U_bc) from its neighbour's rho, u.
-struct BcParams {
-  double inlet[3];        // inlet velocity vector at this iteration
-  double outlet_drho;     // outlet density - 1
-  double pdrho[9];        // density - 1 of codes 1001..1009 at this iteration
-};
 
 template <int S>
 __device__ __forceinline__ void write_bc(typename Store<S>::T *__restrict__ dst, size_t stride,
@@ -66,14 +56,6 @@ __device__ __forceinline__ void write_bc(typename Store<S>::T *__restrict__ dst,
   }
 }
 
-// Workgroup order: the hardware deals workgroups round-robin over the 8
-// XCDs; with `remap`, XCD x runs one contiguous range of node blocks, so the
-// chord boundaries a block shares with its neighbours stay in one L2.
-__device__ __forceinline__ int logical_block(int b, int nblk, bool remap) {
-  if (!remap) return b;
-  const int q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
 
 // Stream + collide, one interior node per lane, with the boundary nodes of
 // the same step fused in.  The pull source of link c is s + delta_c(chord)
@@ -361,17 +343,11 @@ void launch_stream_opts(zf_ctx *ctx, const void *src, void *dst, const StepParam
 // (optional): events recorded before and after the launch.
 template <int S>
 int step_impl(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
-  const zf_config &c = ctx->cfg;
   const void *src = ctx->d_f[ctx->cur];
   void *dst = ctx->d_f[ctx->cur ^ 1];
-  StepParams prm{c.visc_scale, c.force_dt, {c.body_force[0], c.body_force[1], c.body_force[2]}};
-  BcParams bp{};
-  const double ph = 2.0 * 3.14159265358979323846 * iter / (c.inlet_period > 0 ? c.inlet_period : 1.0);
-  const double uin = c.inlet_u0 * (1.0 + c.inlet_amp * std::sin(ph));
-  for (int a = 0; a < 3; a++) bp.inlet[a] = uin * c.inlet_dir[a];
-  bp.outlet_drho = c.outlet_rho - 1.0;
-  for (int k = 0; k < 9; k++)
-    bp.pdrho[k] = bc_pressure(1001 + k, iter * c.t_scale) / c.pressure_scale;
+  StepParams prm;
+  BcParams bp;
+  step_params(ctx, iter, prm, bp);
   if (ev) ZF_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
   if (ctx->d_force) launch_stream_opts<S, true>(ctx, src, dst, prm, bp);
   else launch_stream_opts<S, false>(ctx, src, dst, prm, bp);
@@ -422,6 +398,15 @@ double bc_pressure(int code, double simtime) {
 }
 
 template <class F>
+int dispatch_store(int storage, F &&f) {
+  switch (storage) {
+    case ZF_STORE_F64: return f(std::integral_constant<int, ZF_STORE_F64>());
+    case ZF_STORE_F32DEV: return f(std::integral_constant<int, ZF_STORE_F32DEV>());
+    default: return f(std::integral_constant<int, ZF_STORE_F16DEV>());
+  }
+}
+
+template <class F>
 int dispatch(int storage, F &&f) {
   switch (storage) {
     case ZF_STORE_F64: return f(std::integral_constant<int, ZF_STORE_F64>());
@@ -436,6 +421,32 @@ int require_state(zf_ctx *ctx) {
 }
 
 }  // namespace
+
+void step_params(const zf_ctx *ctx, int iter, StepParams &prm, BcParams &bp) {
+  const zf_config &c = ctx->cfg;
+  prm = StepParams{c.visc_scale, c.force_dt, {c.body_force[0], c.body_force[1], c.body_force[2]}};
+  bp = BcParams{};
+  const double ph = 2.0 * 3.14159265358979323846 * iter / (c.inlet_period > 0 ? c.inlet_period : 1.0);
+  const double uin = c.inlet_u0 * (1.0 + c.inlet_amp * std::sin(ph));
+  for (int a = 0; a < 3; a++) bp.inlet[a] = uin * c.inlet_dir[a];
+  bp.outlet_drho = c.outlet_rho - 1.0;
+  for (int k = 0; k < 9; k++)
+    bp.pdrho[k] = bc_pressure(1001 + k, iter * c.t_scale) / c.pressure_scale;
+}
+
+int step_into(zf_ctx *ctx, const void *src, void *dst, int iter) {
+  StepParams prm;
+  BcParams bp;
+  step_params(ctx, iter, prm, bp);
+  dispatch_store(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    if (ctx->d_force) launch_stream_opts<S, true>(ctx, src, dst, prm, bp);
+    else launch_stream_opts<S, false>(ctx, src, dst, prm, bp);
+    return 0;
+  });
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
 
 size_t store_bytes(int storage) {
   return storage == ZF_STORE_F64 ? 8 : storage == ZF_STORE_F32DEV ? 4 : 2;

@@ -413,6 +413,7 @@ static int build_kernel_tables(zf_ctx *ctx, const int32_t *b, const std::vector<
 }
 
 int upload_lattice_impl(zf_ctx *ctx, const int32_t *b) {
+  free_adjoint(ctx);
   free_cells(ctx);  // cell-list and IBM box tables belong to the old lattice
   free_ibm(ctx);
   free_state(ctx);

@@ -616,6 +616,7 @@ int zf_download_cells(zf_ctx *ctx, double *xyz, double *vel, int32_t *status) {
   if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
   int rc;
   if ((rc = export_soa(ctx, ctx->d_x, xyz)) || (rc = export_soa(ctx, ctx->d_v, vel))) return rc;
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (status && ctx->ncell)
     ZF_HIP(ctx, hipMemcpy(status, ctx->d_status, 4 * (size_t)ctx->ncell, hipMemcpyDeviceToHost));
   return ZF_OK;

@@ -13,7 +13,7 @@
 // one side only; indices outside the box are skipped).  One lane per point,
 // FP64 throughout.  The velocities are the unshifted macro velocities of the
 // listed nodes (ZebraflowWT.c:6606-6614), the ones the reference writes out.
-#include "zf_d3q19.h"
+#include "zf_wss_dev.h"
 
 #include <cmath>
 
@@ -21,49 +21,9 @@ namespace zf {
 namespace {
 
 using namespace d3q19;
+using namespace wssdev;
 
 inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
-
-struct WGrid {
-  double x0, y0, z0, dx;
-  int im, jm, km;
-};
-
-__device__ __forceinline__ size_t wq(const WGrid &g, int i, int j, int k) {
-  return ((size_t)(i - 1) * g.jm + (j - 1)) * g.km + (k - 1);
-}
-
-__device__ __forceinline__ void wss_window(const WGrid &g, double x, double y, double z, int lo[3],
-                                           int hi[3]) {
-  const int ii = (int)((x - g.x0) / g.dx + 1.01), jj = (int)((y - g.y0) / g.dx + 1.01),
-            kk = (int)((z - g.z0) / g.dx + 1.01);
-  lo[0] = ii - 3; hi[0] = ii + 3;
-  if (hi[0] > g.im) hi[0] = g.im;
-  else if (lo[0] < 3) lo[0] = 3;
-  lo[1] = jj - 3; hi[1] = jj + 3;
-  if (hi[1] > g.jm) hi[1] = g.jm;
-  else if (lo[1] < 1) lo[1] = 1;
-  lo[2] = kk - 3; hi[2] = kk + 3;
-  if (hi[2] > g.km) hi[2] = g.km;
-  else if (lo[2] < 1) lo[2] = 1;
-  lo[0] = max(lo[0], 1); lo[1] = max(lo[1], 1); lo[2] = max(lo[2], 1);
-}
-
-// drho and momentum of one listed node from its stored populations.
-template <int S>
-__device__ __forceinline__ void moments(const typename Store<S>::T *__restrict__ f, size_t stride,
-                                        int s, double &drho, double j[3]) {
-  drho = 0;
-  j[0] = j[1] = j[2] = 0;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
-    const double g = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]);
-    drho += g;
-    j[0] += EX[c] * g;
-    j[1] += EY[c] * g;
-    j[2] += EZ[c] * g;
-  }
-}
 
 template <int S>
 __global__ void k_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
@@ -92,35 +52,22 @@ __global__ void k_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
       }
   if (best != ~(size_t)0) {
     double drho, jm[3];
-    moments<S>(f, stride, sbox[best], drho, jm);
+    node_moments<S>(f, stride, sbox[best], drho, jm);
     valid[m] = 1;
     press[m] = drho * pressure_scale - pgauge;
   } else {
     valid[m] = 0;
     press[m] = 0.0;
   }
-  const double xp = x + g.dx * nx, yp = y + g.dx * ny, zp = z + g.dx * nz;
-  const double width = 2.0 * g.dx, pi = 3.14159265358979;
-  wss_window(g, xp, yp, zp, lo, hi);
   double u0 = 0, u1 = 0, u2 = 0;
-  for (int i = lo[0]; i <= hi[0]; i++)
-    for (int j = lo[1]; j <= hi[1]; j++)
-      for (int k = lo[2]; k <= hi[2]; k++) {
-        const double ax = xp - (g.x0 + (i - 1) * g.dx), ay = yp - (g.y0 + (j - 1) * g.dx),
-                     az = zp - (g.z0 + (k - 1) * g.dx);
-        if (sqrt(ax * ax + ay * ay + az * az) > width) continue;
-        const size_t q = wq(g, i, j, k);
-        const int bv = b[q];
-        if (!(bv >= 1 && bv <= 4996)) continue;
-        const double w = (1.0 + cos(pi * fabs(ax) / width)) * (1.0 + cos(pi * fabs(ay) / width)) *
-                         (1.0 + cos(pi * fabs(az) / width)) / 64.0;
-        double drho, jv[3];
-        moments<S>(f, stride, sbox[q], drho, jv);
-        const double r = 1.0 + drho;
-        u0 += w * (jv[0] / r);
-        u1 += w * (jv[1] / r);
-        u2 += w * (jv[2] / r);
-      }
+  wss_stencil(g, b, sbox, x + g.dx * nx, y + g.dx * ny, z + g.dx * nz, [&](int sq, double w) {
+    double drho, jv[3];
+    node_moments<S>(f, stride, sq, drho, jv);
+    const double r = 1.0 + drho;
+    u0 += w * (jv[0] / r);
+    u1 += w * (jv[1] / r);
+    u2 += w * (jv[2] / r);
+  });
   const double un = u0 * nx + u1 * ny + u2 * nz;
   wss[3 * m] = -mu_over_dx * (u0 - un * nx);
   wss[3 * m + 1] = -mu_over_dx * (u1 - un * ny);

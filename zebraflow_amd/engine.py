@@ -283,6 +283,42 @@ class Engine:
                                     _ptr(out), _ptr(pr), _ptr(va)), "zf_wss")
         return out, pr, va
 
+    # -- adjoint ------------------------------------------------------------------
+    def adjoint_setup(self, xyz, normal, mu_over_dx, t_begin, t_end, max_steps, checkpoint=100):
+        from ._abi import ZfAdjointConfig
+        ac = ZfAdjointConfig(checkpoint, max_steps, t_begin, t_end, mu_over_dx)
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        normal = np.ascontiguousarray(normal, np.float64)
+        self._check(self.lib.zf_adjoint_setup(self.h, C.byref(ac), len(xyz), _ptr(xyz), _ptr(normal)),
+                    "zf_adjoint_setup")
+
+    def adjoint_forward(self, iter0, nsteps):
+        j = C.c_double()
+        self._check(self.lib.zf_adjoint_forward(self.h, iter0, nsteps, C.byref(j)), "zf_adjoint_forward")
+        return j.value
+
+    def adjoint_backward(self):
+        """dJ/d(inlet_amp, inlet_u0, body_force x, y, z)."""
+        g = np.zeros(5)
+        self._check(self.lib.zf_adjoint_backward(self.h, _ptr(g)), "zf_adjoint_backward")
+        return g
+
+    def adjoint_download(self):
+        """(dJ/df0 [19, dommax], dJ/dforce [3, dommax])."""
+        lam = np.zeros((19, self.dommax))
+        gf = np.zeros((3, self.dommax))
+        self._check(self.lib.zf_adjoint_download(self.h, _ptr(lam), _ptr(gf)), "zf_adjoint_download")
+        return lam, gf
+
+    def adjoint_timing(self):
+        ms = np.zeros(2)
+        self._check(self.lib.zf_adjoint_timing(self.h, _ptr(ms)), "zf_adjoint_timing")
+        return ms
+
+    def set_forcing(self, inlet_u0, inlet_amp, body_force=(0.0, 0.0, 0.0)):
+        g = np.ascontiguousarray(body_force, np.float64)
+        self._check(self.lib.zf_set_forcing(self.h, inlet_u0, inlet_amp, _ptr(g)), "zf_set_forcing")
+
     def copy_bandwidth(self, nbytes, reps=20):
         g = C.c_double()
         self._check(self.lib.zf_copy_bandwidth(self.h, int(nbytes), reps, C.byref(g)),
