@@ -197,6 +197,62 @@ def cells_leg(name, b, ncell, steps, warmup):
                 cells_live_after=int((st > 0).sum()), rbc_interval=1)
 
 
+def adjoint_leg(steps=3000, t0=1000, checkpoint=100):
+    """Adjoint cost and FP32-vs-FP64 gradient on a 128x64x64 straight vessel
+    of radius 26 (velocity inlet, pressure outlet, pulsatile inflow): J =
+    time-averaged |WSS| over the wall patch 48 <= x <= 80 for iterations
+    t0..steps, populations checkpointed every `checkpoint` iterations.
+    Fluid only: the membrane/IBM terms of the adjoint are not included."""
+    from tests import geometry as g
+    from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, Engine,
+                                      lattice_config)
+    b = g.tube((128, 64, 64), axis=0, r=26.0, inlet=B_VELOCITY_INLET, outlet=B_PRESSURE_OUTLET)
+    wx, wt, wn = g.wall_mesh(b)
+    p = wx[wt - 1]
+    area = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+    acc = np.zeros_like(wx)
+    for k in range(3):
+        np.add.at(acc, wt[:, k] - 1, area[:, None] * wn)
+    nrm = acc / np.linalg.norm(acc, axis=1, keepdims=True)
+    patch = (wx[:, 0] >= 48) & (wx[:, 0] <= 80)
+    out = {}
+    for store, k in (("f32dev", checkpoint), ("f32dev", 1), ("f64", checkpoint)):
+        cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], inlet_u0=0.03,
+                             inlet_amp=0.5, inlet_period=1000.0, inlet_dir=(1, 0, 0))
+        eng = Engine(cfg)
+        eng.upload_lattice(b)
+        eng.run(0, 10)  # warm-up of the step kernel
+        eng.init_equilibrium()
+        eng.sync()
+        eng.record(0)
+        eng.run(0, steps)
+        eng.record(1)
+        eng.sync()
+        plain_ms = eng.elapsed_ms(0, 1)
+        eng.init_equilibrium()
+        eng.adjoint_setup(wx[patch], nrm[patch], 0.1, t0, steps, steps, k)
+        J = eng.adjoint_forward(0, steps)
+        grad = eng.adjoint_backward()
+        fwd_ms, bwd_ms = eng.adjoint_timing()
+        out[(store, k)] = dict(J=J, grad=grad, plain_ms=plain_ms, fwd_ms=fwd_ms, bwd_ms=bwd_ms,
+                               n=eng.n_interior)
+        eng.close()
+    r32, r32all, r64 = out[("f32dev", checkpoint)], out[("f32dev", 1)], out[("f64", checkpoint)]
+    rel = np.abs(r32["grad"][:2] - r64["grad"][:2]) / np.abs(r64["grad"][:2])
+    return dict(workload="straight vessel 128x64x64, r=26, fluid only", fluid_nodes=r32["n"],
+                steps=steps, objective_window=[t0, steps], wall_points=int(patch.sum()),
+                forward_ms=round(r32["plain_ms"], 2),
+                checkpoint=checkpoint, forward_recorded_ms=round(r32["fwd_ms"], 2),
+                backward_ms=round(r32["bwd_ms"], 2),
+                adjoint_over_forward=round(r32["bwd_ms"] / r32["plain_ms"], 3),
+                every_state_stored=dict(forward_recorded_ms=round(r32all["fwd_ms"], 2),
+                                        backward_ms=round(r32all["bwd_ms"], 2),
+                                        adjoint_over_forward=round(r32all["bwd_ms"] / r32all["plain_ms"], 3)),
+                dJ_damp=[r32["grad"][0], r64["grad"][0]], dJ_du0=[r32["grad"][1], r64["grad"][1]],
+                grad_rel_err_fp32_vs_fp64=[float(x) for x in rel],
+                J=[r32["J"], r64["J"]])
+
+
 def probe(name, b):
     """A copy of known size (256 MiB read + 256 MiB written per launch, three
     launches) to calibrate the HBM counters, then 20 steps of each form."""
@@ -224,6 +280,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip FP64 form and error legs")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg and print its JSON object")
     ap.add_argument("--cells", type=int, default=1200,
                     help="cells of the dense-cell leg (0 skips it)")
     ap.add_argument("--cell-steps", type=int, default=200)
@@ -243,6 +302,9 @@ def main():
     b = make_geometry(args.workload)
     if args.probe:
         probe(args.workload, b)
+        return
+    if args.adjoint_only:
+        print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)
         return
     n_fluid = int(((b > 0) & (b < 10)).sum())
     eng, ms, wall = time_form(args.workload, b, args.store, args.steps, args.warmup)
@@ -268,6 +330,8 @@ def main():
         extra["speedup_vs_fp64_form"] = round(mlups / mlups64, 3)
         extra["error"] = dict(vel_rel_l2_vs_fp64=velocity_error(args.workload, b, args.err_steps),
                               steps=args.err_steps)
+        if not args.no_adjoint:
+            extra["adjoint"] = adjoint_leg()
         if args.cells > 0:
             extra["dense_cells"] = cells_leg(args.workload, b, args.cells, args.cell_steps,
                                              max(1, args.cell_steps // 10))

@@ -65,9 +65,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sums five per-lane values over the workgroup (256 lanes) into
-// part[5 * blockIdx.x ..]; k_reduce_grads adds the workgroups' sums in
-// workgroup order, so the gradients do not depend on scheduling.
+// Adds the workgroup's sums of five per-lane values (256 lanes) to its own
+// row part[5 * blockIdx.x ..], over all adjoint iterations; k_reduce_grads
+// adds the rows in row order at the end, so the gradients do not depend on
+// scheduling.
 __device__ __forceinline__ void block_sum(double r[5], double *__restrict__ part) {
   __shared__ double wp[4][5];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -77,30 +78,48 @@ __device__ __forceinline__ void block_sum(double r[5], double *__restrict__ part
   }
   __syncthreads();
   if (threadIdx.x < 5)
-    part[5 * (size_t)blockIdx.x + threadIdx.x] =
+    part[5 * (size_t)blockIdx.x + threadIdx.x] +=
         wp[0][threadIdx.x] + wp[1][threadIdx.x] + wp[2][threadIdx.x] + wp[3][threadIdx.x];
 }
 
-// grads[i] += sum over the n partial rows, in row order (one lane per i).
+// grads[i] = sum over the n partial rows, in row order (one lane per i).
 __global__ void k_reduce_grads(const double *__restrict__ part, int n, double *__restrict__ grads) {
   const int i = threadIdx.x;
   if (i >= 5) return;
   double acc = 0;
   for (int r = 0; r < n; r++) acc += part[5 * (size_t)r + i];
-  grads[i] += acc;
+  grads[i] = acc;
+}
+
+// Storage index the forward pulls g_c(s) from (the adjoint's push target),
+// 32-bit: the set-up checks that 19 * stride fits, so the loads and stores
+// take a scalar base plus one vector offset register.
+__device__ __forceinline__ uint32_t pull_index(int c, uint32_t w, const int d[20], int s,
+                                               uint32_t stride) {
+  if (c == 18) return 18u * stride + (uint32_t)s;
+  return ((w >> c) & 1u) ? (uint32_t)(c ^ 1) * stride + (uint32_t)s
+                         : (uint32_t)c * stride + (uint32_t)(s + d[c]);
 }
 
 template <int S, bool FIELD>
-__global__ __launch_bounds__(256) void k_adjoint(
+__global__ __launch_bounds__(256, S == ZF_STORE_F64 ? 1 : 3) void k_adjoint(
     const typename Store<S>::T *__restrict__ src, const typename Adj<S>::A *__restrict__ lin,
-    typename Adj<S>::A *__restrict__ lout, size_t stride, int n_int, int nblk, int remap,
+    typename Adj<S>::A *__restrict__ lout, uint32_t stride, int n_int, int nblk, int remap,
     const uint32_t *__restrict__ word, const int32_t *__restrict__ block0,
     const int32_t *__restrict__ chords, const float *__restrict__ force, StepParams prm,
     const BcEntry *__restrict__ bc, const int32_t *__restrict__ fused_s,
     const int32_t *__restrict__ att_start, int n_fused, BcParams bp, AdjParams ap,
-    double *__restrict__ gF, double *__restrict__ part) {
+    const int64_t *__restrict__ unpulled, int64_t n_unpulled, double *__restrict__ gF,
+    double *__restrict__ part) {
   using R = typename Store<S>::R;
   using A = typename Adj<S>::A;
+  // entries read by the adjoint that no lane pushes to: cleared here, each
+  // workgroup one slice of the list (no lane of this launch writes them)
+  {
+    const int64_t chunk = (n_unpulled + nblk - 1) / nblk;
+    const int64_t e1 = min(n_unpulled, ((int64_t)blockIdx.x + 1) * chunk);
+    for (int64_t e = (int64_t)blockIdx.x * chunk + threadIdx.x; e < e1; e += 256) lout[unpulled[e]] = (A)0;
+  }
   const int s = logical_block(blockIdx.x, nblk, remap != 0) * 256 + threadIdx.x;
   double red[5] = {0, 0, 0, 0, 0};
   if (s < n_int) {
@@ -113,16 +132,16 @@ __global__ __launch_bounds__(256) void k_adjoint(
       const int4 x = cr[v];
       d[4 * v] = x.x; d[4 * v + 1] = x.y; d[4 * v + 2] = x.z; d[4 * v + 3] = x.w;
     }
-    size_t idx[Q];
-    R g[Q];
+    // moments of the pulled populations (the forward's, not kept)
+    R drho = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
-    for (int c = 0; c < 18; c++) {
-      const bool bb = (w >> c) & 1u;
-      idx[c] = bb ? (size_t)(c ^ 1) * stride + s : (size_t)c * stride + (s + d[c]);
-      g[c] = Store<S>::ld(src, idx[c], WQ[c]);
+    for (int c = 0; c < Q; c++) {
+      const R gc = Store<S>::ld(src, pull_index(c, w, d, s, stride), WQ[c]);
+      drho += gc;
+      jx += (R)EX[c] * gc;
+      jy += (R)EY[c] * gc;
+      jz += (R)EZ[c] * gc;
     }
-    idx[18] = (size_t)18 * stride + s;
-    g[18] = Store<S>::ld(src, idx[18], WQ[18]);
     R fx = (R)prm.g[0], fy = (R)prm.g[1], fz = (R)prm.g[2];
     if (FIELD) {
       fx += (R)force[s];
@@ -130,14 +149,6 @@ __global__ __launch_bounds__(256) void k_adjoint(
       fz += (R)force[2 * stride + s];
     }
     const R dt = (R)prm.dt;
-    R drho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-    for (int c = 0; c < Q; c++) {
-      drho += g[c];
-      jx += (R)EX[c] * g[c];
-      jy += (R)EY[c] * g[c];
-      jz += (R)EZ[c] * g[c];
-    }
     const R rho = (R)1 + drho;
     const R q = drho / rho;
     const R vx = jx + (R)0.5 * dt * fx, vy = jy + (R)0.5 * dt * fy, vz = jz + (R)0.5 * dt * fz;
@@ -145,9 +156,8 @@ __global__ __launch_bounds__(256) void k_adjoint(
     const R usq = ux * ux + uy * uy + uz * uz;
     const R om = (R)1 / ((R)0.5 + (R)prm.visc_scale * (R)d[18]);
     const R fp = dt * ((R)1 - (R)0.5 * om);
-    R lam[Q];
-#pragma unroll
-    for (int c = 0; c < Q; c++) lam[c] = (R)lin[(size_t)c * stride + s];
+    // lambda_k(s, :) is streamed twice (sums here, mu below; the second
+    // read hits the cache) rather than held in 19 registers
     R add = 0, aux = 0, auy = 0, auz = 0, afx = 0, afy = 0, afz = 0;
 #pragma unroll
     for (int c = 0; c < Q; c++) {
@@ -155,7 +165,7 @@ __global__ __launch_bounds__(256) void k_adjoint(
       const R cu = ex * ux + ey * uy + ez * uz;
       const R cf = ex * fx + ey * fy + ez * fz;
       const R X = (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq;
-      const R wl = wmul<R>(c, lam[c]);
+      const R wl = wmul<R>(c, (R)lin[(uint32_t)c * stride + (uint32_t)s]);
       add += om * wl * ((R)1 + X);
       const R k1 = om * wl * rho, t1 = (R)3 + (R)9 * cu;
       const R k2 = fp * wl, t2 = (R)9 * cf;
@@ -186,7 +196,7 @@ __global__ __launch_bounds__(256) void k_adjoint(
           const R ubx = inlet ? (R)bp.inlet[0] : ux, uby = inlet ? (R)bp.inlet[1] : uy,
                   ubz = inlet ? (R)bp.inlet[2] : uz;
           const R ubsq = ubx * ubx + uby * uby + ubz * ubz;
-#pragma unroll
+#pragma nounroll
           for (int c = 0; c < Q; c++) {
             const R wl = wmul<R>(c, (R)lin[(size_t)c * stride + e.s]);
             const R ex = (R)EX[c], ey = (R)EY[c], ez = (R)EZ[c];
@@ -217,8 +227,9 @@ __global__ __launch_bounds__(256) void k_adjoint(
     afz += (R)0.5 * dt * avz;
 #pragma unroll
     for (int c = 0; c < Q; c++) {
-      const R mu = ((R)1 - om) * lam[c] + adt + (R)EX[c] * avx + (R)EY[c] * avy + (R)EZ[c] * avz;
-      lout[idx[c]] = (A)mu;
+      const R lc = (R)lin[(uint32_t)c * stride + (uint32_t)s];
+      const R mu = ((R)1 - om) * lc + adt + (R)EX[c] * avx + (R)EY[c] * avy + (R)EZ[c] * avz;
+      lout[pull_index(c, w, d, s, stride)] = (A)mu;
     }
     if (FIELD && gF) {
       gF[s] += (double)afx;
@@ -288,9 +299,8 @@ __global__ __launch_bounds__(64) void k_adjoint_late(
     gacc0 += gdir * (1.0 + ap.amp * ap.sin_phase);
   }
   if (c == 0) {
-    part_row[0] = gacc;
-    part_row[1] = gacc0;
-    part_row[2] = part_row[3] = part_row[4] = 0.0;
+    part_row[0] += gacc;
+    part_row[1] += gacc0;
   }
 }
 
@@ -344,90 +354,109 @@ __global__ void k_stencil_table(const int32_t *__restrict__ b, const int32_t *__
   cnt[m] = k;
 }
 
+// Unshifted velocity and density of every support node (stencil nodes of
+// the probes): su[4 i ..] = (u_x, u_y, u_z, 1 + drho).
 template <int S>
-__device__ void probe_tau(const typename Store<S>::T *__restrict__ f, size_t stride,
-                          const int32_t *__restrict__ b, const int32_t *__restrict__ sbox,
-                          const WGrid &g, const double *x, const double *nv, double mu_over_dx,
-                          double tau[3]) {
-  double u0 = 0, u1 = 0, u2 = 0;
-  wss_stencil(g, b, sbox, x[0] + g.dx * nv[0], x[1] + g.dx * nv[1], x[2] + g.dx * nv[2],
-              [&](int sq, double w) {
-                double drho, jv[3];
-                node_moments<S>(f, stride, sq, drho, jv);
-                const double r = 1.0 + drho;
-                u0 += w * (jv[0] / r);
-                u1 += w * (jv[1] / r);
-                u2 += w * (jv[2] / r);
-              });
-  const double un = u0 * nv[0] + u1 * nv[1] + u2 * nv[2];
-  tau[0] = -mu_over_dx * (u0 - un * nv[0]);
-  tau[1] = -mu_over_dx * (u1 - un * nv[1]);
-  tau[2] = -mu_over_dx * (u2 - un * nv[2]);
-}
-
-template <int S>
-__global__ void k_obj_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
-                          const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, WGrid g,
-                          int n, const double *__restrict__ pts, double mu_over_dx,
-                          double *__restrict__ out) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  double tau[3];
-  probe_tau<S>(f, stride, b, sbox, g, pts + 3 * (size_t)m, pts + 3 * (size_t)n + 3 * (size_t)m,
-               mu_over_dx, tau);
-  out[m] = sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
-}
-
-// dJ/dtau of each probe, times -(mu/DX): J gains scale |tau|, so
-// dJ/dtau = scale tau/|tau|, and dJ/du_q = -(mu/DX) w_q (I - n n) dJ/dtau,
-// which is w_q a_m with a_m = -(mu/DX) dJ/dtau (dJ/dtau is tangential).
-template <int S>
-__global__ void k_obj_wss_adj(const typename Store<S>::T *__restrict__ f, size_t stride,
-                              const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, WGrid g,
-                              int n, const double *__restrict__ pts, double mu_over_dx, double scale,
-                              double *__restrict__ a) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  double tau[3];
-  probe_tau<S>(f, stride, b, sbox, g, pts + 3 * (size_t)m, pts + 3 * (size_t)n + 3 * (size_t)m,
-               mu_over_dx, tau);
-  const double mag = sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
-  const double k = mag > 0 ? -mu_over_dx * scale / mag : 0.0;
-  a[3 * m] = k * tau[0];
-  a[3 * m + 1] = k * tau[1];
-  a[3 * m + 2] = k * tau[2];
-}
-
-// At each support node s (one lane each): dJ/du_s = sum over the probes
-// whose stencil holds s of w_ms a_m (a fixed order), then
-// lambda(s, c) += dJ/du_s . (c - u_s)/(1 + drho_s).
-template <int S>
-__global__ void k_obj_apply(const typename Store<S>::T *__restrict__ f, size_t stride,
-                            const int32_t *__restrict__ list, int nlist,
-                            const int32_t *__restrict__ off, const int32_t *__restrict__ probe,
-                            const double *__restrict__ weight, const double *__restrict__ a,
-                            typename Adj<S>::A *__restrict__ lam) {
-  using A = typename Adj<S>::A;
+__global__ void k_support_moments(const typename Store<S>::T *__restrict__ f, size_t stride,
+                                  const int32_t *__restrict__ list, int nlist,
+                                  double *__restrict__ su) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nlist) return;
-  const int s = list[i];
-  double d0 = 0, d1 = 0, d2 = 0;
-  for (int e = off[i]; e < off[i + 1]; e++) {
-    const double w = weight[e];
-    const double *am = a + 3 * (size_t)probe[e];
-    d0 += w * am[0];
-    d1 += w * am[1];
-    d2 += w * am[2];
-  }
   double drho, jv[3];
-  node_moments<S>(f, stride, s, drho, jv);
-  const double r = 1.0 + drho, ux = jv[0] / r, uy = jv[1] / r, uz = jv[2] / r;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
-    const double v = (d0 * (EX[c] - ux) + d1 * (EY[c] - uy) + d2 * (EZ[c] - uz)) / r;
-    const size_t o = (size_t)c * stride + s;
-    lam[o] = (A)((double)lam[o] + v);
+  node_moments<S>(f, stride, list[i], drho, jv);
+  const double r = 1.0 + drho;
+  su[4 * (size_t)i] = jv[0] / r;
+  su[4 * (size_t)i + 1] = jv[1] / r;
+  su[4 * (size_t)i + 2] = jv[2] / r;
+  su[4 * (size_t)i + 3] = r;
+}
+
+// Wall shear stress of each probe from its stencil table (support index,
+// weight); one 64-lane wave per probe, lane e taking entries e, e+64, ...
+// and a fixed-order wave reduction.  grad == 0: out[m] = |tau|.
+// grad != 0: out[3m..] = a_m = -(mu/DX) dJ/dtau with J gaining scale |tau|,
+// so dJ/du at stencil node q is w_q a_m (dJ/dtau is tangential, so the
+// projection I - n n leaves it unchanged).
+__global__ __launch_bounds__(256) void k_probe_tau(int n, const int32_t *__restrict__ off,
+                                                   const int32_t *__restrict__ sup,
+                                                   const double *__restrict__ wgt,
+                                                   const double *__restrict__ su,
+                                                   const double *__restrict__ pts, double mu_over_dx,
+                                                   double scale, int grad, double *__restrict__ out) {
+  const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (m >= n) return;  // whole waves leave together
+  double u0 = 0, u1 = 0, u2 = 0;
+  for (int e = off[m] + lane; e < off[m + 1]; e += 64) {
+    const double w = wgt[e];
+    const double *v = su + 4 * (size_t)sup[e];
+    u0 += w * v[0];
+    u1 += w * v[1];
+    u2 += w * v[2];
   }
+  u0 = wave_sum(u0);
+  u1 = wave_sum(u1);
+  u2 = wave_sum(u2);
+  if (lane != 0) return;
+  const double *nv = pts + 3 * (size_t)n + 3 * (size_t)m;
+  const double un = u0 * nv[0] + u1 * nv[1] + u2 * nv[2];
+  const double t0 = -mu_over_dx * (u0 - un * nv[0]), t1 = -mu_over_dx * (u1 - un * nv[1]),
+               t2 = -mu_over_dx * (u2 - un * nv[2]);
+  const double mag = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+  if (!grad) {
+    out[m] = mag;
+    return;
+  }
+  const double k = mag > 0 ? -mu_over_dx * scale / mag : 0.0;
+  out[3 * (size_t)m] = k * t0;
+  out[3 * (size_t)m + 1] = k * t1;
+  out[3 * (size_t)m + 2] = k * t2;
+}
+
+// dJ/du at each support node i: the sum over the probes whose stencil holds
+// the node of w a_m; 8 lanes per node take entries e, e+8, ... and a
+// fixed-order shuffle reduction finishes it; sg[3 i ..].
+__global__ __launch_bounds__(256) void k_obj_gather(int nlist, const int32_t *__restrict__ off,
+                                                    const int32_t *__restrict__ probe,
+                                                    const double *__restrict__ weight,
+                                                    const double *__restrict__ a,
+                                                    double *__restrict__ sg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t >> 3, sub = t & 7;
+  double d0 = 0, d1 = 0, d2 = 0;
+  if (i < nlist)
+    for (int e = off[i] + sub; e < off[i + 1]; e += 8) {
+      const double w = weight[e];
+      const double *am = a + 3 * (size_t)probe[e];
+      d0 += w * am[0];
+      d1 += w * am[1];
+      d2 += w * am[2];
+    }
+  for (int o = 4; o > 0; o >>= 1) {
+    d0 += __shfl_xor(d0, o);
+    d1 += __shfl_xor(d1, o);
+    d2 += __shfl_xor(d2, o);
+  }
+  if (i < nlist && sub == 0) {
+    sg[3 * (size_t)i] = d0;
+    sg[3 * (size_t)i + 1] = d1;
+    sg[3 * (size_t)i + 2] = d2;
+  }
+}
+
+// lambda(s_i, c) += dJ/du_i . (c - u_i)/(1 + drho_i), one lane per
+// (direction c, support node i), consecutive lanes on consecutive nodes.
+template <int S>
+__global__ void k_obj_apply(const int32_t *__restrict__ list, int nlist, size_t stride,
+                            const double *__restrict__ sg, const double *__restrict__ su,
+                            typename Adj<S>::A *__restrict__ lam) {
+  using A = typename Adj<S>::A;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Q * nlist) return;
+  const int c = t / nlist, i = t - c * nlist;
+  const double *d = sg + 3 * (size_t)i, *v = su + 4 * (size_t)i;
+  const double g = (d[0] * (EX[c] - v[0]) + d[1] * (EY[c] - v[1]) + d[2] * (EZ[c] - v[2])) / v[3];
+  const size_t o = (size_t)c * stride + list[i];
+  lam[o] = (A)((double)lam[o] + g);
 }
 
 // lambda (storage order) -> host layout, in double.
@@ -465,36 +494,49 @@ int require_adj(zf_ctx *ctx) {
   return ZF_OK;
 }
 
-// Objective terms |tau| of the current state, stored for iteration `it`.
-int obj_forward(zf_ctx *ctx, const void *f, int it) {
-  const zf_adjoint_config &a = ctx->acfg;
-  double *out = ctx->d_objbuf + (size_t)(it - a.t_begin) * ctx->adj_npts;
+int support_moments(zf_ctx *ctx, const void *f) {
+  if (ctx->n_support == 0) return ZF_OK;
   with_store(ctx->cfg.storage, [&](auto tag) {
     constexpr int S = decltype(tag)::value;
-    hipLaunchKernelGGL(k_obj_wss<S>, dim3(nb(ctx->adj_npts, 64)), dim3(64), 0, ctx->stream,
-                       st<S>(f), ctx->stride, ctx->d_b, ctx->d_sbox, wgrid(ctx), ctx->adj_npts,
-                       ctx->d_adj_pts, a.mu_over_dx, out);
+    hipLaunchKernelGGL(k_support_moments<S>, dim3(nb(ctx->n_support, 256)), dim3(256), 0,
+                       ctx->stream, st<S>(f), ctx->stride, ctx->d_support, ctx->n_support,
+                       ctx->d_sup_u);
     return 0;
   });
   return ZF_OK;
 }
 
-// lambda += dJ/dS for state f (iteration `it` in the window).
+// Objective terms |tau| of the current state, stored for iteration `it`.
+int obj_forward(zf_ctx *ctx, const void *f, int it) {
+  const zf_adjoint_config &a = ctx->acfg;
+  double *out = ctx->d_objbuf + (size_t)(it - a.t_begin) * ctx->adj_npts;
+  support_moments(ctx, f);
+  hipLaunchKernelGGL(k_probe_tau, dim3(nb(ctx->adj_npts, 4)), dim3(256), 0, ctx->stream,
+                     ctx->adj_npts, ctx->d_prb_off, ctx->d_prb_sup, ctx->d_prb_w, ctx->d_sup_u,
+                     ctx->d_adj_pts, a.mu_over_dx, 1.0, 0, out);
+  return ZF_OK;
+}
+
+// lambda += dJ/dS for state f (an iteration of the window).
 int obj_adjoint(zf_ctx *ctx, const void *f, void *lam) {
   const zf_adjoint_config &a = ctx->acfg;
   const double scale = 1.0 / ((double)(a.t_end - a.t_begin + 1) * ctx->adj_npts);
-  with_store(ctx->cfg.storage, [&](auto tag) {
-    constexpr int S = decltype(tag)::value;
-    using A = typename Adj<S>::A;
-    hipLaunchKernelGGL(k_obj_wss_adj<S>, dim3(nb(ctx->adj_npts, 64)), dim3(64), 0, ctx->stream,
-                       st<S>(f), ctx->stride, ctx->d_b, ctx->d_sbox, wgrid(ctx), ctx->adj_npts,
-                       ctx->d_adj_pts, a.mu_over_dx, scale, ctx->d_adj_a);
-    if (ctx->n_support > 0)
-      hipLaunchKernelGGL(k_obj_apply<S>, dim3(nb(ctx->n_support, 256)), dim3(256), 0, ctx->stream,
-                         st<S>(f), ctx->stride, ctx->d_support, ctx->n_support, ctx->d_sup_off,
-                         ctx->d_sup_probe, ctx->d_sup_w, ctx->d_adj_a, static_cast<A *>(lam));
-    return 0;
-  });
+  support_moments(ctx, f);
+  hipLaunchKernelGGL(k_probe_tau, dim3(nb(ctx->adj_npts, 4)), dim3(256), 0, ctx->stream,
+                     ctx->adj_npts, ctx->d_prb_off, ctx->d_prb_sup, ctx->d_prb_w, ctx->d_sup_u,
+                     ctx->d_adj_pts, a.mu_over_dx, scale, 1, ctx->d_adj_a);
+  if (ctx->n_support > 0)
+    with_store(ctx->cfg.storage, [&](auto tag) {
+      constexpr int S = decltype(tag)::value;
+      using A = typename Adj<S>::A;
+      hipLaunchKernelGGL(k_obj_gather, dim3(nb(8 * (size_t)ctx->n_support, 256)), dim3(256), 0, ctx->stream,
+                         ctx->n_support, ctx->d_sup_off, ctx->d_sup_probe, ctx->d_sup_w,
+                         ctx->d_adj_a, ctx->d_sup_g);
+      hipLaunchKernelGGL(k_obj_apply<S>, dim3(nb((size_t)Q * ctx->n_support, 256)), dim3(256), 0,
+                         ctx->stream, ctx->d_support, ctx->n_support, ctx->stride, ctx->d_sup_g,
+                         ctx->d_sup_u, static_cast<A *>(lam));
+      return 0;
+    });
   return ZF_OK;
 }
 
@@ -517,27 +559,23 @@ int adjoint_step(zf_ctx *ctx, const void *src, const void *lin, void *lout, int 
                            static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride,
                            ctx->n_int, nblk, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
                            ctx->d_force, prm, ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
-                           ctx->n_fused_nodes, bp, ap, ctx->d_gF, ctx->d_gpart);
+                           ctx->n_fused_nodes, bp, ap, ctx->d_unpulled, ctx->n_unpulled,
+                           ctx->d_gF, ctx->d_gpart);
       else
         hipLaunchKernelGGL((k_adjoint<S, false>), dim3(nblk), dim3(256), 0, ctx->stream, st<S>(src),
                            static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride,
                            ctx->n_int, nblk, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
                            nullptr, prm, ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
-                           ctx->n_fused_nodes, bp, ap, nullptr, ctx->d_gpart);
+                           ctx->n_fused_nodes, bp, ap, ctx->d_unpulled, ctx->n_unpulled,
+                           nullptr, ctx->d_gpart);
     }
-    if (ctx->n_unpulled > 0)
+    if (nblk == 0 && ctx->n_unpulled > 0)
       hipLaunchKernelGGL(k_zero_idx<A>, dim3(nb((size_t)ctx->n_unpulled, 256)), dim3(256), 0,
                          ctx->stream, static_cast<A *>(lout), ctx->d_unpulled, ctx->n_unpulled);
-    int rows = nblk;
-    if (ctx->n_bc > ctx->n_bc_fused) {
+    if (ctx->n_bc > ctx->n_bc_fused)
       hipLaunchKernelGGL(k_adjoint_late<S>, dim3(1), dim3(64), 0, ctx->stream, st<S>(src),
                          static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride, ctx->d_bc,
                          ctx->n_bc_fused, ctx->n_bc, bp, ap, ctx->d_gpart + 5 * (size_t)nblk);
-      rows++;
-    }
-    if (rows > 0)
-      hipLaunchKernelGGL(k_reduce_grads, dim3(1), dim3(64), 0, ctx->stream, ctx->d_gpart, rows,
-                         ctx->d_grads);
     return 0;
   });
   ZF_HIP(ctx, hipGetLastError());
@@ -550,6 +588,9 @@ void free_adjoint(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_adj_pts, &ctx->d_ckpt, &ctx->d_seg, &ctx->d_lam[0],
                    &ctx->d_lam[1], (void **)&ctx->d_adj_a, (void **)&ctx->d_support,
                    (void **)&ctx->d_sup_off, (void **)&ctx->d_sup_probe, (void **)&ctx->d_sup_w,
+                   (void **)&ctx->d_sup_u, (void **)&ctx->d_sup_g, (void **)&ctx->d_prb_off,
+                   (void **)&ctx->d_prb_sup,
+                   (void **)&ctx->d_prb_w,
                    (void **)&ctx->d_gpart,
                    (void **)&ctx->d_unpulled, (void **)&ctx->d_grads, (void **)&ctx->d_gF,
                    (void **)&ctx->d_objbuf};
@@ -581,6 +622,8 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
   if (!ctx->have_lattice || !ctx->d_f[0]) return fail(ctx, ZF_ERR_STATE, "no lattice uploaded");
   if (ctx->cfg.storage == ZF_STORE_F16DEV)
     return fail(ctx, ZF_ERR_ARG, "the adjoint supports FP64 and FP32 storage only");
+  if ((uint64_t)Q * ctx->stride >= (1ull << 32))
+    return fail(ctx, ZF_ERR_ARG, "lattice too large for the adjoint's 32-bit population offsets");
   ZF_HIP(ctx, hipSetDevice(ctx->device));
   free_adjoint(ctx);
   ctx->acfg = *ac;
@@ -680,10 +723,11 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
     for (int k = 0; k < hcnt[m]; k++)
       per[hidx[(size_t)m * kStencilMax + k]].push_back({(int32_t)m, hw[(size_t)m * kStencilMax + k]});
   }
-  std::vector<int32_t> support, off(1, 0), sprobe;
+  std::vector<int32_t> support, off(1, 0), sprobe, sup_of(stride, -1);
   std::vector<double> sw;
   for (size_t q = 0; q < stride; q++)
     if (!per[q].empty()) {
+      sup_of[q] = (int32_t)support.size();
       support.push_back((int32_t)q);
       for (auto &pw : per[q]) {
         sprobe.push_back(pw.first);
@@ -691,15 +735,35 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
       }
       off.push_back((int32_t)sprobe.size());
     }
+  // probe-major table in stencil order: (support index, weight)
+  std::vector<int32_t> poff(1, 0), psup;
+  std::vector<double> pw;
+  for (int64_t m = 0; m < npts; m++) {
+    for (int k = 0; k < hcnt[m]; k++) {
+      psup.push_back(sup_of[hidx[(size_t)m * kStencilMax + k]]);
+      pw.push_back(hw[(size_t)m * kStencilMax + k]);
+    }
+    poff.push_back((int32_t)psup.size());
+  }
   ctx->n_unpulled = (int64_t)unpulled.size();
   ctx->n_support = (int32_t)support.size();
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_unpulled, 8 * std::max<size_t>(unpulled.size(), 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_support, 4 * std::max<size_t>(support.size(), 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_sup_off, 4 * off.size())) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_sup_probe, 4 * std::max<size_t>(sprobe.size(), 1))) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_w, 8 * std::max<size_t>(sw.size(), 1)))) {
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_w, 8 * std::max<size_t>(sw.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_u, 32 * std::max<size_t>(support.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_g, 24 * std::max<size_t>(support.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_prb_off, 4 * poff.size())) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_prb_sup, 4 * std::max<size_t>(psup.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_prb_w, 8 * std::max<size_t>(pw.size(), 1)))) {
     free_adjoint(ctx);
     return rc;
+  }
+  ZF_HIP(ctx, hipMemcpy(ctx->d_prb_off, poff.data(), 4 * poff.size(), hipMemcpyHostToDevice));
+  if (!psup.empty()) {
+    ZF_HIP(ctx, hipMemcpy(ctx->d_prb_sup, psup.data(), 4 * psup.size(), hipMemcpyHostToDevice));
+    ZF_HIP(ctx, hipMemcpy(ctx->d_prb_w, pw.data(), 8 * pw.size(), hipMemcpyHostToDevice));
   }
   ZF_HIP(ctx, hipMemcpy(ctx->d_sup_off, off.data(), 4 * off.size(), hipMemcpyHostToDevice));
   if (!sprobe.empty()) {
@@ -758,7 +822,9 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
   const int K = a.checkpoint_interval, iter0 = ctx->adj_iter0, N = ctx->adj_nsteps;
   const size_t stride = ctx->stride, ab = ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4;
   ZF_HIP(ctx, hipEventRecord(ctx->events[12], ctx->stream));
-  hipLaunchKernelGGL(k_fill_a<double>, dim3(1), dim3(64), 0, ctx->stream, ctx->d_grads, (size_t)8);
+  const int rows = (int)nb(ctx->n_int, 256) + 1;
+  hipLaunchKernelGGL(k_fill_a<double>, dim3(nb(5 * (size_t)rows, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_gpart, 5 * (size_t)rows);
   if (ctx->d_gF)
     hipLaunchKernelGGL(k_fill_a<double>, dim3(nb(3 * stride, 256)), dim3(256), 0, ctx->stream,
                        ctx->d_gF, 3 * stride);
@@ -794,6 +860,8 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
     }
   }
   ctx->adj_cur = cur;
+  hipLaunchKernelGGL(k_reduce_grads, dim3(1), dim3(64), 0, ctx->stream, ctx->d_gpart, rows,
+                     ctx->d_grads);
   ZF_HIP(ctx, hipEventRecord(ctx->events[13], ctx->stream));
   ZF_HIP(ctx, hipGetLastError());
   double g[8];

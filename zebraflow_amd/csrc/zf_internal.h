@@ -162,6 +162,11 @@ struct zf_ctx {
   int32_t *d_sup_off = nullptr;    // n_support + 1: CSR over (probe, weight) pairs
   int32_t *d_sup_probe = nullptr;
   double *d_sup_w = nullptr;
+  double *d_sup_u = nullptr;       // 4 x n_support: u and 1 + drho of the support nodes
+  double *d_sup_g = nullptr;       // 3 x n_support: dJ/du of the support nodes
+  int32_t *d_prb_off = nullptr;    // npts + 1: CSR over (support index, weight) per probe
+  int32_t *d_prb_sup = nullptr;
+  double *d_prb_w = nullptr;
   int32_t n_support = 0;
   double *d_gpart = nullptr;       // 5 per workgroup (+1 row): gradient partial sums
   int64_t *d_unpulled = nullptr;   // (c*stride + s) read by the adjoint but pulled by no node
