@@ -122,7 +122,19 @@ HOST_FUNCS = {
                                 C.POINTER(i64), C.POINTER(i64)]),
     "zfh_write_bindex_csv":(C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64]),
     "zfh_read_bindex_csv": (i64, [P, P, i32, i32, i32, f64, f64, f64, f64]),
-    "zfh_write_fluid_csv": (C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64, P, P, P, P, P, i64]),
+    "zfh_write_fluid_csv": (C.c_int, [P, P, i32, i32, i32, f64, f64, f64, f64, P, P, P, P, P, P, P,
+                                      i64]),
+    "zfh_read_points": (i64, [C.c_char_p, P, i64]),
+    "zfh_read_rbc_triangles": (i64, [C.c_char_p, P, i64]),
+    "zfh_read_wall_cells": (i64, [C.c_char_p, P, i64]),
+    "zfh_read_wall_normals": (i64, [C.c_char_p, P, i64]),
+    "zfh_write_points": (C.c_int, [C.c_char_p, P, i64]),
+    "zfh_write_rbc_triangles": (C.c_int, [C.c_char_p, P, i64]),
+    "zfh_write_wall_mesh": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, P, i64, P, P, i64]),
+    "zfh_read_cells_csv": (C.c_int, [C.c_char_p, i32, P, P, i32]),
+    "zfh_write_cells_csv": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P, P, P]),
+    "zfh_append_volume_area": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P]),
+    "zfh_membrane_config_lattice": (None, [P, i32, P, i32, f64, f64, i32, P]),
 }
 
 

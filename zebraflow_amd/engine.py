@@ -350,29 +350,16 @@ def membrane_config(d, max_cells):
 
 def lattice_membrane_config(xyz, tris, max_cells, es=5e-3, kb=2e-3):
     """Membrane constants in lattice units for a rest mesh (xyz [nn,3], tris
-    [nt,3] 1-based): the reference's moduli expressed as ratios to the shear
-    modulus Es (kd = 12 Es, ka = 25.67 Es, kv = 9.17 Es, kT = 0.0036 Es l0^2
-    with l0 the mean rest edge length, ZebraflowWT.c:124-134), bending
-    modulus kb (kBend = 2/sqrt(3) kb), the rest area and volume as the
-    global targets, and force guards scaled by Es l0."""
+    [nt,3] 1-based): zfh_membrane_config_lattice of the host library (the
+    reference's moduli as ratios to the shear modulus es, ZebraflowWT.c:124-134;
+    bending modulus kb; rest area and volume as the global targets)."""
+    from . import load_host_library
     from ._abi import ZfMembraneConfig
-    xyz = np.asarray(xyz, np.float64)
-    tris = np.asarray(tris, np.int64)
-    e = np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]]) - 1
-    e = np.unique(np.sort(e, axis=1), axis=0)
-    l0 = np.linalg.norm(xyz[e[:, 0]] - xyz[e[:, 1]], axis=1).mean()
-    p = xyz[tris - 1]
-    av = 0.5 * np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    tris = np.ascontiguousarray(tris, np.int32)
     mc = ZfMembraneConfig()
-    load_library().zf_membrane_config_reference(C.byref(mc))
-    d = dict(len_scale=1.0, es=es, kT=0.0036 * es * l0 * l0, kbend=2 / np.sqrt(3) * kb,
-             kd=12 * es, ka=25.67 * es, kv=9.17 * es, area0=np.linalg.norm(av, axis=1).sum(),
-             vol0=(av * p.mean(1)).sum() / 3, force_limit=60 * es * l0, vol_limit=1e30,
-             sliver_len=0.1, sliver_len_min=0.002, sliver_f1=0.01 * es * l0,
-             sliver_f2=6 * es * l0)
-    for k, v in d.items():
-        setattr(mc, k, float(v))
-    mc.max_cells = int(max_cells)
+    load_host_library().zfh_membrane_config_lattice(_ptr(xyz), len(xyz), _ptr(tris), len(tris),
+                                                    es, kb, int(max_cells), C.byref(mc))
     return mc
 
 

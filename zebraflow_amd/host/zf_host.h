@@ -86,7 +86,59 @@ int64_t zfh_read_bindex_csv(const char *path, int32_t *b, int im, int jm, int km
  * u,v,w,d,force are in list order p (force may be NULL). */
 int zfh_write_fluid_csv(const char *path, const int32_t *b, int im, int jm, int km, double x0,
                         double y0, double z0, double dx, const double *d, const double *u,
-                        const double *v, const double *w, const double *force, int64_t dommax);
+                        const double *v, const double *w, const double *force,
+                        const int32_t *eulneigh, const int32_t *windex, int64_t dommax);
+
+/* Meshes (the reference's VTK-style .dat files).  With a NULL array the
+ * readers return the count only; -2 when it exceeds cap, -1 on a bad file.
+ *   RBC nodes        "POINTS n type", coordinates   (ZebraflowWT.c:1602-1613)
+ *   RBC triangles    "CELLS n m", 1-based triples   (ZebraflowWT.c:1617-1626)
+ *   wall nodes       "POINTS n type", coordinates
+ *   wall triangles   "POLYGONS n m", rows "3 a b c" 0-based (stored 1-based)
+ *   wall normals     "CELL_DATA n", "NORMALS ...", one normal per triangle */
+int64_t zfh_read_points(const char *path, double *xyz, int64_t cap);
+int64_t zfh_read_rbc_triangles(const char *path, int32_t *tri, int64_t cap);
+int64_t zfh_read_wall_cells(const char *path, int32_t *tri, int64_t cap);
+int64_t zfh_read_wall_normals(const char *path, double *nrm, int64_t cap);
+int zfh_write_points(const char *path, const double *xyz, int64_t n);
+int zfh_write_rbc_triangles(const char *path, const int32_t *tri, int64_t n);
+int zfh_write_wall_mesh(const char *nodes_path, const char *cells_path, const char *normals_path,
+                        const double *xyz, int64_t nwn, const int32_t *tri, const double *nrm,
+                        int64_t nwt);
+
+/* Cells in the AllCellMemb layout (ZebraflowWT.c:435-446, read back as
+ * AllCellMembStart, :5327-5430): 21 columns per node row. */
+int zfh_read_cells_csv(const char *path, int nn, double *xyz, int32_t *status, int cap_cells);
+int zfh_write_cells_csv(const char *path, int ncell, int nn, const double *xyz, const double *vel,
+                        const int32_t *status, const int32_t *wall_tri, const double *vrep,
+                        const double *force);
+/* VolumeAreaHistory.dat rows (ZebraflowWT.c:346-351), appended. */
+int zfh_append_volume_area(const char *path, int iter, int ncell, const double *vol,
+                           const double *area, const double *centroid, const int32_t *status);
+
+/* Membrane constants in lattice units for a rest mesh (1-based tri): the
+ * reference's moduli as ratios to the shear modulus es (kd = 12 es,
+ * ka = 25.67 es, kv = 9.17 es, kT = 0.0036 es l0^2 with l0 the mean rest
+ * edge, kBend = 2/sqrt(3) kb), rest area and volume as the global targets. */
+void zfh_membrane_config_lattice(const double *xyz, int nn, const int32_t *tri, int nt, double es,
+                                 double kb, int max_cells, zf_membrane_config *mc);
+
+/* ---- the step/output loop -------------------------------------------------- */
+typedef struct zfh_run_config {
+  const char *out_dir;    /* SimulationOutputData */
+  int32_t im, jm, km;     /* the box of b */
+  int32_t iter0, steps;   /* iterations iter0+1 .. iter0+steps */
+  int32_t output_every;   /* OUTPUT() when iter % output_every == 0 (0: never) */
+  double x0, y0, z0, dx;  /* node (i,j,k) at (x0 + (i-1) dx, ...) in the CSVs */
+  int32_t ncell, nn, nt;  /* cells on the context (0: none) */
+  int32_t have_rep, have_wall;
+} zfh_run_config;
+
+/* The reference's time loop (ZebraflowWT.c:5646-12412): zf_step per
+ * iteration, then on output iterations the fluid CSV, the cell CSV and the
+ * volume/area history, as OUTPUT() writes them (ZebraflowWT.c:282-446).
+ * b: the box (for the CSV row order).  Returns 0 or the failing code. */
+int zfh_run(zf_ctx *ctx, const int32_t *b, const zfh_run_config *rc);
 
 #ifdef __cplusplus
 }

@@ -2,6 +2,10 @@
  * zf_io.c -- the reference's text formats for the lattice and fluid state.
  *   FLUIDdomainBindex.csv   read at ZebraflowWT.c:2574-2619
  *   FLUIDdomain-%09d.csv    written by OUTPUT(), ZebraflowWT.c:359-373
+ *   AllCellMemb-%09d.csv    written by OUTPUT(), ZebraflowWT.c:435-446
+ *   VolumeAreaHistory.dat   appended by OUTPUT(), ZebraflowWT.c:346-351
+ *   RBC and wall meshes     ZebraflowWT.c:1601-1626, the post-processor's
+ *                           WallMesh*.dat readers
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -57,7 +61,8 @@ int64_t zfh_read_bindex_csv(const char *path, int32_t *b, int im, int jm, int km
 
 int zfh_write_fluid_csv(const char *path, const int32_t *b, int im, int jm, int km, double x0,
                         double y0, double z0, double dx, const double *d, const double *u,
-                        const double *v, const double *w, const double *force, int64_t dommax) {
+                        const double *v, const double *w, const double *force,
+                        const int32_t *eulneigh, const int32_t *windex, int64_t dommax) {
   FILE *f = fopen(path, "w");
   if (!f) return -1;
   fprintf(f, "X,Y,Z,Bindex,U,V,W,D,Bx,By,Bz,EulNeigh,Windex\n");
@@ -72,8 +77,216 @@ int zfh_write_fluid_csv(const char *path, const int32_t *b, int im, int jm, int 
                      fz = force ? force[2 * dommax + o] : 0;
         fprintf(f, "%f,%f,%f,%d,%.7e,%.7e,%.7e,%.7e,%.7e,%.7e,%.7e,%d,%d\n", x0 + (i - 1) * dx,
                 y0 + (j - 1) * dx, z0 + (k - 1) * dx, bv, (float)u[o], (float)v[o], (float)w[o],
-                (float)d[o], (float)fx, (float)fy, (float)fz, 0, 0);
+                (float)d[o], (float)fx, (float)fy, (float)fz, eulneigh ? eulneigh[o] : 0,
+                windex ? windex[bq(i, j, k, jm, km)] : 0);
         o++;
       }
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+/* ---- meshes: the reference's VTK-style .dat files ------------------------- */
+
+/* Skips `lines` header lines. */
+static int skip_lines(FILE *f, int lines) {
+  char buf[512];
+  for (int l = 0; l < lines; l++)
+    if (!fgets(buf, sizeof buf, f)) return -1;
+  return 0;
+}
+
+/* "POINTS n type" then whitespace-separated coordinates
+ * (ZFISHRBCNODES.dat, ZebraflowWT.c:1602-1613; WallMeshNodeCoord.dat,
+ * PostProWSSandPressure.c:57-65). */
+int64_t zfh_read_points(const char *path, double *xyz, int64_t cap) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char kw[32];
+  long long n = 0;
+  if (fscanf(f, "%31s %lld", kw, &n) != 2 || strcmp(kw, "POINTS") != 0 || n < 0) {
+    fclose(f);
+    return -1;
+  }
+  if (skip_lines(f, 1)) { fclose(f); return -1; }
+  if (!xyz) { fclose(f); return n; }
+  if (n > cap) { fclose(f); return -2; }
+  for (long long i = 0; i < 3 * n; i++)
+    if (fscanf(f, "%lf", &xyz[i]) != 1) { fclose(f); return -1; }
+  fclose(f);
+  return n;
+}
+
+/* "CELLS n m" then n triples of 1-based node ids (ZFISHRBCTRIANGLES.dat,
+ * ZebraflowWT.c:1617-1626). */
+int64_t zfh_read_rbc_triangles(const char *path, int32_t *tri, int64_t cap) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char kw[32];
+  long long n = 0, m = 0;
+  if (fscanf(f, "%31s %lld %lld", kw, &n, &m) != 3 || strcmp(kw, "CELLS") != 0 || n < 0) {
+    fclose(f);
+    return -1;
+  }
+  if (!tri) { fclose(f); return n; }
+  if (n > cap) { fclose(f); return -2; }
+  for (long long i = 0; i < 3 * n; i++)
+    if (fscanf(f, "%d", &tri[i]) != 1) { fclose(f); return -1; }
+  fclose(f);
+  return n;
+}
+
+/* "POLYGONS n m" then n rows "3 a b c" of 0-based ids, stored 1-based
+ * (WallMeshCells.dat, PostProWSSandPressure.c:84-96). */
+int64_t zfh_read_wall_cells(const char *path, int32_t *tri, int64_t cap) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char kw[32];
+  long long n = 0, m = 0;
+  if (fscanf(f, "%31s %lld %lld", kw, &n, &m) != 3 || strcmp(kw, "POLYGONS") != 0 || n < 0) {
+    fclose(f);
+    return -1;
+  }
+  if (!tri) { fclose(f); return n; }
+  if (n > cap) { fclose(f); return -2; }
+  for (long long t = 0; t < n; t++) {
+    int k, a, b, c;
+    if (fscanf(f, "%d %d %d %d", &k, &a, &b, &c) != 4 || k != 3) { fclose(f); return -1; }
+    tri[3 * t] = a + 1;
+    tri[3 * t + 1] = b + 1;
+    tri[3 * t + 2] = c + 1;
+  }
+  fclose(f);
+  return n;
+}
+
+/* "CELL_DATA n", "NORMALS Normals float", then n normals
+ * (WallMeshCellNormals.dat). */
+int64_t zfh_read_wall_normals(const char *path, double *nrm, int64_t cap) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char kw[32];
+  long long n = 0;
+  if (fscanf(f, "%31s %lld", kw, &n) != 2 || strcmp(kw, "CELL_DATA") != 0 || n < 0) {
+    fclose(f);
+    return -1;
+  }
+  if (skip_lines(f, 2)) { fclose(f); return -1; }
+  if (!nrm) { fclose(f); return n; }
+  if (n > cap) { fclose(f); return -2; }
+  for (long long i = 0; i < 3 * n; i++)
+    if (fscanf(f, "%lf", &nrm[i]) != 1) { fclose(f); return -1; }
+  fclose(f);
+  return n;
+}
+
+int zfh_write_points(const char *path, const double *xyz, int64_t n) {
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "POINTS %lld double\n", (long long)n);
+  for (int64_t i = 0; i < n; i++)
+    fprintf(f, "%.15g %.15g %.15g\n", xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+int zfh_write_rbc_triangles(const char *path, const int32_t *tri, int64_t n) {
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "CELLS %lld %lld\n", (long long)n, (long long)(4 * n));
+  for (int64_t t = 0; t < n; t++) fprintf(f, "%d %d %d\n", tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]);
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+int zfh_write_wall_mesh(const char *nodes_path, const char *cells_path, const char *normals_path,
+                        const double *xyz, int64_t nwn, const int32_t *tri, const double *nrm,
+                        int64_t nwt) {
+  FILE *f = fopen(nodes_path, "w");
+  if (!f) return -1;
+  fprintf(f, "POINTS %lld float\n", (long long)nwn);
+  for (int64_t i = 0; i < nwn; i++) fprintf(f, "%.9g %.9g %.9g\n", xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  if (fclose(f)) return -1;
+  if (!(f = fopen(cells_path, "w"))) return -1;
+  fprintf(f, "POLYGONS %lld %lld\n", (long long)nwt, (long long)(4 * nwt));
+  for (int64_t t = 0; t < nwt; t++) fprintf(f, "3 %d %d %d\n", tri[3 * t] - 1, tri[3 * t + 1] - 1, tri[3 * t + 2] - 1);
+  if (fclose(f)) return -1;
+  if (!(f = fopen(normals_path, "w"))) return -1;
+  fprintf(f, "CELL_DATA %lld\nNORMALS Normals float\n", (long long)nwt);
+  for (int64_t t = 0; t < nwt; t++) fprintf(f, "%.9g %.9g %.9g\n", nrm[3 * t], nrm[3 * t + 1], nrm[3 * t + 2]);
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+/* ---- cells: the AllCellMemb layout (ZebraflowWT.c:435-446, read back as
+ * AllCellMembStart at :5327-5430) -------------------------------------------- */
+
+static const char *kCellHeader =
+    "X,Y,Z,Cell,CellStatus,Vx,Vy,Vz,WallTri,WallRepulfX,WallRepulfY,WallRepulfZ,regressflag,"
+    "Aratio,Fx,Fy,Fz,FVx,FVy,FVz,OutNeighFlag\n";
+
+/* Reads node rows (header, then nn rows per cell in node order): X, Y, Z,
+ * Cell (1-based), CellStatus and 16 further columns.  xyz: cap_cells*nn*3,
+ * status: cap_cells.  Returns the number of cells (rows / nn) or < 0. */
+int zfh_read_cells_csv(const char *path, int nn, double *xyz, int32_t *status, int cap_cells) {
+  FILE *f = fopen(path, "r");
+  if (!f || nn <= 0) { if (f) fclose(f); return -1; }
+  if (skip_lines(f, 1)) { fclose(f); return -1; }
+  char line[1024];
+  int64_t row = 0;
+  while (fgets(line, sizeof line, f)) {
+    double x, y, z;
+    int cell, st;
+    if (sscanf(line, "%lf,%lf,%lf,%d,%d", &x, &y, &z, &cell, &st) != 5) break;
+    const int64_t c = row / nn, node = row % nn;
+    if (c >= cap_cells) { fclose(f); return -2; }
+    xyz[3 * (c * nn + node)] = x;
+    xyz[3 * (c * nn + node) + 1] = y;
+    xyz[3 * (c * nn + node) + 2] = z;
+    if (status) status[c] = st;
+    row++;
+  }
+  fclose(f);
+  if (row % nn) return -1;
+  return (int)(row / nn);
+}
+
+/* Writes the cells with status > -5000.  vel, vrep, force: ncell*nn*3;
+ * wall_tri: ncell*nn (any of these may be NULL: zeros).  The reference's
+ * regress flag, area ratio, volume-force split and neighbour flag columns
+ * have no counterpart here and are written as 0. */
+int zfh_write_cells_csv(const char *path, int ncell, int nn, const double *xyz, const double *vel,
+                        const int32_t *status, const int32_t *wall_tri, const double *vrep,
+                        const double *force) {
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fputs(kCellHeader, f);
+  for (int c = 0; c < ncell; c++) {
+    if (status && status[c] <= -5000) continue;
+    for (int n = 0; n < nn; n++) {
+      const size_t q = (size_t)c * nn + n;
+      const double *x = &xyz[3 * q];
+      const double v0 = vel ? vel[3 * q] : 0, v1 = vel ? vel[3 * q + 1] : 0, v2 = vel ? vel[3 * q + 2] : 0;
+      const double r0 = vrep ? vrep[3 * q] : 0, r1 = vrep ? vrep[3 * q + 1] : 0, r2 = vrep ? vrep[3 * q + 2] : 0;
+      const double f0 = force ? force[3 * q] : 0, f1 = force ? force[3 * q + 1] : 0,
+                   f2 = force ? force[3 * q + 2] : 0;
+      fprintf(f, "%f,%f,%f,%d,%d,%.7e,%.7e,%.7e,%d,%.7e,%.7e,%.7e,%d,%.7e,%.7e,%.7e,%.7e,%.7e,%.7e,%.7e,%d\n",
+              x[0], x[1], x[2], c + 1, status ? status[c] : 1, v0, v1, v2, wall_tri ? wall_tri[q] : 0,
+              r0, r1, r2, 0, 0.0, f0, f1, f2, 0.0, 0.0, 0.0, 0);
+    }
+  }
+  return fclose(f) == 0 ? 0 : -1;
+}
+
+/* Appends one row per live cell to VolumeAreaHistory.dat
+ * (ZebraflowWT.c:346-351): Iteration#,Cell,Vol,Area,CenX,CenY,CenZ,
+ * CellState,RegressFlag; the header is written when the file is new. */
+int zfh_append_volume_area(const char *path, int iter, int ncell, const double *vol,
+                           const double *area, const double *centroid, const int32_t *status) {
+  FILE *t = fopen(path, "r");
+  const int fresh = t == NULL;
+  if (t) fclose(t);
+  FILE *f = fopen(path, "a");
+  if (!f) return -1;
+  if (fresh) fprintf(f, "Iteration#,Cell,Vol,Area,CenX,CenY,CenZ,CellState,RegressFlag\n");
+  for (int c = 0; c < ncell; c++)
+    if (!status || status[c] > -5000)
+      fprintf(f, "%d,%d,%.7e,%.7e,%.7e,%.7e,%.7e,%d,%d\n", iter, c + 1, vol[c], area[c],
+              centroid[3 * c], centroid[3 * c + 1], centroid[3 * c + 2], status ? status[c] : 1, 0);
   return fclose(f) == 0 ? 0 : -1;
 }

@@ -221,3 +221,51 @@ int zfh_place_cells_dense(const int32_t *b, int im, int jm, int km, const double
   free(cand);
   return placed;
 }
+
+void zfh_membrane_config_lattice(const double *xyz, int nn, const int32_t *tri, int nt, double es,
+                                 double kb, int max_cells, zf_membrane_config *mc) {
+  zf_membrane_config_reference(mc);
+  /* mean rest edge length over the unique edges (each interior edge of a
+   * closed mesh is shared by two triangles, counted once by a < b) */
+  double lsum = 0;
+  int64_t ne = 0;
+  double area = 0, vol = 0;
+  for (int t = 0; t < nt; t++) {
+    const int v[3] = {tri[3 * t] - 1, tri[3 * t + 1] - 1, tri[3 * t + 2] - 1};
+    for (int s = 0; s < 3; s++) {
+      const int a = v[s], b = v[(s + 1) % 3];
+      if (a < b) {
+        const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+                     dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+        lsum += sqrt(dx * dx + dy * dy + dz * dz);
+        ne++;
+      }
+    }
+    const double *p0 = &xyz[3 * v[0]], *p1 = &xyz[3 * v[1]], *p2 = &xyz[3 * v[2]];
+    const double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double av[3] = {0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
+                          0.5 * (e1[0] * e2[1] - e1[1] * e2[0])};
+    area += sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+    vol += (av[0] * (p0[0] + p1[0] + p2[0]) + av[1] * (p0[1] + p1[1] + p2[1]) +
+            av[2] * (p0[2] + p1[2] + p2[2])) / 9.0;
+  }
+  (void)nn;
+  const double l0 = ne > 0 ? lsum / (double)ne : 1.0;
+  mc->len_scale = 1.0;
+  mc->es = es;
+  mc->kT = 0.0036 * es * l0 * l0;
+  mc->kbend = 2.0 / sqrt(3.0) * kb;
+  mc->kd = 12.0 * es;
+  mc->ka = 25.67 * es;
+  mc->kv = 9.17 * es;
+  mc->area0 = area;
+  mc->vol0 = vol;
+  mc->force_limit = 60.0 * es * l0;
+  mc->vol_limit = 1e30;
+  mc->sliver_len = 0.1;
+  mc->sliver_len_min = 0.002;
+  mc->sliver_f1 = 0.01 * es * l0;
+  mc->sliver_f2 = 6.0 * es * l0;
+  mc->max_cells = max_cells;
+}
