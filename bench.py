@@ -114,16 +114,34 @@ def kernel_roofline(eng, store, n_int, ms, steps, workload):
                 note="boundary nodes are updated inside the same launch")
 
 
-def velocity_error(name, b, steps):
-    """Both forms from the same equilibrium start; rel. L2 velocity error."""
+def node_normals(wx, wt, wn):
+    """Area-weighted unit normals of the wall-mesh nodes."""
+    p = wx[wt - 1]
+    area = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+    acc = np.zeros_like(wx)
+    for k in range(3):
+        np.add.at(acc, wt[:, k] - 1, area[:, None] * wn)
+    return acc / np.linalg.norm(acc, axis=1, keepdims=True)
+
+
+def form_errors(name, b, steps):
+    """Both forms from the same equilibrium start: rel. L2 error of the
+    FP32-deviation form's velocity and of its wall shear stress (zf_wss at
+    every lumen wall-mesh node, mu/DX = 1) against the FP64 form."""
+    from tests import geometry as g
+    wx, wt, wn = g.wall_mesh(b)
+    nrm = node_normals(wx, wt, wn)
     out = {}
     for store in ("f64", "f32dev"):
         eng = engine_for(name, b, store)
         eng.run(0, steps)
-        out[store] = eng.macro()
+        out[store] = (eng.macro(), eng.wss(wx, nrm, 1.0)[0])
         eng.close()
-    u64, u32 = out["f64"][1:4], out["f32dev"][1:4]
-    return float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    u64, u32 = out["f64"][0][1:4], out["f32dev"][0][1:4]
+    w64, w32 = out["f64"][1], out["f32dev"][1]
+    return dict(vel_rel_l2_vs_fp64=float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+                wss_rel_l2_vs_fp64=float(np.linalg.norm(w32 - w64) / np.linalg.norm(w64)),
+                wall_points=len(wx), steps=steps)
 
 
 def cpu_baseline(name, b, seconds):
@@ -161,40 +179,49 @@ def cells_leg(name, b, ncell, steps, warmup):
     nodes, radius 4 lu) placed at random without overlap, membrane shear
     modulus 5e-3 and bending modulus 2e-3 lu, cell-cell (cutoff 1 lu, 2-lu
     cell list) and cell-wall repulsion against the lumen mesh, membrane/IBM
-    work on every step.  Times whole zf_step iterations."""
+    work on every step.  Times whole zf_step iterations of the FP32 form;
+    the FP64 form runs the same iterations for the membrane-shape error
+    (node positions, relative to the mean node displacement)."""
     from tests import geometry as g  # the product's C generators (libzfhost.so)
     from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
                                       lattice_repulsion_config)
     xyz, tris = g.rbc_mesh(radius=4.0)
     cells = g.place_cells_dense(b, xyz, 0.6, ncell, 20261206)
     wall = g.wall_mesh(b)
-    eng = engine_for(name, b, "f32dev")
-    eng.membrane_setup(lattice_membrane_config(xyz, tris, len(cells)), tris, xyz)
-    eng.upload_cells(cells)
-    eng.ibm_setup(lattice_ibm_config(rbc_interval=1))
-    eng.repulsion_setup(lattice_repulsion_config())
-    eng.wall_setup(*wall)
-    for it in range(1, warmup + 1):
-        eng.full_step(it)
-    eng.sync()
-    eng.record(0)
-    for it in range(warmup + 1, warmup + steps + 1):
-        eng.full_step(it)
-    eng.record(1)
-    eng.sync()
-    ms = eng.elapsed_ms(0, 1)
-    _, _, st = eng.cells()
+    res = {}
+    for store in ("f32dev", "f64"):
+        eng = engine_for(name, b, store)
+        eng.membrane_setup(lattice_membrane_config(xyz, tris, len(cells)), tris, xyz)
+        eng.upload_cells(cells)
+        eng.ibm_setup(lattice_ibm_config(rbc_interval=1))
+        eng.repulsion_setup(lattice_repulsion_config())
+        eng.wall_setup(*wall)
+        for it in range(1, warmup + 1):
+            eng.full_step(it)
+        eng.sync()
+        eng.record(0)
+        for it in range(warmup + 1, warmup + steps + 1):
+            eng.full_step(it)
+        eng.record(1)
+        eng.sync()
+        x, _, st = eng.cells()
+        res[store] = (eng.elapsed_ms(0, 1), x, st, eng.n_interior)
+        eng.close()
+    ms, x32, st, n_int = res["f32dev"]
+    x64 = res["f64"][1]
+    disp = np.linalg.norm(x64 - cells, axis=-1).mean()
     nn = len(xyz)
     p = xyz[tris - 1]
     vol = (0.5 * np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]) * p.mean(1)).sum() / 3
-    n_int = eng.n_interior
-    eng.close()
     return dict(cells=len(cells), membrane_nodes=len(cells) * nn,
                 haematocrit=round(len(cells) * vol / n_int, 4), steps=steps, warmup=warmup,
                 ms_per_step=round(ms / steps, 4),
                 rbc_node_updates_per_s=round(len(cells) * nn * steps / (ms * 1e-3), 1),
                 fluid_mlups=round(n_int * steps / (ms * 1e-3) / 1e6, 1),
-                cells_live_after=int((st > 0).sum()), rbc_interval=1)
+                cells_live_after=int((st > 0).sum()), rbc_interval=1,
+                shape_error_vs_fp64=dict(
+                    max_node_dev=float(np.abs(x32 - x64).max()), mean_displacement=float(disp),
+                    rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))))
 
 
 def adjoint_leg(steps=3000, t0=1000, checkpoint=100):
@@ -328,8 +355,7 @@ def main():
                                   ms_per_step=round(ms64 / args.steps, 5),
                                   kernel_avg_us=roof64["avg_us"], kernel_frac=roof64["frac"])
         extra["speedup_vs_fp64_form"] = round(mlups / mlups64, 3)
-        extra["error"] = dict(vel_rel_l2_vs_fp64=velocity_error(args.workload, b, args.err_steps),
-                              steps=args.err_steps)
+        extra["error"] = form_errors(args.workload, b, args.err_steps)
         if not args.no_adjoint:
             extra["adjoint"] = adjoint_leg()
         if args.cells > 0:
