@@ -325,6 +325,20 @@ int zf_adjoint_timing(zf_ctx *ctx, double ms[2]);
  * variables), without rebuilding the context. */
 int zf_set_forcing(zf_ctx *ctx, double inlet_u0, double inlet_amp, const double body_force[3]);
 
+/* ---- transposes of the immersed-boundary coupling --------------------------
+ * No counterpart in the reference (it has no adjoint); for callers that
+ * differentiate through zf_ibm_interpolate / zf_ibm_spread.  With the
+ * current positions X and populations f, for the interpolation
+ * V = vs sum_q w(x_q - X) u_q and the spreading F(s) = fs sum_n w(x_s - X_n) G_n:
+ *   lam_node_force = (dF/dG)^T lam_force               ncell*nn*3 (optional)
+ *   lam_pos        = (dF/dX)^T lam_force + (dV/dX)^T lam_vel    ncell*nn*3
+ *   lam_f          = (dV/df)^T lam_vel                  19*dommax (optional)
+ * lam_force: 3*dommax (list order, NULL = 0); lam_vel, node_force:
+ * ncell*nn*3 (NULL = 0; node_force is required with lam_force).  FP64
+ * arithmetic on the stored populations. */
+int zf_ibm_vjp(zf_ctx *ctx, const double *lam_force, const double *lam_vel, const double *node_force,
+               double *lam_node_force, double *lam_pos, double *lam_f);
+
 /* ---- timing and synchronisation ------------------------------------------ */
 /* Device-copy ceiling: a plain 16-byte-per-lane copy of `bytes` between two
  * fresh buffers, `reps` times; *gbps = 2*bytes*reps / time (read + write). */

@@ -241,6 +241,19 @@ class Engine:
     def full_step(self, it):
         self._check(self.lib.zf_step(self.h, it), "zf_step")
 
+    def ibm_vjp(self, lam_force=None, lam_vel=None, node_force=None, want_f=True):
+        """Transposes of spreading and interpolation at the current state:
+        (lam_node_force [ncell, nn, 3], lam_pos [ncell, nn, 3], lam_f [19, dommax])."""
+        nc, nn = self.ncell, self.nn
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        lf, lv, g = arr(lam_force), arr(lam_vel), arr(node_force)
+        lg = np.zeros((nc, nn, 3))
+        lx = np.zeros((nc, nn, 3))
+        lam = np.zeros((19, self.dommax)) if want_f else None
+        self._check(self.lib.zf_ibm_vjp(self.h, *[None if a is None else _ptr(a)
+                                                  for a in (lf, lv, g, lg, lx, lam)]), "zf_ibm_vjp")
+        return lg, lx, lam
+
     # -- cell list and repulsion ------------------------------------------------
     def repulsion_setup(self, rc):
         self._check(self.lib.zf_repulsion_setup(self.h, C.byref(rc)), "zf_repulsion_setup")
