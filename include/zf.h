@@ -339,6 +339,18 @@ int zf_set_forcing(zf_ctx *ctx, double inlet_u0, double inlet_amp, const double 
 int zf_ibm_vjp(zf_ctx *ctx, const double *lam_force, const double *lam_vel, const double *node_force,
                double *lam_node_force, double *lam_pos, double *lam_f);
 
+/* ---- transpose of the membrane forces ---------------------------------------
+ * No counterpart in the reference; for callers that differentiate through
+ * zf_membrane_forces.  At the current positions, for J = <lam_force, F>:
+ *   lam_pos  = dJ/dX (ncell*nn*3, as lam_force)
+ *   dparam[0] = dJ/d ln es: every constant proportional to the shear modulus
+ *               (kT, kp, kd, ka, kv, the anti-sliver forces, as the lattice-
+ *               unit set-up makes them) scaled together
+ *   dparam[1] = dJ/d ln kbend
+ * Exact local derivatives (dual-number evaluation of the force terms) plus
+ * the cell centroid, area and volume terms; FP64. */
+int zf_membrane_vjp(zf_ctx *ctx, const double *lam_force, double *lam_pos, double dparam[2]);
+
 /* ---- timing and synchronisation ------------------------------------------ */
 /* Device-copy ceiling: a plain 16-byte-per-lane copy of `bytes` between two
  * fresh buffers, `reps` times; *gbps = 2*bytes*reps / time (read + write). */

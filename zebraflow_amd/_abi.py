@@ -106,6 +106,7 @@ FUNCS = {
     "zf_adjoint_timing": (C.c_int, [P, P]),
     "zf_set_forcing": (C.c_int, [P, f64, f64, P]),
     "zf_ibm_vjp": (C.c_int, [P, P, P, P, P, P, P]),
+    "zf_membrane_vjp": (C.c_int, [P, P, P, P]),
     "zf_copy_bandwidth": (C.c_int, [P, i64, i32, C.POINTER(f64)]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),

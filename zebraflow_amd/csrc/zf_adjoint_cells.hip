@@ -1,0 +1,279 @@
+// zf_adjoint_cells.hip -- transpose of the membrane forces (vector-Jacobian
+// product with respect to the node positions and the two modulus scales).
+//
+// F_n(X) of zf_membrane.hip depends on X through the rhombi (edges) that
+// contain node n, the anti-sliver terms of its triangles, and three cell
+// scalars: the centroid cc = mean(X), the global area force
+// f_ag = -ka relax (A - A0)/A0 and the volume force f_vs = -kv (V - V0)/V0.
+// For an adjoint lam (one 3-vector per node):
+//   lam_X(m) = sum_{edges e containing m} sum_{roles r} lam(node_r(e)) . dF_e,r/dX_m
+//            + sum_{triangles t of m} sum_{vertices n of t} lam(n) . dsliver(n, t)/dX_m
+//            + Lag df_ag/dX_m + Lvs df_vs/dX_m + Lcc / nn,
+// with Lag = sum_n lam(n) . dF_n/df_ag (and Lvs, Lcc alike) per cell.  The
+// local derivatives come from the force code itself evaluated in dual
+// numbers (zf_membrane_dev.h): one lane per (cell, node m) seeds X_m and
+// gathers every term that X_m enters, in a fixed order, so no atomics.
+//
+// Modulus scales.  With the lattice-unit constants (kT, kd, ka, kv, kp and
+// the anti-sliver forces proportional to es; kbend to kb), F = F_nb + F_b
+// with F_nb proportional to es and F_b (bending) to kbend, so
+//   dJ/d ln es = sum lam . F_nb,   dJ/d ln kbend = sum lam . F_b.
+#include "zf_membrane_dev.h"
+
+#include <vector>
+
+namespace zf {
+namespace {
+
+using namespace memb;
+using D3 = Dual<3>;
+using D5 = Dual<5>;
+
+inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// Per node: lam . dF/d(f_ag, cc_x, cc_y, cc_z, f_vs), lam . F_nb, lam . F_b.
+// part: 7 x NT.
+// (Dual<5> force terms: 256 VGPRs without spills at 128 lanes per group.)
+__global__ __launch_bounds__(128) void k_memb_cell_terms(
+    const double *__restrict__ X, size_t NT, int ncell, MeshTables mt, const double *__restrict__ tg,
+    size_t TT, const double *__restrict__ cell, const int32_t *__restrict__ status, MembParams mp,
+    const double *__restrict__ lam, double *__restrict__ part) {
+  const int nn = mt.nn;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * nn) return;
+  const int c = idx / nn, n = idx % nn;
+  const size_t q = (size_t)c * nn + n;
+  if (status[c] < 0) {
+    for (int k = 0; k < 7; k++) part[k * NT + q] = 0.0;
+    return;
+  }
+  D5 f_ag(cell[8 * c + 5]), f_vs(cell[8 * c + 6]);
+  V3<D5> cc = {D5(cell[8 * c + 2]), D5(cell[8 * c + 3]), D5(cell[8 * c + 4])};
+  f_ag.d[0] = 1.0;
+  cc.x.d[1] = 1.0;
+  cc.y.d[2] = 1.0;
+  cc.z.d[3] = 1.0;
+  f_vs.d[4] = 1.0;
+  const NodeForce<D5> r = node_force<D5, true>(n, (size_t)c * nn, (size_t)c * mt.nt, X, NT, tg, TT,
+                                               mt, cc, f_ag, f_vs, mp);
+  const double l0 = lam[q], l1 = lam[NT + q], l2 = lam[2 * NT + q];
+#pragma unroll
+  for (int k = 0; k < 5; k++) part[k * NT + q] = l0 * r.f.x.d[k] + l1 * r.f.y.d[k] + l2 * r.f.z.d[k];
+  const double lb = l0 * r.fb.x.v + l1 * r.fb.y.v + l2 * r.fb.z.v;
+  part[5 * NT + q] = l0 * r.f.x.v + l1 * r.f.y.v + l2 * r.f.z.v - lb;
+  part[6 * NT + q] = lb;
+}
+
+// One workgroup per cell: fixed-order tree sums of the 7 node terms.
+// out: 8 per cell.
+__global__ __launch_bounds__(256) void k_memb_cell_sum(const double *__restrict__ part, size_t NT,
+                                                       int nn, const int32_t *__restrict__ status,
+                                                       double *__restrict__ out) {
+  __shared__ double sh[7][256];
+  const int c = blockIdx.x;
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (status[c] >= 0)
+    for (int n = threadIdx.x; n < nn; n += 256)
+      for (int k = 0; k < 7; k++) s[k] += part[k * NT + (size_t)c * nn + n];
+  for (int k = 0; k < 7; k++) sh[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off)
+      for (int k = 0; k < 7; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) out[8 * c + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__device__ __forceinline__ V3<D3> seeded(const double *__restrict__ X, size_t NT, size_t nbase, int node,
+                                         int m) {
+  V3<D3> p = lift<D3>(ld3(X, NT, nbase + node));
+  if (node == m) {
+    p.x.d[0] = 1.0;
+    p.y.d[1] = 1.0;
+    p.z.d[2] = 1.0;
+  }
+  return p;
+}
+
+__device__ __forceinline__ void acc(double g[3], const double *__restrict__ lam, size_t NT, size_t q,
+                                    const V3<D3> &f) {
+  const double l0 = lam[q], l1 = lam[NT + q], l2 = lam[2 * NT + q];
+#pragma unroll
+  for (int a = 0; a < 3; a++) g[a] += l0 * f.x.d[a] + l1 * f.y.d[a] + l2 * f.z.d[a];
+}
+
+// One lane per (cell, node m): lam_X(m) (accumulated into lamX).
+__global__ __launch_bounds__(128) void k_memb_adj_pos(
+    const double *__restrict__ X, size_t NT, int ncell, MeshTables mt, const double *__restrict__ cell,
+    const double *__restrict__ cterm, const int32_t *__restrict__ status, MembParams mp,
+    const double *__restrict__ lam, double *__restrict__ lamX) {
+  const int nn = mt.nn, ne = mt.ne;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * nn) return;
+  const int c = idx / nn, m = idx % nn;
+  if (status[c] < 0) return;
+  const size_t nbase = (size_t)c * nn, q = nbase + m;
+  const V3<D3> cc = {D3(cell[8 * c + 2]), D3(cell[8 * c + 3]), D3(cell[8 * c + 4])};
+  const D3 f_ag(cell[8 * c + 5]);
+  const double fvs = mp.relax * cell[8 * c + 6];
+  double g[3] = {0, 0, 0};
+  // rhombi containing m
+  for (int k = mt.inc_start[m]; k < mt.inc_start[m + 1]; k++) {
+    const int e = mt.inc[k] >> 2;
+    int nd[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) nd[r] = mt.E[r * ne + e];
+    const V3<D3> p1 = seeded(X, NT, nbase, nd[0], m), p2 = seeded(X, NT, nbase, nd[1], m),
+                 p3 = seeded(X, NT, nbase, nd[2], m), p4 = seeded(X, NT, nbase, nd[3], m);
+    const int t1 = mt.E[4 * ne + e], t2 = mt.E[5 * ne + e];
+    const TriG<D3> g1 = tri_geom(seeded(X, NT, nbase, mt.tri[3 * t1], m),
+                                 seeded(X, NT, nbase, mt.tri[3 * t1 + 1], m),
+                                 seeded(X, NT, nbase, mt.tri[3 * t1 + 2], m), mp.L);
+    const TriG<D3> g2 = tri_geom(seeded(X, NT, nbase, mt.tri[3 * t2], m),
+                                 seeded(X, NT, nbase, mt.tri[3 * t2 + 1], m),
+                                 seeded(X, NT, nbase, mt.tri[3 * t2 + 2], m), mp.L);
+    const EdgeConst ec = edge_const(mt, e);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const EdgeOut<D3> o = edge_role(r, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp);
+      acc(g, lam, NT, nbase + nd[r], o.f + fvs * o.fv);
+    }
+  }
+  // anti-sliver terms of m's triangles (every vertex of each), and the
+  // area and volume of those triangles
+  double dA[3] = {0, 0, 0}, dV[3] = {0, 0, 0};
+  const int ntri = mt.node_tri[10 * m];
+  for (int k = 1; k <= ntri && k < 10; k++) {
+    const int t = mt.node_tri[10 * m + k];
+    const int v[3] = {mt.tri[3 * t], mt.tri[3 * t + 1], mt.tri[3 * t + 2]};
+    const V3<D3> pv[3] = {seeded(X, NT, nbase, v[0], m), seeded(X, NT, nbase, v[1], m),
+                          seeded(X, NT, nbase, v[2], m)};
+    const TriG<D3> tgm = tri_geom(pv[0], pv[1], pv[2], mp.L);
+#pragma unroll
+    for (int j = 0; j < 3; j++) acc(g, lam, NT, nbase + v[j], sliver(pv[j], tgm.C, mp));
+    const D3 vol = tgm.a * dot(tgm.C, tgm.m);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      dA[a] += tgm.a.d[a];
+      dV[a] += vol.d[a];
+    }
+  }
+  const double *L = cterm + 8 * c;  // Lag, Lcc(3), Lvs
+  const double kA = -mp.ka * mp.relax / mp.area0, kV = -mp.kv / mp.vol0 * (mp.L / 3.0);
+  const double inv_nn = 1.0 / nn;
+  lamX[q] += g[0] + L[0] * kA * dA[0] + L[4] * kV * dV[0] + L[1] * inv_nn;
+  lamX[NT + q] += g[1] + L[0] * kA * dA[1] + L[4] * kV * dV[1] + L[2] * inv_nn;
+  lamX[2 * NT + q] += g[2] + L[0] * kA * dA[2] + L[4] * kV * dV[2] + L[3] * inv_nn;
+}
+
+// Modulus-scale gradients: sum over live cells in cell order.
+__global__ void k_memb_param_sum(const double *__restrict__ cterm, int ncell,
+                                 const int32_t *__restrict__ status, double *__restrict__ g) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0, b = 0;
+  for (int c = 0; c < ncell; c++)
+    if (status[c] >= 0) {
+      a += cterm[8 * c + 5];
+      b += cterm[8 * c + 6];
+    }
+  g[0] += a;
+  g[1] += b;
+}
+
+__global__ void k_aos_soa(const double *__restrict__ h, int n, size_t NT, double *__restrict__ A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int a = 0; a < 3; a++) A[a * NT + i] = h[3 * (size_t)i + a];
+}
+__global__ void k_soa_aos(const double *__restrict__ A, int n, size_t NT, double *__restrict__ h) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int a = 0; a < 3; a++) h[3 * (size_t)i + a] = A[a * NT + i];
+}
+
+}  // namespace
+
+// Device-level membrane transpose at positions X (SoA over the cell
+// capacity).  lamX += (dF/dX)^T lam; gparam[0..1] += dJ/d ln(es), dJ/d ln(kbend).
+// scratch: 7*ncap*nt (geometry) + 8*ncap (cell scalars) + 7*ncap*nn
+// (node terms) + 8*ncap (cell terms) doubles.
+size_t membrane_vjp_scratch(const zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells;
+  return 8 * (7 * ncap * ctx->nt + 16 * ncap + 7 * ncap * ctx->nn);
+}
+
+int membrane_vjp_device(zf_ctx *ctx, const double *X, const double *lam, double *lamX, double *gparam,
+                        double *scratch) {
+  const int nc = ctx->ncell;
+  if (nc == 0) return ZF_OK;
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
+  double *tg = scratch, *cell = tg + 7 * TT, *part = cell + 8 * ncap, *cterm = part + 7 * NT;
+  int rc = membrane_geometry(ctx, X, tg, cell);
+  if (rc) return rc;
+  const MembParams mp = params_of(ctx->mcfg);
+  const MeshTables mt = mesh_tables(ctx);
+  const int n = nc * ctx->nn;
+  hipLaunchKernelGGL(k_memb_cell_terms, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X, NT, nc, mt, tg,
+                     TT, cell, ctx->d_status, mp, lam, part);
+  hipLaunchKernelGGL(k_memb_cell_sum, dim3(nc), dim3(256), 0, ctx->stream, part, NT, ctx->nn,
+                     ctx->d_status, cterm);
+  hipLaunchKernelGGL(k_memb_adj_pos, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X, NT, nc, mt, cell,
+                     cterm, ctx->d_status, mp, lam, lamX);
+  if (gparam)
+    hipLaunchKernelGGL(k_memb_param_sum, dim3(1), dim3(64), 0, ctx->stream, cterm, nc, ctx->d_status,
+                       gparam);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+}  // namespace zf
+
+using namespace zf;
+
+extern "C" {
+
+int zf_membrane_vjp(zf_ctx *ctx, const double *lam_force, double *lam_pos, double dparam[2]) {
+  if (!ctx || !lam_force || !lam_pos) return ZF_ERR_ARG;
+  if (!ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "no membrane mesh");
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const size_t sbytes = membrane_vjp_scratch(ctx);
+  const size_t bytes = sbytes + 8 * (6 * NT + 3 * (size_t)n + 2);
+  char *buf = nullptr;
+  int rc = dev_alloc(ctx, (void **)&buf, bytes);
+  if (rc) return rc;
+  double *scratch = (double *)buf, *lam = (double *)(buf + sbytes), *lamX = lam + 3 * NT,
+         *stage = lamX + 3 * NT, *gp = stage + 3 * (size_t)n;
+  hipError_t e = hipMemsetAsync(lam, 0, 8 * (6 * NT + 3 * (size_t)n + 2), ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(stage, lam_force, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_aos_soa, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, stage, n, NT, lam);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && (rc = membrane_vjp_device(ctx, ctx->d_x, lam, lamX, gp, scratch))) {
+    ctx->device_bytes -= (int64_t)bytes;
+    dev_free(ctx, (void **)&buf);
+    return rc;
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_soa_aos, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, lamX, n, NT, stage);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(lam_pos, stage, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  double g[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(g, gp, 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)bytes;
+  dev_free(ctx, (void **)&buf);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "membrane vjp: %s", hipGetErrorString(e));
+  if (dparam) {
+    dparam[0] = g[0];
+    dparam[1] = g[1];
+  }
+  return ZF_OK;
+}
+
+}  // extern "C"

@@ -18,7 +18,7 @@
 //                  anti-sliver terms and every rhombus (edge) it belongs to in
 //                  edge order, so F_Total is summed in the reference's order
 //                  with no atomics and no per-edge scratch.
-#include "zf_internal.h"
+#include "zf_membrane_dev.h"
 
 #include <algorithm>
 #include <cmath>
@@ -27,37 +27,9 @@
 namespace zf {
 namespace {
 
-struct MembParams {
-  double L, kT, kbend, kd, ka, kv, area0, vol0, x_max, x_floor, cos_cut, area_lo, area_hi;
-  double force_limit, vol_limit, sliver_len, sliver_len_min, sliver_f1, sliver_f2, relax, relax2;
-};
+using namespace memb;
 
-MembParams params_of(const zf_membrane_config &m) {
-  return MembParams{m.len_scale, m.kT, m.kbend, m.kd, m.ka, m.kv, m.area0, m.vol0, m.x_max,
-                    m.x_floor, m.cos_cut, m.area_lo, m.area_hi, m.force_limit, m.vol_limit,
-                    m.sliver_len, m.sliver_len_min, m.sliver_f1, m.sliver_f2, m.relax, m.relax2};
-}
-
-struct D3 {
-  double x, y, z;
-};
-__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
-__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ D3 cross(D3 a, D3 b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ __forceinline__ D3 ld3(const double *a, size_t stride, size_t i) {
-  return {a[i], a[stride + i], a[2 * stride + i]};
-}
-__device__ __forceinline__ int nbad(D3 f, double lim) {
-  auto b = [lim](double v) { return (isnan(v) || isinf(v) || fabs(v) > lim) ? 1 : 0; };
-  return b(f.x) + b(f.y) + b(f.z);
-}
-
-// Triangle geometry, ZebraflowWT.c:5800-5831: A = (e1 x e3)/2 with
-// e1 = n2-n1, e3 = n1-n3; the outward normal is -A/|A|.
+// Triangle geometry, ZebraflowWT.c:5800-5831: area, outward normal, centroid.
 __global__ void k_tri_geom(const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
                            const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
                            double L, double *__restrict__ tg, size_t TT) {
@@ -66,20 +38,16 @@ __global__ void k_tri_geom(const double *__restrict__ X, size_t NT, int nn, int 
   const int c = idx / nt, t = idx % nt;
   if (status[c] < 0) return;
   const size_t base = (size_t)c * nn;
-  const D3 p1 = ld3(X, NT, base + tri[3 * t]), p2 = ld3(X, NT, base + tri[3 * t + 1]),
-           p3 = ld3(X, NT, base + tri[3 * t + 2]);
-  const D3 cen = (1.0 / 3.0) * (p1 + p2 + p3);
-  const D3 e1 = p2 - p1, e3 = p1 - p3;
-  const D3 A = (0.5 * L * L) * cross(e1, e3);
-  const double a = sqrt(dot(A, A));
+  const TriG<double> g = tri_geom(ld3(X, NT, base + tri[3 * t]), ld3(X, NT, base + tri[3 * t + 1]),
+                                  ld3(X, NT, base + tri[3 * t + 2]), L);
   const size_t q = (size_t)c * nt + t;
-  tg[q] = a;
-  tg[TT + q] = -A.x / a;
-  tg[2 * TT + q] = -A.y / a;
-  tg[3 * TT + q] = -A.z / a;
-  tg[4 * TT + q] = cen.x;
-  tg[5 * TT + q] = cen.y;
-  tg[6 * TT + q] = cen.z;
+  tg[q] = g.a;
+  tg[TT + q] = g.m.x;
+  tg[2 * TT + q] = g.m.y;
+  tg[3 * TT + q] = g.m.z;
+  tg[4 * TT + q] = g.C.x;
+  tg[5 * TT + q] = g.C.y;
+  tg[6 * TT + q] = g.C.z;
 }
 
 // Per-cell FP64 reductions (ZebraflowWT.c:6729-6791, :7517-7524).
@@ -123,116 +91,15 @@ __global__ __launch_bounds__(256) void k_cell_reduce(const double *__restrict__ 
   }
 }
 
-struct EdgeOut {
-  D3 f, fv;
-  int fails;
-  bool has_fv;
-};
-
-// One rhombus (edge) evaluated for one of its four nodes (role r),
-// ZebraflowWT.c:6888-7384.  Within the edge the terms are summed in the
-// reference's order: spring, bending, tri1 local+global area, tri2.
-__device__ EdgeOut edge_role(int r, const int32_t *__restrict__ E, int ne, int e,
-                             const double *__restrict__ EQ, const double *__restrict__ X, size_t NT,
-                             size_t nbase, const double *__restrict__ tg, size_t TT, size_t tbase,
-                             const double *__restrict__ tae, D3 cc, double f_ag,
-                             const MembParams &mp) {
-  const int n1 = E[e], n2 = E[ne + e], n3 = E[2 * ne + e], n4 = E[3 * ne + e];
-  const int t1 = E[4 * ne + e], t2 = E[5 * ne + e];
-  const bool own1 = E[6 * ne + e] != 0, own2 = E[7 * ne + e] != 0;
-  const double lmax = EQ[e], plen = EQ[ne + e], kp = EQ[2 * ne + e], s0 = EQ[3 * ne + e],
-               c0 = EQ[4 * ne + e];
-  const double L = mp.L;
-  const D3 p1 = ld3(X, NT, nbase + n1), p2 = ld3(X, NT, nbase + n2), p3 = ld3(X, NT, nbase + n3),
-           p4 = ld3(X, NT, nbase + n4);
-  EdgeOut o{{0, 0, 0}, {0, 0, 0}, 0, false};
-  // WLC + POW spring (:6888-6924)
-  const D3 d12 = p1 - p2;
-  double len = sqrt(dot(d12, d12));
-  const D3 e1v = (1.0 / len) * d12;
-  double ratio = len * L / lmax;
-  if (ratio > mp.x_max) ratio = mp.x_max;
-  if (ratio < mp.x_floor) len = mp.x_floor * lmax / L;
-  const double wlc = -mp.kT / plen * (ratio - 0.25 + 1.0 / (4.0 * (1 - ratio) * (1 - ratio)));
-  const double comp = kp / (len * len * L * L);
-  if (r == 0) o.f = (mp.relax2 * wlc) * e1v + (mp.relax2 * comp) * e1v;
-  if (r == 1) o.f = (-mp.relax2 * wlc) * e1v + (-mp.relax2 * comp) * e1v;
-  // clamped triangle areas (:6929-6951)
-  const size_t q1 = tbase + t1, q2 = tbase + t2;
-  double a1 = tg[q1], a2 = tg[q2];
-  const double ae1 = tae[t1], ae2 = tae[t2];
-  if (a1 < mp.area_lo * ae1) a1 = mp.area_lo * ae1;
-  else if (a1 > mp.area_hi * ae1) a1 = mp.area_hi * ae1;
-  if (a2 < mp.area_lo * ae2) a2 = mp.area_lo * ae2;
-  else if (a2 > mp.area_hi * ae2) a2 = mp.area_hi * ae2;
-  const D3 m1 = {tg[TT + q1], tg[2 * TT + q1], tg[3 * TT + q1]};
-  const D3 m2 = {tg[TT + q2], tg[2 * TT + q2], tg[3 * TT + q2]};
-  const D3 C1 = {tg[4 * TT + q1], tg[5 * TT + q1], tg[6 * TT + q1]};
-  const D3 C2 = {tg[4 * TT + q2], tg[5 * TT + q2], tg[6 * TT + q2]};
-  double ct = dot(m1, m2);
-  if (ct > 1) ct = 1;
-  else if (ct < -1) ct = -1;
-  const double chk = dot(m1 - m2, L * (C1 - C2));
-  const double st = chk >= 0 ? sqrt(1 - ct * ct) : -sqrt(1 - ct * ct);
-  // bending (:6984-7081)
-  const double bb = fabs(ct) > mp.cos_cut ? 0.0 : mp.kbend * (st * c0 - ct * s0) / st;
-  const D3 dp21 = L * (p2 - p1), dp24 = L * (p2 - p4), dp32 = L * (p3 - p2), dp41 = L * (p4 - p1),
-           dp13 = L * (p1 - p3);
-  const D3 cp1 = a1 * cross(m1, dp21), cp2 = a2 * cross(m2, dp21), cp3 = a1 * cross(m1, dp24),
-           cp4 = a2 * cross(m2, dp24), cp5 = a1 * cross(m1, dp32), cp6 = a2 * cross(m2, dp32),
-           cp7 = a1 * cross(m1, dp41), cp8 = a2 * cross(m2, dp41), cp9 = a1 * cross(m1, dp13),
-           cp10 = a2 * cross(m2, dp13);
-  const double b11 = -bb * ct / (a1 * a1), b12 = bb / (a1 * a2), b22 = -bb * ct / (a2 * a2);
-  D3 fb;
-  if (r == 0) fb = b11 * cp5 + b12 * (cp3 + cp6) + b22 * cp4;
-  else if (r == 1) fb = b11 * cp9 + b12 * (cp7 + cp10) + b22 * cp8;
-  else if (r == 2) fb = b11 * cp1 + b12 * cp2;
-  else fb = (-b12) * cp1 + (-b22) * cp2;
-  const int kb = nbad(fb, mp.force_limit);
-  o.fails += kb;
-  if (!kb) o.f = o.f + fb;
-  // area and volume terms of the triangles this edge owns (:7126-7359)
-  for (int side = 0; side < 2; side++) {
-    if (side == 0 && (!own1 || r == 3)) continue;
-    if (side == 1 && (!own2 || r == 2)) continue;
-    const double a = side ? a2 : a1, ae = side ? ae2 : ae1;
-    const D3 m = side ? m2 : m1;
-    const D3 lc = L * ((side ? C2 : C1) - cc);
-    D3 g, gv;
-    if (side == 0) {
-      g = r == 0 ? cp5 : r == 1 ? cp9 : cp1;
-      gv = r == 0 ? cross(lc, dp32) : r == 1 ? cross(lc, dp13) : cross(lc, dp21);
-    } else {
-      g = r == 0 ? cp4 : r == 1 ? cp8 : (-1.0) * cp2;
-      gv = r == 0 ? cross(lc, dp24) : r == 1 ? cross(lc, dp41) : (-1.0) * cross(lc, dp21);
-    }
-    const double alpha = -mp.kd * mp.relax * (a - ae) / (4.0 * ae * a);
-    const D3 fa = alpha * g;
-    const D3 faa = (0.25 * f_ag / a) * g;
-    const D3 fv = (1.0 / 6.0) * ((a / 3.0) * m + gv);
-    const int ka = nbad(fa, mp.force_limit);
-    o.fails += ka;
-    if (!ka) o.f = o.f + fa;
-    const int kaa = nbad(faa, mp.force_limit);
-    o.fails += kaa;
-    if (!kaa) o.f = o.f + faa;
-    if (!nbad(fv, mp.vol_limit)) {
-      o.fv = o.fv + fv;
-      o.has_fv = true;
-    }
-  }
-  return o;
-}
-
-__global__ void k_node_force(const double *__restrict__ X, size_t NT, int nn, int nt, int ne,
-                             int ncell, const int32_t *__restrict__ tri,
-                             const int32_t *__restrict__ node_tri, const int32_t *__restrict__ inc_start,
-                             const int32_t *__restrict__ inc, const int32_t *__restrict__ E,
-                             const double *__restrict__ EQ, const double *__restrict__ tae,
+// One lane per (cell, node): the node's normal, its anti-sliver terms and
+// every rhombus (edge) it belongs to in edge order, so F_Total is summed in
+// the reference's order with no atomics and no per-edge scratch.
+__global__ void k_node_force(const double *__restrict__ X, size_t NT, int ncell, MeshTables mt,
                              const double *__restrict__ tg, size_t TT, const double *__restrict__ cell,
                              const int32_t *__restrict__ status, MembParams mp,
                              double *__restrict__ F, double *__restrict__ NN,
                              int32_t *__restrict__ fail) {
+  const int nn = mt.nn;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ncell * nn) return;
   const int c = idx / nn, n = idx % nn;
@@ -241,50 +108,16 @@ __global__ void k_node_force(const double *__restrict__ X, size_t NT, int nn, in
     F[q] = F[NT + q] = F[2 * NT + q] = 0;
     return;
   }
-  const size_t nbase = (size_t)c * nn, tbase = (size_t)c * nt;
-  const D3 xn = ld3(X, NT, q);
-  // node area and normal, anti-sliver terms, in triangle order
-  D3 f = {0, 0, 0}, nrm = {0, 0, 0};
-  double narea = 0;
-  const int ntri = node_tri[10 * n];
-  for (int k = 1; k <= ntri && k < 10; k++) {
-    const int t = node_tri[10 * n + k];
-    const size_t qt = tbase + t;
-    const double a = tg[qt];
-    const D3 mt = {tg[TT + qt], tg[2 * TT + qt], tg[3 * TT + qt]};
-    const D3 ct = {tg[4 * TT + qt], tg[5 * TT + qt], tg[6 * TT + qt]};
-    narea += a / 3.0;
-    nrm = nrm + (a / 3.0) * mt;
-    const D3 ev = xn - ct;
-    const double l = sqrt(dot(ev, ev));
-    if (l < mp.sliver_len) {
-      const double mag = l > mp.sliver_len_min ? mp.relax2 * mp.sliver_f1 / l : mp.relax2 * mp.sliver_f2;
-      f = f + (mag / l) * ev;
-    }
-  }
-  const double inv = 1.0 / narea;
-  NN[q] = nrm.x * inv;
-  NN[NT + q] = nrm.y * inv;
-  NN[2 * NT + q] = nrm.z * inv;
-  // every rhombus containing the node, in edge order
-  const D3 cc = {cell[8 * c + 2], cell[8 * c + 3], cell[8 * c + 4]};
-  const double f_ag = cell[8 * c + 5], f_vs = cell[8 * c + 6];
-  D3 fv = {0, 0, 0};
-  int fails = 0;
-  for (int k = inc_start[n]; k < inc_start[n + 1]; k++) {
-    const int code = inc[k];
-    const EdgeOut o = edge_role(code & 3, E, ne, code >> 2, EQ, X, NT, nbase, tg, TT, tbase, tae,
-                                cc, f_ag, mp);
-    f = f + o.f;
-    if (o.has_fv) fv = fv + o.fv;
-    fails += o.fails;
-  }
-  // volume constraint (:7527-7531)
-  f = f + (mp.relax * f_vs) * fv;
-  F[q] = f.x;
-  F[NT + q] = f.y;
-  F[2 * NT + q] = f.z;
-  if (fails) atomicAdd(&fail[c], fails);
+  const V3<double> cc = {cell[8 * c + 2], cell[8 * c + 3], cell[8 * c + 4]};
+  const NodeForce<double> r = node_force<double>(n, (size_t)c * nn, (size_t)c * mt.nt, X, NT, tg, TT,
+                                                 mt, cc, cell[8 * c + 5], cell[8 * c + 6], mp);
+  NN[q] = r.nrm.x;
+  NN[NT + q] = r.nrm.y;
+  NN[2 * NT + q] = r.nrm.z;
+  F[q] = r.f.x;
+  F[NT + q] = r.f.y;
+  F[2 * NT + q] = r.f.z;
+  if (r.fails) atomicAdd(&fail[c], r.fails);
 }
 
 __global__ void k_zero_i32(int32_t *a, int n) {
@@ -630,6 +463,24 @@ int zf_membrane_forces(zf_ctx *ctx) {
 
 }  // extern "C"
 
+memb::MeshTables zf::memb::mesh_tables(const zf_ctx *ctx) {
+  return MeshTables{ctx->d_tri, ctx->d_node_tri, ctx->d_inc_start, ctx->d_inc, ctx->d_edge,
+                    ctx->d_edge_eq, ctx->d_tri_area_eq, ctx->nn, ctx->nt, ctx->ne};
+}
+
+int zf::memb::membrane_geometry(zf_ctx *ctx, const double *X, double *tg, double *cell) {
+  const MembParams mp = params_of(ctx->mcfg);
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
+  const int nc = ctx->ncell;
+  if (nc == 0) return ZF_OK;
+  hipLaunchKernelGGL(k_tri_geom, dim3(nb((size_t)nc * ctx->nt, 256)), dim3(256), 0, ctx->stream, X,
+                     NT, ctx->nn, ctx->nt, nc, ctx->d_tri, ctx->d_status, mp.L, tg, TT);
+  hipLaunchKernelGGL(k_cell_reduce, dim3(nc), dim3(256), 0, ctx->stream, X, NT, ctx->nn, ctx->nt, tg,
+                     TT, ctx->d_status, mp, cell);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
 int zf::membrane_forces_impl(zf_ctx *ctx) {
   ctx->geom_gen++;
   if (ctx->ncell == 0) return ZF_OK;
@@ -637,15 +488,11 @@ int zf::membrane_forces_impl(zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
   const int nc = ctx->ncell;
   hipLaunchKernelGGL(k_zero_i32, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, ctx->d_fail, nc);
-  hipLaunchKernelGGL(k_tri_geom, dim3(nb((size_t)nc * ctx->nt, 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_x, NT, ctx->nn, ctx->nt, nc, ctx->d_tri, ctx->d_status, mp.L, ctx->d_tgeo, TT);
-  hipLaunchKernelGGL(k_cell_reduce, dim3(nc), dim3(256), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
-                     ctx->nt, ctx->d_tgeo, TT, ctx->d_status, mp, ctx->d_cell);
+  int rc = memb::membrane_geometry(ctx, ctx->d_x, ctx->d_tgeo, ctx->d_cell);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_node_force, dim3(nb((size_t)nc * ctx->nn, 128)), dim3(128), 0, ctx->stream,
-                     ctx->d_x, NT, ctx->nn, ctx->nt, ctx->ne, nc, ctx->d_tri, ctx->d_node_tri,
-                     ctx->d_inc_start, ctx->d_inc, ctx->d_edge, ctx->d_edge_eq, ctx->d_tri_area_eq,
-                     ctx->d_tgeo, TT, ctx->d_cell, ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal,
-                     ctx->d_fail);
+                     ctx->d_x, NT, nc, memb::mesh_tables(ctx), ctx->d_tgeo, TT, ctx->d_cell,
+                     ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal, ctx->d_fail);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }

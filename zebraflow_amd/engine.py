@@ -254,6 +254,14 @@ class Engine:
                                                   for a in (lf, lv, g, lg, lx, lam)]), "zf_ibm_vjp")
         return lg, lx, lam
 
+    def membrane_vjp(self, lam_force):
+        """(dJ/dX [ncell, nn, 3], (dJ/d ln es, dJ/d ln kbend)) for J = <lam_force, F(X)>."""
+        lf = np.ascontiguousarray(lam_force, np.float64)
+        lx = np.zeros((self.ncell, self.nn, 3))
+        dp = np.zeros(2)
+        self._check(self.lib.zf_membrane_vjp(self.h, _ptr(lf), _ptr(lx), _ptr(dp)), "zf_membrane_vjp")
+        return lx, dp
+
     # -- cell list and repulsion ------------------------------------------------
     def repulsion_setup(self, rc):
         self._check(self.lib.zf_repulsion_setup(self.h, C.byref(rc)), "zf_repulsion_setup")
