@@ -224,15 +224,20 @@ def cells_leg(name, b, ncell, steps, warmup):
                     rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))))
 
 
-def adjoint_leg(steps=3000, t0=1000, checkpoint=100):
-    """Adjoint cost and FP32-vs-FP64 gradient on a 128x64x64 straight vessel
-    of radius 26 (velocity inlet, pressure outlet, pulsatile inflow): J =
+def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=50, fd_h=1e-4):
+    """Adjoint cost and gradients on a 128x64x64 straight vessel of radius 26
+    (velocity inlet, pressure outlet, pulsatile inflow u0 (1 + 0.5 sin 2 pi
+    t/1000), u0 = 0.005) with `ncell` RBCs
+    (642-node Evans-Fung meshes of radius 4, shear modulus 5e-3, bending
+    modulus 2e-3 lu), membrane/IBM every `interval` iterations: J =
     time-averaged |WSS| over the wall patch 48 <= x <= 80 for iterations
     t0..steps, populations checkpointed every `checkpoint` iterations.
-    Fluid only: the membrane/IBM terms of the adjoint are not included."""
+    Gradients w.r.t. inlet amplitude, es and kb in FP32 and FP64, and central
+    differences of the FP64 forward pass (h = fd_h relative)."""
     from tests import geometry as g
     from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, Engine,
-                                      lattice_config)
+                                      lattice_config, lattice_ibm_config, lattice_membrane_config)
+    es0, kb0, amp0, u0 = 5e-3, 2e-3, 0.5, 0.005
     b = g.tube((128, 64, 64), axis=0, r=26.0, inlet=B_VELOCITY_INLET, outlet=B_PRESSURE_OUTLET)
     wx, wt, wn = g.wall_mesh(b)
     p = wx[wt - 1]
@@ -242,41 +247,89 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100):
         np.add.at(acc, wt[:, k] - 1, area[:, None] * wn)
     nrm = acc / np.linalg.norm(acc, axis=1, keepdims=True)
     patch = (wx[:, 0] >= 48) & (wx[:, 0] <= 80)
-    out = {}
-    for store, k in (("f32dev", checkpoint), ("f32dev", 1), ("f64", checkpoint)):
-        cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], inlet_u0=0.03,
-                             inlet_amp=0.5, inlet_period=1000.0, inlet_dir=(1, 0, 0))
+    xyz, tris = g.rbc_mesh(radius=4.0)
+    # cells start in the upstream half (x < 60), so that none reaches the
+    # outlet within the pass (the peak velocity moves a cell ~45 lu)
+    bp = b.copy()
+    bp[60:] = -1
+    cells = g.place_cells(bp, xyz, 4.0, 1.0, ncell, 20261207)
+    assert len(cells) == ncell
+
+    def build(store, k, es=es0, kb=kb0):
+        cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], inlet_u0=u0,
+                             inlet_amp=amp0, inlet_period=1000.0, inlet_dir=(1, 0, 0))
         eng = Engine(cfg)
         eng.upload_lattice(b)
-        eng.run(0, 10)  # warm-up of the step kernel
+        eng.membrane_setup(lattice_membrane_config(xyz, tris, ncell, es=es, kb=kb), tris, xyz)
+        eng.upload_cells(cells)
+        ic = lattice_ibm_config(rbc_interval=interval)
+        ic.dt_advance = float(interval)
+        eng.ibm_setup(ic)
+        eng.adjoint_setup(wx[patch], nrm[patch], 0.1, t0, steps, steps, k, cells=True)
+        eng.zero_force = np.zeros((3, eng.dommax))
+        return eng
+
+    def reset(eng, amp=amp0):
         eng.init_equilibrium()
+        eng.upload_force(eng.zero_force)
+        eng.upload_cells(cells)
+        eng.set_forcing(u0, amp)
+
+    def J_at(eng, amp):
+        reset(eng, amp)
+        return eng.adjoint_forward(0, steps)
+
+    out = {}
+    for store, k in (("f32dev", checkpoint), ("f32dev", 1), ("f64", checkpoint)):
+        eng = build(store, k)
+        eng.full_run(0, 2 * interval)  # warm-up of every kernel
+        reset(eng)
         eng.sync()
         eng.record(0)
-        eng.run(0, steps)
+        eng.full_run(0, steps)
         eng.record(1)
         eng.sync()
         plain_ms = eng.elapsed_ms(0, 1)
-        eng.init_equilibrium()
-        eng.adjoint_setup(wx[patch], nrm[patch], 0.1, t0, steps, steps, k)
+        reset(eng)
         J = eng.adjoint_forward(0, steps)
         grad = eng.adjoint_backward()
+        dp, _ = eng.adjoint_cell_gradients()
         fwd_ms, bwd_ms = eng.adjoint_timing()
-        out[(store, k)] = dict(J=J, grad=grad, plain_ms=plain_ms, fwd_ms=fwd_ms, bwd_ms=bwd_ms,
+        # natural parameters: dJ/d amp, dJ/d es, dJ/d kb
+        gn = np.array([grad[0], dp[0] / es0, dp[1] / kb0])
+        out[(store, k)] = dict(J=J, g=gn, plain_ms=plain_ms, fwd_ms=fwd_ms, bwd_ms=bwd_ms,
                                n=eng.n_interior)
+        if store == "f64":
+            fd_amp = (J_at(eng, amp0 * (1 + fd_h)) - J_at(eng, amp0 * (1 - fd_h))) / (2 * fd_h * amp0)
         eng.close()
+        print(f"adjoint leg: {store} checkpoint {k} done", flush=True)
+
+    def J_mod(es, kb):
+        eng = build("f64", checkpoint, es, kb)
+        v = J_at(eng, amp0)
+        eng.close()
+        return v
+
+    fd = np.array([fd_amp,
+                   (J_mod(es0 * (1 + fd_h), kb0) - J_mod(es0 * (1 - fd_h), kb0)) / (2 * fd_h * es0),
+                   (J_mod(es0, kb0 * (1 + fd_h)) - J_mod(es0, kb0 * (1 - fd_h))) / (2 * fd_h * kb0)])
     r32, r32all, r64 = out[("f32dev", checkpoint)], out[("f32dev", 1)], out[("f64", checkpoint)]
-    rel = np.abs(r32["grad"][:2] - r64["grad"][:2]) / np.abs(r64["grad"][:2])
-    return dict(workload="straight vessel 128x64x64, r=26, fluid only", fluid_nodes=r32["n"],
-                steps=steps, objective_window=[t0, steps], wall_points=int(patch.sum()),
-                forward_ms=round(r32["plain_ms"], 2),
-                checkpoint=checkpoint, forward_recorded_ms=round(r32["fwd_ms"], 2),
-                backward_ms=round(r32["bwd_ms"], 2),
+    rel = lambda a, b: [float(x) for x in np.abs(a - b) / np.abs(b)]
+    return dict(workload=f"straight vessel 128x64x64, r=26, {ncell} RBCs of {len(xyz)} nodes",
+                fluid_nodes=r32["n"], steps=steps, rbc_interval=interval,
+                objective_window=[t0, steps], wall_points=int(patch.sum()),
+                forward_ms=round(r32["plain_ms"], 2), checkpoint=checkpoint,
+                forward_recorded_ms=round(r32["fwd_ms"], 2), backward_ms=round(r32["bwd_ms"], 2),
                 adjoint_over_forward=round(r32["bwd_ms"] / r32["plain_ms"], 3),
+                adjoint_over_recorded_forward=round(r32["bwd_ms"] / r32["fwd_ms"], 3),
                 every_state_stored=dict(forward_recorded_ms=round(r32all["fwd_ms"], 2),
                                         backward_ms=round(r32all["bwd_ms"], 2),
                                         adjoint_over_forward=round(r32all["bwd_ms"] / r32all["plain_ms"], 3)),
-                dJ_damp=[r32["grad"][0], r64["grad"][0]], dJ_du0=[r32["grad"][1], r64["grad"][1]],
-                grad_rel_err_fp32_vs_fp64=[float(x) for x in rel],
+                gradient_names=["dJ/d inlet_amp", "dJ/d es", "dJ/d kb"],
+                grad_fp32=[float(x) for x in r32["g"]], grad_fp64=[float(x) for x in r64["g"]],
+                grad_fd_fp64=[float(x) for x in fd], fd_h_rel=fd_h,
+                grad_rel_err_fp32_vs_fp64=rel(r32["g"], r64["g"]),
+                grad_rel_err_fp64_vs_fd=rel(r64["g"], fd),
                 J=[r32["J"], r64["J"]])
 
 

@@ -267,6 +267,9 @@ int zf_download_wall_neighbour(zf_ctx *ctx, int32_t *nearest_wall);
  * field), membrane forces, interpolation, repulsion, spreading and the
  * position update; otherwise the LBM step alone. */
 int zf_step(zf_ctx *ctx, int32_t iter);
+/* zf_step for iterations iter0+1 .. iter0+nsteps (the time loop's body,
+ * ZebraflowWT.c:5646-12412, without output). */
+int zf_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps);
 
 /* ---- wall shear stress and wall pressure ----------------------------------
  * At n wall-mesh points (xyz: n*3, unit normals into the fluid: n*3), from
@@ -305,6 +308,9 @@ typedef struct zf_adjoint_config {
   int32_t max_steps;            /* longest forward pass to record */
   int32_t t_begin, t_end;       /* objective window (iteration numbers, inclusive) */
   double mu_over_dx;            /* WSS scale, as in zf_wss */
+  int32_t cells;                /* 1: the forward pass is zf_step with its RBC
+                                   iterations, and the gradient runs through them */
+  int32_t reserved;
 } zf_adjoint_config;
 
 int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, const double *xyz,
@@ -319,6 +325,17 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]);
 /* dJ/d(initial populations), 19*dommax in the layout of
  * zf_upload_populations; dJ/d(force field) 3*dommax (zeros without one). */
 int zf_adjoint_download(zf_ctx *ctx, double *dJ_df0, double *dJ_dforce);
+/* With cells: dparam[0] = dJ/d ln es (every constant proportional to the
+ * shear modulus scaled together, as the lattice-unit set-up makes them),
+ * dparam[1] = dJ/d ln kbend, dJ_dx0 = dJ/d(cell positions at the start of
+ * the forward pass), ncell*nn*3 (either may be NULL).  Each RBC iteration
+ * of the forward pass is recorded (positions, spread node forces, force
+ * field); the backward pass runs the transposes of the position update,
+ * interpolation, spreading and membrane forces through each.  The
+ * repulsion velocities enter as recorded values (their dependence on the
+ * positions is not differentiated: exact while no pair is within range),
+ * and no cell may be removed by the speed check during the pass. */
+int zf_adjoint_cell_gradients(zf_ctx *ctx, double dparam[2], double *dJ_dx0);
 /* Milliseconds of the last forward and backward passes (HIP events). */
 int zf_adjoint_timing(zf_ctx *ctx, double ms[2]);
 /* Inlet and body-force parameters of later iterations (the gradients'

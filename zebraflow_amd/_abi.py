@@ -47,7 +47,7 @@ class ZfRepulsionConfig(C.Structure):
 
 class ZfAdjointConfig(C.Structure):
     _fields_ = [("checkpoint_interval", i32), ("max_steps", i32), ("t_begin", i32), ("t_end", i32),
-                ("mu_over_dx", f64)]
+                ("mu_over_dx", f64), ("cells", i32), ("reserved", i32)]
 
 
 # name -> (restype, argtypes)
@@ -107,6 +107,8 @@ FUNCS = {
     "zf_set_forcing": (C.c_int, [P, f64, f64, P]),
     "zf_ibm_vjp": (C.c_int, [P, P, P, P, P, P, P]),
     "zf_membrane_vjp": (C.c_int, [P, P, P, P]),
+    "zf_adjoint_cell_gradients": (C.c_int, [P, P, P]),
+    "zf_run": (C.c_int, [P, i32, i32]),
     "zf_copy_bandwidth": (C.c_int, [P, i64, i32, C.POINTER(f64)]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256, S == ZF_STORE_F64 ? 1 : 3) void k_adjoint(
     const typename Store<S>::T *__restrict__ src, const typename Adj<S>::A *__restrict__ lin,
     typename Adj<S>::A *__restrict__ lout, uint32_t stride, int n_int, int nblk, int remap,
     const uint32_t *__restrict__ word, const int32_t *__restrict__ block0,
-    const int32_t *__restrict__ chords, const float *__restrict__ force, StepParams prm,
+    const int32_t *__restrict__ chords, const typename Store<S>::F *__restrict__ force, StepParams prm,
     const BcEntry *__restrict__ bc, const int32_t *__restrict__ fused_s,
     const int32_t *__restrict__ att_start, int n_fused, BcParams bp, AdjParams ap,
     const int64_t *__restrict__ unpulled, int64_t n_unpulled, double *__restrict__ gF,
@@ -558,7 +558,8 @@ int adjoint_step(zf_ctx *ctx, const void *src, const void *lin, void *lout, int 
         hipLaunchKernelGGL((k_adjoint<S, true>), dim3(nblk), dim3(256), 0, ctx->stream, st<S>(src),
                            static_cast<const A *>(lin), static_cast<A *>(lout), ctx->stride,
                            ctx->n_int, nblk, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
-                           ctx->d_force, prm, ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
+                           static_cast<const typename Store<S>::F *>(ctx->d_force), prm, ctx->d_bc,
+                           ctx->d_fused_s, ctx->d_att_start,
                            ctx->n_fused_nodes, bp, ap, ctx->d_unpulled, ctx->n_unpulled,
                            ctx->d_gF, ctx->d_gpart);
       else
@@ -595,6 +596,7 @@ void free_adjoint(zf_ctx *ctx) {
                    (void **)&ctx->d_unpulled, (void **)&ctx->d_grads, (void **)&ctx->d_gF,
                    (void **)&ctx->d_objbuf};
   for (auto b : bufs) dev_free(ctx, b);
+  free_adjoint_cells(ctx);
   ctx->have_adj = ctx->adj_ready = false;
   ctx->n_support = 0;
   ctx->n_unpulled = 0;
@@ -645,6 +647,10 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
       (rc = dev_alloc(ctx, (void **)&ctx->d_gpart, 40 * ((size_t)nb(ctx->n_int, 256) + 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_grads, 8 * 8)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_objbuf, 8 * nobj))) {
+    free_adjoint(ctx);
+    return rc;
+  }
+  if (ac->cells && (rc = adjoint_cells_alloc(ctx))) {
     free_adjoint(ctx);
     return rc;
   }
@@ -788,17 +794,46 @@ int zf_adjoint_forward(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double *objec
     return fail(ctx, ZF_ERR_ARG, "objective window [%d, %d] outside iterations %d..%d", a.t_begin,
                 a.t_end, iter0 + 1, iter0 + nsteps);
   const int K = a.checkpoint_interval;
+  const bool cells = ctx->adj_cells && ctx->ncell > 0;
+  const int I = cells ? ctx->icfg.rbc_interval : 1;
+  if (ctx->adj_cells) {
+    // the field in effect at the start; the cell states to compare at the end
+    if ((rc = ensure_force_field(ctx))) return rc;
+    ctx->h_status0.assign(ctx->ncell, 0);
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->ncell)
+      ZF_HIP(ctx, hipMemcpy(ctx->h_status0.data(), ctx->d_status, 4 * (size_t)ctx->ncell,
+                            hipMemcpyDeviceToHost));
+    ZF_HIP(ctx, hipMemcpyAsync(ctx->d_rec_field, ctx->d_force, 3 * ctx->stride * field_elem(ctx),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  ctx->adj_nrec = 0;
   ZF_HIP(ctx, hipEventRecord(ctx->events[12], ctx->stream));
   for (int k = iter0 + 1; k <= iter0 + nsteps; k++) {
     const int r = k - 1 - iter0;  // state S_{k-1}
     if (r % K == 0)
       ZF_HIP(ctx, hipMemcpyAsync(state_at(ctx, ctx->d_ckpt, r / K), ctx->d_f[ctx->cur],
                                  ctx->adj_state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((rc = zf_lbm_step(ctx, k, 0))) return rc;
+    const bool rbc = cells && k % I == 0;
+    if ((rc = zf_lbm_step(ctx, k, rbc ? 1 : 0))) return rc;
+    if (rbc) {
+      // the cell stages, recorded between spreading and the position update
+      if ((rc = rbc_update(ctx, false)) || (rc = adjoint_cells_record(ctx, ctx->adj_nrec)) ||
+          (rc = advance_impl(ctx, nullptr)))
+        return rc;
+      ctx->adj_nrec++;
+    }
     if (k >= a.t_begin && k <= a.t_end) obj_forward(ctx, ctx->d_f[ctx->cur], k);
   }
   ZF_HIP(ctx, hipEventRecord(ctx->events[13], ctx->stream));
   ZF_HIP(ctx, hipGetLastError());
+  if (ctx->adj_cells && ctx->ncell) {
+    std::vector<int32_t> st(ctx->ncell);
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ZF_HIP(ctx, hipMemcpy(st.data(), ctx->d_status, 4 * (size_t)ctx->ncell, hipMemcpyDeviceToHost));
+    if (st != ctx->h_status0)
+      return fail(ctx, ZF_ERR_STATE, "a cell was removed during the recorded forward pass");
+  }
   const size_t nobj = (size_t)(a.t_end - a.t_begin + 1) * ctx->adj_npts;
   std::vector<double> h(nobj);
   ZF_HIP(ctx, hipMemcpyAsync(h.data(), ctx->d_objbuf, 8 * nobj, hipMemcpyDeviceToHost, ctx->stream));
@@ -821,7 +856,26 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
   const zf_adjoint_config &a = ctx->acfg;
   const int K = a.checkpoint_interval, iter0 = ctx->adj_iter0, N = ctx->adj_nsteps;
   const size_t stride = ctx->stride, ab = ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4;
+  // a force field uploaded after the set-up
+  if (ctx->d_force && !ctx->d_gF && (rc = dev_alloc(ctx, (void **)&ctx->d_gF, 24 * ctx->stride)))
+    return rc;
+  const bool cells = ctx->adj_cells && ctx->ncell > 0;
+  const int I = cells ? ctx->icfg.rbc_interval : 1;
+  void *const field0 = ctx->d_force;
+  // the force field iteration k read: the initial one, or the one spread at
+  // the last RBC iteration before k
+  auto field_of = [&](int k) -> void * {
+    if (!cells) return field0;
+    const int m = (k - 1) / I - iter0 / I;
+    return static_cast<char *>(ctx->d_rec_field) + 3 * ctx->stride * field_elem(ctx) * (size_t)m;
+  };
   ZF_HIP(ctx, hipEventRecord(ctx->events[12], ctx->stream));
+  if (ctx->adj_cells) {
+    const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+    hipLaunchKernelGGL(k_fill_a<double>, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_lamX, 3 * NT);
+    hipLaunchKernelGGL(k_fill_a<double>, dim3(1), dim3(64), 0, ctx->stream, ctx->d_cgrad, (size_t)2);
+  }
   const int rows = (int)nb(ctx->n_int, 256) + 1;
   hipLaunchKernelGGL(k_fill_a<double>, dim3(nb(5 * (size_t)rows, 256)), dim3(256), 0, ctx->stream,
                      ctx->d_gpart, 5 * (size_t)rows);
@@ -849,16 +903,29 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
     auto state = [&](int r) -> const void * {  // S_{iter0 + jK + r}
       return r == 0 ? ck : state_at(ctx, ctx->d_seg, r - 1);
     };
-    for (int r = 1; r < len; r++)
-      if ((rc = step_into(ctx, state(r - 1), state_at(ctx, ctx->d_seg, r - 1), iter0 + j * K + r)))
+    for (int r = 1; r < len; r++) {
+      ctx->d_force = field_of(iter0 + j * K + r);
+      if ((rc = step_into(ctx, state(r - 1), state_at(ctx, ctx->d_seg, r - 1), iter0 + j * K + r))) {
+        ctx->d_force = field0;
         return rc;
+      }
+    }
     for (int r = len; r >= 1; r--) {
       const int k = iter0 + j * K + r;  // iteration k reads S_{k-1} = state(r - 1)
-      if ((rc = adjoint_step(ctx, state(r - 1), ctx->d_lam[cur], ctx->d_lam[cur ^ 1], k))) return rc;
+      ctx->d_force = field_of(k);
+      rc = adjoint_step(ctx, state(r - 1), ctx->d_lam[cur], ctx->d_lam[cur ^ 1], k);
+      ctx->d_force = field0;
+      if (rc) return rc;
       cur ^= 1;
       if (k - 1 >= a.t_begin && k - 1 <= a.t_end) obj_adjoint(ctx, state(r - 1), ctx->d_lam[cur]);
+      // RBC iteration k - 1: its cell stages read S_{k-1}; the field they
+      // spread was read by iterations k .. (its gradient is complete now)
+      if (cells && k - 1 > iter0 && (k - 1) % I == 0 &&
+          (rc = adjoint_cells_backward(ctx, (k - 1) / I - iter0 / I - 1, state(r - 1), ctx->d_lam[cur])))
+        return rc;
     }
   }
+  ctx->d_force = field0;
   ctx->adj_cur = cur;
   hipLaunchKernelGGL(k_reduce_grads, dim3(1), dim3(64), 0, ctx->stream, ctx->d_gpart, rows,
                      ctx->d_grads);

@@ -181,6 +181,23 @@ __global__ void k_memb_param_sum(const double *__restrict__ cterm, int ncell,
   g[1] += b;
 }
 
+// G = gamma_p v_rep + F_Total, the node forces spreading received.
+__global__ void k_make_g(const double *__restrict__ vrep, const double *__restrict__ F, size_t n3,
+                         double gamma_p, double *__restrict__ G) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) G[i] = gamma_p * vrep[i] + F[i];
+}
+
+__global__ void k_scale(const double *__restrict__ a, size_t n, double s, double *__restrict__ b) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = s * a[i];
+}
+
+__global__ void k_zero(double *__restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
 __global__ void k_aos_soa(const double *__restrict__ h, int n, size_t NT, double *__restrict__ A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -227,11 +244,96 @@ int membrane_vjp_device(zf_ctx *ctx, const double *X, const double *lam, double 
   return ZF_OK;
 }
 
+void free_adjoint_cells(zf_ctx *ctx) {
+  void **bufs[] = {(void **)&ctx->d_rec_x, (void **)&ctx->d_rec_g, (void **)&ctx->d_rec_field,
+                   (void **)&ctx->d_lamX,  (void **)&ctx->d_lamV,  (void **)&ctx->d_lamG,
+                   (void **)&ctx->d_mscratch, (void **)&ctx->d_cgrad};
+  for (auto b : bufs) dev_free(ctx, b);
+  ctx->adj_cells = false;
+  ctx->adj_rec_cap = ctx->adj_nrec = 0;
+}
+
+int adjoint_cells_alloc(zf_ctx *ctx) {
+  if (!ctx->have_ibm || !ctx->have_mesh) return fail(ctx, ZF_ERR_STATE, "cells need zf_ibm_setup");
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const int cap = ctx->acfg.max_steps / ctx->icfg.rbc_interval + 1;
+  int rc;
+  if ((rc = ensure_force_field(ctx)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_rec_x, 24 * NT * cap)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_rec_g, 24 * NT * cap)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_rec_field, 3 * ctx->stride * field_elem(ctx) * (cap + 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_lamX, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_lamV, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_lamG, 24 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_mscratch, membrane_vjp_scratch(ctx))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_cgrad, 16)))
+    return rc;
+  ctx->adj_rec_cap = cap;
+  ctx->adj_cells = true;
+  return ZF_OK;
+}
+
+int adjoint_cells_record(zf_ctx *ctx, int j) {
+  if (j < 0 || j >= ctx->adj_rec_cap) return fail(ctx, ZF_ERR_STATE, "RBC record %d out of range", j);
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  ZF_HIP(ctx, hipMemcpyAsync(ctx->d_rec_x + 3 * NT * j, ctx->d_x, 24 * NT, hipMemcpyDeviceToDevice,
+                             ctx->stream));
+  hipLaunchKernelGGL(k_make_g, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep,
+                     ctx->d_mforce, 3 * NT, ctx->icfg.gamma_p, ctx->d_rec_g + 3 * NT * j);
+  const size_t fb = 3 * ctx->stride * field_elem(ctx);
+  ZF_HIP(ctx, hipMemcpyAsync(static_cast<char *>(ctx->d_rec_field) + fb * (j + 1), ctx->d_force, fb,
+                             hipMemcpyDeviceToDevice, ctx->stream));
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int adjoint_cells_backward(zf_ctx *ctx, int j, const void *f, void *lam) {
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const double *X = ctx->d_rec_x + 3 * NT * j, *G = ctx->d_rec_g + 3 * NT * j;
+  // x' = x + dt V, V = interpolation + v_rep (v_rep held fixed)
+  hipLaunchKernelGGL(k_scale, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_lamX, 3 * NT,
+                     ctx->icfg.dt_advance, ctx->d_lamV);
+  int rc = ibm_vjp_device(ctx, X, f, G, ctx->d_lamV, ctx->d_gF, ctx->d_lamG, ctx->d_lamX, lam,
+                          ctx->cfg.storage == ZF_STORE_F64);
+  if (rc) return rc;
+  // G = gamma_p v_rep + F_Total(X): the membrane part
+  if ((rc = membrane_vjp_device(ctx, X, ctx->d_lamG, ctx->d_lamX, ctx->d_cgrad, ctx->d_mscratch)))
+    return rc;
+  hipLaunchKernelGGL(k_zero, dim3(nb(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream, ctx->d_gF,
+                     3 * ctx->stride);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
 }  // namespace zf
 
 using namespace zf;
 
 extern "C" {
+
+int zf_adjoint_cell_gradients(zf_ctx *ctx, double dparam[2], double *dJ_dx0) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!ctx->have_adj || !ctx->adj_cells) return fail(ctx, ZF_ERR_STATE, "no adjoint with cells set up");
+  if (!ctx->adj_ready) return fail(ctx, ZF_ERR_STATE, "zf_adjoint_forward not run");
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  double g[2] = {0, 0};
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ZF_HIP(ctx, hipMemcpy(g, ctx->d_cgrad, 16, hipMemcpyDeviceToHost));
+  if (dparam) {
+    dparam[0] = g[0];
+    dparam[1] = g[1];
+  }
+  if (dJ_dx0 && n > 0) {
+    // lamV is free between backward passes
+    hipLaunchKernelGGL(k_soa_aos, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_lamX, n, NT,
+                       ctx->d_lamV);
+    ZF_HIP(ctx, hipGetLastError());
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ZF_HIP(ctx, hipMemcpy(dJ_dx0, ctx->d_lamV, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  }
+  return ZF_OK;
+}
 
 int zf_membrane_vjp(zf_ctx *ctx, const double *lam_force, double *lam_pos, double dparam[2]) {
   if (!ctx || !lam_force || !lam_pos) return ZF_ERR_ARG;

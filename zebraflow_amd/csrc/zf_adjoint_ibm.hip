@@ -284,11 +284,9 @@ int with_form(int storage, F &&f) {
 
 }  // namespace
 
-// Device-level transposes (all arrays on the device; SoA over the cell
-// capacity; lam_F, lam per storage index).  lamX is accumulated; lamG
-// written when non-null; lam (populations) accumulated when non-null.
-int ibm_vjp_device(zf_ctx *ctx, const void *f, const double *G, const double *lamV,
-                   const double *lamF, double *lamG, double *lamX, void *lam, bool lam_double) {
+int ibm_vjp_device(zf_ctx *ctx, const double *X, const void *f, const double *G,
+                   const double *lamV, const double *lamF, double *lamG, double *lamX, void *lam,
+                   bool lam_double) {
   const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
   const int n = ctx->ncell * ctx->nn;
   if (n == 0) return ZF_OK;
@@ -296,14 +294,14 @@ int ibm_vjp_device(zf_ctx *ctx, const void *f, const double *G, const double *la
   with_form(ctx->cfg.storage, [&](auto tag) {
     constexpr int S = decltype(tag)::value;
     using T = typename Store<S>::T;
-    hipLaunchKernelGGL(k_ibm_adj_node<S>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x,
+    hipLaunchKernelGGL(k_ibm_adj_node<S>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X,
                        NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox,
                        static_cast<const T *>(f), ctx->stride, p, G, lamV, lamF, lamG, lamX);
     return 0;
   });
   if (lam && lamV) {
     const uint64_t ntiles = (uint64_t)ctx->ntx * ctx->nty * ctx->ntz;
-    hipLaunchKernelGGL(k_tile_keys, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, NT,
+    hipLaunchKernelGGL(k_tile_keys, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, X, NT,
                        ctx->nn, ctx->ncell, ctx->d_status, p, ctx->d_keys[0]);
     int tbits = 1;
     while ((1ull << tbits) <= ntiles) tbits++;
@@ -319,12 +317,12 @@ int ibm_vjp_device(zf_ctx *ctx, const void *f, const double *G, const double *la
       using T = typename Store<S>::T;
       if (lam_double)
         hipLaunchKernelGGL((k_interp_adj_gather<S, double>), dim3((unsigned)ntiles), dim3(64), 0,
-                           ctx->stream, ctx->d_keys[1], ctx->d_tile_range, ctx->d_x, lamV, NT,
+                           ctx->stream, ctx->d_keys[1], ctx->d_tile_range, X, lamV, NT,
                            ctx->d_b, ctx->d_sbox, static_cast<const T *>(f), ctx->stride, p,
                            static_cast<double *>(lam));
       else
         hipLaunchKernelGGL((k_interp_adj_gather<S, float>), dim3((unsigned)ntiles), dim3(64), 0,
-                           ctx->stream, ctx->d_keys[1], ctx->d_tile_range, ctx->d_x, lamV, NT,
+                           ctx->stream, ctx->d_keys[1], ctx->d_tile_range, X, lamV, NT,
                            ctx->d_b, ctx->d_sbox, static_cast<const T *>(f), ctx->stride, p,
                            static_cast<float *>(lam));
       return 0;
@@ -373,7 +371,7 @@ int zf_ibm_vjp(zf_ctx *ctx, const double *lam_force, const double *lam_vel, cons
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e == hipSuccess &&
-      (rc = ibm_vjp_device(ctx, ctx->d_f[ctx->cur], node_force ? G : nullptr, lam_vel ? lamV : nullptr,
+      (rc = ibm_vjp_device(ctx, ctx->d_x, ctx->d_f[ctx->cur], node_force ? G : nullptr, lam_vel ? lamV : nullptr,
                            lam_force ? lamF : nullptr, lamG, lamX, lam_f ? lam : nullptr, true))) {
     ctx->device_bytes -= (int64_t)bytes;
     dev_free(ctx, (void **)&buf);

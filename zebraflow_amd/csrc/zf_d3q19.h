@@ -12,10 +12,12 @@ namespace d3q19 {
 
 template <int S>
 struct Store;
+// F: element of the force field and of the IBM velocity/force buffers.
 template <>
 struct Store<ZF_STORE_F64> {
   using T = double;
   using R = double;
+  using F = double;
   __device__ static R ld(const T *a, size_t i, double w) { return a[i] - w; }
   __device__ static T st(R g, double w) { return g + w; }
 };
@@ -23,6 +25,7 @@ template <>
 struct Store<ZF_STORE_F32DEV> {
   using T = float;
   using R = float;
+  using F = float;
   __device__ static R ld(const T *a, size_t i, double) { return a[i]; }
   __device__ static T st(R g, double) { return g; }
 };
@@ -30,6 +33,7 @@ template <>
 struct Store<ZF_STORE_F16DEV> {
   using T = _Float16;
   using R = float;
+  using F = float;
   // w applied as hi + lo floats (the float weights alone are biased high)
   __device__ static R ld(const T *a, size_t i, double w) {
     const float h = (float)a[i];

@@ -32,8 +32,8 @@ constexpr double kPi = 3.14159265358979;  // PI, ZebraflowWT.c:33
 inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 // U,V,W of every listed node without the force shift (ZebraflowWT.c:6606-6614).
-template <class T, bool DEV16, bool FULL>
-__global__ void k_macro_vel(const T *__restrict__ f, size_t stride, int n, float *__restrict__ u) {
+template <class T, bool DEV16, bool FULL, class U>
+__global__ void k_macro_vel(const T *__restrict__ f, size_t stride, int n, U *__restrict__ u) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   constexpr int EX[19] = {1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0, 0};
@@ -52,9 +52,9 @@ __global__ void k_macro_vel(const T *__restrict__ f, size_t stride, int n, float
     jz += EZ[c] * g;
   }
   const double rho = 1.0 + drho;
-  u[s] = (float)(jx / rho);
-  u[stride + s] = (float)(jy / rho);
-  u[2 * stride + s] = (float)(jz / rho);
+  u[s] = (U)(jx / rho);
+  u[stride + s] = (U)(jy / rho);
+  u[2 * stride + s] = (U)(jz / rho);
 }
 
 struct IbmParams {
@@ -69,9 +69,10 @@ __device__ __forceinline__ double kern1(double d, double width) {
 // Interpolation, one lane per membrane node: the nodes within the spherical
 // cut-off |d| <= 2 DX all lie in the window [i-2, i+3] of the reference's
 // (int)((x-X0)/DX + 1.01) index, which is what is scanned.
+template <class U>
 __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int ncell,
                          const int32_t *__restrict__ status, const int32_t *__restrict__ b,
-                         const int32_t *__restrict__ sbox, const float *__restrict__ umac,
+                         const int32_t *__restrict__ sbox, const U *__restrict__ umac,
                          size_t stride, IbmParams p, double *__restrict__ V) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
@@ -113,10 +114,11 @@ __device__ __forceinline__ int tile_of(double pos, double origin, double dx, int
 
 // v += v_rep; F_fluid = gamma_p v_rep + F_Total (ZebraflowWT.c:10749-10754);
 // sort key = tile << 32 | node (dead cells sort past the last tile).
+template <class U>
 __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V,
                             const double *__restrict__ vrep, const double *__restrict__ F,
                             size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
-                            IbmParams p, float *__restrict__ ffl, uint64_t *__restrict__ keys) {
+                            IbmParams p, U *__restrict__ ffl, uint64_t *__restrict__ keys) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
   const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
@@ -130,9 +132,9 @@ __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V
     V[a * NT + g] += vr;
     fl[a] = p.gamma_p * vr + F[a * NT + g];
   }
-  ffl[g] = (float)fl[0];
-  ffl[NT + g] = (float)fl[1];
-  ffl[2 * NT + g] = (float)fl[2];
+  ffl[g] = (U)fl[0];
+  ffl[NT + g] = (U)fl[1];
+  ffl[2 * NT + g] = (U)fl[2];
   const int tx = tile_of(X[g], p.x0, p.dx, p.ntx), ty = tile_of(X[NT + g], p.y0, p.dx, p.nty),
             tz = tile_of(X[2 * NT + g], p.z0, p.dx, p.ntz);
   const uint64_t tile = ((uint64_t)tx * p.nty + ty) * p.ntz + tz;
@@ -154,14 +156,19 @@ __global__ void k_tile_ranges(const uint64_t *__restrict__ keys, int n, uint64_t
   if (i == n - 1 || (keys[i + 1] >> 32) != t) range[2 * t + 1] = i + 1;
 }
 
+__device__ __forceinline__ float kcos_pi(float x) { return cospif(x); }
+__device__ __forceinline__ double kcos_pi(double x) { return cospi(x); }
+
 // One wave per tile; lane = lattice node (4x4x4).  Gathers every membrane
-// node of the 27 neighbouring tiles in (tile, node) order.
+// node of the 27 neighbouring tiles in (tile, node) order, in the field's
+// type F (positions relative to the tile origin, weights and sums).
+template <class F>
 __global__ __launch_bounds__(64) void k_spread_gather(
     const uint64_t *__restrict__ keys, const int32_t *__restrict__ range,
-    const double *__restrict__ X, const float *__restrict__ ffl, size_t NT,
+    const double *__restrict__ X, const F *__restrict__ ffl, size_t NT,
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, IbmParams p,
-    float *__restrict__ force, size_t stride) {
-  __shared__ float sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
+    F *__restrict__ force, size_t stride) {
+  __shared__ F sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
   const int tile = blockIdx.x;
   const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
   const int lane = threadIdx.x;
@@ -170,8 +177,8 @@ __global__ __launch_bounds__(64) void k_spread_gather(
   const int li = tx * kTileW + lane / 16, lj = ty * kTileW + (lane / 4) % 4, lk = tz * kTileW + lane % 4;
   const double ox = p.x0 + (double)(tx * kTileW) * p.dx, oy = p.y0 + (double)(ty * kTileW) * p.dx,
                oz = p.z0 + (double)(tz * kTileW) * p.dx;
-  const float rx = (float)(lane / 16), ry = (float)((lane / 4) % 4), rz = (float)(lane % 4);
-  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  const F rx = (F)(lane / 16), ry = (F)((lane / 4) % 4), rz = (F)(lane % 4);
+  F a0 = 0, a1 = 0, a2 = 0;
   bool any = false;
   for (int dxT = -1; dxT <= 1; dxT++)
     for (int dyT = -1; dyT <= 1; dyT++)
@@ -184,9 +191,9 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           const int cnt = min(64, end - c0);
           if (lane < cnt) {
             const int g = (int)(keys[c0 + lane] & 0xffffffffu);
-            sx[lane] = (float)((X[g] - ox) / p.dx);
-            sy[lane] = (float)((X[NT + g] - oy) / p.dx);
-            sz[lane] = (float)((X[2 * NT + g] - oz) / p.dx);
+            sx[lane] = (F)((X[g] - ox) / p.dx);
+            sy[lane] = (F)((X[NT + g] - oy) / p.dx);
+            sz[lane] = (F)((X[2 * NT + g] - oz) / p.dx);
             sf0[lane] = ffl[g];
             sf1[lane] = ffl[NT + g];
             sf2[lane] = ffl[2 * NT + g];
@@ -194,11 +201,11 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           __syncthreads();
           any = true;
           for (int m = 0; m < cnt; m++) {
-            const float dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
-            if (dx * dx + dy * dy + dz * dz > 4.0f) continue;
+            const F dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
+            if (dx * dx + dy * dy + dz * dz > (F)4) continue;
             // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
-            const float w = (1.0f + cospif(0.5f * fabsf(dx))) * (1.0f + cospif(0.5f * fabsf(dy))) *
-                            (1.0f + cospif(0.5f * fabsf(dz))) * (1.0f / 64.0f);
+            const F w = ((F)1 + kcos_pi((F)0.5 * fabs(dx))) * ((F)1 + kcos_pi((F)0.5 * fabs(dy))) *
+                        ((F)1 + kcos_pi((F)0.5 * fabs(dz))) * ((F)1 / (F)64);
             a0 += w * sf0[m];
             a1 += w * sf1[m];
             a2 += w * sf2[m];
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(64) void k_spread_gather(
   const size_t q = ((size_t)li * p.jm + lj) * p.km + lk;
   if (b[q] < 0) return;
   const int s = sbox[q];
-  const float fs = (float)p.force_scale;
+  const F fs = (F)p.force_scale;
   force[s] = a0 * fs;
   force[stride + s] = a1 * fs;
   force[2 * stride + s] = a2 * fs;
@@ -276,16 +283,17 @@ int require_ibm(zf_ctx *ctx) {
   return ZF_OK;
 }
 
+}  // namespace
+
 int ensure_force_field(zf_ctx *ctx) {
   if (ctx->d_force) return ZF_OK;
-  int rc = dev_alloc(ctx, (void **)&ctx->d_force, sizeof(float) * 3 * ctx->stride);
+  const size_t nw = 3 * ctx->stride * field_elem(ctx) / 4;
+  int rc = dev_alloc(ctx, (void **)&ctx->d_force, 4 * nw);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_fill_i32, dim3(nb(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
-                     (int32_t *)ctx->d_force, 3 * ctx->stride, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(nw, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<int32_t *>(ctx->d_force), nw, 0);
   return ZF_OK;
 }
-
-}  // namespace
 
 void free_ibm(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_umac, (void **)&ctx->d_vrep,
@@ -302,21 +310,27 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
   // macro velocities of the current populations
   const void *f = ctx->d_f[ctx->cur];
   const unsigned g = nb(ctx->dommax, 256);
+  float *um = static_cast<float *>(ctx->d_umac);
   switch (ctx->cfg.storage) {
     case ZF_STORE_F64:
-      hipLaunchKernelGGL((k_macro_vel<double, false, true>), dim3(g), dim3(256), 0, ctx->stream,
-                         (const double *)f, ctx->stride, ctx->dommax, ctx->d_umac);
-      break;
+      hipLaunchKernelGGL((k_macro_vel<double, false, true, double>), dim3(g), dim3(256), 0,
+                         ctx->stream, (const double *)f, ctx->stride, ctx->dommax,
+                         static_cast<double *>(ctx->d_umac));
+      hipLaunchKernelGGL(k_interp<double>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
+                         ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox,
+                         static_cast<const double *>(ctx->d_umac), ctx->stride, params(ctx), ctx->d_v);
+      ZF_HIP(ctx, hipGetLastError());
+      return ZF_OK;
     case ZF_STORE_F32DEV:
-      hipLaunchKernelGGL((k_macro_vel<float, false, false>), dim3(g), dim3(256), 0, ctx->stream,
-                         (const float *)f, ctx->stride, ctx->dommax, ctx->d_umac);
+      hipLaunchKernelGGL((k_macro_vel<float, false, false, float>), dim3(g), dim3(256), 0, ctx->stream,
+                         (const float *)f, ctx->stride, ctx->dommax, um);
       break;
     default:
-      hipLaunchKernelGGL((k_macro_vel<_Float16, true, false>), dim3(g), dim3(256), 0, ctx->stream,
-                         (const _Float16 *)f, ctx->stride, ctx->dommax, ctx->d_umac);
+      hipLaunchKernelGGL((k_macro_vel<_Float16, true, false, float>), dim3(g), dim3(256), 0,
+                         ctx->stream, (const _Float16 *)f, ctx->stride, ctx->dommax, um);
   }
-  hipLaunchKernelGGL(k_interp, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
-                     ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, ctx->d_umac, ctx->stride,
+  hipLaunchKernelGGL(k_interp<float>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
+                     ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, um, ctx->stride,
                      params(ctx), ctx->d_v);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
@@ -330,12 +344,19 @@ int ibm_spread_impl(zf_ctx *ctx) {
   const IbmParams p = params(ctx);
   const uint64_t ntiles = (uint64_t)ctx->ntx * ctx->nty * ctx->ntz;
   // the force field is rebuilt from zero (cleared by a fill kernel)
-  hipLaunchKernelGGL(k_fill_i32, dim3(nb(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
-                     (int32_t *)ctx->d_force, 3 * ctx->stride, 0);
+  const size_t nw = 3 * ctx->stride * field_elem(ctx) / 4;
+  const bool f64 = field_elem(ctx) == 8;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(nw, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<int32_t *>(ctx->d_force), nw, 0);
   if (n == 0) return ZF_OK;
-  hipLaunchKernelGGL(k_node_prep, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, ctx->d_v,
-                     ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
-                     ctx->d_ffl, ctx->d_keys[0]);
+  if (f64)
+    hipLaunchKernelGGL(k_node_prep<double>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
+                       ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
+                       static_cast<double *>(ctx->d_ffl), ctx->d_keys[0]);
+  else
+    hipLaunchKernelGGL(k_node_prep<float>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
+                       ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
+                       static_cast<float *>(ctx->d_ffl), ctx->d_keys[0]);
   // end bit: tile ids need ceil(log2(ntiles+1)) bits above the 32 node bits
   int tbits = 1;
   while ((1ull << tbits) <= ntiles) tbits++;
@@ -346,9 +367,16 @@ int ibm_spread_impl(zf_ctx *ctx) {
                      ctx->d_tile_range, (size_t)(2 * ntiles), 0);
   hipLaunchKernelGGL(k_tile_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1], n,
                      ntiles, ctx->d_tile_range);
-  hipLaunchKernelGGL(k_spread_gather, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
-                     ctx->d_keys[1], ctx->d_tile_range, ctx->d_x, ctx->d_ffl, NT, ctx->d_b,
-                     ctx->d_sbox, p, ctx->d_force, ctx->stride);
+  if (f64)
+    hipLaunchKernelGGL(k_spread_gather<double>, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
+                       ctx->d_keys[1], ctx->d_tile_range, ctx->d_x,
+                       static_cast<const double *>(ctx->d_ffl), NT, ctx->d_b, ctx->d_sbox, p,
+                       static_cast<double *>(ctx->d_force), ctx->stride);
+  else
+    hipLaunchKernelGGL(k_spread_gather<float>, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
+                       ctx->d_keys[1], ctx->d_tile_range, ctx->d_x,
+                       static_cast<const float *>(ctx->d_ffl), NT, ctx->d_b, ctx->d_sbox, p,
+                       static_cast<float *>(ctx->d_force), ctx->stride);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }
@@ -376,6 +404,21 @@ int advance_impl(zf_ctx *ctx, int32_t *n_removed) {
     ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   return ZF_OK;
+}
+
+int rbc_update(zf_ctx *ctx, bool advance) {
+  int rc;
+  if ((rc = membrane_forces_impl(ctx))) return rc;
+  if (ctx->have_rep && (rc = cell_list_impl(ctx))) return rc;
+  if ((rc = ibm_interpolate_impl(ctx))) return rc;
+  if (ctx->have_rep) {
+    if ((rc = repulsion_impl(ctx))) return rc;
+  } else {
+    const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+    hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
+  }
+  if ((rc = ibm_spread_impl(ctx))) return rc;
+  return advance ? advance_impl(ctx, nullptr) : ZF_OK;
 }
 
 }  // namespace zf
@@ -420,9 +463,9 @@ int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
   const size_t ntiles = (size_t)ctx->ntx * ctx->nty * ctx->ntz;
   if (NT >= (1ull << 32) || ntiles >= (1ull << 31)) return fail(ctx, ZF_ERR_ARG, "too many nodes or tiles");
   int rc;
-  if ((rc = dev_alloc(ctx, (void **)&ctx->d_umac, 12 * ctx->stride)) ||
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_umac, 3 * field_elem(ctx) * ctx->stride)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_vrep, 24 * NT)) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_ffl, 12 * NT)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_ffl, 3 * field_elem(ctx) * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_keys[0], 8 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_keys[1], 8 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_tile_range, 8 * ntiles)) ||
@@ -513,17 +556,16 @@ int zf_step(zf_ctx *ctx, int32_t iter) {
   const bool rbc = ctx->have_ibm && ctx->ncell > 0 && iter % ctx->icfg.rbc_interval == 0;
   int rc = zf_lbm_step(ctx, iter, rbc ? 1 : 0);
   if (rc || !rbc) return rc;
-  if ((rc = membrane_forces_impl(ctx))) return rc;
-  if (ctx->have_rep && (rc = cell_list_impl(ctx))) return rc;
-  if ((rc = ibm_interpolate_impl(ctx))) return rc;
-  if (ctx->have_rep) {
-    if ((rc = repulsion_impl(ctx))) return rc;
-  } else {
-    const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
-    hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
+  return rbc_update(ctx, true);
+}
+
+int zf_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps) {
+  if (!ctx || nsteps < 0) return ZF_ERR_ARG;
+  for (int t = 1; t <= nsteps; t++) {
+    const int rc = zf_step(ctx, iter0 + t);
+    if (rc) return rc;
   }
-  if ((rc = ibm_spread_impl(ctx))) return rc;
-  return advance_impl(ctx, nullptr);
+  return ZF_OK;
 }
 
 }  // extern "C"

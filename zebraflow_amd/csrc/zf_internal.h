@@ -87,7 +87,7 @@ struct zf_ctx {
   // state
   void *d_f[2] = {nullptr, nullptr};  // population buffers (storage-typed)
   int cur = 0;                        // buffer holding the current state
-  float *d_force = nullptr;           // 3 x stride, storage-indexed; null = none
+  void *d_force = nullptr;            // 3 x stride, storage-indexed, field_elem bytes each; null = none
   double *d_sums = nullptr;           // reduction scratch
   int64_t device_bytes = 0;
 
@@ -116,9 +116,9 @@ struct zf_ctx {
   bool have_ibm = false;
   zf_ibm_config icfg{};
   int32_t *d_sbox = nullptr;      // box: storage index of the listed node or -1
-  float *d_umac = nullptr;        // 3 x stride: U,V,W without force shift (s order)
+  void *d_umac = nullptr;         // 3 x stride: U,V,W without force shift (s order), field_elem
   double *d_vrep = nullptr;       // 3 x (ncap*nn) repulsion velocity v_rep
-  float *d_ffl = nullptr;         // 3 x (ncap*nn): F_fluid of each node
+  void *d_ffl = nullptr;          // 3 x (ncap*nn): F_fluid of each node, field_elem
   uint64_t *d_keys[2] = {nullptr, nullptr};  // (tile << 32 | node) sort keys
   void *d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
@@ -175,6 +175,17 @@ struct zf_ctx {
   double *d_gF = nullptr;          // 3 x stride: dJ/d(force field), when a field exists
   double *d_objbuf = nullptr;      // |tau| per objective iteration and probe
   float adj_ms[2] = {0, 0};        // forward, backward
+  // cells in the adjoint: records of every RBC iteration of the forward pass
+  bool adj_cells = false;
+  int32_t adj_rec_cap = 0, adj_nrec = 0;
+  double *d_rec_x = nullptr;       // per RBC iteration: positions before the update, 3 x ncap*nn
+  double *d_rec_g = nullptr;       // per RBC iteration: spread node forces gamma_p v_rep + F_Total
+  void *d_rec_field = nullptr;     // force field at the start, then after each RBC iteration, 3 x stride
+  double *d_lamX = nullptr;        // dJ/d positions, 3 x ncap*nn
+  double *d_lamV = nullptr, *d_lamG = nullptr;  // scratch of one RBC iteration, 3 x ncap*nn
+  double *d_mscratch = nullptr;    // membrane transpose scratch
+  double *d_cgrad = nullptr;       // dJ/d ln es, dJ/d ln kbend
+  std::vector<int32_t> h_status0;  // cell status at the start of the forward pass
 };
 
 namespace zf {
@@ -209,5 +220,38 @@ struct GridGeom {
 GridGeom grid_geom(const zf_ctx *ctx);
 int alloc_state(zf_ctx *ctx);
 size_t store_bytes(int storage);
+// Bytes of one element of the force field and of the IBM velocity and
+// node-force buffers: FP64 with FP64 storage (the FP64 form is FP64
+// throughout), FP32 otherwise.
+inline size_t field_elem(const zf_ctx *ctx) { return ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4; }
+// The cell stages of an RBC iteration after its LBM step (membrane forces,
+// cell list, interpolation, repulsion, spreading) and, with `advance`, the
+// position update: the order of zf_step.
+int rbc_update(zf_ctx *ctx, bool advance);
+int advance_impl(zf_ctx *ctx, int32_t *n_removed);
+int ensure_force_field(zf_ctx *ctx);
+// Transposes of the IBM coupling at positions X and populations f (device
+// arrays, SoA over the cell capacity; lamF and lam storage-indexed; lam of
+// the adjoint arithmetic type, double when lam_double).  lamX and lam are
+// accumulated, lamG written; null inputs count as zero.
+int ibm_vjp_device(zf_ctx *ctx, const double *X, const void *f, const double *G,
+                   const double *lamV, const double *lamF, double *lamG, double *lamX, void *lam,
+                   bool lam_double);
+// Transpose of the membrane forces at positions X: lamX += (dF/dX)^T lam,
+// gparam[0..1] += dJ/d ln es, dJ/d ln kbend; scratch of
+// membrane_vjp_scratch(ctx) bytes.
+size_t membrane_vjp_scratch(const zf_ctx *ctx);
+// Cells in the adjoint (zf_adjoint_cells.hip): buffers, the record of RBC
+// iteration j of the forward pass (after spreading, before the position
+// update), and its backward step: lamX holds dJ/d(positions after the
+// update) on entry and dJ/d(positions before it) on return; gF (dJ/d the
+// field spread at this iteration) is consumed and cleared; lam (dJ/d the
+// populations the interpolation read) gains the interpolation's transpose.
+int adjoint_cells_alloc(zf_ctx *ctx);
+void free_adjoint_cells(zf_ctx *ctx);
+int adjoint_cells_record(zf_ctx *ctx, int j);
+int adjoint_cells_backward(zf_ctx *ctx, int j, const void *f, void *lam);
+int membrane_vjp_device(zf_ctx *ctx, const double *X, const double *lam, double *lamX, double *gparam,
+                        double *scratch);
 
 }  // namespace zf

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     const typename Store<S>::T *__restrict__ src, typename Store<S>::T *__restrict__ dst,
     size_t stride, int n_int, int nblk_int, int remap, const uint32_t *__restrict__ word,
     const int32_t *__restrict__ block0, const int32_t *__restrict__ chords,
-    const float *__restrict__ force, StepParams prm, const BcEntry *__restrict__ bc,
+    const typename Store<S>::F *__restrict__ force, StepParams prm, const BcEntry *__restrict__ bc,
     const int32_t *__restrict__ fused_s, const int32_t *__restrict__ att_start, int n_fused,
     int n_bc_fused, int n_bc, BcParams bp) {
   using R = typename Store<S>::R;
@@ -239,15 +239,17 @@ __global__ void k_init_eq(typename Store<S>::T *__restrict__ f0, typename Store<
   }
 }
 
+template <class F>
 __global__ void k_force_import(const double *__restrict__ fp, int n, const int32_t *__restrict__ s_of_p,
-                               float *__restrict__ force, size_t stride) {
+                               F *__restrict__ force, size_t stride) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int s = s_of_p[p];
-  for (int a = 0; a < 3; a++) force[(size_t)a * stride + s] = (float)fp[(size_t)a * n + p];
+  for (int a = 0; a < 3; a++) force[(size_t)a * stride + s] = (F)fp[(size_t)a * n + p];
 }
 
-__global__ void k_force_export(const float *__restrict__ force, size_t stride, int n,
+template <class F>
+__global__ void k_force_export(const F *__restrict__ force, size_t stride, int n,
                                const int32_t *__restrict__ s_of_p, double *__restrict__ fp) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -255,9 +257,10 @@ __global__ void k_force_export(const float *__restrict__ force, size_t stride, i
   for (int a = 0; a < 3; a++) fp[(size_t)a * n + p] = force[(size_t)a * stride + s];
 }
 
-__global__ void k_fill_zero(float *__restrict__ a, size_t n) {
+// Clears n 32-bit words (a force field is field_elem / 4 words per entry).
+__global__ void k_fill_zero(uint32_t *__restrict__ a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0.0f;
+  if (i < n) a[i] = 0u;
 }
 
 // Conservation sums over interior nodes in FP64: per-block partials, then a
@@ -320,7 +323,8 @@ void launch_stream(zf_ctx *ctx, const void *src, void *dst, const StepParams &pr
   hipLaunchKernelGGL((k_stream_collide<S, FIELD, NT, ACC>), dim3(nblk_int + nblk_late), dim3(256), 0,
                      ctx->stream, static_cast<const T *>(src), static_cast<T *>(dst), ctx->stride,
                      ctx->n_int, nblk_int, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
-                     FIELD ? ctx->d_force : nullptr, prm, ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
+                     FIELD ? static_cast<const typename Store<S>::F *>(ctx->d_force) : nullptr, prm,
+                     ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
                      ctx->n_fused_nodes, ctx->n_bc_fused, ctx->n_bc, bp);
 }
 
@@ -352,9 +356,11 @@ int step_impl(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
   if (ctx->d_force) launch_stream_opts<S, true>(ctx, src, dst, prm, bp);
   else launch_stream_opts<S, false>(ctx, src, dst, prm, bp);
   if (ev) ZF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-  if (rbc_step && ctx->d_force)
-    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
-                       ctx->d_force, 3 * ctx->stride);
+  if (rbc_step && ctx->d_force) {
+    const size_t nw = 3 * ctx->stride * field_elem(ctx) / 4;
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(nw, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<uint32_t *>(ctx->d_force), nw);
+  }
   ZF_HIP(ctx, hipGetLastError());
   ctx->cur ^= 1;
   return ZF_OK;
@@ -550,24 +556,31 @@ int zf_upload_force(zf_ctx *ctx, const double *force) {
   if (!ctx) return ZF_ERR_ARG;
   int rc = require_state(ctx);
   if (rc) return rc;
+  const size_t fbytes = field_elem(ctx) * 3 * ctx->stride;
   if (!force) {
     ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->d_force) ctx->device_bytes -= (int64_t)(sizeof(float) * 3 * ctx->stride);
+    if (ctx->d_force) ctx->device_bytes -= (int64_t)fbytes;
     dev_free(ctx, (void **)&ctx->d_force);
     return ZF_OK;
   }
   if (!ctx->d_force) {
-    if ((rc = dev_alloc(ctx, (void **)&ctx->d_force, sizeof(float) * 3 * ctx->stride))) return rc;
-    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(3 * ctx->stride, 256)), dim3(256), 0, ctx->stream,
-                       ctx->d_force, 3 * ctx->stride);
+    if ((rc = dev_alloc(ctx, (void **)&ctx->d_force, fbytes))) return rc;
+    hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(fbytes / 4, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<uint32_t *>(ctx->d_force), fbytes / 4);
   }
   double *tmp = nullptr;
   const size_t nbytes = sizeof(double) * 3 * (size_t)ctx->dommax;
   if ((rc = dev_alloc(ctx, (void **)&tmp, nbytes))) return rc;
   hipError_t e = hipMemcpyAsync(tmp, force, nbytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_force_import, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
-                       tmp, ctx->dommax, ctx->d_s_of_p, ctx->d_force, ctx->stride);
+    if (field_elem(ctx) == 8)
+      hipLaunchKernelGGL(k_force_import<double>, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0,
+                         ctx->stream, tmp, ctx->dommax, ctx->d_s_of_p,
+                         static_cast<double *>(ctx->d_force), ctx->stride);
+    else
+      hipLaunchKernelGGL(k_force_import<float>, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0,
+                         ctx->stream, tmp, ctx->dommax, ctx->d_s_of_p,
+                         static_cast<float *>(ctx->d_force), ctx->stride);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -588,8 +601,14 @@ int zf_download_force(zf_ctx *ctx, double *force) {
   double *tmp = nullptr;
   const size_t nbytes = sizeof(double) * 3 * (size_t)ctx->dommax;
   if ((rc = dev_alloc(ctx, (void **)&tmp, nbytes))) return rc;
-  hipLaunchKernelGGL(k_force_export, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_force, ctx->stride, ctx->dommax, ctx->d_s_of_p, tmp);
+  if (field_elem(ctx) == 8)
+    hipLaunchKernelGGL(k_force_export<double>, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0,
+                       ctx->stream, static_cast<const double *>(ctx->d_force), ctx->stride,
+                       ctx->dommax, ctx->d_s_of_p, tmp);
+  else
+    hipLaunchKernelGGL(k_force_export<float>, dim3(nblocks(ctx->dommax, 256)), dim3(256), 0,
+                       ctx->stream, static_cast<const float *>(ctx->d_force), ctx->stride,
+                       ctx->dommax, ctx->d_s_of_p, tmp);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(force, tmp, nbytes, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -241,6 +241,9 @@ class Engine:
     def full_step(self, it):
         self._check(self.lib.zf_step(self.h, it), "zf_step")
 
+    def full_run(self, iter0, nsteps):
+        self._check(self.lib.zf_run(self.h, iter0, nsteps), "zf_run")
+
     def ibm_vjp(self, lam_force=None, lam_vel=None, node_force=None, want_f=True):
         """Transposes of spreading and interpolation at the current state:
         (lam_node_force [ncell, nn, 3], lam_pos [ncell, nn, 3], lam_f [19, dommax])."""
@@ -305,9 +308,10 @@ class Engine:
         return out, pr, va
 
     # -- adjoint ------------------------------------------------------------------
-    def adjoint_setup(self, xyz, normal, mu_over_dx, t_begin, t_end, max_steps, checkpoint=100):
+    def adjoint_setup(self, xyz, normal, mu_over_dx, t_begin, t_end, max_steps, checkpoint=100,
+                      cells=False):
         from ._abi import ZfAdjointConfig
-        ac = ZfAdjointConfig(checkpoint, max_steps, t_begin, t_end, mu_over_dx)
+        ac = ZfAdjointConfig(checkpoint, max_steps, t_begin, t_end, mu_over_dx, int(cells), 0)
         xyz = np.ascontiguousarray(xyz, np.float64)
         normal = np.ascontiguousarray(normal, np.float64)
         self._check(self.lib.zf_adjoint_setup(self.h, C.byref(ac), len(xyz), _ptr(xyz), _ptr(normal)),
@@ -330,6 +334,14 @@ class Engine:
         gf = np.zeros((3, self.dommax))
         self._check(self.lib.zf_adjoint_download(self.h, _ptr(lam), _ptr(gf)), "zf_adjoint_download")
         return lam, gf
+
+    def adjoint_cell_gradients(self):
+        """((dJ/d ln es, dJ/d ln kbend), dJ/dX0 [ncell, nn, 3])."""
+        dp = np.zeros(2)
+        gx = np.zeros((self.ncell, self.nn, 3))
+        self._check(self.lib.zf_adjoint_cell_gradients(self.h, _ptr(dp), _ptr(gx)),
+                    "zf_adjoint_cell_gradients")
+        return dp, gx
 
     def adjoint_timing(self):
         ms = np.zeros(2)
