@@ -202,8 +202,15 @@ typedef struct zf_ibm_config {
   double dt_advance;      /* DT*SOLVERSTAGGER */
   double vmax;            /* a cell is removed above this node speed (5e-3) */
   int32_t rbc_interval;   /* SOLVERSTAGGER: membrane/IBM every n-th iteration */
-  int32_t reserved;
+  int32_t flags;          /* ZF_IBM_* */
 } zf_ibm_config;
+
+/* Kernel support of interpolation and spreading: the reference's is the
+ * 4x4x4 cosine kernel cut to the sphere |d| <= 2 DX, where the weight is not
+ * zero, so the coupling jumps when a lattice node crosses that sphere.
+ * ZF_IBM_CUBE_SUPPORT keeps the whole cube |d_a| < 2 DX instead: continuous
+ * (C1) in the positions, for differentiating the model at scale. */
+#define ZF_IBM_CUBE_SUPPORT 1
 
 /* Reference values (ZebraflowWT.c:40-74, :95-97, :124, :151). */
 void zf_ibm_config_reference(zf_ibm_config *ic);

@@ -105,3 +105,33 @@ def test_position_transpose():
         fd = (J(cells + h * d) - J(cells - h * d)) / (2 * h)
         np.testing.assert_allclose(lx[1, 17, a], fd, rtol=1e-6, atol=1e-9)
     eng.close()
+
+
+def test_cube_support_transpose():
+    """ZF_IBM_CUBE_SUPPORT (no spherical cut-off): the position transpose
+    against central differences of the device's own FP64 interpolation and
+    spreading with that support."""
+    b, domid, n, eng, cells, f0, p, rng = setup()
+    print("case cube-support transpose", flush=True)
+    eng.ibm_setup(ibm_config(x0=ORIGIN[0], y0=ORIGIN[1], z0=ORIGIN[2], dx=DX, vel_scale=VS,
+                             force_scale=FS, flags=1))
+    G = 1e-3 * rng.standard_normal(cells.shape)
+    lamF = rng.standard_normal((3, n))
+    lamV = rng.standard_normal(cells.shape)
+    _, lx, _ = eng.ibm_vjp(lam_force=lamF, lam_vel=lamV, node_force=G)
+
+    def J(x):
+        eng.upload_cells(x)
+        eng.ibm_interpolate()
+        _, v, _ = eng.cells()
+        eng.upload_membrane_force(G)
+        eng.upload_repulsion(np.zeros_like(G))
+        eng.ibm_spread()
+        return (lamF * eng.force()).sum() + (lamV * v).sum()
+
+    for trial in range(2):
+        d = rng.standard_normal(cells.shape)
+        h = 1e-6
+        fd = (J(cells + h * d) - J(cells - h * d)) / (2 * h)
+        np.testing.assert_allclose((lx * d).sum(), fd, rtol=1e-6)
+    eng.close()

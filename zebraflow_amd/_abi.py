@@ -37,7 +37,7 @@ class ZfMembraneConfig(C.Structure):
 class ZfIbmConfig(C.Structure):
     _fields_ = [(n, f64) for n in (
         "x0", "y0", "z0", "dx", "vel_scale", "force_scale", "gamma_p", "dt_advance", "vmax")] + [
-        ("rbc_interval", i32), ("reserved", i32)]
+        ("rbc_interval", i32), ("flags", i32)]
 
 
 class ZfRepulsionConfig(C.Structure):

@@ -37,12 +37,19 @@ inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 struct AP {
   double x0, y0, z0, dx, vs, fs;
   int im, jm, km, ntx, nty, ntz;
+  int cube;  // ZF_IBM_CUBE_SUPPORT
 };
+
+// Outside the kernel support (d in lattice units), as the forward kernels.
+__device__ __forceinline__ bool off_support(double dx, double dy, double dz, int cube) {
+  if (cube) return fabs(dx) >= 2.0 || fabs(dy) >= 2.0 || fabs(dz) >= 2.0;
+  return dx * dx + dy * dy + dz * dz > 4.0;
+}
 
 AP aparams(const zf_ctx *ctx) {
   return AP{ctx->icfg.x0, ctx->icfg.y0, ctx->icfg.z0, ctx->icfg.dx, ctx->icfg.vel_scale,
             ctx->icfg.force_scale, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km, ctx->ntx, ctx->nty,
-            ctx->ntz};
+            ctx->ntz, (ctx->icfg.flags & ZF_IBM_CUBE_SUPPORT) ? 1 : 0};
 }
 
 // Kernel weight and its gradient with respect to the membrane node position.
@@ -106,7 +113,7 @@ __global__ void k_ibm_adj_node(const double *__restrict__ X, size_t NT, int nn, 
       const double dyl = (p.y0 + (jj - 1) * p.dx - yp) * inv;
       for (int kk = k1; kk <= k2; kk++) {
         const double dzl = (p.z0 + (kk - 1) * p.dx - zp) * inv;
-        if (dxl * dxl + dyl * dyl + dzl * dzl > 4.0) continue;
+        if (off_support(dxl, dyl, dzl, p.cube)) continue;
         const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
         const int bv = b[q];
         if (bv < 0) continue;
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(64) void k_interp_adj_gather(
           __syncthreads();
           for (int m = 0; m < cnt; m++) {
             const double dxl = (qx - sx[m]) * inv, dyl = (qy - sy[m]) * inv, dzl = (qz - sz[m]) * inv;
-            if (dxl * dxl + dyl * dyl + dzl * dzl > 4.0) continue;
+            if (off_support(dxl, dyl, dzl, p.cube)) continue;
             const double w = (1.0 + cos(kPiA * fabs(dxl) * 0.5)) * (1.0 + cos(kPiA * fabs(dyl) * 0.5)) *
                              (1.0 + cos(kPiA * fabs(dzl) * 0.5)) / 64.0;
             a0 += w * s0[m];

@@ -60,7 +60,16 @@ __global__ void k_macro_vel(const T *__restrict__ f, size_t stride, int n, U *__
 struct IbmParams {
   double x0, y0, z0, dx, vel_scale, force_scale, gamma_p, dt_advance, vmax;
   int im, jm, km, ntx, nty, ntz;
+  int cube;  // ZF_IBM_CUBE_SUPPORT: no spherical cut-off
 };
+
+// Outside the kernel support: the sphere |d| > 2 (reference), or the cube
+// max |d_a| >= 2; d in lattice units.
+template <class F>
+__device__ __forceinline__ bool off_support(F dx, F dy, F dz, int cube) {
+  if (cube) return fabs(dx) >= (F)2 || fabs(dy) >= (F)2 || fabs(dz) >= (F)2;
+  return dx * dx + dy * dy + dz * dz > (F)4;
+}
 
 __device__ __forceinline__ double kern1(double d, double width) {
   return 1.0 + cos(kPi * fabs(d) / width);
@@ -90,7 +99,9 @@ __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int nc
       const double dyv = p.y0 + (jj - 1) * p.dx - yp;
       for (int kk = k1; kk <= k2; kk++) {
         const double dzv = p.z0 + (kk - 1) * p.dx - zp;
-        if (dxv * dxv + dyv * dyv + dzv * dzv > w2) continue;
+        if (p.cube ? (fabs(dxv) >= width || fabs(dyv) >= width || fabs(dzv) >= width)
+                   : dxv * dxv + dyv * dyv + dzv * dzv > w2)
+          continue;
         const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
         const int bv = b[q];
         if (!(bv > 0 && bv < 4998)) continue;
@@ -202,7 +213,7 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           any = true;
           for (int m = 0; m < cnt; m++) {
             const F dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
-            if (dx * dx + dy * dy + dz * dz > (F)4) continue;
+            if (off_support(dx, dy, dz, p.cube)) continue;
             // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
             const F w = ((F)1 + kcos_pi((F)0.5 * fabs(dx))) * ((F)1 + kcos_pi((F)0.5 * fabs(dy))) *
                         ((F)1 + kcos_pi((F)0.5 * fabs(dz))) * ((F)1 / (F)64);
@@ -273,7 +284,8 @@ __global__ void k_zero_f64(double *a, size_t n) {
 IbmParams params(const zf_ctx *ctx) {
   const zf_ibm_config &c = ctx->icfg;
   return IbmParams{c.x0, c.y0, c.z0, c.dx, c.vel_scale, c.force_scale, c.gamma_p, c.dt_advance,
-                   c.vmax, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km, ctx->ntx, ctx->nty, ctx->ntz};
+                   c.vmax, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km, ctx->ntx, ctx->nty, ctx->ntz,
+                   (c.flags & ZF_IBM_CUBE_SUPPORT) ? 1 : 0};
 }
 
 int require_ibm(zf_ctx *ctx) {
