@@ -31,50 +31,88 @@ using D5 = Dual<5>;
 
 inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-// Per node: lam . dF/d(f_ag, cc_x, cc_y, cc_z, f_vs), lam . F_nb, lam . F_b.
-// part: 7 x NT.
-// (Dual<5> force terms: 256 VGPRs without spills at 128 lanes per group.)
+// The transposes run one lane per (cell, node m, slot j), j < J = inc_max + 1:
+// slot j < count(m) takes the j-th rhombus containing m, slot count(m) the
+// node's own terms (anti-sliver, and in pass 3 its triangles' area and
+// volume); the rest write zeros.  Partial results go to slot-major scratch
+// and are summed in slot order, so the sums do not depend on scheduling.
+
+// Pass 1: lam(m) . d(contribution)/d(f_ag, cc_x, cc_y, cc_z, f_vs), and the
+// contribution's non-bending and bending parts against lam(m).  part: 7 x
+// NT*J.  (Dual<5> terms: 256 VGPRs without spills at 128 lanes per group.)
 __global__ __launch_bounds__(128) void k_memb_cell_terms(
-    const double *__restrict__ X, size_t NT, int ncell, MeshTables mt, const double *__restrict__ tg,
-    size_t TT, const double *__restrict__ cell, const int32_t *__restrict__ status, MembParams mp,
-    const double *__restrict__ lam, double *__restrict__ part) {
-  const int nn = mt.nn;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ncell * nn) return;
-  const int c = idx / nn, n = idx % nn;
-  const size_t q = (size_t)c * nn + n;
-  if (status[c] < 0) {
-    for (int k = 0; k < 7; k++) part[k * NT + q] = 0.0;
-    return;
-  }
-  D5 f_ag(cell[8 * c + 5]), f_vs(cell[8 * c + 6]);
-  V3<D5> cc = {D5(cell[8 * c + 2]), D5(cell[8 * c + 3]), D5(cell[8 * c + 4])};
-  f_ag.d[0] = 1.0;
-  cc.x.d[1] = 1.0;
-  cc.y.d[2] = 1.0;
-  cc.z.d[3] = 1.0;
-  f_vs.d[4] = 1.0;
-  const NodeForce<D5> r = node_force<D5, true>(n, (size_t)c * nn, (size_t)c * mt.nt, X, NT, tg, TT,
-                                               mt, cc, f_ag, f_vs, mp);
-  const double l0 = lam[q], l1 = lam[NT + q], l2 = lam[2 * NT + q];
+    const double *__restrict__ X, size_t NT, int ncell, MeshTables mt, int J,
+    const double *__restrict__ tg, size_t TT, const double *__restrict__ cell,
+    const int32_t *__restrict__ status, MembParams mp, const double *__restrict__ lam,
+    double *__restrict__ part) {
+  const int nn = mt.nn, ne = mt.ne;
+  const size_t NJ = NT * J;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)ncell * nn * J) return;
+  const int j = (int)(idx % J);
+  const size_t q = idx / J;
+  const int c = (int)(q / nn), m = (int)(q % nn);
+  const int k0 = mt.inc_start[m], cnt = mt.inc_start[m + 1] - k0;
+  double out[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (status[c] >= 0 && j <= cnt) {
+    const size_t nbase = (size_t)c * nn, tbase = (size_t)c * mt.nt;
+    const double l0 = lam[q], l1 = lam[NT + q], l2 = lam[2 * NT + q];
+    if (j < cnt) {
+      D5 f_ag(cell[8 * c + 5]), f_vs(cell[8 * c + 6]);
+      V3<D5> cc = {D5(cell[8 * c + 2]), D5(cell[8 * c + 3]), D5(cell[8 * c + 4])};
+      f_ag.d[0] = 1.0;
+      cc.x.d[1] = 1.0;
+      cc.y.d[2] = 1.0;
+      cc.z.d[3] = 1.0;
+      f_vs.d[4] = 1.0;
+      const int code = mt.inc[k0 + j], e = code >> 2;
+      const V3<D5> p1 = lift<D5>(ld3(X, NT, nbase + mt.E[e])),
+                   p2 = lift<D5>(ld3(X, NT, nbase + mt.E[ne + e])),
+                   p3 = lift<D5>(ld3(X, NT, nbase + mt.E[2 * ne + e])),
+                   p4 = lift<D5>(ld3(X, NT, nbase + mt.E[3 * ne + e]));
+      const TriG<D5> g1 = tri_load<D5>(tg, TT, tbase + mt.E[4 * ne + e]);
+      const TriG<D5> g2 = tri_load<D5>(tg, TT, tbase + mt.E[5 * ne + e]);
+      const EdgeConst ec = edge_const(mt, e);
+      EdgeOut<D5> o;
+      switch (code & 3) {  // one inlined copy per role
+        case 0: o = edge_role(0, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp); break;
+        case 1: o = edge_role(1, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp); break;
+        case 2: o = edge_role(2, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp); break;
+        default: o = edge_role(3, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp);
+      }
+      // this rhombus's share of F(m): f + relax f_vs fv
+      const V3<D5> t = o.f + (mp.relax * f_vs) * o.fv;
 #pragma unroll
-  for (int k = 0; k < 5; k++) part[k * NT + q] = l0 * r.f.x.d[k] + l1 * r.f.y.d[k] + l2 * r.f.z.d[k];
-  const double lb = l0 * r.fb.x.v + l1 * r.fb.y.v + l2 * r.fb.z.v;
-  part[5 * NT + q] = l0 * r.f.x.v + l1 * r.f.y.v + l2 * r.f.z.v - lb;
-  part[6 * NT + q] = lb;
+      for (int k = 0; k < 5; k++) out[k] = l0 * t.x.d[k] + l1 * t.y.d[k] + l2 * t.z.d[k];
+      const double lb = l0 * o.fb.x.v + l1 * o.fb.y.v + l2 * o.fb.z.v;
+      out[5] = l0 * t.x.v + l1 * t.y.v + l2 * t.z.v - lb;
+      out[6] = lb;
+    } else {  // the anti-sliver terms of m (shear-modulus part)
+      const V3<double> xn = ld3(X, NT, q);
+      V3<double> f = {0, 0, 0};
+      const int ntri = mt.node_tri[10 * m];
+      for (int k = 1; k <= ntri && k < 10; k++) {
+        const size_t qt = tbase + mt.node_tri[10 * m + k];
+        f = f + sliver(xn, V3<double>{tg[4 * TT + qt], tg[5 * TT + qt], tg[6 * TT + qt]}, mp);
+      }
+      out[5] = l0 * f.x + l1 * f.y + l2 * f.z;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 7; k++) part[k * NJ + idx] = out[k];
 }
 
-// One workgroup per cell: fixed-order tree sums of the 7 node terms.
-// out: 8 per cell.
-__global__ __launch_bounds__(256) void k_memb_cell_sum(const double *__restrict__ part, size_t NT,
-                                                       int nn, const int32_t *__restrict__ status,
+// Pass 2, one workgroup per cell: fixed-order tree sums of the 7 terms over
+// the cell's slots.  out: 8 per cell.
+__global__ __launch_bounds__(256) void k_memb_cell_sum(const double *__restrict__ part, size_t NJ,
+                                                       int nslot, const int32_t *__restrict__ status,
                                                        double *__restrict__ out) {
   __shared__ double sh[7][256];
   const int c = blockIdx.x;
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
   if (status[c] >= 0)
-    for (int n = threadIdx.x; n < nn; n += 256)
-      for (int k = 0; k < 7; k++) s[k] += part[k * NT + (size_t)c * nn + n];
+    for (int i = threadIdx.x; i < nslot; i += 256)
+      for (int k = 0; k < 7; k++) s[k] += part[k * NJ + (size_t)c * nslot + i];
   for (int k = 0; k < 7; k++) sh[k][threadIdx.x] = s[k];
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
@@ -103,68 +141,91 @@ __device__ __forceinline__ void acc(double g[3], const double *__restrict__ lam,
   for (int a = 0; a < 3; a++) g[a] += l0 * f.x.d[a] + l1 * f.y.d[a] + l2 * f.z.d[a];
 }
 
-// One lane per (cell, node m): lam_X(m) (accumulated into lamX).
+// Pass 3: d/dX_m of the slot's terms against lam.  Slot j < count(m): the
+// j-th rhombus containing m, all four roles; slot count(m): the anti-sliver
+// terms of m's triangles (every vertex of each) and the cell-scalar chain
+// through those triangles' area and volume and the centroid.  part: 3 x NT*J.
 __global__ __launch_bounds__(128) void k_memb_adj_pos(
-    const double *__restrict__ X, size_t NT, int ncell, MeshTables mt, const double *__restrict__ cell,
-    const double *__restrict__ cterm, const int32_t *__restrict__ status, MembParams mp,
-    const double *__restrict__ lam, double *__restrict__ lamX) {
+    const double *__restrict__ X, size_t NT, int ncell, MeshTables mt, int J,
+    const double *__restrict__ cell, const double *__restrict__ cterm,
+    const int32_t *__restrict__ status, MembParams mp, const double *__restrict__ lam,
+    double *__restrict__ part) {
   const int nn = mt.nn, ne = mt.ne;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ncell * nn) return;
-  const int c = idx / nn, m = idx % nn;
-  if (status[c] < 0) return;
-  const size_t nbase = (size_t)c * nn, q = nbase + m;
-  const V3<D3> cc = {D3(cell[8 * c + 2]), D3(cell[8 * c + 3]), D3(cell[8 * c + 4])};
-  const D3 f_ag(cell[8 * c + 5]);
-  const double fvs = mp.relax * cell[8 * c + 6];
+  const size_t NJ = NT * J;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)ncell * nn * J) return;
+  const int j = (int)(idx % J);
+  const size_t q = idx / J;
+  const int c = (int)(q / nn), m = (int)(q % nn);
+  const int k0 = mt.inc_start[m], cnt = mt.inc_start[m + 1] - k0;
   double g[3] = {0, 0, 0};
-  // rhombi containing m
-  for (int k = mt.inc_start[m]; k < mt.inc_start[m + 1]; k++) {
-    const int e = mt.inc[k] >> 2;
-    int nd[4];
+  if (status[c] >= 0 && j <= cnt) {
+    const size_t nbase = (size_t)c * nn;
+    if (j < cnt) {
+      const V3<D3> cc = {D3(cell[8 * c + 2]), D3(cell[8 * c + 3]), D3(cell[8 * c + 4])};
+      const D3 f_ag(cell[8 * c + 5]);
+      const double fvs = mp.relax * cell[8 * c + 6];
+      const int e = mt.inc[k0 + j] >> 2;
+      int nd[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) nd[r] = mt.E[r * ne + e];
-    const V3<D3> p1 = seeded(X, NT, nbase, nd[0], m), p2 = seeded(X, NT, nbase, nd[1], m),
-                 p3 = seeded(X, NT, nbase, nd[2], m), p4 = seeded(X, NT, nbase, nd[3], m);
-    const int t1 = mt.E[4 * ne + e], t2 = mt.E[5 * ne + e];
-    const TriG<D3> g1 = tri_geom(seeded(X, NT, nbase, mt.tri[3 * t1], m),
-                                 seeded(X, NT, nbase, mt.tri[3 * t1 + 1], m),
-                                 seeded(X, NT, nbase, mt.tri[3 * t1 + 2], m), mp.L);
-    const TriG<D3> g2 = tri_geom(seeded(X, NT, nbase, mt.tri[3 * t2], m),
-                                 seeded(X, NT, nbase, mt.tri[3 * t2 + 1], m),
-                                 seeded(X, NT, nbase, mt.tri[3 * t2 + 2], m), mp.L);
-    const EdgeConst ec = edge_const(mt, e);
+      for (int r = 0; r < 4; r++) nd[r] = mt.E[r * ne + e];
+      const V3<D3> p1 = seeded(X, NT, nbase, nd[0], m), p2 = seeded(X, NT, nbase, nd[1], m),
+                   p3 = seeded(X, NT, nbase, nd[2], m), p4 = seeded(X, NT, nbase, nd[3], m);
+      const int t1 = mt.E[4 * ne + e], t2 = mt.E[5 * ne + e];
+      const TriG<D3> g1 = tri_geom(seeded(X, NT, nbase, mt.tri[3 * t1], m),
+                                   seeded(X, NT, nbase, mt.tri[3 * t1 + 1], m),
+                                   seeded(X, NT, nbase, mt.tri[3 * t1 + 2], m), mp.L);
+      const TriG<D3> g2 = tri_geom(seeded(X, NT, nbase, mt.tri[3 * t2], m),
+                                   seeded(X, NT, nbase, mt.tri[3 * t2 + 1], m),
+                                   seeded(X, NT, nbase, mt.tri[3 * t2 + 2], m), mp.L);
+      const EdgeConst ec = edge_const(mt, e);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const EdgeOut<D3> o = edge_role(r, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp);
-      acc(g, lam, NT, nbase + nd[r], o.f + fvs * o.fv);
+      for (int r = 0; r < 4; r++) {
+        const EdgeOut<D3> o = edge_role(r, p1, p2, p3, p4, g1, g2, ec, cc, f_ag, mp);
+        acc(g, lam, NT, nbase + nd[r], o.f + fvs * o.fv);
+      }
+    } else {
+      double dA[3] = {0, 0, 0}, dV[3] = {0, 0, 0};
+      const int ntri = mt.node_tri[10 * m];
+      for (int k = 1; k <= ntri && k < 10; k++) {
+        const int t = mt.node_tri[10 * m + k];
+        const int v[3] = {mt.tri[3 * t], mt.tri[3 * t + 1], mt.tri[3 * t + 2]};
+        const V3<D3> pv[3] = {seeded(X, NT, nbase, v[0], m), seeded(X, NT, nbase, v[1], m),
+                              seeded(X, NT, nbase, v[2], m)};
+        const TriG<D3> tgm = tri_geom(pv[0], pv[1], pv[2], mp.L);
+#pragma unroll
+        for (int i = 0; i < 3; i++) acc(g, lam, NT, nbase + v[i], sliver(pv[i], tgm.C, mp));
+        const D3 vol = tgm.a * dot(tgm.C, tgm.m);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          dA[a] += tgm.a.d[a];
+          dV[a] += vol.d[a];
+        }
+      }
+      const double *L = cterm + 8 * c;  // Lag, Lcc(3), Lvs
+      const double kA = -mp.ka * mp.relax / mp.area0, kV = -mp.kv / mp.vol0 * (mp.L / 3.0);
+      const double inv_nn = 1.0 / nn;
+#pragma unroll
+      for (int a = 0; a < 3; a++) g[a] += L[0] * kA * dA[a] + L[4] * kV * dV[a] + L[1 + a] * inv_nn;
     }
   }
-  // anti-sliver terms of m's triangles (every vertex of each), and the
-  // area and volume of those triangles
-  double dA[3] = {0, 0, 0}, dV[3] = {0, 0, 0};
-  const int ntri = mt.node_tri[10 * m];
-  for (int k = 1; k <= ntri && k < 10; k++) {
-    const int t = mt.node_tri[10 * m + k];
-    const int v[3] = {mt.tri[3 * t], mt.tri[3 * t + 1], mt.tri[3 * t + 2]};
-    const V3<D3> pv[3] = {seeded(X, NT, nbase, v[0], m), seeded(X, NT, nbase, v[1], m),
-                          seeded(X, NT, nbase, v[2], m)};
-    const TriG<D3> tgm = tri_geom(pv[0], pv[1], pv[2], mp.L);
 #pragma unroll
-    for (int j = 0; j < 3; j++) acc(g, lam, NT, nbase + v[j], sliver(pv[j], tgm.C, mp));
-    const D3 vol = tgm.a * dot(tgm.C, tgm.m);
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      dA[a] += tgm.a.d[a];
-      dV[a] += vol.d[a];
-    }
+  for (int a = 0; a < 3; a++) part[a * NJ + idx] = g[a];
+}
+
+// lamX(m) += the node's slots, in slot order.
+__global__ void k_memb_adj_sum(const double *__restrict__ part, size_t NT, int n, int J,
+                               const int32_t *__restrict__ inc_start, int nn,
+                               const int32_t *__restrict__ status, double *__restrict__ lamX) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n || status[q / nn] < 0) return;
+  const int m = q % nn, cnt = inc_start[m + 1] - inc_start[m];
+  const size_t NJ = NT * J;
+  for (int a = 0; a < 3; a++) {
+    double s = 0;
+    for (int j = 0; j <= cnt; j++) s += part[a * NJ + (size_t)q * J + j];
+    lamX[a * NT + q] += s;
   }
-  const double *L = cterm + 8 * c;  // Lag, Lcc(3), Lvs
-  const double kA = -mp.ka * mp.relax / mp.area0, kV = -mp.kv / mp.vol0 * (mp.L / 3.0);
-  const double inv_nn = 1.0 / nn;
-  lamX[q] += g[0] + L[0] * kA * dA[0] + L[4] * kV * dV[0] + L[1] * inv_nn;
-  lamX[NT + q] += g[1] + L[0] * kA * dA[1] + L[4] * kV * dV[1] + L[2] * inv_nn;
-  lamX[2 * NT + q] += g[2] + L[0] * kA * dA[2] + L[4] * kV * dV[2] + L[3] * inv_nn;
 }
 
 // Modulus-scale gradients: sum over live cells in cell order.
@@ -217,7 +278,7 @@ __global__ void k_soa_aos(const double *__restrict__ A, int n, size_t NT, double
 // (node terms) + 8*ncap (cell terms) doubles.
 size_t membrane_vjp_scratch(const zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells;
-  return 8 * (7 * ncap * ctx->nt + 16 * ncap + 7 * ncap * ctx->nn);
+  return 8 * (7 * ncap * ctx->nt + 16 * ncap + 7 * ncap * ctx->nn * (ctx->inc_max + 1));
 }
 
 int membrane_vjp_device(zf_ctx *ctx, const double *X, const double *lam, double *lamX, double *gparam,
@@ -225,18 +286,21 @@ int membrane_vjp_device(zf_ctx *ctx, const double *X, const double *lam, double 
   const int nc = ctx->ncell;
   if (nc == 0) return ZF_OK;
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
-  double *tg = scratch, *cell = tg + 7 * TT, *part = cell + 8 * ncap, *cterm = part + 7 * NT;
+  double *tg = scratch, *cell = tg + 7 * TT, *cterm = cell + 8 * ncap, *part = cterm + 8 * ncap;
   int rc = membrane_geometry(ctx, X, tg, cell);
   if (rc) return rc;
   const MembParams mp = params_of(ctx->mcfg);
   const MeshTables mt = mesh_tables(ctx);
-  const int n = nc * ctx->nn;
-  hipLaunchKernelGGL(k_memb_cell_terms, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X, NT, nc, mt, tg,
-                     TT, cell, ctx->d_status, mp, lam, part);
-  hipLaunchKernelGGL(k_memb_cell_sum, dim3(nc), dim3(256), 0, ctx->stream, part, NT, ctx->nn,
-                     ctx->d_status, cterm);
-  hipLaunchKernelGGL(k_memb_adj_pos, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X, NT, nc, mt, cell,
-                     cterm, ctx->d_status, mp, lam, lamX);
+  const int n = nc * ctx->nn, J = ctx->inc_max + 1;
+  const size_t lanes = (size_t)n * J;
+  hipLaunchKernelGGL(k_memb_cell_terms, dim3(nb(lanes, 128)), dim3(128), 0, ctx->stream, X, NT, nc, mt,
+                     J, tg, TT, cell, ctx->d_status, mp, lam, part);
+  hipLaunchKernelGGL(k_memb_cell_sum, dim3(nc), dim3(256), 0, ctx->stream, part, NT * J,
+                     ctx->nn * J, ctx->d_status, cterm);
+  hipLaunchKernelGGL(k_memb_adj_pos, dim3(nb(lanes, 128)), dim3(128), 0, ctx->stream, X, NT, nc, mt, J,
+                     cell, cterm, ctx->d_status, mp, lam, part);
+  hipLaunchKernelGGL(k_memb_adj_sum, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, part, NT, n, J,
+                     ctx->d_inc_start, ctx->nn, ctx->d_status, lamX);
   if (gparam)
     hipLaunchKernelGGL(k_memb_param_sum, dim3(1), dim3(64), 0, ctx->stream, cterm, nc, ctx->d_status,
                        gparam);

@@ -52,20 +52,52 @@ AP aparams(const zf_ctx *ctx) {
             ctx->ntz, (ctx->icfg.flags & ZF_IBM_CUBE_SUPPORT) ? 1 : 0};
 }
 
-// Kernel weight and its gradient with respect to the membrane node position.
-__device__ __forceinline__ double weight_grad(double dxl, double dyl, double dzl, double inv_dx,
-                                              double g[3]) {
-  const double ax = fabs(dxl), ay = fabs(dyl), az = fabs(dzl);
-  const double px = 1.0 + cos(kPiA * ax * 0.5), py = 1.0 + cos(kPiA * ay * 0.5),
-               pz = 1.0 + cos(kPiA * az * 0.5);
-  const double k = kPiA * 0.5 * inv_dx / 64.0;
-  const double sx = sin(kPiA * ax * 0.5) * (dxl > 0 ? 1.0 : dxl < 0 ? -1.0 : 0.0);
-  const double sy = sin(kPiA * ay * 0.5) * (dyl > 0 ? 1.0 : dyl < 0 ? -1.0 : 0.0);
-  const double sz = sin(kPiA * az * 0.5) * (dzl > 0 ? 1.0 : dzl < 0 ? -1.0 : 0.0);
+// Weights in the arithmetic of the forward's fields: FP64 for the FP64
+// form, FP32 otherwise (as the forward's spreading).
+template <int S>
+using W = typename Store<S>::F;
+__device__ __forceinline__ float cos_pi(float x) { return cospif(x); }
+__device__ __forceinline__ double cos_pi(double x) { return cospi(x); }
+__device__ __forceinline__ float sin_pi(float x) { return sinpif(x); }
+__device__ __forceinline__ double sin_pi(double x) { return sinpi(x); }
+
+// Kernel weight w(d) = prod_a (1 + cos(pi |d_a| / 2)) / 64 and its gradient
+// with respect to the membrane node position, in units of 1/DX.
+template <class R>
+__device__ __forceinline__ R weight_grad(R dxl, R dyl, R dzl, R g[3]) {
+  const R h = (R)0.5;
+  const R ax = fabs(dxl), ay = fabs(dyl), az = fabs(dzl);
+  const R px = (R)1 + cos_pi(h * ax), py = (R)1 + cos_pi(h * ay), pz = (R)1 + cos_pi(h * az);
+  const R k = (R)(kPiA * 0.5 / 64.0);
+  const R sx = sin_pi(h * ax) * (dxl > 0 ? (R)1 : dxl < 0 ? (R)-1 : (R)0);
+  const R sy = sin_pi(h * ay) * (dyl > 0 ? (R)1 : dyl < 0 ? (R)-1 : (R)0);
+  const R sz = sin_pi(h * az) * (dzl > 0 ? (R)1 : dzl < 0 ? (R)-1 : (R)0);
   g[0] = k * sx * py * pz;
   g[1] = k * px * sy * pz;
   g[2] = k * px * py * sz;
-  return px * py * pz / 64.0;
+  return px * py * pz * (R)(1.0 / 64.0);
+}
+
+// Unshifted velocity of every listed node (the forward's interpolation
+// input), in the field type.
+template <int S>
+__global__ void k_adj_velocity(const typename Store<S>::T *__restrict__ f, size_t stride, int n,
+                               W<S> *__restrict__ u) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double drho = 0, j0 = 0, j1 = 0, j2 = 0;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const double g = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]);
+    drho += g;
+    j0 += EX[c] * g;
+    j1 += EY[c] * g;
+    j2 += EZ[c] * g;
+  }
+  const double rho = 1.0 + drho;
+  u[s] = (W<S>)(j0 / rho);
+  u[stride + s] = (W<S>)(j1 / rho);
+  u[2 * stride + s] = (W<S>)(j2 / rho);
 }
 
 template <int S>
@@ -86,12 +118,13 @@ __device__ __forceinline__ void node_u(const typename Store<S>::T *__restrict__ 
   u[2] = j2 / rho;
 }
 
-// One lane per membrane node: lam_G and the position adjoint.
+// One lane per membrane node: lam_G and the position adjoint (velocities
+// of the support nodes from uq).
 template <int S>
 __global__ void k_ibm_adj_node(const double *__restrict__ X, size_t NT, int nn, int ncell,
                                const int32_t *__restrict__ status, const int32_t *__restrict__ b,
-                               const int32_t *__restrict__ sbox,
-                               const typename Store<S>::T *__restrict__ f, size_t stride, AP p,
+                               const int32_t *__restrict__ sbox, const W<S> *__restrict__ uq,
+                               size_t stride, AP p,
                                const double *__restrict__ G, const double *__restrict__ lamV,
                                const double *__restrict__ lamF, double *__restrict__ lamG,
                                double *__restrict__ lamX) {
@@ -118,8 +151,9 @@ __global__ void k_ibm_adj_node(const double *__restrict__ X, size_t NT, int nn, 
         const int bv = b[q];
         if (bv < 0) continue;
         const int s = sbox[q];
-        double gw[3];
-        const double w = weight_grad(dxl, dyl, dzl, inv, gw);
+        W<S> gr[3];
+        const double w = (double)weight_grad((W<S>)dxl, (W<S>)dyl, (W<S>)dzl, gr);
+        const double gw[3] = {(double)gr[0] * inv, (double)gr[1] * inv, (double)gr[2] * inv};
         if (lamF) {  // spreading onto every B >= 0 node
           const double l0 = lamF[s], l1 = lamF[stride + s], l2 = lamF[2 * stride + s];
           lg0 += p.fs * w * l0;
@@ -131,9 +165,8 @@ __global__ void k_ibm_adj_node(const double *__restrict__ X, size_t NT, int nn, 
           lx2 += lg * gw[2];
         }
         if (lamV && bv > 0 && bv < 4998) {  // interpolation from fluid nodes
-          double u[3], rho;
-          node_u<S>(f, stride, s, u, rho);
-          const double lv = p.vs * (V0 * u[0] + V1 * u[1] + V2 * u[2]);
+          const double lv = p.vs * (V0 * (double)uq[s] + V1 * (double)uq[stride + s] +
+                                    V2 * (double)uq[2 * stride + s]);
           lx0 += lv * gw[0];
           lx1 += lv * gw[1];
           lx2 += lv * gw[2];
@@ -225,8 +258,11 @@ __global__ __launch_bounds__(64) void k_interp_adj_gather(
           for (int m = 0; m < cnt; m++) {
             const double dxl = (qx - sx[m]) * inv, dyl = (qy - sy[m]) * inv, dzl = (qz - sz[m]) * inv;
             if (off_support(dxl, dyl, dzl, p.cube)) continue;
-            const double w = (1.0 + cos(kPiA * fabs(dxl) * 0.5)) * (1.0 + cos(kPiA * fabs(dyl) * 0.5)) *
-                             (1.0 + cos(kPiA * fabs(dzl) * 0.5)) / 64.0;
+            using R = W<S>;
+            const R h = (R)0.5;
+            const double w = (double)(((R)1 + cos_pi(h * fabs((R)dxl))) *
+                                      ((R)1 + cos_pi(h * fabs((R)dyl))) *
+                                      ((R)1 + cos_pi(h * fabs((R)dzl))) * (R)(1.0 / 64.0));
             a0 += w * s0[m];
             a1 += w * s1[m];
             a2 += w * s2[m];
@@ -298,12 +334,18 @@ int ibm_vjp_device(zf_ctx *ctx, const double *X, const void *f, const double *G,
   const int n = ctx->ncell * ctx->nn;
   if (n == 0) return ZF_OK;
   const AP p = aparams(ctx);
+  // the interpolation's velocity field of f, in the forward's velocity
+  // buffer (which the next forward interpolation overwrites)
   with_form(ctx->cfg.storage, [&](auto tag) {
     constexpr int S = decltype(tag)::value;
     using T = typename Store<S>::T;
+    W<S> *uq = static_cast<W<S> *>(ctx->d_umac);
+    if (lamV)
+      hipLaunchKernelGGL(k_adj_velocity<S>, dim3(nb(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                         static_cast<const T *>(f), ctx->stride, ctx->dommax, uq);
     hipLaunchKernelGGL(k_ibm_adj_node<S>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X,
-                       NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox,
-                       static_cast<const T *>(f), ctx->stride, p, G, lamV, lamF, lamG, lamX);
+                       NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, uq,
+                       ctx->stride, p, G, lamV, lamF, lamG, lamX);
     return 0;
   });
   if (lam && lamV) {

@@ -95,6 +95,7 @@ struct zf_ctx {
   bool have_mesh = false;
   zf_membrane_config mcfg{};
   int32_t nn = 0, nt = 0, ne = 0, ncell = 0;
+  int32_t inc_max = 0;            // largest (edge, role) incidence count of a node
   std::vector<int32_t> h_edge, h_nbr_node, h_nbr_tri;  // reference row formats
   std::vector<double> h_eq;       // 6 x ne: lmax plength kp theta0 sin0 cos0
   int32_t *d_tri = nullptr;       // 3 x nt, 0-based

@@ -361,6 +361,8 @@ int zf_membrane_setup(zf_ctx *ctx, const zf_membrane_config *mc, int32_t nn, int
     inc.insert(inc.end(), per[n].begin(), per[n].end());
   }
   istart[nn] = (int32_t)inc.size();
+  ctx->inc_max = 0;
+  for (int n = 0; n < nn; n++) ctx->inc_max = std::max(ctx->inc_max, istart[n + 1] - istart[n]);
   int rc;
   if ((rc = upload(ctx, &ctx->d_tri, dtri)) || (rc = upload(ctx, &ctx->d_edge, dedge)) ||
       (rc = upload(ctx, &ctx->d_edge_eq, eqd)) || (rc = upload(ctx, &ctx->d_tri_area_eq, tae)) ||
