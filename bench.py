@@ -224,6 +224,70 @@ def cells_leg(name, b, ncell, steps, warmup):
                     rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))))
 
 
+def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
+    """The large mixed-precision run: the 4-generation tree (1024x256x256)
+    with as many Evans-Fung cells (642 nodes, radius 4) as the dense placer
+    fits up to `ncell`, membrane/IBM/repulsion every `interval` iterations
+    with `interval`-step position updates, `steps` iterations in the FP16,
+    FP32 and FP64 forms from the same start.  Velocity, wall shear stress
+    (every lumen wall-mesh node, mu/DX = 1) and cell shape (node positions
+    relative to the mean node displacement, cells live in every form) of the
+    FP16 and FP32 forms against the FP64 form, and FP16 against FP32."""
+    from tests import geometry as g
+    from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
+                                      lattice_repulsion_config)
+    name = "tree_cfg5"
+    b = make_geometry(name)
+    xyz, tris = g.rbc_mesh(radius=4.0)
+    cells = g.place_cells_dense(b, xyz, 0.6, ncell, 20261208)
+    wx, wt, wn = g.wall_mesh(b)
+    nrm = node_normals(wx, wt, wn)
+    ok = np.isfinite(nrm).all(axis=1)  # nodes of degenerate wall triangles only: no normal
+    px, pn = wx[ok], nrm[ok]
+    print(f"mixed precision: {len(cells)} cells placed, {len(px)} wall points", flush=True)
+    res = {}
+    for store in ("f16dev", "f32dev", "f64"):
+        eng = engine_for(name, b, store)
+        eng.membrane_setup(lattice_membrane_config(xyz, tris, len(cells)), tris, xyz)
+        eng.upload_cells(cells)
+        ic = lattice_ibm_config(rbc_interval=interval)
+        ic.dt_advance = float(interval)
+        eng.ibm_setup(ic)
+        eng.repulsion_setup(lattice_repulsion_config())
+        eng.wall_setup(wx, wt, wn)
+        eng.sync()
+        eng.record(0)
+        for chunk in range(0, steps, 500):
+            eng.full_run(chunk, min(500, steps - chunk))
+            eng.sync()
+            print(f"mixed precision: {store} {chunk + min(500, steps - chunk)} steps", flush=True)
+        eng.record(1)
+        eng.sync()
+        x, _, st = eng.cells()
+        d, u, v, w = eng.macro()
+        res[store] = dict(ms=eng.elapsed_ms(0, 1), u=np.stack([u, v, w]), wss=eng.wss(px, pn, 1.0)[0],
+                          x=x, st=st, n=eng.n_interior)
+        eng.close()
+    live = (res["f16dev"]["st"] > 0) & (res["f32dev"]["st"] > 0) & (res["f64"]["st"] > 0)
+
+    def err(a, ref):
+        ea, er = res[a], res[ref]
+        dx = np.linalg.norm(er["x"][live] - cells[live])
+        return dict(vel_rel_l2=float(np.linalg.norm(ea["u"] - er["u"]) / np.linalg.norm(er["u"])),
+                    wss_rel_l2=float(np.linalg.norm(ea["wss"] - er["wss"]) / np.linalg.norm(er["wss"])),
+                    shape_rel=float(np.linalg.norm(ea["x"][live] - er["x"][live]) / dx))
+
+    nn = len(xyz)
+    return dict(workload=f"{name}: 1024x256x256 tree, {res['f64']['n']} fluid nodes, "
+                         f"{len(cells)} cells of {nn} nodes",
+                cells=len(cells), membrane_nodes=len(cells) * nn, steps=steps, rbc_interval=interval,
+                cells_live_all_forms=int(live.sum()), wall_points=len(px),
+                ms={k: round(r["ms"], 1) for k, r in res.items()},
+                fluid_mlups={k: round(r["n"] * steps / (r["ms"] * 1e-3) / 1e6, 1) for k, r in res.items()},
+                f16_vs_f64=err("f16dev", "f64"), f32_vs_f64=err("f32dev", "f64"),
+                f16_vs_f32=err("f16dev", "f32dev"))
+
+
 def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h=(1e-4, 1e-6),
                 fd_h_smooth=(1e-3, 1e-4)):
     """Adjoint cost and gradients on a 128x64x64 straight vessel of radius 26
@@ -316,7 +380,7 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
     res = {}
     for store, k, smooth, mesh in (("f32dev", checkpoint, False, "642"), ("f32dev", 1, False, "642"),
                                    ("f64", checkpoint, False, "642"), ("f64", checkpoint, True, "642"),
-                                   ("f64", checkpoint, True, "162")):
+                                   ("f64", checkpoint, True, "162"), ("f32dev", checkpoint, True, "162")):
         eng = build(store, k, smooth=smooth, mesh=mesh)
         eng.full_run(0, 2 * interval)  # warm-up of every kernel
         eng.init_equilibrium()
@@ -361,6 +425,8 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
                 smooth_variant={f"cells_{m}": dict(grad_fp64=[float(x) for x in rs["g"]], fd_fp64=fdd(rs),
                                                    grad_rel_err_vs_fd=fde(rs), J=rs["J"])
                                 for m, rs in smooth.items()},
+                smooth_variant_fp32_vs_fp64_cells_162=rel(res[("f32dev", checkpoint, True, "162")]["g"],
+                                                          smooth["162"]["g"]),
                 J=[r32["J"], r64["J"]])
 
 
@@ -397,6 +463,9 @@ def main():
     ap.add_argument("--cells", type=int, default=1200,
                     help="cells of the dense-cell leg (0 skips it)")
     ap.add_argument("--cell-steps", type=int, default=200)
+    ap.add_argument("--mixed-precision-only", action="store_true",
+                    help="run only the large mixed-precision leg (tree, FP16/FP32/FP64 with "
+                         "cells) and print its JSON object")
     ap.add_argument("--probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: three 256 MiB copies, "
                          "then 20 steps of each storage form; prints no JSON line")
@@ -413,6 +482,9 @@ def main():
     b = make_geometry(args.workload)
     if args.probe:
         probe(args.workload, b)
+        return
+    if args.mixed_precision_only:
+        print(json.dumps(dict(mixed_precision=mixed_precision_leg())), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)

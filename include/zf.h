@@ -112,6 +112,8 @@ typedef struct zf_lattice_info {
   int32_t n_macro_nodes, stride;
   int64_t device_bytes;
 } zf_lattice_info;
+/* The configuration the context was created with (storage form etc.). */
+int zf_get_config(const zf_ctx *ctx, zf_config *cfg);
 int zf_get_lattice_info(zf_ctx *ctx, zf_lattice_info *info);
 
 /* ---- LBM state -------------------------------------------------------------
@@ -374,6 +376,21 @@ int zf_ibm_vjp(zf_ctx *ctx, const double *lam_force, const double *lam_vel, cons
  * Exact local derivatives (dual-number evaluation of the force terms) plus
  * the cell centroid, area and volume terms; FP64. */
 int zf_membrane_vjp(zf_ctx *ctx, const double *lam_force, double *lam_pos, double dparam[2]);
+
+/* ---- state checkpoints ------------------------------------------------------
+ * The reference restarts from its macroscopic CSV output and re-equilibrates
+ * the populations (ZebraflowWT.c:5253-5262); these keep the exact state.
+ * zf_checkpoint_save copies the current populations (as stored), the force
+ * field, and with a membrane mesh the cells (positions, velocities, membrane
+ * force, repulsion velocity, status) into device slot `slot` (0..1023,
+ * allocated on first use); zf_checkpoint_restore copies them back, after
+ * which the context continues bit-identically.  A slot restores only into
+ * the configuration it was saved from. */
+int zf_checkpoint_save(zf_ctx *ctx, int32_t slot);
+int zf_checkpoint_restore(zf_ctx *ctx, int32_t slot);
+/* Cell status (ncell; >= 0 live, < 0 removed), for restarting from files
+ * (zf_upload_cells marks every uploaded cell live). */
+int zf_upload_cell_status(zf_ctx *ctx, const int32_t *status);
 
 /* ---- timing and synchronisation ------------------------------------------ */
 /* Device-copy ceiling: a plain 16-byte-per-lane copy of `bytes` between two

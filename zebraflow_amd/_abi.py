@@ -109,6 +109,10 @@ FUNCS = {
     "zf_membrane_vjp": (C.c_int, [P, P, P, P]),
     "zf_adjoint_cell_gradients": (C.c_int, [P, P, P]),
     "zf_run": (C.c_int, [P, i32, i32]),
+    "zf_checkpoint_save": (C.c_int, [P, i32]),
+    "zf_checkpoint_restore": (C.c_int, [P, i32]),
+    "zf_upload_cell_status": (C.c_int, [P, P]),
+    "zf_get_config": (C.c_int, [P, C.POINTER(ZfConfig)]),
     "zf_copy_bandwidth": (C.c_int, [P, i64, i32, C.POINTER(f64)]),
     "zf_sync": (C.c_int, [P]),
     "zf_event_record": (C.c_int, [P, i32]),

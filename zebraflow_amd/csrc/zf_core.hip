@@ -101,6 +101,7 @@ int zf_destroy(zf_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_adjoint(ctx);
+  free_state_slots(ctx);
   free_cells(ctx);
   free_ibm(ctx);
   free_membrane(ctx);
@@ -114,6 +115,12 @@ int zf_destroy(zf_ctx *ctx) {
 }
 
 const char *zf_last_error(const zf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int zf_get_config(const zf_ctx *ctx, zf_config *cfg) {
+  if (!ctx || !cfg) return ZF_ERR_ARG;
+  *cfg = ctx->cfg;
+  return ZF_OK;
+}
 
 int zf_sync(zf_ctx *ctx) {
   if (!ctx) return ZF_ERR_ARG;

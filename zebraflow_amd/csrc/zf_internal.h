@@ -47,6 +47,14 @@ struct BcEntry {
                   // come from this step's macro output, else from moments
 };
 
+// One saved state (zf_checkpoint_save): regions packed at 256-byte offsets.
+struct StateSlot {
+  void *buf = nullptr;
+  size_t bytes = 0;
+  bool force = false;
+  int32_t ncell = -1;  // -1: no cells in the slot
+};
+
 template <class T>
 struct DevBuf {
   T *p = nullptr;
@@ -187,6 +195,8 @@ struct zf_ctx {
   double *d_mscratch = nullptr;    // membrane transpose scratch
   double *d_cgrad = nullptr;       // dJ/d ln es, dJ/d ln kbend
   std::vector<int32_t> h_status0;  // cell status at the start of the forward pass
+
+  std::vector<zf::StateSlot> slots;  // zf_checkpoint_save
 };
 
 namespace zf {
@@ -210,6 +220,7 @@ void free_membrane(zf_ctx *ctx);
 void free_ibm(zf_ctx *ctx);
 void free_cells(zf_ctx *ctx);
 void free_adjoint(zf_ctx *ctx);
+void free_state_slots(zf_ctx *ctx);
 int membrane_forces_impl(zf_ctx *ctx);
 int cell_list_impl(zf_ctx *ctx);
 int repulsion_impl(zf_ctx *ctx);

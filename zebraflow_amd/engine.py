@@ -241,6 +241,16 @@ class Engine:
     def full_step(self, it):
         self._check(self.lib.zf_step(self.h, it), "zf_step")
 
+    def checkpoint_save(self, slot=0):
+        self._check(self.lib.zf_checkpoint_save(self.h, slot), "zf_checkpoint_save")
+
+    def checkpoint_restore(self, slot=0):
+        self._check(self.lib.zf_checkpoint_restore(self.h, slot), "zf_checkpoint_restore")
+
+    def upload_cell_status(self, status):
+        st = np.ascontiguousarray(status, np.int32)
+        self._check(self.lib.zf_upload_cell_status(self.h, _ptr(st)), "zf_upload_cell_status")
+
     def full_run(self, iter0, nsteps):
         self._check(self.lib.zf_run(self.h, iter0, nsteps), "zf_run")
 

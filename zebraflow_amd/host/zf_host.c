@@ -24,6 +24,8 @@
  *   rbc_interval = 50              gamma_p = G          repulsion = 0 | 1
  *   wall_nodes / wall_cells / wall_normals = WallMesh*.dat, or wall = surface
  *   steps = N   output_every = M   output_dir = SimulationOutputData
+ *   checkpoint_every = C           exact restart file ZFstate-%09d.bin every C
+ *   restart = ZFstate-000000100.bin   continue from such a file (steps more)
  * Outputs: FLUIDdomain-%09d.csv, AllCellMemb-%09d.csv, VolumeAreaHistory.dat
  * (:282-446).
  */
@@ -251,6 +253,14 @@ int main(int argc, char **argv) {
   run.iter0 = 0;
   run.steps = geti("steps", 1000);
   run.output_every = geti("output_every", 0);
+  run.checkpoint_every = geti("checkpoint_every", 0);
+  if (get("restart")) {
+    int32_t it = 0;
+    if ((rc = zfh_load_state(ctx, get("restart"), &it, run.ncell, run.nn)))
+      return die(ctx, rc, "restart file");
+    run.iter0 = it;
+    printf("zf_host: restarted from %s at iteration %d\n", get("restart"), it);
+  }
   if ((rc = zfh_run(ctx, b, &run))) return die(ctx, rc, "the time loop");
   printf("zf_host: %d iterations done\n", run.steps);
   zf_destroy(ctx);

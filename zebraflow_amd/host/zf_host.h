@@ -124,6 +124,17 @@ void zfh_membrane_config_lattice(const double *xyz, int nn, const int32_t *tri, 
                                  double kb, int max_cells, zf_membrane_config *mc);
 
 /* ---- the step/output loop -------------------------------------------------- */
+/* ---- exact restart files ------------------------------------------------------
+ * The state as the engine stores it (populations, the force field of the
+ * last RBC iteration, cell positions and status), so that a run continues
+ * bit-identically; the reference's own restart re-equilibrates the
+ * populations from its CSV output (ZebraflowWT.c:5253-5262).  ncell, nn:
+ * the cells on the context (0, 0 for none).  zfh_load_state checks that the
+ * file matches the context (storage form, fluid nodes, cells) and returns
+ * the iteration it was written at in *iter. */
+int zfh_save_state(zf_ctx *ctx, const char *path, int32_t iter, int32_t ncell, int32_t nn);
+int zfh_load_state(zf_ctx *ctx, const char *path, int32_t *iter, int32_t ncell, int32_t nn);
+
 typedef struct zfh_run_config {
   const char *out_dir;    /* SimulationOutputData */
   int32_t im, jm, km;     /* the box of b */
@@ -132,6 +143,7 @@ typedef struct zfh_run_config {
   double x0, y0, z0, dx;  /* node (i,j,k) at (x0 + (i-1) dx, ...) in the CSVs */
   int32_t ncell, nn, nt;  /* cells on the context (0: none) */
   int32_t have_rep, have_wall;
+  int32_t checkpoint_every; /* restart file ZFstate-%09d.bin when iter % checkpoint_every == 0 */
 } zfh_run_config;
 
 /* The reference's time loop (ZebraflowWT.c:5646-12412): zf_step per

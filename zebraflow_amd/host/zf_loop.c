@@ -5,8 +5,9 @@
  * iteration; membrane, IBM and repulsion every SOLVERSTAGGER-th, in the
  * order of zf_step) and call OUTPUT() every outputfreq iterations
  * (:12408), which writes the fluid domain, the cells and the volume/area
- * history (:282-446).  Parent domains, nuclei and restart files are not
- * part of this engine.
+ * history (:282-446); optionally an exact restart file every
+ * checkpoint_every iterations.  Parent domains and nuclei are not part of
+ * this engine.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -93,6 +94,11 @@ int zfh_run(zf_ctx *ctx, const int32_t *b, const zfh_run_config *rc) {
     if (rc->output_every > 0 && iter % rc->output_every == 0) {
       if ((err = output_fluid(ctx, b, rc, iter))) return err;
       if (rc->ncell > 0 && (err = output_cells(ctx, rc, iter))) return err;
+    }
+    if (rc->checkpoint_every > 0 && iter % rc->checkpoint_every == 0) {
+      char path[1024];
+      snprintf(path, sizeof path, "%s/ZFstate-%09d.bin", rc->out_dir, iter);
+      if ((err = zfh_save_state(ctx, path, iter, rc->ncell, rc->nn))) return err;
     }
   }
   return zf_sync(ctx);
