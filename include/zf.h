@@ -52,10 +52,17 @@ enum {
 };
 
 /* Kernel options (zf_config.flags); results do not depend on them except
- * ZF_FLAG_F64_ARITH, which computes the FP32/FP16 storage forms in FP64. */
-#define ZF_FLAG_LINEAR_BLOCKS 1u    /* plain block order instead of one range per XCD */
-#define ZF_FLAG_NT_STORES 2u        /* non-temporal population stores */
+ * ZF_FLAG_F64_ARITH, which computes the FP32/FP16 storage forms in FP64.
+ * By default the stream kernel sets its cache policy from the population
+ * footprint: every direction cached when both population buffers fit the
+ * 256 MB last-level cache, otherwise as many directions as take about
+ * 120 MB of one buffer, the rest loaded and stored non-temporally. */
+#define ZF_FLAG_LINEAR_BLOCKS 1u    /* plain block order instead of the XCD-aware one */
+#define ZF_FLAG_NT_POPULATIONS 2u   /* every population load and store non-temporal */
 #define ZF_FLAG_F64_ARITH 4u        /* FP64 arithmetic on FP32/FP16 storage */
+/* Keep directions 0..k-1 cached (k = 19: all; an odd k < 19 counts as k-1),
+ * the others non-temporal, instead of the footprint rule. */
+#define ZF_FLAG_CACHED_DIRS(k) ((uint32_t)((k) + 1) << 8)
 
 /* Boundary codes in b_index beyond the reference's (0 wall, 1..9 fluid with
  * tau = 0.5 + visc_scale*B, 1001..1009 pressure waveforms). */

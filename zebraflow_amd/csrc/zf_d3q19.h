@@ -7,6 +7,9 @@
 
 #include "zf_internal.h"
 
+#include <type_traits>
+#include <utility>
+
 namespace zf {
 namespace d3q19 {
 
@@ -77,12 +80,44 @@ __device__ __forceinline__ R geq(int c, R drho, R ux, R uy, R uz, R usq) {
 }
 
 // Workgroup order: the hardware deals workgroups round-robin over the 8
-// XCDs; with `remap`, XCD x runs one contiguous range of node blocks, so the
-// chord boundaries a block shares with its neighbours stay in one L2.
-__device__ __forceinline__ int logical_block(int b, int nblk, bool remap) {
-  if (!remap) return b;
-  const int q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
+// XCDs.  order 0: plain.  order 1: XCD x runs one contiguous range of node
+// blocks, so the chord boundaries a block shares with its neighbours stay in
+// one L2.  order k > 1: the XCDs sweep one common front, XCD x taking chunk
+// x of every 8 consecutive chunks of k blocks (blocks past the last whole
+// round of 8k keep their place).  Every order is a permutation of [0, nblk).
+__device__ __forceinline__ int logical_block(int b, int nblk, int order) {
+  if (order == 0) return b;
+  const int x = b % 8, i = b / 8;
+  if (order > 1) {
+    const int round = 8 * order, full = nblk / round * round;
+    if (b >= full) return b;
+    return (i / order) * round + x * order + i % order;
+  }
+  const int q = nblk / 8, r = nblk % 8;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Population load and store with a compile-time cache policy: non-temporal
+// accesses do not allocate in the last-level cache.
+template <bool NT, class T>
+__device__ __forceinline__ T ld_policy(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_policy(T *p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// f(integral_constant<int, c>) for c = 0..Q-1, each with its own constant c.
+template <class Fn, int... C>
+__device__ __forceinline__ void for_dirs_seq(Fn &&f, std::integer_sequence<int, C...>) {
+  (f(std::integral_constant<int, C>()), ...);
+}
+template <class Fn>
+__device__ __forceinline__ void for_dirs(Fn &&f) {
+  for_dirs_seq(f, std::make_integer_sequence<int, Q>());
 }
 
 struct StepParams {

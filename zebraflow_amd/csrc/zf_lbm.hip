@@ -68,16 +68,21 @@ __device__ __forceinline__ void write_bc(typename Store<S>::T *__restrict__ dst,
 //   - workgroups past the interior range handle the remaining boundary nodes
 //     from their neighbour's moments before this step (the reference's
 //     previous macro pass, :6606-6614).
-template <int S, bool FIELD, bool NT, bool ACC>
+// Directions c < KT are loaded and stored through the last-level cache, the
+// others non-temporally (KT is 19 or even, so c and c^1 share a policy).
+template <int S, bool FIELD, int KT, bool ACC>
 __global__ __launch_bounds__(256) void k_stream_collide(
     const typename Store<S>::T *__restrict__ src, typename Store<S>::T *__restrict__ dst,
-    size_t stride, int n_int, int nblk_int, int remap, const uint32_t *__restrict__ word,
+    size_t stride, int n_int, int nblk_int, int order, const uint32_t *__restrict__ word,
     const int32_t *__restrict__ block0, const int32_t *__restrict__ chords,
     const typename Store<S>::F *__restrict__ force, StepParams prm, const BcEntry *__restrict__ bc,
     const int32_t *__restrict__ fused_s, const int32_t *__restrict__ att_start, int n_fused,
     int n_bc_fused, int n_bc, BcParams bp) {
+  static_assert(KT == Q || (KT >= 0 && KT % 2 == 0), "cached directions: 19 or even");
   using R = typename Store<S>::R;
   using Sx = Store<S>;
+  using T = typename Sx::T;
+  constexpr bool NT_ALL = KT == 0;
   if ((int)blockIdx.x >= nblk_int) {
     // late boundary entries
     const int t = n_bc_fused + ((int)blockIdx.x - nblk_int) * 256 + threadIdx.x;
@@ -93,10 +98,10 @@ __global__ __launch_bounds__(256) void k_stream_collide(
       jz += EZ[c] * gv;
     }
     const double rm = 1.0 + drho;
-    write_bc<S>(dst, stride, e, (R)drho, (R)(jx / rm), (R)(jy / rm), (R)(jz / rm), bp, NT);
+    write_bc<S>(dst, stride, e, (R)drho, (R)(jx / rm), (R)(jy / rm), (R)(jz / rm), bp, NT_ALL);
     return;
   }
-  const int s = logical_block(blockIdx.x, nblk_int, remap != 0) * 256 + threadIdx.x;
+  const int s = logical_block(blockIdx.x, nblk_int, order) * 256 + threadIdx.x;
   if (s >= n_int) return;
   const uint32_t w = word[s];
   const int ch = block0[s >> 6] + (int)(w >> kWordChordShift);
@@ -111,13 +116,14 @@ __global__ __launch_bounds__(256) void k_stream_collide(
   // own type, so only the arithmetic rounding changes)
   using CT = typename std::conditional<ACC, double, R>::type;
   CT g[Q];
-#pragma unroll
-  for (int c = 0; c < 18; c++) {
-    const bool bb = (w >> c) & 1u;
-    const size_t idx = bb ? (size_t)(c ^ 1) * stride + s : (size_t)c * stride + (s + d[c]);
-    g[c] = (CT)Sx::ld(src, idx, WQ[c]);
-  }
-  g[18] = (CT)Sx::ld(src, (size_t)18 * stride + s, WQ[18]);
+  for_dirs([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    size_t idx = (size_t)c * stride + s;
+    if constexpr (c < 18)
+      idx = ((w >> c) & 1u) ? (size_t)(c ^ 1) * stride + s : idx + d[c];
+    const T raw = ld_policy<(c >= KT)>(src + idx);
+    g[c] = (CT)Sx::ld(&raw, 0, WQ[c]);
+  });
 
   CT fx = (CT)prm.g[0], fy = (CT)prm.g[1], fz = (CT)prm.g[2];
   if (FIELD) {
@@ -147,17 +153,16 @@ __global__ __launch_bounds__(256) void k_stream_collide(
   const CT omega = (CT)1 / ((CT)0.5 + (CT)prm.visc_scale * (CT)d[18]);
   const CT fpref = dt * ((CT)1 - (CT)0.5 * omega);  // DT*(1 - 1/(2 tau))
   const CT uf = ux * fx + uy * fy + uz * fz;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
+  for_dirs([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
     const CT cu = (CT)EX[c] * ux + (CT)EY[c] * uy + (CT)EZ[c] * uz;
     const CT cf = (CT)EX[c] * fx + (CT)EY[c] * fy + (CT)EZ[c] * fz;
     const CT ge = geq<CT>(c, drho, ux, uy, uz, usq);
     // Guo source, ZebraflowWT.c:6140: ((c-u).F)/cs^2 + (c.u)(c.F)/cs^4
     const CT src_c = (CT)WQ[c] * ((CT)3 * (cf - uf) + (CT)9 * cu * cf);
-    const auto v = Sx::st((R)(g[c] - omega * (g[c] - ge) + fpref * src_c), WQ[c]);
-    if (NT) __builtin_nontemporal_store(v, &dst[(size_t)c * stride + s]);
-    else dst[(size_t)c * stride + s] = v;
-  }
+    const T v = Sx::st((R)(g[c] - omega * (g[c] - ge) + fpref * src_c), WQ[c]);
+    st_policy<(c >= KT)>(dst + (size_t)c * stride + s, v);
+  });
   if (w & kWordMacro) {
     // binary search of this node among the neighbours of fused entries
     int lo = 0, hi = n_fused - 1, k = -1;
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     }
     if (k >= 0)
       for (int t = att_start[k]; t < att_start[k + 1]; t++)
-        write_bc<S>(dst, stride, bc[t], (R)drho, (R)ux, (R)uy, (R)uz, bp, NT);
+        write_bc<S>(dst, stride, bc[t], (R)drho, (R)ux, (R)uy, (R)uz, bp, NT_ALL);
   }
 }
 
@@ -312,35 +317,80 @@ inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / 
 
 double bc_pressure(int code, double simtime);
 
-template <int S, bool FIELD, bool NT, bool ACC>
+// Cache policy of the population arrays.  MI355X keeps a 256 MB last-level
+// cache (Infinity Cache) in front of HBM.  When both population buffers fit
+// in it, every direction goes through it (a step then reads what the previous
+// one wrote from there) and each XCD sweeps one contiguous range of node
+// blocks.  When they do not fit, directions c < kt stay cached, kt even and
+// as large as keeps one buffer's share of them within about 120 MB, and the
+// others are loaded and stored non-temporally so that they do not evict the
+// cached ones; the XCDs then sweep one common front in chunks of 128 blocks.
+// Measured (fluid MLUPS, one MI355X): 2.75 M-node tree FP32 27.7k -> 33.4k
+// (kt 8), FP64 15.7k -> 17.6k (kt 4); 1.28 M-node vessel pair FP64 14.7k ->
+// 18.4k (kt 10); all-cached stays best for every footprint that fits.
+constexpr size_t kLlcBytes = 256ull << 20;
+constexpr size_t kCachedShareBytes = 120ull << 20;
+constexpr int kFrontChunk = 128;
+
+struct CachePlan {
+  int kt;     // directions 0..kt-1 cached
+  int order;  // logical_block order
+};
+
+CachePlan cache_plan(const zf_ctx *ctx) {
+  const size_t per_dir = ctx->stride * store_bytes(ctx->cfg.storage);  // one direction, one buffer
+  const uint32_t fl = (uint32_t)ctx->cfg.flags;
+  int kt;
+  if (fl & ZF_FLAG_NT_POPULATIONS) {
+    kt = 0;
+  } else if ((fl >> 8) & 31u) {
+    kt = std::min((int)((fl >> 8) & 31u) - 1, Q);
+    if (kt < Q) kt &= ~1;
+  } else if (2 * (size_t)Q * per_dir <= kLlcBytes) {
+    kt = Q;
+  } else {
+    kt = (int)std::min<size_t>(kCachedShareBytes / std::max<size_t>(per_dir, 1), Q - 1) & ~1;
+  }
+  const int order = (fl & ZF_FLAG_LINEAR_BLOCKS) ? 0 : (kt == Q ? 1 : kFrontChunk);
+  return {kt, order};
+}
+
+template <int S, bool FIELD, int KT, bool ACC>
 void launch_stream(zf_ctx *ctx, const void *src, void *dst, const StepParams &prm,
-                   const BcParams &bp) {
+                   const BcParams &bp, int order) {
   using T = typename Store<S>::T;
   const int nblk_int = (int)nblocks(ctx->n_int, 256);
   const int nblk_late = (int)nblocks(ctx->n_bc - ctx->n_bc_fused, 256);
   if (nblk_int + nblk_late == 0) return;
-  const int remap = (ctx->cfg.flags & ZF_FLAG_LINEAR_BLOCKS) ? 0 : 1;
-  hipLaunchKernelGGL((k_stream_collide<S, FIELD, NT, ACC>), dim3(nblk_int + nblk_late), dim3(256), 0,
+  hipLaunchKernelGGL((k_stream_collide<S, FIELD, KT, ACC>), dim3(nblk_int + nblk_late), dim3(256), 0,
                      ctx->stream, static_cast<const T *>(src), static_cast<T *>(dst), ctx->stride,
-                     ctx->n_int, nblk_int, remap, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
+                     ctx->n_int, nblk_int, order, ctx->d_word, ctx->d_block_chord0, ctx->d_chords,
                      FIELD ? static_cast<const typename Store<S>::F *>(ctx->d_force) : nullptr, prm,
                      ctx->d_bc, ctx->d_fused_s, ctx->d_att_start,
                      ctx->n_fused_nodes, ctx->n_bc_fused, ctx->n_bc, bp);
 }
 
+template <int S, bool FIELD, bool ACC>
+void launch_stream_kt(zf_ctx *ctx, const void *src, void *dst, const StepParams &prm,
+                      const BcParams &bp, const CachePlan &cp) {
+#define ZF_KT_CASE(K) \
+  case K: launch_stream<S, FIELD, K, ACC>(ctx, src, dst, prm, bp, cp.order); break;
+  switch (cp.kt) {
+    ZF_KT_CASE(0) ZF_KT_CASE(2) ZF_KT_CASE(4) ZF_KT_CASE(6) ZF_KT_CASE(8) ZF_KT_CASE(10)
+    ZF_KT_CASE(12) ZF_KT_CASE(14) ZF_KT_CASE(16) ZF_KT_CASE(18)
+    default: launch_stream<S, FIELD, Q, ACC>(ctx, src, dst, prm, bp, cp.order); break;
+  }
+#undef ZF_KT_CASE
+}
+
 template <int S, bool FIELD>
 void launch_stream_opts(zf_ctx *ctx, const void *src, void *dst, const StepParams &prm,
                         const BcParams &bp) {
-  const bool nt = ctx->cfg.flags & ZF_FLAG_NT_STORES;
+  const CachePlan cp = cache_plan(ctx);
   // FP64 storage already computes in FP64
   const bool acc = S != ZF_STORE_F64 && (ctx->cfg.flags & ZF_FLAG_F64_ARITH);
-  if (nt) {
-    if (acc) launch_stream<S, FIELD, true, true>(ctx, src, dst, prm, bp);
-    else launch_stream<S, FIELD, true, false>(ctx, src, dst, prm, bp);
-  } else {
-    if (acc) launch_stream<S, FIELD, false, true>(ctx, src, dst, prm, bp);
-    else launch_stream<S, FIELD, false, false>(ctx, src, dst, prm, bp);
-  }
+  if (acc) launch_stream_kt<S, FIELD, true>(ctx, src, dst, prm, bp, cp);
+  else launch_stream_kt<S, FIELD, false>(ctx, src, dst, prm, bp, cp);
 }
 
 // One LBM iteration: a single fused launch, then the buffer swap.  ev

@@ -14,6 +14,13 @@ from ._abi import ZfConfig, ZfLatticeInfo
 STORE_F64, STORE_F32DEV, STORE_F16DEV = 0, 1, 2
 WALL_EQUILIBRIUM, WALL_HALFWAY_BB = 0, 1
 B_VELOCITY_INLET, B_PRESSURE_OUTLET = 2001, 3001
+# zf_config.flags (include/zf.h)
+FLAG_LINEAR_BLOCKS, FLAG_NT_POPULATIONS, FLAG_F64_ARITH = 1, 2, 4
+
+
+def flag_cached_dirs(k):
+    """ZF_FLAG_CACHED_DIRS(k): directions 0..k-1 cached, the rest non-temporal."""
+    return (k + 1) << 8
 
 
 class ZfError(RuntimeError):
