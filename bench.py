@@ -41,6 +41,10 @@ WORKLOADS = {
     "tube_cfg1": dict(box=(32, 32, 128), kind="tube", r=14.0),
     "tree_cfg5": dict(box=(1024, 256, 256), kind="tree", r_root=24.0, ratio=0.79, gens=4,
                       a_lo=30.0, a_hi=45.0, seed=20261207),
+    # the same tree with the root radius scaled to 42.4 for 12 % fluid
+    # (8.12 M fluid nodes); radius 24 gives 4.1 %
+    "tree_cfg5_12": dict(box=(1024, 256, 256), kind="tree", r_root=42.4, ratio=0.79, gens=4,
+                         a_lo=30.0, a_hi=45.0, seed=20261207),
 }
 
 
@@ -229,10 +233,10 @@ def cells_leg(name, b, ncell, steps, warmup):
 
 
 def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
-    """The large mixed-precision run: the 4-generation tree (1024x256x256)
-    with as many Evans-Fung cells (642 nodes, radius 4) as the dense placer
-    fits up to `ncell`, membrane/IBM/repulsion every `interval` iterations
-    with `interval`-step position updates, `steps` iterations in the FP16,
+    """The large mixed-precision run: the 4-generation tree (1024x256x256,
+    12 % fluid) with as many Evans-Fung cells (642 nodes, radius 4) as the
+    dense placer fits up to `ncell`, membrane/IBM/repulsion every `interval`
+    iterations with `interval`-step position updates, `steps` iterations in the FP16,
     FP32 and FP64 forms from the same start.  Velocity, wall shear stress
     (every lumen wall-mesh node, mu/DX = 1) and cell shape (node positions
     relative to the mean node displacement, cells live in every form) of the
@@ -240,7 +244,7 @@ def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
     from tests import geometry as g
     from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
                                       lattice_repulsion_config)
-    name = "tree_cfg5"
+    name = "tree_cfg5_12"
     b = make_geometry(name)
     xyz, tris = g.rbc_mesh(radius=4.0)
     cells = g.place_cells_dense(b, xyz, 0.6, ncell, 20261208)
@@ -264,7 +268,10 @@ def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
         for chunk in range(0, steps, 500):
             eng.full_run(chunk, min(500, steps - chunk))
             eng.sync()
-            print(f"mixed precision: {store} {chunk + min(500, steps - chunk)} steps", flush=True)
+            _, u, v, w = eng.macro()
+            print(f"mixed precision: {store} {chunk + min(500, steps - chunk)} steps, "
+                  f"{int((eng.cells()[2] > 0).sum())} cells live, max |u| "
+                  f"{float(np.sqrt(u * u + v * v + w * w).max()):.4f}", flush=True)
         eng.record(1)
         eng.sync()
         x, _, st = eng.cells()
@@ -307,10 +314,9 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
     and central differences of the FP64 forward pass for each relative step
     in fd_h, for the model as the reference has it and for its smooth
     variant (IBM kernel on the whole 4^3 cube instead of the sphere, bending
-    never switched off, no anti-sliver threshold), whose forward map has no
-    jumps, with the 642-node cells and with 162-node cells (radius 3): the
-    finer mesh has hinges within 1e-6 of flat, where the bending force
-    without its cut-off behaves as 1/sin(dihedral)."""
+    faded to zero by a smoothstep over |cos dihedral| 0.97..0.99 instead of
+    the reference's cut at 0.99, no anti-sliver threshold), whose forward map
+    is C1, with the 642-node cells and with 162-node cells (radius 3)."""
     from tests import geometry as g
     from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, Engine,
                                       lattice_config, lattice_ibm_config, lattice_membrane_config)
@@ -345,7 +351,9 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
         ic = lattice_ibm_config(rbc_interval=interval)
         ic.dt_advance = float(interval)
         if smooth:
-            mc.cos_cut, mc.sliver_len, ic.flags = 1.0, 0.0, 1  # ZF_IBM_CUBE_SUPPORT
+            # bending faded out over |cos| 0.97..0.99 instead of cut at 0.99;
+            # ic.flags 1 = ZF_IBM_CUBE_SUPPORT
+            mc.cos_cut, mc.cos_blend, mc.sliver_len, ic.flags = 0.99, 0.02, 0.0, 1
         eng.membrane_setup(mc, tris, xyz)
         eng.upload_cells(cells)
         eng.ibm_setup(ic)

@@ -163,6 +163,8 @@ typedef struct zf_membrane_config {
   double kbend, kd, ka, kv;
   double area0, vol0;               /* global targets (RBC_SURF, MCV) */
   double x_max, x_floor, cos_cut, area_lo, area_hi;
+  double cos_blend;                 /* 0: bending off above cos_cut (reference); w > 0:
+                                       faded out by smoothstep over [cos_cut - w, cos_cut] */
   double force_limit, vol_limit;    /* per-component guards (1e-10) */
   double sliver_len, sliver_len_min, sliver_f1, sliver_f2;
   double relax, relax2;

@@ -183,7 +183,7 @@ def max_threads():
 class MembParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "len_scale", "kT", "x0", "kbend", "kd", "ka", "kv", "area0", "vol0", "x_max", "x_floor",
-        "cos_cut", "area_lo", "area_hi", "force_limit", "vol_limit", "sliver_len",
+        "cos_cut", "cos_blend", "area_lo", "area_hi", "force_limit", "vol_limit", "sliver_len",
         "sliver_len_min", "sliver_f1", "sliver_f2", "relax", "relax2")]
 
 
@@ -191,7 +191,7 @@ def reference_membrane_params():
     """SI constants of ZebraflowWT.c:34-36, :124-134, :5654-5655."""
     return dict(len_scale=1e-6, kT=300 * 1.3806488e-23, x0=0.3154574132, kbend=1.3856406e-19,
                 kd=3.6e-5, ka=7.7e-5, kv=55.0, area0=144.916e-12, vol0=97.5671e-18, x_max=0.95,
-                x_floor=0.02435, cos_cut=0.99, area_lo=0.1, area_hi=10.0, force_limit=1e-10,
+                x_floor=0.02435, cos_cut=0.99, cos_blend=0.0, area_lo=0.1, area_hi=10.0, force_limit=1e-10,
                 vol_limit=1e-10, sliver_len=0.05, sliver_len_min=0.001, sliver_f1=1e-14,
                 sliver_f2=1e-11, relax=1.0, relax2=1.0)
 

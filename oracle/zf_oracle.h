@@ -116,6 +116,7 @@ typedef struct zfo_memb_params {
   double area0, vol0;        /* global area and volume targets (RBC_SURF, MCV) */
   double x_max, x_floor;     /* e_ratio clamp 0.95, length floor 0.02435 */
   double cos_cut;            /* |cos theta| above which bending is off (0.99) */
+  double cos_blend;          /* > 0: smoothstep fade over [cos_cut - w, cos_cut] (not in the reference) */
   double area_lo, area_hi;   /* triangle area clamp factors (0.1, 10) */
   double force_limit;        /* per-component guard of force terms (1e-10) */
   double vol_limit;          /* guard of the volume-gradient terms (1e-10) */

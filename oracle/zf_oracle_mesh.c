@@ -250,7 +250,12 @@ void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
       for (int d = 0; d < 3; d++) chk += (m1[d] - m2[d]) * (TC[3 * t1 + d] - TC[3 * t2 + d]) * L;
       const double st = chk >= 0 ? sqrt(1 - ct * ct) : -sqrt(1 - ct * ct);
       /* bending (:6984-7081) */
-      const double bb = fabs(ct) > mp->cos_cut ? 0.0 : mp->kbend * (st * cos0[e] - ct * sin0[e]) / st;
+      double bb = fabs(ct) > mp->cos_cut ? 0.0 : mp->kbend * (st * cos0[e] - ct * sin0[e]) / st;
+      if (mp->cos_blend > 0 && fabs(ct) <= mp->cos_cut && fabs(ct) > mp->cos_cut - mp->cos_blend) {
+        /* smooth variant, not in the reference: smoothstep fade to zero at cos_cut */
+        const double t = (mp->cos_cut - fabs(ct)) * (1.0 / mp->cos_blend);
+        bb = bb * (t * t * (3.0 - 2.0 * t));
+      }
       double dp21[3], dp24[3], dp32[3], dp41[3], dp13[3];
       for (int d = 0; d < 3; d++) {
         dp21[d] = L * (p2[d] - p1[d]); dp24[d] = L * (p2[d] - p4[d]); dp32[d] = L * (p3[d] - p2[d]);

@@ -43,14 +43,22 @@ def test_reference_mesh_topology_and_equilibrium():
     eng.close()
 
 
-@pytest.mark.parametrize("ncell", [1, 7])
-def test_forces_match_oracle(ncell):
+@pytest.mark.parametrize("ncell,blend", [(1, 0.0), (7, 0.0), (3, 0.02)])
+def test_forces_match_oracle(ncell, blend):
+    """blend > 0: the smooth bending cut-off over |cos dihedral| in
+    [0.97, 0.99] (its formula is pinned on the oracle)."""
     xyz, x1, tris = deformed(amp=0.1)
-    mp, edge, eq, tae = lattice_params(xyz, tris)
+    mp, edge, eq, tae = lattice_params(xyz, tris, cos_blend=blend)
+    if blend:
+        p = x1[tris - 1]
+        av = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+        n = av / np.linalg.norm(av, axis=1)[:, None]
+        ac = np.abs((n[edge[:, 4] - 1] * n[edge[:, 5] - 1]).sum(1))
+        assert ((ac > 0.99 - blend) & (ac <= 0.99)).sum() > 50  # hinges inside the fade
     rng = np.random.default_rng(ncell)
     cells = np.stack([x1 * (1 + 0.02 * rng.standard_normal()) + rng.uniform(-50, 50, 3)
                       for _ in range(ncell)])
-    print(f"case lattice-unit RBCs ncell {ncell}", flush=True)
+    print(f"case lattice-unit RBCs ncell {ncell} cos_blend {blend}", flush=True)
     eng = engine()
     eng.membrane_setup(membrane_config(dict(mp, es=5e-3), ncell + 1), tris, xyz)
     eng.upload_cells(cells)

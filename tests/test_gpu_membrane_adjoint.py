@@ -33,9 +33,9 @@ def forces(eng, X):
     return m["force"]
 
 
-@pytest.mark.parametrize("sliver", [False, True])
-def test_position_gradient(sliver):
-    print(f"case membrane vjp positions, sliver terms {sliver}", flush=True)
+@pytest.mark.parametrize("sliver,blend", [(False, False), (True, False), (False, True)])
+def test_position_gradient(sliver, blend):
+    print(f"case membrane vjp positions, sliver terms {sliver}, smooth cut-off {blend}", flush=True)
     xyz, x1, tris = deformed(amp=0.1)
     # The force is discontinuous where a term switches (bending off above
     # |cos dihedral| = cos_cut, the anti-sliver thresholds); with 3840 edges
@@ -43,7 +43,9 @@ def test_position_gradient(sliver):
     # thresholds are put in gaps of these cells' values (|cos| 0.985165 ..
     # 0.985216; node-to-centroid 0.4931..0.4956 and 0.0758..0.0876) so that
     # no difference below crosses one.
-    over = dict(cos_cut=0.98519)
+    # With the smooth cut-off (bending faded to zero over |cos| 0.97..0.99)
+    # the bending term has no jump and the reference's 0.99 is kept.
+    over = dict(cos_cut=0.99, cos_blend=0.02) if blend else dict(cos_cut=0.98519)
     if sliver:
         over.update(sliver_len=0.4943, sliver_len_min=0.08)
     mp, edge, eq, tae = lattice_params(xyz, tris, **over)

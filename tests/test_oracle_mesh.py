@@ -173,3 +173,40 @@ def test_volume_force_restores_volume():
     assert (outward > 0).mean() > 0.95
     out2 = po.membrane_forces(x[None], tris, edge, eq0, tae, dict(mp2, kv=2 * mp2["kv"]))
     np.testing.assert_allclose(out2["force"][0], 2 * F, rtol=1e-12, atol=1e-18)
+
+
+def test_bending_blend_scales_one_hinge_by_smoothstep():
+    """Smooth bending cut-off (cos_blend = w > 0, an option without a
+    counterpart in the reference): bending is the reference's for |cos| <=
+    cos_cut - w, off above cos_cut, and in between scaled by the smoothstep
+    s = t^2 (3 - 2t), t = (cos_cut - |cos|)/w.  With exactly one hinge e in
+    the window, the force is F(sharp cut just below e) + s B_e, where B_e,
+    e's whole bending contribution, is the difference of two sharp cut-offs
+    bracketing e (cos_blend = 0 is the reference rule, pinned elsewhere)."""
+    xyz, x1, tris = deformed(amp=0.15)
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    p = x1[tris - 1]
+    av = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    n = av / np.linalg.norm(av, axis=1)[:, None]
+    ac = np.abs(np.clip((n[edge[:, 4] - 1] * n[edge[:, 5] - 1]).sum(1), -1, 1))
+    srt = np.sort(ac)
+    gaps = np.minimum(srt[1:-1] - srt[:-2], srt[2:] - srt[1:-1])
+    gaps[(srt[1:-1] < 0.5) | (srt[1:-1] > 0.99)] = 0
+    k = int(np.argmax(gaps)) + 1
+    c_prev, c_e, c_next = srt[k - 1], srt[k], srt[k + 1]
+    assert gaps[k - 1] > 1e-5
+    F = lambda **o: po.membrane_forces(x1[None], tris, edge, eq, tae, dict(mp, **o))["force"][0]
+    f_lo = F(cos_cut=0.5 * (c_prev + c_e))   # e and every flatter hinge off
+    f_hi = F(cos_cut=0.5 * (c_e + c_next))   # e on
+    B = f_hi - f_lo
+    assert np.abs(B).max() > 1e-3 * np.abs(f_hi).max()
+    np.testing.assert_array_equal(F(cos_cut=0.5 * (c_e + c_next), cos_blend=0.0), f_hi)
+    gap = min(c_e - c_prev, c_next - c_e)
+    w = 0.9 * gap
+    for t in (0.05, 0.3, 0.5, 0.8, 0.97):
+        cut = c_e + t * w          # window [cut - w, cut] holds e only
+        s = t * t * (3 - 2 * t)
+        f = F(cos_cut=cut, cos_blend=w)
+        np.testing.assert_allclose(f, f_lo + s * B, rtol=0,
+                                   atol=1e-9 * np.abs(B).max() + 1e-13 * np.abs(f_hi).max(),
+                                   err_msg=f"t={t}")

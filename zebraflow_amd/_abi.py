@@ -29,7 +29,7 @@ class ZfLatticeInfo(C.Structure):
 class ZfMembraneConfig(C.Structure):
     _fields_ = [(n, f64) for n in (
         "len_scale", "kT", "x0", "es", "kbend", "kd", "ka", "kv", "area0", "vol0", "x_max",
-        "x_floor", "cos_cut", "area_lo", "area_hi", "force_limit", "vol_limit", "sliver_len",
+        "x_floor", "cos_cut", "area_lo", "area_hi", "cos_blend", "force_limit", "vol_limit", "sliver_len",
         "sliver_len_min", "sliver_f1", "sliver_f2", "relax", "relax2")] + [
         ("passes", i32), ("max_cells", i32)]
 

@@ -281,6 +281,7 @@ void zf_membrane_config_reference(zf_membrane_config *m) {
   m->x_max = 0.95;
   m->x_floor = 0.02435;
   m->cos_cut = 0.99;
+  m->cos_blend = 0.0;
   m->area_lo = 0.1;
   m->area_hi = 10.0;
   m->force_limit = 1e-10;

@@ -15,13 +15,13 @@ namespace zf {
 namespace memb {
 
 struct MembParams {
-  double L, kT, kbend, kd, ka, kv, area0, vol0, x_max, x_floor, cos_cut, area_lo, area_hi;
+  double L, kT, kbend, kd, ka, kv, area0, vol0, x_max, x_floor, cos_cut, area_lo, area_hi, cos_blend;
   double force_limit, vol_limit, sliver_len, sliver_len_min, sliver_f1, sliver_f2, relax, relax2;
 };
 
 inline MembParams params_of(const zf_membrane_config &m) {
   return MembParams{m.len_scale, m.kT, m.kbend, m.kd, m.ka, m.kv, m.area0, m.vol0, m.x_max,
-                    m.x_floor, m.cos_cut, m.area_lo, m.area_hi, m.force_limit, m.vol_limit,
+                    m.x_floor, m.cos_cut, m.area_lo, m.area_hi, m.cos_blend, m.force_limit, m.vol_limit,
                     m.sliver_len, m.sliver_len_min, m.sliver_f1, m.sliver_f2, m.relax, m.relax2};
 }
 
@@ -283,7 +283,15 @@ __device__ __forceinline__ EdgeOut<T> edge_role(int r, const V3<T> &p1, const V3
   const T chk = dot(m1 - m2, L * (g1.C - g2.C));
   const T st = chk >= 0.0 ? sqrt(1 - ct * ct) : -sqrt(1 - ct * ct);
   // bending (:6984-7081)
-  const T bb = fabs(val(ct)) > mp.cos_cut ? T(0.0) : mp.kbend * (st * ec.c0 - ct * ec.s0) / st;
+  T bb = fabs(val(ct)) > mp.cos_cut ? T(0.0) : mp.kbend * (st * ec.c0 - ct * ec.s0) / st;
+  if (mp.cos_blend > 0.0 && fabs(val(ct)) <= mp.cos_cut && fabs(val(ct)) > mp.cos_cut - mp.cos_blend) {
+    // smooth cut-off (no counterpart in the reference): bb times the
+    // smoothstep t^2 (3 - 2t) of t = (cos_cut - |cos|)/cos_blend, C1 in the
+    // positions, equal to the reference's bb below cos_cut - cos_blend
+    const T ac = val(ct) < 0.0 ? -ct : ct;
+    const T t = (T(mp.cos_cut) - ac) * (1.0 / mp.cos_blend);
+    bb = bb * (t * t * (T(3.0) - 2.0 * t));
+  }
   const V3<T> dp21 = L * (p2 - p1), dp24 = L * (p2 - p4), dp32 = L * (p3 - p2), dp41 = L * (p4 - p1),
               dp13 = L * (p1 - p3);
   const V3<T> cp1 = a1 * cross(m1, dp21), cp2 = a2 * cross(m2, dp21), cp3 = a1 * cross(m1, dp24),
