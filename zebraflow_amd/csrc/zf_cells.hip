@@ -146,25 +146,6 @@ __global__ void k_fill(int32_t *a, size_t n, int32_t v) {
   if (i < n) a[i] = v;
 }
 
-// The 16 sub-triangle centroids of a triangle refined four times
-// (ZebraflowWT.c:9990-10025), one axis at a time.
-__device__ __forceinline__ void sub16(double a, double b, double c, double tc[16]) {
-  double tp[16];
-  tp[1] = a; tp[2] = b; tp[3] = c;
-  tp[4] = 0.5 * (tp[1] + tp[2]); tp[5] = 0.5 * (tp[2] + tp[3]); tp[6] = 0.5 * (tp[1] + tp[3]);
-  tp[7] = 0.5 * (tp[1] + tp[4]); tp[8] = 0.5 * (tp[2] + tp[4]); tp[9] = 0.5 * (tp[2] + tp[5]);
-  tp[10] = 0.5 * (tp[5] + tp[3]); tp[11] = 0.5 * (tp[3] + tp[6]); tp[12] = 0.5 * (tp[6] + tp[1]);
-  tp[13] = 0.5 * (tp[4] + tp[6]); tp[14] = 0.5 * (tp[4] + tp[5]); tp[15] = 0.5 * (tp[5] + tp[6]);
-  tc[0] = (tp[1] + tp[7] + tp[12]) / 3.0;   tc[1] = (tp[7] + tp[13] + tp[12]) / 3.0;
-  tc[2] = (tp[7] + tp[4] + tp[13]) / 3.0;   tc[3] = (tp[4] + tp[14] + tp[13]) / 3.0;
-  tc[4] = (tp[4] + tp[8] + tp[14]) / 3.0;   tc[5] = (tp[8] + tp[9] + tp[14]) / 3.0;
-  tc[6] = (tp[8] + tp[2] + tp[9]) / 3.0;    tc[7] = (tp[12] + tp[13] + tp[6]) / 3.0;
-  tc[8] = (tp[13] + tp[15] + tp[6]) / 3.0;  tc[9] = (tp[13] + tp[14] + tp[15]) / 3.0;
-  tc[10] = (tp[14] + tp[5] + tp[15]) / 3.0; tc[11] = (tp[14] + tp[9] + tp[5]) / 3.0;
-  tc[12] = (tp[6] + tp[15] + tp[11]) / 3.0; tc[13] = (tp[15] + tp[10] + tp[11]) / 3.0;
-  tc[14] = (tp[15] + tp[5] + tp[10]) / 3.0; tc[15] = (tp[11] + tp[10] + tp[3]) / 3.0;
-}
-
 struct RepParams {
   double k_cell, cutoff, search, k_wall, wall_d0, wall_dmax, wall_search;
   double wall_reach2;  // (wall_search + largest wall-triangle extent)^2, see k_wall_repulsion
@@ -175,15 +156,32 @@ struct RepParams {
 // distances and far below any geometric scale.
 constexpr double kReachSlack = 1e-9;
 
+// Sub-triangle centroids of a triangle refined four times
+// (ZebraflowWT.c:9990-10025): the reference forms them through chained edge
+// midpoints; centroid s lies at (A a + B b + C c)/12 with integer weights
+// (10,1,1) (8,2,2) (7,4,1) (5,5,2) (4,7,1) (2,8,2) (1,10,1) (7,1,4) (5,2,5)
+// (4,4,4) (2,5,5) (1,7,4) (4,1,7) (2,2,8) (1,4,7) (1,1,10), so the values
+// agree to rounding.  The weights are packed 4 bits per centroid (s in bits
+// 4s..4s+3), which lets a lane form one centroid at a time from scalar
+// constants instead of holding all 16.
+constexpr uint64_t kSubWA = 0x112412457124578aull, kSubWB = 0x142175421a875421ull,
+                   kSubWC = 0xa787454541212121ull;
+
 // Cell-cell repulsion, one lane per node of a released cell (status > 0).
-__global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
-                                 const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
-                                 const double *__restrict__ nnormal, const double *__restrict__ tg,
-                                 size_t TT, const int32_t *__restrict__ domid,
-                                 const uint64_t *__restrict__ keys,
-                                 const int32_t *__restrict__ range,
-                                 const unsigned long long *__restrict__ bounds, Grid g,
-                                 RepParams r, double *__restrict__ vrep) {
+// The window is walked row by row (fixed i, j): DOMID is k-fastest and the
+// cell list is sorted by bin, so the triangles of a row's bins in reach are
+// one contiguous run of the sorted keys, visited in the same (bin, key)
+// order as a bin-by-bin walk.  A row's 9 DOMID and bin-range loads are
+// issued together (in-bounds addresses, results selected afterwards), and
+// candidates are fetched 4 at a time, so a lane waits on a few batched
+// loads per row instead of a chain of dependent loads per bin.
+__global__ __launch_bounds__(128) void k_cell_repulsion(
+    const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
+    const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
+    const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
+    const int32_t *__restrict__ domid, const uint64_t *__restrict__ keys,
+    const int32_t *__restrict__ range, const unsigned long long *__restrict__ bounds, Grid g,
+    RepParams r, double *__restrict__ vrep) {
   const int gq = blockIdx.x * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
   const int c = gq / nn, node = gq % nn;
@@ -211,20 +209,42 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
     breach = treach + bd + kReachSlack;
   }
   const double treach2 = treach * treach, breach2 = breach * breach;
+  const int2 *range2 = reinterpret_cast<const int2 *>(range);
   for (int i = lo[0]; i <= hi[0]; i++)
     for (int j = lo[1]; j <= hi[1]; j++) {
       const double qx = g.x0 + (i - 1) * g.dx - xc, qy = g.y0 + (j - 1) * g.dx - yc;
       const double rij = qx * qx + qy * qy;
       if (rij >= breach2) continue;
-      for (int k = lo[2]; k <= hi[2]; k++) {
+      const size_t row = bq(g, i, j, 1);
+      int ov[9];
+#pragma unroll
+      for (int t = 0; t < 9; t++) ov[t] = domid[row + (size_t)(min(lo[2] + t, hi[2]) - 1)];
+      int beg = 0x7fffffff, end = 0;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int k = lo[2] + t;
         const double qz = g.z0 + (k - 1) * g.dx - zc;
-        if (rij + qz * qz >= breach2) continue;
-        const int o = domid[bq(g, i, j, k)];
-        if (o <= 0) continue;
-        const int beg = range[2 * (o - 1)], end = range[2 * (o - 1) + 1];
-        for (int e = beg; e < end; e++) {
-          const int q = (int)(keys[e] & 0xffffffffu);
-          const double dx = xc - tg[4 * TT + q], dy = yc - tg[5 * TT + q], dz = zc - tg[6 * TT + q];
+        const bool ok = k <= hi[2] && rij + qz * qz < breach2 && ov[t] > 0;
+        const int2 rg = range2[ok ? ov[t] - 1 : 0];  // loaded for every t, used by select
+        const bool use = ok && rg.y > rg.x;
+        beg = use ? min(beg, rg.x) : beg;
+        end = use ? max(end, rg.y) : end;
+      }
+      for (int e0 = beg; e0 < end; e0 += 4) {
+        int qv[4];
+        double cxv[4], cyv[4], czv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) qv[u] = (int)(keys[min(e0 + u, end - 1)] & 0xffffffffu);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          cxv[u] = tg[4 * TT + qv[u]];
+          cyv[u] = tg[5 * TT + qv[u]];
+          czv[u] = tg[6 * TT + qv[u]];
+        }
+#pragma unroll 1
+        for (int u = 0; u < 4 && e0 + u < end; u++) {
+          const int q = qv[u];
+          const double dx = xc - cxv[u], dy = yc - cyv[u], dz = zc - czv[u];
           const double dc2 = dx * dx + dy * dy + dz * dz;
           if (dc2 >= treach2 || sqrt(dc2) >= r.search) continue;
           const int m = q / nt, t = q % nt;
@@ -232,18 +252,23 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
           if (m == c && (n1 == node || n2 == node || n3 == node)) continue;
           if (e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0) continue;
           const size_t b0 = (size_t)m * nn;
-          double tx[16], ty[16], tz[16];
-          sub16(X[b0 + n1], X[b0 + n2], X[b0 + n3], tx);
-          sub16(X[NT + b0 + n1], X[NT + b0 + n2], X[NT + b0 + n3], ty);
-          sub16(X[2 * NT + b0 + n1], X[2 * NT + b0 + n2], X[2 * NT + b0 + n3], tz);
-          for (int s = 0; s < 16; s++) {
-            const double ax = tx[s] - xc, ay = ty[s] - yc, az = tz[s] - zc;
-            const double d = sqrt(ax * ax + ay * ay + az * az);
+          const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = X[2 * NT + b0 + n1];
+          const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = X[2 * NT + b0 + n2];
+          const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = X[2 * NT + b0 + n3];
+#pragma unroll 2
+          for (int sc = 0; sc < 16; sc++) {
+            const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
+            const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
+            const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
+            const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
+            const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
+            const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
+            const double d = sqrt(px * px + py * py + pz * pz);
             if (d < r.cutoff) {
               const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
-              v0 += f * ax / d;
-              v1 += f * ay / d;
-              v2 += f * az / d;
+              v0 += f * px / d;
+              v1 += f * py / d;
+              v2 += f * pz / d;
             }
           }
         }
@@ -255,11 +280,12 @@ __global__ void k_cell_repulsion(const double *__restrict__ X, size_t NT, int nn
 }
 
 // Cell-wall repulsion, one lane per node of a released cell.
-__global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn, int ncell,
+__global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn, int ncell,
                                  const int32_t *__restrict__ status, const int32_t *__restrict__ windex,
                                  const float *__restrict__ wdist,
                                  const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
-                                 const double *__restrict__ wxyz, const double *__restrict__ wnormal,
+                                 const double *__restrict__ wxyz, const double *__restrict__ wtgeo,
+                                 const double *__restrict__ wnormal,
                                  Grid g, RepParams r, double *__restrict__ vrep,
                                  int32_t *__restrict__ nearest) {
   const int gq = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,35 +314,47 @@ __global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn
   int lo[3], hi[3];
   window(g, x, y, z, lo, hi);
   double mind = r.wall_search;
-  int triwall = 0;
+  int triwall = 0, last = 0;
   for (int i = lo[0]; i <= hi[0]; i++)
-    for (int j = lo[1]; j <= hi[1]; j++)
-      for (int k = lo[2]; k <= hi[2]; k++) {
-        const int wn = windex[bq(g, i, j, k)];
-        if (wn <= 0) continue;
+    for (int j = lo[1]; j <= hi[1]; j++) {
+      // the row's 9 W_index entries in one batch of loads (in-bounds
+      // addresses; entries past hi[2] are not used)
+      const size_t row = bq(g, i, j, 1);
+      int wv[9];
+#pragma unroll
+      for (int t = 0; t < 9; t++) wv[t] = windex[row + (size_t)(min(lo[2] + t, hi[2]) - 1)];
+#pragma unroll 1
+      for (int t = 0; t < 9 && lo[2] + t <= hi[2]; t++) {
+        const int wn = wv[t];
+        // a wall node just examined cannot lower mindis again (strict <),
+        // and neighbouring lattice nodes mostly share their nearest wall node
+        if (wn <= 0 || wn == last) continue;
+        last = wn;
         // every vertex and the centroid of a triangle of wall node wn lie
         // within wall_lmax of it: if wn is farther than wall_search +
         // wall_lmax, none of them can beat mindis <= wall_search
         const double *wp = &wxyz[3 * (size_t)(wn - 1)];
         const double ex = x - wp[0], ey = y - wp[1], ez = z - wp[2];
         if (ex * ex + ey * ey + ez * ez >= r.wall_reach2) continue;
-        const int32_t *row = &wnode_tri[13 * (size_t)(wn - 1)];
-        const int cnt = min(row[0], 12);
+        const int32_t *trow = &wnode_tri[13 * (size_t)(wn - 1)];
+        const int cnt = min(trow[0], 12);
+#pragma unroll 1
         for (int l = 1; l <= cnt; l++) {
-          const int t = row[l];
-          double cx = 0, cy = 0, cz = 0;
-          for (int s = 0; s < 3; s++) {
-            const double *w = &wxyz[3 * (size_t)(wtri[3 * (size_t)(t - 1) + s] - 1)];
-            const double d = sqrt((x - w[0]) * (x - w[0]) + (y - w[1]) * (y - w[1]) + (z - w[2]) * (z - w[2]));
-            if (d < mind) { mind = d; triwall = t; }
-            cx += w[0] / 3.0;
-            cy += w[1] / 3.0;
-            cz += w[2] / 3.0;
+          const int tw = trow[l];
+          // vertices and centroid of wall triangle tw, one contiguous record
+          const double *tgp = &wtgeo[12 * (size_t)(tw - 1)];
+          double gv[12];
+#pragma unroll
+          for (int u = 0; u < 12; u++) gv[u] = tgp[u];
+#pragma unroll
+          for (int sv = 0; sv < 4; sv++) {  // vertices 0..2, then the centroid
+            const double wx = gv[3 * sv], wy = gv[3 * sv + 1], wz = gv[3 * sv + 2];
+            const double d = sqrt((x - wx) * (x - wx) + (y - wy) * (y - wy) + (z - wz) * (z - wz));
+            if (d < mind) { mind = d; triwall = tw; }
           }
-          const double d = sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz));
-          if (d < mind) { mind = d; triwall = t; }
         }
       }
+    }
   if (triwall == 0) return;
   nearest[gq] = triwall;
   const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
@@ -329,6 +367,29 @@ __global__ void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn
     vrep[NT + gq] += f * r.k_wall * n[1];
     vrep[2 * NT + gq] += f * r.k_wall * n[2];
   }
+}
+
+// Vertices and centroid of every wall triangle as one 12-double record (the
+// centroid summed as w/3 vertex by vertex, the rounding the repulsion search
+// compares against).
+__global__ void k_wall_geom(const double *__restrict__ wxyz, const int32_t *__restrict__ wtri,
+                            int nwt, double *__restrict__ wtgeo) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nwt) return;
+  double *o = &wtgeo[12 * (size_t)m];
+  double cx = 0, cy = 0, cz = 0;
+  for (int s = 0; s < 3; s++) {
+    const double *w = &wxyz[3 * (size_t)(wtri[3 * (size_t)m + s] - 1)];
+    o[3 * s] = w[0];
+    o[3 * s + 1] = w[1];
+    o[3 * s + 2] = w[2];
+    cx += w[0] / 3.0;
+    cy += w[1] / 3.0;
+    cz += w[2] / 3.0;
+  }
+  o[9] = cx;
+  o[10] = cy;
+  o[11] = cz;
 }
 
 // W_index: every lattice node in a wall node's window keeps the nearest wall
@@ -377,7 +438,8 @@ int require_rep(zf_ctx *ctx) {
 
 void free_wall(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
-                   (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist};
+                   (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist,
+                   (void **)&ctx->d_wtgeo};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_wall = false;
 }
@@ -433,7 +495,7 @@ int repulsion_impl(zf_ctx *ctx) {
     hipLaunchKernelGGL(k_wall_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                        ctx->nn, ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wdist,
                        ctx->d_wnode_tri,
-                       ctx->d_wtri, ctx->d_wxyz, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep,
+                       ctx->d_wtri, ctx->d_wxyz, ctx->d_wtgeo, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep,
                        ctx->d_near);
   hipLaunchKernelGGL(k_cell_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
@@ -521,6 +583,7 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_wxyz, 24 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnormal, 24 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wtri, 12 * (size_t)nwt)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wtgeo, 96 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnode_tri, 52 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_windex, 4 * ctx->box)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wdist, 4 * ctx->box)) ||
@@ -530,6 +593,8 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   ZF_HIP(ctx, hipMemcpy(ctx->d_wnormal, wall_normal, 24 * (size_t)nwt, hipMemcpyHostToDevice));
   ZF_HIP(ctx, hipMemcpy(ctx->d_wtri, wall_tri, 12 * (size_t)nwt, hipMemcpyHostToDevice));
   ZF_HIP(ctx, hipMemcpy(ctx->d_wnode_tri, rows.data(), 52 * (size_t)nwn, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_wall_geom, dim3(nb(nwt, 256)), dim3(256), 0, ctx->stream, ctx->d_wxyz,
+                     ctx->d_wtri, nwt, ctx->d_wtgeo);
   hipLaunchKernelGGL(k_fill_u64, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key, ctx->box);
   hipLaunchKernelGGL(k_windex, dim3(nb(nwn, 64)), dim3(64), 0, ctx->stream, ctx->d_wxyz, nwn,
                      grid_of(ctx), key);

@@ -172,7 +172,11 @@ __device__ __forceinline__ double kcos_pi(double x) { return cospi(x); }
 
 // One wave per tile; lane = lattice node (4x4x4).  Gathers every membrane
 // node of the 27 neighbouring tiles in (tile, node) order, in the field's
-// type F (positions relative to the tile origin, weights and sums).
+// type F (positions relative to the tile origin, weights and sums).  The
+// kernel weight is a product of one factor per axis, and a lane's offset
+// from a membrane node along an axis takes one of 4 values (its row in the
+// tile), so the lane staging a node evaluates its 12 factors once and the
+// other lanes read theirs from LDS (same expressions, same values).
 template <class F>
 __global__ __launch_bounds__(64) void k_spread_gather(
     const uint64_t *__restrict__ keys, const int32_t *__restrict__ range,
@@ -180,6 +184,7 @@ __global__ __launch_bounds__(64) void k_spread_gather(
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, IbmParams p,
     F *__restrict__ force, size_t stride) {
   __shared__ F sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
+  __shared__ F kx[4][64], ky[4][64], kz[4][64];  // (1 + cos(pi |r - s| / 2)) for tile row r
   const int tile = blockIdx.x;
   const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
   const int lane = threadIdx.x;
@@ -188,7 +193,8 @@ __global__ __launch_bounds__(64) void k_spread_gather(
   const int li = tx * kTileW + lane / 16, lj = ty * kTileW + (lane / 4) % 4, lk = tz * kTileW + lane % 4;
   const double ox = p.x0 + (double)(tx * kTileW) * p.dx, oy = p.y0 + (double)(ty * kTileW) * p.dx,
                oz = p.z0 + (double)(tz * kTileW) * p.dx;
-  const F rx = (F)(lane / 16), ry = (F)((lane / 4) % 4), rz = (F)(lane % 4);
+  const int ix = lane / 16, iy = (lane / 4) % 4, iz = lane % 4;
+  const F rx = (F)ix, ry = (F)iy, rz = (F)iz;
   F a0 = 0, a1 = 0, a2 = 0;
   bool any = false;
   for (int dxT = -1; dxT <= 1; dxT++)
@@ -202,21 +208,28 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           const int cnt = min(64, end - c0);
           if (lane < cnt) {
             const int g = (int)(keys[c0 + lane] & 0xffffffffu);
-            sx[lane] = (F)((X[g] - ox) / p.dx);
-            sy[lane] = (F)((X[NT + g] - oy) / p.dx);
-            sz[lane] = (F)((X[2 * NT + g] - oz) / p.dx);
+            const F px = (F)((X[g] - ox) / p.dx), py = (F)((X[NT + g] - oy) / p.dx),
+                    pz = (F)((X[2 * NT + g] - oz) / p.dx);
+            sx[lane] = px;
+            sy[lane] = py;
+            sz[lane] = pz;
             sf0[lane] = ffl[g];
             sf1[lane] = ffl[NT + g];
             sf2[lane] = ffl[2 * NT + g];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
+              kx[r][lane] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
+              ky[r][lane] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
+              kz[r][lane] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
+            }
           }
           __syncthreads();
           any = true;
           for (int m = 0; m < cnt; m++) {
             const F dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
             if (off_support(dx, dy, dz, p.cube)) continue;
-            // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
-            const F w = ((F)1 + kcos_pi((F)0.5 * fabs(dx))) * ((F)1 + kcos_pi((F)0.5 * fabs(dy))) *
-                        ((F)1 + kcos_pi((F)0.5 * fabs(dz))) * ((F)1 / (F)64);
+            const F w = kx[ix][m] * ky[iy][m] * kz[iz][m] * ((F)1 / (F)64);
             a0 += w * sf0[m];
             a1 += w * sf1[m];
             a2 += w * sf2[m];
