@@ -475,6 +475,8 @@ def main():
     ap.add_argument("--cells", type=int, default=1200,
                     help="cells of the dense-cell leg (0 skips it)")
     ap.add_argument("--cell-steps", type=int, default=200)
+    ap.add_argument("--cells-only", action="store_true",
+                    help="only the dense-cell leg (for profiling its kernels)")
     ap.add_argument("--mixed-precision-only", action="store_true",
                     help="run only the large mixed-precision leg (tree, FP16/FP32/FP64 with "
                          "cells) and print its JSON object")
@@ -500,6 +502,10 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)
+        return
+    if args.cells_only:
+        print(json.dumps(dict(dense_cells=cells_leg(args.workload, b, args.cells, args.cell_steps,
+                                                     max(1, args.cell_steps // 10)))), flush=True)
         return
     n_fluid = int(((b > 0) & (b < 10)).sum())
     eng, ms, wall = time_form(args.workload, b, args.store, args.steps, args.warmup)

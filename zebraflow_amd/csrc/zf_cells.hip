@@ -64,7 +64,7 @@ __global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double
                           const int32_t *__restrict__ domid, Grid g, double cutoff, int dommax,
                           int32_t *__restrict__ tbin, uint64_t *__restrict__ keys,
                           unsigned long long *__restrict__ bounds) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool live = q < ncell * nt;
   int bin = 0;
   double bdist = 0, ext = 0;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
     const int32_t *__restrict__ domid, const uint64_t *__restrict__ keys,
     const int32_t *__restrict__ range, const unsigned long long *__restrict__ bounds, Grid g,
     RepParams r, double *__restrict__ vrep) {
-  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
   const int c = gq / nn, node = gq % nn;
   if (status[c] <= 0) return;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
                                  const double *__restrict__ wnormal,
                                  Grid g, RepParams r, double *__restrict__ vrep,
                                  int32_t *__restrict__ nearest) {
-  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
   nearest[gq] = 0;
   if (status[gq / nn] <= 0) return;

@@ -87,14 +87,13 @@ __device__ __forceinline__ R geq(int c, R drho, R ux, R uy, R uz, R usq) {
 // round of 8k keep their place).  Every order is a permutation of [0, nblk).
 __device__ __forceinline__ int logical_block(int b, int nblk, int order) {
   if (order == 0) return b;
-  const int x = b % 8, i = b / 8;
   if (order > 1) {
+    const int x = b % 8, i = b / 8;
     const int round = 8 * order, full = nblk / round * round;
     if (b >= full) return b;
     return (i / order) * round + x * order + i % order;
   }
-  const int q = nblk / 8, r = nblk % 8;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+  return xcd_range_block(b, nblk);
 }
 
 // Population load and store with a compile-time cache policy: non-temporal

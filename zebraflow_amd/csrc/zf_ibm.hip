@@ -83,7 +83,7 @@ __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int nc
                          const int32_t *__restrict__ status, const int32_t *__restrict__ b,
                          const int32_t *__restrict__ sbox, const U *__restrict__ umac,
                          size_t stride, IbmParams p, double *__restrict__ V) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
   if (status[g / nn] < 0) return;
   const double xp = X[g], yp = X[NT + g], zp = X[2 * NT + g];
@@ -130,7 +130,7 @@ __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V
                             const double *__restrict__ vrep, const double *__restrict__ F,
                             size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
                             IbmParams p, U *__restrict__ ffl, uint64_t *__restrict__ keys) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
   const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
   if (status[g / nn] < 0) {

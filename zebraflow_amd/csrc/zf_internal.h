@@ -61,6 +61,15 @@ struct DevBuf {
   size_t n = 0;
 };
 
+// Workgroup b of nblk renumbered so that each of the 8 XCDs (the hardware
+// deals workgroups to them round-robin) runs one contiguous range of the
+// grid: neighbouring workgroups (the nodes of one cell, cells adjacent in
+// index order) then share one XCD's L2.  A permutation of [0, nblk).
+__device__ __forceinline__ int xcd_range_block(int b, int nblk) {
+  const int q = nblk / 8, r = nblk % 8, x = b % 8, i = b / 8;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 }  // namespace zf
 
 struct zf_ctx {

@@ -33,7 +33,7 @@ using namespace memb;
 __global__ void k_tri_geom(const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
                            const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
                            double L, double *__restrict__ tg, size_t TT) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (idx >= ncell * nt) return;
   const int c = idx / nt, t = idx % nt;
   if (status[c] < 0) return;
@@ -100,7 +100,7 @@ __global__ void k_node_force(const double *__restrict__ X, size_t NT, int ncell,
                              double *__restrict__ F, double *__restrict__ NN,
                              int32_t *__restrict__ fail) {
   const int nn = mt.nn;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (idx >= ncell * nn) return;
   const int c = idx / nn, n = idx % nn;
   const size_t q = (size_t)c * nn + n;
