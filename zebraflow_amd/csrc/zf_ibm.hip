@@ -205,23 +205,37 @@ __global__ __launch_bounds__(64) void k_spread_gather(
         const int nt = (nx * p.nty + ny) * p.ntz + nz;
         const int beg = range[2 * nt], end = range[2 * nt + 1];
         for (int c0 = beg; c0 < end; c0 += 64) {
-          const int cnt = min(64, end - c0);
-          if (lane < cnt) {
-            const int g = (int)(keys[c0 + lane] & 0xffffffffu);
-            const F px = (F)((X[g] - ox) / p.dx), py = (F)((X[NT + g] - oy) / p.dx),
-                    pz = (F)((X[2 * NT + g] - oz) / p.dx);
-            sx[lane] = px;
-            sy[lane] = py;
-            sz[lane] = pz;
-            sf0[lane] = ffl[g];
-            sf1[lane] = ffl[NT + g];
-            sf2[lane] = ffl[2 * NT + g];
+          // staged nodes are compacted in order to those within 2 of this
+          // tile along every axis: any other one is off the support of all
+          // 64 lanes, so the sums are unchanged
+          bool rel = false;
+          int slot = 0;
+          F px = 0, py = 0, pz = 0;
+          int g = 0;
+          if (lane < end - c0) {
+            g = (int)(keys[c0 + lane] & 0xffffffffu);
+            px = (F)((X[g] - ox) / p.dx);
+            py = (F)((X[NT + g] - oy) / p.dx);
+            pz = (F)((X[2 * NT + g] - oz) / p.dx);
+            rel = px >= (F)-2 && px <= (F)5 && py >= (F)-2 && py <= (F)5 && pz >= (F)-2 && pz <= (F)5;
+          }
+          const unsigned long long relm = __ballot(rel);
+          const int cnt = __popcll(relm);
+          slot = __popcll(relm & ((1ull << lane) - 1ull));
+          if (rel) {
+            const int lane_ = slot;
+            sx[lane_] = px;
+            sy[lane_] = py;
+            sz[lane_] = pz;
+            sf0[lane_] = ffl[g];
+            sf1[lane_] = ffl[NT + g];
+            sf2[lane_] = ffl[2 * NT + g];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
-              kx[r][lane] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
-              ky[r][lane] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
-              kz[r][lane] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
+              kx[r][lane_] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
+              ky[r][lane_] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
+              kz[r][lane_] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
             }
           }
           __syncthreads();
