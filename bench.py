@@ -131,8 +131,8 @@ def node_normals(wx, wt, wn):
 
 
 def form_errors(name, b, steps):
-    """Both forms from the same equilibrium start: rel. L2 error of the
-    FP32-deviation form's velocity and of its wall shear stress (zf_wss at
+    """The three forms from the same equilibrium start: rel. L2 error of the
+    FP32- and FP16-deviation forms' velocity and wall shear stress (zf_wss at
     every lumen wall-mesh node, mu/DX = 1) against the FP64 form."""
     from tests import geometry as g
     wx, wt, wn = g.wall_mesh(b)
@@ -140,15 +140,15 @@ def form_errors(name, b, steps):
     ok = np.isfinite(nrm).all(axis=1)  # nodes of degenerate wall triangles only: no normal
     wx, nrm = wx[ok], nrm[ok]
     out = {}
-    for store in ("f64", "f32dev"):
+    for store in ("f64", "f32dev", "f16dev"):
         eng = engine_for(name, b, store)
         eng.run(0, steps)
-        out[store] = (eng.macro(), eng.wss(wx, nrm, 1.0)[0])
+        out[store] = (eng.macro()[1:4], eng.wss(wx, nrm, 1.0)[0])
         eng.close()
-    u64, u32 = out["f64"][0][1:4], out["f32dev"][0][1:4]
-    w64, w32 = out["f64"][1], out["f32dev"][1]
-    return dict(vel_rel_l2_vs_fp64=float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
-                wss_rel_l2_vs_fp64=float(np.linalg.norm(w32 - w64) / np.linalg.norm(w64)),
+    rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    (u64, w64), (u32, w32), (u16, w16) = out["f64"], out["f32dev"], out["f16dev"]
+    return dict(vel_rel_l2_vs_fp64=rel(u32, u64), wss_rel_l2_vs_fp64=rel(w32, w64),
+                fp16_vel_rel_l2_vs_fp64=rel(u16, u64), fp16_wss_rel_l2_vs_fp64=rel(w16, w64),
                 wall_points=len(wx), steps=steps)
 
 
@@ -232,15 +232,17 @@ def cells_leg(name, b, ncell, steps, warmup):
                     rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))))
 
 
-def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
+def mixed_precision_leg(steps=3000, ncell=6000, interval=1, chunk=500):
     """The large mixed-precision run: the 4-generation tree (1024x256x256,
     12 % fluid) with as many Evans-Fung cells (642 nodes, radius 4) as the
     dense placer fits up to `ncell`, membrane/IBM/repulsion every `interval`
-    iterations with `interval`-step position updates, `steps` iterations in the FP16,
-    FP32 and FP64 forms from the same start.  Velocity, wall shear stress
-    (every lumen wall-mesh node, mu/DX = 1) and cell shape (node positions
-    relative to the mean node displacement, cells live in every form) of the
-    FP16 and FP32 forms against the FP64 form, and FP16 against FP32."""
+    iterations with `interval`-step position updates, `steps` iterations in
+    the FP16, FP32 and FP64 forms from the same start, advanced in lockstep.
+    Every `chunk` steps: velocity, wall shear stress (every lumen wall-mesh
+    node, mu/DX = 1) and cell shape (node positions relative to the mean node
+    displacement, cells live in every form) of the FP16 and FP32 forms
+    against the FP64 form and of FP16 against FP32, so that the growth of
+    the differences along the run is recorded."""
     from tests import geometry as g
     from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
                                       lattice_repulsion_config)
@@ -253,8 +255,9 @@ def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
     ok = np.isfinite(nrm).all(axis=1)  # nodes of degenerate wall triangles only: no normal
     px, pn = wx[ok], nrm[ok]
     print(f"mixed precision: {len(cells)} cells placed, {len(px)} wall points", flush=True)
-    res = {}
-    for store in ("f16dev", "f32dev", "f64"):
+    forms = ("f16dev", "f32dev", "f64")
+    engs = {}
+    for store in forms:
         eng = engine_for(name, b, store)
         eng.membrane_setup(lattice_membrane_config(xyz, tris, len(cells)), tris, xyz)
         eng.upload_cells(cells)
@@ -264,39 +267,53 @@ def mixed_precision_leg(steps=3000, ncell=6000, interval=1):
         eng.repulsion_setup(lattice_repulsion_config())
         eng.wall_setup(wx, wt, wn)
         eng.sync()
-        eng.record(0)
-        for chunk in range(0, steps, 500):
-            eng.full_run(chunk, min(500, steps - chunk))
-            eng.sync()
-            _, u, v, w = eng.macro()
-            print(f"mixed precision: {store} {chunk + min(500, steps - chunk)} steps, "
-                  f"{int((eng.cells()[2] > 0).sum())} cells live, max |u| "
-                  f"{float(np.sqrt(u * u + v * v + w * w).max()):.4f}", flush=True)
-        eng.record(1)
-        eng.sync()
-        x, _, st = eng.cells()
-        d, u, v, w = eng.macro()
-        res[store] = dict(ms=eng.elapsed_ms(0, 1), u=np.stack([u, v, w]), wss=eng.wss(px, pn, 1.0)[0],
-                          x=x, st=st, n=eng.n_interior)
-        eng.close()
-    live = (res["f16dev"]["st"] > 0) & (res["f32dev"]["st"] > 0) & (res["f64"]["st"] > 0)
+        engs[store] = eng
+    ms = {k: 0.0 for k in forms}
 
-    def err(a, ref):
+    def err(res, a, ref, live):
         ea, er = res[a], res[ref]
-        dx = np.linalg.norm(er["x"][live] - cells[live])
+        dx = np.linalg.norm(res["f64"]["x"][live] - cells[live])
         return dict(vel_rel_l2=float(np.linalg.norm(ea["u"] - er["u"]) / np.linalg.norm(er["u"])),
                     wss_rel_l2=float(np.linalg.norm(ea["wss"] - er["wss"]) / np.linalg.norm(er["wss"])),
                     shape_rel=float(np.linalg.norm(ea["x"][live] - er["x"][live]) / dx))
 
+    growth = []
+    for c0 in range(0, steps, chunk):
+        n = min(chunk, steps - c0)
+        res = {}
+        for store, eng in engs.items():
+            eng.record(0)
+            eng.full_run(c0, n)
+            eng.record(1)
+            eng.sync()
+            ms[store] += eng.elapsed_ms(0, 1)
+            x, _, st = eng.cells()
+            _, u, v, w = eng.macro()
+            res[store] = dict(u=np.stack([u, v, w]), wss=eng.wss(px, pn, 1.0)[0], x=x, st=st)
+        live = (res["f16dev"]["st"] > 0) & (res["f32dev"]["st"] > 0) & (res["f64"]["st"] > 0)
+        umax = float(np.sqrt((res["f64"]["u"] ** 2).sum(0)).max())
+        row = dict(step=c0 + n, cells_live=int(live.sum()), max_u_f64=round(umax, 5),
+                   f16_vs_f64=err(res, "f16dev", "f64", live), f32_vs_f64=err(res, "f32dev", "f64", live),
+                   f16_vs_f32=err(res, "f16dev", "f32dev", live))
+        growth.append(row)
+        print(f"mixed precision: {c0 + n} steps, {row['cells_live']} cells live in all forms, "
+              f"max |u| {umax:.4f}, vel f16/f64 {row['f16_vs_f64']['vel_rel_l2']:.2e} "
+              f"f32/f64 {row['f32_vs_f64']['vel_rel_l2']:.2e}", flush=True)
+    n_int = engs["f64"].n_interior
+    for eng in engs.values():
+        eng.close()
     nn = len(xyz)
-    return dict(workload=f"{name}: 1024x256x256 tree, {res['f64']['n']} fluid nodes, "
+    last = growth[-1]
+    return dict(workload=f"{name}: 1024x256x256 tree, {n_int} fluid nodes, "
                          f"{len(cells)} cells of {nn} nodes",
                 cells=len(cells), membrane_nodes=len(cells) * nn, steps=steps, rbc_interval=interval,
-                cells_live_all_forms=int(live.sum()), wall_points=len(px),
-                ms={k: round(r["ms"], 1) for k, r in res.items()},
-                fluid_mlups={k: round(r["n"] * steps / (r["ms"] * 1e-3) / 1e6, 1) for k, r in res.items()},
-                f16_vs_f64=err("f16dev", "f64"), f32_vs_f64=err("f32dev", "f64"),
-                f16_vs_f32=err("f16dev", "f32dev"))
+                cells_live_all_forms=last["cells_live"], wall_points=len(px),
+                ms={k: round(v, 1) for k, v in ms.items()},
+                ms_per_step={k: round(v / steps, 3) for k, v in ms.items()},
+                rbc_node_updates_per_s={k: round(len(cells) * nn * steps / interval / (v * 1e-3), 1)
+                                        for k, v in ms.items()},
+                f16_vs_f64=last["f16_vs_f64"], f32_vs_f64=last["f32_vs_f64"],
+                f16_vs_f32=last["f16_vs_f32"], growth=growth)
 
 
 def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h=(1e-4, 1e-6),
@@ -477,6 +494,8 @@ def main():
     ap.add_argument("--cell-steps", type=int, default=200)
     ap.add_argument("--cells-only", action="store_true",
                     help="only the dense-cell leg (for profiling its kernels)")
+    ap.add_argument("--mixed-steps", type=int, default=3000)
+    ap.add_argument("--mixed-cells", type=int, default=6000)
     ap.add_argument("--mixed-precision-only", action="store_true",
                     help="run only the large mixed-precision leg (tree, FP16/FP32/FP64 with "
                          "cells) and print its JSON object")
@@ -498,7 +517,8 @@ def main():
         probe(args.workload, b)
         return
     if args.mixed_precision_only:
-        print(json.dumps(dict(mixed_precision=mixed_precision_leg())), flush=True)
+        print(json.dumps(dict(mixed_precision=mixed_precision_leg(
+            steps=args.mixed_steps, ncell=args.mixed_cells, chunk=min(500, args.mixed_steps)))), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_cell_reduce(const double *__restrict__ 
 // One lane per (cell, node): the node's normal, its anti-sliver terms and
 // every rhombus (edge) it belongs to in edge order, so F_Total is summed in
 // the reference's order with no atomics and no per-edge scratch.
-__global__ void k_node_force(const double *__restrict__ X, size_t NT, int ncell, MeshTables mt,
+__global__ __launch_bounds__(128) void k_node_force(const double *__restrict__ X, size_t NT, int ncell, MeshTables mt,
                              const double *__restrict__ tg, size_t TT, const double *__restrict__ cell,
                              const int32_t *__restrict__ status, MembParams mp,
                              double *__restrict__ F, double *__restrict__ NN,
