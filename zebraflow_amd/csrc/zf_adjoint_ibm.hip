@@ -228,13 +228,18 @@ __global__ __launch_bounds__(64) void k_interp_adj_gather(
     const double *__restrict__ X, const double *__restrict__ lamV, size_t NT,
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox,
     const typename Store<S>::T *__restrict__ f, size_t stride, AP p, A *__restrict__ lam) {
-  __shared__ double sx[64], sy[64], sz[64], s0[64], s1[64], s2[64];
+  using R = W<S>;
+  __shared__ double s0[64], s1[64], s2[64];
+  __shared__ R kx[4][64], ky[4][64], kz[4][64];  // (1 + cos(pi |d| / 2)) for tile row r
+  __shared__ double sx[64], sy[64], sz[64];
   const int tile = blockIdx.x;
   const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
   const int lane = threadIdx.x;
-  const int li = tx * kTile + lane / 16, lj = ty * kTile + (lane / 4) % 4, lk = tz * kTile + lane % 4;
+  const int ix = lane / 16, iy = (lane / 4) % 4, iz = lane % 4;
+  const int li = tx * kTile + ix, lj = ty * kTile + iy, lk = tz * kTile + iz;
   const double qx = p.x0 + li * p.dx, qy = p.y0 + lj * p.dx, qz = p.z0 + lk * p.dx;
   const double inv = 1.0 / p.dx;
+  const R h = (R)0.5;
   double a0 = 0, a1 = 0, a2 = 0;
   for (int dxT = -1; dxT <= 1; dxT++)
     for (int dyT = -1; dyT <= 1; dyT++)
@@ -244,25 +249,53 @@ __global__ __launch_bounds__(64) void k_interp_adj_gather(
         const int nt = (nx * p.nty + ny) * p.ntz + nz;
         const int beg = range[2 * nt], end = range[2 * nt + 1];
         for (int c0 = beg; c0 < end; c0 += 64) {
-          const int cnt = min(64, end - c0);
-          if (lane < cnt) {
-            const int g = (int)(keys[c0 + lane] & 0xffffffffu);
-            sx[lane] = X[g];
-            sy[lane] = X[NT + g];
-            sz[lane] = X[2 * NT + g];
-            s0[lane] = lamV[g];
-            s1[lane] = lamV[NT + g];
-            s2[lane] = lamV[2 * NT + g];
+          // The weight is a product of one factor per axis, and a lane's
+          // offset from a node along an axis is one of 4 (its tile row): the
+          // staging lane evaluates the node's 12 offsets and factors with the
+          // lanes' own expressions, and nodes at more than 2 DX from every
+          // row of some axis (off the support of all 64 lanes) are dropped;
+          // the others are compacted in order, so the sums are unchanged.
+          bool rel = false;
+          double px = 0, py = 0, pz = 0, dv[3][4];
+          int g = 0;
+          if (lane < end - c0) {
+            g = (int)(keys[c0 + lane] & 0xffffffffu);
+            px = X[g];
+            py = X[NT + g];
+            pz = X[2 * NT + g];
+            bool ax[3] = {false, false, false};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              dv[0][r] = (p.x0 + (tx * kTile + r) * p.dx - px) * inv;
+              dv[1][r] = (p.y0 + (ty * kTile + r) * p.dx - py) * inv;
+              dv[2][r] = (p.z0 + (tz * kTile + r) * p.dx - pz) * inv;
+#pragma unroll
+              for (int a = 0; a < 3; a++) ax[a] = ax[a] || fabs(dv[a][r]) <= 2.0;
+            }
+            rel = ax[0] && ax[1] && ax[2];
+          }
+          const unsigned long long relm = __ballot(rel);
+          const int cnt = __popcll(relm);
+          if (rel) {
+            const int sl = __popcll(relm & ((1ull << lane) - 1ull));
+            sx[sl] = px;
+            sy[sl] = py;
+            sz[sl] = pz;
+            s0[sl] = lamV[g];
+            s1[sl] = lamV[NT + g];
+            s2[sl] = lamV[2 * NT + g];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              kx[r][sl] = (R)1 + cos_pi(h * fabs((R)dv[0][r]));
+              ky[r][sl] = (R)1 + cos_pi(h * fabs((R)dv[1][r]));
+              kz[r][sl] = (R)1 + cos_pi(h * fabs((R)dv[2][r]));
+            }
           }
           __syncthreads();
           for (int m = 0; m < cnt; m++) {
             const double dxl = (qx - sx[m]) * inv, dyl = (qy - sy[m]) * inv, dzl = (qz - sz[m]) * inv;
             if (off_support(dxl, dyl, dzl, p.cube)) continue;
-            using R = W<S>;
-            const R h = (R)0.5;
-            const double w = (double)(((R)1 + cos_pi(h * fabs((R)dxl))) *
-                                      ((R)1 + cos_pi(h * fabs((R)dyl))) *
-                                      ((R)1 + cos_pi(h * fabs((R)dzl))) * (R)(1.0 / 64.0));
+            const double w = (double)(kx[ix][m] * ky[iy][m] * kz[iz][m] * (R)(1.0 / 64.0));
             a0 += w * s0[m];
             a1 += w * s1[m];
             a2 += w * s2[m];
