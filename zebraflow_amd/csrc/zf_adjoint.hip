@@ -355,13 +355,20 @@ __global__ void k_stencil_table(const int32_t *__restrict__ b, const int32_t *__
 }
 
 // Unshifted velocity and density of every support node (stencil nodes of
-// the probes): su[4 i ..] = (u_x, u_y, u_z, 1 + drho).
+// the probes): su[4 i ..] = (u_x, u_y, u_z, 1 + drho).  Batched over states
+// (blockIdx.y = b): state r0 + b is f0 for r0 + b == 0, else the
+// (r0 + b - 1)-th state of the array `fr` (state_elems each); its moments go
+// to su + 4 nlist b.
 template <int S>
-__global__ void k_support_moments(const typename Store<S>::T *__restrict__ f, size_t stride,
-                                  const int32_t *__restrict__ list, int nlist,
-                                  double *__restrict__ su) {
+__global__ void k_support_moments(const typename Store<S>::T *__restrict__ f0,
+                                  const typename Store<S>::T *__restrict__ fr, size_t state_elems,
+                                  int r0, size_t stride, const int32_t *__restrict__ list,
+                                  int nlist, double *__restrict__ su) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nlist) return;
+  const int rs = r0 + (int)blockIdx.y;
+  const typename Store<S>::T *f = rs == 0 ? f0 : fr + (size_t)(rs - 1) * state_elems;
+  su += 4 * (size_t)nlist * blockIdx.y;
   double drho, jv[3];
   node_moments<S>(f, stride, list[i], drho, jv);
   const double r = 1.0 + drho;
@@ -382,9 +389,13 @@ __global__ __launch_bounds__(256) void k_probe_tau(int n, const int32_t *__restr
                                                    const double *__restrict__ wgt,
                                                    const double *__restrict__ su,
                                                    const double *__restrict__ pts, double mu_over_dx,
-                                                   double scale, int grad, double *__restrict__ out) {
+                                                   double scale, int grad, int nsup,
+                                                   double *__restrict__ out) {
   const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (m >= n) return;  // whole waves leave together
+  // batch b = blockIdx.y: support moments of state b, outputs of state b
+  su += 4 * (size_t)nsup * blockIdx.y;
+  out += (grad ? 3 : 1) * (size_t)n * blockIdx.y;
   double u0 = 0, u1 = 0, u2 = 0;
   for (int e = off[m] + lane; e < off[m + 1]; e += 64) {
     const double w = wgt[e];
@@ -418,10 +429,12 @@ __global__ __launch_bounds__(256) void k_probe_tau(int n, const int32_t *__restr
 __global__ __launch_bounds__(256) void k_obj_gather(int nlist, const int32_t *__restrict__ off,
                                                     const int32_t *__restrict__ probe,
                                                     const double *__restrict__ weight,
-                                                    const double *__restrict__ a,
+                                                    const double *__restrict__ a, int npts,
                                                     double *__restrict__ sg) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = t >> 3, sub = t & 7;
+  a += 3 * (size_t)npts * blockIdx.y;  // batch b = blockIdx.y
+  sg += 3 * (size_t)nlist * blockIdx.y;
   double d0 = 0, d1 = 0, d2 = 0;
   if (i < nlist)
     for (int e = off[i] + sub; e < off[i + 1]; e += 8) {
@@ -494,13 +507,15 @@ int require_adj(zf_ctx *ctx) {
   return ZF_OK;
 }
 
-int support_moments(zf_ctx *ctx, const void *f) {
+// Support moments of nb states: r0 + b == 0 is f0, else the (r0 + b - 1)-th
+// state of fr; batch slot b of d_sup_u.
+int support_moments(zf_ctx *ctx, const void *f0, const void *fr, int r0, int nbatch) {
   if (ctx->n_support == 0) return ZF_OK;
   with_store(ctx->cfg.storage, [&](auto tag) {
     constexpr int S = decltype(tag)::value;
-    hipLaunchKernelGGL(k_support_moments<S>, dim3(nb(ctx->n_support, 256)), dim3(256), 0,
-                       ctx->stream, st<S>(f), ctx->stride, ctx->d_support, ctx->n_support,
-                       ctx->d_sup_u);
+    hipLaunchKernelGGL(k_support_moments<S>, dim3(nb(ctx->n_support, 256), nbatch), dim3(256), 0,
+                       ctx->stream, st<S>(f0), st<S>(fr), (size_t)Q * ctx->stride, r0, ctx->stride,
+                       ctx->d_support, ctx->n_support, ctx->d_sup_u);
     return 0;
   });
   return ZF_OK;
@@ -510,33 +525,44 @@ int support_moments(zf_ctx *ctx, const void *f) {
 int obj_forward(zf_ctx *ctx, const void *f, int it) {
   const zf_adjoint_config &a = ctx->acfg;
   double *out = ctx->d_objbuf + (size_t)(it - a.t_begin) * ctx->adj_npts;
-  support_moments(ctx, f);
+  support_moments(ctx, f, f, 0, 1);
   hipLaunchKernelGGL(k_probe_tau, dim3(nb(ctx->adj_npts, 4)), dim3(256), 0, ctx->stream,
                      ctx->adj_npts, ctx->d_prb_off, ctx->d_prb_sup, ctx->d_prb_w, ctx->d_sup_u,
-                     ctx->d_adj_pts, a.mu_over_dx, 1.0, 0, out);
+                     ctx->d_adj_pts, a.mu_over_dx, 1.0, 0, ctx->n_support, out);
   return ZF_OK;
 }
 
-// lambda += dJ/dS for state f (an iteration of the window).
-int obj_adjoint(zf_ctx *ctx, const void *f, void *lam) {
+// dJ/du at the support nodes for nb states (see support_moments), in batch
+// slots 0..nb-1 of d_sup_g (with their moments in d_sup_u).  The three
+// passes read only the states, so the states of a checkpoint segment are
+// processed in a few batched launches instead of three launches each.
+int obj_prepare(zf_ctx *ctx, const void *f0, const void *fr, int r0, int nbatch) {
   const zf_adjoint_config &a = ctx->acfg;
+  if (nbatch < 1 || nbatch > ctx->obj_batch) return fail(ctx, ZF_ERR_ARG, "objective batch");
   const double scale = 1.0 / ((double)(a.t_end - a.t_begin + 1) * ctx->adj_npts);
-  support_moments(ctx, f);
-  hipLaunchKernelGGL(k_probe_tau, dim3(nb(ctx->adj_npts, 4)), dim3(256), 0, ctx->stream,
+  support_moments(ctx, f0, fr, r0, nbatch);
+  hipLaunchKernelGGL(k_probe_tau, dim3(nb(ctx->adj_npts, 4), nbatch), dim3(256), 0, ctx->stream,
                      ctx->adj_npts, ctx->d_prb_off, ctx->d_prb_sup, ctx->d_prb_w, ctx->d_sup_u,
-                     ctx->d_adj_pts, a.mu_over_dx, scale, 1, ctx->d_adj_a);
+                     ctx->d_adj_pts, a.mu_over_dx, scale, 1, ctx->n_support, ctx->d_adj_a);
   if (ctx->n_support > 0)
-    with_store(ctx->cfg.storage, [&](auto tag) {
-      constexpr int S = decltype(tag)::value;
-      using A = typename Adj<S>::A;
-      hipLaunchKernelGGL(k_obj_gather, dim3(nb(8 * (size_t)ctx->n_support, 256)), dim3(256), 0, ctx->stream,
-                         ctx->n_support, ctx->d_sup_off, ctx->d_sup_probe, ctx->d_sup_w,
-                         ctx->d_adj_a, ctx->d_sup_g);
-      hipLaunchKernelGGL(k_obj_apply<S>, dim3(nb((size_t)Q * ctx->n_support, 256)), dim3(256), 0,
-                         ctx->stream, ctx->d_support, ctx->n_support, ctx->stride, ctx->d_sup_g,
-                         ctx->d_sup_u, static_cast<A *>(lam));
-      return 0;
-    });
+    hipLaunchKernelGGL(k_obj_gather, dim3(nb(8 * (size_t)ctx->n_support, 256), nbatch), dim3(256), 0,
+                       ctx->stream, ctx->n_support, ctx->d_sup_off, ctx->d_sup_probe, ctx->d_sup_w,
+                       ctx->d_adj_a, ctx->adj_npts, ctx->d_sup_g);
+  return ZF_OK;
+}
+
+// lambda += dJ/dS for the state prepared in batch slot `slot`.
+int obj_apply(zf_ctx *ctx, int slot, void *lam) {
+  if (ctx->n_support == 0) return ZF_OK;
+  const size_t ns = ctx->n_support;
+  with_store(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using A = typename Adj<S>::A;
+    hipLaunchKernelGGL(k_obj_apply<S>, dim3(nb((size_t)Q * ns, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_support, ctx->n_support, ctx->stride, ctx->d_sup_g + 3 * ns * slot,
+                       ctx->d_sup_u + 4 * ns * slot, static_cast<A *>(lam));
+    return 0;
+  });
   return ZF_OK;
 }
 
@@ -630,6 +656,8 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
   free_adjoint(ctx);
   ctx->acfg = *ac;
   ctx->adj_npts = (int32_t)npts;
+  // objective batch: the states of up to 64 iterations of a segment
+  ctx->obj_batch = std::min(64, ac->checkpoint_interval);
   const size_t stride = ctx->stride;
   const size_t eb = store_bytes(ctx->cfg.storage);
   const size_t ab = ctx->cfg.storage == ZF_STORE_F64 ? 8 : 4;
@@ -643,7 +671,7 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
       (rc = dev_alloc(ctx, &ctx->d_seg, (size_t)std::max(K - 1, 1) * ctx->adj_state_bytes)) ||
       (rc = dev_alloc(ctx, &ctx->d_lam[0], ab * Q * stride)) ||
       (rc = dev_alloc(ctx, &ctx->d_lam[1], ab * Q * stride)) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_adj_a, 24 * (size_t)npts)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_adj_a, 24 * (size_t)npts * ctx->obj_batch)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_gpart, 40 * ((size_t)nb(ctx->n_int, 256) + 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_grads, 8 * 8)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_objbuf, 8 * nobj))) {
@@ -758,8 +786,8 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
       (rc = dev_alloc(ctx, (void **)&ctx->d_sup_off, 4 * off.size())) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_sup_probe, 4 * std::max<size_t>(sprobe.size(), 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_sup_w, 8 * std::max<size_t>(sw.size(), 1))) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_u, 32 * std::max<size_t>(support.size(), 1))) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_g, 24 * std::max<size_t>(support.size(), 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_u, 32 * std::max<size_t>(support.size(), 1) * ctx->obj_batch)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sup_g, 24 * std::max<size_t>(support.size(), 1) * ctx->obj_batch)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_prb_off, 4 * poff.size())) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_prb_sup, 4 * std::max<size_t>(psup.size(), 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_prb_w, 8 * std::max<size_t>(pw.size(), 1)))) {
@@ -893,7 +921,10 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
   (void)ab;
   int cur = 0;
   // lambda_N = dJ/dS_N
-  if (iter0 + N >= a.t_begin && iter0 + N <= a.t_end) obj_adjoint(ctx, ctx->d_f[ctx->cur], ctx->d_lam[cur]);
+  if (iter0 + N >= a.t_begin && iter0 + N <= a.t_end) {
+    if ((rc = obj_prepare(ctx, ctx->d_f[ctx->cur], ctx->d_f[ctx->cur], 0, 1))) return rc;
+    obj_apply(ctx, 0, ctx->d_lam[cur]);
+  }
   const int nseg = (N + K - 1) / K;
   for (int j = nseg - 1; j >= 0; j--) {
     // segment j: iterations iter0 + jK + 1 .. iter0 + min((j+1)K, N), which
@@ -910,6 +941,10 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
         return rc;
       }
     }
+    // objective data of the segment's states in the window, prepared in
+    // batches of up to obj_batch states as the sweep reaches them
+    int blo = 0, bhi = -1;
+    const int rwin = std::max(0, a.t_begin - (iter0 + j * K));  // first state index in the window
     for (int r = len; r >= 1; r--) {
       const int k = iter0 + j * K + r;  // iteration k reads S_{k-1} = state(r - 1)
       ctx->d_force = field_of(k);
@@ -917,7 +952,15 @@ int zf_adjoint_backward(zf_ctx *ctx, double grad[5]) {
       ctx->d_force = field0;
       if (rc) return rc;
       cur ^= 1;
-      if (k - 1 >= a.t_begin && k - 1 <= a.t_end) obj_adjoint(ctx, state(r - 1), ctx->d_lam[cur]);
+      if (k - 1 >= a.t_begin && k - 1 <= a.t_end) {
+        const int rs = r - 1;
+        if (rs < blo || rs > bhi) {
+          bhi = rs;
+          blo = std::max(rwin, rs - ctx->obj_batch + 1);
+          if ((rc = obj_prepare(ctx, ck, ctx->d_seg, blo, bhi - blo + 1))) return rc;
+        }
+        obj_apply(ctx, rs - blo, ctx->d_lam[cur]);
+      }
       // RBC iteration k - 1: its cell stages read S_{k-1}; the field they
       // spread was read by iterations k .. (its gradient is complete now)
       if (cells && k - 1 > iter0 && (k - 1) % I == 0 &&

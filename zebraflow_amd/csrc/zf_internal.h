@@ -170,19 +170,20 @@ struct zf_ctx {
   bool have_adj = false, adj_ready = false;  // set up / forward recorded
   zf_adjoint_config acfg{};
   int32_t adj_npts = 0, adj_iter0 = 0, adj_nsteps = 0;
+  int32_t obj_batch = 1;           // states per batch of the objective's adjoint passes
   size_t adj_state_bytes = 0;
   double *d_adj_pts = nullptr;     // 6 x npts: xyz then unit normals
   void *d_ckpt = nullptr;          // checkpoints, one state every K iterations
   void *d_seg = nullptr;           // K - 1 recomputed states of one segment
   void *d_lam[2] = {nullptr, nullptr};  // adjoint populations (ping-pong)
   int adj_cur = 0;
-  double *d_adj_a = nullptr;       // 3 x npts: -(mu/DX) dJ/dtau of each probe
+  double *d_adj_a = nullptr;       // obj_batch x 3 x npts: -(mu/DX) dJ/dtau of each probe
   int32_t *d_support = nullptr;    // storage indices in any probe stencil
   int32_t *d_sup_off = nullptr;    // n_support + 1: CSR over (probe, weight) pairs
   int32_t *d_sup_probe = nullptr;
   double *d_sup_w = nullptr;
-  double *d_sup_u = nullptr;       // 4 x n_support: u and 1 + drho of the support nodes
-  double *d_sup_g = nullptr;       // 3 x n_support: dJ/du of the support nodes
+  double *d_sup_u = nullptr;       // obj_batch x 4 x n_support: u and 1 + drho of the support nodes
+  double *d_sup_g = nullptr;       // obj_batch x 3 x n_support: dJ/du of the support nodes
   int32_t *d_prb_off = nullptr;    // npts + 1: CSR over (support index, weight) per probe
   int32_t *d_prb_sup = nullptr;
   double *d_prb_w = nullptr;
