@@ -284,8 +284,8 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
                                  const int32_t *__restrict__ status, const int32_t *__restrict__ windex,
                                  const float *__restrict__ wdist,
                                  const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
-                                 const double *__restrict__ wxyz, const double *__restrict__ wtgeo,
-                                 const double *__restrict__ wnormal,
+                                 const double *__restrict__ wxyz, const double *__restrict__ wrad,
+                                 const double *__restrict__ wtgeo, const double *__restrict__ wnormal,
                                  Grid g, RepParams r, double *__restrict__ vrep,
                                  int32_t *__restrict__ nearest) {
   const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -335,7 +335,13 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
         // wall_lmax, none of them can beat mindis <= wall_search
         const double *wp = &wxyz[3 * (size_t)(wn - 1)];
         const double ex = x - wp[0], ey = y - wp[1], ez = z - wp[2];
-        if (ex * ex + ey * ey + ez * ez >= r.wall_reach2) continue;
+        const double e2 = ex * ex + ey * ey + ez * ez;
+        if (e2 >= r.wall_reach2) continue;
+        // every vertex and centroid of wn's triangles lies within wrad of
+        // wn (with slack over rounding): none can beat the current mindis
+        // if wn is farther than mindis + wrad (branch and bound; the result
+        // is unchanged)
+        if (sqrt(e2) - wrad[wn - 1] >= mind) continue;
         const int32_t *trow = &wnode_tri[13 * (size_t)(wn - 1)];
         const int cnt = min(trow[0], 12);
 #pragma unroll 1
@@ -439,7 +445,7 @@ int require_rep(zf_ctx *ctx) {
 void free_wall(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
                    (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist,
-                   (void **)&ctx->d_wtgeo};
+                   (void **)&ctx->d_wtgeo, (void **)&ctx->d_wrad};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_wall = false;
 }
@@ -495,7 +501,8 @@ int repulsion_impl(zf_ctx *ctx) {
     hipLaunchKernelGGL(k_wall_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                        ctx->nn, ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wdist,
                        ctx->d_wnode_tri,
-                       ctx->d_wtri, ctx->d_wxyz, ctx->d_wtgeo, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep,
+                       ctx->d_wtri, ctx->d_wxyz, ctx->d_wrad, ctx->d_wtgeo, ctx->d_wnormal, grid_of(ctx), r,
+                       ctx->d_vrep,
                        ctx->d_near);
   hipLaunchKernelGGL(k_cell_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
@@ -564,8 +571,11 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
       if (row[0] <= 12) row[row[0]] = m;
     }
   // largest distance from a wall triangle's vertex to its other vertices
-  // and its centroid (the reach of a wall node's triangles)
+  // and its centroid (the reach of a wall node's triangles), overall and
+  // per wall node (the latter with a relative and absolute slack that
+  // covers the rounding of the device's centroid and distances)
   double lmax = 0;
+  std::vector<double> wrad(nwn, 0.0);
   for (int m = 0; m < nwt; m++) {
     const double *v[3];
     for (int s = 0; s < 3; s++) v[s] = &wall_xyz[3 * (size_t)(wall_tri[3 * (size_t)m + s] - 1)];
@@ -573,9 +583,12 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
     for (int a = 0; a < 3; a++) cen[a] = (v[0][a] + v[1][a] + v[2][a]) / 3.0;
     for (int s = 0; s < 3; s++) {
       const double *pts[3] = {v[(s + 1) % 3], v[(s + 2) % 3], cen};
+      double &rv = wrad[wall_tri[3 * (size_t)m + s] - 1];
       for (const double *p : pts) {
         const double dx = v[s][0] - p[0], dy = v[s][1] - p[1], dz = v[s][2] - p[2];
-        lmax = std::max(lmax, std::sqrt(dx * dx + dy * dy + dz * dz));
+        const double dd = std::sqrt(dx * dx + dy * dy + dz * dz);
+        lmax = std::max(lmax, dd);
+        rv = std::max(rv, dd);
       }
     }
   }
@@ -584,6 +597,7 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnormal, 24 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wtri, 12 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wtgeo, 96 * (size_t)nwt)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wrad, 8 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnode_tri, 52 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_windex, 4 * ctx->box)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wdist, 4 * ctx->box)) ||
@@ -593,6 +607,8 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   ZF_HIP(ctx, hipMemcpy(ctx->d_wnormal, wall_normal, 24 * (size_t)nwt, hipMemcpyHostToDevice));
   ZF_HIP(ctx, hipMemcpy(ctx->d_wtri, wall_tri, 12 * (size_t)nwt, hipMemcpyHostToDevice));
   ZF_HIP(ctx, hipMemcpy(ctx->d_wnode_tri, rows.data(), 52 * (size_t)nwn, hipMemcpyHostToDevice));
+  for (double &rv : wrad) rv = rv * (1.0 + 1e-12) + 1e-12;
+  ZF_HIP(ctx, hipMemcpy(ctx->d_wrad, wrad.data(), 8 * (size_t)nwn, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_wall_geom, dim3(nb(nwt, 256)), dim3(256), 0, ctx->stream, ctx->d_wxyz,
                      ctx->d_wtri, nwt, ctx->d_wtgeo);
   hipLaunchKernelGGL(k_fill_u64, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key, ctx->box);

@@ -163,6 +163,7 @@ struct zf_ctx {
   double *d_wxyz = nullptr, *d_wnormal = nullptr;  // 3 x nwn, 3 x nwt (AoS)
   int32_t *d_wtri = nullptr, *d_wnode_tri = nullptr;  // 3 x nwt (1-based), 13 x nwn
   double *d_wtgeo = nullptr;  // 12 x nwt: vertices and centroid of each wall triangle
+  double *d_wrad = nullptr;   // nwn: largest distance from a wall node to its triangles' vertices/centroids
   int32_t *d_windex = nullptr;    // box
   float *d_wdist = nullptr;       // box: FP32 distance to that wall node (or +inf)
 
