@@ -121,59 +121,73 @@ __device__ __forceinline__ void node_u(const typename Store<S>::T *__restrict__ 
 // One lane per membrane node: lam_G and the position adjoint (velocities
 // of the support nodes from uq).
 template <int S>
-__global__ void k_ibm_adj_node(const double *__restrict__ X, size_t NT, int nn, int ncell,
-                               const int32_t *__restrict__ status, const int32_t *__restrict__ b,
-                               const int32_t *__restrict__ sbox, const W<S> *__restrict__ uq,
-                               size_t stride, AP p,
-                               const double *__restrict__ G, const double *__restrict__ lamV,
-                               const double *__restrict__ lamF, double *__restrict__ lamG,
-                               double *__restrict__ lamX) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ncell * nn) return;
-  if (status[g / nn] < 0) return;
-  const double xp = X[g], yp = X[NT + g], zp = X[2 * NT + g];
-  const double G0 = G ? G[g] : 0, G1 = G ? G[NT + g] : 0, G2 = G ? G[2 * NT + g] : 0;
-  const double V0 = lamV ? lamV[g] : 0, V1 = lamV ? lamV[NT + g] : 0, V2 = lamV ? lamV[2 * NT + g] : 0;
-  const int i = (int)((xp - p.x0) / p.dx + 1.01), j = (int)((yp - p.y0) / p.dx + 1.01),
-            k = (int)((zp - p.z0) / p.dx + 1.01);
-  const int i1 = max(1, i - 3), i2 = min(p.im, i + 3), j1 = max(1, j - 3), j2 = min(p.jm, j + 3),
-            k1 = max(1, k - 3), k2 = min(p.km, k + 3);
-  const double inv = 1.0 / p.dx;
+__global__ __launch_bounds__(128) void k_ibm_adj_node(
+    const double *__restrict__ X, size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
+    const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, const W<S> *__restrict__ uq,
+    size_t stride, AP p, const double *__restrict__ G, const double *__restrict__ lamV,
+    const double *__restrict__ lamF, double *__restrict__ lamG, double *__restrict__ lamX) {
+  // 8 lanes per membrane node: lane `sub` takes the window plane i - 3 + sub
+  // (sub < 7), and a fixed-order shuffle reduction over the 8 lanes sums the
+  // planes (deterministic); 8x the lanes of one lane per node, which left
+  // most of the device idle for a few tens of cells
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = t >> 3, sub = t & 7;
+  const bool live = g < ncell * nn && status[min(g, ncell * nn - 1) / nn] >= 0;
   double lg0 = 0, lg1 = 0, lg2 = 0, lx0 = 0, lx1 = 0, lx2 = 0;
-  for (int ii = i1; ii <= i2; ii++) {
-    const double dxl = (p.x0 + (ii - 1) * p.dx - xp) * inv;
-    for (int jj = j1; jj <= j2; jj++) {
-      const double dyl = (p.y0 + (jj - 1) * p.dx - yp) * inv;
-      for (int kk = k1; kk <= k2; kk++) {
-        const double dzl = (p.z0 + (kk - 1) * p.dx - zp) * inv;
-        if (off_support(dxl, dyl, dzl, p.cube)) continue;
-        const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
-        const int bv = b[q];
-        if (bv < 0) continue;
-        const int s = sbox[q];
-        W<S> gr[3];
-        const double w = (double)weight_grad((W<S>)dxl, (W<S>)dyl, (W<S>)dzl, gr);
-        const double gw[3] = {(double)gr[0] * inv, (double)gr[1] * inv, (double)gr[2] * inv};
-        if (lamF) {  // spreading onto every B >= 0 node
-          const double l0 = lamF[s], l1 = lamF[stride + s], l2 = lamF[2 * stride + s];
-          lg0 += p.fs * w * l0;
-          lg1 += p.fs * w * l1;
-          lg2 += p.fs * w * l2;
-          const double lg = p.fs * (l0 * G0 + l1 * G1 + l2 * G2);
-          lx0 += lg * gw[0];
-          lx1 += lg * gw[1];
-          lx2 += lg * gw[2];
-        }
-        if (lamV && bv > 0 && bv < 4998) {  // interpolation from fluid nodes
-          const double lv = p.vs * (V0 * (double)uq[s] + V1 * (double)uq[stride + s] +
-                                    V2 * (double)uq[2 * stride + s]);
-          lx0 += lv * gw[0];
-          lx1 += lv * gw[1];
-          lx2 += lv * gw[2];
+  if (live) {
+    const double xp = X[g], yp = X[NT + g], zp = X[2 * NT + g];
+    const double G0 = G ? G[g] : 0, G1 = G ? G[NT + g] : 0, G2 = G ? G[2 * NT + g] : 0;
+    const double V0 = lamV ? lamV[g] : 0, V1 = lamV ? lamV[NT + g] : 0, V2 = lamV ? lamV[2 * NT + g] : 0;
+    const int i = (int)((xp - p.x0) / p.dx + 1.01), j = (int)((yp - p.y0) / p.dx + 1.01),
+              k = (int)((zp - p.z0) / p.dx + 1.01);
+    const int j1 = max(1, j - 3), j2 = min(p.jm, j + 3), k1 = max(1, k - 3), k2 = min(p.km, k + 3);
+    const double inv = 1.0 / p.dx;
+    const int ii = i - 3 + sub;
+    if (sub < 7 && ii >= 1 && ii <= p.im) {
+      const double dxl = (p.x0 + (ii - 1) * p.dx - xp) * inv;
+      for (int jj = j1; jj <= j2; jj++) {
+        const double dyl = (p.y0 + (jj - 1) * p.dx - yp) * inv;
+        for (int kk = k1; kk <= k2; kk++) {
+          const double dzl = (p.z0 + (kk - 1) * p.dx - zp) * inv;
+          if (off_support(dxl, dyl, dzl, p.cube)) continue;
+          const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
+          const int bv = b[q];
+          if (bv < 0) continue;
+          const int s = sbox[q];
+          W<S> gr[3];
+          const double w = (double)weight_grad((W<S>)dxl, (W<S>)dyl, (W<S>)dzl, gr);
+          const double gw[3] = {(double)gr[0] * inv, (double)gr[1] * inv, (double)gr[2] * inv};
+          if (lamF) {  // spreading onto every B >= 0 node
+            const double l0 = lamF[s], l1 = lamF[stride + s], l2 = lamF[2 * stride + s];
+            lg0 += p.fs * w * l0;
+            lg1 += p.fs * w * l1;
+            lg2 += p.fs * w * l2;
+            const double lg = p.fs * (l0 * G0 + l1 * G1 + l2 * G2);
+            lx0 += lg * gw[0];
+            lx1 += lg * gw[1];
+            lx2 += lg * gw[2];
+          }
+          if (lamV && bv > 0 && bv < 4998) {  // interpolation from fluid nodes
+            const double lv = p.vs * (V0 * (double)uq[s] + V1 * (double)uq[stride + s] +
+                                      V2 * (double)uq[2 * stride + s]);
+            lx0 += lv * gw[0];
+            lx1 += lv * gw[1];
+            lx2 += lv * gw[2];
+          }
         }
       }
     }
   }
+  // every lane of the 8 takes part (the group never straddles a wave)
+  for (int o = 4; o > 0; o >>= 1) {
+    lg0 += __shfl_xor(lg0, o, 8);
+    lg1 += __shfl_xor(lg1, o, 8);
+    lg2 += __shfl_xor(lg2, o, 8);
+    lx0 += __shfl_xor(lx0, o, 8);
+    lx1 += __shfl_xor(lx1, o, 8);
+    lx2 += __shfl_xor(lx2, o, 8);
+  }
+  if (!live || sub != 0) return;
   if (lamG) {
     lamG[g] = lg0;
     lamG[NT + g] = lg1;
@@ -376,7 +390,7 @@ int ibm_vjp_device(zf_ctx *ctx, const double *X, const void *f, const double *G,
     if (lamV)
       hipLaunchKernelGGL(k_adj_velocity<S>, dim3(nb(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
                          static_cast<const T *>(f), ctx->stride, ctx->dommax, uq);
-    hipLaunchKernelGGL(k_ibm_adj_node<S>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, X,
+    hipLaunchKernelGGL(k_ibm_adj_node<S>, dim3(nb(8 * (size_t)n, 128)), dim3(128), 0, ctx->stream, X,
                        NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, uq,
                        ctx->stride, p, G, lamV, lamF, lamG, lamX);
     return 0;
