@@ -209,7 +209,6 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           // tile along every axis: any other one is off the support of all
           // 64 lanes, so the sums are unchanged
           bool rel = false;
-          int slot = 0;
           F px = 0, py = 0, pz = 0;
           int g = 0;
           if (lane < end - c0) {
@@ -221,21 +220,20 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           }
           const unsigned long long relm = __ballot(rel);
           const int cnt = __popcll(relm);
-          slot = __popcll(relm & ((1ull << lane) - 1ull));
           if (rel) {
-            const int lane_ = slot;
-            sx[lane_] = px;
-            sy[lane_] = py;
-            sz[lane_] = pz;
-            sf0[lane_] = ffl[g];
-            sf1[lane_] = ffl[NT + g];
-            sf2[lane_] = ffl[2 * NT + g];
+            const int sl = __popcll(relm & ((1ull << lane) - 1ull));
+            sx[sl] = px;
+            sy[sl] = py;
+            sz[sl] = pz;
+            sf0[sl] = ffl[g];
+            sf1[sl] = ffl[NT + g];
+            sf2[sl] = ffl[2 * NT + g];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
-              kx[r][lane_] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
-              ky[r][lane_] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
-              kz[r][lane_] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
+              kx[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
+              ky[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
+              kz[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
             }
           }
           __syncthreads();
