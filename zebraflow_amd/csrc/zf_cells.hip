@@ -164,6 +164,9 @@ constexpr double kReachSlack = 1e-9;
 // agree to rounding.  The weights are packed 4 bits per centroid (s in bits
 // 4s..4s+3), which lets a lane form one centroid at a time from scalar
 // constants instead of holding all 16.
+// entries of a lane's queue of passing triangles in the repulsion kernel
+constexpr int kRepQueue = 32;
+
 constexpr uint64_t kSubWA = 0x112412457124578aull, kSubWB = 0x142175421a875421ull,
                    kSubWC = 0xa787454541212121ull;
 
@@ -210,6 +213,42 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
   }
   const double treach2 = treach * treach, breach2 = breach * breach;
   const int2 *range2 = reinterpret_cast<const int2 *>(range);
+  // Triangles that pass the three filters go to the lane's queue in LDS,
+  // and the 16-point sub-triangle evaluation runs when any active lane's
+  // queue is nearly full (and once at the end), every active lane draining
+  // its own queue in order: the lanes then evaluate their candidates
+  // together instead of one or two at a time, and each lane's sum keeps its
+  // candidate order.
+  __shared__ int queue[kRepQueue][128];
+  int qlen = 0;
+  auto drain = [&]() {
+    for (int k = 0; k < qlen; k++) {
+      const int q = queue[k][threadIdx.x];
+      const int m = q / nt, t = q % nt;
+      const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
+      const size_t b0 = (size_t)m * nn;
+      const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = X[2 * NT + b0 + n1];
+      const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = X[2 * NT + b0 + n2];
+      const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = X[2 * NT + b0 + n3];
+#pragma unroll 2
+      for (int sc = 0; sc < 16; sc++) {
+        const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
+        const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
+        const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
+        const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
+        const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
+        const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
+        const double d = sqrt(px * px + py * py + pz * pz);
+        if (d < r.cutoff) {
+          const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
+          v0 += f * px / d;
+          v1 += f * py / d;
+          v2 += f * pz / d;
+        }
+      }
+    }
+    qlen = 0;
+  };
   for (int i = lo[0]; i <= hi[0]; i++)
     for (int j = lo[1]; j <= hi[1]; j++) {
       const double qx = g.x0 + (i - 1) * g.dx - xc, qy = g.y0 + (j - 1) * g.dx - yc;
@@ -231,6 +270,8 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
         end = use ? max(end, rg.y) : end;
       }
       for (int e0 = beg; e0 < end; e0 += 4) {
+        // room for this batch's 4 candidates in every active lane's queue
+        if (__ballot(qlen > kRepQueue - 4)) drain();
         int qv[4];
         double cxv[4], cyv[4], czv[4];
 #pragma unroll
@@ -251,29 +292,12 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
           const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
           if (m == c && (n1 == node || n2 == node || n3 == node)) continue;
           if (e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0) continue;
-          const size_t b0 = (size_t)m * nn;
-          const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = X[2 * NT + b0 + n1];
-          const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = X[2 * NT + b0 + n2];
-          const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = X[2 * NT + b0 + n3];
-#pragma unroll 2
-          for (int sc = 0; sc < 16; sc++) {
-            const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
-            const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
-            const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
-            const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
-            const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
-            const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
-            const double d = sqrt(px * px + py * py + pz * pz);
-            if (d < r.cutoff) {
-              const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
-              v0 += f * px / d;
-              v1 += f * py / d;
-              v2 += f * pz / d;
-            }
-          }
+          queue[qlen][threadIdx.x] = q;
+          qlen++;
         }
       }
     }
+  drain();
   vrep[gq] += v0;
   vrep[NT + gq] += v1;
   vrep[2 * NT + gq] += v2;
