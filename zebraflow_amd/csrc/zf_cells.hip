@@ -166,6 +166,8 @@ constexpr double kReachSlack = 1e-9;
 // constants instead of holding all 16.
 // entries of a lane's queue of passing triangles in the repulsion kernel
 constexpr int kRepQueue = 32;
+// entries of a lane's queue of wall nodes in the wall-repulsion kernel
+constexpr int kWallQueue = 12;
 
 constexpr uint64_t kSubWA = 0x112412457124578aull, kSubWB = 0x142175421a875421ull,
                    kSubWC = 0xa787454541212121ull;
@@ -339,8 +341,45 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
   window(g, x, y, z, lo, hi);
   double mind = r.wall_search;
   int triwall = 0, last = 0;
+  // Wall nodes that pass the two distance tests go to the lane's queue in
+  // LDS with their lower bound (rounded down to float), and their triangles
+  // are examined when any active lane's queue could overflow in the next
+  // row (and once at the end), every active lane draining its own queue in
+  // order.  The bound is tested again at the drain against the mindis of
+  // that moment, which is the mindis an in-order walk would hold there; a
+  // node queued under an older, larger mindis and examined anyway cannot
+  // lower it.  Either way the same triangle wins, strict < included.
+  __shared__ int wq[kWallQueue][128];
+  __shared__ float wlb[kWallQueue][128];
+  int qlen = 0;
+  auto drain = [&]() {
+    for (int k = 0; k < qlen; k++) {
+      if ((double)wlb[k][threadIdx.x] >= mind) continue;
+      const int wn = wq[k][threadIdx.x];
+      const int32_t *trow = &wnode_tri[13 * (size_t)(wn - 1)];
+      const int cnt = min(trow[0], 12);
+#pragma unroll 1
+      for (int l = 1; l <= cnt; l++) {
+        const int tw = trow[l];
+        // vertices and centroid of wall triangle tw, one contiguous record
+        const double *tgp = &wtgeo[12 * (size_t)(tw - 1)];
+        double gv[12];
+#pragma unroll
+        for (int u = 0; u < 12; u++) gv[u] = tgp[u];
+#pragma unroll
+        for (int sv = 0; sv < 4; sv++) {  // vertices 0..2, then the centroid
+          const double wx = gv[3 * sv], wy = gv[3 * sv + 1], wz = gv[3 * sv + 2];
+          const double d = sqrt((x - wx) * (x - wx) + (y - wy) * (y - wy) + (z - wz) * (z - wz));
+          if (d < mind) { mind = d; triwall = tw; }
+        }
+      }
+    }
+    qlen = 0;
+  };
   for (int i = lo[0]; i <= hi[0]; i++)
     for (int j = lo[1]; j <= hi[1]; j++) {
+      // room for this row's up to 9 wall nodes in every active lane's queue
+      if (__ballot(qlen > kWallQueue - 9)) drain();
       // the row's 9 W_index entries in one batch of loads (in-bounds
       // addresses; entries past hi[2] are not used)
       const size_t row = bq(g, i, j, 1);
@@ -365,26 +404,16 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
         // wn (with slack over rounding): none can beat the current mindis
         // if wn is farther than mindis + wrad (branch and bound; the result
         // is unchanged)
-        if (sqrt(e2) - wrad[wn - 1] >= mind) continue;
-        const int32_t *trow = &wnode_tri[13 * (size_t)(wn - 1)];
-        const int cnt = min(trow[0], 12);
-#pragma unroll 1
-        for (int l = 1; l <= cnt; l++) {
-          const int tw = trow[l];
-          // vertices and centroid of wall triangle tw, one contiguous record
-          const double *tgp = &wtgeo[12 * (size_t)(tw - 1)];
-          double gv[12];
-#pragma unroll
-          for (int u = 0; u < 12; u++) gv[u] = tgp[u];
-#pragma unroll
-          for (int sv = 0; sv < 4; sv++) {  // vertices 0..2, then the centroid
-            const double wx = gv[3 * sv], wy = gv[3 * sv + 1], wz = gv[3 * sv + 2];
-            const double d = sqrt((x - wx) * (x - wx) + (y - wy) * (y - wy) + (z - wz) * (z - wz));
-            if (d < mind) { mind = d; triwall = tw; }
-          }
-        }
+        const double lb = sqrt(e2) - wrad[wn - 1];
+        if (lb >= mind) continue;
+        float lbf = (float)lb;
+        if ((double)lbf > lb) lbf = nextafterf(lbf, -INFINITY);  // never above lb
+        wq[qlen][threadIdx.x] = wn;
+        wlb[qlen][threadIdx.x] = lbf;
+        qlen++;
       }
     }
+  drain();
   if (triwall == 0) return;
   nearest[gq] = triwall;
   const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
