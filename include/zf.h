@@ -154,6 +154,30 @@ int zf_conservation_sums(zf_ctx *ctx, double out[4]);
  * boundary update runs inside that kernel), out_ms[2] = launches timed. */
 int zf_lbm_time_kernels(zf_ctx *ctx, int32_t iter0, int32_t nsteps, double out_ms[3]);
 
+/* ---- parent (feeder) lattices ----------------------------------------------
+ * The reference's three parent lattices (ZebraflowWT.c:91-112; lists
+ * LBM_PARENTDOMAIN_SETUP :515-1013 and LBM_PARENTBC_SETUP :1015-1350; step
+ * :6148-6286; boundary pass :6436-6594) are contexts of their own.
+ * zf_set_parent, before zf_upload_lattice, makes ctx parent p (1..3; 0
+ * undoes it): its BC list takes B == 0 or B >= 10, wall and upstream
+ * (20+p) nodes take the parent thresholds for their neighbour, and the
+ * copy code 10+p and the upstream velocity source name the main-domain node
+ * (i + offset[0], j + offset[1], k_over + offset[2]) over the node's column
+ * at the overlap plane k_over (the reference's (int)((X - X0)/DX + 1.01)
+ * with its lattice origins; parent 1: k_over = KM_1). */
+int zf_set_parent(zf_ctx *ctx, int32_t p, int32_t k_over, const int32_t offset[3]);
+/* vBCNLIST of a parent lattice, bcmax*3 (zero except for upstream nodes). */
+int zf_download_parent_lists(zf_ctx *ctx, int32_t *vbcn);
+/* Couples parent lattice `parent` to `main` (both uploaded, same device):
+ * every zf_lbm_step of main then exports the moments of the nodes the
+ * parent reads, and every zf_lbm_step of the parent ends with its boundary
+ * pass (:6436-6594) for codes 10+p and 20+p; walls stay with its stream
+ * kernel.  Step main first, then its parents, in each iteration; the two
+ * streams are ordered with events.  Fails with ZF_ERR_GEOMETRY if a coupled
+ * node maps outside the main box or onto an unlisted node.  Destroying or
+ * re-uploading either context unlinks them. */
+int zf_parent_link(zf_ctx *parent, zf_ctx *main);
+
 /* ---- RBC membranes ---------------------------------------------------------
  * Constants of the membrane model (ZebraflowWT.c:34-36, :118-134, :5654).
  * len_scale converts a coordinate difference to the force formulas' length

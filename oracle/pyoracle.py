@@ -41,6 +41,7 @@ def lib():
         _lib = C.CDLL(LIB)
         _lib.zfo_dom_setup.restype = C.c_int
         _lib.zfo_bc_setup.restype = C.c_int
+        _lib.zfo_parent_bc_setup.restype = C.c_int
         _lib.zfo_feq.restype = C.c_double
         _lib.zfo_feq.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         _lib.zfo_bc_pressure.restype = C.c_double
@@ -85,6 +86,22 @@ def bc_setup(b, periodic=(0, 0, 0)):
     return bl[:n], bn[:n], nf.value
 
 
+def parent_bc_setup(b, p, k_over, off):
+    """BC list, neighbour list and upstream velocity sources of parent lattice p."""
+    b = np.ascontiguousarray(b, np.int32)
+    im, jm, km = b.shape
+    cap = int(((b == 0) | (b >= 10)).sum())
+    bl = np.zeros((max(cap, 1), 4), np.int32)
+    bn = np.zeros((max(cap, 1), 3), np.int32)
+    vb = np.zeros((max(cap, 1), 3), np.int32)
+    o = (C.c_int * 3)(*off)
+    nf = C.c_int()
+    n = lib().zfo_parent_bc_setup(_p(b), im, jm, km, p, k_over, o, _p(bl), _p(bn), _p(vb), cap,
+                                  C.byref(nf))
+    assert n == cap
+    return bl[:n], bn[:n], vb[:n], nf.value
+
+
 def neighbours(domid, domlist, periodic=(0, 0, 0)):
     im, jm, km = domid.shape
     n = len(domlist)
@@ -107,12 +124,19 @@ def feq(rho, u):
 class LbmOracle:
     """Reference-structured LBM (F, F_OLD, FEQ, D, U, V, W, BODYFORCE)."""
 
-    def __init__(self, b, params, dtype=np.float64, periodic=None):
+    def __init__(self, b, params, dtype=np.float64, periodic=None, parent=None):
+        """parent: (p, k_over, offset) for a parent lattice (its BC lists and
+        upstream velocity sources from parent_bc_setup)."""
         self.b = np.ascontiguousarray(b, np.int32)
         self.im, self.jm, self.km = self.b.shape
         per = periodic if periodic is not None else tuple(params.get("periodic", (0, 0, 0)))
         self.domid, self.domlist = dom_setup(self.b)
-        self.bclist, self.bcnlist, self.n_fail = bc_setup(self.b, per)
+        self.parent = parent
+        if parent is None:
+            self.bclist, self.bcnlist, self.n_fail = bc_setup(self.b, per)
+            self.vbcn = None
+        else:
+            self.bclist, self.bcnlist, self.vbcn, self.n_fail = parent_bc_setup(self.b, *parent)
         self.n = len(self.domlist)
         self.dtype = np.dtype(dtype)
         p = LbmParams()
@@ -162,6 +186,12 @@ class LbmOracle:
            _p(self.D), _p(self.U), _p(self.V), _p(self.W), _p(self.force), int(it), int(rbc_step),
            int(nthreads))
 
+    def phase(self, ph, it, rbc_step=0, nthreads=0):
+        assert self.dtype == np.float64
+        lib().zfo_lbm_phase_f64(C.byref(self.lat), C.byref(self.params), int(ph), _p(self.F),
+                                _p(self.F_OLD), _p(self.FEQ), _p(self.D), _p(self.U), _p(self.V),
+                                _p(self.W), _p(self.force), int(it), int(rbc_step), int(nthreads))
+
     @property
     def populations(self):
         return self.F_OLD
@@ -169,6 +199,24 @@ class LbmOracle:
     def bnode(self):
         d = self.domlist
         return self.b[d[:, 0] - 1, d[:, 1] - 1, d[:, 2] - 1]
+
+
+def step_with_parents(main, parents, it, rbc_step=0, nthreads=0):
+    """One iteration of a main lattice and its parent lattices (FP64) in the
+    reference's order (ZebraflowWT.c:6101-6626): every stream + collide, the
+    main boundary pass, the parents' boundary passes, every macro + copy."""
+    main.phase(0, it, nthreads=nthreads)
+    for p in parents:
+        p.phase(0, it, nthreads=nthreads)
+    main.phase(1, it, nthreads=nthreads)
+    for p in parents:
+        vb = np.ascontiguousarray(p.vbcn)
+        lib().zfo_parent_boundary_f64(C.byref(p.lat), C.byref(p.params), int(p.parent[0]), _p(vb),
+                                      _p(p.F), _p(p.D), C.byref(main.lat), _p(main.D), _p(main.U),
+                                      _p(main.V), _p(main.W), int(nthreads))
+    main.phase(2, it, rbc_step=rbc_step, nthreads=nthreads)
+    for p in parents:
+        p.phase(2, it, nthreads=nthreads)
 
 
 def wtime():

@@ -26,6 +26,16 @@
  *               NodeM, NEIGHNODE_LIST rows (10 ints), NEIGHTRI_LIST rows
  *               (10 ints), then float32 LMAX, Plength, kp, Theta0,
  *               SINETHETA0, COSINETHETA0 (EdgeM each).
+ *
+ *   ref_harness parents <workdir> <out.bin>
+ *     Runs LBM_PARENTDOMAIN_SETUP (:515) and LBM_PARENTBC_SETUP (:1015) in
+ *     <workdir>, which must hold InputData/DAparentBoundary.csv,
+ *     InputData/PCVparent1Boundary.csv and InputData/PCVparent2Boundary.csv
+ *     and an empty LBMGridDataStructure/ (the setup writes its domain files
+ *     there).
+ *     out.bin, for parents 1..3 in turn: int32 im, jm, km, B_index_p
+ *     (im*jm*km, i outer, k inner), DOMMAX_p, BCMAX_p, BCLIST_p (BCMAX*4),
+ *     BCNLIST_p (BCMAX*3), vBCNLIST_p (BCMAX*3).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -51,6 +61,25 @@ extern int NEIGHNODE_LIST[R_NODEM + 1][R_NLISTM + 1];
 extern int NEIGHTRI_LIST[R_NODEM + 1][R_NLISTM + 1];
 extern float LMAX[R_EDGEM + 1], Plength[R_EDGEM + 1], kp[R_EDGEM + 1],
     Theta0[R_EDGEM + 1], SINETHETA0[R_EDGEM + 1], COSINETHETA0[R_EDGEM + 1];
+
+/* the three parent (feeder) lattices, ZebraflowWT.c:91-112, :188-230 */
+#define P1_IM 69
+#define P1_JM 56
+#define P1_KM 121
+#define P2_IM 64
+#define P2_JM 60
+#define P2_KM 121
+#define P3_IM 42
+#define P3_JM 27
+#define P3_KM 121
+extern int B_index_1[P1_IM + 1][P1_JM + 1][P1_KM + 1];
+extern int B_index_2[P2_IM + 1][P2_JM + 1][P2_KM + 1];
+extern int B_index_3[P3_IM + 1][P3_JM + 1][P3_KM + 1];
+extern int BCLIST_1[60000][4], BCNLIST_1[60000][3], BCMAX_1, DOMMAX_1, vBCNLIST_1[60000][3];
+extern int BCLIST_2[60000][4], BCNLIST_2[60000][3], BCMAX_2, DOMMAX_2, vBCNLIST_2[60000][3];
+extern int BCLIST_3[60000][4], BCNLIST_3[60000][3], BCMAX_3, DOMMAX_3, vBCNLIST_3[60000][3];
+void LBM_PARENTDOMAIN_SETUP(void);
+void LBM_PARENTBC_SETUP(void);
 
 void LBM_DOM_SETUP(void);
 void LBM_BC_SETUP(void);
@@ -137,9 +166,48 @@ static int run_mesh(const char *dir, const char *out) {
   return 0;
 }
 
+static void wr_parent(FILE *o, int im, int jm, int km, const int *b, int dommax, int bcmax,
+                      const int (*bcl)[4], const int (*bcn)[3], const int (*vbc)[3]) {
+  const int dims[3] = {im, jm, km};
+  wr(o, dims, 4, 3);
+  for (int i = 1; i <= im; i++)
+    for (int j = 1; j <= jm; j++)
+      wr(o, &b[((size_t)i * (jm + 1) + j) * (km + 1) + 1], 4, km);
+  wr(o, &dommax, 4, 1);
+  wr(o, &bcmax, 4, 1);
+  wr(o, &bcl[1][0], 4, (size_t)4 * bcmax);
+  wr(o, &bcn[1][0], 4, (size_t)3 * bcmax);
+  wr(o, &vbc[1][0], 4, (size_t)3 * bcmax);
+}
+
+static int run_parents(const char *dir, const char *out) {
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd)) return 2;
+  char outabs[4096];
+  if (out[0] == '/') snprintf(outabs, sizeof outabs, "%s", out);
+  else snprintf(outabs, sizeof outabs, "%s/%s", cwd, out);
+  if (chdir(dir) != 0) { perror(dir); return 2; }
+  for (int c = 1; c <= 19; c++)
+    for (int d = 0; d < 3; d++) Xlattice[d][c] = (float)E[c - 1][d];
+  iter = 0;
+  LBM_PARENTDOMAIN_SETUP();
+  LBM_PARENTBC_SETUP();
+  FILE *o = fopen(outabs, "wb");
+  if (!o) { perror(outabs); return 2; }
+  wr_parent(o, P1_IM, P1_JM, P1_KM, &B_index_1[0][0][0], DOMMAX_1, BCMAX_1, BCLIST_1, BCNLIST_1,
+            vBCNLIST_1);
+  wr_parent(o, P2_IM, P2_JM, P2_KM, &B_index_2[0][0][0], DOMMAX_2, BCMAX_2, BCLIST_2, BCNLIST_2,
+            vBCNLIST_2);
+  wr_parent(o, P3_IM, P3_JM, P3_KM, &B_index_3[0][0][0], DOMMAX_3, BCMAX_3, BCLIST_3, BCNLIST_3,
+            vBCNLIST_3);
+  fclose(o);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   if (argc == 4 && strcmp(argv[1], "lists") == 0) return run_lists(argv[2], argv[3]);
   if (argc == 4 && strcmp(argv[1], "mesh") == 0) return run_mesh(argv[2], argv[3]);
-  fprintf(stderr, "usage: ref_harness lists <in.bin> <out.bin> | mesh <dir> <out.bin>\n");
+  if (argc == 4 && strcmp(argv[1], "parents") == 0) return run_parents(argv[2], argv[3]);
+  fprintf(stderr, "usage: ref_harness lists <in.bin> <out.bin> | mesh <dir> <out.bin> | parents <dir> <out.bin>\n");
   return 1;
 }

@@ -48,6 +48,16 @@ int zfo_bc_setup(const int32_t *b_index, int im, int jm, int km,
                  const int periodic[3], int32_t *bclist, int32_t *bcnlist,
                  int cap, int *n_fail);
 
+/* ---- parent (feeder) lattices: BC list, LBM_PARENTBC_SETUP
+ * ZebraflowWT.c:1015-1350.  As zfo_bc_setup with the parent's thresholds;
+ * codes 10+p (boundary copied from the main domain) and 20+p (upstream)
+ * map to the main-domain node (i + off[0], j + off[1], k_over + off[2]);
+ * vbcn: 3*cap.  Returns BCMAX or -1 on overflow.  Pinned by reference
+ * goldens. */
+int zfo_parent_bc_setup(const int32_t *b, int im, int jm, int km, int p, int k_over,
+                        const int off[3], int32_t *bclist, int32_t *bcnlist, int32_t *vbcn,
+                        int cap, int *n_fail);
+
 /* ---- a2: pull-neighbour list, implicit in ZebraflowWT.c:6113-6125 --------
  * nbr[c*dommax + (l-1)] = DOMID[(i,j,k) - e_c] for c = 0..17 (1-based, or
  * -1 if unlisted/outside the box; periodic axes wrap).  Pinned by brute
@@ -101,6 +111,18 @@ void zfo_lbm_step_f32(const zfo_lattice *lat, const zfo_lbm_params *p,
                       int rbc_step, int nthreads);
 
 /* Equilibrium, ZebraflowWT.c:2568 with C_LBM = 1 (EQ_A=3, EQ_B=4.5, EQ_C=-1.5). */
+/* zfo_lbm_step split into its phases (0 stream + collide, 1 boundary,
+ * 2 macro + copy), and the parent lattices' boundary pass
+ * (ZebraflowWT.c:6436-6594) that runs between the phases of a main lattice
+ * and its parent: main phase 0, parent phase 0, main phase 1,
+ * zfo_parent_boundary, then phase 2 of both. */
+void zfo_lbm_phase_f64(const zfo_lattice *lat, const zfo_lbm_params *p, int phase, double *F,
+                       double *F_OLD, double *FEQ, double *D, double *U, double *V, double *W,
+                       double *force, int iter, int rbc_step, int nthreads);
+void zfo_parent_boundary_f64(const zfo_lattice *par, const zfo_lbm_params *pp, int p,
+                             const int32_t *vbcn, double *F, const double *D_p,
+                             const zfo_lattice *mainl, const double *D, const double *U,
+                             const double *V, const double *W, int nthreads);
 double zfo_feq(int c, double rho, double u, double v, double w);
 
 /* Pressure waveform of the BC codes 1001-1009, ZebraflowWT.c:6318-6421

@@ -103,6 +103,65 @@ int zfo_bc_setup(const int32_t *b, int im, int jm, int km, const int per[3],
   return n;
 }
 
+/* Parent (feeder) lattice p, LBM_PARENTBC_SETUP ZebraflowWT.c:1015-1350
+ * (parent 1 :1028-1137; parents 2 and 3 repeat it with their own codes and
+ * overlap plane).  BC list: B == 0 or B >= 10 (:1045), lexicographic.
+ * Neighbour of a B == 0 or B == 20+p node (:1053-1100): the first of
+ * c = 1..18 inside the box with 0 < B < 10, else with B > 10, else with
+ * B == 0.  A B == 10+p node (:1103-1110) and the velocity source (vBCNLIST,
+ * :1112-1122) of a B == 20+p node are the main-domain node over the parent
+ * node's column at the overlap plane k_over: the reference maps the parent
+ * coordinate (X_p at k_over) to the main index with
+ * (int)((X - X0)/DX + 1.01); the parent origins sit on the main lattice
+ * (ZebraflowWT.c:85-112), so that is (i + off[0], j + off[1], k_over +
+ * off[2]).  vbcn is (0,0,0) for the other nodes; a node of any other code
+ * >= 10, or with no neighbour, gets (0,0,0) and is counted in *n_fail (the
+ * reference leaves stale values there).  Pinned by reference goldens on the
+ * three parent lattices the reference ships. */
+int zfo_parent_bc_setup(const int32_t *b, int im, int jm, int km, int p, int k_over,
+                        const int off[3], int32_t *bclist, int32_t *bcnlist, int32_t *vbcn,
+                        int cap, int *n_fail) {
+  const int per[3] = {0, 0, 0};
+  int n = 0, fails = 0;
+  for (int i = 1; i <= im; i++)
+    for (int j = 1; j <= jm; j++)
+      for (int k = 1; k <= km; k++) {
+        const int bv = b[bidx(i, j, k, jm, km)];
+        if (!(bv >= 10 || bv == 0)) continue;
+        if (n >= cap) return -1;
+        bclist[4 * n + 0] = i;
+        bclist[4 * n + 1] = j;
+        bclist[4 * n + 2] = k;
+        bclist[4 * n + 3] = bv;
+        int found = 0, fi = 0, fj = 0, fk = 0, vi = 0, vj = 0, vk = 0;
+        if (bv == 0 || bv == 20 + p) {
+          for (int tier = 0; tier < 3 && !found; tier++) {
+            for (int c = 0; c < 18; c++) {
+              int ii, jj, kk;
+              if (!step_to(i, j, k, c, +1, im, jm, km, per, &ii, &jj, &kk)) continue;
+              const int nb = b[bidx(ii, jj, kk, jm, km)];
+              const int ok = (tier == 0) ? (nb > 0 && nb < 10) : (tier == 1) ? (nb > 10) : (nb == 0);
+              if (ok) { fi = ii; fj = jj; fk = kk; found = 1; break; }
+            }
+          }
+          if (bv == 20 + p) { vi = i + off[0]; vj = j + off[1]; vk = k_over + off[2]; }
+        } else if (bv == 10 + p) {
+          fi = i + off[0]; fj = j + off[1]; fk = k_over + off[2];
+          found = 1;
+        }
+        if (!found) fails++;
+        bcnlist[3 * n + 0] = fi;
+        bcnlist[3 * n + 1] = fj;
+        bcnlist[3 * n + 2] = fk;
+        vbcn[3 * n + 0] = vi;
+        vbcn[3 * n + 1] = vj;
+        vbcn[3 * n + 2] = vk;
+        n++;
+      }
+  if (n_fail) *n_fail = fails;
+  return n;
+}
+
 /* Pull source of direction c is (i,j,k) - e_c: the reference steps along the
  * opposite direction cc (ZebraflowWT.c:6114-6119). */
 void zfo_neighbours(const int32_t *domid, const int32_t *domlist, int dommax,

@@ -73,6 +73,9 @@ FUNCS = {
     "zf_download_macro": (C.c_int, [P, P, P, P, P]),
     "zf_conservation_sums": (C.c_int, [P, P]),
     "zf_lbm_time_kernels": (C.c_int, [P, i32, i32, P]),
+    "zf_set_parent": (C.c_int, [P, i32, i32, P]),
+    "zf_download_parent_lists": (C.c_int, [P, P]),
+    "zf_parent_link": (C.c_int, [P, P]),
     "zf_membrane_config_reference": (None, [C.POINTER(ZfMembraneConfig)]),
     "zf_membrane_setup": (C.c_int, [P, C.POINTER(ZfMembraneConfig), i32, i32, P, P]),
     "zf_download_mesh_topology": (C.c_int, [P, C.POINTER(i32), P, P, P]),

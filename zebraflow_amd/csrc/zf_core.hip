@@ -100,6 +100,7 @@ int zf_destroy(zf_ctx *ctx) {
   if (!ctx) return ZF_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  free_parent(ctx);
   free_adjoint(ctx);
   free_state_slots(ctx);
   free_cells(ctx);

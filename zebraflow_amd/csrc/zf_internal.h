@@ -47,6 +47,17 @@ struct BcEntry {
                   // come from this step's macro output, else from moments
 };
 
+// One coupled boundary node of a parent lattice: its populations are set to
+// feq of moments taken from the main lattice (and, upstream, the parent's
+// own density at its neighbour).
+struct ParentBc {
+  int32_t s;     // storage index of the BC node (parent)
+  int32_t sm;    // storage index of its parent neighbour (upstream), else -1
+  int32_t slot;  // slot of the main-lattice moments in the main's export buffer
+  int32_t kind;  // 0 upstream, neighbour interior; 1 upstream, neighbour not
+                 // interior; 2 copied from the main lattice
+};
+
 // One saved state (zf_checkpoint_save): regions packed at 256-byte offsets.
 struct StateSlot {
   void *buf = nullptr;
@@ -209,6 +220,22 @@ struct zf_ctx {
   std::vector<int32_t> h_status0;  // cell status at the start of the forward pass
 
   std::vector<zf::StateSlot> slots;  // zf_checkpoint_save
+
+  // parent (feeder) lattice p = 1..3 (0: a main lattice), its overlap plane
+  // and the offset of its index space in the main lattice's
+  int32_t parent_p = 0, parent_kover = 0, parent_off[3] = {0, 0, 0};
+  int32_t *d_vbcn = nullptr;       // 3 x bcmax: upstream velocity source (main indices)
+  zf_ctx *link_main = nullptr;     // main lattice this parent takes its boundary from
+  zf::ParentBc *d_pbc = nullptr;   // coupled boundary nodes
+  int32_t n_pbc = 0;
+  hipEvent_t couple_done = nullptr;  // after this parent's coupling pass
+  // main lattice: moments of the nodes its linked parents read, written
+  // by every step after the stream kernel
+  std::vector<zf_ctx *> linked;
+  std::vector<int32_t> h_export;   // per slot: storage index << 1 | interior
+  int32_t *d_export_s = nullptr;
+  double *d_export = nullptr;      // 4 x n_export: rho, u, v, w
+  hipEvent_t export_done = nullptr;
 };
 
 namespace zf {
@@ -252,6 +279,13 @@ inline size_t field_elem(const zf_ctx *ctx) { return ctx->cfg.storage == ZF_STOR
 // cell list, interpolation, repulsion, spreading) and, with `advance`, the
 // position update: the order of zf_step.
 int rbc_update(zf_ctx *ctx, bool advance);
+// Parent coupling (zf_parent.hip): the main lattice's export of the moments
+// its parents read (in_step: after its stream kernel, before the buffer
+// swap and the force field's clearing; else of the current state), and a
+// parent's coupled boundary pass (after its own step and swap).
+int parent_export(zf_ctx *ctx, bool in_step);
+int parent_couple(zf_ctx *ctx);
+void free_parent(zf_ctx *ctx);
 int advance_impl(zf_ctx *ctx, int32_t *n_removed);
 int ensure_force_field(zf_ctx *ctx);
 // Transposes of the IBM coupling at positions X and populations f (device

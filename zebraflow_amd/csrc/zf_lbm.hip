@@ -406,6 +406,9 @@ int step_impl(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
   if (ctx->d_force) launch_stream_opts<S, true>(ctx, src, dst, prm, bp);
   else launch_stream_opts<S, false>(ctx, src, dst, prm, bp);
   if (ev) ZF_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+  int rc;
+  // moments the linked parent lattices read (with the force of this step)
+  if (!ctx->h_export.empty() && (rc = parent_export(ctx, true))) return rc;
   if (rbc_step && ctx->d_force) {
     const size_t nw = 3 * ctx->stride * field_elem(ctx) / 4;
     hipLaunchKernelGGL(k_fill_zero, dim3(nblocks(nw, 256)), dim3(256), 0, ctx->stream,
@@ -413,6 +416,8 @@ int step_impl(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
   }
   ZF_HIP(ctx, hipGetLastError());
   ctx->cur ^= 1;
+  // a parent lattice: its coupled boundary nodes from the main lattice
+  if (ctx->link_main && (rc = parent_couple(ctx))) return rc;
   return ZF_OK;
 }
 

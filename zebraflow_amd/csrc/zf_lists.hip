@@ -25,9 +25,18 @@ struct Box {
   int per[3];
 };
 
+// 0: DOMLIST (B >= 0); 1: BCLIST of a main lattice (B == 0 or B >= 1000,
+// ZebraflowWT.c:1515); 2: BCLIST of a parent lattice (B == 0 or B >= 10,
+// :1045).
 __device__ __forceinline__ bool pred(int mode, int b) {
-  return mode == 0 ? (b >= 0) : (b == 0 || b >= 1000);
+  return mode == 0 ? (b >= 0) : (b == 0 || b >= (mode == 1 ? 1000 : 10));
 }
+
+// Parent lattice p (0: main lattice): overlap plane and index offset of the
+// main-domain node over a parent column.
+struct ParentMap {
+  int p, kover, off[3];
+};
 
 __device__ __forceinline__ void unflatten(size_t q, int jm, int km, int &i, int &j, int &k) {
   k = (int)(q % (size_t)km) + 1;
@@ -153,26 +162,44 @@ __global__ __launch_bounds__(kScanThreads) void k_tile_scatter(
 }
 
 // BCNLIST: three tiers, each scanning c = 1..18 in order
-// (ZebraflowWT.c:1535-1574).  No neighbour -> (0,0,0).
+// (ZebraflowWT.c:1535-1574).  No neighbour -> (0,0,0).  On a parent lattice
+// (LBM_PARENTBC_SETUP, :1053-1122) the tiers use its thresholds (0<B<10,
+// B>10, B==0) and run for B == 0 and the upstream code 20+p only; the copy
+// code 10+p takes the main-domain node over its column at the overlap plane,
+// and an upstream node's velocity source (vbcn) is that same node.
 __global__ void k_bc_neighbour(const int32_t *__restrict__ b, Box bx, int bcmax,
-                               const int32_t *__restrict__ bclist, int32_t *__restrict__ bcnlist) {
+                               const int32_t *__restrict__ bclist, int32_t *__restrict__ bcnlist,
+                               ParentMap pm, int32_t *__restrict__ vbcn) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= bcmax) return;
-  const int i = bclist[4 * t], j = bclist[4 * t + 1], k = bclist[4 * t + 2];
+  const int i = bclist[4 * t], j = bclist[4 * t + 1], k = bclist[4 * t + 2], code = bclist[4 * t + 3];
+  const int lim = pm.p ? 10 : 1000;
   int fi = 0, fj = 0, fk = 0;
   bool found = false;
-  for (int tier = 0; tier < 3 && !found; tier++) {
+  const bool search = !pm.p || code == 0 || code == 20 + pm.p;
+  for (int tier = 0; search && tier < 3 && !found; tier++) {
     for (int c = 0; c < 18; c++) {
       int ii, jj, kk;
       if (!step_to(bx, i, j, k, c, +1, ii, jj, kk)) continue;
       const int nb = b[flatten(ii, jj, kk, bx.jm, bx.km)];
-      const bool ok = tier == 0 ? (nb > 0 && nb < 1000) : tier == 1 ? (nb > 1000) : (nb == 0);
+      const bool ok = tier == 0 ? (nb > 0 && nb < lim) : tier == 1 ? (nb > lim) : (nb == 0);
       if (ok) { fi = ii; fj = jj; fk = kk; found = true; break; }
     }
+  }
+  if (pm.p && code == 10 + pm.p) {
+    fi = i + pm.off[0];
+    fj = j + pm.off[1];
+    fk = pm.kover + pm.off[2];
   }
   bcnlist[3 * t] = fi;
   bcnlist[3 * t + 1] = fj;
   bcnlist[3 * t + 2] = fk;
+  if (pm.p) {
+    const bool up = code == 20 + pm.p;
+    vbcn[3 * t] = up ? i + pm.off[0] : 0;
+    vbcn[3 * t + 1] = up ? j + pm.off[1] : 0;
+    vbcn[3 * t + 2] = up ? pm.kover + pm.off[2] : 0;
+  }
 }
 
 // Pull sources: DOMID of (i,j,k) - e_c (ZebraflowWT.c:6114-6119).
@@ -221,7 +248,8 @@ void free_lattice(zf_ctx *ctx) {
                    (void **)&ctx->d_bclist,   (void **)&ctx->d_bcnlist, (void **)&ctx->d_nbr,
                    (void **)&ctx->d_s_of_p,   (void **)&ctx->d_p_of_s, (void **)&ctx->d_word,
                    (void **)&ctx->d_block_chord0, (void **)&ctx->d_chords, (void **)&ctx->d_bc,
-                   (void **)&ctx->d_fused_s, (void **)&ctx->d_att_start, (void **)&ctx->d_sbox};
+                   (void **)&ctx->d_fused_s, (void **)&ctx->d_att_start, (void **)&ctx->d_sbox,
+                   (void **)&ctx->d_vbcn};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_lists = false;
   ctx->have_lattice = false;
@@ -413,6 +441,7 @@ static int build_kernel_tables(zf_ctx *ctx, const int32_t *b, const std::vector<
 }
 
 int upload_lattice_impl(zf_ctx *ctx, const int32_t *b) {
+  free_parent(ctx);  // coupling tables index the old lists
   free_adjoint(ctx);
   free_cells(ctx);  // cell-list and IBM box tables belong to the old lattice
   free_ibm(ctx);
@@ -434,19 +463,24 @@ int upload_lattice_impl(zf_ctx *ctx, const int32_t *b) {
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_domlist, sizeof(int32_t) * 3 * (size_t)std::max(nd, 1))))
     goto out;
   if ((rc = compact(ctx, 0, d_counts, ntiles, ctx->d_domlist, &nd))) goto out;
-  if ((rc = compact(ctx, 1, d_counts, ntiles, nullptr, &nb))) goto out;
+  if ((rc = compact(ctx, ctx->parent_p ? 2 : 1, d_counts, ntiles, nullptr, &nb))) goto out;
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_bclist, sizeof(int32_t) * 4 * (size_t)std::max(nb, 1))))
     goto out;
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_bcnlist, sizeof(int32_t) * 3 * (size_t)std::max(nb, 1))))
     goto out;
-  if ((rc = compact(ctx, 1, d_counts, ntiles, ctx->d_bclist, &nb))) goto out;
+  if (ctx->parent_p &&
+      (rc = dev_alloc(ctx, (void **)&ctx->d_vbcn, sizeof(int32_t) * 3 * (size_t)std::max(nb, 1))))
+    goto out;
+  if ((rc = compact(ctx, ctx->parent_p ? 2 : 1, d_counts, ntiles, ctx->d_bclist, &nb))) goto out;
   ctx->dommax = nd;
   ctx->bcmax = nb;
   {
     const Box bx{c.im, c.jm, c.km, {c.periodic[0], c.periodic[1], c.periodic[2]}};
+    const ParentMap pm{ctx->parent_p, ctx->parent_kover,
+                       {ctx->parent_off[0], ctx->parent_off[1], ctx->parent_off[2]}};
     if (nb > 0)
       hipLaunchKernelGGL(k_bc_neighbour, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream,
-                         ctx->d_b, bx, nb, ctx->d_bclist, ctx->d_bcnlist);
+                         ctx->d_b, bx, nb, ctx->d_bclist, ctx->d_bcnlist, pm, ctx->d_vbcn);
     if ((rc = dev_alloc(ctx, (void **)&ctx->d_nbr, sizeof(int32_t) * 18 * (size_t)std::max(nd, 1))))
       goto out;
     if (nd > 0)

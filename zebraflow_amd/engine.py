@@ -91,6 +91,20 @@ class Engine:
         self.dommax, self.bcmax, self.n_interior = nd.value, nb.value, ni.value
         return self.dommax, self.bcmax, self.n_interior
 
+    def set_parent(self, p, k_over, offset):
+        """Make this context parent lattice p (before upload_lattice)."""
+        off = np.ascontiguousarray(offset, np.int32)
+        self._check(self.lib.zf_set_parent(self.h, p, k_over, _ptr(off)), "zf_set_parent")
+
+    def parent_lists(self):
+        vb = np.zeros((self.bcmax, 3), np.int32)
+        self._check(self.lib.zf_download_parent_lists(self.h, _ptr(vb)), "zf_download_parent_lists")
+        return vb
+
+    def parent_link(self, main):
+        """Couple this parent lattice's boundary to the main lattice `main`."""
+        self._check(self.lib.zf_parent_link(self.h, main.h), "zf_parent_link")
+
     def lists(self):
         dl = np.zeros((self.dommax, 3), np.int32)
         bl = np.zeros((self.bcmax, 4), np.int32)
