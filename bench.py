@@ -232,6 +232,75 @@ def cells_leg(name, b, ncell, steps, warmup):
                     rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))))
 
 
+def parent_leg(steps=500, warmup=50):
+    """A main lattice with one parent (feeder) lattice in the layout of the
+    reference's DA parent: the main lattice a straight vessel (96x96x512,
+    radius 40, periodic along k, driven by a body force), the parent its
+    cross-section extruded over 121 planes with the upstream code 21 at
+    k = 1 and the copy code 11 at k = 121, which lies on main plane k = 3.
+    FP32-deviation storage.  Times main-only iterations and coupled ones
+    (main step, then parent step with its boundary pass) by the host clock
+    around whole loops, the parent's stream kernel by HIP events, and checks
+    that the parent's copy plane carries the main plane's axial velocity."""
+    from tests import geometry as g  # the product's C generators (libzfhost.so)
+    from zebraflow_amd.engine import Engine, lattice_config
+    bm = g.tube((96, 96, 512), axis=2, r=40.0, periodic=True)
+    kp = 121
+    plane = bm[:, :, 0]
+    bp = np.repeat(plane[:, :, None], kp, axis=2).astype(np.int32)
+    bp[bp > 0] = 1
+    bp[:, :, kp - 1][plane > 0] = 11
+    bp[:, :, 0][plane > 0] = 21
+    cm = lattice_config(*bm.shape, 0.8, storage=STORE_NAMES["f32dev"], periodic=(0, 0, 1),
+                        body_force=(0, 0, 2e-6), flags=FLAGS)
+    cp = lattice_config(*bp.shape, 0.8, storage=STORE_NAMES["f32dev"], flags=FLAGS)
+    em, ep = Engine(cm), Engine(cp)
+    em.upload_lattice(bm)
+    ep.set_parent(1, kp, (0, 0, 3 - kp))
+    ep.upload_lattice(bp)
+    em.init_equilibrium()
+    ep.init_equilibrium()
+    ep.parent_link(em)
+
+    def loop(it0, n, coupled):
+        em.sync()
+        ep.sync()
+        t = time.perf_counter()
+        for it in range(it0, it0 + n):
+            em.step(it)
+            if coupled:
+                ep.step(it)
+        em.sync()
+        ep.sync()
+        return (time.perf_counter() - t) * 1e3
+
+    loop(1, warmup, True)
+    ms_pair = loop(warmup + 1, steps, True)
+    ms_main = loop(warmup + steps + 1, steps, False)
+    # the parent's own stream kernel (its boundary pass runs after it)
+    pk_ms, _, pk_n = ep.time_kernels(warmup + 2 * steps, 100)
+    pk_us = pk_ms * 1e3 / pk_n
+    loop(warmup + 2 * steps + 101, 2000, True)  # let the flow reach the parent
+    dm, um, vm, wm = em.macro()
+    dp, up, vp, wp = ep.macro()
+    dlm, dlp = em.lists()[0], ep.lists()[0]
+    n_main, n_par = em.n_interior, ep.n_interior
+    ep.close()
+    em.close()
+    sel_m = (dlm[:, 2] == 3) & (bm[dlm[:, 0] - 1, dlm[:, 1] - 1, 2] > 0)
+    sel_p = (dlp[:, 2] == kp) & (bp[dlp[:, 0] - 1, dlp[:, 1] - 1, kp - 1] == 11)
+    km = {tuple(x[:2]): w for x, w in zip(dlm[sel_m], wm[sel_m])}
+    diff = [abs(w - km[tuple(x[:2])]) for x, w in zip(dlp[sel_p], wp[sel_p])]
+    return dict(main_fluid_nodes=n_main, parent_fluid_nodes=n_par, coupled_nodes=int(sel_p.sum() * 2),
+                steps=steps, main_ms_per_step=round(ms_main / steps, 4),
+                coupled_ms_per_iteration=round(ms_pair / steps, 4),
+                parent_kernel_us=round(pk_us, 2),
+                parent_mlups=round(n_par / pk_us, 1),
+                coupling_overhead_ms=round((ms_pair - ms_main) / steps - pk_us * 1e-3, 4),
+                copy_plane_w_max_abs_diff=float(max(diff)), main_plane_w_max=float(max(km.values())),
+                timing="host clock around whole loops (includes the Python call per step)")
+
+
 def mixed_precision_leg(steps=3000, ncell=6000, interval=1, chunk=500):
     """The large mixed-precision run: the 4-generation tree (1024x256x256,
     12 % fluid) with as many Evans-Fung cells (642 nodes, radius 4) as the
@@ -492,6 +561,9 @@ def main():
     ap.add_argument("--cells", type=int, default=1200,
                     help="cells of the dense-cell leg (0 skips it)")
     ap.add_argument("--cell-steps", type=int, default=200)
+    ap.add_argument("--no-parents", action="store_true", help="skip the parent-lattice leg")
+    ap.add_argument("--parents-only", action="store_true",
+                    help="only the parent-lattice coupling leg (for profiling)")
     ap.add_argument("--cells-only", action="store_true",
                     help="only the dense-cell leg (for profiling its kernels)")
     ap.add_argument("--mixed-steps", type=int, default=3000)
@@ -522,6 +594,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(dict(adjoint=adjoint_leg())), flush=True)
+        return
+    if args.parents_only:
+        print(json.dumps(dict(parent_coupling=parent_leg())), flush=True)
         return
     if args.cells_only:
         print(json.dumps(dict(dense_cells=cells_leg(args.workload, b, args.cells, args.cell_steps,
@@ -555,6 +630,8 @@ def main():
         if args.cells > 0:
             extra["dense_cells"] = cells_leg(args.workload, b, args.cells, args.cell_steps,
                                              max(1, args.cell_steps // 10))
+        if not args.no_parents:
+            extra["parent_coupling"] = parent_leg()
 
     line = {
         "metric": "fluid MLUPS (FP32-deviation D3Q19 step) + % HBM roofline vs FP64 form",
