@@ -175,7 +175,9 @@ int zf_download_parent_lists(zf_ctx *ctx, int32_t *vbcn);
  * kernel.  Step main first, then its parents, in each iteration; the two
  * streams are ordered with events.  Fails with ZF_ERR_GEOMETRY if a coupled
  * node maps outside the main box or onto an unlisted node.  Destroying or
- * re-uploading either context unlinks them. */
+ * re-uploading either context unlinks them.  The adjoint entry points do
+ * not cover the coupling: linked contexts refuse zf_adjoint_setup and a
+ * context with an adjoint set up refuses the link. */
 int zf_parent_link(zf_ctx *parent, zf_ctx *main);
 
 /* ---- RBC membranes ---------------------------------------------------------

@@ -56,3 +56,76 @@ def test_parent_tiers_differ_from_the_main_rule():
     b = g["b1"].astype(np.int32)
     bl_main, _, _ = po.bc_setup(b)
     assert len(bl_main) < len(g["bclist1"])
+
+
+def _pulled_moments(b, rho, periodic_k):
+    """Moments after the pull of rest-equilibrium populations w_c rho(x)
+    (numpy, over the box): D = sum_c w_c rho(x - e_c), U = sum_c c w_c
+    rho(x - e_c) / D, at every box node whose pull sources are in the box."""
+    im, jm, km = b.shape
+    D = np.zeros(b.shape)
+    J = np.zeros((3,) + b.shape)
+    for c in range(19):
+        e = po.E[c]
+        src = np.roll(rho, shift=(e[0], e[1], e[2]), axis=(0, 1, 2))  # rho(x - e_c)
+        D += po.W[c] * src
+        for a in range(3):
+            J[a] += e[a] * po.W[c] * src
+    if not periodic_k:
+        D[:, :, 0] = D[:, :, -1] = np.nan  # sources outside the box
+    return D, J / D
+
+
+def test_parent_boundary_pass_against_brute_force():
+    """One paired step (main lattice, then parent 1; ZebraflowWT.c:6101-6626)
+    from rest-equilibrium populations with different density fields on the
+    two lattices.  The parent's copy nodes must hold feq(D, U) of the main
+    node over their column at the overlap plane, its upstream nodes
+    feq(D_p at their neighbour, U of that main node): both recomputed here by
+    brute force over the box, independently of the oracle's lists."""
+    from tests import geometry
+    from zebraflow_amd.engine import WALL_EQUILIBRIUM, lattice_config
+    from tests.test_gpu_lbm import oracle_params
+    bm = geometry.tube((16, 16, 24), axis=2, r=6.5, periodic=True)
+    kp, k_main = 12, 10
+    plane = bm[:, :, 0]
+    bp = np.repeat(plane[:, :, None], kp, axis=2).astype(np.int32)
+    bp[bp > 0] = 1
+    bp[:, :, kp - 1][plane > 0] = 11
+    bp[:, :, 0][plane > 0] = 21
+    off = (0, 0, k_main - kp)
+    cm = lattice_config(*bm.shape, 0.8, periodic=(0, 0, 1), wall_mode=WALL_EQUILIBRIUM)
+    cp = lattice_config(*bp.shape, 0.8, wall_mode=WALL_EQUILIBRIUM)
+    om = po.LbmOracle(bm, oracle_params(cm))
+    op = po.LbmOracle(bp, oracle_params(cp), parent=(1, kp, off))
+    ii, jj, kk = np.meshgrid(np.arange(1, 17), np.arange(1, 17), np.arange(1, 25), indexing="ij")
+    rho_m = 1 + 0.01 * np.sin(ii) + 0.02 * np.cos(0.7 * jj) + 0.03 * kk / 24
+    ii, jj, kk = np.meshgrid(np.arange(1, 17), np.arange(1, 17), np.arange(1, kp + 1), indexing="ij")
+    rho_p = 1 - 0.02 * np.cos(ii) + 0.01 * np.sin(1.3 * jj) + 0.02 * kk / kp
+    for o, rho in ((om, rho_m), (op, rho_p)):
+        d = o.domlist - 1
+        r = rho[d[:, 0], d[:, 1], d[:, 2]]
+        o.set_populations(po.W[:, None] * r[None, :])
+    po.step_with_parents(om, [op], 1)
+    Dm, Um = _pulled_moments(bm, rho_m, True)
+    Dp, _ = _pulled_moments(bp, rho_p, False)
+    bl, bn = op.bclist, op.bcnlist
+    checked = 0
+    for row, nb in zip(bl, bn):
+        i, j, k, code = row
+        if code not in (11, 21):
+            continue
+        q = op.domid[i - 1, j - 1, k - 1] - 1
+        v = (i - 1, j - 1, k_main - 1)  # main node over the column
+        u = Um[:, v[0], v[1], v[2]]
+        if code == 11:
+            dbc = Dm[v]
+        else:
+            # the upstream neighbour: the first interior node among c = 1..18
+            nbr = next((i + e[0], j + e[1], k + e[2]) for e in po.E[:18]
+                       if 1 <= k + e[2] <= kp and 0 < bp[i + e[0] - 1, j + e[1] - 1, k + e[2] - 1] < 10)
+            dbc = Dp[nbr[0] - 1, nbr[1] - 1, nbr[2] - 1]
+        want = po.feq(np.array([dbc]), u[:, None])[:, 0]
+        np.testing.assert_allclose(op.populations[:, q], want, rtol=0, atol=1e-15)
+        checked += 1
+    assert checked == 2 * int((plane > 0).sum())

@@ -336,6 +336,8 @@ int zf_parent_link(zf_ctx *parent, zf_ctx *main) {
     return fail(parent, ZF_ERR_STATE, "zf_parent_link needs a main lattice uploaded");
   if (main->device != parent->device) return fail(parent, ZF_ERR_ARG, "contexts on different devices");
   if (main->linked.size() >= 8) return fail(parent, ZF_ERR_ARG, "at most 8 parents per main lattice");
+  if (main->have_adj || parent->have_adj)
+    return fail(parent, ZF_ERR_STATE, "the adjoint does not cover parent-lattice coupling");
   free_parent(parent);
   ZF_HIP(parent, hipSetDevice(parent->device));
   ZF_HIP(parent, hipEventCreateWithFlags(&parent->couple_done, hipEventDisableTiming));
