@@ -126,3 +126,51 @@ def test_link_rejects_a_parent_outside_the_main_box():
         ep.parent_link(em)
     ep.close()
     em.close()
+
+
+def test_two_parents_share_one_main_lattice():
+    """Parents 1 and 2 on one main lattice (the reference runs three): their
+    export slots are shared, and both match the oracle's paired step."""
+    bm = geometry.tube((16, 16, 24), axis=2, r=6.5, periodic=True)
+    kp = 12
+    cm = lattice_config(*bm.shape, 0.8, storage=STORE_F64, periodic=(0, 0, 1), body_force=(0, 0, 1e-4),
+                        wall_mode=WALL_EQUILIBRIUM)
+    om = po.LbmOracle(bm, oracle_params(cm))
+    em = Engine(cm)
+    em.upload_lattice(bm)
+    parents = []
+    # parent 1 over main plane 10 (its k = 12), parent 2 over main plane 10 too
+    # (its k = 1): the two share every exported node
+    for p, k_over in ((1, kp), (2, 1)):
+        bp = parent_of(bm, kp, p)
+        off = (0, 0, 10 - k_over)
+        cp = lattice_config(*bp.shape, 0.8, storage=STORE_F64, wall_mode=WALL_EQUILIBRIUM)
+        op = po.LbmOracle(bp, oracle_params(cp), parent=(p, k_over, off))
+        ep = Engine(cp)
+        ep.set_parent(p, k_over, off)
+        ep.upload_lattice(bp)
+        ep.parent_link(em)
+        parents.append((op, ep))
+    print("case two parents on one main lattice", flush=True)
+    rng = np.random.default_rng(20261205)
+    for o, e in [(om, em)] + parents:
+        rho = 1 + 0.002 * rng.standard_normal(o.n)
+        u = 0.01 * rng.standard_normal((3, o.n))
+        f0 = po.feq(rho, u)
+        o.set_populations(f0)
+        e.upload_populations(f0)
+    for it in range(1, 31):
+        po.step_with_parents(om, [op for op, _ in parents], it)
+        em.step(it)
+        for _, ep in parents:
+            ep.step(it)
+    np.testing.assert_allclose(em.populations(), om.populations, rtol=0, atol=1e-13)
+    for op, ep in parents:
+        np.testing.assert_allclose(ep.populations(), op.populations, rtol=0, atol=1e-13)
+    # destroying the main lattice first unlinks its parents, which then step
+    # on their own
+    em.close()
+    for _, ep in parents:
+        ep.step(31)
+        assert np.isfinite(ep.populations()).all()
+        ep.close()
