@@ -114,6 +114,16 @@ def test_coupled_step_f32dev_matches_oracle():
     np.testing.assert_allclose(fp, op.populations, rtol=0, atol=2e-7)
 
 
+def test_coupled_step_f16dev_matches_oracle():
+    # FP16 deviations: the tolerance of the plain FP16 form (velocity within
+    # 1e-2 relative L2, density within 2e-3)
+    from zebraflow_amd.engine import STORE_F16DEV
+    om, op, (fm, fp, mp) = run_coupled(STORE_F16DEV, WALL_EQUILIBRIUM, 1)
+    uo = np.stack([op.U, op.V, op.W])
+    assert np.linalg.norm(mp[1:4] - uo) / np.linalg.norm(uo) <= 1e-2
+    np.testing.assert_allclose(mp[0], op.D, rtol=0, atol=2e-3)
+
+
 def test_link_rejects_a_parent_outside_the_main_box():
     bm = geometry.tube((16, 16, 24), axis=2, r=6.5, periodic=True)
     bp = parent_of(bm, 12)
