@@ -177,7 +177,9 @@ int zf_download_parent_lists(zf_ctx *ctx, int32_t *vbcn);
  * node maps outside the main box or onto an unlisted node.  Destroying or
  * re-uploading either context unlinks them.  The adjoint entry points do
  * not cover the coupling: linked contexts refuse zf_adjoint_setup and a
- * context with an adjoint set up refuses the link. */
+ * context with an adjoint set up refuses the link.  Linked contexts are
+ * called from one host thread.  The link reads the main lattice's DOMID
+ * box to the host once (4 bytes per box node). */
 int zf_parent_link(zf_ctx *parent, zf_ctx *main);
 
 /* ---- RBC membranes ---------------------------------------------------------
