@@ -178,8 +178,7 @@ int zf_download_parent_lists(zf_ctx *ctx, int32_t *vbcn);
  * re-uploading either context unlinks them.  The adjoint entry points do
  * not cover the coupling: linked contexts refuse zf_adjoint_setup and a
  * context with an adjoint set up refuses the link.  Linked contexts are
- * called from one host thread.  The link reads the main lattice's DOMID
- * box to the host once (4 bytes per box node). */
+ * called from one host thread. */
 int zf_parent_link(zf_ctx *parent, zf_ctx *main);
 
 /* ---- RBC membranes ---------------------------------------------------------

@@ -120,6 +120,18 @@ __global__ void k_parent_bc(typename Store<S>::T *__restrict__ fnew,
     fnew[(size_t)c * stride + b.s] = Store<S>::st((R)geq<double>(c, dr, u, v, w, usq), WQ[c]);
 }
 
+// Storage index of the main-lattice node at each (i, j, k) triple (1-based,
+// inside the box), or -1 where it is unlisted.
+__global__ void k_main_index(const int32_t *__restrict__ ijk, int n, const int32_t *__restrict__ domid,
+                             const int32_t *__restrict__ s_of_p, int jm, int km,
+                             int32_t *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const size_t q = ((size_t)(ijk[3 * e] - 1) * jm + (ijk[3 * e + 1] - 1)) * km + (ijk[3 * e + 2] - 1);
+  const int d = domid[q];
+  out[e] = d > 0 ? s_of_p[d - 1] : -1;
+}
+
 template <class F>
 int by_store(int storage, F &&f) {
   switch (storage) {
@@ -217,55 +229,73 @@ static int download_ints(zf_ctx *ctx, std::vector<int32_t> &h, const int32_t *d,
 // by the parents of one main lattice; each parent's table names its slots.
 static int link_tables(zf_ctx *pc, zf_ctx *mc, std::vector<int32_t> &slot_of_main_s) {
   const int nb = pc->bcmax, p = pc->parent_p;
-  std::vector<int32_t> bcl, bcn, vb, pdomid, psop, mdomid, msop;
+  std::vector<int32_t> bcl, bcn, vb, pdomid, psop;
   int rc;
   if ((rc = download_ints(pc, bcl, pc->d_bclist, 4 * (size_t)nb))) return rc;
   if ((rc = download_ints(pc, bcn, pc->d_bcnlist, 3 * (size_t)nb))) return rc;
   if ((rc = download_ints(pc, vb, pc->d_vbcn, 3 * (size_t)nb))) return rc;
   if ((rc = download_ints(pc, pdomid, pc->d_domid, pc->box))) return rc;
   if ((rc = download_ints(pc, psop, pc->d_s_of_p, pc->dommax))) return rc;
-  if ((rc = download_ints(mc, mdomid, mc->d_domid, mc->box))) return rc;
-  if ((rc = download_ints(mc, msop, mc->d_s_of_p, mc->dommax))) return rc;
   const zf_config &P = pc->cfg, &M = mc->cfg;
   auto pq = [&](int i, int j, int k) { return ((size_t)(i - 1) * P.jm + (j - 1)) * P.km + (k - 1); };
-  auto main_s = [&](int i, int j, int k, int l) -> int {
-    if (i < 1 || i > M.im || j < 1 || j > M.jm || k < 1 || k > M.km) {
-      fail(pc, ZF_ERR_GEOMETRY, "parent BC entry %d maps to (%d,%d,%d), outside the main box", l + 1,
-           i, j, k);
-      return -1;
-    }
-    const int d = mdomid[((size_t)(i - 1) * M.jm + (j - 1)) * M.km + (k - 1)];
-    if (d <= 0) {
-      fail(pc, ZF_ERR_GEOMETRY, "parent BC entry %d maps to (%d,%d,%d), not a listed main node", l + 1,
-           i, j, k);
-      return -1;
-    }
-    return msop[d - 1];
-  };
+  // coupled entries and the main node each one reads
   std::vector<ParentBc> tab;
+  std::vector<int32_t> ijk, row;
   for (int l = 0; l < nb; l++) {
     const int code = bcl[4 * l + 3];
     if (code != 10 + p && code != 20 + p) continue;
     if (bcn[3 * l] == 0) continue;  // no neighbour: left alone, as the reference's stale entry
     ParentBc e{};
     e.s = psop[pdomid[pq(bcl[4 * l], bcl[4 * l + 1], bcl[4 * l + 2])] - 1];
-    int ms;
+    const int32_t *m3 = code == 20 + p ? &vb[3 * l] : &bcn[3 * l];
     if (code == 20 + p) {
       e.sm = psop[pdomid[pq(bcn[3 * l], bcn[3 * l + 1], bcn[3 * l + 2])] - 1];
       e.kind = e.sm < pc->n_int ? 0 : 1;
-      ms = main_s(vb[3 * l], vb[3 * l + 1], vb[3 * l + 2], l);
     } else {
       e.sm = -1;
       e.kind = 2;
-      ms = main_s(bcn[3 * l], bcn[3 * l + 1], bcn[3 * l + 2], l);
     }
-    if (ms < 0) return ZF_ERR_GEOMETRY;
-    if (slot_of_main_s[ms] < 0) {
-      slot_of_main_s[ms] = (int32_t)mc->h_export.size();
-      mc->h_export.push_back(ms << 1 | (ms < mc->n_int ? 1 : 0));
-    }
-    e.slot = slot_of_main_s[ms];
+    if (m3[0] < 1 || m3[0] > M.im || m3[1] < 1 || m3[1] > M.jm || m3[2] < 1 || m3[2] > M.km)
+      return fail(pc, ZF_ERR_GEOMETRY, "parent BC entry %d maps to (%d,%d,%d), outside the main box",
+                  l + 1, m3[0], m3[1], m3[2]);
+    ijk.insert(ijk.end(), m3, m3 + 3);
+    row.push_back(l);
     tab.push_back(e);
+  }
+  // their main storage indices, gathered on the device
+  const int n = (int)tab.size();
+  std::vector<int32_t> ms(n);
+  if (n) {
+    int32_t *d_ijk = nullptr, *d_out = nullptr;
+    if ((rc = dev_alloc(pc, (void **)&d_ijk, 12 * (size_t)n))) return rc;
+    if ((rc = dev_alloc(pc, (void **)&d_out, 4 * (size_t)n))) {
+      dev_free(pc, (void **)&d_ijk);
+      return rc;
+    }
+    hipError_t err = hipMemcpy(d_ijk, ijk.data(), 12 * (size_t)n, hipMemcpyHostToDevice);
+    if (err == hipSuccess) {
+      hipLaunchKernelGGL(k_main_index, dim3(nblk(n)), dim3(256), 0, mc->stream, d_ijk, n, mc->d_domid,
+                         mc->d_s_of_p, M.jm, M.km, d_out);
+      err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(mc->stream);
+    if (err == hipSuccess) err = hipMemcpy(ms.data(), d_out, 4 * (size_t)n, hipMemcpyDeviceToHost);
+    dev_free(pc, (void **)&d_ijk);
+    dev_free(pc, (void **)&d_out);
+    if (err != hipSuccess)
+      return fail(pc, ZF_ERR_HIP, "parent link gather failed: %s", hipGetErrorString(err));
+  }
+  for (int e = 0; e < n; e++) {
+    if (ms[e] < 0) {
+      const int32_t *m3 = &ijk[3 * (size_t)e];
+      return fail(pc, ZF_ERR_GEOMETRY, "parent BC entry %d maps to (%d,%d,%d), not a listed main node",
+                  row[e] + 1, m3[0], m3[1], m3[2]);
+    }
+    if (slot_of_main_s[ms[e]] < 0) {
+      slot_of_main_s[ms[e]] = (int32_t)mc->h_export.size();
+      mc->h_export.push_back(ms[e] << 1 | (ms[e] < mc->n_int ? 1 : 0));
+    }
+    tab[e].slot = slot_of_main_s[ms[e]];
   }
   dev_free(pc, (void **)&pc->d_pbc);
   pc->n_pbc = (int32_t)tab.size();
