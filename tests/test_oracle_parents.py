@@ -129,3 +129,25 @@ def test_parent_boundary_pass_against_brute_force():
         np.testing.assert_allclose(op.populations[:, q], want, rtol=0, atol=1e-15)
         checked += 1
     assert checked == 2 * int((plane > 0).sum())
+
+
+def test_phase_split_equals_the_whole_step():
+    """zfo_lbm_phase (0, 1, 2 in turn) is zfo_lbm_step, bit for bit."""
+    from tests import geometry
+    from zebraflow_amd.engine import lattice_config, reference_config
+    from tests.test_gpu_lbm import oracle_params
+    for cfg, b in ((reference_config(20, 18, 18), geometry.tube((20, 18, 18), axis=0, r=6.0,
+                                                                 inlet=1001, outlet=1003)),
+                   (lattice_config(16, 16, 24, 0.8, periodic=(0, 0, 1), body_force=(0, 0, 1e-4)),
+                    geometry.tube((16, 16, 24), axis=2, r=6.5, periodic=True))):
+        a, c = po.LbmOracle(b, oracle_params(cfg)), po.LbmOracle(b, oracle_params(cfg))
+        rng = np.random.default_rng(7)
+        f0 = po.feq(1 + 0.002 * rng.standard_normal(a.n), 0.01 * rng.standard_normal((3, a.n)))
+        a.set_populations(f0)
+        c.set_populations(f0)
+        for it in range(1, 6):
+            a.step(it)
+            for ph in (0, 1, 2):
+                c.phase(ph, it)
+        np.testing.assert_array_equal(a.populations, c.populations)
+        np.testing.assert_array_equal(a.U, c.U)
