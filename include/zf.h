@@ -172,8 +172,8 @@ int zf_download_parent_lists(zf_ctx *ctx, int32_t *vbcn);
  * every zf_lbm_step of main then exports the moments of the nodes the
  * parent reads, and every zf_lbm_step of the parent ends with its boundary
  * pass (:6436-6594) for codes 10+p and 20+p; walls stay with its stream
- * kernel.  Step main first, then its parents, in each iteration; the two
- * streams are ordered with events.  Fails with ZF_ERR_GEOMETRY if a coupled
+ * kernel.  Step main first, then its parents, in each iteration; while
+ * linked, a parent issues its work on the main context's stream.  Fails with ZF_ERR_GEOMETRY if a coupled
  * node maps outside the main box or onto an unlisted node.  Destroying or
  * re-uploading either context unlinks them.  The adjoint entry points do
  * not cover the coupling: linked contexts refuse zf_adjoint_setup and a

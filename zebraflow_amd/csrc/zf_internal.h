@@ -228,14 +228,13 @@ struct zf_ctx {
   zf_ctx *link_main = nullptr;     // main lattice this parent takes its boundary from
   zf::ParentBc *d_pbc = nullptr;   // coupled boundary nodes
   int32_t n_pbc = 0;
-  hipEvent_t couple_done = nullptr;  // after this parent's coupling pass
+  hipStream_t own_stream = nullptr;  // while linked: its own stream (it uses the main's)
   // main lattice: moments of the nodes its linked parents read, written
   // by every step after the stream kernel
   std::vector<zf_ctx *> linked;
   std::vector<int32_t> h_export;   // per slot: storage index << 1 | interior
   int32_t *d_export_s = nullptr;
   double *d_export = nullptr;      // 4 x n_export: rho, u, v, w
-  hipEvent_t export_done = nullptr;
 };
 
 namespace zf {

@@ -27,10 +27,12 @@
 // populations and the force the step used; a node the main stream kernel
 // does not update keeps its previous populations, whose moments are its
 // last macro recompute.  The results equal the reference's to rounding.
-// The two contexts' streams are ordered with events: the parent's coupling
-// waits for the main's export of the step, and the main's next export waits
-// for every linked parent's coupling.  Call order per iteration: the main
-// lattice's step, then its parents' steps.
+// While linked, a parent issues its work on the main context's stream, so
+// the one stream orders main step, export, parent step and coupling pass.
+// Separate streams ordered by events were measured slower: each event wait
+// cost more than the overlap it allowed (the parent's kernels are short
+// next to the main step).  Call order per iteration: the main lattice's
+// step, then its parents' steps.
 #include "zf_d3q19.h"
 
 #include <algorithm>
@@ -142,21 +144,24 @@ int by_store(int storage, F &&f) {
 }
 
 int nblk(int n) { return (n + 255) / 256; }
+// The export and coupling passes cover a few thousand nodes, each lane a
+// gather of 19 scattered population words: one wave per workgroup spreads
+// them over as many CUs as possible.
+constexpr int kGatherLanes = 64;
+int nwg(int n) { return (n + kGatherLanes - 1) / kGatherLanes; }
 
 }  // namespace
 
 int parent_export(zf_ctx *ctx, bool in_step) {
   const int n = (int)ctx->h_export.size();
   if (n == 0) return ZF_OK;
-  for (zf_ctx *p : ctx->linked)
-    if (p->couple_done) ZF_HIP(ctx, hipStreamWaitEvent(ctx->stream, p->couple_done, 0));
   const zf_config &c = ctx->cfg;
   by_store(c.storage, [&](auto tag) {
     constexpr int S = decltype(tag)::value;
     using T = typename Store<S>::T;
     // inside a step (before the buffer swap) d_f[cur] is its input and
     // d_f[cur ^ 1] its output; outside one, d_f[cur] is the state
-    hipLaunchKernelGGL(k_export<S>, dim3(nblk(n)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_export<S>, dim3(nwg(n)), dim3(kGatherLanes), 0, ctx->stream,
                        static_cast<const T *>(ctx->d_f[ctx->cur ^ 1]),
                        static_cast<const T *>(ctx->d_f[ctx->cur]), ctx->stride, ctx->d_export_s, n,
                        in_step ? 0 : 1,
@@ -165,26 +170,23 @@ int parent_export(zf_ctx *ctx, bool in_step) {
     return 0;
   });
   ZF_HIP(ctx, hipGetLastError());
-  ZF_HIP(ctx, hipEventRecord(ctx->export_done, ctx->stream));
   return ZF_OK;
 }
 
 int parent_couple(zf_ctx *ctx) {
   zf_ctx *mc = ctx->link_main;
   if (!mc || ctx->n_pbc == 0) return ZF_OK;
-  ZF_HIP(ctx, hipStreamWaitEvent(ctx->stream, mc->export_done, 0));
   const int nexp = (int)mc->h_export.size();
   by_store(ctx->cfg.storage, [&](auto tag) {
     constexpr int S = decltype(tag)::value;
     using T = typename Store<S>::T;
-    hipLaunchKernelGGL(k_parent_bc<S>, dim3(nblk(ctx->n_pbc)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL(k_parent_bc<S>, dim3(nwg(ctx->n_pbc)), dim3(kGatherLanes), 0, ctx->stream,
                        static_cast<T *>(ctx->d_f[ctx->cur]),
                        static_cast<const T *>(ctx->d_f[ctx->cur ^ 1]), ctx->stride, ctx->d_pbc,
                        ctx->n_pbc, mc->d_export, nexp);
     return 0;
   });
   ZF_HIP(ctx, hipGetLastError());
-  ZF_HIP(ctx, hipEventRecord(ctx->couple_done, ctx->stream));
   return ZF_OK;
 }
 
@@ -195,18 +197,20 @@ static int rebuild_export(zf_ctx *mc);
 void free_parent(zf_ctx *ctx) {
   if (ctx->link_main) {
     zf_ctx *mc = ctx->link_main;
-    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(mc->stream);
+    ctx->stream = ctx->own_stream;  // back to its own stream
+    ctx->own_stream = nullptr;
     mc->linked.erase(std::remove(mc->linked.begin(), mc->linked.end(), ctx), mc->linked.end());
     ctx->link_main = nullptr;
     (void)rebuild_export(mc);
   }
   dev_free(ctx, (void **)&ctx->d_pbc);
   ctx->n_pbc = 0;
-  if (ctx->couple_done) (void)hipEventDestroy(ctx->couple_done);
-  ctx->couple_done = nullptr;
   // a main lattice: unlink its parents (they keep their lattices)
+  if (!ctx->linked.empty()) (void)hipStreamSynchronize(ctx->stream);
   for (zf_ctx *p : std::vector<zf_ctx *>(ctx->linked)) {
-    (void)hipStreamSynchronize(p->stream);
+    p->stream = p->own_stream;
+    p->own_stream = nullptr;
     p->link_main = nullptr;
     dev_free(p, (void **)&p->d_pbc);
     p->n_pbc = 0;
@@ -215,8 +219,6 @@ void free_parent(zf_ctx *ctx) {
   ctx->h_export.clear();
   dev_free(ctx, (void **)&ctx->d_export_s);
   dev_free(ctx, (void **)&ctx->d_export);
-  if (ctx->export_done) (void)hipEventDestroy(ctx->export_done);
-  ctx->export_done = nullptr;
 }
 
 static int download_ints(zf_ctx *ctx, std::vector<int32_t> &h, const int32_t *d, size_t n) {
@@ -370,8 +372,7 @@ int zf_parent_link(zf_ctx *parent, zf_ctx *main) {
     return fail(parent, ZF_ERR_STATE, "the adjoint does not cover parent-lattice coupling");
   free_parent(parent);
   ZF_HIP(parent, hipSetDevice(parent->device));
-  ZF_HIP(parent, hipEventCreateWithFlags(&parent->couple_done, hipEventDisableTiming));
-  if (!main->export_done) ZF_HIP(main, hipEventCreateWithFlags(&main->export_done, hipEventDisableTiming));
+  ZF_HIP(parent, hipStreamSynchronize(parent->stream));
   parent->link_main = main;
   main->linked.push_back(parent);
   int rc = rebuild_export(main);
@@ -382,6 +383,9 @@ int zf_parent_link(zf_ctx *parent, zf_ctx *main) {
     if (parent->err.empty()) parent->err = main->err;
     return rc;
   }
+  // the parent's work goes to the main lattice's stream while linked
+  parent->own_stream = parent->stream;
+  parent->stream = main->stream;
   return ZF_OK;
 }
 
