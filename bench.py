@@ -49,7 +49,7 @@ WORKLOADS = {
 
 
 def make_geometry(name):
-    from tests import geometry as g  # the product's C generators (libzfhost.so)
+    from zebraflow_amd import geometry as g
     w = WORKLOADS[name]
     if w["kind"] == "pair":
         return g.vessel_pair(w["box"], w["r_main"], w["r_conn"], w["amp"], w["n_conn"], w["seed"])
@@ -134,7 +134,7 @@ def form_errors(name, b, steps):
     """The three forms from the same equilibrium start: rel. L2 error of the
     FP32- and FP16-deviation forms' velocity and wall shear stress (zf_wss at
     every lumen wall-mesh node, mu/DX = 1) against the FP64 form."""
-    from tests import geometry as g
+    from zebraflow_amd import geometry as g
     wx, wt, wn = g.wall_mesh(b)
     nrm = node_normals(wx, wt, wn)
     ok = np.isfinite(nrm).all(axis=1)  # nodes of degenerate wall triangles only: no normal
@@ -190,7 +190,7 @@ def cells_leg(name, b, ncell, steps, warmup):
     work on every step.  Times whole zf_step iterations of the FP32 form;
     the FP64 form runs the same iterations for the membrane-shape error
     (node positions, relative to the mean node displacement)."""
-    from tests import geometry as g  # the product's C generators (libzfhost.so)
+    from zebraflow_amd import geometry as g
     from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
                                       lattice_repulsion_config)
     xyz, tris = g.rbc_mesh(radius=4.0)
@@ -242,7 +242,7 @@ def parent_leg(steps=500, warmup=50):
     (main step, then parent step with its boundary pass) by the host clock
     around whole loops, the parent's stream kernel by HIP events, and checks
     that the parent's copy plane carries the main plane's axial velocity."""
-    from tests import geometry as g  # the product's C generators (libzfhost.so)
+    from zebraflow_amd import geometry as g
     from zebraflow_amd.engine import Engine, lattice_config
     bm = g.tube((96, 96, 512), axis=2, r=40.0, periodic=True)
     kp = 121
@@ -312,7 +312,7 @@ def mixed_precision_leg(steps=3000, ncell=6000, interval=1, chunk=500):
     displacement, cells live in every form) of the FP16 and FP32 forms
     against the FP64 form and of FP16 against FP32, so that the growth of
     the differences along the run is recorded."""
-    from tests import geometry as g
+    from zebraflow_amd import geometry as g
     from zebraflow_amd.engine import (lattice_ibm_config, lattice_membrane_config,
                                       lattice_repulsion_config)
     name = "tree_cfg5_12"
@@ -403,7 +403,7 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
     faded to zero by a smoothstep over |cos dihedral| 0.97..0.99 instead of
     the reference's cut at 0.99, no anti-sliver threshold), whose forward map
     is C1, with the 642-node cells and with 162-node cells (radius 3)."""
-    from tests import geometry as g
+    from zebraflow_amd import geometry as g
     from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, Engine,
                                       lattice_config, lattice_ibm_config, lattice_membrane_config)
     es0, kb0, amp0, u0 = 5e-3, 2e-3, 0.5, 0.005

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from tests.test_gpu_wss import node_normals
 from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, STORE_F32DEV, STORE_F64,
                                   Engine, lattice_config)

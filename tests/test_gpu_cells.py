@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from tests.test_gpu_ibm import ibm_config, macro_of
 from tests.test_oracle_cells import membrane_geometry, two_cells, wall_case
 from tests.test_oracle_mesh import lattice_params

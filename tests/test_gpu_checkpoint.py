@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests import geometry
+from zebraflow_amd import geometry
 from tests.test_host_program import write_case
 from zebraflow_amd import PKG
 from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, STORE_F32DEV, STORE_F64,

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import bench
-from tests import geometry
+from zebraflow_amd import geometry
 from zebraflow_amd.engine import (FLAG_LINEAR_BLOCKS, FLAG_NT_POPULATIONS, STORE_F32DEV, STORE_F64,
                                   Engine, flag_cached_dirs, lattice_config)
 

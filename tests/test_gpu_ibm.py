@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from tests.test_oracle_mesh import lattice_params
 from zebraflow_amd.engine import STORE_F32DEV, STORE_F64, Engine, lattice_config, membrane_config
 from zebraflow_amd._abi import ZfIbmConfig

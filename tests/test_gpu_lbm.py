@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, STORE_F16DEV, STORE_F32DEV,
                                   STORE_F64, Engine, lattice_config, reference_config)
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from zebraflow_amd.engine import Engine, ZfError, lattice_config, reference_config
 
 pytestmark = pytest.mark.gpu

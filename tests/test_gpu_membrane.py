@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from tests.test_oracle_mesh import deformed, lattice_params
 from zebraflow_amd.engine import Engine, lattice_config, membrane_config
 

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 from tests.test_gpu_lbm import oracle_params
 from zebraflow_amd.engine import (STORE_F32DEV, STORE_F64, WALL_EQUILIBRIUM, WALL_HALFWAY_BB, Engine,
                                   ZfError, lattice_config)

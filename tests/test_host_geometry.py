@@ -1,7 +1,7 @@
 """Host-side geometry and reference-format I/O (CPU)."""
 import numpy as np
 
-from tests import geometry
+from zebraflow_amd import geometry
 from zebraflow_amd import load_host_library
 from oracle import pyoracle as po
 

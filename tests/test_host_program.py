@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests import geometry
+from zebraflow_amd import geometry
 from zebraflow_amd import PKG, load_host_library
 from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, STORE_F64, Engine,
                                   lattice_config, lattice_ibm_config, lattice_membrane_config,

@@ -60,7 +60,7 @@ def test_momentum_grows_by_dt_force():
 
 
 def test_mass_conserved_with_bounce_back():
-    from tests.geometry import tube
+    from zebraflow_amd.geometry import tube
     b = tube((6, 18, 18), axis=0, r=6.5, periodic=True)
     o = po.LbmOracle(b, lattice_params(0.8, periodic=(1, 0, 0), body_force=(1e-4, 0, 0)))
     interior = (o.bnode() > 0) & (o.bnode() < 10)

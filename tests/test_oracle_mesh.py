@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests import geometry
+from zebraflow_amd import geometry
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "mesh_zfish.npz")
 

@@ -83,7 +83,7 @@ def test_parent_boundary_pass_against_brute_force():
     node over their column at the overlap plane, its upstream nodes
     feq(D_p at their neighbour, U of that main node): both recomputed here by
     brute force over the box, independently of the oracle's lists."""
-    from tests import geometry
+    from zebraflow_amd import geometry
     from zebraflow_amd.engine import WALL_EQUILIBRIUM, lattice_config
     from tests.test_gpu_lbm import oracle_params
     bm = geometry.tube((16, 16, 24), axis=2, r=6.5, periodic=True)
@@ -133,7 +133,7 @@ def test_parent_boundary_pass_against_brute_force():
 
 def test_phase_split_equals_the_whole_step():
     """zfo_lbm_phase (0, 1, 2 in turn) is zfo_lbm_step, bit for bit."""
-    from tests import geometry
+    from zebraflow_amd import geometry
     from zebraflow_amd.engine import lattice_config, reference_config
     from tests.test_gpu_lbm import oracle_params
     for cfg, b in ((reference_config(20, 18, 18), geometry.tube((20, 18, 18), axis=0, r=6.0,

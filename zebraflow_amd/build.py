@@ -59,13 +59,19 @@ def build_gpu(verbose=False, force=True):
     hipcc = _hipcc()
     headers = [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
     headers.append(os.path.join(REPO, "include", "zf.h"))
-    objs = []
+    objs, cmds = [], []
     for src in HIP_SOURCES:
         s = os.path.join(CSRC, src)
         o = os.path.join(OBJ, src.replace(".hip", ".o"))
         if force or _stale(o, [s] + headers):
-            _run([hipcc, "-c", *HIPCC_FLAGS, s, "-o", o], verbose)
+            cmds.append([hipcc, "-c", *HIPCC_FLAGS, s, "-o", o])
         objs.append(o)
+    # one hipcc per translation unit, a few at a time
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
     lib = os.path.join(PKG, "libzfgpu.so")
     tmp = lib + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs], verbose)

@@ -1,7 +1,9 @@
-"""Small test geometries from the product's C generators (libzfhost.so)."""
+"""Python binding of the product's C geometry generators (libzfhost.so): tubes,
+vessel pairs, branching trees, RBC meshes, cell placement and wall meshes
+in the layouts the C ABI takes."""
 import numpy as np
 
-from zebraflow_amd import load_host_library
+from . import load_host_library
 
 
 def tube(shape, axis, r, periodic=False, inlet=0, outlet=0):
