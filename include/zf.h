@@ -329,6 +329,20 @@ int zf_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps);
 int zf_wss(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal, double mu_over_dx,
            double pgauge, double *wss, double *pressure, int32_t *valid);
 
+/* Allocation-free zf_wss: same arguments, kernel and results, with the
+ * points staged in a fixed device scratch of ZF_WSS_NOALLOC_MAX_POINTS
+ * points (84 bytes each, about 16 MB per device, held until the process
+ * ends) instead of buffers allocated per call; larger n fails with
+ * ZF_ERR_ARG.  It calls no hipMalloc/hipFree.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (output
+ * loops under HIP graph capture, serving loops that forbid allocation).
+ * Calls on one device share the scratch: overlapping calls on one device
+ * (other host threads, other contexts) race. */
+#define ZF_WSS_NOALLOC_MAX_POINTS 196608
+int zf_wss_noalloc(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal,
+                   double mu_over_dx, double pgauge, double *wss, double *pressure,
+                   int32_t *valid);
+
 /* ---- adjoint --------------------------------------------------------------
  * Reverse-mode gradients of a wall-shear-stress objective through the LBM
  * time step (the stream-collide of ZebraflowWT.c:6101-6145 with the

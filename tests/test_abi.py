@@ -59,3 +59,10 @@ def test_create_without_gpu_fails_cleanly():
     except ZfError:
         return
     raise AssertionError("zf_create succeeded without a GPU")
+
+
+def test_noalloc_limit_matches_header():
+    from zebraflow_amd.engine import WSS_NOALLOC_MAX_POINTS
+    text = open(os.path.join(REPO, "include", "zf.h")).read()
+    m = re.search(r"#define ZF_WSS_NOALLOC_MAX_POINTS (\d+)", text)
+    assert m and int(m.group(1)) == WSS_NOALLOC_MAX_POINTS

@@ -102,6 +102,7 @@ FUNCS = {
     "zf_repulsion": (C.c_int, [P]),
     "zf_download_wall_neighbour": (C.c_int, [P, P]),
     "zf_wss": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
+    "zf_wss_noalloc": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
     "zf_adjoint_setup": (C.c_int, [P, C.POINTER(ZfAdjointConfig), i64, P, P]),
     "zf_adjoint_forward": (C.c_int, [P, i32, i32, C.POINTER(f64)]),
     "zf_adjoint_backward": (C.c_int, [P, P]),

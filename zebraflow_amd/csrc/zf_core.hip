@@ -92,6 +92,10 @@ int zf_create(const zf_config *cfg, zf_ctx **out) {
       return ZF_ERR_HIP;
     }
   }
+  if (wss_scratch_init(ctx) != ZF_OK) {
+    zf_destroy(ctx);
+    return ZF_ERR_HIP;
+  }
   *out = ctx;
   return ZF_OK;
 }

@@ -268,6 +268,8 @@ struct GridGeom {
   double x0, y0, z0, dx;
 };
 GridGeom grid_geom(const zf_ctx *ctx);
+// Looks up the allocation-free WSS scratch of ctx's device (once per device).
+int wss_scratch_init(zf_ctx *ctx);
 int alloc_state(zf_ctx *ctx);
 size_t store_bytes(int storage);
 // Bytes of one element of the force field and of the IBM velocity and

@@ -16,6 +16,7 @@
 #include "zf_wss_dev.h"
 
 #include <cmath>
+#include <mutex>
 
 namespace zf {
 namespace {
@@ -24,6 +25,20 @@ using namespace d3q19;
 using namespace wssdev;
 
 inline unsigned nb(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+// Scratch of the allocation-free entry point zf_wss_noalloc: per point the
+// inputs (xyz, normal: 6 doubles), the outputs (wss, pressure: 4 doubles)
+// and the valid flag (4 bytes), 84 bytes, for up to ZF_WSS_NOALLOC_MAX_POINTS
+// points.  Each device holds its own copy for the life of the process; host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the device (the array's name is not a device address in host code).
+constexpr size_t kWssPointBytes = 84;
+constexpr size_t kWssScratchBytes = kWssPointBytes * ZF_WSS_NOALLOC_MAX_POINTS;
+__device__ __align__(256) unsigned char scratch[kWssScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
 
 template <int S>
 __global__ void k_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
@@ -74,7 +89,53 @@ __global__ void k_wss(const typename Store<S>::T *__restrict__ f, size_t stride,
   wss[3 * m + 2] = -mu_over_dx * (u2 - un * nz);
 }
 
+template <int S>
+void launch_wss(const zf_ctx *ctx, const WGrid &g, int n, const double *d_xyz, const double *d_nrm,
+                double mu_over_dx, double pgauge, double *d_wss, double *d_p, int32_t *d_valid) {
+  const dim3 grid(nb((size_t)n, 64)), blk(64);
+  hipLaunchKernelGGL(k_wss<S>, grid, blk, 0, ctx->stream,
+                     static_cast<const typename Store<S>::T *>(ctx->d_f[ctx->cur]), ctx->stride,
+                     ctx->d_b, ctx->d_sbox, g, n, d_xyz, d_nrm, mu_over_dx,
+                     ctx->cfg.pressure_scale, pgauge, d_wss, d_p, d_valid);
+}
+
+void launch_wss_any(const zf_ctx *ctx, const WGrid &g, int n, const double *d_xyz,
+                    const double *d_nrm, double mu_over_dx, double pgauge, double *d_wss,
+                    double *d_p, int32_t *d_valid) {
+  switch (ctx->cfg.storage) {
+    case ZF_STORE_F64:
+      launch_wss<ZF_STORE_F64>(ctx, g, n, d_xyz, d_nrm, mu_over_dx, pgauge, d_wss, d_p, d_valid);
+      break;
+    case ZF_STORE_F32DEV:
+      launch_wss<ZF_STORE_F32DEV>(ctx, g, n, d_xyz, d_nrm, mu_over_dx, pgauge, d_wss, d_p, d_valid);
+      break;
+    default:
+      launch_wss<ZF_STORE_F16DEV>(ctx, g, n, d_xyz, d_nrm, mu_over_dx, pgauge, d_wss, d_p, d_valid);
+  }
+}
+
+WGrid wss_grid(const zf_ctx *ctx) {
+  const GridGeom gp = grid_geom(ctx);
+  return WGrid{gp.x0, gp.y0, gp.z0, gp.dx, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km};
+}
+
 }  // namespace
+
+// Device address of the allocation-free scratch on ctx's device, looked up
+// once per device with that device current (zf_create calls this).
+int wss_scratch_init(zf_ctx *ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return fail(ctx, ZF_ERR_ARG, "device %d beyond the scratch table", ctx->device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[ctx->device]) return ZF_OK;
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(ctx, ZF_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_scratch_ptr[ctx->device] = ptr;
+  return ZF_OK;
+}
 
 GridGeom grid_geom(const zf_ctx *ctx) {
   if (ctx->have_ibm) return GridGeom{ctx->icfg.x0, ctx->icfg.y0, ctx->icfg.z0, ctx->icfg.dx};
@@ -104,31 +165,12 @@ int zf_wss(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal, doub
     return rc;
   }
   double *d_xyz = buf, *d_nrm = buf + 3 * n, *d_wss = buf + 6 * n, *d_p = buf + 9 * n;
-  const GridGeom gp = grid_geom(ctx);
-  const WGrid g{gp.x0, gp.y0, gp.z0, gp.dx, ctx->cfg.im, ctx->cfg.jm, ctx->cfg.km};
+  const WGrid g = wss_grid(ctx);
   hipError_t e = hipMemcpyAsync(d_xyz, xyz, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(d_nrm, normal, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) {
-    const void *f = ctx->d_f[ctx->cur];
-    const dim3 grid(nb((size_t)n, 64)), blk(64);
-    const double ps = ctx->cfg.pressure_scale;
-    switch (ctx->cfg.storage) {
-      case ZF_STORE_F64:
-        hipLaunchKernelGGL(k_wss<ZF_STORE_F64>, grid, blk, 0, ctx->stream, (const double *)f,
-                           ctx->stride, ctx->d_b, ctx->d_sbox, g, (int)n, d_xyz, d_nrm, mu_over_dx,
-                           ps, pgauge, d_wss, d_p, vb);
-        break;
-      case ZF_STORE_F32DEV:
-        hipLaunchKernelGGL(k_wss<ZF_STORE_F32DEV>, grid, blk, 0, ctx->stream, (const float *)f,
-                           ctx->stride, ctx->d_b, ctx->d_sbox, g, (int)n, d_xyz, d_nrm, mu_over_dx,
-                           ps, pgauge, d_wss, d_p, vb);
-        break;
-      default:
-        hipLaunchKernelGGL(k_wss<ZF_STORE_F16DEV>, grid, blk, 0, ctx->stream, (const _Float16 *)f,
-                           ctx->stride, ctx->d_b, ctx->d_sbox, g, (int)n, d_xyz, d_nrm, mu_over_dx,
-                           ps, pgauge, d_wss, d_p, vb);
-    }
+    launch_wss_any(ctx, g, (int)n, d_xyz, d_nrm, mu_over_dx, pgauge, d_wss, d_p, vb);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(wss, d_wss, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
@@ -139,6 +181,49 @@ int zf_wss(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal, doub
   dev_free(ctx, (void **)&buf);
   dev_free(ctx, (void **)&vb);
   if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "wall shear stress: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+// The allocation-free variant of zf_wss: the same kernel and results, with
+// the inputs and outputs staged in the file-scope scratch instead of
+// buffers allocated per call.  Calls on one device share that scratch, so
+// overlapping calls on one device (from several host threads or contexts)
+// race; calls on one context are ordered by its stream.  Every staged word
+// is written before it is read, so nothing is cleared between calls.
+int zf_wss_noalloc(zf_ctx *ctx, int64_t n, const double *xyz, const double *normal,
+                   double mu_over_dx, double pgauge, double *wss, double *pressure,
+                   int32_t *valid) {
+  if (!ctx || n < 0 || (n > 0 && (!xyz || !normal || !wss || !pressure || !valid)))
+    return ZF_ERR_ARG;
+  if (!ctx->have_lattice || !ctx->d_f[0]) return fail(ctx, ZF_ERR_STATE, "no lattice state");
+  if (n > ZF_WSS_NOALLOC_MAX_POINTS)
+    return fail(ctx, ZF_ERR_ARG, "zf_wss_noalloc: %lld wall points exceed its limit of %d",
+                (long long)n, (int)ZF_WSS_NOALLOC_MAX_POINTS);
+  if (n == 0) return ZF_OK;
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  }
+  if (!base) return fail(ctx, ZF_ERR_STATE, "zf_wss_noalloc: scratch of device %d not set up", ctx->device);
+  ZF_HIP(ctx, hipSetDevice(ctx->device));
+  // layout for this call's n: xyz, normal, wss (3n doubles each), pressure
+  // (n doubles), valid (n ints)
+  double *d_xyz = static_cast<double *>(base), *d_nrm = d_xyz + 3 * n, *d_wss = d_xyz + 6 * n,
+         *d_p = d_xyz + 9 * n;
+  int32_t *d_valid = reinterpret_cast<int32_t *>(d_xyz + 10 * n);
+  hipError_t e = hipMemcpyAsync(d_xyz, xyz, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_nrm, normal, 24 * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    launch_wss_any(ctx, wss_grid(ctx), (int)n, d_xyz, d_nrm, mu_over_dx, pgauge, d_wss, d_p, d_valid);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(wss, d_wss, 24 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pressure, d_p, 8 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(valid, d_valid, 4 * (size_t)n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "wall shear stress (no alloc): %s", hipGetErrorString(e));
   return ZF_OK;
 }
 

@@ -14,6 +14,7 @@ from ._abi import ZfConfig, ZfLatticeInfo
 STORE_F64, STORE_F32DEV, STORE_F16DEV = 0, 1, 2
 WALL_EQUILIBRIUM, WALL_HALFWAY_BB = 0, 1
 B_VELOCITY_INLET, B_PRESSURE_OUTLET = 2001, 3001
+WSS_NOALLOC_MAX_POINTS = 196608  # ZF_WSS_NOALLOC_MAX_POINTS of include/zf.h
 # zf_config.flags (include/zf.h)
 FLAG_LINEAR_BLOCKS, FLAG_NT_POPULATIONS, FLAG_F64_ARITH = 1, 2, 4
 
@@ -328,14 +329,17 @@ class Engine:
         return out
 
     # -- wall shear stress ------------------------------------------------------
-    def wss(self, xyz, normal, mu_over_dx, pgauge=0.0):
-        """(wss [n,3], pressure [n], valid [n]) at wall points."""
+    def wss(self, xyz, normal, mu_over_dx, pgauge=0.0, noalloc=False):
+        """(wss [n,3], pressure [n], valid [n]) at wall points; noalloc calls
+        the allocation-free zf_wss_noalloc instead of zf_wss."""
         xyz = np.ascontiguousarray(xyz, np.float64)
         normal = np.ascontiguousarray(normal, np.float64)
         n = len(xyz)
         out, pr, va = np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.int32)
-        self._check(self.lib.zf_wss(self.h, n, _ptr(xyz), _ptr(normal), mu_over_dx, pgauge,
-                                    _ptr(out), _ptr(pr), _ptr(va)), "zf_wss")
+        fn = self.lib.zf_wss_noalloc if noalloc else self.lib.zf_wss
+        name = "zf_wss_noalloc" if noalloc else "zf_wss"
+        self._check(fn(self.h, n, _ptr(xyz), _ptr(normal), mu_over_dx, pgauge,
+                       _ptr(out), _ptr(pr), _ptr(va)), name)
         return out, pr, va
 
     # -- adjoint ------------------------------------------------------------------
