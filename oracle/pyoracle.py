@@ -44,8 +44,8 @@ def lib():
         _lib.zfo_parent_bc_setup.restype = C.c_int
         _lib.zfo_feq.restype = C.c_double
         _lib.zfo_feq.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
-        _lib.zfo_bc_pressure.restype = C.c_double
-        _lib.zfo_bc_pressure.argtypes = [C.c_int, C.c_double]
+        _lib.zfo_bc_pressure.restype = C.c_float
+        _lib.zfo_bc_pressure.argtypes = [C.c_int, C.c_float]
         _lib.zfo_wtime.restype = C.c_double
         _lib.zfo_max_threads.restype = C.c_int
     return _lib

@@ -127,7 +127,7 @@ double zfo_feq(int c, double rho, double u, double v, double w);
 
 /* Pressure waveform of the BC codes 1001-1009, ZebraflowWT.c:6318-6421
  * (pressure in the reference's units, before /LBM_PRESSURE_SCALE). */
-double zfo_bc_pressure(int code, double simtime);
+float zfo_bc_pressure(int code, float simtime);
 
 /* ---- RBC membrane (zf_oracle_mesh.c) -------------------------------------- */
 typedef struct zfo_memb_params {

@@ -19,9 +19,14 @@ double zfo_feq(int c, double rho, double u, double v, double w) {
   return rho * ZFO_W[c] * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * (u * u + v * v + w * w));
 }
 
-/* Cardiac pressure waveforms of BC codes 1001-1009, ZebraflowWT.c:6318-6421.
- * Each code is (phase shift, systolic split, base, amplitude, shape). */
-double zfo_bc_pressure(int code, double simtime) {
+/* Cardiac pressure waveforms of BC codes 1001-1009, ZebraflowWT.c:6318-6421,
+ * with the reference's types: simtime, cycletime, taucycle and pressure are
+ * float (:294), the constants double, so every expression is evaluated in
+ * double and rounded to float at each assignment, and the sines are sinf.
+ * Each code is (phase shift, base, amplitude, shape); code 1001 has no shift
+ * (its fmod of a non-negative time needs no wrap, which the shared code
+ * below then never takes). */
+float zfo_bc_pressure(int code, float simtime) {
   const double P = 3.14159265359, T = 0.36;
   double shift = 0.2 * T, base, amp;
   int shape_a = 1; /* 1: sin^2 rise to 0.23 then cosine fall; 0: sin rise to 0.3 */
@@ -33,22 +38,23 @@ double zfo_bc_pressure(int code, double simtime) {
     case 1005: base = 23.0; amp = 27.0; shape_a = 0; break;
     case 1006: case 1008: base = 29.0; amp = 66.0; break;
     case 1007: case 1009: base = 28.5; amp = 60.0; break;
-    default: return 0.0;
+    default: return 0.0f;
   }
-  double cyc = fmod(simtime - shift, T);
-  if (cyc < 0) cyc += T;
-  const double tc = cyc / T;
-  double pr;
+  float cyc = (float)fmod(simtime - shift, T);
+  if (cyc < 0) cyc = (float)(cyc + T);
+  const float tc = (float)(cyc / T);
+  float pr;
   if (shape_a) {
     if (tc <= 0.23) {
-      const double s = sin(P * tc / 0.46);
-      pr = 120.0 + base + amp * s * s;
+      const float s = sinf((float)(P * tc / 0.46));
+      pr = (float)(120.0 + base + amp * s * s);
     } else {
-      pr = 120.0 + base + amp * (1.0 + sin(2 * P * (tc + 0.54) / 1.54 - P / 2.0)) / 2.0;
+      pr = (float)(120.0 + base +
+                   amp * (1.0 + sinf((float)(2 * P * (tc + 0.54) / 1.54 - P / 2.0))) / 2.0);
     }
   } else {
-    if (tc <= 0.3) pr = 120.0 + base + amp * sin(P * tc / 0.6);
-    else pr = 120.0 + base + amp * (1.0 + sin(2 * P * (tc + 0.4) / 1.4 - P / 2.0)) / 2.0;
+    if (tc <= 0.3) pr = (float)(120.0 + base + amp * sinf((float)(P * tc / 0.6)));
+    else pr = (float)(120.0 + base + amp * (1.0 + sinf((float)(2 * P * (tc + 0.4) / 1.4 - P / 2.0))) / 2.0);
   }
   return pr;
 }

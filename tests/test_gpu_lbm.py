@@ -172,3 +172,35 @@ def test_cache_policies_bit_identical(store):
             ref = f
         else:
             np.testing.assert_array_equal(f, ref, err_msg=f"flags {flags:#x}")
+
+
+@pytest.mark.parametrize("store", [STORE_F64, STORE_F32DEV])
+@pytest.mark.parametrize("iter0", [0, 150000, 165590, 300000, 359990, 700000, 1234560])
+def test_pressure_codes_over_the_cycle(store, iter0):
+    """All nine cardiac pressure codes (ZebraflowWT.c:6318-6427) on the inlet
+    and outlet planes of a tube with equilibrium walls, 12 steps from
+    iterations spread over the 0.36 s cycle (720,000 steps of 5e-7 s): rise
+    and fall of every shape, the joins at 23 % and 30 %, a cycle wrap, and a
+    late float time; the device against the oracle, whose waveforms are the
+    reference's to the bit."""
+    print(f"case pressure codes store {store} iter0 {iter0}", flush=True)
+    b = geometry.tube((18, 16, 16), axis=0, r=6.0, inlet=1001, outlet=1001)
+    plane = np.nonzero(b[0] == 1001)
+    for n, (j, k) in enumerate(zip(*plane)):
+        b[0, j, k] = 1001 + n % 9
+        b[-1, j, k] = 1001 + (n + 4) % 9
+    cfg = reference_config(*b.shape, storage=store)
+    ora = po.LbmOracle(b, oracle_params(cfg))
+    eng = Engine(cfg)
+    eng.upload_lattice(b)
+    rng = np.random.default_rng(iter0 + 11)
+    f0 = po.feq(1.04 + 0.002 * rng.standard_normal(ora.n), 1e-3 * rng.standard_normal((3, ora.n)))
+    ora.set_populations(f0)
+    eng.upload_populations(f0)
+    for it in range(iter0 + 1, iter0 + 13):
+        ora.step(it)
+        eng.step(it)
+    fg = eng.populations()
+    eng.close()
+    tol = 1e-13 if store == STORE_F64 else 2e-7
+    np.testing.assert_allclose(fg, ora.populations, rtol=0, atol=tol)

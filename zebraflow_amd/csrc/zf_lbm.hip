@@ -229,6 +229,32 @@ __global__ void k_export(const typename Store<S>::T *__restrict__ f, size_t stri
     fp[(size_t)c * n + p] = (double)Store<S>::ld(f, (size_t)c * stride + s, WQ[c]) + WQ[c];
 }
 
+// The stored words of every listed node in list order p, as they are
+// (no conversion): for restart files that continue bit-identically.
+template <class T>
+__global__ void k_export_raw(const T *__restrict__ f, size_t stride, int n,
+                             const int32_t *__restrict__ s_of_p, T *__restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+#pragma unroll
+  for (int c = 0; c < Q; c++) out[(size_t)c * n + p] = f[(size_t)c * stride + s];
+}
+
+template <class T>
+__global__ void k_import_raw(const T *__restrict__ in, int n, const int32_t *__restrict__ s_of_p,
+                             T *__restrict__ f0, T *__restrict__ f1, size_t stride) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int s = s_of_p[p];
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    const T v = in[(size_t)c * n + p];
+    f0[(size_t)c * stride + s] = v;
+    f1[(size_t)c * stride + s] = v;
+  }
+}
+
 template <int S>
 __global__ void k_init_eq(typename Store<S>::T *__restrict__ f0, typename Store<S>::T *__restrict__ f1,
                           size_t stride, int n, double rho, double ux, double uy, double uz) {
@@ -315,7 +341,7 @@ __global__ void k_sums_final(const double *__restrict__ part, int nblk, double *
 
 inline unsigned nblocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
-double bc_pressure(int code, double simtime);
+float bc_pressure(int code, float simtime);
 
 // Cache policy of the population arrays.  MI355X keeps a 256 MB last-level
 // cache (Infinity Cache) in front of HBM.  When both population buffers fit
@@ -429,8 +455,13 @@ int step(zf_ctx *ctx, int iter, int rbc_step, hipEvent_t *ev = nullptr) {
   }
 }
 
-// Cardiac pressure waveforms of codes 1001-1009 (ZebraflowWT.c:6318-6421).
-double bc_pressure(int code, double simtime) {
+// Cardiac pressure waveforms of codes 1001-1009 (ZebraflowWT.c:6318-6421),
+// host side, once per step: a rise (sin^2 to 23 % of the 0.36 s cycle, or sin
+// to 30 %) joined to a cosine fall, per code a phase shift, base and
+// amplitude.  Evaluated with the reference's types -- time, cycle time,
+// cycle fraction and pressure are float (ZebraflowWT.c:294), the constants
+// double, sinf -- so the pressure is the reference's to the bit.
+float bc_pressure(int code, float simtime) {
   const double P = 3.14159265359, T = 0.36;
   double shift = 0.2 * T, base = 0, amp = 0;
   bool sin2 = true;
@@ -442,20 +473,20 @@ double bc_pressure(int code, double simtime) {
     case 1005: base = 23.0; amp = 27.0; sin2 = false; break;
     case 1006: case 1008: base = 29.0; amp = 66.0; break;
     case 1007: case 1009: base = 28.5; amp = 60.0; break;
-    default: return 0.0;
+    default: return 0.0f;
   }
-  double cyc = std::fmod(simtime - shift, T);
-  if (cyc < 0) cyc += T;
-  const double tc = cyc / T;
+  float cyc = (float)std::fmod((double)simtime - shift, T);
+  if (cyc < 0) cyc = (float)(cyc + T);
+  const float tc = (float)(cyc / T);
   if (sin2) {
     if (tc <= 0.23) {
-      const double sv = std::sin(P * tc / 0.46);
-      return 120.0 + base + amp * sv * sv;
+      const float sv = sinf((float)(P * tc / 0.46));
+      return (float)(120.0 + base + amp * sv * sv);
     }
-    return 120.0 + base + amp * (1.0 + std::sin(2 * P * (tc + 0.54) / 1.54 - P / 2.0)) / 2.0;
+    return (float)(120.0 + base + amp * (1.0 + sinf((float)(2 * P * (tc + 0.54) / 1.54 - P / 2.0))) / 2.0);
   }
-  if (tc <= 0.3) return 120.0 + base + amp * std::sin(P * tc / 0.6);
-  return 120.0 + base + amp * (1.0 + std::sin(2 * P * (tc + 0.4) / 1.4 - P / 2.0)) / 2.0;
+  if (tc <= 0.3) return (float)(120.0 + base + amp * sinf((float)(P * tc / 0.6)));
+  return (float)(120.0 + base + amp * (1.0 + sinf((float)(2 * P * (tc + 0.4) / 1.4 - P / 2.0))) / 2.0);
 }
 
 template <class F>
@@ -492,7 +523,7 @@ void step_params(const zf_ctx *ctx, int iter, StepParams &prm, BcParams &bp) {
   for (int a = 0; a < 3; a++) bp.inlet[a] = uin * c.inlet_dir[a];
   bp.outlet_drho = c.outlet_rho - 1.0;
   for (int k = 0; k < 9; k++)
-    bp.pdrho[k] = bc_pressure(1001 + k, iter * c.t_scale) / c.pressure_scale;
+    bp.pdrho[k] = (double)bc_pressure(1001 + k, (float)(iter * c.t_scale)) / c.pressure_scale;
 }
 
 int step_into(zf_ctx *ctx, const void *src, void *dst, int iter) {
@@ -604,6 +635,61 @@ int zf_download_populations(zf_ctx *ctx, double *f) {
   ctx->device_bytes -= (int64_t)nbytes;
   dev_free(ctx, (void **)&tmp);
   if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "population download: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_download_populations_raw(zf_ctx *ctx, void *f, int64_t bytes) {
+  if (!ctx || !f) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  const size_t eb = store_bytes(ctx->cfg.storage), nbytes = eb * Q * (size_t)ctx->dommax;
+  if (bytes != (int64_t)nbytes)
+    return fail(ctx, ZF_ERR_ARG, "raw populations: %lld bytes given, %zu stored", (long long)bytes, nbytes);
+  void *tmp = nullptr;
+  if ((rc = dev_alloc(ctx, &tmp, nbytes))) return rc;
+  dispatch(ctx->cfg.storage, [&](auto tag) {
+    constexpr int S = decltype(tag)::value;
+    using T = typename Store<S>::T;
+    hipLaunchKernelGGL((k_export_raw<T>), dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                       static_cast<const T *>(ctx->d_f[ctx->cur]), ctx->stride, ctx->dommax,
+                       ctx->d_s_of_p, static_cast<T *>(tmp));
+    return ZF_OK;
+  });
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, nbytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)nbytes;
+  dev_free(ctx, &tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "raw population download: %s", hipGetErrorString(e));
+  return ZF_OK;
+}
+
+int zf_upload_populations_raw(zf_ctx *ctx, const void *f, int64_t bytes) {
+  if (!ctx || !f) return ZF_ERR_ARG;
+  int rc = require_state(ctx);
+  if (rc) return rc;
+  const size_t eb = store_bytes(ctx->cfg.storage), nbytes = eb * Q * (size_t)ctx->dommax;
+  if (bytes != (int64_t)nbytes)
+    return fail(ctx, ZF_ERR_ARG, "raw populations: %lld bytes given, %zu stored", (long long)bytes, nbytes);
+  void *tmp = nullptr;
+  if ((rc = dev_alloc(ctx, &tmp, nbytes))) return rc;
+  hipError_t e = hipMemcpyAsync(tmp, f, nbytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    dispatch(ctx->cfg.storage, [&](auto tag) {
+      constexpr int S = decltype(tag)::value;
+      using T = typename Store<S>::T;
+      hipLaunchKernelGGL((k_import_raw<T>), dim3(nblocks(ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                         static_cast<const T *>(tmp), ctx->dommax, ctx->d_s_of_p,
+                         static_cast<T *>(ctx->d_f[0]), static_cast<T *>(ctx->d_f[1]), ctx->stride);
+      return ZF_OK;
+    });
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)nbytes;
+  dev_free(ctx, &tmp);
+  ctx->cur = 0;
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "raw population upload: %s", hipGetErrorString(e));
   return ZF_OK;
 }
 
