@@ -129,6 +129,15 @@ int zf_get_lattice_info(zf_ctx *ctx, zf_lattice_info *info);
 int zf_init_equilibrium(zf_ctx *ctx, double rho, const double u[3]);
 int zf_upload_populations(zf_ctx *ctx, const double *f);
 int zf_download_populations(zf_ctx *ctx, double *f);
+/* The populations as stored, without conversion, in the host layout
+ * [c*dommax + p]: double f (ZF_STORE_F64), float f - w (ZF_STORE_F32DEV) or
+ * half (f - w)/w (ZF_STORE_F16DEV); bytes must be 19*dommax times 8, 4 or
+ * 2.  The double layout above rounds the FP32 deviations of f near w (below
+ * about 5e-10); these keep every bit, for restart files that continue
+ * bit-identically.  No counterpart in the reference, which restarts from
+ * its macroscopic output (ZebraflowWT.c:5253-5262). */
+int zf_download_populations_raw(zf_ctx *ctx, void *f, int64_t bytes);
+int zf_upload_populations_raw(zf_ctx *ctx, const void *f, int64_t bytes);
 /* Per-node force field BODYFORCE (ZebraflowWT.c:183), 3*dommax SoA; NULL
  * clears it. */
 int zf_upload_force(zf_ctx *ctx, const double *force);

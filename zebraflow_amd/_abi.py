@@ -66,6 +66,8 @@ FUNCS = {
     "zf_init_equilibrium": (C.c_int, [P, f64, P]),
     "zf_upload_populations": (C.c_int, [P, P]),
     "zf_download_populations": (C.c_int, [P, P]),
+    "zf_download_populations_raw": (C.c_int, [P, P, i64]),
+    "zf_upload_populations_raw": (C.c_int, [P, P, i64]),
     "zf_upload_force": (C.c_int, [P, P]),
     "zf_download_force": (C.c_int, [P, P]),
     "zf_lbm_step": (C.c_int, [P, i32, i32]),
@@ -147,6 +149,8 @@ HOST_FUNCS = {
     "zfh_write_cells_csv": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P, P, P]),
     "zfh_append_volume_area": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P]),
     "zfh_membrane_config_lattice": (None, [P, i32, P, i32, f64, f64, i32, P]),
+    "zfh_save_state": (C.c_int, [P, C.c_char_p, i32, i32, i32]),
+    "zfh_load_state": (C.c_int, [P, C.c_char_p, C.POINTER(i32), i32, i32]),
 }
 
 

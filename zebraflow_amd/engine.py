@@ -144,6 +144,34 @@ class Engine:
         self._check(self.lib.zf_download_populations(self.h, _ptr(f)), "zf_download_populations")
         return f
 
+    def populations_raw(self):
+        """The populations as stored ([19, dommax] of float64, float32 or
+        float16 by the storage form), without conversion."""
+        dt = {STORE_F64: np.float64, STORE_F32DEV: np.float32}.get(self.cfg.storage, np.float16)
+        f = np.zeros((19, self.dommax), dt)
+        self._check(self.lib.zf_download_populations_raw(self.h, _ptr(f), f.nbytes),
+                    "zf_download_populations_raw")
+        return f
+
+    def upload_populations_raw(self, f):
+        f = np.ascontiguousarray(f)
+        self._check(self.lib.zf_upload_populations_raw(self.h, _ptr(f), f.nbytes),
+                    "zf_upload_populations_raw")
+
+    def save_state(self, path, iter_, ncell=0, nn=0):
+        """Restart file of libzfhost.so (zfh_save_state)."""
+        from . import load_host_library
+        self._check(load_host_library().zfh_save_state(self.h, str(path).encode(), iter_, ncell, nn),
+                    "zfh_save_state")
+
+    def load_state(self, path, ncell=0, nn=0):
+        import ctypes as C
+        from . import load_host_library
+        it = C.c_int32()
+        self._check(load_host_library().zfh_load_state(self.h, str(path).encode(), C.byref(it),
+                                                       ncell, nn), "zfh_load_state")
+        return it.value
+
     def upload_force(self, force):
         if force is None:
             self._check(self.lib.zf_upload_force(self.h, None), "zf_upload_force")
