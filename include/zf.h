@@ -303,6 +303,35 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
                   const int32_t *wall_tri, const double *wall_normal);
 /* W_index over the box (1-based wall node or 0). */
 int zf_download_windex(zf_ctx *ctx, int32_t *windex);
+
+/* Wall-layer velocity Uvoid (built at ZebraflowWT.c:4788-4851, read by the
+ * interpolation at :8980-8996): every box node with W_index > 0, B <= 0 and
+ * k_lo <= k <= KM - k_hi carries Uvoid = w * speed * n, where w ramps
+ * linearly from 0 at Wdistance (the FP32 distance to its wall node) d0 to 1
+ * at d1, and n is the unit normal of the wall triangle, among those of its
+ * wall node in order, with the vertex or centroid nearest to the selection
+ * point (FP32 distances, strict <: vertices then centroid of each triangle).
+ * zf_ibm_interpolate then adds vel_scale * w_q * Uvoid_q over the window
+ * nodes that are not fluid (B <= 0 or B >= 4998) and have W_index > 0, with
+ * the same kernel as the fluid nodes (WallStencilWidth = StencilWidth = 2).
+ * The reference's selection point is not the node: it is whatever its
+ * variables xp, yp, zp held when the loop ran, the coordinates of the last
+ * row of the last start file read (PCV2parentdomainStart.csv, :3579);
+ * select = 1 with select_point reproduces that, select = 0 uses the node's
+ * own position.  Call after zf_wall_setup and zf_ibm_setup; NULL turns the
+ * term off.  The node transpose of zf_ibm_vjp and the adjoint include it. */
+typedef struct zf_uvoid_config {
+  double speed;            /* 1e-3 / VEL_SCALE (ZebraflowWT.c:4845), units of U */
+  double d0, d1;           /* 0.7, 3.0: ramp of Wdistance (coordinate units) */
+  int32_t k_lo, k_hi;      /* nodes k_lo <= k <= KM - k_hi: 3, 2 */
+  int32_t select;          /* 0: the node's own position, 1: select_point */
+  int32_t reserved;
+  double select_point[3];  /* the reference's: 78.5, 214.5, 392.5 */
+} zf_uvoid_config;
+void zf_uvoid_config_reference(zf_uvoid_config *uc);
+int zf_uvoid_setup(zf_ctx *ctx, const zf_uvoid_config *uc);
+/* Uvoid over the box, SoA uvw[a*box + q] (zeros where none). */
+int zf_download_uvoid(zf_ctx *ctx, double *uvw);
 /* Triangle cell list (EulNeigh, ZebraflowWT.c:5851-5900) from the triangle
  * centroids of the last membrane evaluation: bin[c*nt + t] = DOMID of the
  * triangle's bin (1-based) or 0.  Built as (bin, triangle) sorted keys, so

@@ -318,15 +318,47 @@ def ibm_params(shape, **kw):
     return p
 
 
-def ibm_interpolate(p, b, domid, uvw, xyz):
+def ibm_interpolate(p, b, domid, uvw, xyz, windex=None, uvoid=None):
+    """Interpolated node velocities; with windex and uvoid ([3, im, jm, km])
+    the wall layer's Uvoid terms are added (zfo_ibm_interpolate_wall)."""
     b = np.ascontiguousarray(b, np.int32)
     domid = np.ascontiguousarray(domid, np.int32)
     u = [np.ascontiguousarray(a, np.float64) for a in uvw]
     x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
     v = np.zeros_like(x)
-    lib().zfo_ibm_interpolate(C.byref(p), _p(b), _p(domid), _p(u[0]), _p(u[1]), _p(u[2]),
-                              C.c_int64(len(x)), _p(x), _p(v))
+    if uvoid is None:
+        lib().zfo_ibm_interpolate(C.byref(p), _p(b), _p(domid), _p(u[0]), _p(u[1]), _p(u[2]),
+                                  C.c_int64(len(x)), _p(x), _p(v))
+    else:
+        wi = np.ascontiguousarray(windex, np.int32)
+        uv = np.ascontiguousarray(uvoid, np.float64)
+        lib().zfo_ibm_interpolate_wall(C.byref(p), _p(b), _p(domid), _p(u[0]), _p(u[1]), _p(u[2]),
+                                       _p(wi), _p(uv), C.c_int64(len(x)), _p(x), _p(v))
     return v.reshape(np.shape(xyz))
+
+
+def windex_dist(p, wall_xyz):
+    wx = np.ascontiguousarray(wall_xyz, np.float64)
+    windex = np.zeros((p.im, p.jm, p.km), np.int32)
+    wdist = np.zeros((p.im, p.jm, p.km), np.float32)
+    lib().zfo_windex_dist(C.byref(p), len(wx), _p(wx), _p(windex), _p(wdist))
+    return windex, wdist
+
+
+def uvoid(p, b, windex, wdist, wall_xyz, wall_tri, wall_normal, wall_node_tri, speed=1e-3,
+          d0=0.7, d1=3.0, k_lo=3, k_hi=2, select=1, point=(78.5, 214.5, 392.5)):
+    out = np.zeros((3, p.im, p.jm, p.km))
+    sp = np.ascontiguousarray(point, np.float64)
+    lib().zfo_uvoid(C.byref(p), _p(np.ascontiguousarray(b, np.int32)),
+                    _p(np.ascontiguousarray(windex, np.int32)),
+                    _p(np.ascontiguousarray(wdist, np.float32)),
+                    _p(np.ascontiguousarray(wall_xyz, np.float64)),
+                    _p(np.ascontiguousarray(wall_tri, np.int32)),
+                    _p(np.ascontiguousarray(wall_normal, np.float64)),
+                    _p(np.ascontiguousarray(wall_node_tri, np.int32)), C.c_double(speed),
+                    C.c_double(d0), C.c_double(d1), int(k_lo), int(k_hi), int(select), _p(sp),
+                    _p(out))
+    return out
 
 
 def ibm_spread(p, b, domid, dommax, xyz, ffluid, force=None):

@@ -179,6 +179,13 @@ typedef struct zfo_ibm_params {
 void zfo_ibm_interpolate(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
                          const double *U, const double *V, const double *W, int64_t nnode,
                          const double *xyz, double *vel);
+/* zfo_ibm_interpolate plus the wall layer: window nodes that are not fluid
+ * and have W_index > 0 add w * vel_scale * Uvoid (ZebraflowWT.c:8980-8996);
+ * windex and uvoid (3*box SoA) may be NULL. */
+void zfo_ibm_interpolate_wall(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
+                              const double *U, const double *V, const double *W,
+                              const int32_t *windex, const double *uvoid, int64_t nnode,
+                              const double *xyz, double *vel);
 /* a16.  ffluid: nnode*3; force: 3*dommax, accumulated (+=). */
 void zfo_ibm_spread(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
                     int64_t dommax, int64_t nnode, const double *xyz, const double *ffluid,
@@ -212,6 +219,18 @@ void zfo_cell_repulsion(const zfo_ibm_params *p, const zfo_rep_params *r, const 
                         double *vrep);
 void zfo_wall_node_tri(int nwn, int nwt, const int32_t *wall_tri, int32_t *wall_node_tri);
 void zfo_windex(const zfo_ibm_params *p, int nwn, const double *wall_xyz, int32_t *windex);
+/* zfo_windex with Wdistance, the FP32 distance to the chosen wall node
+ * (5000 where none), box-sized. */
+void zfo_windex_dist(const zfo_ibm_params *p, int nwn, const double *wall_xyz, int32_t *windex,
+                     float *wdist);
+/* Wall-layer velocity Uvoid over the box (ZebraflowWT.c:4788-4851), SoA
+ * uvoid[a*box + q]: at W_index > 0, B <= 0, k_lo <= k <= KM - k_hi,
+ * w(Wdistance) speed n(triwall), triwall the wall node's triangle nearest
+ * to the selection point (select: 0 the node, 1 the point sp). */
+void zfo_uvoid(const zfo_ibm_params *p, const int32_t *b, const int32_t *windex, const float *wdist,
+               const double *wall_xyz, const int32_t *wall_tri, const double *wall_normal,
+               const int32_t *wall_node_tri, double speed, double d0, double d1, int k_lo, int k_hi,
+               int select, const double *sp, double *uvoid);
 /* a14.  Adds to vrep; nearest: ncell*nn wall triangle (1-based) or 0. */
 void zfo_wall_repulsion(const zfo_ibm_params *p, const zfo_rep_params *r, const int32_t *windex,
                         int nwn, const double *wall_xyz, int nwt, const int32_t *wall_tri,

@@ -165,6 +165,13 @@ void zfo_wall_node_tri(int nwn, int nwt, const int32_t *wall_tri, int32_t *wall_
 void zfo_windex(const zfo_ibm_params *p, int nwn, const double *wall_xyz, int32_t *windex) {
   const size_t box = (size_t)p->im * p->jm * p->km;
   float *wd = malloc(sizeof(float) * box);
+  zfo_windex_dist(p, nwn, wall_xyz, windex, wd);
+  free(wd);
+}
+
+void zfo_windex_dist(const zfo_ibm_params *p, int nwn, const double *wall_xyz, int32_t *windex,
+                     float *wd) {
+  const size_t box = (size_t)p->im * p->jm * p->km;
   for (size_t q = 0; q < box; q++) { wd[q] = 5000.0f; windex[q] = 0; }
   for (int l = 1; l <= nwn; l++) {
     const double *w = &wall_xyz[3 * (size_t)(l - 1)];
@@ -180,7 +187,56 @@ void zfo_windex(const zfo_ibm_params *p, int nwn, const double *wall_xyz, int32_
           if (d < wd[q]) { wd[q] = d; windex[q] = l; }
         }
   }
-  free(wd);
+}
+
+/* ZebraflowWT.c:4788-4851, serially in (i, j, k) order over k = k_lo ..
+ * KM - k_hi.  The reference's float coordinates: distances from the
+ * selection point to the FP32 wall node coordinates and to the FP32
+ * centroid (x1 + x2 + x3)/3 (:4660), strict <, mindis from 5000; the ramp
+ * weight1 of its FP32 Wdistance. */
+void zfo_uvoid(const zfo_ibm_params *p, const int32_t *b, const int32_t *windex, const float *wdist,
+               const double *wall_xyz, const int32_t *wall_tri, const double *wall_normal,
+               const int32_t *wall_node_tri, double speed, double d0, double d1, int k_lo, int k_hi,
+               int select, const double *sp, double *uvoid) {
+  const size_t box = (size_t)p->im * p->jm * p->km;
+  memset(uvoid, 0, sizeof(double) * 3 * box);
+  for (int i = 1; i <= p->im; i++)
+    for (int j = 1; j <= p->jm; j++)
+      for (int k = k_lo; k <= p->km - k_hi; k++) {
+        const size_t q = bq(i, j, k, p->jm, p->km);
+        if (windex[q] <= 0 || b[q] > 0) continue;
+        float xp, yp, zp;
+        if (select) { xp = (float)sp[0]; yp = (float)sp[1]; zp = (float)sp[2]; }
+        else {
+          xp = (float)(p->x0 + (i - 1) * p->dx);
+          yp = (float)(p->y0 + (j - 1) * p->dx);
+          zp = (float)(p->z0 + (k - 1) * p->dx);
+        }
+        const int32_t *row = &wall_node_tri[13 * (size_t)(windex[q] - 1)];
+        const int cnt = row[0] < 12 ? row[0] : 12;
+        float mindis = 5000.0f;
+        int triwall = 0;
+        for (int l = 1; l <= cnt; l++) {
+          const int tri = row[l];
+          float c[3] = {0, 0, 0};
+          for (int s = 0; s < 3; s++) {
+            const double *w = &wall_xyz[3 * (size_t)(wall_tri[3 * (size_t)(tri - 1) + s] - 1)];
+            const float x = (float)w[0], y = (float)w[1], z = (float)w[2];
+            const float d = sqrtf((xp - x) * (xp - x) + (yp - y) * (yp - y) + (zp - z) * (zp - z));
+            if (d < mindis) { mindis = d; triwall = tri; }
+            c[0] += x; c[1] += y; c[2] += z;
+          }
+          for (int a = 0; a < 3; a++) c[a] = (float)(c[a] / 3.);
+          const float d = sqrtf((xp - c[0]) * (xp - c[0]) + (yp - c[1]) * (yp - c[1]) +
+                                (zp - c[2]) * (zp - c[2]));
+          if (d < mindis) { mindis = d; triwall = tri; }
+        }
+        if (!triwall) continue;
+        const double wd = wdist[q];
+        const double w1 = wd <= d0 ? 0.0 : wd <= d1 ? (wd - d0) / (d1 - d0) : 1.0;
+        for (int a = 0; a < 3; a++)
+          uvoid[(size_t)a * box + q] = w1 * speed * wall_normal[3 * (size_t)(triwall - 1) + a];
+      }
 }
 
 void zfo_wall_repulsion(const zfo_ibm_params *p, const zfo_rep_params *r, const int32_t *windex,

@@ -37,7 +37,18 @@ static inline double ibm_weight(double xc, double yc, double zc, double xp, doub
 void zfo_ibm_interpolate(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
                          const double *U, const double *V, const double *W, int64_t nnode,
                          const double *xyz, double *vel) {
+  zfo_ibm_interpolate_wall(p, b, domid, U, V, W, NULL, NULL, nnode, xyz, vel);
+}
+
+/* The window's non-fluid nodes: B == 4998 reads Uvoid (:8980-8987), any
+ * other with W_index > 0 reads Uvoid with WallStencilWidth (:8988-8995),
+ * which equals StencilWidth (2, :147-148), so both take the same weight. */
+void zfo_ibm_interpolate_wall(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
+                              const double *U, const double *V, const double *W,
+                              const int32_t *windex, const double *uvoid, int64_t nnode,
+                              const double *xyz, double *vel) {
   const int im = p->im, jm = p->jm, km = p->km;
+  const size_t box = (size_t)im * jm * km;
   const double width = 2.0 * p->dx; /* StencilWidth*DX */
 #pragma omp parallel for schedule(static, 64)
   for (int64_t n = 0; n < nnode; n++) {
@@ -57,7 +68,14 @@ void zfo_ibm_interpolate(const zfo_ibm_params *p, const int32_t *b, const int32_
         for (int kk = k1; kk <= k2; kk++) {
           const size_t q = bq(ii, jj, kk, jm, km);
           const int bv = b[q];
-          if (!(bv > 0 && bv < 4998)) continue;
+          if (!(bv > 0 && bv < 4998)) {
+            if (!uvoid || !(bv == 4998 || windex[q] > 0)) continue;
+            const double w = ibm_weight(p->x0 + (ii - 1) * p->dx, p->y0 + (jj - 1) * p->dx,
+                                        p->z0 + (kk - 1) * p->dx, xp, yp, zp, width);
+            if (w == 0.0) continue;
+            for (int a = 0; a < 3; a++) v[a] += w * p->vel_scale * uvoid[(size_t)a * box + q];
+            continue;
+          }
           const double w = ibm_weight(p->x0 + (ii - 1) * p->dx, p->y0 + (jj - 1) * p->dx,
                                       p->z0 + (kk - 1) * p->dx, xp, yp, zp, width);
           if (w == 0.0) continue;

@@ -146,3 +146,51 @@ def test_full_step_matches_composed_oracle():
     np.testing.assert_allclose(fg, o.force, rtol=0, atol=2e-6 * scale)
     np.testing.assert_allclose(eng.populations(), o.populations, rtol=0, atol=1e-9)
     eng.close()
+
+
+@pytest.mark.parametrize("select", [0, 1])
+def test_uvoid_and_wall_layer_interpolation_match_oracle(select):
+    """The wall layer's Uvoid (ZebraflowWT.c:4788-4851) built on the device
+    equals the oracle's (pinned by brute force), and the interpolation of a
+    cell 0.3 lu from the wall, which reaches that layer, matches the
+    oracle's interpolation with the Uvoid terms (:8980-8996)."""
+    print(f"case uvoid, select {select}", flush=True)
+    b, wall, cells, tris = wall_case(0.3)
+    xyz_rest = geometry.rbc_mesh(radius=3.0, level=2)[0]
+    eng = engine(b, cells, tris, xyz_rest, wall=wall)
+    point = (30.0, -5.0, 7.5)
+    eng.uvoid_setup(speed=2e-3, select=select, select_point=point)
+    p = po.ibm_params(b.shape)
+    nwt, _ = po.wall_tables(p, wall[0], wall[1])
+    windex, wdist = po.windex_dist(p, wall[0])
+    uv = po.uvoid(p, b, windex, wdist, *wall, nwt, speed=2e-3, select=select, point=point)
+    np.testing.assert_array_equal(eng.uvoid(), uv)
+    assert np.abs(uv).max() > 1e-3
+    rng = np.random.default_rng(6)
+    domid, dl = po.dom_setup(b)
+    n = len(dl)
+    f0 = po.feq(1 + 0.001 * rng.standard_normal(n), 0.01 * rng.standard_normal((3, n)))
+    eng.upload_populations(f0)
+    eng.ibm_interpolate()
+    _, v, _ = eng.cells()
+    ref = po.ibm_interpolate(p, b, domid, macro_of(f0), cells, windex=windex, uvoid=uv)
+    plain = po.ibm_interpolate(p, b, domid, macro_of(f0), cells)
+    assert np.abs(ref - plain).max() > 1e-5  # the layer is reached
+    np.testing.assert_allclose(v, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    # the position transpose includes the layer's terms (central differences
+    # of the oracle's interpolation with them)
+    lamV = rng.standard_normal(cells.shape)
+    _, lx, _ = eng.ibm_vjp(lam_vel=lamV)
+    u = macro_of(f0)
+    J = lambda x: (lamV * po.ibm_interpolate(p, b, domid, u, x, windex=windex, uvoid=uv)).sum()
+    for trial in range(2):
+        d = rng.standard_normal(cells.shape)
+        h = 1e-6
+        fd = (J(cells + h * d) - J(cells - h * d)) / (2 * h)
+        np.testing.assert_allclose((lx * d).sum(), fd, rtol=1e-6)
+    # off again: the plain interpolation
+    eng.uvoid_setup(False)
+    eng.ibm_interpolate()
+    _, v0, _ = eng.cells()
+    np.testing.assert_allclose(v0, plain, rtol=0, atol=1e-12 * np.abs(plain).max())
+    eng.close()

@@ -263,3 +263,97 @@ def test_wall_repulsion_pushes_into_the_fluid(offset, pushes):
     assert (v[0, pushed, 1] < 0).mean() > 0.9
     # nodes far from the wall are not pushed
     assert not pushed[cells[0][:, 1] < cells[0][:, 1].max() - 2.0].any()
+
+
+# ---- the wall layer's Uvoid (ZebraflowWT.c:4788-4851, :8980-8996) ----------
+
+def brute_uvoid(b, wx, wt, wn, select, point, speed=1e-3, d0=0.7, d1=3.0, k_lo=3, k_hi=2):
+    """Uvoid by its rule, from the wall mesh alone: the nearest wall node of
+    every box node (FP32 distances, ties to the lower node, within the 9^3
+    window), its triangles in index order, the vertex-then-centroid FP32
+    distances to the selection point, the first minimum; the ramp of the FP32
+    distance to the nearest wall node."""
+    shape = b.shape
+    windex = brute_windex(shape, wx)
+    pos = node_xyz(shape).reshape(shape + (3,))
+    out = np.zeros((3,) + shape)
+    f32 = np.float32
+    for (i, j, k) in zip(*np.nonzero((windex > 0) & (b <= 0))):
+        if not (k_lo <= k + 1 <= shape[2] - k_hi):
+            continue
+        nw = windex[i, j, k]
+        wd = f32(np.sqrt(((wx[nw - 1] - pos[i, j, k]) ** 2).sum()))
+        P = np.asarray(point, f32) if select else pos[i, j, k].astype(f32)
+        tris = np.nonzero((wt == nw).any(axis=1))[0][:12]
+        cands, owner = [], []
+        for t in tris:
+            v = wx[wt[t] - 1].astype(f32)
+            for s in range(3):
+                cands.append(v[s])
+                owner.append(t)
+            c = ((v[0] + v[1]) + v[2]).astype(np.float64) / 3.0
+            cands.append(c.astype(f32))
+            owner.append(t)
+        d = np.array([np.sqrt(f32(((P - c) ** 2).sum(dtype=f32))) for c in cands], f32)
+        t = owner[int(np.argmin(d))]
+        wd = float(wd)
+        w1 = 0.0 if wd <= d0 else (wd - d0) / (d1 - d0) if wd <= d1 else 1.0
+        out[:, i, j, k] = w1 * speed * wn[t]
+    return out
+
+
+@pytest.mark.parametrize("select", [0, 1])
+def test_uvoid_against_brute_force(select):
+    b, (wx, wt, wn), _, _ = wall_case(0.5)
+    p = po.ibm_params(b.shape)
+    nwt, _ = po.wall_tables(p, wx, wt)
+    windex, wdist = po.windex_dist(p, wx)
+    np.testing.assert_array_equal(windex, brute_windex(b.shape, wx))
+    point = (30.0, -5.0, 7.5)
+    got = po.uvoid(p, b, windex, wdist, wx, wt, wn, nwt, select=select, point=point)
+    want = brute_uvoid(b, wx, wt, wn, select, point)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-18)
+    # a layer of solid nodes carries it, ramping with the wall distance
+    mag = np.linalg.norm(got, axis=0)
+    assert (mag > 0).sum() > 200 and mag.max() == pytest.approx(1e-3)
+    assert ((mag > 0) & (b > 0)).sum() == 0
+    # it points into the lumen: towards the axis of the tube
+    pos = node_xyz(b.shape).reshape(b.shape + (3,))
+    radial = pos[..., 1:] - np.array([10.5, 10.5])
+    inward = -(got[1] * radial[..., 0] + got[2] * radial[..., 1])
+    assert (inward[mag > 0] > 0).mean() > 0.95
+
+
+def test_interpolation_with_uvoid_against_brute_force():
+    """A membrane node within 2 DX of the wall takes the fluid terms plus
+    w * Uvoid over the non-fluid window nodes with W_index > 0; checked by a
+    sum over the whole box."""
+    b, (wx, wt, wn), cell, _ = wall_case(0.3)
+    p = po.ibm_params(b.shape)
+    nwt, _ = po.wall_tables(p, wx, wt)
+    windex, wdist = po.windex_dist(p, wx)
+    uv = po.uvoid(p, b, windex, wdist, wx, wt, wn, nwt, select=0)
+    domid, dl = po.dom_setup(b)
+    rng = np.random.default_rng(8)
+    u = 0.01 * rng.standard_normal((3, len(dl)))
+    x = cell[0]
+    got = po.ibm_interpolate(p, b, domid, u, x, windex=windex, uvoid=uv)
+    plain = po.ibm_interpolate(p, b, domid, u, x)
+    pos = node_xyz(b.shape)
+    bf = b.reshape(-1)
+    fluid = (bf > 0) & (bf < 4998)
+    wall = ~fluid & (windex.reshape(-1) > 0)
+    ufull = np.zeros((3, bf.size))
+    idx = domid.reshape(-1)
+    ufull[:, fluid] = u[:, idx[fluid] - 1]
+    ufull[:, wall] = uv.reshape(3, -1)[:, wall]
+    want = np.zeros_like(x)
+    for n, xp in enumerate(x):
+        d = pos - xp
+        r = np.sqrt((d ** 2).sum(1))
+        w = np.prod(1 + np.cos(np.pi * np.abs(d) / 2.0), axis=1) / 64.0
+        w[(r > 2.0) | ~(fluid | wall)] = 0.0
+        want[n] = ufull @ w
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+    # the wall term matters for this cell
+    assert np.abs(got - plain).max() > 1e-5

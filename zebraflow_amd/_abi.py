@@ -45,6 +45,11 @@ class ZfRepulsionConfig(C.Structure):
                                    "wall_dmax", "wall_search")]
 
 
+class ZfUvoidConfig(C.Structure):
+    _fields_ = [("speed", f64), ("d0", f64), ("d1", f64), ("k_lo", i32), ("k_hi", i32),
+                ("select", i32), ("reserved", i32), ("select_point", f64 * 3)]
+
+
 class ZfAdjointConfig(C.Structure):
     _fields_ = [("checkpoint_interval", i32), ("max_steps", i32), ("t_begin", i32), ("t_end", i32),
                 ("mu_over_dx", f64), ("cells", i32), ("reserved", i32)]
@@ -104,6 +109,9 @@ FUNCS = {
     "zf_repulsion": (C.c_int, [P]),
     "zf_download_wall_neighbour": (C.c_int, [P, P]),
     "zf_wss": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
+    "zf_uvoid_config_reference": (None, [C.POINTER(ZfUvoidConfig)]),
+    "zf_uvoid_setup": (C.c_int, [P, C.POINTER(ZfUvoidConfig)]),
+    "zf_download_uvoid": (C.c_int, [P, P]),
     "zf_wss_noalloc": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
     "zf_adjoint_setup": (C.c_int, [P, C.POINTER(ZfAdjointConfig), i64, P, P]),
     "zf_adjoint_forward": (C.c_int, [P, i32, i32, C.POINTER(f64)]),
@@ -150,6 +158,8 @@ HOST_FUNCS = {
     "zfh_append_volume_area": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P]),
     "zfh_membrane_config_lattice": (None, [P, i32, P, i32, f64, f64, i32, P]),
     "zfh_save_state": (C.c_int, [P, C.c_char_p, i32, i32, i32]),
+    "zfh_read_fluid_start": (i64, [i32, P, P, i32, i32, i32, f64, f64, f64, f64, i64, P, P, P, P, P]),
+    "zfh_init_from_macro": (C.c_int, [P, P, P, P, P, P]),
     "zfh_load_state": (C.c_int, [P, C.c_char_p, C.POINTER(i32), i32, i32]),
 }
 

@@ -124,8 +124,9 @@ template <int S>
 __global__ __launch_bounds__(128) void k_ibm_adj_node(
     const double *__restrict__ X, size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, const W<S> *__restrict__ uq,
-    size_t stride, AP p, const double *__restrict__ G, const double *__restrict__ lamV,
-    const double *__restrict__ lamF, double *__restrict__ lamG, double *__restrict__ lamX) {
+    size_t stride, AP p, const int32_t *__restrict__ vslot, const double *__restrict__ uvoid,
+    const double *__restrict__ G, const double *__restrict__ lamV, const double *__restrict__ lamF,
+    double *__restrict__ lamG, double *__restrict__ lamX) {
   // 8 lanes per membrane node: lane `sub` takes the window plane i - 3 + sub
   // (sub < 7), and a fixed-order shuffle reduction over the 8 lanes sums the
   // planes (deterministic); 8x the lanes of one lane per node, which left
@@ -152,12 +153,13 @@ __global__ __launch_bounds__(128) void k_ibm_adj_node(
           if (off_support(dxl, dyl, dzl, p.cube)) continue;
           const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
           const int bv = b[q];
-          if (bv < 0) continue;
-          const int s = sbox[q];
+          const int sl = (vslot && !(bv > 0 && bv < 4998)) ? vslot[q] : -1;
+          if (bv < 0 && sl < 0) continue;
+          const int s = bv >= 0 ? sbox[q] : -1;
           W<S> gr[3];
           const double w = (double)weight_grad((W<S>)dxl, (W<S>)dyl, (W<S>)dzl, gr);
           const double gw[3] = {(double)gr[0] * inv, (double)gr[1] * inv, (double)gr[2] * inv};
-          if (lamF) {  // spreading onto every B >= 0 node
+          if (lamF && bv >= 0) {  // spreading onto every B >= 0 node
             const double l0 = lamF[s], l1 = lamF[stride + s], l2 = lamF[2 * stride + s];
             lg0 += p.fs * w * l0;
             lg1 += p.fs * w * l1;
@@ -170,6 +172,12 @@ __global__ __launch_bounds__(128) void k_ibm_adj_node(
           if (lamV && bv > 0 && bv < 4998) {  // interpolation from fluid nodes
             const double lv = p.vs * (V0 * (double)uq[s] + V1 * (double)uq[stride + s] +
                                       V2 * (double)uq[2 * stride + s]);
+            lx0 += lv * gw[0];
+            lx1 += lv * gw[1];
+            lx2 += lv * gw[2];
+          } else if (lamV && sl >= 0) {  // the wall layer's fixed Uvoid
+            const double lv = p.vs * (V0 * uvoid[3 * (size_t)sl] + V1 * uvoid[3 * (size_t)sl + 1] +
+                                      V2 * uvoid[3 * (size_t)sl + 2]);
             lx0 += lv * gw[0];
             lx1 += lv * gw[1];
             lx2 += lv * gw[2];
@@ -392,7 +400,8 @@ int ibm_vjp_device(zf_ctx *ctx, const double *X, const void *f, const double *G,
                          static_cast<const T *>(f), ctx->stride, ctx->dommax, uq);
     hipLaunchKernelGGL(k_ibm_adj_node<S>, dim3(nb(8 * (size_t)n, 128)), dim3(128), 0, ctx->stream, X,
                        NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, uq,
-                       ctx->stride, p, G, lamV, lamF, lamG, lamX);
+                       ctx->stride, p, ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, G,
+                       lamV, lamF, lamG, lamX);
     return 0;
   });
   if (lam && lamV) {

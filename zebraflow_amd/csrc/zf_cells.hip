@@ -484,6 +484,110 @@ __global__ void k_windex_final(const unsigned long long *__restrict__ key, size_
   wdist[q] = kv == ~0ull ? __builtin_inff() : __uint_as_float((unsigned)(kv >> 32));
 }
 
+// Wall-layer velocity Uvoid at box node q (ZebraflowWT.c:4788-4851), zero
+// where the node does not carry one.  FP32 distances as the reference's
+// float coordinates give them (wall node coordinates and centroids in FP32,
+// the centroid (x1 + x2 + x3)/3 as :4660 forms it), strict < in triangle
+// order, vertices before the centroid.
+struct VoidParams {
+  double speed, d0, d1;
+  int k_lo, k_hi, select;
+  float px, py, pz;
+};
+
+__device__ bool uvoid_at(size_t q, const Grid &g, const int32_t *__restrict__ b,
+                         const int32_t *__restrict__ windex, const float *__restrict__ wdist,
+                         const double *__restrict__ wxyz, const int32_t *__restrict__ wtri,
+                         const double *__restrict__ wnormal, const int32_t *__restrict__ wnode_tri,
+                         const VoidParams &vp, double u[3]) {
+  const int k = (int)(q % g.km) + 1;
+  const int j = (int)((q / g.km) % g.jm) + 1;
+  const int i = (int)(q / ((size_t)g.km * g.jm)) + 1;
+  const int nw = windex[q];
+  if (nw <= 0 || b[q] > 0 || k < vp.k_lo || k > g.km - vp.k_hi) return false;
+  float px = vp.px, py = vp.py, pz = vp.pz;
+  if (!vp.select) {
+    px = (float)(g.x0 + (i - 1) * g.dx);
+    py = (float)(g.y0 + (j - 1) * g.dx);
+    pz = (float)(g.z0 + (k - 1) * g.dx);
+  }
+  const int32_t *row = &wnode_tri[13 * (size_t)(nw - 1)];
+  const int cnt = min(row[0], 12);
+  float mindis = 5000.0f;
+  int triwall = 0;
+  for (int l = 1; l <= cnt; l++) {
+    const int tri = row[l];
+    float cx = 0, cy = 0, cz = 0;
+    for (int sv = 0; sv < 3; sv++) {
+      const double *w = &wxyz[3 * (size_t)(wtri[3 * (size_t)(tri - 1) + sv] - 1)];
+      const float x = (float)w[0], y = (float)w[1], z = (float)w[2];
+      const float dx = px - x, dy = py - y, dz = pz - z;
+      const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (d < mindis) { mindis = d; triwall = tri; }
+      if (sv == 0) { cx = x; cy = y; cz = z; }
+      else { cx += x; cy += y; cz += z; }
+    }
+    cx = (float)((double)cx / 3.0);
+    cy = (float)((double)cy / 3.0);
+    cz = (float)((double)cz / 3.0);
+    const float dx = px - cx, dy = py - cy, dz = pz - cz;
+    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (d < mindis) { mindis = d; triwall = tri; }
+  }
+  if (triwall == 0) return false;
+  const double wd = (double)wdist[q];
+  const double w1 = wd <= vp.d0 ? 0.0 : wd <= vp.d1 ? (wd - vp.d0) / (vp.d1 - vp.d0) : 1.0;
+  if (w1 == 0.0) return false;
+  const double *n = &wnormal[3 * (size_t)(triwall - 1)];
+  u[0] = w1 * vp.speed * n[0];
+  u[1] = w1 * vp.speed * n[1];
+  u[2] = w1 * vp.speed * n[2];
+  return true;
+}
+
+__global__ void k_uvoid_flag(size_t box, Grid g, const int32_t *__restrict__ b,
+                             const int32_t *__restrict__ windex, const float *__restrict__ wdist,
+                             const double *__restrict__ wxyz, const int32_t *__restrict__ wtri,
+                             const double *__restrict__ wnormal,
+                             const int32_t *__restrict__ wnode_tri, VoidParams vp,
+                             int32_t *__restrict__ flag) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= box) {
+    if (q == box) flag[q] = 0;  // the scan's last element: the total lands in slot[box]
+    return;
+  }
+  double u[3];
+  flag[q] = uvoid_at(q, g, b, windex, wdist, wxyz, wtri, wnormal, wnode_tri, vp, u) ? 1 : 0;
+}
+
+// slot = exclusive prefix count of the flags, so the slots follow box order
+__global__ void k_uvoid_fill(size_t box, Grid g, const int32_t *__restrict__ b,
+                             const int32_t *__restrict__ windex, const float *__restrict__ wdist,
+                             const double *__restrict__ wxyz, const int32_t *__restrict__ wtri,
+                             const double *__restrict__ wnormal,
+                             const int32_t *__restrict__ wnode_tri, VoidParams vp,
+                             int32_t *__restrict__ slot_io, double *__restrict__ uvoid) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= box) return;
+  double u[3];
+  if (uvoid_at(q, g, b, windex, wdist, wxyz, wtri, wnormal, wnode_tri, vp, u)) {
+    const int sl = slot_io[q];
+    uvoid[3 * (size_t)sl] = u[0];
+    uvoid[3 * (size_t)sl + 1] = u[1];
+    uvoid[3 * (size_t)sl + 2] = u[2];
+  } else {
+    slot_io[q] = -1;
+  }
+}
+
+__global__ void k_uvoid_export(size_t box, const int32_t *__restrict__ slot,
+                               const double *__restrict__ uvoid, double *__restrict__ out) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= box) return;
+  const int sl = slot[q];
+  for (int a = 0; a < 3; a++) out[(size_t)a * box + q] = sl >= 0 ? uvoid[3 * (size_t)sl + a] : 0.0;
+}
+
 __global__ void k_fill_u64(unsigned long long *a, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = ~0ull;
@@ -495,11 +599,19 @@ int require_rep(zf_ctx *ctx) {
   return ZF_OK;
 }
 
+void free_void(zf_ctx *ctx) {
+  dev_free(ctx, (void **)&ctx->d_vslot);
+  dev_free(ctx, (void **)&ctx->d_uvoid);
+  ctx->have_void = false;
+  ctx->n_void = 0;
+}
+
 void free_wall(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
                    (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist,
                    (void **)&ctx->d_wtgeo, (void **)&ctx->d_wrad};
   for (auto b : bufs) dev_free(ctx, b);
+  free_void(ctx);
   ctx->have_wall = false;
 }
 
@@ -678,6 +790,93 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   ctx->nwt = nwt;
   ctx->wall_lmax = lmax;
   ctx->have_wall = true;
+  return ZF_OK;
+}
+
+void zf_uvoid_config_reference(zf_uvoid_config *uc) {
+  if (!uc) return;
+  *uc = zf_uvoid_config{};
+  uc->speed = 1e-3 / 1.0;  // 1e-3 m/s over VEL_SCALE = L_SCALE/T_SCALE = 1 m/s
+  uc->d0 = 0.7;
+  uc->d1 = 3.0;
+  uc->k_lo = 3;
+  uc->k_hi = 2;
+  uc->select = 1;
+  uc->select_point[0] = 78.5;
+  uc->select_point[1] = 214.5;
+  uc->select_point[2] = 392.5;
+}
+
+int zf_uvoid_setup(zf_ctx *ctx, const zf_uvoid_config *uc) {
+  if (!ctx) return ZF_ERR_ARG;
+  if (!uc) {
+    ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_void(ctx);
+    return ZF_OK;
+  }
+  if (!ctx->have_wall) return fail(ctx, ZF_ERR_STATE, "zf_wall_setup not called");
+  if (!(uc->d1 > uc->d0) || uc->k_lo < 1 || uc->k_hi < 0 || (uc->select != 0 && uc->select != 1))
+    return fail(ctx, ZF_ERR_ARG, "uvoid configuration");
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  free_void(ctx);
+  const VoidParams vp{uc->speed, uc->d0, uc->d1, uc->k_lo, uc->k_hi, uc->select,
+                      (float)uc->select_point[0], (float)uc->select_point[1],
+                      (float)uc->select_point[2]};
+  const Grid g = grid_of(ctx);
+  const size_t box = ctx->box;
+  int32_t *flag = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc;
+  if ((rc = dev_alloc(ctx, (void **)&flag, 4 * (box + 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_vslot, 4 * (box + 1))))
+    return rc;
+  hipLaunchKernelGGL(k_uvoid_flag, dim3(nb(box + 1, 256)), dim3(256), 0, ctx->stream, box, g, ctx->d_b,
+                     ctx->d_windex, ctx->d_wdist, ctx->d_wxyz, ctx->d_wtri, ctx->d_wnormal,
+                     ctx->d_wnode_tri, vp, flag);
+  ZF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flag, ctx->d_vslot, (int)(box + 1),
+                                               ctx->stream));
+  if ((rc = dev_alloc(ctx, &tmp, tmp_bytes))) {
+    dev_free(ctx, (void **)&flag);
+    return rc;
+  }
+  ZF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flag, ctx->d_vslot, (int)(box + 1),
+                                               ctx->stream));
+  int32_t total = 0;
+  ZF_HIP(ctx, hipMemcpyAsync(&total, ctx->d_vslot + box, 4, hipMemcpyDeviceToHost, ctx->stream));
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, &tmp);
+  dev_free(ctx, (void **)&flag);
+  ctx->device_bytes -= (int64_t)(4 * (box + 1) + tmp_bytes);
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_uvoid, 24 * (size_t)std::max(total, 1)))) return rc;
+  hipLaunchKernelGGL(k_uvoid_fill, dim3(nb(box, 256)), dim3(256), 0, ctx->stream, box, g, ctx->d_b,
+                     ctx->d_windex, ctx->d_wdist, ctx->d_wxyz, ctx->d_wtri, ctx->d_wnormal,
+                     ctx->d_wnode_tri, vp, ctx->d_vslot, ctx->d_uvoid);
+  ZF_HIP(ctx, hipGetLastError());
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->n_void = total;
+  ctx->have_void = true;
+  return ZF_OK;
+}
+
+int zf_download_uvoid(zf_ctx *ctx, double *uvw) {
+  if (!ctx || !uvw) return ZF_ERR_ARG;
+  const size_t box = ctx->box;
+  if (!ctx->have_void) {
+    for (size_t i = 0; i < 3 * box; i++) uvw[i] = 0.0;
+    return ZF_OK;
+  }
+  double *tmp = nullptr;
+  int rc = dev_alloc(ctx, (void **)&tmp, 24 * box);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_uvoid_export, dim3(nb(box, 256)), dim3(256), 0, ctx->stream, box,
+                     ctx->d_vslot, ctx->d_uvoid, tmp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(uvw, tmp, 24 * box, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  ctx->device_bytes -= (int64_t)(24 * box);
+  dev_free(ctx, (void **)&tmp);
+  if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "uvoid download: %s", hipGetErrorString(e));
   return ZF_OK;
 }
 

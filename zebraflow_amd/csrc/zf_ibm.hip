@@ -78,11 +78,15 @@ __device__ __forceinline__ double kern1(double d, double width) {
 // Interpolation, one lane per membrane node: the nodes within the spherical
 // cut-off |d| <= 2 DX all lie in the window [i-2, i+3] of the reference's
 // (int)((x-X0)/DX + 1.01) index, which is what is scanned.
+// With a wall layer (vslot non-null), window nodes that are not fluid add
+// the same weight times their Uvoid (ZebraflowWT.c:8980-8996), in the
+// window's order like the fluid terms.
 template <class U>
 __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int ncell,
                          const int32_t *__restrict__ status, const int32_t *__restrict__ b,
                          const int32_t *__restrict__ sbox, const U *__restrict__ umac,
-                         size_t stride, IbmParams p, double *__restrict__ V) {
+                         size_t stride, IbmParams p, const int32_t *__restrict__ vslot,
+                         const double *__restrict__ uvoid, double *__restrict__ V) {
   const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
   if (status[g / nn] < 0) return;
@@ -104,7 +108,15 @@ __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int nc
           continue;
         const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
         const int bv = b[q];
-        if (!(bv > 0 && bv < 4998)) continue;
+        if (!(bv > 0 && bv < 4998)) {
+          const int sl = vslot ? vslot[q] : -1;
+          if (sl < 0) continue;
+          const double w = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
+          v0 += w * p.vel_scale * uvoid[3 * (size_t)sl];
+          v1 += w * p.vel_scale * uvoid[3 * (size_t)sl + 1];
+          v2 += w * p.vel_scale * uvoid[3 * (size_t)sl + 2];
+          continue;
+        }
         const double w = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
         const int s = sbox[q];
         v0 += w * p.vel_scale * (double)umac[s];
@@ -355,7 +367,8 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
                          static_cast<double *>(ctx->d_umac));
       hipLaunchKernelGGL(k_interp<double>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                          ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox,
-                         static_cast<const double *>(ctx->d_umac), ctx->stride, params(ctx), ctx->d_v);
+                         static_cast<const double *>(ctx->d_umac), ctx->stride, params(ctx),
+                         ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, ctx->d_v);
       ZF_HIP(ctx, hipGetLastError());
       return ZF_OK;
     case ZF_STORE_F32DEV:
@@ -368,7 +381,7 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
   }
   hipLaunchKernelGGL(k_interp<float>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, um, ctx->stride,
-                     params(ctx), ctx->d_v);
+                     params(ctx), ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, ctx->d_v);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }

@@ -177,6 +177,11 @@ struct zf_ctx {
   double *d_wrad = nullptr;   // nwn: largest distance from a wall node to its triangles' vertices/centroids
   int32_t *d_windex = nullptr;    // box
   float *d_wdist = nullptr;       // box: FP32 distance to that wall node (or +inf)
+  // wall-layer velocity Uvoid: slot per box node (-1: none), 3 doubles per slot
+  bool have_void = false;
+  int32_t n_void = 0;
+  int32_t *d_vslot = nullptr;
+  double *d_uvoid = nullptr;
 
   // adjoint of the LBM step with a wall-shear-stress objective
   bool have_adj = false, adj_ready = false;  // set up / forward recorded

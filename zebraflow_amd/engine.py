@@ -350,6 +350,29 @@ class Engine:
     def repulse(self):
         self._check(self.lib.zf_repulsion(self.h), "zf_repulsion")
 
+    def uvoid_setup(self, uc=None, **kw):
+        """zf_uvoid_setup with the reference configuration updated by kw
+        (speed, d0, d1, k_lo, k_hi, select, select_point); uc=False turns it off."""
+        from ._abi import ZfUvoidConfig
+        if uc is False:
+            self._check(self.lib.zf_uvoid_setup(self.h, None), "zf_uvoid_setup")
+            return
+        if uc is None:
+            uc = ZfUvoidConfig()
+            self.lib.zf_uvoid_config_reference(C.byref(uc))
+        for k, v in kw.items():
+            if k == "select_point":
+                for a in range(3):
+                    uc.select_point[a] = v[a]
+            else:
+                setattr(uc, k, v)
+        self._check(self.lib.zf_uvoid_setup(self.h, C.byref(uc)), "zf_uvoid_setup")
+
+    def uvoid(self):
+        out = np.zeros((3, self.cfg.im, self.cfg.jm, self.cfg.km))
+        self._check(self.lib.zf_download_uvoid(self.h, _ptr(out)), "zf_download_uvoid")
+        return out
+
     def wall_neighbour(self):
         out = np.zeros((self.ncell, self.nn), np.int32)
         self._check(self.lib.zf_download_wall_neighbour(self.h, _ptr(out)),

@@ -26,6 +26,12 @@
  *   steps = N   output_every = M   output_dir = SimulationOutputData
  *   checkpoint_every = C           exact restart file ZFstate-%09d.bin every C
  *   restart = ZFstate-000000100.bin   continue from such a file (steps more)
+ *   uvoid = reference | node | off  wall-layer velocity Uvoid (:4788-4851; default
+ *                                  reference with units = reference, else off)
+ *   uvoid_speed = S                its speed (1e-3, units of U)
+ *   fluid_start = FLUIDdomainStart.aa FLUIDdomainStart.ab ...
+ *                                  the reference's start state (:2626-3387),
+ *                                  populations at equilibrium (:5253-5262)
  * Outputs: FLUIDdomain-%09d.csv, AllCellMemb-%09d.csv, VolumeAreaHistory.dat
  * (:282-446).
  */
@@ -152,6 +158,26 @@ int main(int argc, char **argv) {
   printf("zf_host: %s, %lld rows, %d listed nodes, %d fluid, %d boundary\n", zf_version(),
          (long long)rows, dommax, nint, bcmax);
 
+  /* fluid start state */
+  if (get("fluid_start")) {
+    char buf[960], *paths[64];
+    snprintf(buf, sizeof buf, "%s", get("fluid_start"));
+    int np = 0;
+    for (char *t = strtok(buf, " \t"); t && np < 64; t = strtok(NULL, " \t")) paths[np++] = t;
+    const size_t n = (size_t)dommax;
+    double *mac = calloc(7 * (n ? n : 1), sizeof(double));
+    if (!mac) return die(ctx, ZF_ERR_NOMEM, "fluid start state");
+    for (size_t p = 0; p < n; p++) mac[p] = 1.0;  /* D = 1, U = 0 where no row sets them */
+    const int64_t got = zfh_read_fluid_start(np, (const char *const *)paths, b, im, jm, km, x0, y0,
+                                             z0, dx, dommax, mac, mac + n, mac + 2 * n, mac + 3 * n,
+                                             mac + 4 * n);
+    if (got < 0) return die(ctx, ZF_ERR_ARG, "reading fluid_start");
+    if ((rc = zfh_init_from_macro(ctx, mac, mac + n, mac + 2 * n, mac + 3 * n, mac + 4 * n)))
+      return die(ctx, rc, "fluid start state");
+    free(mac);
+    printf("zf_host: fluid start state from %d file(s), %lld rows\n", np, (long long)got);
+  }
+
   /* cells */
   zfh_run_config run;
   memset(&run, 0, sizeof run);
@@ -233,6 +259,16 @@ int main(int argc, char **argv) {
           return die(ctx, rc, "zf_wall_setup");
         run.have_wall = 1;
         printf("zf_host: wall mesh %lld nodes, %lld triangles\n", (long long)nwn, (long long)nwt);
+        /* wall-layer velocity: on by default with the reference's units */
+        const char *uv = get("uvoid") ? get("uvoid") : reference ? "reference" : "off";
+        if (strcmp(uv, "off") != 0) {
+          zf_uvoid_config uc;
+          zf_uvoid_config_reference(&uc);
+          if (strcmp(uv, "node") == 0) uc.select = 0;
+          uc.speed = getd("uvoid_speed", uc.speed);
+          if ((rc = zf_uvoid_setup(ctx, &uc))) return die(ctx, rc, "zf_uvoid_setup");
+          printf("zf_host: wall-layer velocity (%s)\n", uv);
+        }
       }
       free(wx);
       free(wt);

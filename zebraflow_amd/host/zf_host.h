@@ -124,6 +124,23 @@ void zfh_membrane_config_lattice(const double *xyz, int nn, const int32_t *tri, 
                                  double kb, int max_cells, zf_membrane_config *mc);
 
 /* ---- the step/output loop -------------------------------------------------- */
+/* ---- the fluid start state -------------------------------------------------
+ * The reference's start files in OUTPUT()'s 13-column layout
+ * (FLUIDdomainStart.aa .. .aj, ZebraflowWT.c:2626-3387; the parents'
+ * *parentdomainStart.csv, :3394-3620): the first of the npaths files holds
+ * the header row.  Rows with Bindex >= 0 fill d, u, v, w (dommax each) and
+ * force (3*dommax SoA, may be NULL) at the list position of their node in b.
+ * Returns the rows applied, or -1 (file), -2 (dommax does not match b),
+ * -3 (a row outside the box or on a node b does not list). */
+int64_t zfh_read_fluid_start(int32_t npaths, const char *const *paths, const int32_t *b, int im,
+                             int jm, int km, double x0, double y0, double z0, double dx,
+                             int64_t dommax, double *d, double *u, double *v, double *w,
+                             double *force);
+/* F = feq(D, U) at every listed node (ZebraflowWT.c:5253-5262), uploaded,
+ * and the force field (NULL or all zero: cleared). */
+int zfh_init_from_macro(zf_ctx *ctx, const double *d, const double *u, const double *v,
+                        const double *w, const double *force);
+
 /* ---- exact restart files ------------------------------------------------------
  * The state as the engine stores it (populations, the force field of the
  * last RBC iteration, cell positions and status), so that a run continues

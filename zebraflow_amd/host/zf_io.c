@@ -1,6 +1,8 @@
 /*
  * zf_io.c -- the reference's text formats for the lattice and fluid state.
  *   FLUIDdomainBindex.csv   read at ZebraflowWT.c:2574-2619
+ *   FLUIDdomainStart.*      read at ZebraflowWT.c:2626-3387 (and the
+ *                           parents' start files, :3394-3620)
  *   FLUIDdomain-%09d.csv    written by OUTPUT(), ZebraflowWT.c:359-373
  *   AllCellMemb-%09d.csv    written by OUTPUT(), ZebraflowWT.c:435-446
  *   VolumeAreaHistory.dat   appended by OUTPUT(), ZebraflowWT.c:346-351
@@ -82,6 +84,92 @@ int zfh_write_fluid_csv(const char *path, const int32_t *b, int im, int jm, int 
         o++;
       }
   return fclose(f) == 0 ? 0 : -1;
+}
+
+/* ---- the fluid start state ----------------------------------------------- */
+
+/* The start state in OUTPUT()'s 13-column layout, read as the reference
+ * reads FLUIDdomainStart.aa .. .aj (ZebraflowWT.c:2626-3387, the pieces of
+ * one CSV split by lines, so only the first holds the header row) and the
+ * parents' *parentdomainStart.csv (:3394-3620).  Per row
+ * "%f,%f,%f,%d,%f,%f,%f,%f,%f,%f,%f,%d,%d" = X,Y,Z,Bindex,U,V,W,D,Bx,By,Bz,
+ * EulNeigh,Windex; a row with Bindex >= 0 sets D, U, V, W and the force of
+ * the node (int)((X-X0)/DX + 1.01), ... at its list position p = DOMID - 1 of
+ * the box b (lexicographic over b >= 0, i outer, k inner, ZebraflowWT.c:
+ * 1481-1502); the EulNeigh and Windex columns are not used (the cell list
+ * and W_index are rebuilt from the cells and the wall mesh).  Rows outside
+ * the box, or on a node that b does not list, are an error (the reference
+ * would write outside its arrays).  Returns the rows applied or < 0. */
+int64_t zfh_read_fluid_start(int32_t npaths, const char *const *paths, const int32_t *b, int im,
+                             int jm, int km, double x0, double y0, double z0, double dx,
+                             int64_t dommax, double *d, double *u, double *v, double *w,
+                             double *force) {
+  if (npaths <= 0 || !paths || !b || !d || !u || !v || !w) return -1;
+  const size_t n = (size_t)im * jm * km;
+  int32_t *pos = malloc(sizeof(int32_t) * (n ? n : 1));
+  if (!pos) return -1;
+  int64_t nl = 0;
+  for (size_t q = 0; q < n; q++) pos[q] = b[q] >= 0 ? (int32_t)nl++ : -1;
+  if (nl != dommax) { free(pos); return -2; }
+  int64_t applied = 0;
+  for (int32_t fi = 0; fi < npaths; fi++) {
+    FILE *f = fopen(paths[fi], "r");
+    if (!f) { free(pos); return -1; }
+    char line[512];
+    if (fi == 0 && !fgets(line, sizeof line, f)) { fclose(f); free(pos); return -1; }
+    float xp, yp, zp, uu, vv, ww, dd, bx, by, bz;
+    int bv, eul, win;
+    char c[12];
+    while (fscanf(f, "%f%c%f%c%f%c%d%c%f%c%f%c%f%c%f%c%f%c%f%c%f%c%d%c%d", &xp, &c[0], &yp, &c[1],
+                  &zp, &c[2], &bv, &c[3], &uu, &c[4], &vv, &c[5], &ww, &c[6], &dd, &c[7], &bx,
+                  &c[8], &by, &c[9], &bz, &c[10], &eul, &c[11], &win) == 25) {
+      if (bv < 0) continue;
+      const int i = (int)((xp - x0) / dx + 1.01), j = (int)((yp - y0) / dx + 1.01),
+                k = (int)((zp - z0) / dx + 1.01);
+      if (i < 1 || i > im || j < 1 || j > jm || k < 1 || k > km) { fclose(f); free(pos); return -3; }
+      const int32_t p = pos[bq(i, j, k, jm, km)];
+      if (p < 0) { fclose(f); free(pos); return -3; }
+      d[p] = dd; u[p] = uu; v[p] = vv; w[p] = ww;
+      if (force) { force[p] = bx; force[dommax + p] = by; force[2 * dommax + p] = bz; }
+      applied++;
+    }
+    fclose(f);
+  }
+  free(pos);
+  return applied;
+}
+
+/* Populations at equilibrium with the given moments, as the reference
+ * initialises them from its start state (ZebraflowWT.c:5253-5262:
+ * F = F_OLD = feq(D, U), C_LBM = 1), uploaded with the force field
+ * (NULL or all zero: none). */
+int zfh_init_from_macro(zf_ctx *ctx, const double *d, const double *u, const double *v,
+                        const double *w, const double *force) {
+  if (!ctx || !d || !u || !v || !w) return ZF_ERR_ARG;
+  zf_lattice_info info;
+  int err = zf_get_lattice_info(ctx, &info);
+  if (err) return err;
+  const size_t n = (size_t)info.dommax;
+  double *f = malloc(sizeof(double) * 19 * (n ? n : 1));
+  if (!f) return ZF_ERR_NOMEM;
+  static const int E[19][3] = {{1, 0, 0},  {-1, 0, 0}, {0, 1, 0},   {0, -1, 0}, {0, 0, 1},
+                               {0, 0, -1}, {1, 1, 0},  {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0},
+                               {1, 0, 1},  {-1, 0, -1}, {-1, 0, 1}, {1, 0, -1}, {0, 1, 1},
+                               {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 0, 0}};
+  for (size_t p = 0; p < n; p++) {
+    const double usq = u[p] * u[p] + v[p] * v[p] + w[p] * w[p];
+    for (int c = 0; c < 19; c++) {
+      const double wc = c < 6 ? 1.0 / 18.0 : c < 18 ? 1.0 / 36.0 : 1.0 / 3.0;
+      const double cu = u[p] * E[c][0] + v[p] * E[c][1] + w[p] * E[c][2];
+      f[(size_t)c * n + p] = d[p] * wc * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+    }
+  }
+  err = zf_upload_populations(ctx, f);
+  free(f);
+  if (err) return err;
+  int any = 0;
+  for (size_t i = 0; force && i < 3 * n && !any; i++) any = force[i] != 0.0;
+  return zf_upload_force(ctx, any ? force : NULL);
 }
 
 /* ---- meshes: the reference's VTK-style .dat files ------------------------- */
