@@ -186,8 +186,9 @@ int zf_download_parent_lists(zf_ctx *ctx, int32_t *vbcn);
  * node maps outside the main box or onto an unlisted node.  Destroying or
  * re-uploading either context unlinks them.  The adjoint entry points do
  * not cover the coupling: linked contexts refuse zf_adjoint_setup and a
- * context with an adjoint set up refuses the link.  Linked contexts are
- * called from one host thread. */
+ * context with an adjoint set up refuses the link.  A main lattice and its
+ * linked parents are one unit: drive them from one host thread, main first
+ * in each iteration; synchronising either context waits for both. */
 int zf_parent_link(zf_ctx *parent, zf_ctx *main);
 
 /* ---- RBC membranes ---------------------------------------------------------
