@@ -24,6 +24,9 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
+# the reference's OpenMP environment (OMPenvironment.txt): no thread binding;
+# set before any OpenMP runtime starts (the host and oracle libraries)
+os.environ.setdefault("OMP_PROC_BIND", "false")
 
 HBM_PEAK_GBS = 8000.0
 STORE_NAMES = {"f64": 0, "f32dev": 1, "f16dev": 2}
@@ -180,6 +183,92 @@ def cpu_baseline(name, b, seconds):
                 sample=f"{n} steps of the full {name} lattice ({n_int} fluid nodes), FP32 "
                        f"reference loop structure (F/F_OLD/FEQ arrays, DOMID lookups, "
                        f"schedule(static,5000)), {el:.1f} s")
+
+
+def beyond_llc_leg(store, steps, warmup, name="tree_cfg5_12"):
+    """The same stream kernel beyond the last-level cache: the 12 %-fluid
+    tree (8.12 M fluid nodes, 684 MB per FP32 population buffer), timed as
+    the headline (HIP events around back-to-back steps)."""
+    bt = make_geometry(name)
+    eng, ms, _ = time_form(name, bt, store, steps, warmup)
+    n = eng.n_interior
+    info = eng.info()
+    eng.close()
+    avg_ms = ms / steps
+    algo = 2 * 19 * STORE_BYTES[store] * n
+    ach = algo / (avg_ms * 1e-3) / 1e9
+    return dict(workload=name, fluid_nodes=n, steps=steps, avg_us=round(avg_ms * 1e3, 3),
+                mlups=round(n / (avg_ms * 1e-3) / 1e6, 1), achieved=round(ach, 1),
+                frac=round(ach / HBM_PEAK_GBS, 4),
+                population_footprint_bytes=2 * 19 * STORE_BYTES[store] * int(info["stride"]))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_rbc_baseline(name, b, ncell, seconds):
+    """The RBC iteration on the CPU, reference-structured OpenMP (oracle
+    restatements of ZebraflowWT.c:6800-7535 membrane forces, one thread per
+    cell; :5851-5900 cell list; :8693-9144 interpolation and wall
+    repulsion, :9913-10197 cell-cell repulsion, parallel over nodes;
+    :10739-11173 spreading with OpenMP atomics; :11975-11989 position
+    update), plus the FP32 LBM step of the same iteration, on the dense-cell
+    workload of the GPU leg (same placement); whole iterations until about
+    `seconds`."""
+    from oracle import pyoracle as po
+    from zebraflow_amd import geometry as g
+    from zebraflow_amd.engine import lattice_membrane_config
+    xyz, tris = g.rbc_mesh(radius=4.0)
+    cells = g.place_cells_dense(b, xyz, 0.6, ncell, 20261206)
+    wx, wt, wn = g.wall_mesh(b)
+    mc = lattice_membrane_config(xyz, tris, len(cells))
+    mp = {f: getattr(mc, f) for f, _ in po.MembParams._fields_}
+    edge, _, _ = po.mesh_topology(len(xyz), tris)
+    eq = po.mesh_equilibrium(xyz, edge, mp, mc.es)
+    tae = po.tri_areas(xyz, tris, 1.0)
+    p = po.ibm_params(b.shape, vmax=1.0)  # lattice units, as lattice_ibm_config
+    r = po.rep_params(bin_cutoff=2.0, cutoff=1.0, search=2.0)
+    nwt, windex = po.wall_tables(p, wx, wt)
+    params = dict(wall_mode=1, visc_scale=0.3, force_dt=1.0, inlet_u0=0.03, inlet_amp=0.5,
+                  inlet_period=4000.0, inlet_dir=(1, 0, 0), outlet_rho=1.0)
+    o = po.LbmOracle(b, params, dtype=np.float32)
+    domid = o.domid
+    x = cells.copy()
+    status = np.ones(len(cells), np.int32)
+    threads = po.max_threads()
+    t0 = po.wtime()
+    n = 0
+    while True:
+        o.step(1 + n, rbc_step=1)
+        gm = po.membrane_forces(x, tris, edge, eq, tae, mp)
+        bins = po.cell_list(p, b, domid, gm["tri_centroid"], status, r.bin_cutoff)
+        v = po.ibm_interpolate(p, b, domid, (o.U, o.V, o.W), x)
+        vrep = po.cell_repulsion(p, r, b, domid, tris, status, x, gm["node_normal"],
+                                 gm["tri_centroid"], gm["tri_normal"], bins)
+        vrep += po.wall_repulsion(p, r, windex, wx, wt, wn, nwt, status, x)[0]
+        frc = np.zeros((3, o.n), np.float32)
+        f64 = po.ibm_spread(p, b, domid, o.n, x, 0.5 * vrep + gm["force"], parallel=True)
+        frc[...] = f64
+        o.force[...] = frc
+        x, status, _ = po.advance(p, x, v + vrep, status)
+        n += 1
+        el = po.wtime() - t0
+        if el >= seconds or n >= 50:
+            break
+    nn = len(xyz)
+    return dict(value=round(len(cells) * nn * n / el, 1), unit="RBC node-updates/s",
+                cores=threads, kind="port",
+                sample=f"{n} RBC iterations (membrane/IBM/repulsion and the FP32 LBM step) of "
+                       f"{len(cells)} cells ({len(cells) * nn} membrane nodes) on the full "
+                       f"{name} lattice, {el:.1f} s",
+                ms_per_iteration=round(el / n * 1e3, 2))
 
 
 def cells_leg(name, b, ncell, steps, warmup):
@@ -483,7 +572,8 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
     res = {}
     for store, k, smooth, mesh in (("f32dev", checkpoint, False, "642"), ("f32dev", 1, False, "642"),
                                    ("f64", checkpoint, False, "642"), ("f64", checkpoint, True, "642"),
-                                   ("f64", checkpoint, True, "162"), ("f32dev", checkpoint, True, "162")):
+                                   ("f64", checkpoint, True, "162"), ("f32dev", checkpoint, True, "162"),
+                                   ("f32dev", checkpoint, True, "642")):
         eng = build(store, k, smooth=smooth, mesh=mesh)
         eng.full_run(0, 2 * interval)  # warm-up of every kernel
         eng.init_equilibrium()
@@ -530,6 +620,8 @@ def adjoint_leg(steps=3000, t0=1000, checkpoint=100, ncell=32, interval=10, fd_h
                                 for m, rs in smooth.items()},
                 smooth_variant_fp32_vs_fp64_cells_162=rel(res[("f32dev", checkpoint, True, "162")]["g"],
                                                           smooth["162"]["g"]),
+                smooth_variant_fp32_vs_fp64_cells_642=rel(res[("f32dev", checkpoint, True, "642")]["g"],
+                                                          smooth["642"]["g"]),
                 J=[r32["J"], r64["J"]])
 
 
@@ -618,6 +710,12 @@ def main():
     roof["copy_ceiling"] = round(eng.copy_bandwidth(2 << 30, 20), 1)
     info = eng.info()
     eng.close()
+    # cache residency of the measured workload: both population buffers
+    # against MI355X's 256 MB last-level cache (Infinity Cache)
+    foot = 2 * 19 * STORE_BYTES[args.store] * int(info["stride"])
+    roof["population_footprint_bytes"] = foot
+    roof["llc_bytes"] = 256 << 20
+    roof["llc_resident"] = foot <= (256 << 20)
 
     extra = {}
     if not args.no_extra:
@@ -629,6 +727,8 @@ def main():
                                   ms_per_step=round(ms64 / args.steps, 5),
                                   kernel_avg_us=roof64["avg_us"], kernel_frac=roof64["frac"])
         extra["speedup_vs_fp64_form"] = round(mlups / mlups64, 3)
+        if roof["llc_resident"]:
+            roof["beyond_llc"] = beyond_llc_leg(args.store, 200, 20)
         extra["error"] = form_errors(args.workload, b, args.err_steps)
         if not args.no_adjoint:
             extra["adjoint"] = adjoint_leg()
@@ -662,7 +762,11 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.workload, b, args.cpu_seconds)
+        cb = cpu_baseline(args.workload, b, args.cpu_seconds)
+        cb.update(cpu_model=cpu_model(), omp_proc_bind=os.environ.get("OMP_PROC_BIND"))
+        if args.cells > 0 and not args.no_extra:
+            cb["rbc"] = cpu_rbc_baseline(args.workload, b, args.cells, args.cpu_seconds)
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 

@@ -361,14 +361,16 @@ def uvoid(p, b, windex, wdist, wall_xyz, wall_tri, wall_normal, wall_node_tri, s
     return out
 
 
-def ibm_spread(p, b, domid, dommax, xyz, ffluid, force=None):
+def ibm_spread(p, b, domid, dommax, xyz, ffluid, force=None, parallel=False):
+    """Spreading onto force [3, dommax] (added); parallel: the reference's
+    OpenMP-atomic structure (zfo_ibm_spread_omp, order of sums free)."""
     b = np.ascontiguousarray(b, np.int32)
     domid = np.ascontiguousarray(domid, np.int32)
     x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
     f = np.ascontiguousarray(ffluid, np.float64).reshape(-1, 3)
     out = np.zeros((3, dommax)) if force is None else force
-    lib().zfo_ibm_spread(C.byref(p), _p(b), _p(domid), C.c_int64(dommax), C.c_int64(len(x)), _p(x),
-                         _p(f), _p(out))
+    fn = lib().zfo_ibm_spread_omp if parallel else lib().zfo_ibm_spread
+    fn(C.byref(p), _p(b), _p(domid), C.c_int64(dommax), C.c_int64(len(x)), _p(x), _p(f), _p(out))
     return out
 
 

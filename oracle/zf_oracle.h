@@ -190,6 +190,10 @@ void zfo_ibm_interpolate_wall(const zfo_ibm_params *p, const int32_t *b, const i
 void zfo_ibm_spread(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
                     int64_t dommax, int64_t nnode, const double *xyz, const double *ffluid,
                     double *force);
+/* zfo_ibm_spread with the reference's OpenMP atomics (CPU-baseline timing). */
+void zfo_ibm_spread_omp(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
+                        int64_t dommax, int64_t nnode, const double *xyz, const double *ffluid,
+                        double *force);
 /* Wall shear stress and wall pressure at n wall-mesh points (xyz, unit
  * normals into the fluid nrm: n*3); D,U,V,W per listed node. */
 void zfo_wss(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid, const double *D,

@@ -44,9 +44,12 @@ static void window(const zfo_ibm_params *p, double x, double y, double z, int w,
 void zfo_cell_list(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid, int ncell,
                    int nt, const int32_t *status, const double *tri_centroid, double bin_cutoff,
                    int32_t *bin) {
-  for (int c = 0; c < ncell; c++)
-    for (int t = 0; t < nt; t++) {
-      const size_t q = (size_t)c * nt + t;
+  /* each triangle's bin is its own nearest-node search: parallel over
+   * triangles, as the reference's per-cell OpenMP loop (:5851-5900) */
+#pragma omp parallel for schedule(static, 64)
+  for (int64_t qq = 0; qq < (int64_t)ncell * nt; qq++) {
+      const int c = (int)(qq / nt);
+      const size_t q = (size_t)qq;
       bin[q] = 0;
       if (status[c] < 0) continue;
       const double *x = &tri_centroid[3 * q];
@@ -64,7 +67,7 @@ void zfo_cell_list(const zfo_ibm_params *p, const int32_t *b, const int32_t *dom
             if (d < mindis) { mindis = d; bi = i; bj = j; bk = k; }
           }
       if (mindis < bin_cutoff) bin[q] = domid[bq(bi, bj, bk, p->jm, p->km)];
-    }
+  }
 }
 
 /* 16 sub-triangle centroids of the 4-times refined triangle (:9990-10025). */

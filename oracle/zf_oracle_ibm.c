@@ -121,6 +121,45 @@ void zfo_ibm_spread(const zfo_ibm_params *p, const int32_t *b, const int32_t *do
   }
 }
 
+/* The same spreading with the reference's parallel structure, for the CPU
+ * baseline's timing only: nodes shared among threads, every lattice-node
+ * update an atomic add (ZebraflowWT.c:10746-10867, its float atomics at
+ * :10857), so the sums' order is the threads' (results agree with
+ * zfo_ibm_spread to rounding). */
+void zfo_ibm_spread_omp(const zfo_ibm_params *p, const int32_t *b, const int32_t *domid,
+                        int64_t dommax, int64_t nnode, const double *xyz, const double *ffluid,
+                        double *force) {
+  const int im = p->im, jm = p->jm, km = p->km;
+  const double width = 2.0 * p->dx;
+#pragma omp parallel for schedule(static, 50)
+  for (int64_t n = 0; n < nnode; n++) {
+    const double xp = xyz[3 * n], yp = xyz[3 * n + 1], zp = xyz[3 * n + 2];
+    const int i = (int)((xp - p->x0) / p->dx + 1.01), j = (int)((yp - p->y0) / p->dx + 1.01),
+              k = (int)((zp - p->z0) / p->dx + 1.01);
+    int i1 = i - 4, i2 = i + 4, j1 = j - 4, j2 = j + 4, k1 = k - 4, k2 = k + 4;
+    i1 = i1 < 1 ? 1 : i1;
+    i2 = i2 > im ? im : i2;
+    j1 = j1 < 1 ? 1 : j1;
+    j2 = j2 > jm ? jm : j2;
+    k1 = k1 < 1 ? 1 : k1;
+    k2 = k2 > km ? km : k2;
+    for (int ii = i1; ii <= i2; ii++)
+      for (int jj = j1; jj <= j2; jj++)
+        for (int kk = k1; kk <= k2; kk++) {
+          const size_t q = bq(ii, jj, kk, jm, km);
+          if (b[q] < 0) continue;
+          const double w = ibm_weight(p->x0 + (ii - 1) * p->dx, p->y0 + (jj - 1) * p->dx,
+                                      p->z0 + (kk - 1) * p->dx, xp, yp, zp, width);
+          if (w == 0.0) continue;
+          const int o = domid[q] - 1;
+          for (int a = 0; a < 3; a++) {
+#pragma omp atomic
+            force[a * dommax + o] += w * ffluid[3 * n + a] * p->force_scale;
+          }
+        }
+  }
+}
+
 /* Returns the number of cells removed.  status < 0 cells are skipped. */
 int zfo_advance(const zfo_ibm_params *p, int ncell, int nn, double *xyz, const double *vel,
                 int32_t *status) {

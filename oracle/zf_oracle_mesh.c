@@ -157,9 +157,15 @@ void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
                          double *tri_centroid, double *cell_area, double *cell_vol,
                          double *cell_centroid, int *guard_fail) {
   const double L = mp->len_scale, lim = mp->force_limit;
+  /* cells are independent: one thread per cell, as the reference's
+   * per-cell OpenMP loops (ZebraflowWT.c:6800-7535), each with its own
+   * scratch, so every cell's sums keep their order */
+#pragma omp parallel
+  {
   double *fvol = calloc((size_t)3 * nn, sizeof(double));
   double *narea = calloc((size_t)nn, sizeof(double));
   int *owned = calloc((size_t)nt, sizeof(int));
+#pragma omp for schedule(static, 1)
   for (int c = 0; c < ncell; c++) {
     const double *x = xyz + (size_t)3 * nn * c;
     double *F = force + (size_t)3 * nn * c;
@@ -350,4 +356,5 @@ void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
   free(fvol);
   free(narea);
   free(owned);
+  }
 }
