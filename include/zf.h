@@ -241,6 +241,59 @@ int zf_membrane_forces(zf_ctx *ctx);
 int zf_download_membrane(zf_ctx *ctx, double *force, double *node_normal, double *cell_area,
                          double *cell_vol, double *cell_centroid, int32_t *guard_fail);
 
+/* ---- nucleus network (nucleated RBCs) --------------------------------------
+ * Each cell of the context may carry a nucleus: a network of nnuc nodes and
+ * ne WLC-POW filaments (the reference's 121-node, 544-edge zebrafish
+ * nucleus, NucleusNodes.dat / NucleusEdges.dat read at ZebraflowWT.c:
+ * 1903-1922).  Its nodes are immersed-boundary points of their cell:
+ * zf_step interpolates their velocities, spreads their forces and
+ * advances them with the membrane's nodes (:8693-8996, :10875-10990,
+ * :11975-11989; the speed check stays with the membrane nodes).  Per RBC
+ * iteration:
+ *   - spring forces F_Total_N (:7388-7466): the membrane's WLC + POW
+ *     spring on every edge, with equilibrium parameters from the rest
+ *     network (the 51-pass fixed point of :2003-2050 with es_factor * es,
+ *     lmax of :2203-2221);
+ *   - contact (:9519-9730): every list_interval iterations (and when cells
+ *     or nuclei are uploaded) each nucleus node lists the triangles of its
+ *     own membrane whose centroid lies within list_radius; a listed
+ *     triangle whose centroid is within near_radius is probed at 19 points
+ *     (centroid, sub-triangle centroids and midpoints), and every point
+ *     within `contact` of the nucleus node pushes the triangle's three
+ *     nodes along its normal: v_rep += k_contact (contact - d)/contact n/3.
+ *     The contact runs after the cell-cell and wall repulsion.
+ * The adjoint does not cover the nucleus: a context with one refuses
+ * zf_adjoint_setup with cells. */
+typedef struct zf_nucleus_config {
+  double es_factor;     /* 0.1: network shear modulus / membrane's (:2040) */
+  double list_radius;   /* 0.5: triangle list radius (:9582) */
+  double near_radius;   /* 1.0: listed triangles probed within (:9629) */
+  double contact;       /* 0.2: contact range of the probe points (:9705) */
+  double k_contact;     /* 4.1666667e-07: contact speed scale (:9707) */
+  int32_t list_interval; /* 1000: lists rebuilt when iter % list_interval == 0 (:9527) */
+  int32_t list_cap;      /* list entries per nucleus node (the reference's NucMembMAX 1004) */
+} zf_nucleus_config;
+
+void zf_nucleus_config_reference(zf_nucleus_config *nc);
+/* After zf_membrane_setup (and before or after zf_ibm_setup): edge 2*ne
+ * 1-based node ids, rest_xyz 3*nnuc in the frame of the membrane rest
+ * shape. */
+int zf_nucleus_setup(zf_ctx *ctx, const zf_nucleus_config *nc, int32_t nnuc, int32_t ne,
+                     const int32_t *edge, const double *rest_xyz);
+/* Per edge: LMAX_N, Plength_N, kp_N. */
+int zf_download_nucleus_equilibrium(zf_ctx *ctx, double *lmax, double *plength, double *kp);
+/* Nucleus node coordinates of every uploaded cell, ncell*nnuc*3 (cell,
+ * node, axis), after zf_upload_cells; resets their velocities. */
+int zf_upload_nuclei(zf_ctx *ctx, const double *xyz);
+/* xyz, vel, force (F_Total_N): ncell*nnuc*3 each (any may be NULL). */
+int zf_download_nuclei(zf_ctx *ctx, double *xyz, double *vel, double *force);
+/* The two nucleus stages alone (zf_step runs them in its order):
+ * spring forces of the current positions, and the contact of iteration
+ * iter added to the repulsion velocity. *list_overflow (nullable): list
+ * entries dropped for want of room since the set-up. */
+int zf_nucleus_forces(zf_ctx *ctx);
+int zf_nucleus_contact(zf_ctx *ctx, int32_t iter, int32_t *list_overflow);
+
 /* ---- immersed-boundary coupling ------------------------------------------- */
 typedef struct zf_ibm_config {
   double x0, y0, z0, dx;  /* node (i,j,k) at (x0+(i-1)dx, ...) (X0, Y0, Z0, DX) */

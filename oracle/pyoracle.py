@@ -48,6 +48,7 @@ def lib():
         _lib.zfo_bc_pressure.argtypes = [C.c_int, C.c_float]
         _lib.zfo_wtime.restype = C.c_double
         _lib.zfo_max_threads.restype = C.c_int
+        _lib.zfo_nucleus_contact.restype = C.c_int
     return _lib
 
 
@@ -298,6 +299,48 @@ def membrane_forces(xyz_cells, tris, edge, eq, tri_area_eq, mp):
                                                      "tri_normal", "tri_centroid", "cell_area",
                                                      "cell_vol", "cell_centroid", "guard_fail")])
     return out
+
+
+def nucleus_equilibrium(xyz, edge, mp, es, passes=51):
+    """LMAX_N, Plength_N, kp_N of the rest network (es: the network's
+    shear modulus, 0.1 of the membrane's in the reference)."""
+    x = np.ascontiguousarray(xyz, np.float64)
+    e = np.ascontiguousarray(edge, np.int32)
+    ne = len(e)
+    out = [np.zeros(ne) for _ in range(3)]
+    lib().zfo_nucleus_equilibrium(len(x), ne, _p(x), _p(e), C.byref(memb_params(mp)),
+                                  C.c_double(es), passes, *[_p(a) for a in out])
+    return dict(zip(("lmax", "plength", "kp"), out))
+
+
+def nucleus_forces(xyzn_cells, edge, eq, mp):
+    x = np.ascontiguousarray(xyzn_cells, np.float64)
+    nc, nnuc, _ = x.shape
+    e = np.ascontiguousarray(edge, np.int32)
+    out = np.zeros_like(x)
+    arrs = [np.ascontiguousarray(eq[k], np.float64) for k in ("lmax", "plength", "kp")]
+    lib().zfo_nucleus_forces(nc, nnuc, len(e), _p(e), *[_p(a) for a in arrs],
+                             C.byref(memb_params(mp)), _p(x), _p(out))
+    return out
+
+
+def nucleus_contact(xyzn_cells, xyz_cells, tris, status, tri_centroid, tri_normal, list_r=0.5,
+                    near_r=1.0, contact=0.2, k=4.1666667e-07, list_cap=64, vrep=None):
+    """Adds the contact's pushes to vrep [ncell, nn, 3] (zeros if None);
+    returns (vrep, entries dropped from full lists)."""
+    xn = np.ascontiguousarray(xyzn_cells, np.float64)
+    x = np.ascontiguousarray(xyz_cells, np.float64)
+    nc, nnuc, _ = xn.shape
+    nn = x.shape[1]
+    t = np.ascontiguousarray(tris, np.int32)
+    v = np.zeros_like(x) if vrep is None else vrep
+    lost = lib().zfo_nucleus_contact(nc, nnuc, nn, len(t), _p(t),
+                                     _p(np.ascontiguousarray(status, np.int32)), _p(xn), _p(x),
+                                     _p(np.ascontiguousarray(tri_centroid, np.float64)),
+                                     _p(np.ascontiguousarray(tri_normal, np.float64)),
+                                     C.c_double(list_r), C.c_double(near_r), C.c_double(contact),
+                                     C.c_double(k), int(list_cap), _p(v))
+    return v, lost
 
 
 # ---- immersed boundary (zf_oracle_ibm.c) ----------------------------------

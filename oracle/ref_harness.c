@@ -25,7 +25,10 @@
  *     out.bin : int32 EdgeM, EDGENODE_LIST rows 1..EdgeM (6 ints each),
  *               NodeM, NEIGHNODE_LIST rows (10 ints), NEIGHTRI_LIST rows
  *               (10 ints), then float32 LMAX, Plength, kp, Theta0,
- *               SINETHETA0, COSINETHETA0 (EdgeM each).
+ *               SINETHETA0, COSINETHETA0 (EdgeM each); then the nucleus:
+ *               int32 EdgeM_N, EDGENODE_LIST_N rows (2 ints), NodeM_N,
+ *               double NODECOORD_N of cell 1 (3 per node), float32 LMAX_N,
+ *               Plength_N, kp_N (EdgeM_N each).
  *
  *   ref_harness parents <workdir> <out.bin>
  *     Runs LBM_PARENTDOMAIN_SETUP (:515) and LBM_PARENTBC_SETUP (:1015) in
@@ -61,6 +64,14 @@ extern int NEIGHNODE_LIST[R_NODEM + 1][R_NLISTM + 1];
 extern int NEIGHTRI_LIST[R_NODEM + 1][R_NLISTM + 1];
 extern float LMAX[R_EDGEM + 1], Plength[R_EDGEM + 1], kp[R_EDGEM + 1],
     Theta0[R_EDGEM + 1], SINETHETA0[R_EDGEM + 1], COSINETHETA0[R_EDGEM + 1];
+
+/* the nucleus network, ZebraflowWT.c:120-122, :237-240 */
+#define R_CAPSM 3000
+#define R_NODEM_N 121
+#define R_EDGEM_N 544
+extern int EDGENODE_LIST_N[R_EDGEM_N + 1][2 + 1];
+extern double NODECOORD_N[3][R_CAPSM + 1][R_NODEM_N + 1];
+extern float LMAX_N[R_EDGEM_N + 1], Plength_N[R_EDGEM_N + 1], kp_N[R_EDGEM_N + 1];
 
 /* the three parent (feeder) lattices, ZebraflowWT.c:91-112, :188-230 */
 #define P1_IM 69
@@ -162,6 +173,18 @@ static int run_mesh(const char *dir, const char *out) {
   wr(o, &Theta0[1], 4, R_EDGEM);
   wr(o, &SINETHETA0[1], 4, R_EDGEM);
   wr(o, &COSINETHETA0[1], 4, R_EDGEM);
+  /* the nucleus network of the same set-up: edges, the node coordinates as
+   * read (cell 1), and its equilibrium parameters (0.1 Es, :2003-2050, and
+   * :2203-2221) */
+  int en = R_EDGEM_N, nn_n = R_NODEM_N;
+  wr(o, &en, 4, 1);
+  for (int e = 1; e <= R_EDGEM_N; e++) wr(o, &EDGENODE_LIST_N[e][1], 4, 2);
+  wr(o, &nn_n, 4, 1);
+  for (int n = 1; n <= R_NODEM_N; n++)
+    for (int a = 0; a < 3; a++) wr(o, &NODECOORD_N[a][1][n], 8, 1);
+  wr(o, &LMAX_N[1], 4, R_EDGEM_N);
+  wr(o, &Plength_N[1], 4, R_EDGEM_N);
+  wr(o, &kp_N[1], 4, R_EDGEM_N);
   fclose(o);
   return 0;
 }

@@ -165,6 +165,19 @@ void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
                          double *tri_centroid, double *cell_area, double *cell_vol,
                          double *cell_centroid, int *guard_fail);
 
+/* Nucleus network (ZebraflowWT.c:2003-2050 / :2203-2221 parameters,
+ * :7388-7466 spring forces, :9519-9730 contact); edge 2*ne 1-based. */
+void zfo_nucleus_equilibrium(int nnuc, int ne, const double *xyz, const int32_t *edge,
+                             const zfo_memb_params *mp, double es, int passes, double *lmax,
+                             double *plength, double *kp);
+void zfo_nucleus_forces(int ncell, int nnuc, int ne, const int32_t *edge, const double *lmax,
+                        const double *plength, const double *kp, const zfo_memb_params *mp,
+                        const double *xyzn, double *force);
+int zfo_nucleus_contact(int ncell, int nnuc, int nn, int nt, const int32_t *tri,
+                        const int32_t *status, const double *xyzn, const double *xyz,
+                        const double *tri_centroid, const double *tri_normal, double list_r,
+                        double near_r, double contact, double k, int list_cap, double *vrep);
+
 /* ---- immersed boundary (zf_oracle_ibm.c) ---------------------------------- */
 typedef struct zfo_ibm_params {
   int im, jm, km;

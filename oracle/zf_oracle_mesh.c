@@ -358,3 +358,143 @@ void zfo_membrane_forces(int ncell, int nn, int nt, int ne, const int32_t *tri,
   free(owned);
   }
 }
+
+/* ---- nucleus network (ZebraflowWT.c:1903-1922, :2003-2050, :2203-2221,
+ * :7388-7466, :9519-9730) ------------------------------------------------- */
+
+/* Filament parameters of the rest network: the 51-pass fixed point of
+ * :2003-2050 with es (the caller passes 0.1 Es), LMAX_N of :2203-2221. */
+void zfo_nucleus_equilibrium(int nnuc, int ne, const double *xyz, const int32_t *edge,
+                             const zfo_memb_params *mp, double es, int passes, double *lmax,
+                             double *plength, double *kp) {
+  (void)nnuc;
+  const double L = mp->len_scale, x0 = mp->x0;
+  const double beta = 0.5 * x0 / ((1. - x0) * (1. - x0) * (1. - x0)) - 0.25 / ((1. - x0) * (1. - x0)) + 0.25;
+  for (int e = 0; e < ne; e++) {
+    const double *p1 = &xyz[3 * (size_t)(edge[2 * e] - 1)], *p2 = &xyz[3 * (size_t)(edge[2 * e + 1] - 1)];
+    const double l = sqrt((p1[0] - p2[0]) * (p1[0] - p2[0]) + (p1[1] - p2[1]) * (p1[1] - p2[1]) +
+                          (p1[2] - p2[2]) * (p1[2] - p2[2]));
+    const double alpha = sqrt(3.) * 0.25 * mp->kT / (l * L);
+    const double eta = sqrt(3.) * 0.25 * (2. + 1.) / (l * l * l * L * L * L);
+    double pl = 1e-9 * (L / 1e-6), k = 0;
+    for (int it = 0; it < passes; it++) {
+      k = (es - alpha * beta / pl) / eta;
+      pl = (l * l * L * L) * mp->kT * (x0 - 0.25 + 1. / (4. * (1. - x0) * (1. - x0))) / k;
+    }
+    lmax[e] = L * l / x0;
+    plength[e] = pl;
+    kp[e] = k;
+  }
+}
+
+/* F_Total_N (:7388-7466): per cell, edges in order, the WLC + POW spring
+ * added to both end nodes. */
+void zfo_nucleus_forces(int ncell, int nnuc, int ne, const int32_t *edge, const double *lmax,
+                        const double *plength, const double *kp, const zfo_memb_params *mp,
+                        const double *xyzn, double *force) {
+  const double L = mp->len_scale;
+  for (int c = 0; c < ncell; c++) {
+    const double *x = xyzn + (size_t)3 * nnuc * c;
+    double *F = force + (size_t)3 * nnuc * c;
+    memset(F, 0, sizeof(double) * 3 * nnuc);
+    for (int e = 0; e < ne; e++) {
+      const int n1 = edge[2 * e] - 1, n2 = edge[2 * e + 1] - 1;
+      double d[3], len = 0;
+      for (int a = 0; a < 3; a++) { d[a] = x[3 * n1 + a] - x[3 * n2 + a]; len += d[a] * d[a]; }
+      len = sqrt(len);
+      double e1v[3], e2v[3];
+      for (int a = 0; a < 3; a++) { e1v[a] = d[a] / len; e2v[a] = -d[a] / len; }
+      double ratio = len * L / lmax[e];
+      if (ratio > mp->x_max) ratio = mp->x_max;
+      if (ratio < mp->x_floor) len = mp->x_floor * lmax[e] / L;
+      const double wlc = -mp->kT / plength[e] * (ratio - 0.25 + 1. / (4. * (1. - ratio) * (1. - ratio)));
+      const double comp = kp[e] * (1. / (len * len * L * L));
+      for (int a = 0; a < 3; a++) {
+        double f1 = mp->relax2 * e1v[a] * wlc;
+        f1 = f1 + mp->relax2 * e1v[a] * comp;
+        double f2 = mp->relax2 * e2v[a] * wlc;
+        f2 = f2 + mp->relax2 * e2v[a] * comp;
+        F[3 * n1 + a] += f1;
+        F[3 * n2 + a] += f2;
+      }
+    }
+  }
+}
+
+/* Nucleus-membrane contact of the cells with status >= 0 (:9524-9730):
+ * each nucleus node lists its cell's triangles with centroid closer than
+ * list_r (in order); per nucleus node in order, per listed triangle whose
+ * centroid is closer than near_r, the 19 probe points of :9662-9699, and
+ * every point within `contact` adds k (contact - d)/contact n/3 to the
+ * repulsion velocity of the triangle's three nodes.  List entries beyond
+ * list_cap are dropped (counted in the return value). */
+int zfo_nucleus_contact(int ncell, int nnuc, int nn, int nt, const int32_t *tri,
+                        const int32_t *status, const double *xyzn, const double *xyz,
+                        const double *tri_centroid, const double *tri_normal, double list_r,
+                        double near_r, double contact, double k, int list_cap, double *vrep) {
+  int lost = 0;
+  int *lst = malloc(sizeof(int) * (size_t)(list_cap > 0 ? list_cap : 1));
+  for (int c = 0; c < ncell; c++) {
+    if (status[c] < 0) continue;
+    const double *xn = xyzn + (size_t)3 * nnuc * c, *x = xyz + (size_t)3 * nn * c;
+    const double *TC = tri_centroid + (size_t)3 * nt * c, *TN = tri_normal + (size_t)3 * nt * c;
+    double *V = vrep + (size_t)3 * nn * c;
+    for (int node = 0; node < nnuc; node++) {
+      const double xc = xn[3 * node], yc = xn[3 * node + 1], zc = xn[3 * node + 2];
+      int cnt = 0;
+      for (int m = 0; m < nt; m++) {
+        const double dx = xc - TC[3 * m], dy = yc - TC[3 * m + 1], dz = zc - TC[3 * m + 2];
+        if (sqrt(dx * dx + dy * dy + dz * dz) < list_r) {
+          if (cnt < list_cap) lst[cnt++] = m;
+          else lost++;
+        }
+      }
+      for (int l = 0; l < cnt; l++) {
+        const int t = lst[l];
+        const double dx = xc - TC[3 * t], dy = yc - TC[3 * t + 1], dz = zc - TC[3 * t + 2];
+        if (!(sqrt(dx * dx + dy * dy + dz * dz) < near_r)) continue;
+        const int n1 = tri[3 * t] - 1, n2 = tri[3 * t + 1] - 1, n3 = tri[3 * t + 2] - 1;
+        double ep[10][3], tp[20][3];
+        for (int v = 0; v < 3; v++) {
+          const double a = x[3 * n1 + v], b = x[3 * n2 + v], cc = x[3 * n3 + v];
+          ep[1][v] = (a + b) / 2.;
+          ep[2][v] = (b + cc) / 2.;
+          ep[3][v] = (cc + a) / 2.;
+          tp[1][v] = (a + b + cc) / 3.;
+          tp[2][v] = (ep[3][v] + a + ep[1][v]) / 3.;
+          tp[3][v] = (ep[1][v] + b + ep[2][v]) / 3.;
+          tp[4][v] = (ep[2][v] + cc + ep[3][v]) / 3.;
+          tp[5][v] = (ep[1][v] + ep[3][v]) / 2.;
+          tp[6][v] = (ep[1][v] + ep[2][v]) / 2.;
+          tp[7][v] = (ep[2][v] + ep[3][v]) / 2.;
+          tp[8][v] = (a + tp[2][v]) / 2.;
+          tp[9][v] = (ep[1][v] + tp[2][v]) / 2.;
+          tp[10][v] = (ep[3][v] + tp[2][v]) / 2.;
+          tp[11][v] = (ep[1][v] + tp[3][v]) / 2.;
+          tp[12][v] = (b + tp[3][v]) / 2.;
+          tp[13][v] = (ep[2][v] + tp[3][v]) / 2.;
+          tp[14][v] = (ep[3][v] + tp[4][v]) / 2.;
+          tp[15][v] = (ep[2][v] + tp[4][v]) / 2.;
+          tp[16][v] = (cc + tp[4][v]) / 2.;
+          tp[17][v] = (ep[1][v] + tp[1][v]) / 2.;
+          tp[18][v] = (ep[2][v] + tp[1][v]) / 2.;
+          tp[19][v] = (ep[3][v] + tp[1][v]) / 2.;
+        }
+        for (int v = 1; v <= 19; v++) {
+          const double d = sqrt((tp[v][0] - xc) * (tp[v][0] - xc) + (tp[v][1] - yc) * (tp[v][1] - yc) +
+                                (tp[v][2] - zc) * (tp[v][2] - zc));
+          if (d <= contact) {
+            const double rf = (contact - d) / contact * k;
+            for (int i = 0; i < 3; i++) {
+              V[3 * n1 + i] += rf * TN[3 * t + i] / 3.;
+              V[3 * n2 + i] += rf * TN[3 * t + i] / 3.;
+              V[3 * n3 + i] += rf * TN[3 * t + i] / 3.;
+            }
+          }
+        }
+      }
+    }
+  }
+  free(lst);
+  return lost;
+}

@@ -9,7 +9,9 @@ Development container only:
 
 Output mesh_zfish.npz: input nodes and triangles, the reference's
 EDGENODE_LIST (1-based n1 n2 n3 n4 t1 t2), NEIGHNODE_LIST, NEIGHTRI_LIST and
-the float32 LMAX, Plength, kp, Theta0, SINETHETA0, COSINETHETA0.
+the float32 LMAX, Plength, kp, Theta0, SINETHETA0, COSINETHETA0; and the
+nucleus network (NucleusNodes/Edges.dat): its node coordinates as read,
+EDGENODE_LIST_N (1-based) and the float32 LMAX_N, Plength_N, kp_N.
 """
 import gzip
 import os
@@ -55,11 +57,18 @@ def main():
     nm = raw[o]; o += 1
     nbr_node = raw[o:o + 10 * nm].reshape(nm, 10); o += 10 * nm
     nbr_tri = raw[o:o + 10 * nm].reshape(nm, 10); o += 10 * nm
-    fl = raw[o:].view(np.float32).reshape(6, em)
+    fl = raw[o:o + 6 * em].view(np.float32).reshape(6, em); o += 6 * em
+    enn = raw[o]; o += 1
+    nedge = raw[o:o + 2 * enn].reshape(enn, 2); o += 2 * enn
+    nnn = raw[o]; o += 1
+    nxyz = raw[o:o + 6 * nnn].view(np.float64).reshape(nnn, 3); o += 6 * nnn
+    nfl = raw[o:o + 3 * enn].view(np.float32).reshape(3, enn)
     np.savez_compressed(os.path.join(HERE, "mesh_zfish.npz"), nodes=nodes[:nm], tris=tris,
                         edge=edge, nbr_node=nbr_node, nbr_tri=nbr_tri, lmax=fl[0], plength=fl[1],
-                        kp=fl[2], theta0=fl[3], sin0=fl[4], cos0=fl[5])
-    print(f"zebrafish mesh: {nm} nodes, {len(tris)} triangles, {em} edges")
+                        kp=fl[2], theta0=fl[3], sin0=fl[4], cos0=fl[5], nuc_nodes=nxyz,
+                        nuc_edge=nedge, nuc_lmax=nfl[0], nuc_plength=nfl[1], nuc_kp=nfl[2])
+    print(f"zebrafish mesh: {nm} nodes, {len(tris)} triangles, {em} edges; "
+          f"nucleus {nnn} nodes, {enn} edges")
 
 
 if __name__ == "__main__":

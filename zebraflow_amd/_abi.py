@@ -50,6 +50,11 @@ class ZfUvoidConfig(C.Structure):
                 ("select", i32), ("reserved", i32), ("select_point", f64 * 3)]
 
 
+class ZfNucleusConfig(C.Structure):
+    _fields_ = [(n, f64) for n in ("es_factor", "list_radius", "near_radius", "contact",
+                                   "k_contact")] + [("list_interval", i32), ("list_cap", i32)]
+
+
 class ZfAdjointConfig(C.Structure):
     _fields_ = [("checkpoint_interval", i32), ("max_steps", i32), ("t_begin", i32), ("t_end", i32),
                 ("mu_over_dx", f64), ("cells", i32), ("reserved", i32)]
@@ -109,6 +114,13 @@ FUNCS = {
     "zf_repulsion": (C.c_int, [P]),
     "zf_download_wall_neighbour": (C.c_int, [P, P]),
     "zf_wss": (C.c_int, [P, i64, P, P, f64, f64, P, P, P]),
+    "zf_nucleus_config_reference": (None, [C.POINTER(ZfNucleusConfig)]),
+    "zf_nucleus_setup": (C.c_int, [P, C.POINTER(ZfNucleusConfig), i32, i32, P, P]),
+    "zf_download_nucleus_equilibrium": (C.c_int, [P, P, P, P]),
+    "zf_upload_nuclei": (C.c_int, [P, P]),
+    "zf_download_nuclei": (C.c_int, [P, P, P, P]),
+    "zf_nucleus_forces": (C.c_int, [P]),
+    "zf_nucleus_contact": (C.c_int, [P, i32, C.POINTER(i32)]),
     "zf_uvoid_config_reference": (None, [C.POINTER(ZfUvoidConfig)]),
     "zf_uvoid_setup": (C.c_int, [P, C.POINTER(ZfUvoidConfig)]),
     "zf_download_uvoid": (C.c_int, [P, P]),

@@ -652,6 +652,8 @@ int zf_adjoint_setup(zf_ctx *ctx, const zf_adjoint_config *ac, int64_t npts, con
     return fail(ctx, ZF_ERR_ARG, "the adjoint supports FP64 and FP32 storage only");
   if (ctx->link_main || !ctx->linked.empty())
     return fail(ctx, ZF_ERR_STATE, "the adjoint does not cover parent-lattice coupling");
+  if (ac->cells && ctx->have_nuc)
+    return fail(ctx, ZF_ERR_STATE, "the adjoint does not cover the nucleus network");
   if ((uint64_t)Q * ctx->stride >= (1ull << 32))
     return fail(ctx, ZF_ERR_ARG, "lattice too large for the adjoint's 32-bit population offsets");
   ZF_HIP(ctx, hipSetDevice(ctx->device));

@@ -108,6 +108,7 @@ int zf_destroy(zf_ctx *ctx) {
   free_adjoint(ctx);
   free_state_slots(ctx);
   free_cells(ctx);
+  free_nucleus(ctx);
   free_ibm(ctx);
   free_membrane(ctx);
   free_state(ctx);

@@ -137,16 +137,20 @@ __device__ __forceinline__ int tile_of(double pos, double origin, double dx, int
 
 // v += v_rep; F_fluid = gamma_p v_rep + F_Total (ZebraflowWT.c:10749-10754);
 // sort key = tile << 32 | node (dead cells sort past the last tile).
+// id_base: added to the node index in the key (the nucleus nodes follow
+// the membrane's, from id_base = cap * nn).
 template <class U>
 __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V,
                             const double *__restrict__ vrep, const double *__restrict__ F,
                             size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
-                            IbmParams p, U *__restrict__ ffl, uint64_t *__restrict__ keys) {
+                            IbmParams p, U *__restrict__ ffl, uint64_t *__restrict__ keys,
+                            uint32_t id_base) {
   const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
   const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
+  const uint64_t id = (uint64_t)id_base + (uint64_t)g;
   if (status[g / nn] < 0) {
-    keys[g] = (ntiles << 32) | (uint64_t)g;
+    keys[g] = (ntiles << 32) | id;
     return;
   }
   double fl[3];
@@ -161,7 +165,7 @@ __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V
   const int tx = tile_of(X[g], p.x0, p.dx, p.ntx), ty = tile_of(X[NT + g], p.y0, p.dx, p.nty),
             tz = tile_of(X[2 * NT + g], p.z0, p.dx, p.ntz);
   const uint64_t tile = ((uint64_t)tx * p.nty + ty) * p.ntz + tz;
-  keys[g] = (tile << 32) | (uint64_t)g;
+  keys[g] = (tile << 32) | id;
 }
 
 __global__ void k_fill_i32(int32_t *a, size_t n, int32_t v) {
@@ -189,10 +193,12 @@ __device__ __forceinline__ double kcos_pi(double x) { return cospi(x); }
 // from a membrane node along an axis takes one of 4 values (its row in the
 // tile), so the lane staging a node evaluates its 12 factors once and the
 // other lanes read theirs from LDS (same expressions, same values).
+// Node ids from NT on are nucleus nodes (Xn, ffln, stride NTn).
 template <class F>
 __global__ __launch_bounds__(64) void k_spread_gather(
     const uint64_t *__restrict__ keys, const int32_t *__restrict__ range,
     const double *__restrict__ X, const F *__restrict__ ffl, size_t NT,
+    const double *__restrict__ Xn, const F *__restrict__ ffln, size_t NTn,
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, IbmParams p,
     F *__restrict__ force, size_t stride) {
   __shared__ F sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
@@ -222,12 +228,21 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           // 64 lanes, so the sums are unchanged
           bool rel = false;
           F px = 0, py = 0, pz = 0;
-          int g = 0;
+          size_t g = 0;
+          const double *Xs = X;
+          const F *fs = ffl;
+          size_t st = NT;
           if (lane < end - c0) {
-            g = (int)(keys[c0 + lane] & 0xffffffffu);
-            px = (F)((X[g] - ox) / p.dx);
-            py = (F)((X[NT + g] - oy) / p.dx);
-            pz = (F)((X[2 * NT + g] - oz) / p.dx);
+            g = (size_t)(keys[c0 + lane] & 0xffffffffu);
+            if (g >= NT) {
+              g -= NT;
+              Xs = Xn;
+              fs = ffln;
+              st = NTn;
+            }
+            px = (F)((Xs[g] - ox) / p.dx);
+            py = (F)((Xs[st + g] - oy) / p.dx);
+            pz = (F)((Xs[2 * st + g] - oz) / p.dx);
             rel = px >= (F)-2 && px <= (F)5 && py >= (F)-2 && py <= (F)5 && pz >= (F)-2 && pz <= (F)5;
           }
           const unsigned long long relm = __ballot(rel);
@@ -237,9 +252,9 @@ __global__ __launch_bounds__(64) void k_spread_gather(
             sx[sl] = px;
             sy[sl] = py;
             sz[sl] = pz;
-            sf0[sl] = ffl[g];
-            sf1[sl] = ffl[NT + g];
-            sf2[sl] = ffl[2 * NT + g];
+            sf0[sl] = fs[g];
+            sf1[sl] = fs[st + g];
+            sf2[sl] = fs[2 * st + g];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
@@ -349,7 +364,29 @@ void free_ibm(zf_ctx *ctx) {
                    (void **)&ctx->d_ffl,   (void **)&ctx->d_keys[0], (void **)&ctx->d_keys[1],
                    (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag};
   for (auto b : bufs) dev_free(ctx, b);
+  ctx->keys_cap = 0;
   ctx->have_ibm = false;
+}
+
+int ensure_sort_keys(zf_ctx *ctx, size_t n) {
+  if (n <= ctx->keys_cap) return ZF_OK;
+  if (n >= (1ull << 32)) return fail(ctx, ZF_ERR_ARG, "too many immersed-boundary points");
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dev_free(ctx, (void **)&ctx->d_keys[0]);
+  dev_free(ctx, (void **)&ctx->d_keys[1]);
+  dev_free(ctx, &ctx->d_sort_tmp);
+  ctx->keys_cap = 0;
+  int rc;
+  if ((rc = dev_alloc(ctx, (void **)&ctx->d_keys[0], 8 * n)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_keys[1], 8 * n)))
+    return rc;
+  size_t tmp = 0;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ctx->d_keys[0], ctx->d_keys[1],
+                                                  (int)n, 0, 64, ctx->stream));
+  ctx->sort_tmp_bytes = tmp;
+  if ((rc = dev_alloc(ctx, &ctx->d_sort_tmp, tmp))) return rc;
+  ctx->keys_cap = n;
+  return ZF_OK;
 }
 
 int ibm_interpolate_impl(zf_ctx *ctx) {
@@ -369,6 +406,12 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
                          ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox,
                          static_cast<const double *>(ctx->d_umac), ctx->stride, params(ctx),
                          ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, ctx->d_v);
+      if (ctx->have_nuc && ctx->nnuc > 0)
+        hipLaunchKernelGGL(k_interp<double>, dim3(nb(ctx->ncell * ctx->nnuc, 128)), dim3(128), 0,
+                           ctx->stream, ctx->d_xn, ncap * ctx->nnuc, ctx->nnuc, ctx->ncell,
+                           ctx->d_status, ctx->d_b, ctx->d_sbox,
+                           static_cast<const double *>(ctx->d_umac), ctx->stride, params(ctx),
+                           ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, ctx->d_vn);
       ZF_HIP(ctx, hipGetLastError());
       return ZF_OK;
     case ZF_STORE_F32DEV:
@@ -382,6 +425,11 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
   hipLaunchKernelGGL(k_interp<float>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, um, ctx->stride,
                      params(ctx), ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, ctx->d_v);
+  if (ctx->have_nuc && ctx->nnuc > 0)
+    hipLaunchKernelGGL(k_interp<float>, dim3(nb(ctx->ncell * ctx->nnuc, 128)), dim3(128), 0,
+                       ctx->stream, ctx->d_xn, ncap * ctx->nnuc, ctx->nnuc, ctx->ncell, ctx->d_status,
+                       ctx->d_b, ctx->d_sbox, um, ctx->stride, params(ctx),
+                       ctx->have_void ? ctx->d_vslot : nullptr, ctx->d_uvoid, ctx->d_vn);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }
@@ -402,30 +450,49 @@ int ibm_spread_impl(zf_ctx *ctx) {
   if (f64)
     hipLaunchKernelGGL(k_node_prep<double>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
                        ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
-                       static_cast<double *>(ctx->d_ffl), ctx->d_keys[0]);
+                       static_cast<double *>(ctx->d_ffl), ctx->d_keys[0], 0u);
   else
     hipLaunchKernelGGL(k_node_prep<float>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
                        ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
-                       static_cast<float *>(ctx->d_ffl), ctx->d_keys[0]);
+                       static_cast<float *>(ctx->d_ffl), ctx->d_keys[0], 0u);
+  // nucleus nodes (F_fluid = gamma_p NODE_WALLREPUL_N + F_Total_N,
+  // ZebraflowWT.c:10879-10884), keyed after the membrane's
+  const size_t NTn = ctx->have_nuc ? (size_t)ctx->mcfg.max_cells * ctx->nnuc : 0;
+  const int nnuc_pts = ctx->have_nuc ? ctx->ncell * ctx->nnuc : 0;
+  if (nnuc_pts > 0) {
+    if (f64)
+      hipLaunchKernelGGL(k_node_prep<double>, dim3(nb(nnuc_pts, 256)), dim3(256), 0, ctx->stream,
+                         ctx->d_xn, ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell,
+                         ctx->d_status, p, static_cast<double *>(ctx->d_ffln), ctx->d_keys[0] + n,
+                         (uint32_t)NT);
+    else
+      hipLaunchKernelGGL(k_node_prep<float>, dim3(nb(nnuc_pts, 256)), dim3(256), 0, ctx->stream,
+                         ctx->d_xn, ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell,
+                         ctx->d_status, p, static_cast<float *>(ctx->d_ffln), ctx->d_keys[0] + n,
+                         (uint32_t)NT);
+  }
+  const int nkeys = n + nnuc_pts;
   // end bit: tile ids need ceil(log2(ntiles+1)) bits above the 32 node bits
   int tbits = 1;
   while ((1ull << tbits) <= ntiles) tbits++;
   size_t tmp = ctx->sort_tmp_bytes;
   ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
-                                                  n, 0, 32 + tbits, ctx->stream));
+                                                  nkeys, 0, 32 + tbits, ctx->stream));
   hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * ntiles, 256)), dim3(256), 0, ctx->stream,
                      ctx->d_tile_range, (size_t)(2 * ntiles), 0);
-  hipLaunchKernelGGL(k_tile_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1], n,
-                     ntiles, ctx->d_tile_range);
+  hipLaunchKernelGGL(k_tile_ranges, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1],
+                     nkeys, ntiles, ctx->d_tile_range);
   if (f64)
     hipLaunchKernelGGL(k_spread_gather<double>, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
                        ctx->d_keys[1], ctx->d_tile_range, ctx->d_x,
-                       static_cast<const double *>(ctx->d_ffl), NT, ctx->d_b, ctx->d_sbox, p,
+                       static_cast<const double *>(ctx->d_ffl), NT, ctx->d_xn,
+                       static_cast<const double *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox, p,
                        static_cast<double *>(ctx->d_force), ctx->stride);
   else
     hipLaunchKernelGGL(k_spread_gather<float>, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
                        ctx->d_keys[1], ctx->d_tile_range, ctx->d_x,
-                       static_cast<const float *>(ctx->d_ffl), NT, ctx->d_b, ctx->d_sbox, p,
+                       static_cast<const float *>(ctx->d_ffl), NT, ctx->d_xn,
+                       static_cast<const float *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox, p,
                        static_cast<float *>(ctx->d_force), ctx->stride);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
@@ -446,6 +513,12 @@ int advance_impl(zf_ctx *ctx, int32_t *n_removed) {
                      nc, ctx->d_status, ctx->icfg.vmax, ctx->d_flag);
   hipLaunchKernelGGL(k_move, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, ctx->d_v, NT,
                      ctx->nn, nc, ctx->d_status, ctx->d_flag, ctx->icfg.dt_advance);
+  // the nucleus nodes move with their cell's update (:11950-11956, no
+  // speed check of their own)
+  if (ctx->have_nuc && ctx->nnuc > 0)
+    hipLaunchKernelGGL(k_move, dim3(nb(nc * ctx->nnuc, 256)), dim3(256), 0, ctx->stream, ctx->d_xn,
+                       ctx->d_vn, ncap * ctx->nnuc, ctx->nnuc, nc, ctx->d_status, ctx->d_flag,
+                       ctx->icfg.dt_advance);
   hipLaunchKernelGGL(k_remove, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, ctx->d_status,
                      ctx->d_flag, nc, ctx->d_flag + nc);
   ZF_HIP(ctx, hipGetLastError());
@@ -456,9 +529,10 @@ int advance_impl(zf_ctx *ctx, int32_t *n_removed) {
   return ZF_OK;
 }
 
-int rbc_update(zf_ctx *ctx, bool advance) {
+int rbc_update(zf_ctx *ctx, bool advance, int iter) {
   int rc;
   if ((rc = membrane_forces_impl(ctx))) return rc;
+  if (ctx->have_nuc && (rc = nucleus_forces_impl(ctx))) return rc;
   if (ctx->have_rep && (rc = cell_list_impl(ctx))) return rc;
   if ((rc = ibm_interpolate_impl(ctx))) return rc;
   if (ctx->have_rep) {
@@ -467,6 +541,7 @@ int rbc_update(zf_ctx *ctx, bool advance) {
     const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
     hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
   }
+  if (ctx->have_nuc && (rc = nucleus_contact_impl(ctx, iter))) return rc;
   if ((rc = ibm_spread_impl(ctx))) return rc;
   return advance ? advance_impl(ctx, nullptr) : ZF_OK;
 }
@@ -516,16 +591,11 @@ int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
   if ((rc = dev_alloc(ctx, (void **)&ctx->d_umac, 3 * field_elem(ctx) * ctx->stride)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_vrep, 24 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_ffl, 3 * field_elem(ctx) * NT)) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_keys[0], 8 * NT)) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_keys[1], 8 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_tile_range, 8 * ntiles)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_flag, 4 * (ncap + 1))))
     return rc;
-  size_t tmp = 0;
-  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ctx->d_keys[0], ctx->d_keys[1],
-                                                  (int)NT, 0, 64, ctx->stream));
-  ctx->sort_tmp_bytes = tmp;
-  if ((rc = dev_alloc(ctx, &ctx->d_sort_tmp, tmp))) return rc;
+  // sort keys for the membrane nodes and, with a nucleus, its nodes after them
+  if ((rc = ensure_sort_keys(ctx, NT + (ctx->have_nuc ? ncap * ctx->nnuc : 0)))) return rc;
   hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
   ZF_HIP(ctx, hipGetLastError());
   ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -606,7 +676,7 @@ int zf_step(zf_ctx *ctx, int32_t iter) {
   const bool rbc = ctx->have_ibm && ctx->ncell > 0 && iter % ctx->icfg.rbc_interval == 0;
   int rc = zf_lbm_step(ctx, iter, rbc ? 1 : 0);
   if (rc || !rbc) return rc;
-  return rbc_update(ctx, true);
+  return rbc_update(ctx, true, iter);
 }
 
 int zf_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps) {

@@ -141,6 +141,23 @@ struct zf_ctx {
   int32_t *d_status = nullptr;    // ncap: >=0 live, <0 removed
   int32_t *d_fail = nullptr;      // ncap: guard failures this evaluation
 
+  // nucleus network of each cell (nucleated RBCs)
+  bool have_nuc = false;
+  zf_nucleus_config ncfg{};
+  int32_t nnuc = 0, ne_n = 0;
+  std::vector<int32_t> h_nuc_edge;  // 2 x ne_n, 1-based (input order)
+  std::vector<double> h_nuc_eq;     // 3 x ne_n: lmax, plength, kp
+  int32_t *d_nuc_edge = nullptr;    // 2 x ne_n, 0-based
+  double *d_nuc_eq = nullptr;       // 3 x ne_n
+  int32_t *d_nuc_inc_start = nullptr, *d_nuc_inc = nullptr;  // node -> edge << 1 | role, edge order
+  double *d_xn = nullptr, *d_vn = nullptr;    // 3 x (ncap*nnuc): positions, velocities
+  double *d_fn = nullptr, *d_vrepn = nullptr; // 3 x (ncap*nnuc): F_Total_N, NODE_WALLREPUL_N
+  void *d_ffln = nullptr;           // 3 x (ncap*nnuc): F_fluid of the nucleus nodes, field_elem
+  int32_t *d_nlist = nullptr;       // (ncap*nnuc) x list_cap: listed triangles (0-based)
+  int32_t *d_nlist_cnt = nullptr;   // ncap*nnuc, then one overflow counter
+  bool nuc_lists_valid = false;     // cleared when cells or nuclei are uploaded
+  size_t keys_cap = 0;              // entries of the spreading sort keys
+
   // immersed boundary
   bool have_ibm = false;
   zf_ibm_config icfg{};
@@ -267,6 +284,14 @@ void free_state_slots(zf_ctx *ctx);
 int membrane_forces_impl(zf_ctx *ctx);
 int cell_list_impl(zf_ctx *ctx);
 int repulsion_impl(zf_ctx *ctx);
+// Nucleus network (zf_nucleus.hip): spring forces F_Total_N of every live
+// cell; the nucleus-membrane contact of iteration iter (lists rebuilt on
+// the rebuild rule), added to the membrane's repulsion velocity.
+int nucleus_forces_impl(zf_ctx *ctx);
+int nucleus_contact_impl(zf_ctx *ctx, int iter);
+void free_nucleus(zf_ctx *ctx);
+// Spreading sort keys for n entries (allocates only when they do not fit).
+int ensure_sort_keys(zf_ctx *ctx, size_t n);
 // Physical position of lattice node (i,j,k): (x0 + (i-1) dx, ...); the IBM
 // configuration's when set up, lattice units otherwise.
 struct GridGeom {
@@ -284,7 +309,8 @@ inline size_t field_elem(const zf_ctx *ctx) { return ctx->cfg.storage == ZF_STOR
 // The cell stages of an RBC iteration after its LBM step (membrane forces,
 // cell list, interpolation, repulsion, spreading) and, with `advance`, the
 // position update: the order of zf_step.
-int rbc_update(zf_ctx *ctx, bool advance);
+// iter: the iteration (the nucleus contact's list rebuild rule).
+int rbc_update(zf_ctx *ctx, bool advance, int iter = 0);
 // Parent coupling (zf_parent.hip): the main lattice's export of the moments
 // its parents read (in_step: after its stream kernel, before the buffer
 // swap and the force field's clearing; else of the current state), and a

@@ -305,6 +305,7 @@ int zf_membrane_setup(zf_ctx *ctx, const zf_membrane_config *mc, int32_t nn, int
   ZF_HIP(ctx, hipSetDevice(ctx->device));
   free_cells(ctx);  // sized by the old mesh
   free_ibm(ctx);
+  free_nucleus(ctx);
   free_mesh(ctx);
   ctx->mcfg = *mc;
   ctx->nn = nn;
@@ -411,6 +412,7 @@ int zf_upload_cells(zf_ctx *ctx, int32_t ncell, const double *xyz) {
   if (ncell > ctx->mcfg.max_cells) return fail(ctx, ZF_ERR_ARG, "%d cells > max_cells %d", ncell, ctx->mcfg.max_cells);
   const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn, n = (size_t)ncell * ctx->nn;
   ctx->ncell = ncell;
+  ctx->nuc_lists_valid = false;
   std::vector<int32_t> st(ctx->mcfg.max_cells, -5000);
   for (int c = 0; c < ncell; c++) st[c] = 1;
   ZF_HIP(ctx, hipMemcpy(ctx->d_status, st.data(), 4 * st.size(), hipMemcpyHostToDevice));

@@ -279,6 +279,44 @@ class Engine:
         f = np.ascontiguousarray(f, np.float64)
         self._check(self.lib.zf_upload_membrane_force(self.h, _ptr(f)), "zf_upload_membrane_force")
 
+    # -- nucleus network ------------------------------------------------------------
+    def nucleus_setup(self, edge, rest_xyz, **kw):
+        """zf_nucleus_setup with the reference configuration updated by kw."""
+        from ._abi import ZfNucleusConfig
+        nc = ZfNucleusConfig()
+        self.lib.zf_nucleus_config_reference(C.byref(nc))
+        for k, v in kw.items():
+            setattr(nc, k, v)
+        e = np.ascontiguousarray(edge, np.int32)
+        x = np.ascontiguousarray(rest_xyz, np.float64)
+        self._check(self.lib.zf_nucleus_setup(self.h, C.byref(nc), len(x), len(e), _ptr(e), _ptr(x)),
+                    "zf_nucleus_setup")
+        self.nnuc, self.ne_n = len(x), len(e)
+
+    def nucleus_equilibrium(self):
+        out = [np.zeros(self.ne_n) for _ in range(3)]
+        self._check(self.lib.zf_download_nucleus_equilibrium(self.h, *[_ptr(a) for a in out]),
+                    "zf_download_nucleus_equilibrium")
+        return dict(zip(("lmax", "plength", "kp"), out))
+
+    def upload_nuclei(self, xyz):
+        x = np.ascontiguousarray(xyz, np.float64)
+        assert x.shape == (self.ncell, self.nnuc, 3)
+        self._check(self.lib.zf_upload_nuclei(self.h, _ptr(x)), "zf_upload_nuclei")
+
+    def nuclei(self):
+        out = [np.zeros((self.ncell, self.nnuc, 3)) for _ in range(3)]
+        self._check(self.lib.zf_download_nuclei(self.h, *[_ptr(a) for a in out]), "zf_download_nuclei")
+        return tuple(out)
+
+    def nucleus_forces(self):
+        self._check(self.lib.zf_nucleus_forces(self.h), "zf_nucleus_forces")
+
+    def nucleus_contact(self, it):
+        lost = C.c_int32()
+        self._check(self.lib.zf_nucleus_contact(self.h, it, C.byref(lost)), "zf_nucleus_contact")
+        return lost.value
+
     def upload_repulsion(self, v):
         v = np.ascontiguousarray(v, np.float64)
         self._check(self.lib.zf_upload_repulsion(self.h, _ptr(v)), "zf_upload_repulsion")
