@@ -293,6 +293,18 @@ int zf_download_nuclei(zf_ctx *ctx, double *xyz, double *vel, double *force);
  * entries dropped for want of room since the set-up. */
 int zf_nucleus_forces(zf_ctx *ctx);
 int zf_nucleus_contact(zf_ctx *ctx, int32_t iter, int32_t *list_overflow);
+/* *nnuc and *list_cap of the set-up nucleus, 0 and 0 without one. */
+int zf_get_nucleus_info(zf_ctx *ctx, int32_t *nnuc, int32_t *list_cap);
+/* The contact lists as they stand (NUCLNEIGH between rebuilds,
+ * ZebraflowWT.c:9527-9590), for exact restart files: the reference
+ * rebuilds them at its first RBC iteration after a restart (NucMembUpdate
+ * starts at 0), so its restarted run leaves the continuous one until the
+ * next multiple of 1000.  list ncell*nnuc*list_cap local triangle ids
+ * (0-based, the first count[q] of row q used), count ncell*nnuc, *valid 0
+ * when the next zf_nucleus_contact rebuilds them whatever its iteration.
+ * The upload checks every entry. */
+int zf_download_nucleus_lists(zf_ctx *ctx, int32_t *list, int32_t *count, int32_t *valid);
+int zf_upload_nucleus_lists(zf_ctx *ctx, const int32_t *list, const int32_t *count, int32_t valid);
 
 /* ---- immersed-boundary coupling ------------------------------------------- */
 typedef struct zf_ibm_config {

@@ -156,3 +156,49 @@ def test_step_with_nuclei_matches_composed_oracle():
     np.testing.assert_allclose(eng.populations(), o.populations, rtol=0, atol=1e-9)
     assert nm > 0
     eng.close()
+
+
+def test_checkpoints_with_nuclei_continue_bit_identically(tmp_path):
+    """A device slot (zf_checkpoint_save/restore) and a restart file
+    (zfh_save_state/load_state) taken between two list rebuilds: the runs
+    continued from them equal the uninterrupted run bit for bit, nuclei and
+    their contact lists included."""
+    print("case checkpoints with nuclei, 2 cells, lists every 3 iterations", flush=True)
+
+    def make():
+        b, g, cells, nuclei, eng, *_ = setup(list_interval=3)
+        eng.repulsion_setup(repulsion_config())
+        eng.wall_setup(*geometry.wall_mesh(b))
+        eng.init_equilibrium(1.0, (0.01, 0.0, 0.0))
+        return eng, len(cells), cells.shape[1]
+
+    def state(eng):
+        x, _, st = eng.cells()
+        xn, vn, fn = eng.nuclei()
+        lst, cnt, valid = eng.nucleus_lists()
+        return [eng.populations_raw(), eng.force(), x, st, xn, vn, fn, cnt, valid,
+                np.concatenate([r[:c] for r, c in zip(lst, cnt)])]
+
+    eng, ncell, nn = make()
+    for it in range(1, 5):  # lists built at 1 (first call) and 3
+        eng.full_step(it)
+    eng.checkpoint_save(0)
+    path = tmp_path / "ZFstate-000000004.bin"
+    eng.save_state(path, 4, ncell, nn)
+    for it in range(5, 8):  # rebuilt at 6
+        eng.full_step(it)
+    ref = state(eng)
+    assert ref[8] == 1 and ref[7].sum() > 0
+    eng.checkpoint_restore(0)
+    for it in range(5, 8):
+        eng.full_step(it)
+    for a, b_ in zip(state(eng), ref):
+        np.testing.assert_array_equal(a, b_)
+    eng.close()
+    eng2, _, _ = make()
+    assert eng2.load_state(path, ncell, nn) == 4
+    for it in range(5, 8):
+        eng2.full_step(it)
+    for a, b_ in zip(state(eng2), ref):
+        np.testing.assert_array_equal(a, b_)
+    eng2.close()

@@ -25,7 +25,23 @@ def p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def write_case(d, b, cells, xyz, tris, steps, every, extra=""):
+def write_nucleus_files(d, rest, edge, nuclei):
+    """NucleusNodes.dat, NucleusEdges.dat and AllCellNucStart.csv as the
+    reference reads them (ZebraflowWT.c:1903-1922, :5541-5593)."""
+    h = load_host_library()
+    assert h.zfh_write_points(os.path.join(d, "NucleusNodes.dat").encode(), p(rest), len(rest)) == 0
+    with open(os.path.join(d, "NucleusEdges.dat"), "w") as f:
+        f.write(f"LINES {len(edge)} {3 * len(edge)}\n")
+        f.writelines(f"{a} {b}\n" for a, b in edge)
+    nuc = np.ascontiguousarray(nuclei, np.float64)
+    assert h.zfh_write_nuclei_csv(os.path.join(d, "AllCellNucStart.csv").encode(), nuc.shape[0],
+                                  nuc.shape[1], p(nuc), None, None, None) == 0
+    return "\n".join(f"{k} = {os.path.join(d, f)}" for k, f in (
+        ("nucleus_nodes", "NucleusNodes.dat"), ("nucleus_edges", "NucleusEdges.dat"),
+        ("nuclei", "AllCellNucStart.csv")))
+
+
+def write_case(d, b, cells, xyz, tris, steps, every, extra="", out="out"):
     h = load_host_library()
     im, jm, km = b.shape
     bpath = os.path.join(d, "FLUIDdomainBindex.csv")
@@ -54,7 +70,7 @@ rbc_interval = 2
 wall = surface
 steps = {steps}
 output_every = {every}
-output_dir = {os.path.join(d, "out")}
+output_dir = {os.path.join(d, out)}
 {extra}
 """
     path = os.path.join(d, "case.cfg")
@@ -105,6 +121,30 @@ def test_mesh_and_cell_files_round_trip(tmp_path):
     assert h.zfh_read_cells_csv(f, len(xyz), p(back), p(st2), 2) == 2
     np.testing.assert_allclose(back, cells, atol=1e-6)  # printed with %f
     assert h.zfh_read_cells_csv(f, len(xyz), p(back), p(st2), 1) == -2
+    # nucleus files: NucleusEdges "LINES n 3n" pairs, AllCellNucleus layout
+    f = str(tmp_path / "NucleusEdges.dat").encode()
+    ned = np.ascontiguousarray([[1, 2], [2, 3], [3, 1], [1, 4]], np.int32)
+    open(f, "w").write(f"LINES {len(ned)} {3 * len(ned)}\n" + "".join(f"{a} {b}\n" for a, b in ned))
+    assert h.zfh_read_lines(f, None, 0) == 4
+    e2 = np.zeros_like(ned)
+    assert h.zfh_read_lines(f, p(e2), 4) == 4 and h.zfh_read_lines(f, p(e2), 3) == -2
+    np.testing.assert_array_equal(e2, ned)
+    nuc = np.stack([xyz[:5] * 0.5 + 5.0, xyz[:5] * 0.5 + 9.0])
+    fn = np.full_like(nuc, -1.5e-6)
+    st3 = np.array([1, -5000, 2], np.int32)  # the dead middle cell is not written
+    nuc3 = np.stack([nuc[0], nuc[0] * 0, nuc[1]])
+    f = str(tmp_path / "AllCellNucleus.csv").encode()
+    assert h.zfh_write_nuclei_csv(f, 3, 5, p(nuc3), None, p(np.stack([fn[0], fn[0], fn[1]])),
+                                  p(st3)) == 0
+    lines = open(f).read().splitlines()
+    assert lines[0] == "X,Y,Z,Cell,CellStatus,Vx,Vy,Vz,Fx,Fy,Fz,Neigh,Neighflag"
+    assert len(lines) == 11 and all(len(l.split(",")) == 13 for l in lines[1:])
+    assert lines[6].split(",")[3:5] == ["3", "2"] and lines[6].split(",")[8] == "-1.5000000e-06"
+    back = np.zeros_like(nuc)
+    assert h.zfh_read_nuclei_csv(f, 5, p(back), 2) == 2
+    np.testing.assert_allclose(back, nuc, atol=1e-6)
+    assert h.zfh_read_nuclei_csv(f, 5, p(back), 1) == -2
+    assert h.zfh_read_nuclei_csv(f, 4, p(back), 3) == -1  # 10 rows are not whole cells of 4
     # the volume/area history appends, header once
     f = str(tmp_path / "VolumeAreaHistory.dat").encode()
     vol, area, cen = np.array([1.0, 2.0]), np.array([3.0, 4.0]), np.zeros((2, 3))
@@ -168,4 +208,65 @@ def test_host_program_matches_engine(tmp_path):
     np.testing.assert_allclose(cm[:, :3], x.reshape(-1, 3), atol=2e-6)
     assert (cm[:, 4] == 1).all()
     assert np.abs(u).max() > 1e-3
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_host_program_with_nuclei_matches_engine(tmp_path):
+    """zf_host reading the nucleus network and the start nuclei in the
+    reference's files: its AllCellNucleus output equals the engine's
+    nuclei after the same iterations, and a restart file written midway
+    continues to the same state."""
+    print("case zf_host with nucleated cells, 2 cells, 8 iterations", flush=True)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mesh_zfish.npz"))
+    b = geometry.tube((30, 20, 20), axis=0, r=8.0, inlet=B_VELOCITY_INLET, outlet=B_PRESSURE_OUTLET)
+    xyz, tris = geometry.rbc_mesh(radius=3.0, level=2)
+    cells = geometry.place_cells(b, xyz, 3.0, 1.0, 2, 5)
+    rest = 1.4 * g["nuc_nodes"]
+    rng = np.random.default_rng(3)
+    nuclei = np.stack([1.05 * rest + 0.02 * rng.standard_normal(rest.shape) + c.mean(0)
+                       + [0.0, 0.0, 0.5] for c in cells])
+    nuclei = np.round(nuclei, 6)  # as printed with %f
+    steps, every = 8, 4
+    extra = write_nucleus_files(str(tmp_path), rest, g["nuc_edge"], nuclei)
+    cfg = write_case(str(tmp_path), b, cells, xyz, tris, steps, every,
+                     extra + "\ncheckpoint_every = 4")
+    exe = os.path.join(PKG, "zf_host")
+    r = subprocess.run([exe, cfg], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0 and "nucleated cells, 121 nucleus nodes, 544 filaments" in r.stdout
+    out = tmp_path / "out"
+    nu = np.genfromtxt(out / f"AllCellNucleus-{steps:09d}.csv", delimiter=",", skip_header=1)
+    assert nu.shape == (2 * 121, 13)
+    cfgz = lattice_config(*b.shape, 0.8, storage=STORE_F64, inlet_u0=0.02, inlet_amp=0.3,
+                          inlet_period=40.0, inlet_dir=(1, 0, 0))
+    eng = Engine(cfgz)
+    eng.upload_lattice(b)
+    eng.membrane_setup(lattice_membrane_config(xyz, tris, 2), tris, xyz)
+    eng.nucleus_setup(g["nuc_edge"], np.loadtxt(open(tmp_path / "NucleusNodes.dat").readlines()[1:]).reshape(-1, 3))
+    eng.upload_cells(np.round(cells, 6))
+    eng.upload_nuclei(nuclei)
+    eng.ibm_setup(lattice_ibm_config(rbc_interval=2))
+    eng.repulsion_setup(lattice_repulsion_config())
+    eng.wall_setup(*geometry.wall_mesh(b))
+    for it in range(1, steps + 1):
+        eng.full_step(it)
+    xn, vn, fn = eng.nuclei()
+    assert np.abs(xn - nuclei).max() > 1e-6  # the nuclei moved
+    np.testing.assert_allclose(nu[:, :3], xn.reshape(-1, 3), atol=2e-6)
+    np.testing.assert_allclose(nu[:, 8:11], fn.reshape(-1, 3), rtol=2e-7, atol=1e-14)
+    assert np.abs(fn).max() > 0
+    # zf_host again from its restart file at iteration 4: the same end state
+    st4 = out / f"ZFstate-{4:09d}.bin"
+    assert st4.exists()
+    cfg2 = write_case(str(tmp_path), b, cells, xyz, tris, steps - 4, every,
+                      extra + f"\nrestart = {st4}", out="out2")
+    r = subprocess.run([exe, cfg2], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(r.stdout[-1000:], flush=True)
+    assert r.returncode == 0
+    nu2 = np.genfromtxt(tmp_path / "out2" / f"AllCellNucleus-{steps:09d}.csv", delimiter=",",
+                        skip_header=1)
+    np.testing.assert_array_equal(nu2, nu)
     eng.close()

@@ -121,6 +121,9 @@ FUNCS = {
     "zf_download_nuclei": (C.c_int, [P, P, P, P]),
     "zf_nucleus_forces": (C.c_int, [P]),
     "zf_nucleus_contact": (C.c_int, [P, i32, C.POINTER(i32)]),
+    "zf_get_nucleus_info": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32)]),
+    "zf_download_nucleus_lists": (C.c_int, [P, P, P, C.POINTER(i32)]),
+    "zf_upload_nucleus_lists": (C.c_int, [P, P, P, i32]),
     "zf_uvoid_config_reference": (None, [C.POINTER(ZfUvoidConfig)]),
     "zf_uvoid_setup": (C.c_int, [P, C.POINTER(ZfUvoidConfig)]),
     "zf_download_uvoid": (C.c_int, [P, P]),
@@ -166,6 +169,9 @@ HOST_FUNCS = {
     "zfh_write_rbc_triangles": (C.c_int, [C.c_char_p, P, i64]),
     "zfh_write_wall_mesh": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, P, i64, P, P, i64]),
     "zfh_read_cells_csv": (C.c_int, [C.c_char_p, i32, P, P, i32]),
+    "zfh_read_lines": (i64, [C.c_char_p, P, i64]),
+    "zfh_read_nuclei_csv": (C.c_int, [C.c_char_p, i32, P, i32]),
+    "zfh_write_nuclei_csv": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P]),
     "zfh_write_cells_csv": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P, P, P]),
     "zfh_append_volume_area": (C.c_int, [C.c_char_p, i32, i32, P, P, P, P]),
     "zfh_membrane_config_lattice": (None, [P, i32, P, i32, f64, f64, i32, P]),

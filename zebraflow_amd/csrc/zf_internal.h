@@ -64,6 +64,7 @@ struct StateSlot {
   size_t bytes = 0;
   bool force = false;
   int32_t ncell = -1;  // -1: no cells in the slot
+  bool nuc = false, nuc_lists_valid = false;  // nucleus regions held; their lists' validity
 };
 
 template <class T>

@@ -410,6 +410,48 @@ int zf_download_nuclei(zf_ctx *ctx, double *xyz, double *vel, double *force) {
   return ZF_OK;
 }
 
+int zf_get_nucleus_info(zf_ctx *ctx, int32_t *nnuc, int32_t *list_cap) {
+  if (!ctx || !nnuc || !list_cap) return ZF_ERR_ARG;
+  *nnuc = ctx->have_nuc ? ctx->nnuc : 0;
+  *list_cap = ctx->have_nuc ? ctx->ncfg.list_cap : 0;
+  return ZF_OK;
+}
+
+int zf_download_nucleus_lists(zf_ctx *ctx, int32_t *list, int32_t *count, int32_t *valid) {
+  if (!ctx || !list || !count || !valid) return ZF_ERR_ARG;
+  int rc = require_nuc(ctx);
+  if (rc) return rc;
+  const size_t n = (size_t)ctx->ncell * ctx->nnuc, cap = ctx->ncfg.list_cap;
+  *valid = ctx->nuc_lists_valid ? 1 : 0;
+  if (n == 0) return ZF_OK;
+  ZF_HIP(ctx, hipMemcpyAsync(list, ctx->d_nlist, 4 * n * cap, hipMemcpyDeviceToHost, ctx->stream));
+  ZF_HIP(ctx, hipMemcpyAsync(count, ctx->d_nlist_cnt, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ZF_OK;
+}
+
+int zf_upload_nucleus_lists(zf_ctx *ctx, const int32_t *list, const int32_t *count, int32_t valid) {
+  if (!ctx || !list || !count) return ZF_ERR_ARG;
+  int rc = require_nuc(ctx);
+  if (rc) return rc;
+  const size_t n = (size_t)ctx->ncell * ctx->nnuc, cap = ctx->ncfg.list_cap;
+  if (n == 0) return fail(ctx, ZF_ERR_STATE, "upload the cells first");
+  const int32_t ntri = ctx->nt;
+  for (size_t q = 0; q < n; q++) {
+    if (count[q] < 0 || (size_t)count[q] > cap)
+      return fail(ctx, ZF_ERR_ARG, "nucleus node %zu: list count %d outside [0, %zu]", q, count[q], cap);
+    for (int32_t k = 0; k < count[q]; k++)
+      if (list[q * cap + k] < 0 || list[q * cap + k] >= ntri)
+        return fail(ctx, ZF_ERR_ARG, "nucleus node %zu: triangle %d outside the mesh", q,
+                    list[q * cap + k]);
+  }
+  ZF_HIP(ctx, hipMemcpyAsync(ctx->d_nlist, list, 4 * n * cap, hipMemcpyHostToDevice, ctx->stream));
+  ZF_HIP(ctx, hipMemcpyAsync(ctx->d_nlist_cnt, count, 4 * n, hipMemcpyHostToDevice, ctx->stream));
+  ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->nuc_lists_valid = valid != 0;
+  return ZF_OK;
+}
+
 int zf_nucleus_forces(zf_ctx *ctx) {
   if (!ctx) return ZF_ERR_ARG;
   int rc = require_nuc(ctx);

@@ -5,7 +5,8 @@
 // the populations (ZebraflowWT.c:5253-5262), which drops the non-equilibrium
 // part; these checkpoints keep the populations as stored, the force field
 // spread at the last RBC iteration, and the cells (positions, velocities,
-// membrane force, repulsion velocity, status), so that a restored context
+// membrane force, repulsion velocity, status; with nuclei their positions,
+// velocities, forces and contact lists), so that a restored context
 // continues bit-identically.
 #include "zf_internal.h"
 
@@ -36,6 +37,14 @@ std::vector<std::pair<void *, size_t>> regions(zf_ctx *ctx, bool cells) {
     r.push_back({ctx->d_mforce, 24 * NT});
     r.push_back({ctx->d_status, 4 * (size_t)ctx->mcfg.max_cells});
     if (ctx->d_vrep) r.push_back({ctx->d_vrep, 24 * NT});
+    if (ctx->have_nuc) {  // nuclei: positions, velocities, forces, contact lists
+      const size_t NTn = (size_t)ctx->mcfg.max_cells * ctx->nnuc;
+      r.push_back({ctx->d_xn, 24 * NTn});
+      r.push_back({ctx->d_vn, 24 * NTn});
+      r.push_back({ctx->d_fn, 24 * NTn});
+      r.push_back({ctx->d_nlist, 4 * NTn * (size_t)ctx->ncfg.list_cap});
+      r.push_back({ctx->d_nlist_cnt, 4 * (NTn + 1)});
+    }
   }
   return r;
 }
@@ -68,6 +77,8 @@ int zf_checkpoint_save(zf_ctx *ctx, int32_t slot) {
   }
   s.force = ctx->d_force != nullptr;
   s.ncell = cells ? ctx->ncell : -1;
+  s.nuc = cells && ctx->have_nuc;
+  s.nuc_lists_valid = ctx->nuc_lists_valid;
   ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return ZF_OK;
 }
@@ -77,7 +88,8 @@ int zf_checkpoint_restore(zf_ctx *ctx, int32_t slot) {
   auto &sl = ctx->slots;
   if ((size_t)slot >= sl.size() || !sl[slot].buf) return fail(ctx, ZF_ERR_STATE, "slot %d is empty", slot);
   StateSlot &s = sl[slot];
-  if (s.force != (ctx->d_force != nullptr) || (s.ncell >= 0) != ctx->have_mesh)
+  if (s.force != (ctx->d_force != nullptr) || (s.ncell >= 0) != ctx->have_mesh ||
+      (s.ncell >= 0 && s.nuc != ctx->have_nuc))
     return fail(ctx, ZF_ERR_STATE, "slot %d holds a state of another configuration", slot);
   if (s.ncell >= 0) ctx->ncell = s.ncell;
   const auto reg = regions(ctx, s.ncell >= 0);
@@ -90,6 +102,7 @@ int zf_checkpoint_restore(zf_ctx *ctx, int32_t slot) {
     p += (r.second + 255) & ~size_t(255);
   }
   ctx->geom_gen++;  // positions changed: the cell list's bounds expire
+  if (s.nuc) ctx->nuc_lists_valid = s.nuc_lists_valid;
   ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return ZF_OK;
 }

@@ -304,6 +304,19 @@ class Engine:
         assert x.shape == (self.ncell, self.nnuc, 3)
         self._check(self.lib.zf_upload_nuclei(self.h, _ptr(x)), "zf_upload_nuclei")
 
+    def nucleus_lists(self):
+        """(list, count, valid) of zf_download_nucleus_lists."""
+        n = self.ncell * self.nnuc
+        nuc, cap = C.c_int32(), C.c_int32()
+        self._check(self.lib.zf_get_nucleus_info(self.h, C.byref(nuc), C.byref(cap)),
+                    "zf_get_nucleus_info")
+        lst = np.zeros((n, cap.value), np.int32)
+        cnt = np.zeros(n, np.int32)
+        valid = C.c_int32()
+        self._check(self.lib.zf_download_nucleus_lists(self.h, _ptr(lst), _ptr(cnt), C.byref(valid)),
+                    "zf_download_nucleus_lists")
+        return lst, cnt, valid.value
+
     def nuclei(self):
         out = [np.zeros((self.ncell, self.nnuc, 3)) for _ in range(3)]
         self._check(self.lib.zf_download_nuclei(self.h, *[_ptr(a) for a in out]), "zf_download_nuclei")

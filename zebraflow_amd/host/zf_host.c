@@ -19,6 +19,10 @@
  *   inlet = U0 AMP PERIOD DX DY DZ  velocity inlet u0 (1 + amp sin(2 pi t/period)) dir
  *   outlet_rho = 1.0
  *   rbc_nodes = ZFISHRBCNODES.dat  rbc_triangles = ZFISHRBCTRIANGLES.dat (:1601-1626)
+ *   nucleus_nodes = NucleusNodes.dat  nucleus_edges = NucleusEdges.dat (:1903-1922)
+ *   nuclei = AllCellNucStart.csv   the start cells' nuclei (:5530-5585); with
+ *                                  these three the cells are nucleated and the
+ *                                  loop also writes AllCellNucleus-%09d.csv
  *   cells = AllCellMembStart.csv   (AllCellMemb layout, :5327-5430)
  *   max_cells = N                  shear_modulus = ES   bending_modulus = KB
  *   rbc_interval = 50              gamma_p = G          repulsion = 0 | 1
@@ -207,6 +211,30 @@ int main(int argc, char **argv) {
     if ((rc = zf_membrane_setup(ctx, &mc, (int32_t)nn, (int32_t)nt, tri, rest)))
       return die(ctx, rc, "zf_membrane_setup");
     if ((rc = zf_upload_cells(ctx, ncell, xyz))) return die(ctx, rc, "zf_upload_cells");
+    if (get("nucleus_nodes") && get("nucleus_edges") && get("nuclei")) {
+      const int64_t nnuc = zfh_read_points(get("nucleus_nodes"), NULL, 0);
+      const int64_t nen = zfh_read_lines(get("nucleus_edges"), NULL, 0);
+      if (nnuc <= 0 || nen <= 0) return die(ctx, ZF_ERR_ARG, "reading the nucleus network");
+      double *nrest = malloc(sizeof(double) * 3 * nnuc);
+      int32_t *nedge = malloc(sizeof(int32_t) * 2 * nen);
+      double *nxyz = malloc(sizeof(double) * 3 * nnuc * (size_t)cap);
+      if (!nrest || !nedge || !nxyz || zfh_read_points(get("nucleus_nodes"), nrest, nnuc) != nnuc ||
+          zfh_read_lines(get("nucleus_edges"), nedge, nen) != nen)
+        return die(ctx, ZF_ERR_ARG, "reading the nucleus network");
+      if (zfh_read_nuclei_csv(get("nuclei"), (int)nnuc, nxyz, cap) != ncell)
+        return die(ctx, ZF_ERR_ARG, "reading the nuclei (one per start cell)");
+      zf_nucleus_config nc;
+      zf_nucleus_config_reference(&nc);
+      if ((rc = zf_nucleus_setup(ctx, &nc, (int32_t)nnuc, (int32_t)nen, nedge, nrest)))
+        return die(ctx, rc, "zf_nucleus_setup");
+      if ((rc = zf_upload_nuclei(ctx, nxyz))) return die(ctx, rc, "zf_upload_nuclei");
+      run.nnuc = (int32_t)nnuc;
+      printf("zf_host: nucleated cells, %lld nucleus nodes, %lld filaments\n", (long long)nnuc,
+             (long long)nen);
+      free(nrest);
+      free(nedge);
+      free(nxyz);
+    }
     zf_ibm_config ic;
     zf_ibm_config_reference(&ic);
     if (!reference) {

@@ -109,6 +109,13 @@ int zfh_write_wall_mesh(const char *nodes_path, const char *cells_path, const ch
 /* Cells in the AllCellMemb layout (ZebraflowWT.c:435-446, read back as
  * AllCellMembStart, :5327-5430): 21 columns per node row. */
 int zfh_read_cells_csv(const char *path, int nn, double *xyz, int32_t *status, int cap_cells);
+/* Nucleus files: NucleusEdges.dat ("LINES n m", ZebraflowWT.c:1903-1911;
+ * the nodes use zfh_read_points), the start nuclei AllCellNucStart.csv
+ * (:5530-5585) and the output AllCellNucleus-%09d.csv (:453-463). */
+int64_t zfh_read_lines(const char *path, int32_t *edge, int64_t cap);
+int zfh_read_nuclei_csv(const char *path, int nnuc, double *xyz, int cap_cells);
+int zfh_write_nuclei_csv(const char *path, int ncell, int nnuc, const double *xyz,
+                         const double *vel, const double *force, const int32_t *status);
 int zfh_write_cells_csv(const char *path, int ncell, int nn, const double *xyz, const double *vel,
                         const int32_t *status, const int32_t *wall_tri, const double *vrep,
                         const double *force);
@@ -161,6 +168,7 @@ typedef struct zfh_run_config {
   int32_t ncell, nn, nt;  /* cells on the context (0: none) */
   int32_t have_rep, have_wall;
   int32_t checkpoint_every; /* restart file ZFstate-%09d.bin when iter % checkpoint_every == 0 */
+  int32_t nnuc;           /* nucleus nodes per cell (0: none): AllCellNucleus output */
 } zfh_run_config;
 
 /* The reference's time loop (ZebraflowWT.c:5646-12412): zf_step per

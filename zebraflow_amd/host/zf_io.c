@@ -361,6 +361,73 @@ int zfh_write_cells_csv(const char *path, int ncell, int nn, const double *xyz, 
   return fclose(f) == 0 ? 0 : -1;
 }
 
+/* "LINES n m" then n pairs of 1-based node ids (NucleusEdges.dat,
+ * ZebraflowWT.c:1903-1911).  Returns n (edge NULL: count only), -2 past cap. */
+int64_t zfh_read_lines(const char *path, int32_t *edge, int64_t cap) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  char kw[32];
+  long long n = 0, m = 0;
+  if (fscanf(f, "%31s %lld %lld", kw, &n, &m) != 3 || strcmp(kw, "LINES") != 0 || n < 0) {
+    fclose(f);
+    return -1;
+  }
+  if (!edge) { fclose(f); return n; }
+  if (n > cap) { fclose(f); return -2; }
+  for (long long i = 0; i < 2 * n; i++)
+    if (fscanf(f, "%d", &edge[i]) != 1) { fclose(f); return -1; }
+  fclose(f);
+  return n;
+}
+
+/* The nuclei of the start cells, AllCellNucStart.csv (ZebraflowWT.c:5530-5585;
+ * the layout OUTPUT() writes as AllCellNucleus, :453-463): a header, then
+ * nnuc rows per cell, X,Y,Z first.  Returns the cells read. */
+int zfh_read_nuclei_csv(const char *path, int nnuc, double *xyz, int cap_cells) {
+  FILE *f = fopen(path, "r");
+  if (!f || nnuc <= 0) { if (f) fclose(f); return -1; }
+  if (skip_lines(f, 1)) { fclose(f); return -1; }
+  char line[1024];
+  int64_t row = 0;
+  while (fgets(line, sizeof line, f)) {
+    double x, y, z;
+    if (sscanf(line, "%lf,%lf,%lf", &x, &y, &z) != 3) break;
+    const int64_t c = row / nnuc, node = row % nnuc;
+    if (c >= cap_cells) { fclose(f); return -2; }
+    xyz[3 * (c * nnuc + node)] = x;
+    xyz[3 * (c * nnuc + node) + 1] = y;
+    xyz[3 * (c * nnuc + node) + 2] = z;
+    row++;
+  }
+  fclose(f);
+  if (row % nnuc) return -1;
+  return (int)(row / nnuc);
+}
+
+/* AllCellNucleus-%09d.csv (ZebraflowWT.c:453-463): X,Y,Z,Cell,CellStatus,
+ * Vx,Vy,Vz,Fx,Fy,Fz,Neigh,Neighflag per nucleus node of every cell with
+ * status > -5000; vel and force ncell*nnuc*3 (NULL: zeros); the neighbour
+ * columns (the reference's list bookkeeping) are written as 0. */
+int zfh_write_nuclei_csv(const char *path, int ncell, int nnuc, const double *xyz,
+                         const double *vel, const double *force, const int32_t *status) {
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "X,Y,Z,Cell,CellStatus,Vx,Vy,Vz,Fx,Fy,Fz,Neigh,Neighflag\n");
+  for (int c = 0; c < ncell; c++) {
+    if (status && status[c] <= -5000) continue;
+    for (int n = 0; n < nnuc; n++) {
+      const size_t q = (size_t)c * nnuc + n;
+      const double *x = &xyz[3 * q];
+      const double v0 = vel ? vel[3 * q] : 0, v1 = vel ? vel[3 * q + 1] : 0, v2 = vel ? vel[3 * q + 2] : 0;
+      const double f0 = force ? force[3 * q] : 0, f1 = force ? force[3 * q + 1] : 0,
+                   f2 = force ? force[3 * q + 2] : 0;
+      fprintf(f, "%f,%f,%f,%d,%d,%.7e,%.7e,%.7e,%.7e,%.7e,%.7e,%d,%d\n", x[0], x[1], x[2], c + 1,
+              status ? status[c] : 1, v0, v1, v2, f0, f1, f2, 0, 0);
+    }
+  }
+  return fclose(f) == 0 ? 0 : -1;
+}
+
 /* Appends one row per live cell to VolumeAreaHistory.dat
  * (ZebraflowWT.c:346-351): Iteration#,Cell,Vol,Area,CenX,CenY,CenZ,
  * CellState,RegressFlag; the header is written when the file is new. */

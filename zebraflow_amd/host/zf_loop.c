@@ -79,7 +79,24 @@ static int output_cells(zf_ctx *ctx, const zfh_run_config *rc, int iter) {
     goto done;
   }
   snprintf(path, sizeof path, "%s/VolumeAreaHistory.dat", rc->out_dir);
-  if (zfh_append_volume_area(path, iter, rc->ncell, vol, area, cen, status)) err = ZF_ERR_ARG;
+  if (zfh_append_volume_area(path, iter, rc->ncell, vol, area, cen, status)) {
+    err = ZF_ERR_ARG;
+    goto done;
+  }
+  if (rc->nnuc > 0) {
+    const size_t nqn = (size_t)rc->ncell * rc->nnuc;
+    double *nb = malloc(sizeof(double) * 9 * (nqn ? nqn : 1));
+    if (!nb) {
+      err = ZF_ERR_NOMEM;
+      goto done;
+    }
+    if (!(err = zf_download_nuclei(ctx, nb, nb + 3 * nqn, nb + 6 * nqn))) {
+      snprintf(path, sizeof path, "%s/AllCellNucleus-%09d.csv", rc->out_dir, iter);
+      if (zfh_write_nuclei_csv(path, rc->ncell, rc->nnuc, nb, nb + 3 * nqn, nb + 6 * nqn, status))
+        err = ZF_ERR_ARG;
+    }
+    free(nb);
+  }
 done:
   free(buf);
   free(ib);
