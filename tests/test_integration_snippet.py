@@ -21,6 +21,9 @@ PRELUDE = r"""
 #define KM_1 121
 #define NodeM 642
 #define TriM 1280
+#define NodeM_N 121
+#define EdgeM_N 544
+#define CapsM 8
 #define NSTEPS 100
 #define OUTPUTSTEP 10
 int B_index[IM + 1][JM + 1][KM + 1];
@@ -36,6 +39,8 @@ int32_t *parent_b[3];
 double *parent_f[3];
 double D1[IM * JM * KM], U1[IM * JM * KM], V1[IM * JM * KM], W1[IM * JM * KM];
 double vel[3 * NodeM];
+int EDGENODE_LIST_N[EdgeM_N + 1][3];
+double NODECOORD_N[3][CapsM + 1][NodeM_N + 1];
 int32_t status[8];
 double area[8], vol[8];
 double dJ_dx0[3 * NodeM], dJ_df0[19 * IM * JM * KM], dJ_dF[3 * IM * JM * KM];
@@ -43,7 +48,7 @@ double dJ_dx0[3 * NodeM], dJ_df0[19 * IM * JM * KM], dJ_dF[3 * IM * JM * KM];
 
 EPILOGUE = r"""
 int main(void) {
-  (void)zf_bind_setup; (void)zf_bind_loop; (void)zf_bind_parents;
+  (void)zf_bind_setup; (void)zf_bind_nuclei; (void)zf_bind_loop; (void)zf_bind_parents;
   (void)zf_bind_parent_steps; (void)zf_bind_gradient;
   return 0;
 }
@@ -53,7 +58,7 @@ int main(void) {
 def test_integration_snippets_compile_and_link(tmp_path):
     text = open(os.path.join(REPO, "INTEGRATION.md")).read()
     blocks = re.findall(r"```c\n(.*?)```", text, flags=re.S)
-    assert len(blocks) >= 5
+    assert len(blocks) >= 6
     src = tmp_path / "binding.c"
     src.write_text(PRELUDE + "\n".join(blocks) + EPILOGUE)
     exe = tmp_path / "binding"
