@@ -420,6 +420,64 @@ int zf_step(zf_ctx *ctx, int32_t iter);
  * ZebraflowWT.c:5646-12412, without output). */
 int zf_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps);
 
+/* ---- cells on the parent lattices ------------------------------------------
+ * The reference's parent lattices carry cells of their own that feed the
+ * main lattice (the region of interest, ROI).  A parent's cells live in a
+ * periodic window along k, [Z0_p, ZM_p] with ZM_p = Z0_p + (KM_p - 1) DX:
+ *   - every node is kept inside the window, wrapping by L = ZM_p - Z0_p
+ *     when it leaves (:12196-12340), and the parent's stages address the
+ *     lattice with k modulo KM_p - 1 (interpolation and wall repulsion
+ *     :7541-8688, cell list :5898-6000, cell-cell repulsion :10199-10733,
+ *     spreading :11179-11917); a cell across the wrap (status > 3, nodes
+ *     spanning more than split_span) is unwrapped for the membrane and
+ *     nucleus stages (:5769-5797, :6738-6756, :7410-7420);
+ *   - status 2 is a cell inside the window; when one of its nodes leaves
+ *     through the exit (ZM_p for direction +1, Z0_p for -1) and its
+ *     REGRESSFLAG is 0, the cell is cloned into the first free slot (status
+ *     -5000) of the main lattice, unwrapped so that it straddles the exit
+ *     plane, with status -(id_base + slot + 1); the parent cell becomes
+ *     status 4 with REGRESSFLAG 1 (:12345-12403), and returns to status 2
+ *     inside the window's first three quarters (:12323-12336).  REGRESSFLAG
+ *     returns to 0 when the cell's centroid is in the window's middle half
+ *     and it is not across the wrap (:6762-6778);
+ *   - in the main lattice a clone is a live cell whose nodes within
+ *     clone_band*DX past the exit plane, or behind it, move with the same
+ *     node of the parent cell and do not spread (:9295-9316, :11002-11015,
+ *     :11927-11958); once all its membrane nodes lie in (roi_lo, roi_hi) it
+ *     is released into status 1 (:11960-11962).
+ * A main lattice's statuses: > 0 live, -4998 < s < 0 a clone, -5000 free.
+ * Give the main lattice free slots for the clones (zf_upload_cells with
+ * room, zf_upload_cell_status with -5000).  Not covered: the main lattice's
+ * exits back into the parents (statuses 3001/3002, :12017-12160). */
+typedef struct zf_cell_wrap_config {
+  int32_t direction;    /* +1: cells leave at ZM_p (parent 1); -1: at Z0_p (parents 2, 3) */
+  int32_t id_base;      /* the reference's number of the parent's first cell slot, minus 1 */
+  double split_span;    /* 30: a cell with status > 3 spanning more lies across the wrap */
+  double clone_band;    /* 5: band past the exit plane, in DX, where clone nodes follow */
+  double roi_lo, roi_hi; /* release planes of the clones in the main lattice (ZM_1, Z0_2) */
+} zf_cell_wrap_config;
+
+/* The reference's values for its parent p = 1..3 (ZebraflowWT.c:91-112,
+ * slots 1-188, 189-329, 330-344). */
+void zf_cell_wrap_config_reference(zf_cell_wrap_config *wc, int32_t p);
+/* Makes parent lattice `parent` (linked to `main` with zf_parent_link, both
+ * with the same membrane mesh, immersed-boundary set-up and RBC interval,
+ * and both with or without a nucleus) carry cells in its periodic window,
+ * cloning into `main`.  Uploaded parent cells start with status 2 and
+ * REGRESSFLAG 0 (zf_upload_cell_status may set 4). */
+int zf_cell_wrap_setup(zf_ctx *parent, zf_ctx *main, const zf_cell_wrap_config *wc);
+/* One iteration of a main lattice and its linked parents in the
+ * reference's order: the LBM steps of the main lattice and of each parent;
+ * then, on an RBC iteration, the cell stages of every parent and of the
+ * main lattice (up to spreading), the main lattice's position update
+ * (clones with their parent cells' velocities), the parents' wrapped
+ * updates and the cloning, parent by parent. */
+int zf_step_linked(zf_ctx *main, int32_t iter);
+/* REGRESSFLAG and CLONEFLAG (of the last update) of a parent's cells,
+ * ncell each (either may be NULL); the upload sets REGRESSFLAG. */
+int zf_download_cell_flags(zf_ctx *ctx, int32_t *regress, int32_t *cloneflag);
+int zf_upload_cell_flags(zf_ctx *ctx, const int32_t *regress);
+
 /* ---- wall shear stress and wall pressure ----------------------------------
  * At n wall-mesh points (xyz: n*3, unit normals into the fluid: n*3), from
  * the current populations (PostProWSSandPressure.c:267-369):
@@ -542,8 +600,9 @@ int zf_membrane_vjp(zf_ctx *ctx, const double *lam_force, double *lam_pos, doubl
  * the configuration it was saved from. */
 int zf_checkpoint_save(zf_ctx *ctx, int32_t slot);
 int zf_checkpoint_restore(zf_ctx *ctx, int32_t slot);
-/* Cell status (ncell; >= 0 live, < 0 removed), for restarting from files
- * (zf_upload_cells marks every uploaded cell live). */
+/* Cell status (ncell; > -4998 live, -5000 removed or free; see the cells
+ * on the parent lattices for 2, 4 and the clones' negative statuses), for
+ * restarting from files (zf_upload_cells marks every uploaded cell live). */
 int zf_upload_cell_status(zf_ctx *ctx, const int32_t *status);
 
 /* ---- timing and synchronisation ------------------------------------------ */

@@ -202,10 +202,12 @@ class LbmOracle:
         return self.b[d[:, 0] - 1, d[:, 1] - 1, d[:, 2] - 1]
 
 
-def step_with_parents(main, parents, it, rbc_step=0, nthreads=0):
+def step_with_parents(main, parents, it, rbc_step=0, nthreads=0, parent_rbc_step=0):
     """One iteration of a main lattice and its parent lattices (FP64) in the
     reference's order (ZebraflowWT.c:6101-6626): every stream + collide, the
-    main boundary pass, the parents' boundary passes, every macro + copy."""
+    main boundary pass, the parents' boundary passes, every macro + copy
+    (rbc_step, parent_rbc_step: the force field is spread again after this
+    step, on the main lattice and on the parents carrying cells)."""
     main.phase(0, it, nthreads=nthreads)
     for p in parents:
         p.phase(0, it, nthreads=nthreads)
@@ -217,7 +219,7 @@ def step_with_parents(main, parents, it, rbc_step=0, nthreads=0):
                                       _p(main.V), _p(main.W), int(nthreads))
     main.phase(2, it, rbc_step=rbc_step, nthreads=nthreads)
     for p in parents:
-        p.phase(2, it, nthreads=nthreads)
+        p.phase(2, it, rbc_step=parent_rbc_step, nthreads=nthreads)
 
 
 def wtime():
@@ -507,3 +509,167 @@ def wall_repulsion(p, r, windex, wall_xyz, wall_tri, wall_normal, wall_node_tri,
                              _p(np.ascontiguousarray(wall_node_tri, np.int32)), nc, nn,
                              _p(np.ascontiguousarray(status, np.int32)), _p(x), _p(v), _p(near))
     return v, near
+
+
+# ---- cells on a parent lattice (zf_oracle_wrap.c) ---------------------------
+class Wrap(C.Structure):
+    _fields_ = [("dir", C.c_int), ("z0", C.c_double), ("zm", C.c_double)]
+
+
+def wrap_of(p, direction):
+    """The parent's cell window along k: Z0_p = z0, ZM_p = z0 + (km-1) dx."""
+    w = Wrap()
+    w.dir, w.z0, w.zm = direction, p.z0, p.z0 + (p.km - 1) * p.dx
+    return w
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def wrap_interpolate(p, w, b, domid, uvw, xyz, status, windex=None, uvoid=None):
+    x = _f64(xyz)
+    nc, nn, _ = x.shape
+    v = np.zeros_like(x)
+    u = [_f64(a) for a in uvw]
+    wi = _i32(windex) if uvoid is not None else None
+    uv = _f64(uvoid) if uvoid is not None else None
+    lib().zfo_wrap_interpolate(C.byref(p), C.byref(w), _p(_i32(b)), _p(_i32(domid)), _p(u[0]),
+                               _p(u[1]), _p(u[2]), _p(wi) if wi is not None else None,
+                               _p(uv) if uv is not None else None, nc, nn, _p(_i32(status)),
+                               _p(x), _p(v))
+    return v
+
+
+def wrap_wall_repulsion(p, w, r, windex, wall_xyz, wall_tri, wall_normal, wall_node_tri, status,
+                        xyz, vrep=None):
+    x = _f64(xyz)
+    nc, nn, _ = x.shape
+    v = np.zeros_like(x) if vrep is None else vrep
+    near = np.zeros((nc, nn), np.int32)
+    lib().zfo_wrap_wall_repulsion(C.byref(p), C.byref(w), C.byref(r), _p(_i32(windex)),
+                                  _p(_f64(wall_xyz)), _p(_i32(wall_tri)), _p(_f64(wall_normal)),
+                                  _p(_i32(wall_node_tri)), nc, nn, _p(_i32(status)), _p(x), _p(v),
+                                  _p(near))
+    return v, near
+
+
+def wrap_cell_list(p, w, b, domid, tri_centroid, status):
+    tc = _f64(tri_centroid)
+    nc, nt, _ = tc.shape
+    out = np.zeros((nc, nt), np.int32)
+    lib().zfo_wrap_cell_list(C.byref(p), C.byref(w), _p(_i32(b)), _p(_i32(domid)), nc, nt,
+                             _p(_i32(status)), _p(tc), _p(out))
+    return out
+
+
+def wrap_cell_repulsion(p, w, r, b, domid, tris, status, xyz, node_normal, tri_centroid,
+                        tri_normal, bins, vrep=None):
+    x = _f64(xyz)
+    nc, nn, _ = x.shape
+    v = np.zeros_like(x) if vrep is None else vrep
+    lib().zfo_wrap_cell_repulsion(C.byref(p), C.byref(w), C.byref(r), _p(_i32(b)),
+                                  _p(_i32(domid)), nc, nn, len(tris), _p(_i32(tris)),
+                                  _p(_i32(status)), _p(x), _p(_f64(node_normal)),
+                                  _p(_f64(tri_centroid)), _p(_f64(tri_normal)), _p(_i32(bins)),
+                                  _p(v))
+    return v
+
+
+def wrap_spread(p, w, b, domid, dommax, xyz, ffluid, status, force=None):
+    x = _f64(xyz)
+    nc, nn, _ = x.shape
+    out = np.zeros((3, dommax)) if force is None else force
+    lib().zfo_wrap_spread(C.byref(p), C.byref(w), _p(_i32(b)), _p(_i32(domid)),
+                          C.c_int64(dommax), nc, nn, _p(_i32(status)), _p(x),
+                          _p(_f64(ffluid)), _p(out))
+    return out
+
+
+def wrap_advance(w, dt, xyz, vel, status, regress, xyzn=None, veln=None):
+    """Moves and wraps in place; returns the clone flags."""
+    nc, nn, _ = xyz.shape
+    nnuc = 0 if xyzn is None else xyzn.shape[1]
+    flags = np.zeros(nc, np.int32)
+    assert xyz.flags.c_contiguous and status.dtype == np.int32 and regress.dtype == np.int32
+    lib().zfo_wrap_advance(C.byref(w), C.c_double(dt), nc, nn, nnuc, _p(xyz), _p(_f64(vel)),
+                           _p(xyzn) if xyzn is not None else None,
+                           _p(_f64(veln)) if veln is not None else None, _p(status),
+                           _p(regress), _p(flags))
+    return flags
+
+
+def wrap_span(xyz, xyzn, status):
+    """CAPS_MAX/MIN along z over membrane and nucleus nodes (ZebraflowWT.c:
+    5709-5741), per cell."""
+    z = xyz[..., 2]
+    if xyzn is not None:
+        z = np.concatenate([z, xyzn[..., 2]], axis=1)
+    return z.min(axis=1), z.max(axis=1)
+
+
+def wrap_unwrap(w, z, status, zmin, zmax, split=30.0):
+    """The coordinates the membrane stages use (:5769-5797, :6738-6756): a
+    cell with status > 3 spanning more than `split` has its nodes behind the
+    window's middle moved by L past its exit."""
+    L, mid = w.zm - w.z0, 0.5 * (w.z0 + w.zm)
+    out = np.array(z, dtype=np.float64, copy=True)
+    cut = (status > 3) & (zmax - zmin > split)
+    for c in np.nonzero(cut)[0]:
+        zc = out[c]
+        if w.dir > 0:
+            zc[zc < mid] += L
+        else:
+            zc[zc > mid] -= L
+    return out, cut
+
+
+def wrap_regress_reset(w, status, regress, zmin, zmax, centroid_z, split=30.0):
+    """REGRESSFLAG back to 0 for a cell not across the wrap whose centroid
+    lies in the window's middle half (:6762-6778)."""
+    L = w.zm - w.z0
+    ok = (status > -5000) & (zmax - zmin < split) & (centroid_z > w.z0 + 0.25 * L) & \
+        (centroid_z < w.z0 + 0.75 * L)
+    regress[ok] = 0
+
+
+def clone_cells(w, id_base, pstatus, pregress, cloneflag, pxyz, pvel, mstatus, mxyz, mvel,
+                pxyzn=None, pveln=None, mxyzn=None, mveln=None):
+    """The serial cloning of :12345-12403 for one parent: each flagged cell
+    (in slot order) goes to the first free main slot (status -5000) with
+    status -(id_base + slot + 1), its nodes behind the window's middle moved
+    by L past the exit; the parent cell becomes status 4 with REGRESSFLAG 1
+    whether or not a slot was free.  Returns the (parent, main) slot pairs."""
+    L, mid = w.zm - w.z0, 0.5 * (w.z0 + w.zm)
+    pairs = []
+    for s in np.nonzero(cloneflag)[0]:
+        free = np.nonzero(mstatus == -5000)[0]
+        if len(free):
+            m = free[0]
+            mstatus[m] = -(id_base + s + 1)
+            for src, dst, vs, vd in ((pxyz, mxyz, pvel, mvel), (pxyzn, mxyzn, pveln, mveln)):
+                if src is None:
+                    continue
+                x = src[s].copy()
+                if w.dir > 0:
+                    x[x[:, 2] < mid, 2] += L
+                else:
+                    x[x[:, 2] > mid, 2] -= L
+                dst[m] = x
+                vd[m] = vs[s]
+            pairs.append((s, m))
+        pstatus[s] = 4
+        pregress[s] = 1
+    return pairs
+
+
+def clone_band(w, z, band):
+    """Clone nodes that move with the parent cell (:11933-11941): within
+    band (5 DX) past the exit plane or behind it."""
+    if w.dir > 0:
+        return ~(z > w.zm + band)
+    return ~(z < w.z0 - band)

@@ -254,6 +254,47 @@ void zfo_wall_repulsion(const zfo_ibm_params *p, const zfo_rep_params *r, const 
                         const double *wall_normal, const int32_t *wall_node_tri, int ncell, int nn,
                         const int32_t *status, const double *xyz, double *vrep, int32_t *nearest);
 
+/* ---- cells on a parent lattice (zf_oracle_wrap.c) -------------------------
+ * The parent's periodic cell window along k: Z0_p = z0, ZM_p = zm =
+ * z0 + (km - 1) dx; dir +1: cells leave at zm (parent 1), -1: at z0
+ * (parents 2 and 3).  Node coordinates are the reference's (wrapped into
+ * the window node by node); status 2 (inside) or 4 (crossing), -5000 free. */
+typedef struct zfo_wrap {
+  int dir;
+  double z0, zm;
+} zfo_wrap;
+
+/* Interpolation (:7541-7732, :8101-8297); vel: ncell*nn*3 (set, not added);
+ * windex and uvoid may be NULL. */
+void zfo_wrap_interpolate(const zfo_ibm_params *p, const zfo_wrap *w, const int32_t *b,
+                          const int32_t *domid, const double *U, const double *V, const double *W,
+                          const int32_t *windex, const double *uvoid, int ncell, int nn,
+                          const int32_t *status, const double *xyz, double *vel);
+/* Wall repulsion of the interpolation loop (same lines); adds to vrep. */
+void zfo_wrap_wall_repulsion(const zfo_ibm_params *p, const zfo_wrap *w, const zfo_rep_params *r,
+                             const int32_t *windex, const double *wall_xyz, const int32_t *wall_tri,
+                             const double *wall_normal, const int32_t *wall_node_tri, int ncell,
+                             int nn, const int32_t *status, const double *xyz, double *vrep,
+                             int32_t *nearest);
+/* Cell list (:5898-5950); tri_centroid as the membrane stage computes it. */
+void zfo_wrap_cell_list(const zfo_ibm_params *p, const zfo_wrap *w, const int32_t *b,
+                        const int32_t *domid, int ncell, int nt, const int32_t *status,
+                        const double *tri_centroid, int32_t *bin);
+/* Cell-cell repulsion (:10204-10383); adds to vrep. */
+void zfo_wrap_cell_repulsion(const zfo_ibm_params *p, const zfo_wrap *w, const zfo_rep_params *r,
+                             const int32_t *b, const int32_t *domid, int ncell, int nn, int nt,
+                             const int32_t *tri, const int32_t *status, const double *xyz,
+                             const double *node_normal, const double *tri_centroid,
+                             const double *tri_normal, const int32_t *bin, double *vrep);
+/* Spreading (:11184-11310, :11547-11669); force 3*dommax, added. */
+void zfo_wrap_spread(const zfo_ibm_params *p, const zfo_wrap *w, const int32_t *b,
+                     const int32_t *domid, int64_t dommax, int ncell, int nn,
+                     const int32_t *status, const double *xyz, const double *ffluid, double *force);
+/* Position update with the wrap and clone flags (:12196-12340). */
+void zfo_wrap_advance(const zfo_wrap *w, double dt, int ncell, int nn, int nnuc, double *xyz,
+                      const double *vel, double *xyzn, const double *veln, int32_t *status,
+                      int32_t *regress, int32_t *cloneflag);
+
 /* Wall-clock helper for the CPU baseline. */
 double zfo_wtime(void);
 int zfo_max_threads(void);

@@ -55,6 +55,11 @@ class ZfNucleusConfig(C.Structure):
                                    "k_contact")] + [("list_interval", i32), ("list_cap", i32)]
 
 
+class ZfCellWrapConfig(C.Structure):
+    _fields_ = [("direction", i32), ("id_base", i32)] + [
+        (n, f64) for n in ("split_span", "clone_band", "roi_lo", "roi_hi")]
+
+
 class ZfAdjointConfig(C.Structure):
     _fields_ = [("checkpoint_interval", i32), ("max_steps", i32), ("t_begin", i32), ("t_end", i32),
                 ("mu_over_dx", f64), ("cells", i32), ("reserved", i32)]
@@ -88,6 +93,11 @@ FUNCS = {
     "zf_set_parent": (C.c_int, [P, i32, i32, P]),
     "zf_download_parent_lists": (C.c_int, [P, P]),
     "zf_parent_link": (C.c_int, [P, P]),
+    "zf_cell_wrap_config_reference": (None, [P, i32]),
+    "zf_cell_wrap_setup": (C.c_int, [P, P, P]),
+    "zf_step_linked": (C.c_int, [P, i32]),
+    "zf_download_cell_flags": (C.c_int, [P, P, P]),
+    "zf_upload_cell_flags": (C.c_int, [P, P]),
     "zf_membrane_config_reference": (None, [C.POINTER(ZfMembraneConfig)]),
     "zf_membrane_setup": (C.c_int, [P, C.POINTER(ZfMembraneConfig), i32, i32, P, P]),
     "zf_download_mesh_topology": (C.c_int, [P, C.POINTER(i32), P, P, P]),

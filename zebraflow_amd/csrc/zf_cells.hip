@@ -68,7 +68,7 @@ __global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double
   const bool live = q < ncell * nt;
   int bin = 0;
   double bdist = 0, ext = 0;
-  if (live && status[q / nt] >= 0) {
+  if (live && status[q / nt] > kStatusDead) {
     const double x = tg[4 * TT + q], y = tg[5 * TT + q], z = tg[6 * TT + q];
     double mind = 5000.0;
     int bi = 0, bj = 0, bk = 0;
@@ -190,7 +190,8 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
   const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
   const int c = gq / nn, node = gq % nn;
-  if (status[c] <= 0) return;
+  // released cells and clones (the reference's :9920-9921, :10055)
+  if (status[c] == 0 || status[c] <= kStatusDead) return;
   const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
   const double e1x = nnormal[gq], e1y = nnormal[NT + gq], e1z = nnormal[2 * NT + gq];
   int lo[3], hi[3];
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
   const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
   nearest[gq] = 0;
-  if (status[gq / nn] <= 0) return;
+  if (status[gq / nn] == 0 || status[gq / nn] <= kStatusDead) return;
   const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
   // Whole-node skip: with q0 the lattice node nearest x and d0 its distance
   // to its nearest wall node (W_index), every wall node is at least
@@ -414,6 +415,197 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
       }
     }
   drain();
+  if (triwall == 0) return;
+  nearest[gq] = triwall;
+  const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
+  const double *n = &wnormal[3 * (size_t)(triwall - 1)];
+  const double d = (x - w1[0]) * n[0] + (y - w1[1]) * n[1] + (z - w1[2]) * n[2];
+  if (d < r.wall_d0 && fabs(d) < r.wall_dmax) {
+    double f = (r.wall_d0 - d) / r.wall_dmax;
+    if (d < r.wall_d0 - r.wall_dmax) f = 1.0;
+    vrep[gq] += f * r.k_wall * n[0];
+    vrep[NT + gq] += f * r.k_wall * n[1];
+    vrep[2 * NT + gq] += f * r.k_wall * n[2];
+  }
+}
+
+// ---- a parent lattice's cells: the periodic window along k (zf_wrap.hip)
+struct WrapCells {
+  int dir;
+  double z0, zm, L, mid;
+};
+
+__device__ __forceinline__ int kmap_ibm(int k, int km) {
+  return k < 1 ? k + (km - 1) : k > km ? k - (km - 1) : k;
+}
+
+// Cell list of a parent (ZebraflowWT.c:5898-5950): a triangle of a cell
+// not in status 2 whose centroid lies past the exit is binned from inside
+// the window; window planes beyond the box wrap to k -+ (KM - 1) at their
+// unwrapped place; the nearest listed node wins with no cut-off.
+__global__ void k_wrap_tri_bin(const double *__restrict__ tg, size_t TT, int nt, int ncell,
+                               const int32_t *__restrict__ status, const int32_t *__restrict__ b,
+                               const int32_t *__restrict__ domid, Grid g, WrapCells w, int dommax,
+                               int32_t *__restrict__ tbin, uint64_t *__restrict__ keys) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncell * nt) return;
+  const int st = status[q / nt];
+  int bin = 0;
+  if (st > kStatusDead) {
+    const double x = tg[4 * TT + q], y = tg[5 * TT + q];
+    double z = tg[6 * TT + q];
+    if (st != 2) {
+      if (w.dir > 0 && z >= w.zm) z -= w.L;
+      if (w.dir < 0 && z <= w.z0) z += w.L;
+    }
+    const int i = (int)((x - g.x0) / g.dx + 1.01), j = (int)((y - g.y0) / g.dx + 1.01),
+              k = (int)((z - g.z0) / g.dx + 1.01);
+    const int i1 = max(1, i - 4), i2 = min(g.im, i + 4), j1 = max(1, j - 4), j2 = min(g.jm, j + 4);
+    double mind = 5000.0;
+    for (int ii = i1; ii <= i2; ii++)
+      for (int jj = j1; jj <= j2; jj++)
+        for (int kk = k - 4; kk <= k + 4; kk++) {
+          const int km = kk > g.km ? kk - g.km + 1 : kk < 1 ? kk + g.km - 1 : kk;
+          const size_t bqq = bq(g, ii, jj, km);
+          if (b[bqq] < 0) continue;
+          const double dx = g.x0 + (ii - 1) * g.dx - x, dy = g.y0 + (jj - 1) * g.dx - y,
+                       dz = g.z0 + (kk - 1) * g.dx - z;
+          const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          if (d < mind) {
+            mind = d;
+            bin = domid[bqq];
+          }
+        }
+  }
+  tbin[q] = bin;
+  const uint64_t key_bin = bin > 0 ? (uint64_t)(bin - 1) : (uint64_t)dommax;
+  keys[q] = (key_bin << 32) | (uint64_t)q;
+}
+
+// Cell-cell repulsion among a parent's cells (:10204-10383), one lane per
+// node of a live cell.  Window planes k >= KM read plane k - KM + 1 (plane
+// KM itself read as plane 1) and k < 1 read k + KM - 1 (checks 1, 2);
+// otherwise a node in the window's last or first quarter is check 3 or 4.
+// Across a wrapped plane a candidate's centroid moves by L (check 1: if
+// behind the middle, 2: if past it); its vertices move by the quarter
+// rules (checks 1, 3: first quarter +L; 2, 4: last quarter -L).
+__global__ __launch_bounds__(128) void k_wrap_cell_repulsion(
+    const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
+    const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
+    const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
+    const int32_t *__restrict__ domid, const uint64_t *__restrict__ keys,
+    const int32_t *__restrict__ range, Grid g, WrapCells w, RepParams r, double *__restrict__ vrep) {
+  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= ncell * nn) return;
+  const int c = gq / nn, node = gq % nn;
+  if (status[c] <= kStatusDead) return;
+  const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
+  const double e1x = nnormal[gq], e1y = nnormal[NT + gq], e1z = nnormal[2 * NT + gq];
+  const double q1 = w.z0 + 0.25 * w.L, q3 = w.z0 + 0.75 * w.L;
+  const int ic = (int)((xc - g.x0) / g.dx + 1.01), jc = (int)((yc - g.y0) / g.dx + 1.01),
+            kc = (int)((zc - g.z0) / g.dx + 1.01);
+  const int i1 = max(1, ic - 4), i2 = min(g.im, ic + 4), j1 = max(1, jc - 4), j2 = min(g.jm, jc + 4);
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int i = i1; i <= i2; i++)
+    for (int j = j1; j <= j2; j++)
+      for (int k = kc - 4; k <= kc + 4; k++) {
+        int check = 0, kk = k;
+        if (k >= g.km) { kk = k - g.km + 1; check = 1; }
+        else if (k < 1) { kk = k + g.km - 1; check = 2; }
+        else if (zc >= q3) check = 3;
+        else if (zc < q1) check = 4;
+        const int o = domid[bq(g, i, j, kk)];
+        if (o <= 0) continue;
+        const int beg = range[2 * (o - 1)], end = range[2 * (o - 1) + 1];
+        for (int e = beg; e < end; e++) {
+          const int q = (int)(keys[e] & 0xffffffffu);
+          const int m = q / nt, t = q % nt;
+          if (status[m] <= kStatusDead) continue;
+          const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
+          if (m == c && (n1 == node || n2 == node || n3 == node)) continue;
+          double zp = tg[6 * TT + q];
+          if (check == 1 && zp < w.mid) zp += w.L;
+          if (check == 2 && zp > w.mid) zp -= w.L;
+          if (e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0) continue;
+          const double dx = xc - tg[4 * TT + q], dy = yc - tg[5 * TT + q], dz = zc - zp;
+          if (sqrt(dx * dx + dy * dy + dz * dz) >= r.search) continue;
+          const size_t b0 = (size_t)m * nn;
+          double vz[3];
+          const int nv[3] = {n1, n2, n3};
+          for (int s = 0; s < 3; s++) {
+            double z = X[2 * NT + b0 + nv[s]];
+            if ((check == 1 || check == 3) && z < q1) z += w.L;
+            else if ((check == 2 || check == 4) && z >= q3) z -= w.L;
+            vz[s] = z;
+          }
+          const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = vz[0];
+          const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = vz[1];
+          const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = vz[2];
+          for (int sc = 0; sc < 16; sc++) {
+            const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
+            const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
+            const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
+            const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
+            const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
+            const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
+            const double d = sqrt(px * px + py * py + pz * pz);
+            if (d < r.cutoff) {
+              const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
+              v0 += f * px / d;
+              v1 += f * py / d;
+              v2 += f * pz / d;
+            }
+          }
+        }
+      }
+  vrep[gq] += v0;
+  vrep[NT + gq] += v1;
+  vrep[2 * NT + gq] += v2;
+}
+
+// Wall repulsion of a parent's cells (the interpolation loop of
+// :7541-8688): the window as the interpolation's (centred at zpmap for a
+// crossing cell, planes through kmap), the nearest wall triangle by vertex
+// or centroid distance from the node's own position, then the plane rule.
+__global__ void k_wrap_wall(const double *__restrict__ X, size_t NT, int nn, int ncell,
+                            const int32_t *__restrict__ status, const int32_t *__restrict__ windex,
+                            const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
+                            const double *__restrict__ wxyz, const double *__restrict__ wtgeo,
+                            const double *__restrict__ wnormal, Grid g, WrapCells w, RepParams r,
+                            double *__restrict__ vrep, int32_t *__restrict__ nearest) {
+  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= ncell * nn) return;
+  nearest[gq] = 0;
+  const int st = status[gq / nn];
+  if (st < 2) return;
+  const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
+  double ze = z;
+  if (st > 2) {
+    if (w.dir > 0 && z < w.mid) ze = z + w.L;
+    if (w.dir < 0 && z > w.mid) ze = z - w.L;
+  }
+  const int i = (int)((x - g.x0) / g.dx + 1.01), j = (int)((y - g.y0) / g.dx + 1.01),
+            k = (int)((ze - g.z0) / g.dx + 1.01);
+  const int i1 = max(1, i - 4), i2 = min(g.im, i + 4), j1 = max(1, j - 4), j2 = min(g.jm, j + 4);
+  double mind = r.wall_search;
+  int triwall = 0;
+  for (int ii = i1; ii <= i2; ii++)
+    for (int jj = j1; jj <= j2; jj++)
+      for (int kk = k - 4; kk <= k + 4; kk++) {
+        const int wn = windex[bq(g, ii, jj, kmap_ibm(kk, g.km))];
+        if (wn <= 0) continue;
+        const int32_t *trow = &wnode_tri[13 * (size_t)(wn - 1)];
+        const int cnt = min(trow[0], 12);
+        for (int l = 1; l <= cnt; l++) {
+          const int tw = trow[l];
+          const double *gv = &wtgeo[12 * (size_t)(tw - 1)];
+          for (int sv = 0; sv < 4; sv++) {
+            const double wx = gv[3 * sv], wy = gv[3 * sv + 1], wz = gv[3 * sv + 2];
+            const double d = sqrt((x - wx) * (x - wx) + (y - wy) * (y - wy) + (z - wz) * (z - wz));
+            if (d < mind) { mind = d; triwall = tw; }
+          }
+        }
+      }
   if (triwall == 0) return;
   nearest[gq] = triwall;
   const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
@@ -647,6 +839,62 @@ int cell_list_impl(zf_ctx *ctx) {
                                                   ctx->d_tkeys[1], n, 0, 32 + bits, ctx->stream));
   hipLaunchKernelGGL(k_bin_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_tkeys[1], n,
                      (uint64_t)ctx->dommax, ctx->d_bin_range);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+namespace {
+WrapCells wrap_cells(const zf_ctx *ctx) {
+  return WrapCells{ctx->wcfg.direction, ctx->wz0, ctx->wzm, ctx->wzm - ctx->wz0,
+                   0.5 * (ctx->wz0 + ctx->wzm)};
+}
+
+RepParams rep_params_of(const zf_ctx *ctx) {
+  return RepParams{ctx->rcfg.k_cell, ctx->rcfg.cutoff, ctx->rcfg.search, ctx->rcfg.k_wall,
+                   ctx->rcfg.wall_d0, ctx->rcfg.wall_dmax, ctx->rcfg.wall_search, 0.0, 0};
+}
+}  // namespace
+
+int wrap_cell_list_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, TT = ncap * ctx->nt;
+  const int n = ctx->ncell * ctx->nt;
+  hipLaunchKernelGGL(k_fill, dim3(nb(2 * (size_t)ctx->dommax, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_bin_range, 2 * (size_t)ctx->dommax, 0);
+  if (n == 0) return ZF_OK;
+  hipLaunchKernelGGL(k_wrap_tri_bin, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_tgeo, TT,
+                     ctx->nt, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_domid, grid_of(ctx),
+                     wrap_cells(ctx), ctx->dommax, ctx->d_tbin, ctx->d_tkeys[0]);
+  int bits = 1;
+  while ((1ull << bits) <= (uint64_t)ctx->dommax) bits++;
+  size_t tmp = ctx->tsort_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_tsort_tmp, tmp, ctx->d_tkeys[0],
+                                                  ctx->d_tkeys[1], n, 0, 32 + bits, ctx->stream));
+  hipLaunchKernelGGL(k_bin_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_tkeys[1], n,
+                     (uint64_t)ctx->dommax, ctx->d_bin_range);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int wrap_wall_impl(zf_ctx *ctx) {
+  const size_t NT = (size_t)ctx->mcfg.max_cells * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0 || !ctx->have_wall) return ZF_OK;
+  hipLaunchKernelGGL(k_wrap_wall, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
+                     ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wnode_tri, ctx->d_wtri,
+                     ctx->d_wxyz, ctx->d_wtgeo, ctx->d_wnormal, grid_of(ctx), wrap_cells(ctx),
+                     rep_params_of(ctx), ctx->d_vrep, ctx->d_near);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int wrap_repulsion_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  hipLaunchKernelGGL(k_wrap_cell_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
+                     ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
+                     ctx->d_tgeo, TT, ctx->d_domid, ctx->d_tkeys[1], ctx->d_bin_range, grid_of(ctx),
+                     wrap_cells(ctx), rep_params_of(ctx), ctx->d_vrep);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }

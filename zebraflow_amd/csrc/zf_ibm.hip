@@ -89,7 +89,7 @@ __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int nc
                          const double *__restrict__ uvoid, double *__restrict__ V) {
   const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
-  if (status[g / nn] < 0) return;
+  if (status[g / nn] <= kStatusDead) return;
   const double xp = X[g], yp = X[NT + g], zp = X[2 * NT + g];
   const int i = (int)((xp - p.x0) / p.dx + 1.01), j = (int)((yp - p.y0) / p.dx + 1.01),
             k = (int)((zp - p.z0) / p.dx + 1.01);
@@ -135,21 +135,31 @@ __device__ __forceinline__ int tile_of(double pos, double origin, double dx, int
   return t < 0 ? 0 : t >= nt ? nt - 1 : t;
 }
 
+// A clone node within its band (band[2c] the plane, band[2c+1] the
+// direction of travel) moves with its parent cell and does not spread
+// (ZebraflowWT.c:11002-11015).
+__device__ __forceinline__ bool clone_band_node(const double *band, int c, int st, double z) {
+  if (!band || !(st < 0 && st > kStatusDead)) return false;
+  const double plane = band[2 * c], dir = band[2 * c + 1];
+  return dir > 0 ? !(z > plane) : dir < 0 ? !(z < plane) : false;
+}
+
 // v += v_rep; F_fluid = gamma_p v_rep + F_Total (ZebraflowWT.c:10749-10754);
-// sort key = tile << 32 | node (dead cells sort past the last tile).
-// id_base: added to the node index in the key (the nucleus nodes follow
-// the membrane's, from id_base = cap * nn).
+// sort key = tile << 32 | node (dead cells and clone band nodes sort past
+// the last tile).  id_base: added to the node index in the key (the
+// nucleus nodes follow the membrane's, from id_base = cap * nn).
 template <class U>
 __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V,
                             const double *__restrict__ vrep, const double *__restrict__ F,
                             size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
                             IbmParams p, U *__restrict__ ffl, uint64_t *__restrict__ keys,
-                            uint32_t id_base) {
+                            uint32_t id_base, const double *__restrict__ band) {
   const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
   const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
   const uint64_t id = (uint64_t)id_base + (uint64_t)g;
-  if (status[g / nn] < 0) {
+  const int st = status[g / nn];
+  if (st <= kStatusDead || clone_band_node(band, g / nn, st, X[2 * NT + g])) {
     keys[g] = (ntiles << 32) | id;
     return;
   }
@@ -194,14 +204,25 @@ __device__ __forceinline__ double kcos_pi(double x) { return cospi(x); }
 // tile), so the lane staging a node evaluates its 12 factors once and the
 // other lanes read theirs from LDS (same expressions, same values).
 // Node ids from NT on are nucleus nodes (Xn, ffln, stride NTn).
-template <class F>
+// WRAP (a parent lattice's periodic window, zf_wrap.hip): bits 30-31 of a
+// key's node id mark a periodic image of the node, seen L lower (1) or
+// higher (2); the reference's kmap sends an image's share only to planes
+// k >= 2 (1) or k <= KM - 1 (2), plane KM and plane 1 staying distinct
+// (ZebraflowWT.c:11220-11222).
+struct WrapSpread {
+  double L;
+  int km;
+};
+
+template <class F, bool WRAP = false>
 __global__ __launch_bounds__(64) void k_spread_gather(
     const uint64_t *__restrict__ keys, const int32_t *__restrict__ range,
     const double *__restrict__ X, const F *__restrict__ ffl, size_t NT,
     const double *__restrict__ Xn, const F *__restrict__ ffln, size_t NTn,
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, IbmParams p,
-    F *__restrict__ force, size_t stride) {
+    F *__restrict__ force, size_t stride, WrapSpread ws = WrapSpread{0.0, 0}) {
   __shared__ F sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
+  __shared__ int scode[WRAP ? 64 : 1];
   __shared__ F kx[4][64], ky[4][64], kz[4][64];  // (1 + cos(pi |r - s| / 2)) for tile row r
   const int tile = blockIdx.x;
   const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
@@ -232,17 +253,24 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           const double *Xs = X;
           const F *fs = ffl;
           size_t st = NT;
+          int code = 0;
           if (lane < end - c0) {
             g = (size_t)(keys[c0 + lane] & 0xffffffffu);
+            if (WRAP) {
+              code = (int)(g >> 30);
+              g &= 0x3fffffffu;
+            }
             if (g >= NT) {
               g -= NT;
               Xs = Xn;
               fs = ffln;
               st = NTn;
             }
+            double zz = Xs[2 * st + g];
+            if (WRAP) zz += code == 1 ? -ws.L : code == 2 ? ws.L : 0.0;
             px = (F)((Xs[g] - ox) / p.dx);
             py = (F)((Xs[st + g] - oy) / p.dx);
-            pz = (F)((Xs[2 * st + g] - oz) / p.dx);
+            pz = (F)((zz - oz) / p.dx);
             rel = px >= (F)-2 && px <= (F)5 && py >= (F)-2 && py <= (F)5 && pz >= (F)-2 && pz <= (F)5;
           }
           const unsigned long long relm = __ballot(rel);
@@ -255,6 +283,7 @@ __global__ __launch_bounds__(64) void k_spread_gather(
             sf0[sl] = fs[g];
             sf1[sl] = fs[st + g];
             sf2[sl] = fs[2 * st + g];
+            if (WRAP) scode[sl] = code;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
               // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
@@ -268,6 +297,7 @@ __global__ __launch_bounds__(64) void k_spread_gather(
           for (int m = 0; m < cnt; m++) {
             const F dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
             if (off_support(dx, dy, dz, p.cube)) continue;
+            if (WRAP && ((scode[m] == 1 && lk == 0) || (scode[m] == 2 && lk == ws.km - 1))) continue;
             const F w = kx[ix][m] * ky[iy][m] * kz[iz][m] * ((F)1 / (F)64);
             a0 += w * sf0[m];
             a1 += w * sf1[m];
@@ -284,6 +314,117 @@ __global__ __launch_bounds__(64) void k_spread_gather(
   force[s] = a0 * fs;
   force[stride + s] = a1 * fs;
   force[2 * stride + s] = a2 * fs;
+}
+
+// ---- a parent lattice's cells: the periodic window along k (zf_wrap.hip)
+struct WrapIbm {
+  int dir, km;
+  double z0, zm, L, mid;
+};
+
+// kmap of the interpolation and spreading windows (ZebraflowWT.c:7574-7576).
+__device__ __forceinline__ int kmap_ibm(int k, int km) {
+  return k < 1 ? k + (km - 1) : k > km ? k - (km - 1) : k;
+}
+
+// Interpolation of a parent's cells (:7541-7732 for status 2, :8101-8297
+// for status > 2), one lane per node: a crossing cell's window is centred
+// at zpmap (its nodes behind the window's middle seen past the exit);
+// window planes wrap through kmap at their unwrapped place.  The cut-off
+// of a fluid node is taken at zpmap and its z factor at the node's own z
+// (:8145-8147), those of the wall layer both at its own z (:8153-8155),
+// so a node seen through zpmap takes no Uvoid term.  Fluid is B > 0.
+template <class U>
+__global__ void k_wrap_interp(const double *__restrict__ X, size_t NT, int nn, int ncell,
+                              const int32_t *__restrict__ status, const int32_t *__restrict__ b,
+                              const int32_t *__restrict__ sbox, const U *__restrict__ umac,
+                              size_t stride, IbmParams p, WrapIbm w, const int32_t *__restrict__ vslot,
+                              const double *__restrict__ uvoid, double *__restrict__ V) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncell * nn) return;
+  const int st = status[g / nn];
+  if (st < 2) return;
+  const double xp = X[g], yp = X[NT + g], zp = X[2 * NT + g];
+  double ze = zp;
+  if (st > 2) {
+    if (w.dir > 0 && zp < w.mid) ze = zp + w.L;
+    if (w.dir < 0 && zp > w.mid) ze = zp - w.L;
+  }
+  const int i = (int)((xp - p.x0) / p.dx + 1.01), j = (int)((yp - p.y0) / p.dx + 1.01),
+            k = (int)((ze - p.z0) / p.dx + 1.01);
+  const int i1 = max(1, i - 3), i2 = min(p.im, i + 3), j1 = max(1, j - 3), j2 = min(p.jm, j + 3);
+  const double width = 2.0 * p.dx, w2 = width * width;
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int ii = i1; ii <= i2; ii++) {
+    const double dxv = p.x0 + (ii - 1) * p.dx - xp;
+    for (int jj = j1; jj <= j2; jj++) {
+      const double dyv = p.y0 + (jj - 1) * p.dx - yp;
+      for (int kk = k - 3; kk <= k + 3; kk++) {
+        const double zc = p.z0 + (kk - 1) * p.dx;
+        const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kmap_ibm(kk, p.km) - 1);
+        const double dzv = zc - zp;
+        if (b[q] > 0) {
+          const double dze = zc - ze;
+          if (dxv * dxv + dyv * dyv + dze * dze > w2) continue;
+          const double wt = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
+          const int s = sbox[q];
+          v0 += wt * p.vel_scale * (double)umac[s];
+          v1 += wt * p.vel_scale * (double)umac[stride + s];
+          v2 += wt * p.vel_scale * (double)umac[2 * stride + s];
+        } else if (vslot) {
+          const int sl = vslot[q];
+          if (sl < 0 || dxv * dxv + dyv * dyv + dzv * dzv > w2) continue;
+          const double wt = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
+          v0 += wt * p.vel_scale * uvoid[3 * (size_t)sl];
+          v1 += wt * p.vel_scale * uvoid[3 * (size_t)sl + 1];
+          v2 += wt * p.vel_scale * uvoid[3 * (size_t)sl + 2];
+        }
+      }
+    }
+  }
+  V[g] = v0;
+  V[NT + g] = v1;
+  V[2 * NT + g] = v2;
+}
+
+// Spreading keys of a parent's cells: each live node at its own (wrapped)
+// z, plus one periodic image when it lies within 3 DX of an end of the
+// window (keys_img; otherwise a key past the last tile).
+template <class U>
+__global__ void k_wrap_node_prep(const double *__restrict__ X, double *__restrict__ V,
+                                 const double *__restrict__ vrep, const double *__restrict__ F,
+                                 size_t NT, int nn, int ncell, const int32_t *__restrict__ status,
+                                 IbmParams p, WrapIbm w, U *__restrict__ ffl,
+                                 uint64_t *__restrict__ keys, uint64_t *__restrict__ keys_img,
+                                 uint32_t id_base) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncell * nn) return;
+  const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
+  const uint64_t id = (uint64_t)id_base + (uint64_t)g;
+  keys_img[g] = (ntiles << 32) | id;
+  if (status[g / nn] < 2) {
+    keys[g] = (ntiles << 32) | id;
+    return;
+  }
+  double fl[3];
+  for (int a = 0; a < 3; a++) {
+    const double vr = vrep[a * NT + g];
+    V[a * NT + g] += vr;
+    fl[a] = p.gamma_p * vr + F[a * NT + g];
+  }
+  ffl[g] = (U)fl[0];
+  ffl[NT + g] = (U)fl[1];
+  ffl[2 * NT + g] = (U)fl[2];
+  const double x = X[g], y = X[NT + g], z = X[2 * NT + g];
+  const int tx = tile_of(x, p.x0, p.dx, p.ntx), ty = tile_of(y, p.y0, p.dx, p.nty);
+  keys[g] = ((((uint64_t)tx * p.nty + ty) * p.ntz + tile_of(z, p.z0, p.dx, p.ntz)) << 32) | id;
+  int code = 0;
+  double zi = z;
+  if (z > w.zm - 3.0 * p.dx) { code = 1; zi = z - w.L; }
+  else if (z < w.z0 + 3.0 * p.dx) { code = 2; zi = z + w.L; }
+  if (code)
+    keys_img[g] = ((((uint64_t)tx * p.nty + ty) * p.ntz + tile_of(zi, p.z0, p.dx, p.ntz)) << 32) |
+                  ((uint64_t)code << 30) | id;
 }
 
 __global__ void k_speed(const double *__restrict__ V, size_t NT, int nn, int ncell,
@@ -360,6 +501,7 @@ int ensure_force_field(zf_ctx *ctx) {
 }
 
 void free_ibm(zf_ctx *ctx) {
+  free_wrap(ctx);
   void **bufs[] = {(void **)&ctx->d_umac, (void **)&ctx->d_vrep,
                    (void **)&ctx->d_ffl,   (void **)&ctx->d_keys[0], (void **)&ctx->d_keys[1],
                    (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag};
@@ -450,11 +592,11 @@ int ibm_spread_impl(zf_ctx *ctx) {
   if (f64)
     hipLaunchKernelGGL(k_node_prep<double>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
                        ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
-                       static_cast<double *>(ctx->d_ffl), ctx->d_keys[0], 0u);
+                       static_cast<double *>(ctx->d_ffl), ctx->d_keys[0], 0u, ctx->d_band);
   else
     hipLaunchKernelGGL(k_node_prep<float>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
                        ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p,
-                       static_cast<float *>(ctx->d_ffl), ctx->d_keys[0], 0u);
+                       static_cast<float *>(ctx->d_ffl), ctx->d_keys[0], 0u, ctx->d_band);
   // nucleus nodes (F_fluid = gamma_p NODE_WALLREPUL_N + F_Total_N,
   // ZebraflowWT.c:10879-10884), keyed after the membrane's
   const size_t NTn = ctx->have_nuc ? (size_t)ctx->mcfg.max_cells * ctx->nnuc : 0;
@@ -464,12 +606,12 @@ int ibm_spread_impl(zf_ctx *ctx) {
       hipLaunchKernelGGL(k_node_prep<double>, dim3(nb(nnuc_pts, 256)), dim3(256), 0, ctx->stream,
                          ctx->d_xn, ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell,
                          ctx->d_status, p, static_cast<double *>(ctx->d_ffln), ctx->d_keys[0] + n,
-                         (uint32_t)NT);
+                         (uint32_t)NT, ctx->d_band);
     else
       hipLaunchKernelGGL(k_node_prep<float>, dim3(nb(nnuc_pts, 256)), dim3(256), 0, ctx->stream,
                          ctx->d_xn, ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell,
                          ctx->d_status, p, static_cast<float *>(ctx->d_ffln), ctx->d_keys[0] + n,
-                         (uint32_t)NT);
+                         (uint32_t)NT, ctx->d_band);
   }
   const int nkeys = n + nnuc_pts;
   // end bit: tile ids need ceil(log2(ntiles+1)) bits above the 32 node bits
@@ -496,6 +638,100 @@ int ibm_spread_impl(zf_ctx *ctx) {
                        static_cast<float *>(ctx->d_force), ctx->stride);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
+}
+
+WrapIbm wrap_ibm(const zf_ctx *ctx) {
+  return WrapIbm{ctx->wcfg.direction, ctx->cfg.km, ctx->wz0, ctx->wzm, ctx->wzm - ctx->wz0,
+                 0.5 * (ctx->wz0 + ctx->wzm)};
+}
+
+int wrap_interpolate_impl(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  if (n == 0) return ZF_OK;
+  const void *f = ctx->d_f[ctx->cur];
+  const unsigned g = nb(ctx->dommax, 256);
+  const WrapIbm w = wrap_ibm(ctx);
+  const int32_t *vs = ctx->have_void ? ctx->d_vslot : nullptr;
+  const int nnp = ctx->have_nuc ? ctx->ncell * ctx->nnuc : 0;
+  if (ctx->cfg.storage == ZF_STORE_F64) {
+    double *um = static_cast<double *>(ctx->d_umac);
+    hipLaunchKernelGGL((k_macro_vel<double, false, true, double>), dim3(g), dim3(256), 0, ctx->stream,
+                       (const double *)f, ctx->stride, ctx->dommax, um);
+    hipLaunchKernelGGL(k_wrap_interp<double>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x,
+                       NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, um, ctx->stride,
+                       params(ctx), w, vs, ctx->d_uvoid, ctx->d_v);
+    if (nnp > 0)
+      hipLaunchKernelGGL(k_wrap_interp<double>, dim3(nb(nnp, 128)), dim3(128), 0, ctx->stream,
+                         ctx->d_xn, ncap * ctx->nnuc, ctx->nnuc, ctx->ncell, ctx->d_status, ctx->d_b,
+                         ctx->d_sbox, um, ctx->stride, params(ctx), w, vs, ctx->d_uvoid, ctx->d_vn);
+  } else {
+    float *um = static_cast<float *>(ctx->d_umac);
+    if (ctx->cfg.storage == ZF_STORE_F32DEV)
+      hipLaunchKernelGGL((k_macro_vel<float, false, false, float>), dim3(g), dim3(256), 0, ctx->stream,
+                         (const float *)f, ctx->stride, ctx->dommax, um);
+    else
+      hipLaunchKernelGGL((k_macro_vel<_Float16, true, false, float>), dim3(g), dim3(256), 0,
+                         ctx->stream, (const _Float16 *)f, ctx->stride, ctx->dommax, um);
+    hipLaunchKernelGGL(k_wrap_interp<float>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x,
+                       NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_b, ctx->d_sbox, um, ctx->stride,
+                       params(ctx), w, vs, ctx->d_uvoid, ctx->d_v);
+    if (nnp > 0)
+      hipLaunchKernelGGL(k_wrap_interp<float>, dim3(nb(nnp, 128)), dim3(128), 0, ctx->stream,
+                         ctx->d_xn, ncap * ctx->nnuc, ctx->nnuc, ctx->ncell, ctx->d_status, ctx->d_b,
+                         ctx->d_sbox, um, ctx->stride, params(ctx), w, vs, ctx->d_uvoid, ctx->d_vn);
+  }
+  // the repulsion velocities of this iteration start from zero: the wall
+  // term of the same loop (zf_cells.hip), then contact and cell-cell
+  hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+template <class F>
+static int wrap_spread_typed(zf_ctx *ctx) {
+  const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
+  const int n = ctx->ncell * ctx->nn;
+  const IbmParams p = params(ctx);
+  const WrapIbm w = wrap_ibm(ctx);
+  const uint64_t ntiles = (uint64_t)ctx->ntx * ctx->nty * ctx->ntz;
+  const size_t NTn = ctx->have_nuc ? ncap * ctx->nnuc : 0;
+  const int nnp = ctx->have_nuc ? ctx->ncell * ctx->nnuc : 0;
+  const int half = n + nnp;  // originals, then images
+  uint64_t *k0 = ctx->d_keys[0];
+  hipLaunchKernelGGL(k_wrap_node_prep<F>, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x,
+                     ctx->d_v, ctx->d_vrep, ctx->d_mforce, NT, ctx->nn, ctx->ncell, ctx->d_status, p, w,
+                     static_cast<F *>(ctx->d_ffl), k0, k0 + half, 0u);
+  if (nnp > 0)
+    hipLaunchKernelGGL(k_wrap_node_prep<F>, dim3(nb(nnp, 256)), dim3(256), 0, ctx->stream, ctx->d_xn,
+                       ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell, ctx->d_status, p,
+                       w, static_cast<F *>(ctx->d_ffln), k0 + n, k0 + half + n, (uint32_t)NT);
+  const int nkeys = 2 * half;
+  int tbits = 1;
+  while ((1ull << tbits) <= ntiles) tbits++;
+  size_t tmp = ctx->sort_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
+                                                  nkeys, 0, 32 + tbits, ctx->stream));
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * ntiles, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_tile_range, (size_t)(2 * ntiles), 0);
+  hipLaunchKernelGGL(k_tile_ranges, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1],
+                     nkeys, ntiles, ctx->d_tile_range);
+  hipLaunchKernelGGL((k_spread_gather<F, true>), dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
+                     ctx->d_keys[1], ctx->d_tile_range, ctx->d_x, static_cast<const F *>(ctx->d_ffl),
+                     NT, ctx->d_xn, static_cast<const F *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox,
+                     p, static_cast<F *>(ctx->d_force), ctx->stride, WrapSpread{w.L, ctx->cfg.km});
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
+int wrap_spread_impl(zf_ctx *ctx) {
+  int rc = ensure_force_field(ctx);
+  if (rc) return rc;
+  const size_t nw = 3 * ctx->stride * field_elem(ctx) / 4;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(nw, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<int32_t *>(ctx->d_force), nw, 0);
+  if (ctx->ncell * ctx->nn == 0) return ZF_OK;
+  return field_elem(ctx) == 8 ? wrap_spread_typed<double>(ctx) : wrap_spread_typed<float>(ctx);
 }
 
 int advance_impl(zf_ctx *ctx, int32_t *n_removed) {

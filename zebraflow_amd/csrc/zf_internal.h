@@ -156,6 +156,7 @@ struct zf_ctx {
   void *d_ffln = nullptr;           // 3 x (ncap*nnuc): F_fluid of the nucleus nodes, field_elem
   int32_t *d_nlist = nullptr;       // (ncap*nnuc) x list_cap: listed triangles (0-based)
   int32_t *d_nlist_cnt = nullptr;   // ncap*nnuc, then one overflow counter
+  int32_t *d_nuc_stale = nullptr;   // ncap: the cell's lists are rebuilt at the next contact
   bool nuc_lists_valid = false;     // cleared when cells or nuclei are uploaded
   size_t keys_cap = 0;              // entries of the spreading sort keys
 
@@ -258,6 +259,24 @@ struct zf_ctx {
   std::vector<int32_t> h_export;   // per slot: storage index << 1 | interior
   int32_t *d_export_s = nullptr;
   double *d_export = nullptr;      // 4 x n_export: rho, u, v, w
+
+  // cells on a parent lattice: the periodic cell window along k
+  // [wz0, wzm] (wzm = wz0 + (km-1) dx), node coordinates wrapped into it
+  bool have_wrap = false;
+  zf_cell_wrap_config wcfg{};
+  double wz0 = 0, wzm = 0;
+  zf_ctx *wrap_main = nullptr;     // main lattice its clones go to
+  int32_t *d_regress = nullptr;    // ncap: REGRESSFLAG
+  int32_t *d_cloneflag = nullptr;  // ncap: CLONEFLAG of the last update
+  double *d_span = nullptr;        // 2 x ncap: smallest and largest z (membrane and nucleus)
+  int32_t *d_split = nullptr;      // ncap: the cell lies across the wrap (status > 3, span > split)
+  double *d_xu = nullptr;          // 3 x (ncap*nn): the positions the membrane stages use
+  double *d_xnu = nullptr;         // 3 x (ncap*nnuc): same for the nucleus
+  int32_t *d_clone_pairs = nullptr;  // 1 + 2 x ncap: count, (parent slot, main slot) pairs
+  // main lattice: parents whose cells clone into it, and per cell the
+  // clone band of a clone (plane, direction; direction 0 otherwise)
+  std::vector<zf_ctx *> cell_parents;
+  double *d_band = nullptr;        // 2 x ncap
 };
 
 namespace zf {
@@ -320,6 +339,31 @@ int parent_export(zf_ctx *ctx, bool in_step);
 int parent_couple(zf_ctx *ctx);
 void free_parent(zf_ctx *ctx);
 int advance_impl(zf_ctx *ctx, int32_t *n_removed);
+// Cells on parent lattices (zf_wrap.hip).  A cell is alive while its
+// status is above -4998 (-5000: free slot or removed; -4998 < s < 0: a
+// clone of parent cell -s still following it).
+constexpr int32_t kStatusDead = -4998;
+// The positions the membrane and nucleus stages read: a parent's cells
+// across the wrap unwrapped (wrap_prepare), the stored ones otherwise.
+inline const double *cell_pos(const zf_ctx *ctx) { return ctx->have_wrap ? ctx->d_xu : ctx->d_x; }
+inline const double *nuc_pos(const zf_ctx *ctx) { return ctx->have_wrap ? ctx->d_xnu : ctx->d_xn; }
+// Parent: span, split flags and unwrapped positions (start of an RBC
+// iteration); REGRESSFLAG reset from the centroids (after the membrane
+// stage); the wrapped update with clone flags; the cloning into the main
+// lattice.  Main: the update of its clones (band nodes with the parent
+// cell's velocity, release into status 1).
+int wrap_prepare(zf_ctx *ctx);
+int wrap_regress(zf_ctx *ctx);
+int wrap_advance(zf_ctx *ctx);
+int wrap_clone(zf_ctx *ctx);
+int clone_advance(zf_ctx *main);
+void free_wrap(zf_ctx *ctx);
+// The parent's stages with the periodic window (zf_ibm.hip, zf_cells.hip).
+int wrap_interpolate_impl(zf_ctx *ctx);
+int wrap_spread_impl(zf_ctx *ctx);
+int wrap_cell_list_impl(zf_ctx *ctx);
+int wrap_wall_impl(zf_ctx *ctx);
+int wrap_repulsion_impl(zf_ctx *ctx);
 int ensure_force_field(zf_ctx *ctx);
 // Transposes of the IBM coupling at positions X and populations f (device
 // arrays, SoA over the cell capacity; lamF and lam storage-indexed; lam of

@@ -36,7 +36,7 @@ __global__ void k_tri_geom(const double *__restrict__ X, size_t NT, int nn, int 
   const int idx = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (idx >= ncell * nt) return;
   const int c = idx / nt, t = idx % nt;
-  if (status[c] < 0) return;
+  if (status[c] <= kStatusDead) return;
   const size_t base = (size_t)c * nn;
   const TriG<double> g = tri_geom(ld3(X, NT, base + tri[3 * t]), ld3(X, NT, base + tri[3 * t + 1]),
                                   ld3(X, NT, base + tri[3 * t + 2]), L);
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_cell_reduce(const double *__restrict__ 
                                                      MembParams mp, double *__restrict__ cell) {
   __shared__ double sh[5][256];
   const int c = blockIdx.x;
-  if (status[c] < 0) return;
+  if (status[c] <= kStatusDead) return;
   double s[5] = {0, 0, 0, 0, 0};
   const size_t nb = (size_t)c * nn, tb = (size_t)c * nt;
   for (int n = threadIdx.x; n < nn; n += 256) {
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(128) void k_node_force(const double *__restrict__ X
   if (idx >= ncell * nn) return;
   const int c = idx / nn, n = idx % nn;
   const size_t q = (size_t)c * nn + n;
-  if (status[c] < 0) {
+  if (status[c] <= kStatusDead) {
     F[q] = F[NT + q] = F[2 * NT + q] = 0;
     return;
   }
@@ -258,7 +258,10 @@ int upload(zf_ctx *ctx, T **d, const std::vector<T> &h) {
 
 }  // namespace
 
-void free_membrane(zf_ctx *ctx) { free_mesh(ctx); }
+void free_membrane(zf_ctx *ctx) {
+  free_wrap(ctx);
+  free_mesh(ctx);
+}
 
 }  // namespace zf
 
@@ -414,8 +417,12 @@ int zf_upload_cells(zf_ctx *ctx, int32_t ncell, const double *xyz) {
   ctx->ncell = ncell;
   ctx->nuc_lists_valid = false;
   std::vector<int32_t> st(ctx->mcfg.max_cells, -5000);
-  for (int c = 0; c < ncell; c++) st[c] = 1;
+  // a parent lattice's cells start inside its window (status 2)
+  for (int c = 0; c < ncell; c++) st[c] = ctx->have_wrap ? 2 : 1;
   ZF_HIP(ctx, hipMemcpy(ctx->d_status, st.data(), 4 * st.size(), hipMemcpyHostToDevice));
+  if (ctx->have_wrap) ZF_HIP(ctx, hipMemset(ctx->d_regress, 0, 4 * (size_t)ctx->mcfg.max_cells));
+  if (ctx->d_band) ZF_HIP(ctx, hipMemset(ctx->d_band, 0, 16 * (size_t)ctx->mcfg.max_cells));
+  if (ctx->have_nuc) ZF_HIP(ctx, hipMemset(ctx->d_nuc_stale, 0, 4 * (size_t)ctx->mcfg.max_cells));
   if (n == 0) return ZF_OK;
   double *tmp = nullptr;
   int rc = dev_alloc(ctx, (void **)&tmp, 24 * n);
@@ -493,10 +500,10 @@ int zf::membrane_forces_impl(zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn, TT = ncap * ctx->nt;
   const int nc = ctx->ncell;
   hipLaunchKernelGGL(k_zero_i32, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, ctx->d_fail, nc);
-  int rc = memb::membrane_geometry(ctx, ctx->d_x, ctx->d_tgeo, ctx->d_cell);
+  int rc = memb::membrane_geometry(ctx, cell_pos(ctx), ctx->d_tgeo, ctx->d_cell);
   if (rc) return rc;
   hipLaunchKernelGGL(k_node_force, dim3(nb((size_t)nc * ctx->nn, 128)), dim3(128), 0, ctx->stream,
-                     ctx->d_x, NT, nc, memb::mesh_tables(ctx), ctx->d_tgeo, TT, ctx->d_cell,
+                     cell_pos(ctx), NT, nc, memb::mesh_tables(ctx), ctx->d_tgeo, TT, ctx->d_cell,
                      ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal, ctx->d_fail);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;

@@ -58,7 +58,7 @@ __global__ void k_nuc_force(const double *__restrict__ X, size_t NT, int nn, int
   if (g >= ncell * nn) return;
   const int c = g / nn, node = g % nn;
   double acc[3] = {0, 0, 0};
-  if (status[c] >= 0) {
+  if (status[c] > kStatusDead) {
     const size_t b0 = (size_t)c * nn;
     for (int t = inc_start[node]; t < inc_start[node + 1]; t++) {
       const int e = inc[t] >> 1, role = inc[t] & 1;
@@ -79,12 +79,15 @@ __global__ void k_nuc_force(const double *__restrict__ X, size_t NT, int nn, int
 __global__ void k_nuc_list(const double *__restrict__ Xn, size_t NTn, int nnuc, int ncell,
                            const int32_t *__restrict__ status, const double *__restrict__ tg,
                            size_t TT, int nt, double r, int cap, int32_t *__restrict__ list,
-                           int32_t *__restrict__ cnt, int32_t *__restrict__ over) {
+                           int32_t *__restrict__ cnt, int32_t *__restrict__ over,
+                           const int32_t *__restrict__ stale, int all) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncell * nnuc) return;
   const int c = g / nnuc;
+  // all cells on the rebuild rule, else the cells marked since (new clones)
+  if (!all && !stale[c]) return;
   int n = 0, lost = 0;
-  if (status[c] >= 0) {
+  if (status[c] > kStatusDead) {
     const double x = Xn[g], y = Xn[NTn + g], z = Xn[2 * NTn + g];
     const size_t t0 = (size_t)c * nt;
     for (int t = 0; t < nt; t++) {
@@ -115,7 +118,7 @@ __global__ void k_nuc_contact(const double *__restrict__ Xn, size_t NTn, int nnu
                               const int32_t *__restrict__ list, const int32_t *__restrict__ cnt,
                               int cap, ContactParams cp, double *__restrict__ vrep) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell || status[c] < 0) return;
+  if (c >= ncell || status[c] <= kStatusDead) return;
   const size_t b0 = (size_t)c * nn, t0 = (size_t)c * nt;
   for (int node = 0; node < nnuc; node++) {
     const size_t g = (size_t)c * nnuc + node;
@@ -235,7 +238,7 @@ void free_nucleus(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_nuc_edge, (void **)&ctx->d_nuc_eq, (void **)&ctx->d_nuc_inc_start,
                    (void **)&ctx->d_nuc_inc, (void **)&ctx->d_xn, (void **)&ctx->d_vn,
                    (void **)&ctx->d_fn, (void **)&ctx->d_vrepn, (void **)&ctx->d_ffln,
-                   (void **)&ctx->d_nlist, (void **)&ctx->d_nlist_cnt};
+                   (void **)&ctx->d_nlist, (void **)&ctx->d_nlist_cnt, (void **)&ctx->d_nuc_stale};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->have_nuc = false;
   ctx->nnuc = ctx->ne_n = 0;
@@ -248,7 +251,7 @@ int nucleus_forces_impl(zf_ctx *ctx) {
   const size_t NTn = (size_t)ctx->mcfg.max_cells * ctx->nnuc;
   const zf_membrane_config &m = ctx->mcfg;
   const SpringParams sp{m.len_scale, m.kT, m.x_max, m.x_floor, m.relax2};
-  hipLaunchKernelGGL(k_nuc_force, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_xn, NTn,
+  hipLaunchKernelGGL(k_nuc_force, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, nuc_pos(ctx), NTn,
                      ctx->nnuc, ctx->ncell, ctx->d_status, ctx->d_nuc_inc_start, ctx->d_nuc_inc,
                      ctx->d_nuc_edge, ctx->d_nuc_eq, ctx->ne_n, sp, ctx->d_fn);
   ZF_HIP(ctx, hipGetLastError());
@@ -263,16 +266,21 @@ int nucleus_contact_impl(zf_ctx *ctx, int iter) {
   const size_t TT = (size_t)ctx->mcfg.max_cells * ctx->nt;
   const zf_nucleus_config &nc = ctx->ncfg;
   const int cap = nc.list_cap;
-  if (!ctx->nuc_lists_valid || (nc.list_interval > 0 && iter % nc.list_interval == 0)) {
-    hipLaunchKernelGGL(k_nuc_list, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_xn, NTn,
+  const bool all = !ctx->nuc_lists_valid || (nc.list_interval > 0 && iter % nc.list_interval == 0);
+  // a main lattice with parents' cells also rebuilds the lists of the
+  // clones made since (the reference's NucMembUpdate of a new cell)
+  if (all || !ctx->cell_parents.empty()) {
+    hipLaunchKernelGGL(k_nuc_list, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, nuc_pos(ctx), NTn,
                        ctx->nnuc, ctx->ncell, ctx->d_status, ctx->d_tgeo, TT, ctx->nt,
                        nc.list_radius, cap, ctx->d_nlist, ctx->d_nlist_cnt,
-                       ctx->d_nlist_cnt + NTn);
+                       ctx->d_nlist_cnt + NTn, ctx->d_nuc_stale, all ? 1 : 0);
+    hipLaunchKernelGGL(k_zero_i32, dim3(nb(ctx->ncell, 256)), dim3(256), 0, ctx->stream,
+                       ctx->d_nuc_stale, ctx->ncell);
     ctx->nuc_lists_valid = true;
   }
   const ContactParams cp{nc.near_radius, nc.contact, nc.k_contact};
-  hipLaunchKernelGGL(k_nuc_contact, dim3(nb(ctx->ncell, 64)), dim3(64), 0, ctx->stream, ctx->d_xn,
-                     NTn, ctx->nnuc, ctx->d_x, NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_tri,
+  hipLaunchKernelGGL(k_nuc_contact, dim3(nb(ctx->ncell, 64)), dim3(64), 0, ctx->stream, nuc_pos(ctx),
+                     NTn, ctx->nnuc, cell_pos(ctx), NT, ctx->nn, ctx->ncell, ctx->d_status, ctx->d_tri,
                      ctx->d_tgeo, TT, ctx->nt, ctx->d_nlist, ctx->d_nlist_cnt, cap, cp, ctx->d_vrep);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
@@ -328,7 +336,8 @@ int zf_nucleus_setup(zf_ctx *ctx, const zf_nucleus_config *nc, int32_t nnuc, int
       (rc = dev_alloc(ctx, (void **)&ctx->d_vrepn, 24 * NTn)) ||
       (rc = dev_alloc(ctx, &ctx->d_ffln, 3 * 8 * NTn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_nlist, 4 * NTn * (size_t)nc->list_cap)) ||
-      (rc = dev_alloc(ctx, (void **)&ctx->d_nlist_cnt, 4 * (NTn + 1)))) {
+      (rc = dev_alloc(ctx, (void **)&ctx->d_nlist_cnt, 4 * (NTn + 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_nuc_stale, 4 * (size_t)ctx->mcfg.max_cells))) {
     free_nucleus(ctx);
     return rc;
   }
@@ -337,6 +346,8 @@ int zf_nucleus_setup(zf_ctx *ctx, const zf_nucleus_config *nc, int32_t nnuc, int
     hipLaunchKernelGGL(k_zero, dim3(nb(3 * NTn, 256)), dim3(256), 0, ctx->stream, z, 3 * NTn);
   hipLaunchKernelGGL(k_zero_i32, dim3(nb(NTn + 1, 256)), dim3(256), 0, ctx->stream,
                      ctx->d_nlist_cnt, NTn + 1);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nb(ctx->mcfg.max_cells, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_nuc_stale, (size_t)ctx->mcfg.max_cells);
   ZF_HIP(ctx, hipGetLastError());
   ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->have_nuc = true;

@@ -195,6 +195,7 @@ int parent_couple(zf_ctx *ctx) {
 static int rebuild_export(zf_ctx *mc);
 
 void free_parent(zf_ctx *ctx) {
+  free_wrap(ctx);
   if (ctx->link_main) {
     zf_ctx *mc = ctx->link_main;
     (void)hipStreamSynchronize(mc->stream);

@@ -6,7 +6,8 @@
 // part; these checkpoints keep the populations as stored, the force field
 // spread at the last RBC iteration, and the cells (positions, velocities,
 // membrane force, repulsion velocity, status; with nuclei their positions,
-// velocities, forces and contact lists), so that a restored context
+// velocities, forces and contact lists; on a parent lattice the cells'
+// flags, on a main lattice its clones' bands), so that a restored context
 // continues bit-identically.
 #include "zf_internal.h"
 
@@ -44,7 +45,13 @@ std::vector<std::pair<void *, size_t>> regions(zf_ctx *ctx, bool cells) {
       r.push_back({ctx->d_fn, 24 * NTn});
       r.push_back({ctx->d_nlist, 4 * NTn * (size_t)ctx->ncfg.list_cap});
       r.push_back({ctx->d_nlist_cnt, 4 * (NTn + 1)});
+      r.push_back({ctx->d_nuc_stale, 4 * (size_t)ctx->mcfg.max_cells});
     }
+    if (ctx->have_wrap) {  // a parent's cells: REGRESSFLAG and CLONEFLAG
+      r.push_back({ctx->d_regress, 4 * (size_t)ctx->mcfg.max_cells});
+      r.push_back({ctx->d_cloneflag, 4 * (size_t)ctx->mcfg.max_cells});
+    }
+    if (ctx->d_band) r.push_back({ctx->d_band, 16 * (size_t)ctx->mcfg.max_cells});  // clone bands
   }
   return r;
 }

@@ -106,6 +106,32 @@ class Engine:
         """Couple this parent lattice's boundary to the main lattice `main`."""
         self._check(self.lib.zf_parent_link(self.h, main.h), "zf_parent_link")
 
+    def cell_wrap_setup(self, main, parent_index=1, **kw):
+        """zf_cell_wrap_setup: this parent lattice's cells in their periodic
+        window, cloning into `main`; the reference configuration of its
+        parent `parent_index` updated by kw."""
+        from ._abi import ZfCellWrapConfig
+        wc = ZfCellWrapConfig()
+        self.lib.zf_cell_wrap_config_reference(C.byref(wc), parent_index)
+        for k, v in kw.items():
+            setattr(wc, k, v)
+        self._check(self.lib.zf_cell_wrap_setup(self.h, main.h, C.byref(wc)), "zf_cell_wrap_setup")
+
+    def step_linked(self, it):
+        """zf_step_linked of this main lattice and its linked parents."""
+        self._check(self.lib.zf_step_linked(self.h, it), "zf_step_linked")
+
+    def cell_flags(self):
+        reg = np.zeros(self.ncell, np.int32)
+        clo = np.zeros(self.ncell, np.int32)
+        self._check(self.lib.zf_download_cell_flags(self.h, _ptr(reg), _ptr(clo)),
+                    "zf_download_cell_flags")
+        return reg, clo
+
+    def upload_cell_flags(self, regress):
+        r = np.ascontiguousarray(regress, np.int32)
+        self._check(self.lib.zf_upload_cell_flags(self.h, _ptr(r)), "zf_upload_cell_flags")
+
     def lists(self):
         dl = np.zeros((self.dommax, 3), np.int32)
         bl = np.zeros((self.bcmax, 4), np.int32)
