@@ -49,7 +49,8 @@ double dJ_dx0[3 * NodeM], dJ_df0[19 * IM * JM * KM], dJ_dF[3 * IM * JM * KM];
 EPILOGUE = r"""
 int main(void) {
   (void)zf_bind_setup; (void)zf_bind_nuclei; (void)zf_bind_loop; (void)zf_bind_parents;
-  (void)zf_bind_parent_steps; (void)zf_bind_gradient;
+  (void)zf_bind_parent_steps; (void)zf_bind_gradient; (void)zf_bind_parent_cells;
+  (void)zf_bind_linked_step;
   return 0;
 }
 """
@@ -58,7 +59,7 @@ int main(void) {
 def test_integration_snippets_compile_and_link(tmp_path):
     text = open(os.path.join(REPO, "INTEGRATION.md")).read()
     blocks = re.findall(r"```c\n(.*?)```", text, flags=re.S)
-    assert len(blocks) >= 6
+    assert len(blocks) >= 7
     src = tmp_path / "binding.c"
     src.write_text(PRELUDE + "\n".join(blocks) + EPILOGUE)
     exe = tmp_path / "binding"
