@@ -71,18 +71,19 @@ FLAGS = 0  # zf_config.flags of the measured forms (--flags)
 TRAFFIC = {"f32dev": 203603931, "f64": 399333437, "f16dev": 106048845}
 
 
-def engine_for(name, b, store):
+def engine_for(name, b, store, extra_flags=0):
     from zebraflow_amd.engine import Engine, lattice_config
     w = WORKLOADS[name]
+    flags = FLAGS | extra_flags
     if w["kind"] == "tube":
         # cfg1: tau 0.8, body force giving a peak velocity of 0.02 (g = 4 nu u / R^2)
         nu = 0.3 / 3
         cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], periodic=(0, 0, 1),
-                             body_force=(0, 0, 4 * nu * 0.02 / w["r"] ** 2), flags=FLAGS)
+                             body_force=(0, 0, 4 * nu * 0.02 / w["r"] ** 2), flags=flags)
     else:
         # pulsatile inlet u(t) = 0.03 (1 + 0.5 sin(2 pi t / 4000)) along +i
         cfg = lattice_config(*b.shape, 0.8, storage=STORE_NAMES[store], inlet_u0=0.03,
-                             inlet_amp=0.5, inlet_period=4000.0, inlet_dir=(1, 0, 0), flags=FLAGS)
+                             inlet_amp=0.5, inlet_period=4000.0, inlet_dir=(1, 0, 0), flags=flags)
     eng = Engine(cfg)
     eng.upload_lattice(b)
     return eng
@@ -285,9 +286,10 @@ def cells_leg(name, b, ncell, steps, warmup):
     xyz, tris = g.rbc_mesh(radius=4.0)
     cells = g.place_cells_dense(b, xyz, 0.6, ncell, 20261206)
     wall = g.wall_mesh(b)
+    from zebraflow_amd.engine import FLAG_F32_CELLS
     res = {}
-    for store in ("f32dev", "f64"):
-        eng = engine_for(name, b, store)
+    for store, fl in (("f32dev", 0), ("f64", 0), ("f32cells", FLAG_F32_CELLS)):
+        eng = engine_for(name, b, "f32dev" if store == "f32cells" else store, fl)
         eng.membrane_setup(lattice_membrane_config(xyz, tris, len(cells)), tris, xyz)
         eng.upload_cells(cells)
         eng.ibm_setup(lattice_ibm_config(rbc_interval=1))
@@ -306,6 +308,7 @@ def cells_leg(name, b, ncell, steps, warmup):
         eng.close()
     ms, x32, st, n_int = res["f32dev"]
     x64 = res["f64"][1]
+    ms_c, x32c = res["f32cells"][0], res["f32cells"][1]
     disp = np.linalg.norm(x64 - cells, axis=-1).mean()
     nn = len(xyz)
     p = xyz[tris - 1]
@@ -318,7 +321,11 @@ def cells_leg(name, b, ncell, steps, warmup):
                 cells_live_after=int((st > 0).sum()), rbc_interval=1,
                 shape_error_vs_fp64=dict(
                     max_node_dev=float(np.abs(x32 - x64).max()), mean_displacement=float(disp),
-                    rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))))
+                    rel=float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64 - cells))),
+                fp32_node_forces=dict(
+                    flag="ZF_FLAG_F32_CELLS", ms_per_step=round(ms_c / steps, 4),
+                    max_node_dev_vs_fp64=float(np.abs(x32c - x64).max()),
+                    rel_vs_fp64=float(np.linalg.norm(x32c - x64) / np.linalg.norm(x64 - cells))))
 
 
 def parent_leg(steps=500, warmup=50):

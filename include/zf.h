@@ -52,7 +52,8 @@ enum {
 };
 
 /* Kernel options (zf_config.flags); results do not depend on them except
- * ZF_FLAG_F64_ARITH, which computes the FP32/FP16 storage forms in FP64.
+ * ZF_FLAG_F64_ARITH, which computes the FP32/FP16 storage forms in FP64,
+ * and ZF_FLAG_F32_CELLS.
  * By default the stream kernel sets its cache policy from the population
  * footprint: every direction cached when both population buffers fit the
  * 256 MB last-level cache, otherwise as many directions as take about
@@ -60,6 +61,11 @@ enum {
 #define ZF_FLAG_LINEAR_BLOCKS 1u    /* plain block order instead of the XCD-aware one */
 #define ZF_FLAG_NT_POPULATIONS 2u   /* every population load and store non-temporal */
 #define ZF_FLAG_F64_ARITH 4u        /* FP64 arithmetic on FP32/FP16 storage */
+/* FP32 arithmetic in the membrane's node forces of the FP32/FP16 storage
+ * forms (positions relative to each node; triangle geometry, the per-cell
+ * area and volume reductions and the cell-cell and wall repulsion stay
+ * FP64).  Changes results at the FP32 level. */
+#define ZF_FLAG_F32_CELLS 8u
 /* Keep directions 0..k-1 cached (k = 19: all; an odd k < 19 counts as k-1),
  * the others non-temporal, instead of the footprint rule. */
 #define ZF_FLAG_CACHED_DIRS(k) ((uint32_t)((k) + 1) << 8)

@@ -75,6 +75,36 @@ def test_forces_match_oracle(ncell, blend):
     eng.close()
 
 
+@pytest.mark.parametrize("amp", [0.1, 0.3])
+def test_fp32_node_forces_match_oracle(amp):
+    """ZF_FLAG_F32_CELLS: the node forces in FP32 arithmetic (positions
+    relative to each node) against the FP64 oracle, to 1e-5 of the largest
+    force (FP32's 6e-8 rounding amplified by the cancellation of the spring,
+    bending and area terms at each node); the triangle geometry and the cell
+    reductions stay FP64 and match as tightly as without the flag."""
+    from zebraflow_amd.engine import FLAG_F32_CELLS, STORE_F32DEV
+    xyz, x1, tris = deformed(amp=amp)
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    rng = np.random.default_rng(3)
+    cells = np.stack([x1 * (1 + 0.02 * rng.standard_normal()) + rng.uniform(100, 400, 3)
+                      for _ in range(4)])
+    print(f"case FP32 node forces, amp {amp}", flush=True)
+    eng = Engine(lattice_config(8, 8, 8, 0.8, storage=STORE_F32DEV, flags=FLAG_F32_CELLS))
+    eng.membrane_setup(membrane_config(dict(mp, es=5e-3), 4), tris, xyz)
+    eng.upload_cells(cells)
+    eng.membrane_forces()
+    got = eng.membrane()
+    ref = po.membrane_forces(cells, tris, edge, eq, tae, mp)
+    scale = np.abs(ref["force"]).max()
+    err = np.abs(got["force"] - ref["force"]).max() / scale
+    print(f"FP32 node force error {err:.3e} of the largest force", flush=True)
+    assert 0 < err <= 1e-5  # FP32, not FP64, arithmetic ran
+    np.testing.assert_allclose(got["cell_area"], ref["cell_area"], rtol=1e-13)
+    np.testing.assert_allclose(got["cell_vol"], ref["cell_vol"], rtol=1e-13)
+    np.testing.assert_array_equal(got["guard_fail"], ref["guard_fail"])
+    eng.close()
+
+
 def test_guards_drop_terms_like_oracle():
     """A crushed triangle trips the force guards; the device drops the same
     terms and counts the same failures as the oracle."""

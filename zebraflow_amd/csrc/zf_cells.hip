@@ -958,6 +958,8 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
       (r = dev_alloc(ctx, (void **)&ctx->d_rep_bounds, 2 * sizeof(unsigned long long))))
     return r;
   ctx->bins_gen = ~0ull;
+  hipLaunchKernelGGL(k_fill, dim3(nb(2 * (size_t)std::max(ctx->dommax, 1), 256)), dim3(256), 0,
+                     ctx->stream, ctx->d_bin_range, 2 * (size_t)std::max(ctx->dommax, 1), 0);
   size_t tmp = 0;
   ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, ctx->d_tkeys[0], ctx->d_tkeys[1],
                                                   (int)TT, 0, 64, ctx->stream));
@@ -1154,7 +1156,10 @@ int zf_download_cell_list(zf_ctx *ctx, int32_t *bin) {
 int zf_repulsion(zf_ctx *ctx) {
   if (!ctx) return ZF_ERR_ARG;
   int rc = require_rep(ctx);
-  return rc ? rc : repulsion_impl(ctx);
+  if (rc) return rc;
+  // the kernel walks the bins of the last cell list
+  if (ctx->bins_gen == ~0ull) return fail(ctx, ZF_ERR_STATE, "build the cell list first (zf_cell_list)");
+  return repulsion_impl(ctx);
 }
 
 int zf_download_wall_neighbour(zf_ctx *ctx, int32_t *nearest) {

@@ -93,7 +93,10 @@ __global__ __launch_bounds__(256) void k_cell_reduce(const double *__restrict__ 
 
 // One lane per (cell, node): the node's normal, its anti-sliver terms and
 // every rhombus (edge) it belongs to in edge order, so F_Total is summed in
-// the reference's order with no atomics and no per-edge scratch.
+// the reference's order with no atomics and no per-edge scratch.  T: the
+// arithmetic of the forces (float: relative to the node, ZF_FLAG_F32_CELLS);
+// the triangle geometry and the cell scalars come in FP64 either way.
+template <class T>
 __global__ __launch_bounds__(128) void k_node_force(const double *__restrict__ X, size_t NT, int ncell, MeshTables mt,
                              const double *__restrict__ tg, size_t TT, const double *__restrict__ cell,
                              const int32_t *__restrict__ status, MembParams mp,
@@ -108,15 +111,16 @@ __global__ __launch_bounds__(128) void k_node_force(const double *__restrict__ X
     F[q] = F[NT + q] = F[2 * NT + q] = 0;
     return;
   }
-  const V3<double> cc = {cell[8 * c + 2], cell[8 * c + 3], cell[8 * c + 4]};
-  const NodeForce<double> r = node_force<double>(n, (size_t)c * nn, (size_t)c * mt.nt, X, NT, tg, TT,
-                                                 mt, cc, cell[8 * c + 5], cell[8 * c + 6], mp);
+  const V3<T> cc = liftp<T>(V3<double>{cell[8 * c + 2], cell[8 * c + 3], cell[8 * c + 4]},
+                           ld3(X, NT, q));
+  const NodeForce<T> r = node_force<T>(n, (size_t)c * nn, (size_t)c * mt.nt, X, NT, tg, TT, mt, cc,
+                                       T(cell[8 * c + 5]), T(cell[8 * c + 6]), mp);
   NN[q] = r.nrm.x;
   NN[NT + q] = r.nrm.y;
   NN[2 * NT + q] = r.nrm.z;
-  F[q] = r.f.x;
-  F[NT + q] = r.f.y;
-  F[2 * NT + q] = r.f.z;
+  F[q] = (double)r.f.x;
+  F[NT + q] = (double)r.f.y;
+  F[2 * NT + q] = (double)r.f.z;
   if (r.fails) atomicAdd(&fail[c], r.fails);
 }
 
@@ -502,9 +506,15 @@ int zf::membrane_forces_impl(zf_ctx *ctx) {
   hipLaunchKernelGGL(k_zero_i32, dim3(nb(nc, 256)), dim3(256), 0, ctx->stream, ctx->d_fail, nc);
   int rc = memb::membrane_geometry(ctx, cell_pos(ctx), ctx->d_tgeo, ctx->d_cell);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_node_force, dim3(nb((size_t)nc * ctx->nn, 128)), dim3(128), 0, ctx->stream,
-                     cell_pos(ctx), NT, nc, memb::mesh_tables(ctx), ctx->d_tgeo, TT, ctx->d_cell,
-                     ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal, ctx->d_fail);
+  // FP32 node forces on request for the FP32 and FP16 storage forms
+  if (ctx->cfg.storage != ZF_STORE_F64 && (ctx->cfg.flags & ZF_FLAG_F32_CELLS))
+    hipLaunchKernelGGL(k_node_force<float>, dim3(nb((size_t)nc * ctx->nn, 128)), dim3(128), 0,
+                       ctx->stream, cell_pos(ctx), NT, nc, memb::mesh_tables(ctx), ctx->d_tgeo, TT,
+                       ctx->d_cell, ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal, ctx->d_fail);
+  else
+    hipLaunchKernelGGL(k_node_force<double>, dim3(nb((size_t)nc * ctx->nn, 128)), dim3(128), 0,
+                       ctx->stream, cell_pos(ctx), NT, nc, memb::mesh_tables(ctx), ctx->d_tgeo, TT,
+                       ctx->d_cell, ctx->d_status, mp, ctx->d_mforce, ctx->d_nnormal, ctx->d_fail);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }

@@ -7,6 +7,7 @@
 #ifndef ZF_MEMBRANE_DEV_H
 #define ZF_MEMBRANE_DEV_H
 
+#include <type_traits>
 #include <cmath>
 
 #include "zf_internal.h"
@@ -187,12 +188,33 @@ template <class T>
 __device__ __forceinline__ V3<T> cross(const V3<T> &a, const V3<T> &b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+// The type of the constants in arithmetic of type T: FP32 with FP32, FP64
+// with FP64 and the dual numbers (so those keep their expressions exactly).
+template <class T>
+struct ScalarOf {
+  using type = double;
+};
+template <>
+struct ScalarOf<float> {
+  using type = float;
+};
+__device__ __forceinline__ V3<float> operator*(float s, const V3<float> &a) {
+  return {s * a.x, s * a.y, s * a.z};
+}
 template <class T>
 __device__ __forceinline__ V3<T> lift(const V3<double> &a) {
   return {T(a.x), T(a.y), T(a.z)};
 }
 __device__ __forceinline__ V3<double> ld3(const double *a, size_t stride, size_t i) {
   return {a[i], a[stride + i], a[2 * stride + i]};
+}
+// A position in the arithmetic type: FP32 works relative to the node the
+// force is for (the difference taken in FP64, so no absolute coordinate is
+// rounded to FP32); FP64 and the dual numbers take it as it is.
+template <class T>
+__device__ __forceinline__ V3<T> liftp(const V3<double> &a, const V3<double> &o) {
+  if constexpr (std::is_same<T, float>::value) return lift<T>(a - o);
+  else return lift<T>(a);
 }
 template <class T>
 __device__ __forceinline__ int nbad(const V3<T> &f, double lim) {
@@ -217,13 +239,15 @@ __device__ __forceinline__ TriG<T> tri_geom(const V3<T> &p1, const V3<T> &p2, co
   g.m = {-A.x / g.a, -A.y / g.a, -A.z / g.a};
   return g;
 }
-// Stored geometry (area, normal, centroid) of triangle q, as constants.
+// Stored geometry (area, normal, centroid) of triangle q, as constants;
+// the centroid as liftp takes positions (relative to o in FP32).
 template <class T>
-__device__ __forceinline__ TriG<T> tri_load(const double *__restrict__ tg, size_t TT, size_t q) {
+__device__ __forceinline__ TriG<T> tri_load(const double *__restrict__ tg, size_t TT, size_t q,
+                                            const V3<double> &o = V3<double>{0, 0, 0}) {
   TriG<T> g;
   g.a = T(tg[q]);
   g.m = {T(tg[TT + q]), T(tg[2 * TT + q]), T(tg[3 * TT + q])};
-  g.C = {T(tg[4 * TT + q]), T(tg[5 * TT + q]), T(tg[6 * TT + q])};
+  g.C = liftp<T>(V3<double>{tg[4 * TT + q], tg[5 * TT + q], tg[6 * TT + q]}, o);
   return g;
 }
 
@@ -252,7 +276,8 @@ __device__ __forceinline__ EdgeOut<T> edge_role(int r, const V3<T> &p1, const V3
                                                 const TriG<T> &g2, const EdgeConst &ec,
                                                 const V3<T> &cc, const T &f_ag,
                                                 const MembParams &mp) {
-  const double L = mp.L;
+  using S = typename ScalarOf<T>::type;  // the constants' type: FP32 in FP32 arithmetic
+  const S L = S(mp.L);
   EdgeOut<T> o;
   o.f = {T(0.0), T(0.0), T(0.0)};
   o.fb = o.f;
@@ -262,14 +287,14 @@ __device__ __forceinline__ EdgeOut<T> edge_role(int r, const V3<T> &p1, const V3
   // WLC + POW spring (:6888-6924)
   const V3<T> d12 = p1 - p2;
   T len = sqrt(dot(d12, d12));
-  const V3<T> e1v = (1.0 / len) * d12;
-  T ratio = len * L / ec.lmax;
-  if (ratio > mp.x_max) ratio = T(mp.x_max);
-  if (ratio < mp.x_floor) len = T(mp.x_floor * ec.lmax / L);
-  const T wlc = (-mp.kT / ec.plen) * (ratio - 0.25 + 1.0 / (4.0 * (1 - ratio) * (1 - ratio)));
-  const T comp = ec.kp / (len * len * L * L);
-  if (r == 0) o.f = (mp.relax2 * wlc) * e1v + (mp.relax2 * comp) * e1v;
-  if (r == 1) o.f = (-mp.relax2 * wlc) * e1v + (-mp.relax2 * comp) * e1v;
+  const V3<T> e1v = (S(1.0) / len) * d12;
+  T ratio = len * L / S(ec.lmax);
+  if (ratio > S(mp.x_max)) ratio = T(mp.x_max);
+  if (ratio < S(mp.x_floor)) len = T(mp.x_floor * ec.lmax / mp.L);
+  const T wlc = S(-mp.kT / ec.plen) * (ratio - S(0.25) + S(1.0) / (S(4.0) * (1 - ratio) * (1 - ratio)));
+  const T comp = S(ec.kp) / (len * len * L * L);
+  if (r == 0) o.f = (S(mp.relax2) * wlc) * e1v + (S(mp.relax2) * comp) * e1v;
+  if (r == 1) o.f = (S(-mp.relax2) * wlc) * e1v + (S(-mp.relax2) * comp) * e1v;
   // clamped triangle areas (:6929-6951)
   T a1 = g1.a, a2 = g2.a;
   if (a1 < mp.area_lo * ec.ae1) a1 = T(mp.area_lo * ec.ae1);
@@ -283,14 +308,14 @@ __device__ __forceinline__ EdgeOut<T> edge_role(int r, const V3<T> &p1, const V3
   const T chk = dot(m1 - m2, L * (g1.C - g2.C));
   const T st = chk >= 0.0 ? sqrt(1 - ct * ct) : -sqrt(1 - ct * ct);
   // bending (:6984-7081)
-  T bb = fabs(val(ct)) > mp.cos_cut ? T(0.0) : mp.kbend * (st * ec.c0 - ct * ec.s0) / st;
+  T bb = fabs(val(ct)) > mp.cos_cut ? T(0.0) : S(mp.kbend) * (st * S(ec.c0) - ct * S(ec.s0)) / st;
   if (mp.cos_blend > 0.0 && fabs(val(ct)) <= mp.cos_cut && fabs(val(ct)) > mp.cos_cut - mp.cos_blend) {
     // smooth cut-off (no counterpart in the reference): bb times the
     // smoothstep t^2 (3 - 2t) of t = (cos_cut - |cos|)/cos_blend, C1 in the
     // positions, equal to the reference's bb below cos_cut - cos_blend
     const T ac = val(ct) < 0.0 ? -ct : ct;
-    const T t = (T(mp.cos_cut) - ac) * (1.0 / mp.cos_blend);
-    bb = bb * (t * t * (T(3.0) - 2.0 * t));
+    const T t = (T(mp.cos_cut) - ac) * S(1.0 / mp.cos_blend);
+    bb = bb * (t * t * (T(3.0) - S(2.0) * t));
   }
   const V3<T> dp21 = L * (p2 - p1), dp24 = L * (p2 - p4), dp32 = L * (p3 - p2), dp41 = L * (p4 - p1),
               dp13 = L * (p1 - p3);
@@ -323,13 +348,13 @@ __device__ __forceinline__ EdgeOut<T> edge_role(int r, const V3<T> &p1, const V3
       g = r == 0 ? cp5 : r == 1 ? cp9 : cp1;
       gv = r == 0 ? cross(lc, dp32) : r == 1 ? cross(lc, dp13) : cross(lc, dp21);
     } else {
-      g = r == 0 ? cp4 : r == 1 ? cp8 : (-1.0) * cp2;
-      gv = r == 0 ? cross(lc, dp24) : r == 1 ? cross(lc, dp41) : (-1.0) * cross(lc, dp21);
+      g = r == 0 ? cp4 : r == 1 ? cp8 : S(-1.0) * cp2;
+      gv = r == 0 ? cross(lc, dp24) : r == 1 ? cross(lc, dp41) : S(-1.0) * cross(lc, dp21);
     }
-    const T alpha = -mp.kd * mp.relax * (a - ae) / (4.0 * ae * a);
+    const T alpha = S(-mp.kd * mp.relax) * (a - S(ae)) / (S(4.0 * ae) * a);
     const V3<T> fa = alpha * g;
-    const V3<T> faa = (0.25 * f_ag / a) * g;
-    const V3<T> fv = (1.0 / 6.0) * ((a / 3.0) * m + gv);
+    const V3<T> faa = (S(0.25) * f_ag / a) * g;
+    const V3<T> fv = S(1.0 / 6.0) * ((a / S(3.0)) * m + gv);
     const int ka = nbad(fa, mp.force_limit);
     o.fails += ka;
     if (!ka) o.f = o.f + fa;
@@ -351,7 +376,8 @@ __device__ __forceinline__ V3<T> sliver(const V3<T> &xn, const V3<T> &ct, const 
   const V3<T> ev = xn - ct;
   const T l = sqrt(dot(ev, ev));
   if (!(l < mp.sliver_len)) return {T(0.0), T(0.0), T(0.0)};
-  const T mag = l > mp.sliver_len_min ? mp.relax2 * mp.sliver_f1 / l : T(mp.relax2 * mp.sliver_f2);
+  using S = typename ScalarOf<T>::type;
+  const T mag = l > mp.sliver_len_min ? S(mp.relax2 * mp.sliver_f1) / l : T(mp.relax2 * mp.sliver_f2);
   return (mag / l) * ev;
 }
 
@@ -391,8 +417,9 @@ struct NodeForce {
 
 // F_Total of node n of the cell whose nodes start at nbase and triangles at
 // tbase, from the stored triangle geometry; the cell scalars (centroid cc,
-// global area force f_ag, volume force f_vs) are of type T.  SPLIT selects
-// the role with a switch (for wide T, fewer live registers).
+// global area force f_ag, volume force f_vs) are of type T, cc relative to
+// the node in FP32 (liftp).  SPLIT selects the role with a switch (for
+// wide T, fewer live registers).
 template <class T, bool SPLIT = false>
 __device__ __forceinline__ NodeForce<T> node_force(int n, size_t nbase, size_t tbase,
                                                    const double *__restrict__ X, size_t NT,
@@ -401,7 +428,8 @@ __device__ __forceinline__ NodeForce<T> node_force(int n, size_t nbase, size_t t
                                                    const T &f_ag, const T &f_vs,
                                                    const MembParams &mp) {
   NodeForce<T> out;
-  const V3<T> xn = lift<T>(ld3(X, NT, nbase + n));
+  const V3<double> org = ld3(X, NT, nbase + n);
+  const V3<T> xn = liftp<T>(org, org);
   // node area and normal, anti-sliver terms, in triangle order
   V3<T> f = {T(0.0), T(0.0), T(0.0)}, fbs = f;
   V3<double> nrm = {0, 0, 0};
@@ -414,7 +442,7 @@ __device__ __forceinline__ NodeForce<T> node_force(int n, size_t nbase, size_t t
     const V3<double> mv = {tg[TT + qt], tg[2 * TT + qt], tg[3 * TT + qt]};
     narea += a / 3.0;
     nrm = nrm + (a / 3.0) * mv;
-    const V3<T> ct = {T(tg[4 * TT + qt]), T(tg[5 * TT + qt]), T(tg[6 * TT + qt])};
+    const V3<T> ct = liftp<T>(V3<double>{tg[4 * TT + qt], tg[5 * TT + qt], tg[6 * TT + qt]}, org);
     f = f + sliver(xn, ct, mp);
   }
   const double inv = 1.0 / narea;
@@ -425,11 +453,12 @@ __device__ __forceinline__ NodeForce<T> node_force(int n, size_t nbase, size_t t
   const int ne = mt.ne;
   for (int k = mt.inc_start[n]; k < mt.inc_start[n + 1]; k++) {
     const int code = mt.inc[k], e = code >> 2;
-    const V3<T> p1 = lift<T>(ld3(X, NT, nbase + mt.E[e])), p2 = lift<T>(ld3(X, NT, nbase + mt.E[ne + e])),
-                p3 = lift<T>(ld3(X, NT, nbase + mt.E[2 * ne + e])),
-                p4 = lift<T>(ld3(X, NT, nbase + mt.E[3 * ne + e]));
-    const TriG<T> g1 = tri_load<T>(tg, TT, tbase + mt.E[4 * ne + e]);
-    const TriG<T> g2 = tri_load<T>(tg, TT, tbase + mt.E[5 * ne + e]);
+    const V3<T> p1 = liftp<T>(ld3(X, NT, nbase + mt.E[e]), org),
+                p2 = liftp<T>(ld3(X, NT, nbase + mt.E[ne + e]), org),
+                p3 = liftp<T>(ld3(X, NT, nbase + mt.E[2 * ne + e]), org),
+                p4 = liftp<T>(ld3(X, NT, nbase + mt.E[3 * ne + e]), org);
+    const TriG<T> g1 = tri_load<T>(tg, TT, tbase + mt.E[4 * ne + e], org);
+    const TriG<T> g2 = tri_load<T>(tg, TT, tbase + mt.E[5 * ne + e], org);
     const EdgeConst ec = edge_const(mt, e);
     EdgeOut<T> o;
     if (SPLIT) {  // one inlined copy per role: only that role's terms stay live
@@ -448,7 +477,7 @@ __device__ __forceinline__ NodeForce<T> node_force(int n, size_t nbase, size_t t
     fails += o.fails;
   }
   // volume constraint (:7527-7531)
-  f = f + (mp.relax * f_vs) * fv;
+  f = f + (typename ScalarOf<T>::type(mp.relax) * f_vs) * fv;
   out.f = f;
   out.fb = fbs;
   out.fails = fails;

@@ -16,7 +16,7 @@ WALL_EQUILIBRIUM, WALL_HALFWAY_BB = 0, 1
 B_VELOCITY_INLET, B_PRESSURE_OUTLET = 2001, 3001
 WSS_NOALLOC_MAX_POINTS = 196608  # ZF_WSS_NOALLOC_MAX_POINTS of include/zf.h
 # zf_config.flags (include/zf.h)
-FLAG_LINEAR_BLOCKS, FLAG_NT_POPULATIONS, FLAG_F64_ARITH = 1, 2, 4
+FLAG_LINEAR_BLOCKS, FLAG_NT_POPULATIONS, FLAG_F64_ARITH, FLAG_F32_CELLS = 1, 2, 4, 8
 
 
 def flag_cached_dirs(k):
