@@ -69,6 +69,13 @@ enum {
 /* Keep directions 0..k-1 cached (k = 19: all; an odd k < 19 counts as k-1),
  * the others non-temporal, instead of the footprint rule. */
 #define ZF_FLAG_CACHED_DIRS(k) ((uint32_t)((k) + 1) << 8)
+/* ZF_FLAG_PAIRED_STEPS: zf_run / zf_lbm_run advance the lattice two steps
+ * per launch (the state between them kept in a ring of slots in the caches
+ * instead of the population buffers; the same results bit for bit) when no
+ * parent lattice is linked.  Off by default: measured slower than one
+ * launch per step.  ZF_FLAG_SINGLE_STEPS: always one launch per step. */
+#define ZF_FLAG_SINGLE_STEPS 16u
+#define ZF_FLAG_PAIRED_STEPS 32u
 
 /* Boundary codes in b_index beyond the reference's (0 wall, 1..9 fluid with
  * tau = 0.5 + visc_scale*B, 1001..1009 pressure waveforms). */
@@ -154,8 +161,16 @@ int zf_download_force(zf_ctx *ctx, double *force);
  * boundary conditions (:6292-6433) and the copy (:6601-6626, a buffer swap
  * here).  rbc_step != 0 clears the force field afterwards, as :6622-6624. */
 int zf_lbm_step(zf_ctx *ctx, int32_t iter, int32_t rbc_step);
-/* nsteps iterations iter0+1 .. iter0+nsteps with rbc_step = 0. */
+/* nsteps iterations iter0+1 .. iter0+nsteps with rbc_step = 0 (two per
+ * launch where the lattice takes them, see ZF_FLAG_PAIRED_STEPS). */
 int zf_lbm_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps);
+/* The two-step launch on this lattice (its tables are built by this call
+ * if not yet): info[0] = 1 if zf_run / zf_lbm_run take it with the current
+ * flags, info[1] = 1 if the lattice admits it at all, info[2] = XCDs,
+ * info[3] = ghost blocks (the farthest pull, in 64-node blocks),
+ * info[4] = ring slots per XCD (64-node blocks), info[5] = workgroups of
+ * 256 lanes per launch.  No counterpart in the reference, whose loop has no launches. */
+int zf_pair_info(zf_ctx *ctx, int32_t info[6]);
 /* D, U, V, W of every listed node from the current populations without the
  * force shift (ZebraflowWT.c:6606-6614); each array dommax (any may be NULL). */
 int zf_download_macro(zf_ctx *ctx, double *d, double *u, double *v, double *w);

@@ -1,0 +1,182 @@
+"""Two LBM steps per launch (zf_run / zf_lbm_run with ZF_FLAG_PAIRED_STEPS)
+against one launch per step (ZF_FLAG_SINGLE_STEPS): the same stored
+population words, bit for bit, on every storage form and boundary kind the
+one-step kernel is tested with, and on a lattice large enough for the ring
+of the two-step launch to be reused many times per launch.  With a force
+field that varies per node the compiler may fuse a multiply-add
+differently in the two kernels: there the words agree to a bounded number
+of units in the last place (after 10 steps: the differences of single
+roundings carried along).  With cells (RBC iterations every third step, so
+pairs end on RBC iterations and single steps fill in) the force field is
+the one the cells spread, and the runs are compared to the same bound.
+Both runs start from the same seeded non-equilibrium state."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.test_gpu_cells import repulsion_config
+from tests.test_gpu_ibm import ibm_config
+from tests.test_oracle_mesh import lattice_params
+from zebraflow_amd import geometry
+from zebraflow_amd.engine import (B_PRESSURE_OUTLET, B_VELOCITY_INLET, FLAG_PAIRED_STEPS,
+                                  FLAG_SINGLE_STEPS, STORE_F16DEV, STORE_F32DEV, STORE_F64, Engine,
+                                  lattice_config, membrane_config)
+from tests.test_gpu_lbm import case
+
+pytestmark = pytest.mark.gpu
+
+STORES = {"f64": STORE_F64, "f32dev": STORE_F32DEV, "f16dev": STORE_F16DEV}
+
+
+def engines(b, cfg_of):
+    out = []
+    for flags in (FLAG_PAIRED_STEPS, FLAG_SINGLE_STEPS):
+        eng = Engine(cfg_of(flags))
+        eng.upload_lattice(b)
+        out.append(eng)
+    return out
+
+
+def seeded_state(n, seed=20261204):
+    rng = np.random.default_rng(seed)
+    rho = 1 + 0.002 * rng.standard_normal(n)
+    u = 0.01 * rng.standard_normal((3, n))
+    return po.feq(rho, u) * (1 + 0.01 * rng.standard_normal((19, n))), rng
+
+
+def ulps_of_largest(a, c):
+    """Largest |a - c| in units in the last place of the largest |word| of
+    the storage type (the stored words are deviations, many near zero)."""
+    fa, fc = a.astype(np.float64), c.astype(np.float64)
+    top = np.abs(fc).max().astype(a.dtype)
+    return float(np.abs(fa - fc).max() / float(np.spacing(top)))
+
+
+def compare_runs(b, cfg_of, steps, force_amp=None, iter0=0, ulps=0):
+    pe, se = engines(b, cfg_of)
+    info = pe.pair_info()
+    print(f"pair info {info}", flush=True)
+    assert info["admitted"] == 1 and info["used"] == 1
+    assert se.pair_info()["used"] == 0
+    f0, rng = seeded_state(pe.dommax)
+    for e in (pe, se):
+        e.upload_populations(f0)
+    if force_amp:
+        force = force_amp * rng.standard_normal((3, pe.dommax))
+        for e in (pe, se):
+            e.upload_force(force)
+    for e in (pe, se):
+        e.run(iter0, steps)
+        e.sync()
+    a, c = pe.populations_raw(), se.populations_raw()
+    pe.close()
+    se.close()
+    if ulps == 0:
+        assert np.array_equal(a, c), f"{np.count_nonzero(a != c)} words differ"
+    else:
+        # the compiler may fuse a multiply-add differently in the two
+        # kernels where the force varies per node: a few units in the last
+        # place, never more
+        worst = ulps_of_largest(a, c)
+        print(f"largest difference {worst:.2f} units in the last place of the largest word", flush=True)
+        assert worst <= ulps
+    return info
+
+
+@pytest.mark.parametrize("store", ["f64", "f32dev", "f16dev"])
+@pytest.mark.parametrize("name", ["tube_io", "pair"])
+def test_pairs_bit_identical_small(name, store):
+    b, cfg, steps, _ = case(name, STORES[store])
+    print(f"case {name} store {store} box {b.shape}", flush=True)
+
+    def cfg_of(flags):
+        _, c, _, _ = case(name, STORES[store])
+        c.flags = flags
+        return c
+
+    # odd count: the last step runs alone
+    compare_runs(b, cfg_of, 2 * (steps // 2) + 1, iter0=3)
+
+
+@pytest.mark.parametrize("store", ["f32dev", "f64"])
+def test_pairs_bit_identical_with_force_field(store):
+    b, cfg, steps, _ = case("periodic_force", STORES[store])
+    print(f"case periodic_force store {store} box {b.shape}", flush=True)
+
+    def cfg_of(flags):
+        _, c, _, _ = case("periodic_force", STORES[store])
+        c.flags = flags
+        return c
+
+    compare_runs(b, cfg_of, 10, force_amp=2e-6, ulps=64)
+
+
+@pytest.mark.parametrize("store", ["f32dev", "f16dev"])
+def test_pairs_bit_identical_vessel_pair_15(store):
+    """The headline lattice (1.28 M fluid nodes)."""
+    import bench
+    b = bench.make_geometry("vessel_pair_15")
+    print(f"case vessel_pair_15 store {store}", flush=True)
+
+    def cfg_of(flags):
+        return lattice_config(*b.shape, 0.8, storage=STORES[store], inlet_u0=0.03, inlet_amp=0.5,
+                              inlet_period=4000.0, inlet_dir=(1, 0, 0), flags=flags)
+
+    compare_runs(b, cfg_of, 6, iter0=100)
+
+
+@pytest.mark.parametrize("store", ["f32dev", "f64"])
+def test_pairs_bit_identical_long_tube_ring_reuse(store):
+    """A 2048-long tube of radius 20 with a velocity inlet and a pressure
+    outlet (2.5 M fluid nodes): each XCD's range is longer than its ring,
+    so ring slots are reused within every launch."""
+    b = geometry.tube((2048, 48, 48), axis=0, r=20.0, inlet=B_VELOCITY_INLET,
+                      outlet=B_PRESSURE_OUTLET)
+    print(f"case long tube store {store}", flush=True)
+
+    def cfg_of(flags):
+        return lattice_config(*b.shape, 0.8, storage=STORES[store], inlet_u0=0.02, inlet_amp=0.5,
+                              inlet_period=200.0, inlet_dir=(1, 0, 0), flags=flags)
+
+    info = compare_runs(b, cfg_of, 7, iter0=10)
+    n_int = int(((b > 0) & (b < 10)).sum())  # interior nodes
+    assert info["ring_blocks"] < n_int // 64 // info["xcds"] - 2 * info["ghost_blocks"]
+
+
+def test_pairs_bit_identical_with_cells():
+    """zf_run with cells (FP32 form, force field): RBC iterations every
+    third step, so a pair can end on an RBC iteration and some steps run
+    alone; populations within 64 units in the last place, positions within
+    1e-9 lu, repulsion velocities within 1e-12, the same cell states."""
+    print("case tube 32x32x128, 4 cells of 642 nodes, FP32, interval 3, 13 iterations", flush=True)
+    b = geometry.tube((32, 32, 128), axis=2, r=14.0, periodic=True)
+    xyz, tris = geometry.rbc_mesh(radius=4.0)
+    cells = geometry.place_cells(b, xyz, 4.0, 1.0, 4, 20261204)
+    wall = geometry.wall_mesh(b)
+    mp, edge, eq, tae = lattice_params(xyz, tris)
+    res = []
+    for flags in (FLAG_PAIRED_STEPS, FLAG_SINGLE_STEPS):
+        cfg = lattice_config(32, 32, 128, 0.8, storage=STORE_F32DEV, periodic=(0, 0, 1),
+                             body_force=(0, 0, 4e-5), flags=flags)
+        eng = Engine(cfg)
+        eng.upload_lattice(b)
+        eng.membrane_setup(membrane_config(dict(mp, es=5e-3), len(cells)), tris, xyz)
+        eng.upload_cells(cells)
+        eng.ibm_setup(ibm_config(rbc_interval=3, dt_advance=3.0))
+        eng.repulsion_setup(repulsion_config())
+        eng.wall_setup(*wall)
+        if flags == FLAG_PAIRED_STEPS:
+            assert eng.pair_info()["used"] == 1
+        eng.full_run(0, 13)
+        eng.sync()
+        res.append((eng.populations_raw(), eng.cells(), eng.repulsion()))
+        eng.close()
+    (fa, ca, va), (fb, cb, vb) = res
+    worst = ulps_of_largest(fa, fb)
+    dx = float(np.abs(ca[0] - cb[0]).max())
+    print(f"populations: {worst:.2f} units in the last place of the largest word; positions: "
+          f"{dx:.2e} lu", flush=True)
+    assert worst <= 64
+    assert dx <= 1e-9
+    assert np.array_equal(ca[2], cb[2])  # status
+    assert float(np.abs(va - vb).max()) <= 1e-12

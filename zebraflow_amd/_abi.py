@@ -87,6 +87,7 @@ FUNCS = {
     "zf_download_force": (C.c_int, [P, P]),
     "zf_lbm_step": (C.c_int, [P, i32, i32]),
     "zf_lbm_run": (C.c_int, [P, i32, i32]),
+    "zf_pair_info": (C.c_int, [P, P]),
     "zf_download_macro": (C.c_int, [P, P, P, P, P]),
     "zf_conservation_sums": (C.c_int, [P, P]),
     "zf_lbm_time_kernels": (C.c_int, [P, i32, i32, P]),

@@ -20,7 +20,7 @@ OBJ = os.path.join(PKG, "_build")
 HIP_SOURCES = ["zf_core.hip", "zf_lists.hip", "zf_lbm.hip", "zf_membrane.hip", "zf_ibm.hip",
                "zf_cells.hip", "zf_wss.hip", "zf_adjoint.hip", "zf_adjoint_ibm.hip",
                "zf_adjoint_cells.hip", "zf_state.hip", "zf_parent.hip", "zf_nucleus.hip",
-               "zf_wrap.hip"]
+               "zf_wrap.hip", "zf_pipe.hip"]
 HOST_SOURCES = ["zf_io.c", "zf_geom.c", "zf_mesh.c", "zf_wallmesh.c", "zf_loop.c", "zf_restart.c"]  # those present are built
 
 # gfx950 only; packed-FP32 instructions are disabled for all device code.

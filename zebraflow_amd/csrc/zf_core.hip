@@ -131,7 +131,7 @@ int zf_get_config(const zf_ctx *ctx, zf_config *cfg) {
 int zf_sync(zf_ctx *ctx) {
   if (!ctx) return ZF_ERR_ARG;
   ZF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return ZF_OK;
+  return pipe_check(ctx);
 }
 
 }  // extern "C"

@@ -917,9 +917,24 @@ int zf_step(zf_ctx *ctx, int32_t iter) {
 
 int zf_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps) {
   if (!ctx || nsteps < 0) return ZF_ERR_ARG;
-  for (int t = 1; t <= nsteps; t++) {
-    const int rc = zf_step(ctx, iter0 + t);
-    if (rc) return rc;
+  const bool pairs = nsteps >= 2 && pair_eligible(ctx);
+  auto rbc_iter = [&](int it) {
+    return ctx->have_ibm && ctx->ncell > 0 && it % ctx->icfg.rbc_interval == 0;
+  };
+  for (int t = 1; t <= nsteps;) {
+    const int it = iter0 + t;
+    int rc;
+    if (pairs && t < nsteps && !rbc_iter(it)) {
+      // two LBM steps in one launch; the cell stages of the second, if it
+      // is an RBC iteration, follow as in zf_step
+      const bool rbc2 = rbc_iter(it + 1);
+      if ((rc = step_pair(ctx, it, rbc2 ? 1 : 0))) return rc;
+      if (rbc2 && (rc = rbc_update(ctx, true, it + 1))) return rc;
+      t += 2;
+    } else {
+      if ((rc = zf_step(ctx, it))) return rc;
+      t++;
+    }
   }
   return ZF_OK;
 }

@@ -82,6 +82,8 @@ __device__ __forceinline__ int xcd_range_block(int b, int nblk) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+struct PipeState;  // two-step launches (zf_pipe.hip)
+
 }  // namespace zf
 
 struct zf_ctx {
@@ -119,6 +121,8 @@ struct zf_ctx {
   void *d_force = nullptr;            // 3 x stride, storage-indexed, field_elem bytes each; null = none
   double *d_sums = nullptr;           // reduction scratch
   int64_t device_bytes = 0;
+  zf::PipeState *pipe = nullptr;      // tables of the two-step launch, built on first use
+  bool pipe_used = false;             // a two-step launch since the last error check
 
   // membranes (shared mesh, per-cell state)
   bool have_mesh = false;
@@ -322,6 +326,18 @@ GridGeom grid_geom(const zf_ctx *ctx);
 int wss_scratch_init(zf_ctx *ctx);
 int alloc_state(zf_ctx *ctx);
 size_t store_bytes(int storage);
+// Two LBM steps per launch (zf_pipe.hip): whether ctx's lattice takes them
+// (builds the tables on first use), the launch pair iter1, iter1 + 1 (the
+// force field cleared after it when iter1 + 1 is an RBC iteration), the
+// error check after a synchronisation, and the tables' release.
+bool pair_eligible(zf_ctx *ctx);
+int step_pair(zf_ctx *ctx, int iter1, int rbc2);
+int pipe_check(zf_ctx *ctx);
+void free_pipe(zf_ctx *ctx);
+// zf_lbm.hip: whether the two-step launch is taken without
+// ZF_FLAG_PAIRED_STEPS (never: measured slower) and its cached directions.
+bool pair_default(const zf_ctx *ctx);
+int pair_cached_dirs(const zf_ctx *ctx);
 // Bytes of one element of the force field and of the IBM velocity and
 // node-force buffers: FP64 with FP64 storage (the FP64 form is FP64
 // throughout), FP32 otherwise.

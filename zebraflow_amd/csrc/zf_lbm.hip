@@ -27,36 +27,6 @@ namespace {
 using namespace d3q19;
 
 
-// ---------------------------------------------------------------------------
-// Boundary values, ZebraflowWT.c:6294-6433: all 19 populations of a BC node
-// set to feq(D_bc, U_bc) from its neighbour's rho, u.
-
-template <int S>
-__device__ __forceinline__ void write_bc(typename Store<S>::T *__restrict__ dst, size_t stride,
-                                         const BcEntry &e, typename Store<S>::R drm,
-                                         typename Store<S>::R um, typename Store<S>::R vm,
-                                         typename Store<S>::R wm, const BcParams &bp, bool nt) {
-  using R = typename Store<S>::R;
-  R drb, ub, vb, wb;  // density deviation and velocity of the boundary value
-  if (e.code == 0 || e.code == 4998) {
-    drb = drm; ub = vb = wb = 0;
-  } else if (e.code >= 1001 && e.code <= 1009) {
-    drb = (R)bp.pdrho[e.code - 1001]; ub = um; vb = vm; wb = wm;
-  } else if (e.code == ZF_B_VELOCITY_INLET) {
-    drb = drm; ub = (R)bp.inlet[0]; vb = (R)bp.inlet[1]; wb = (R)bp.inlet[2];
-  } else {  // ZF_B_PRESSURE_OUTLET
-    drb = (R)bp.outlet_drho; ub = um; vb = vm; wb = wm;
-  }
-  const R usq = ub * ub + vb * vb + wb * wb;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
-    const auto v = Store<S>::st(geq<R>(c, drb, ub, vb, wb, usq), WQ[c]);
-    if (nt) __builtin_nontemporal_store(v, &dst[(size_t)c * stride + e.s]);
-    else dst[(size_t)c * stride + e.s] = v;
-  }
-}
-
-
 // Stream + collide, one interior node per lane, with the boundary nodes of
 // the same step fused in.  The pull source of link c is s + delta_c(chord)
 // or, for a bounce-back link, f_{c^1} of the node itself.  HBM traffic per
@@ -131,37 +101,11 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     fy += (CT)force[stride + s];
     fz += (CT)force[2 * stride + s];
   }
-  const CT dt = (CT)prm.dt;
-  CT drho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
-    drho += g[c];
-    if (EX[c]) jx += EX[c] > 0 ? g[c] : -g[c];
-    if (EY[c]) jy += EY[c] > 0 ? g[c] : -g[c];
-    if (EZ[c]) jz += EZ[c] > 0 ? g[c] : -g[c];
-  }
-  // half-force velocity shift, ZebraflowWT.c:6132-6135: u = v / (1 + drho)
-  // taken as v - v q with q = drho / (1 + drho)
-  const CT q = drho / ((CT)1 + drho);
-  const CT vx = jx + (CT)0.5 * dt * fx;
-  const CT vy = jy + (CT)0.5 * dt * fy;
-  const CT vz = jz + (CT)0.5 * dt * fz;
-  const CT ux = vx - vx * q;
-  const CT uy = vy - vy * q;
-  const CT uz = vz - vz * q;
-  const CT usq = ux * ux + uy * uy + uz * uz;
-  const CT omega = (CT)1 / ((CT)0.5 + (CT)prm.visc_scale * (CT)d[18]);
-  const CT fpref = dt * ((CT)1 - (CT)0.5 * omega);  // DT*(1 - 1/(2 tau))
-  const CT uf = ux * fx + uy * fy + uz * fz;
+  CT drho, ux, uy, uz;
+  bgk_guo<CT>(g, fx, fy, fz, (CT)prm.dt, (CT)prm.visc_scale, d[18], drho, ux, uy, uz);
   for_dirs([&](auto cc) {
     constexpr int c = decltype(cc)::value;
-    const CT cu = (CT)EX[c] * ux + (CT)EY[c] * uy + (CT)EZ[c] * uz;
-    const CT cf = (CT)EX[c] * fx + (CT)EY[c] * fy + (CT)EZ[c] * fz;
-    const CT ge = geq<CT>(c, drho, ux, uy, uz, usq);
-    // Guo source, ZebraflowWT.c:6140: ((c-u).F)/cs^2 + (c.u)(c.F)/cs^4
-    const CT src_c = (CT)WQ[c] * ((CT)3 * (cf - uf) + (CT)9 * cu * cf);
-    const T v = Sx::st((R)(g[c] - omega * (g[c] - ge) + fpref * src_c), WQ[c]);
-    st_policy<(c >= KT)>(dst + (size_t)c * stride + s, v);
+    st_policy<(c >= KT)>(dst + (size_t)c * stride + s, Sx::st((R)g[c], WQ[c]));
   });
   if (w & kWordMacro) {
     // binary search of this node among the neighbours of fused entries
@@ -540,6 +484,13 @@ int step_into(zf_ctx *ctx, const void *src, void *dst, int iter) {
   return ZF_OK;
 }
 
+// Measured slower than one launch per step on every lattice tried (the
+// vessel pair and the 8.1 M-node tree, all storage forms), so never taken
+// by default.
+bool pair_default(const zf_ctx *) { return false; }
+
+int pair_cached_dirs(const zf_ctx *ctx) { return cache_plan(ctx).kt; }
+
 size_t store_bytes(int storage) {
   return storage == ZF_STORE_F64 ? 8 : storage == ZF_STORE_F32DEV ? 4 : 2;
 }
@@ -635,7 +586,7 @@ int zf_download_populations(zf_ctx *ctx, double *f) {
   ctx->device_bytes -= (int64_t)nbytes;
   dev_free(ctx, (void **)&tmp);
   if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "population download: %s", hipGetErrorString(e));
-  return ZF_OK;
+  return pipe_check(ctx);
 }
 
 int zf_download_populations_raw(zf_ctx *ctx, void *f, int64_t bytes) {
@@ -661,7 +612,7 @@ int zf_download_populations_raw(zf_ctx *ctx, void *f, int64_t bytes) {
   ctx->device_bytes -= (int64_t)nbytes;
   dev_free(ctx, &tmp);
   if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "raw population download: %s", hipGetErrorString(e));
-  return ZF_OK;
+  return pipe_check(ctx);
 }
 
 int zf_upload_populations_raw(zf_ctx *ctx, const void *f, int64_t bytes) {
@@ -770,8 +721,16 @@ int zf_lbm_run(zf_ctx *ctx, int32_t iter0, int32_t nsteps) {
   if (!ctx || nsteps < 0) return ZF_ERR_ARG;
   int rc = require_state(ctx);
   if (rc) return rc;
-  for (int t = 1; t <= nsteps; t++)
-    if ((rc = step(ctx, iter0 + t, 0))) return rc;
+  const bool pairs = nsteps >= 2 && pair_eligible(ctx);
+  for (int t = 1; t <= nsteps;) {
+    if (pairs && t < nsteps) {
+      if ((rc = step_pair(ctx, iter0 + t, 0))) return rc;
+      t += 2;
+    } else {
+      if ((rc = step(ctx, iter0 + t, 0))) return rc;
+      t++;
+    }
+  }
   return ZF_OK;
 }
 
@@ -829,7 +788,7 @@ int zf_download_macro(zf_ctx *ctx, double *d, double *u, double *v, double *w) {
   ctx->device_bytes -= (int64_t)(sizeof(double) * 4 * n);
   dev_free(ctx, (void **)&tmp);
   if (e != hipSuccess) return fail(ctx, ZF_ERR_HIP, "macro download: %s", hipGetErrorString(e));
-  return ZF_OK;
+  return pipe_check(ctx);
 }
 
 int zf_conservation_sums(zf_ctx *ctx, double out[4]) {

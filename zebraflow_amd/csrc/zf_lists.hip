@@ -244,6 +244,7 @@ int compact(zf_ctx *ctx, int mode, int32_t *d_counts, int ntiles, int32_t *list,
 }  // namespace
 
 void free_lattice(zf_ctx *ctx) {
+  free_pipe(ctx);
   void **bufs[] = {(void **)&ctx->d_b,        (void **)&ctx->d_domid,  (void **)&ctx->d_domlist,
                    (void **)&ctx->d_bclist,   (void **)&ctx->d_bcnlist, (void **)&ctx->d_nbr,
                    (void **)&ctx->d_s_of_p,   (void **)&ctx->d_p_of_s, (void **)&ctx->d_word,

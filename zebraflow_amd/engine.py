@@ -17,6 +17,7 @@ B_VELOCITY_INLET, B_PRESSURE_OUTLET = 2001, 3001
 WSS_NOALLOC_MAX_POINTS = 196608  # ZF_WSS_NOALLOC_MAX_POINTS of include/zf.h
 # zf_config.flags (include/zf.h)
 FLAG_LINEAR_BLOCKS, FLAG_NT_POPULATIONS, FLAG_F64_ARITH, FLAG_F32_CELLS = 1, 2, 4, 8
+FLAG_SINGLE_STEPS, FLAG_PAIRED_STEPS = 16, 32
 
 
 def flag_cached_dirs(k):
@@ -228,6 +229,13 @@ class Engine:
 
     def run(self, iter0, nsteps):
         self._check(self.lib.zf_lbm_run(self.h, iter0, nsteps), "zf_lbm_run")
+
+    def pair_info(self):
+        """zf_pair_info: the two-step launch on this lattice."""
+        out = np.zeros(6, np.int32)
+        self._check(self.lib.zf_pair_info(self.h, _ptr(out)), "zf_pair_info")
+        keys = ("used", "admitted", "xcds", "ghost_blocks", "ring_blocks", "workgroups")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def time_kernels(self, iter0, nsteps):
         out = np.zeros(3, np.float64)
