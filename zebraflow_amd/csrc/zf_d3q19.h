@@ -79,57 +79,6 @@ __device__ __forceinline__ R geq(int c, R drho, R ux, R uy, R uz, R usq) {
   return wmul<R>(c, drho + x + drho * x);
 }
 
-// BGK collision with Guo forcing of one node (ZebraflowWT.c:6127-6143) on
-// the pulled deviations g (overwritten with the post-collision deviations,
-// in the compute type CT): tau = 0.5 + visc_scale*bclass, half-force
-// velocity shift, DT*(1 - 1/(2 tau)) source.  Returns the density deviation
-// and the shifted velocity the boundary values of the same step take.
-// Both stream kernels (one step per launch, two steps per launch) call it,
-// so their arithmetic is the same expression by expression.  Which
-// multiply-adds the compiler fuses can still differ between the kernels
-// where an operand varies per node (a force field): results then agree to
-// the last bit or two.  Forbidding fusion here makes them bit-identical but
-// costs the one-step kernel 3 % (FP32) to 12 % (FP16), measured.
-template <class CT>
-__device__ __forceinline__ void bgk_guo(CT (&g)[Q], CT fx, CT fy, CT fz, CT dt, CT visc_scale,
-                                        int bclass, CT &drho_out, CT &ux_out, CT &uy_out,
-                                        CT &uz_out) {
-  CT drho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
-    drho += g[c];
-    if (EX[c]) jx += EX[c] > 0 ? g[c] : -g[c];
-    if (EY[c]) jy += EY[c] > 0 ? g[c] : -g[c];
-    if (EZ[c]) jz += EZ[c] > 0 ? g[c] : -g[c];
-  }
-  // half-force velocity shift, ZebraflowWT.c:6132-6135: u = v / (1 + drho)
-  // taken as v - v q with q = drho / (1 + drho)
-  const CT q = drho / ((CT)1 + drho);
-  const CT vx = jx + (CT)0.5 * dt * fx;
-  const CT vy = jy + (CT)0.5 * dt * fy;
-  const CT vz = jz + (CT)0.5 * dt * fz;
-  const CT ux = vx - vx * q;
-  const CT uy = vy - vy * q;
-  const CT uz = vz - vz * q;
-  const CT usq = ux * ux + uy * uy + uz * uz;
-  const CT omega = (CT)1 / ((CT)0.5 + visc_scale * (CT)bclass);
-  const CT fpref = dt * ((CT)1 - (CT)0.5 * omega);  // DT*(1 - 1/(2 tau))
-  const CT uf = ux * fx + uy * fy + uz * fz;
-#pragma unroll
-  for (int c = 0; c < Q; c++) {
-    const CT cu = (CT)EX[c] * ux + (CT)EY[c] * uy + (CT)EZ[c] * uz;
-    const CT cf = (CT)EX[c] * fx + (CT)EY[c] * fy + (CT)EZ[c] * fz;
-    const CT ge = geq<CT>(c, drho, ux, uy, uz, usq);
-    // Guo source, ZebraflowWT.c:6140: ((c-u).F)/cs^2 + (c.u)(c.F)/cs^4
-    const CT src_c = (CT)WQ[c] * ((CT)3 * (cf - uf) + (CT)9 * cu * cf);
-    g[c] = g[c] - omega * (g[c] - ge) + fpref * src_c;
-  }
-  drho_out = drho;
-  ux_out = ux;
-  uy_out = uy;
-  uz_out = uz;
-}
-
 // Boundary values, ZebraflowWT.c:6294-6433: all 19 populations of a BC node
 // set to feq(D_bc, U_bc) from its neighbour's rho, u.  Written to
 // dst[c * stride + e.s - off] (off: the first storage index dst holds).
@@ -198,6 +147,59 @@ __device__ __forceinline__ void for_dirs_seq(Fn &&f, std::integer_sequence<int, 
 template <class Fn>
 __device__ __forceinline__ void for_dirs(Fn &&f) {
   for_dirs_seq(f, std::make_integer_sequence<int, Q>());
+}
+
+// BGK collision with Guo forcing of one node (ZebraflowWT.c:6127-6143) on
+// the pulled deviations g: tau = 0.5 + visc_scale*bclass, half-force
+// velocity shift, DT*(1 - 1/(2 tau)) source.  Each post-collision deviation
+// (compute type CT) goes to out(integral_constant<int, c>, value) as soon
+// as it is formed, so a kernel stores it at once (fewer live registers).
+// Returns the density deviation and the shifted velocity the boundary
+// values of the same step take.  Both stream kernels (one step per launch,
+// two steps per launch) call it, so their arithmetic is the same expression
+// by expression.  Which multiply-adds the compiler fuses can still differ
+// between the kernels where an operand varies per node (a force field):
+// results then agree to the last bit or two.  Forbidding fusion here makes
+// them bit-identical but costs the one-step kernel 3 % (FP32) to 12 % (FP16),
+// measured.
+template <class CT, class Out>
+__device__ __forceinline__ void bgk_guo(const CT (&g)[Q], CT fx, CT fy, CT fz, CT dt, CT visc_scale,
+                                        int bclass, CT &drho_out, CT &ux_out, CT &uy_out,
+                                        CT &uz_out, Out &&out) {
+  CT drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int c = 0; c < Q; c++) {
+    drho += g[c];
+    if (EX[c]) jx += EX[c] > 0 ? g[c] : -g[c];
+    if (EY[c]) jy += EY[c] > 0 ? g[c] : -g[c];
+    if (EZ[c]) jz += EZ[c] > 0 ? g[c] : -g[c];
+  }
+  // half-force velocity shift, ZebraflowWT.c:6132-6135: u = v / (1 + drho)
+  // taken as v - v q with q = drho / (1 + drho)
+  const CT q = drho / ((CT)1 + drho);
+  const CT vx = jx + (CT)0.5 * dt * fx;
+  const CT vy = jy + (CT)0.5 * dt * fy;
+  const CT vz = jz + (CT)0.5 * dt * fz;
+  const CT ux = vx - vx * q;
+  const CT uy = vy - vy * q;
+  const CT uz = vz - vz * q;
+  const CT usq = ux * ux + uy * uy + uz * uz;
+  const CT omega = (CT)1 / ((CT)0.5 + visc_scale * (CT)bclass);
+  const CT fpref = dt * ((CT)1 - (CT)0.5 * omega);  // DT*(1 - 1/(2 tau))
+  const CT uf = ux * fx + uy * fy + uz * fz;
+  for_dirs([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    const CT cu = (CT)EX[c] * ux + (CT)EY[c] * uy + (CT)EZ[c] * uz;
+    const CT cf = (CT)EX[c] * fx + (CT)EY[c] * fy + (CT)EZ[c] * fz;
+    const CT ge = geq<CT>(c, drho, ux, uy, uz, usq);
+    // Guo source, ZebraflowWT.c:6140: ((c-u).F)/cs^2 + (c.u)(c.F)/cs^4
+    const CT src_c = (CT)WQ[c] * ((CT)3 * (cf - uf) + (CT)9 * cu * cf);
+    out(cc, g[c] - omega * (g[c] - ge) + fpref * src_c);
+  });
+  drho_out = drho;
+  ux_out = ux;
+  uy_out = uy;
+  uz_out = uz;
 }
 
 struct StepParams {

@@ -102,11 +102,11 @@ __global__ __launch_bounds__(256) void k_stream_collide(
     fz += (CT)force[2 * stride + s];
   }
   CT drho, ux, uy, uz;
-  bgk_guo<CT>(g, fx, fy, fz, (CT)prm.dt, (CT)prm.visc_scale, d[18], drho, ux, uy, uz);
-  for_dirs([&](auto cc) {
-    constexpr int c = decltype(cc)::value;
-    st_policy<(c >= KT)>(dst + (size_t)c * stride + s, Sx::st((R)g[c], WQ[c]));
-  });
+  bgk_guo<CT>(g, fx, fy, fz, (CT)prm.dt, (CT)prm.visc_scale, d[18], drho, ux, uy, uz,
+              [&](auto cc, CT v) {
+                constexpr int c = decltype(cc)::value;
+                st_policy<(c >= KT)>(dst + (size_t)c * stride + s, Sx::st((R)v, WQ[c]));
+              });
   if (w & kWordMacro) {
     // binary search of this node among the neighbours of fused entries
     int lo = 0, hi = n_fused - 1, k = -1;

@@ -252,17 +252,13 @@ __global__ __launch_bounds__(256) void k_stream_pair(
         fz += (CT)force[2 * stride + s];
       }
       CT drho, ux, uy, uz;
-      bgk_guo<CT>(g, fx, fy, fz, (CT)prm.dt, (CT)prm.visc_scale, d[18], drho, ux, uy, uz);
-      if (phase == 0) {
-        const int rs = ring_slot(s, px.b1_0, a.ring_blocks);
-#pragma unroll
-        for (int c = 0; c < Q; c++) ring[c * a.ring_elems + rs] = Sx::st((R)g[c], WQ[c]);
-      } else {
-        for_dirs([&](auto cc) {
-          constexpr int c = decltype(cc)::value;
-          st_policy<(c >= KT)>(B + (size_t)c * stride + s, Sx::st((R)g[c], WQ[c]));
-        });
-      }
+      const int rs = ring_slot(s, px.b1_0, a.ring_blocks);
+      bgk_guo<CT>(g, fx, fy, fz, (CT)prm.dt, (CT)prm.visc_scale, d[18], drho, ux, uy, uz,
+                  [&](auto cc, CT v) {
+                    constexpr int c = decltype(cc)::value;
+                    if (phase == 0) ring[c * a.ring_elems + rs] = Sx::st((R)v, WQ[c]);
+                    else st_policy<(c >= KT)>(B + (size_t)c * stride + s, Sx::st((R)v, WQ[c]));
+                  });
       if (w & kWordMacro) {
         int lo = 0, hi = a.n_fused - 1, k = -1;
         while (lo <= hi) {
