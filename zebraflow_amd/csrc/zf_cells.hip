@@ -141,6 +141,14 @@ __global__ void k_bin_ranges(const uint64_t *__restrict__ keys, int n, uint64_t 
   if (i == n - 1 || (keys[i + 1] >> 32) != t) range[2 * t + 1] = i + 1;
 }
 
+// Keys per bin (from the bin ranges), the input of the exclusive scan that
+// gives each bin the offset of its first key (bins 0..nbins, the last 0).
+__global__ void k_bin_counts(const int32_t *__restrict__ range, int nbins, int32_t *__restrict__ cnt) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d > nbins) return;
+  cnt[d] = d < nbins ? range[2 * d + 1] - range[2 * d] : 0;
+}
+
 __global__ void k_fill(int32_t *a, size_t n, int32_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -156,6 +164,12 @@ struct RepParams {
 // distances and far below any geometric scale.
 constexpr double kReachSlack = 1e-9;
 
+// entries of a lane's queue of passing triangles in the repulsion kernel
+constexpr int kRepQueue = 32;
+// entries of a lane's queue of wall nodes in the wall-repulsion kernel (a
+// walking node queues 27 on average in the 15-segment vessel pair)
+constexpr int kWallQueue = 40;
+
 // Sub-triangle centroids of a triangle refined four times
 // (ZebraflowWT.c:9990-10025): the reference forms them through chained edge
 // midpoints; centroid s lies at (A a + B b + C c)/12 with integer weights
@@ -164,13 +178,89 @@ constexpr double kReachSlack = 1e-9;
 // agree to rounding.  The weights are packed 4 bits per centroid (s in bits
 // 4s..4s+3), which lets a lane form one centroid at a time from scalar
 // constants instead of holding all 16.
-// entries of a lane's queue of passing triangles in the repulsion kernel
-constexpr int kRepQueue = 32;
-// entries of a lane's queue of wall nodes in the wall-repulsion kernel
-constexpr int kWallQueue = 12;
-
 constexpr uint64_t kSubWA = 0x112412457124578aull, kSubWB = 0x142175421a875421ull,
                    kSubWC = 0xa787454541212121ull;
+
+// Margins of the FP32 prefilters of the repulsion kernel.  A coordinate a
+// rounded to FP32 is off by at most |a| 2^-24, so a node-to-centroid
+// distance formed from FP32 values is off by less than
+// 2^-22 (|x| + |y| + |z| + 3 reach) for candidates within reach; the margin
+// 2^-20 (|x| + |y| + |z| + 4 reach) is four times that.  Unit normals are off
+// by 6e-8 per component and their FP32 dot product by less than 1e-6.  A
+// candidate is dropped in FP32 only when it is beyond the FP64 tests' reach
+// by more than these margins, so the set of candidates reaching the FP64
+// tests, and the sums, are unchanged.  Likewise a candidate is taken as
+// within reach, or facing away, on the FP32 values alone only when it is
+// so by more than the margins.  The normals are stored packed (each
+// component off by at most 1/1022), so their dot product with the node's
+// normal is off by less than 3/1022 + 1e-6; the margin is 4e-3.
+constexpr float kRepDotMargin = 4e-3f;
+
+// FP32 records of the sorted cell list, one float4 per key in sorted order:
+// the centroid, and the normal packed as three 10-bit signed fractions of
+// 511 (each component off by at most 1/1022), so that the repulsion kernel
+// reads a row's candidates contiguously, one 16-byte load each.
+__device__ __forceinline__ uint32_t pack_normal(double x, double y, double z) {
+  auto q = [](double v) { return (uint32_t)((int)rint(fmin(fmax(v, -1.0), 1.0) * 511.0) & 1023); };
+  return q(x) | (q(y) << 10) | (q(z) << 20);
+}
+__device__ __forceinline__ float unpack_comp(uint32_t w, int shift) {
+  return (float)((int)(w << (22 - shift)) >> 22) * (1.0f / 511.0f);
+}
+
+__global__ void k_cand_records(const uint64_t *__restrict__ keys, int n, const double *__restrict__ tg,
+                               size_t TT, float4 *__restrict__ rec) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int q = (int)(keys[e] & 0xffffffffu);
+  rec[e] = make_float4((float)tg[4 * TT + q], (float)tg[5 * TT + q], (float)tg[6 * TT + q],
+                       __uint_as_float(pack_normal(tg[TT + q], tg[2 * TT + q], tg[3 * TT + q])));
+}
+
+// The 16-point contribution of triangle q (cell m = q / nt) to the
+// repulsion velocity of a node at (xc, yc, zc), ZebraflowWT.c:9990-10052,
+// added to (v0, v1, v2) sub-centroid by sub-centroid.
+__device__ __forceinline__ void rep_triangle(int q, int nt, int nn, size_t NT, const int32_t *__restrict__ tri,
+                                             const double *__restrict__ X, double xc, double yc, double zc,
+                                             const RepParams &r, double &v0, double &v1, double &v2) {
+  const int m = q / nt, t = q % nt;
+  const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
+  const size_t b0 = (size_t)m * nn;
+  const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = X[2 * NT + b0 + n1];
+  const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = X[2 * NT + b0 + n2];
+  const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = X[2 * NT + b0 + n3];
+#pragma unroll 2
+  for (int sc = 0; sc < 16; sc++) {
+    const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
+    const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
+    const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
+    const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
+    const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
+    const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
+    const double d = sqrt(px * px + py * py + pz * pz);
+    if (d < r.cutoff) {
+      const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
+      v0 += f * px / d;
+      v1 += f * py / d;
+      v2 += f * pz / d;
+    }
+  }
+}
+
+// Triangles listed per node by the walk (kernel k_cell_repulsion<1>); a
+// node with more goes through the whole walk again with the evaluation
+// (k_cell_repulsion<0>).  At 1200 cells of 642 nodes in the 15-segment
+// vessel pair (haematocrit 0.094) the most any node lists is 47.
+constexpr int kRepPairs = 48;
+// Nodes with listed triangles are appended to one of kRepClasses lists by
+// their count (1-4, 5-8, 9-16, 17-32, 33-48), so an evaluation wave holds
+// nodes of similar cost.  Layout of the list buffer: kRepClasses counts,
+// padded to 64 words, then one region of NT entries per class.
+constexpr int kRepClasses = 5;
+constexpr int kRepListHead = 64;
+__device__ __forceinline__ int rep_class(int np) {
+  return np <= 4 ? 0 : np <= 8 ? 1 : np <= 16 ? 2 : np <= 32 ? 3 : 4;
+}
 
 // Cell-cell repulsion, one lane per node of a released cell (status > 0).
 // The window is walked row by row (fixed i, j): DOMID is k-fastest and the
@@ -180,18 +270,21 @@ constexpr uint64_t kSubWA = 0x112412457124578aull, kSubWB = 0x142175421a875421ul
 // issued together (in-bounds addresses, results selected afterwards), and
 // candidates are fetched 4 at a time, so a lane waits on a few batched
 // loads per row instead of a chain of dependent loads per bin.
-__global__ __launch_bounds__(128) void k_cell_repulsion(
-    const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
+template <int MODE>
+__device__ __forceinline__ void rep_walk(
+    int gq, const double *__restrict__ X, size_t NT, int nn, int nt,
     const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
     const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
-    const int32_t *__restrict__ domid, const uint64_t *__restrict__ keys,
-    const int32_t *__restrict__ range, const unsigned long long *__restrict__ bounds, Grid g,
-    RepParams r, double *__restrict__ vrep) {
-  const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (gq >= ncell * nn) return;
+    const int32_t *__restrict__ domid, const float4 *__restrict__ crec, const uint64_t *__restrict__ keys,
+    const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds, Grid g,
+    RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ lists,
+    int32_t *__restrict__ over, double *__restrict__ vrep) {
   const int c = gq / nn, node = gq % nn;
   // released cells and clones (the reference's :9920-9921, :10055)
-  if (status[c] == 0 || status[c] <= kStatusDead) return;
+  if (status[c] == 0 || status[c] <= kStatusDead) {
+    if (MODE == 1) npair[gq] = -1;
+    return;
+  }
   const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
   const double e1x = nnormal[gq], e1y = nnormal[NT + gq], e1z = nnormal[2 * NT + gq];
   int lo[3], hi[3];
@@ -215,43 +308,29 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
     breach = treach + bd + kReachSlack;
   }
   const double treach2 = treach * treach, breach2 = breach * breach;
-  const int2 *range2 = reinterpret_cast<const int2 *>(range);
+  const double rmargin = std::ldexp(fabs(xc) + fabs(yc) + fabs(zc) + 4.0 * treach, -20);
+  const float treach_f2 = (float)((treach + rmargin) * (treach + rmargin));
+  // within reach for certain: below treach - margin (and so below search,
+  // treach being at most search plus a slack far below the margin)
+  const float tinner_f2 = (float)(fmax(treach - rmargin, 0.0) * fmax(treach - rmargin, 0.0));
+  const float xf = (float)xc, yf = (float)yc, zf = (float)zc;
+  const float e1xf = (float)e1x, e1yf = (float)e1y, e1zf = (float)e1z;
   // Triangles that pass the three filters go to the lane's queue in LDS,
   // and the 16-point sub-triangle evaluation runs when any active lane's
   // queue is nearly full (and once at the end), every active lane draining
   // its own queue in order: the lanes then evaluate their candidates
   // together instead of one or two at a time, and each lane's sum keeps its
   // candidate order.
-  __shared__ int queue[kRepQueue][128];
+  __shared__ int queue[MODE == 0 ? kRepQueue : 1][128];
   int qlen = 0;
   auto drain = [&]() {
-    for (int k = 0; k < qlen; k++) {
-      const int q = queue[k][threadIdx.x];
-      const int m = q / nt, t = q % nt;
-      const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
-      const size_t b0 = (size_t)m * nn;
-      const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = X[2 * NT + b0 + n1];
-      const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = X[2 * NT + b0 + n2];
-      const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = X[2 * NT + b0 + n3];
-#pragma unroll 2
-      for (int sc = 0; sc < 16; sc++) {
-        const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
-        const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
-        const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
-        const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
-        const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
-        const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
-        const double d = sqrt(px * px + py * py + pz * pz);
-        if (d < r.cutoff) {
-          const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
-          v0 += f * px / d;
-          v1 += f * py / d;
-          v2 += f * pz / d;
-        }
-      }
+    if constexpr (MODE == 0) {
+      for (int k = 0; k < qlen; k++)
+        rep_triangle(queue[k][threadIdx.x], nt, nn, NT, tri, X, xc, yc, zc, r, v0, v1, v2);
     }
     qlen = 0;
   };
+  int np = 0;  // mode 1: triangles listed
   for (int i = lo[0]; i <= hi[0]; i++)
     for (int j = lo[1]; j <= hi[1]; j++) {
       const double qx = g.x0 + (i - 1) * g.dx - xc, qy = g.y0 + (j - 1) * g.dx - yc;
@@ -261,171 +340,344 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
       int ov[9];
 #pragma unroll
       for (int t = 0; t < 9; t++) ov[t] = domid[row + (size_t)(min(lo[2] + t, hi[2]) - 1)];
-      int beg = 0x7fffffff, end = 0;
+      // the bins in reach are a k-interval of the row; its listed nodes
+      // have consecutive DOMIDs (k is the fastest index of the list), so
+      // their triangles are the sorted keys from the first one's offset to
+      // past the last one's
+      int dfirst = 0, dlast = -1;
 #pragma unroll
       for (int t = 0; t < 9; t++) {
         const int k = lo[2] + t;
         const double qz = g.z0 + (k - 1) * g.dx - zc;
         const bool ok = k <= hi[2] && rij + qz * qz < breach2 && ov[t] > 0;
-        const int2 rg = range2[ok ? ov[t] - 1 : 0];  // loaded for every t, used by select
-        const bool use = ok && rg.y > rg.x;
-        beg = use ? min(beg, rg.x) : beg;
-        end = use ? max(end, rg.y) : end;
+        dfirst = (ok && dlast < 0) ? ov[t] : dfirst;
+        dlast = ok ? ov[t] : dlast;
       }
+      if (dlast < 0) continue;
+      const int beg = bin_off[dfirst - 1], end = bin_off[dlast];
       for (int e0 = beg; e0 < end; e0 += 4) {
         // room for this batch's 4 candidates in every active lane's queue
-        if (__ballot(qlen > kRepQueue - 4)) drain();
-        int qv[4];
-        double cxv[4], cyv[4], czv[4];
+        if (MODE == 0 && __ballot(qlen > kRepQueue - 4)) drain();
+        float4 ca[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) qv[u] = (int)(keys[min(e0 + u, end - 1)] & 0xffffffffu);
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          cxv[u] = tg[4 * TT + qv[u]];
-          cyv[u] = tg[5 * TT + qv[u]];
-          czv[u] = tg[6 * TT + qv[u]];
-        }
+        for (int u = 0; u < 4; u++) ca[u] = crec[min(e0 + u, end - 1)];
 #pragma unroll 1
         for (int u = 0; u < 4 && e0 + u < end; u++) {
-          const int q = qv[u];
-          const double dx = xc - cxv[u], dy = yc - cyv[u], dz = zc - czv[u];
-          const double dc2 = dx * dx + dy * dy + dz * dz;
-          if (dc2 >= treach2 || sqrt(dc2) >= r.search) continue;
+          // FP32 tests first: a candidate clearly beyond reach or facing
+          // the same way is dropped, one clearly within reach and facing
+          // away passes; only one within a margin of either bound takes
+          // the FP64 test (same decisions as the FP64 tests alone)
+          const float fx = xf - ca[u].x, fy = yf - ca[u].y, fz = zf - ca[u].z;
+          const float d2f = fx * fx + fy * fy + fz * fz;
+          if (d2f > treach_f2) continue;
+          const uint32_t pn = __float_as_uint(ca[u].w);
+          const float dotf = e1xf * unpack_comp(pn, 0) + e1yf * unpack_comp(pn, 10) + e1zf * unpack_comp(pn, 20);
+          if (dotf > kRepDotMargin) continue;
+          const int q = (int)(keys[e0 + u] & 0xffffffffu);
+          if (d2f >= tinner_f2) {
+            const double dx = xc - tg[4 * TT + q], dy = yc - tg[5 * TT + q], dz = zc - tg[6 * TT + q];
+            const double dc2 = dx * dx + dy * dy + dz * dz;
+            if (dc2 >= treach2 || sqrt(dc2) >= r.search) continue;
+          }
           const int m = q / nt, t = q % nt;
           const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
           if (m == c && (n1 == node || n2 == node || n3 == node)) continue;
-          if (e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0) continue;
-          queue[qlen][threadIdx.x] = q;
-          qlen++;
+          if (dotf >= -kRepDotMargin && e1x * tg[TT + q] + e1y * tg[2 * TT + q] + e1z * tg[3 * TT + q] >= 0)
+            continue;
+          if constexpr (MODE == 0) {
+            queue[qlen][threadIdx.x] = q;
+            qlen++;
+          } else {
+            if (np < kRepPairs) pairs[(size_t)gq * kRepPairs + np] = q;
+            np++;
+          }
         }
       }
     }
-  drain();
-  vrep[gq] += v0;
-  vrep[NT + gq] += v1;
-  vrep[2 * NT + gq] += v2;
+  if constexpr (MODE == 1) {
+    npair[gq] = np;
+    if (np > kRepPairs) over[1 + atomicAdd(over, 1)] = gq;
+    const int cls = np >= 1 && np <= kRepPairs ? rep_class(np) : -1;
+    // one class per pass, so each atomic has one address across the wave
+#pragma unroll
+    for (int k = 0; k < kRepClasses; k++)
+      if (cls == k) lists[kRepListHead + (size_t)k * NT + atomicAdd(lists + k, 1)] = gq;
+  } else {
+    drain();
+    vrep[gq] += v0;
+    vrep[NT + gq] += v1;
+    vrep[2 * NT + gq] += v2;
+  }
 }
 
-// Cell-wall repulsion, one lane per node of a released cell.
-__global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict__ X, size_t NT, int nn, int ncell,
-                                 const int32_t *__restrict__ status, const int32_t *__restrict__ windex,
+template <int MODE>
+__global__ __launch_bounds__(128) void k_cell_repulsion(
+    const double *__restrict__ X, size_t NT, int nn, int nt, int ncell,
+    const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
+    const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
+    const int32_t *__restrict__ domid, const float4 *__restrict__ crec, const uint64_t *__restrict__ keys,
+    const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds, Grid g,
+    RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ lists,
+    int32_t *__restrict__ over, double *__restrict__ vrep) {
+  if constexpr (MODE == 1) {
+    const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (gq >= ncell * nn) return;
+    rep_walk<MODE>(gq, X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off, bounds, g, r,
+                   npair, pairs, lists, over, vrep);
+  } else {
+    // the nodes whose passing triangles did not fit their list (over[0]
+    // of them, listed from over[1] on)
+    const int nover = over[0];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nover; i += gridDim.x * blockDim.x)
+      rep_walk<MODE>(over[1 + i], X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off,
+                     bounds, g, r, npair, pairs, lists, over, vrep);
+  }
+}
+
+// The listed triangles of every node that had 1 to kRepPairs, in the walk's
+// order (the same sums as the walk's own evaluation).  Entry i of the class
+// lists taken heaviest class first, grid-stride.
+__global__ __launch_bounds__(128) void k_rep_eval(const double *__restrict__ X, size_t NT, int nn, int nt,
+                                                  const int32_t *__restrict__ tri,
+                                                  const int32_t *__restrict__ npair,
+                                                  const int32_t *__restrict__ pairs,
+                                                  const int32_t *__restrict__ lists, RepParams r,
+                                                  double *__restrict__ vrep) {
+  int cnt[kRepClasses], total = 0;
+#pragma unroll
+  for (int k = 0; k < kRepClasses; k++) total += (cnt[k] = lists[k]);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    int k = kRepClasses - 1, at = i;
+    while (at >= cnt[k]) at -= cnt[k--];
+    const int gq = lists[kRepListHead + (size_t)k * NT + at];
+    const int np = npair[gq];
+    const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
+    double v0 = 0, v1 = 0, v2 = 0;
+    for (int p = 0; p < np; p++)
+      rep_triangle(pairs[(size_t)gq * kRepPairs + p], nt, nn, NT, tri, X, xc, yc, zc, r, v0, v1, v2);
+    vrep[gq] += v0;
+    vrep[NT + gq] += v1;
+    vrep[2 * NT + gq] += v2;
+  }
+}
+
+// Whole-node skip of the wall repulsion: with q0 the lattice node nearest x
+// and d0 its distance to its nearest wall node (W_index), every wall node is
+// at least d0 - |x - q0| from x.  A lattice node without a W_index entry lies
+// outside the 9^3 window of every wall node, at least 3.9 dx from all of
+// them.  If that bound reaches sqrt(wall_reach2), no triangle can be chosen
+// (see the per-node test in k_wall_repulsion) and the node gets nothing.
+__device__ __forceinline__ bool wall_clear(double x, double y, double z, const float *__restrict__ wdist,
+                                           const Grid &g, const RepParams &r) {
+  const int i0 = (int)floor((x - g.x0) / g.dx + 0.5) + 1, j0 = (int)floor((y - g.y0) / g.dx + 0.5) + 1,
+            k0 = (int)floor((z - g.z0) / g.dx + 0.5) + 1;
+  if (i0 < 1 || i0 > g.im || j0 < 1 || j0 > g.jm || k0 < 1 || k0 > g.km) return false;
+  const float w0 = wdist[bq(g, i0, j0, k0)];
+  const double d0 = isinf(w0) ? 3.9 * g.dx : (double)w0 * (1.0 - 1e-6);
+  const double qx = g.x0 + (i0 - 1) * g.dx - x, qy = g.y0 + (j0 - 1) * g.dx - y,
+               qz = g.z0 + (k0 - 1) * g.dx - z;
+  const double lb = d0 - sqrt(qx * qx + qy * qy + qz * qz);
+  return lb > 0 && lb * lb >= r.wall_reach2;
+}
+
+// The nodes of released cells that wall_clear does not clear, appended to
+// list (count in list[0], nodes from list[1] on, in no particular order).
+__global__ __launch_bounds__(256) void k_wall_select(const double *__restrict__ X, size_t NT, int nn, int ncell,
+                                                     const int32_t *__restrict__ status,
+                                                     const float *__restrict__ wdist, Grid g, RepParams r,
+                                                     int32_t *__restrict__ list) {
+  const int gq = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gq >= ncell * nn) return;
+  if (status[gq / nn] == 0 || status[gq / nn] <= kStatusDead) return;
+  if (wall_clear(X[gq], X[NT + gq], X[2 * NT + gq], wdist, g, r)) return;
+  list[1 + atomicAdd(list, 1)] = gq;
+}
+
+// Cell-wall repulsion (ZebraflowWT.c:10030-10052), one lane per listed node.
+// The reference walks the 9^3 window in (i, j, k) order, and for each W_index
+// wall node each of its triangles' three vertices and centroid, keeping the
+// triangle of the first point strictly nearer than all before it, starting
+// from wall_search: the winner is the point of least distance, the earliest
+// in that order among equal distances.  Here every point carries its order
+// (the visit number of its wall node, then its triangle and point), and the
+// lexicographically least (distance, order) wins, whatever the order of
+// evaluation: the same triangle.
+__global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict__ X, size_t NT,
+                                 const int32_t *__restrict__ list, const int32_t *__restrict__ windex,
                                  const float *__restrict__ wdist,
                                  const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
-                                 const double *__restrict__ wxyz, const double *__restrict__ wrad,
-                                 const double *__restrict__ wtgeo, const double *__restrict__ wnormal,
+                                 const double *__restrict__ wxyz, const float4 *__restrict__ wpos,
+                                 const float4 *__restrict__ wn_sph, const double *__restrict__ wn_geo,
+                                 const double *__restrict__ wnormal,
                                  Grid g, RepParams r, double *__restrict__ vrep,
                                  int32_t *__restrict__ nearest) {
-  const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (gq >= ncell * nn) return;
-  nearest[gq] = 0;
-  if (status[gq / nn] == 0 || status[gq / nn] <= kStatusDead) return;
-  const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
-  // Whole-node skip: with q0 the lattice node nearest x and d0 its distance
-  // to its nearest wall node (W_index), every wall node is at least
-  // d0 - |x - q0| from x.  A lattice node without a W_index entry lies
-  // outside the 9^3 window of every wall node, at least 3.9 dx from all of
-  // them.  If that bound reaches sqrt(wall_reach2), no triangle can be
-  // chosen (see the per-node test below) and the node gets nothing.
-  {
-    const int i0 = (int)floor((x - g.x0) / g.dx + 0.5) + 1, j0 = (int)floor((y - g.y0) / g.dx + 0.5) + 1,
-              k0 = (int)floor((z - g.z0) / g.dx + 0.5) + 1;
-    if (i0 >= 1 && i0 <= g.im && j0 >= 1 && j0 <= g.jm && k0 >= 1 && k0 <= g.km) {
-      const float w0 = wdist[bq(g, i0, j0, k0)];
-      const double d0 = isinf(w0) ? 3.9 * g.dx : (double)w0 * (1.0 - 1e-6);
-      const double qx = g.x0 + (i0 - 1) * g.dx - x, qy = g.y0 + (j0 - 1) * g.dx - y,
-                   qz = g.z0 + (k0 - 1) * g.dx - z;
-      const double lb = d0 - sqrt(qx * qx + qy * qy + qz * qz);
-      if (lb > 0 && lb * lb >= r.wall_reach2) return;
-    }
-  }
-  int lo[3], hi[3];
-  window(g, x, y, z, lo, hi);
-  double mind = r.wall_search;
-  int triwall = 0, last = 0;
-  // Wall nodes that pass the two distance tests go to the lane's queue in
-  // LDS with their lower bound (rounded down to float), and their triangles
-  // are examined when any active lane's queue could overflow in the next
-  // row (and once at the end), every active lane draining its own queue in
-  // order.  The bound is tested again at the drain against the mindis of
-  // that moment, which is the mindis an in-order walk would hold there; a
-  // node queued under an older, larger mindis and examined anyway cannot
-  // lower it.  Either way the same triangle wins, strict < included.
+  // the reach in FP32, rounded down: the tests against it below skip only
+  // wall nodes beyond the reach
+  const float reach_f = (float)sqrt(r.wall_reach2) * (1.f - 1e-6f);
   __shared__ int wq[kWallQueue][128];
   __shared__ float wlb[kWallQueue][128];
-  int qlen = 0;
-  auto drain = [&]() {
-    for (int k = 0; k < qlen; k++) {
-      if ((double)wlb[k][threadIdx.x] >= mind) continue;
+  const int nlist = list[0];
+  for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nlist; li += gridDim.x * blockDim.x) {
+    const int gq = list[1 + li];
+    const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
+    int lo[3], hi[3];
+    window(g, x, y, z, lo, hi);
+    double mind = r.wall_search;
+    const float xf = (float)x, yf = (float)y, zf = (float)z;
+    // A coordinate rounded to FP32 is off by at most |a| 2^-24, so the FP32
+    // distance from x to a sphere centre within reach is off by less than
+    // 2^-22 (|x| + |y| + |z| + 3 reach); the margin is four times that
+    const float smargin =
+        (float)std::ldexp(fabs(x) + fabs(y) + fabs(z) + 4.0 * sqrt(r.wall_reach2), -20) + 1e-6f;
+    // order of the current winner: -1 while none (the first point must be
+    // strictly nearer than wall_search)
+    int best = -1, wbest = 0, last = 0, seq = 0;
+    // Wall nodes that pass the distance tests go to the lane's queue in LDS
+    // with their lower bound (rounded down to float); their triangles are
+    // examined when any active lane's queue could overflow in the next row,
+    // and once at the end.  The queued node of least bound goes first: its
+    // triangles usually bring mindis near its final value, so that most of
+    // the others are then skipped by their bound.  A node is skipped when
+    // its bound exceeds mindis, or equals it and the node comes later than
+    // the current winner (it can then neither be nearer nor tie earlier).
+    int qlen = 0;
+    auto examine = [&](int k) {
       const int wn = wq[k][threadIdx.x];
-      const int32_t *trow = &wnode_tri[13 * (size_t)(wn - 1)];
-      const int cnt = min(trow[0], 12);
+      // the bounding spheres of the node's triangles in one batch: a
+      // triangle whose sphere lies beyond mindis (by more than the FP32
+      // rounding of the test) holds no point that can win (unused slots
+      // hold a sphere at infinity)
+      const float4 *sph = &wn_sph[12 * (size_t)(wn - 1)];
+      float4 sp[12];
+#pragma unroll
+      for (int u = 0; u < 12; u++) sp[u] = sph[u];
+      const float lim = (float)mind + smargin;
+      unsigned live = 0;
+#pragma unroll
+      for (int u = 0; u < 12; u++) {
+        const float fx = xf - sp[u].x, fy = yf - sp[u].y, fz = zf - sp[u].z;
+        live |= sqrtf(fx * fx + fy * fy + fz * fz) - sp[u].w <= lim ? 1u << u : 0u;
+      }
+      const int order0 = (seq - qlen + k) * 64;  // visit number, then 4 l + point
 #pragma unroll 1
-      for (int l = 1; l <= cnt; l++) {
-        const int tw = trow[l];
-        // vertices and centroid of wall triangle tw, one contiguous record
-        const double *tgp = &wtgeo[12 * (size_t)(tw - 1)];
+      while (live) {
+        const int u = __builtin_ctz(live);
+        live &= live - 1;
+        // vertices and centroid of the node's triangle u, one record
+        const double *tgp = &wn_geo[12 * (12 * (size_t)(wn - 1) + u)];
         double gv[12];
 #pragma unroll
-        for (int u = 0; u < 12; u++) gv[u] = tgp[u];
+        for (int v = 0; v < 12; v++) gv[v] = tgp[v];
 #pragma unroll
         for (int sv = 0; sv < 4; sv++) {  // vertices 0..2, then the centroid
           const double wx = gv[3 * sv], wy = gv[3 * sv + 1], wz = gv[3 * sv + 2];
           const double d = sqrt((x - wx) * (x - wx) + (y - wy) * (y - wy) + (z - wz) * (z - wz));
-          if (d < mind) { mind = d; triwall = tw; }
+          const int o = order0 + 4 * (u + 1) + sv;
+          if (d < mind || (d == mind && best >= 0 && o < best)) {
+            mind = d;
+            best = o;
+            wbest = wn;
+          }
+        }
+      }
+    };
+    auto skip = [&](int k) {
+      const double b = (double)wlb[k][threadIdx.x];
+      return b > mind || (b == mind && (best < 0 || (seq - qlen + k) * 64 > best));
+    };
+    auto drain = [&]() {
+      int kb = 0;
+      for (int k = 1; k < qlen; k++) kb = wlb[k][threadIdx.x] < wlb[kb][threadIdx.x] ? k : kb;
+      // kb, then the others in queue order (one call site of examine)
+#pragma unroll 1
+      for (int t = 0; t < qlen; t++) {
+        const int k = t == 0 ? kb : t <= kb ? t - 1 : t;
+        if (!skip(k)) examine(k);
+      }
+      qlen = 0;
+    };
+    // rows (i, j) of the window in order, then one more pass that only
+    // drains (a single call site keeps the drain's code and registers once)
+    const int nj = hi[1] - lo[1] + 1, nrow = (hi[0] - lo[0] + 1) * nj;
+    for (int rw = 0; rw <= nrow; rw++) {
+      // room for this row's up to 9 wall nodes in every active lane's queue
+      if (rw == nrow || __ballot(qlen > kWallQueue - 9)) drain();
+      if (rw == nrow) break;
+      {
+        const int i = lo[0] + rw / nj, j = lo[1] + rw % nj;
+        const double qx = g.x0 + (i - 1) * g.dx - x, qy = g.y0 + (j - 1) * g.dx - y;
+        const double rij = qx * qx + qy * qy;
+        // the row's 9 W_index entries and their distances in one batch of
+        // loads (in-bounds addresses; entries past hi[2] are not used)
+        const size_t row = bq(g, i, j, 1);
+        int wv[9];
+        float wd[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const size_t at = row + (size_t)(min(lo[2] + t, hi[2]) - 1);
+          wv[t] = windex[at];
+          wd[t] = wdist[at];
+        }
+        // the wall nodes worth a closer look, and their positions and
+        // radii fetched in one batch
+        bool cand[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const int wn = wv[t];
+          // a wall node just examined cannot win again (it would come
+          // later with the same distances), and neighbouring lattice nodes
+          // mostly share their nearest wall node
+          cand[t] = lo[2] + t <= hi[2] && wn > 0 && wn != last;
+          last = cand[t] ? wn : last;
+          // wn lies wd of lattice node p = (i, j, lo[2] + t), so at least
+          // |x - p| - wd from x: beyond reach (tested below exactly) for
+          // certain if that exceeds the reach lowered by a margin
+          const double qz = g.z0 + (lo[2] + t - 1) * g.dx - z;
+          const float dp = sqrtf((float)(rij + qz * qz));
+          cand[t] = cand[t] && dp - wd[t] - 1e-4f < reach_f;
+        }
+        float4 wp[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+          if (cand[t]) wp[t] = wpos[wv[t] - 1];
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          if (!cand[t]) continue;
+          // a lower bound on the distance from x to wn (FP32 distance less
+          // its rounding margin)
+          const float ex = xf - wp[t].x, ey = yf - wp[t].y, ez = zf - wp[t].z;
+          const float e = sqrtf(ex * ex + ey * ey + ez * ez) - smargin;
+          // every vertex and the centroid of a triangle of wall node wn lie
+          // within wall_lmax of it: if wn is farther than wall_search +
+          // wall_lmax, none of them can beat mindis <= wall_search
+          if (e >= reach_f) continue;
+          // they lie within wrad (rounded up) of wn: none can beat the
+          // current mindis if wn is farther than mindis + wrad (branch and
+          // bound); one at the bound comes later than the winner and cannot
+          // win a tie
+          const float lbf = e - wp[t].w;
+          if ((double)lbf >= mind) continue;
+          wq[qlen][threadIdx.x] = wv[t];
+          wlb[qlen][threadIdx.x] = lbf;
+          qlen++;
+          seq++;
         }
       }
     }
-    qlen = 0;
-  };
-  for (int i = lo[0]; i <= hi[0]; i++)
-    for (int j = lo[1]; j <= hi[1]; j++) {
-      // room for this row's up to 9 wall nodes in every active lane's queue
-      if (__ballot(qlen > kWallQueue - 9)) drain();
-      // the row's 9 W_index entries in one batch of loads (in-bounds
-      // addresses; entries past hi[2] are not used)
-      const size_t row = bq(g, i, j, 1);
-      int wv[9];
-#pragma unroll
-      for (int t = 0; t < 9; t++) wv[t] = windex[row + (size_t)(min(lo[2] + t, hi[2]) - 1)];
-#pragma unroll 1
-      for (int t = 0; t < 9 && lo[2] + t <= hi[2]; t++) {
-        const int wn = wv[t];
-        // a wall node just examined cannot lower mindis again (strict <),
-        // and neighbouring lattice nodes mostly share their nearest wall node
-        if (wn <= 0 || wn == last) continue;
-        last = wn;
-        // every vertex and the centroid of a triangle of wall node wn lie
-        // within wall_lmax of it: if wn is farther than wall_search +
-        // wall_lmax, none of them can beat mindis <= wall_search
-        const double *wp = &wxyz[3 * (size_t)(wn - 1)];
-        const double ex = x - wp[0], ey = y - wp[1], ez = z - wp[2];
-        const double e2 = ex * ex + ey * ey + ez * ez;
-        if (e2 >= r.wall_reach2) continue;
-        // every vertex and centroid of wn's triangles lies within wrad of
-        // wn (with slack over rounding): none can beat the current mindis
-        // if wn is farther than mindis + wrad (branch and bound; the result
-        // is unchanged)
-        const double lb = sqrt(e2) - wrad[wn - 1];
-        if (lb >= mind) continue;
-        float lbf = (float)lb;
-        if ((double)lbf > lb) lbf = nextafterf(lbf, -INFINITY);  // never above lb
-        wq[qlen][threadIdx.x] = wn;
-        wlb[qlen][threadIdx.x] = lbf;
-        qlen++;
-      }
+    if (best < 0) continue;
+    // the winner's triangle: slot (best % 64) / 4 - 1 of wall node wbest
+    const int triwall = wnode_tri[13 * (size_t)(wbest - 1) + (best & 63) / 4];
+    nearest[gq] = triwall;
+    const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
+    const double *n = &wnormal[3 * (size_t)(triwall - 1)];
+    const double d = (x - w1[0]) * n[0] + (y - w1[1]) * n[1] + (z - w1[2]) * n[2];
+    if (d < r.wall_d0 && fabs(d) < r.wall_dmax) {
+      double f = (r.wall_d0 - d) / r.wall_dmax;
+      if (d < r.wall_d0 - r.wall_dmax) f = 1.0;
+      vrep[gq] += f * r.k_wall * n[0];
+      vrep[NT + gq] += f * r.k_wall * n[1];
+      vrep[2 * NT + gq] += f * r.k_wall * n[2];
     }
-  drain();
-  if (triwall == 0) return;
-  nearest[gq] = triwall;
-  const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
-  const double *n = &wnormal[3 * (size_t)(triwall - 1)];
-  const double d = (x - w1[0]) * n[0] + (y - w1[1]) * n[1] + (z - w1[2]) * n[2];
-  if (d < r.wall_d0 && fabs(d) < r.wall_dmax) {
-    double f = (r.wall_d0 - d) / r.wall_dmax;
-    if (d < r.wall_d0 - r.wall_dmax) f = 1.0;
-    vrep[gq] += f * r.k_wall * n[0];
-    vrep[NT + gq] += f * r.k_wall * n[1];
-    vrep[2 * NT + gq] += f * r.k_wall * n[2];
   }
 }
 
@@ -622,9 +874,10 @@ __global__ void k_wrap_wall(const double *__restrict__ X, size_t NT, int nn, int
 
 // Vertices and centroid of every wall triangle as one 12-double record (the
 // centroid summed as w/3 vertex by vertex, the rounding the repulsion search
-// compares against).
+// compares against), and its bounding sphere in FP32: the centroid rounded
+// to FP32 and a radius, rounded up, that reaches every vertex from it.
 __global__ void k_wall_geom(const double *__restrict__ wxyz, const int32_t *__restrict__ wtri,
-                            int nwt, double *__restrict__ wtgeo) {
+                            int nwt, double *__restrict__ wtgeo, float4 *__restrict__ wsph) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nwt) return;
   double *o = &wtgeo[12 * (size_t)m];
@@ -641,6 +894,54 @@ __global__ void k_wall_geom(const double *__restrict__ wxyz, const int32_t *__re
   o[9] = cx;
   o[10] = cy;
   o[11] = cz;
+  const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+  double rad = 0;
+  for (int s = 0; s < 3; s++) {
+    const double dx = o[3 * s] - fx, dy = o[3 * s + 1] - fy, dz = o[3 * s + 2] - fz;
+    rad = fmax(rad, sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  // the centroid point itself lies |c - fc| from the FP32 centre
+  const double dc = sqrt((cx - fx) * (cx - fx) + (cy - fy) * (cy - fy) + (cz - fz) * (cz - fz));
+  rad = fmax(rad, dc) * (1.0 + 1e-12) + 1e-12;
+  float rf = (float)rad;
+  if ((double)rf < rad) rf = nextafterf(rf, INFINITY);
+  wsph[m] = make_float4(fx, fy, fz, rf);
+}
+
+// Per wall node, its position in FP32 and, rounded up, the largest distance
+// from it to its triangles' vertices and centroids (wrad) plus the distance
+// from its FP32 position to its exact one.
+__global__ void k_wall_pos(const double *__restrict__ wxyz, const double *__restrict__ wrad, int nwn,
+                           float4 *__restrict__ wpos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwn) return;
+  const double x = wxyz[3 * (size_t)i], y = wxyz[3 * (size_t)i + 1], z = wxyz[3 * (size_t)i + 2];
+  const float fx = (float)x, fy = (float)y, fz = (float)z;
+  const double off = sqrt((x - fx) * (x - fx) + (y - fy) * (y - fy) + (z - fz) * (z - fz));
+  const double rad = wrad[i] + off;
+  float rf = (float)rad;
+  if ((double)rf < rad) rf = nextafterf(rf, INFINITY);
+  wpos[i] = make_float4(fx, fy, fz, rf);
+}
+
+// Per wall node, for each of its (up to 12) triangle slots in row order, the
+// triangle's bounding sphere and its 12-double record, so that the
+// wall-repulsion search reads them by (node, slot) with no load of the
+// triangle number first; unused slots hold a sphere at infinity.
+__global__ void k_wall_node_recs(const int32_t *__restrict__ wnode_tri, int nwn,
+                                 const double *__restrict__ wtgeo, const float4 *__restrict__ wsph,
+                                 float4 *__restrict__ wn_sph, double *__restrict__ wn_geo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 12 * nwn) return;
+  const int wn = i / 12, u = i % 12;
+  const int32_t *row = &wnode_tri[13 * (size_t)wn];
+  if (u >= min(row[0], 12)) {
+    wn_sph[i] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+    return;
+  }
+  const int tw = row[1 + u];
+  wn_sph[i] = wsph[tw - 1];
+  for (int v = 0; v < 12; v++) wn_geo[12 * (size_t)i + v] = wtgeo[12 * (size_t)(tw - 1) + v];
 }
 
 // W_index: every lattice node in a wall node's window keeps the nearest wall
@@ -801,7 +1102,8 @@ void free_void(zf_ctx *ctx) {
 void free_wall(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
                    (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist,
-                   (void **)&ctx->d_wtgeo, (void **)&ctx->d_wrad};
+                   (void **)&ctx->d_wtgeo, (void **)&ctx->d_wsph, (void **)&ctx->d_wn_sph,
+                   (void **)&ctx->d_wn_geo, (void **)&ctx->d_wpos, (void **)&ctx->d_wrad};
   for (auto b : bufs) dev_free(ctx, b);
   free_void(ctx);
   ctx->have_wall = false;
@@ -810,7 +1112,10 @@ void free_wall(zf_ctx *ctx) {
 }  // namespace
 
 void free_cells(zf_ctx *ctx) {
-  void **bufs[] = {(void **)&ctx->d_tkeys[0], (void **)&ctx->d_tkeys[1], (void **)&ctx->d_tsort_tmp,
+  void **bufs[] = {(void **)&ctx->d_tkeys[0], (void **)&ctx->d_tkeys[1], (void **)&ctx->d_crec,
+                   (void **)&ctx->d_bin_cnt, (void **)&ctx->d_bin_off, &ctx->d_bin_scan_tmp,
+                   (void **)&ctx->d_npair, (void **)&ctx->d_pairs, (void **)&ctx->d_over, (void **)&ctx->d_rep_lists,
+                   (void **)&ctx->d_tsort_tmp,
                    (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near,
                    (void **)&ctx->d_rep_bounds};
   for (auto b : bufs) dev_free(ctx, b);
@@ -818,12 +1123,25 @@ void free_cells(zf_ctx *ctx) {
   ctx->have_rep = false;
 }
 
+// CSR offsets of the cell list: the first key of every bin (empty bins
+// included) and the key count past the last bin.
+static int bin_offsets(zf_ctx *ctx) {
+  const int nb1 = ctx->dommax + 1;
+  hipLaunchKernelGGL(k_bin_counts, dim3(nb(nb1, 256)), dim3(256), 0, ctx->stream, ctx->d_bin_range,
+                     ctx->dommax, ctx->d_bin_cnt);
+  size_t tmp = ctx->bin_scan_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(ctx->d_bin_scan_tmp, tmp, ctx->d_bin_cnt, ctx->d_bin_off,
+                                               nb1, ctx->stream));
+  ZF_HIP(ctx, hipGetLastError());
+  return ZF_OK;
+}
+
 int cell_list_impl(zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells, TT = ncap * ctx->nt;
   const int n = ctx->ncell * ctx->nt;
   hipLaunchKernelGGL(k_fill, dim3(nb(2 * (size_t)ctx->dommax, 256)), dim3(256), 0, ctx->stream,
                      ctx->d_bin_range, 2 * (size_t)ctx->dommax, 0);
-  if (n == 0) return ZF_OK;
+  if (n == 0) return bin_offsets(ctx);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctx->d_rep_bounds,
                      (size_t)4, 0);
   const size_t NT = ncap * ctx->nn;
@@ -839,8 +1157,7 @@ int cell_list_impl(zf_ctx *ctx) {
                                                   ctx->d_tkeys[1], n, 0, 32 + bits, ctx->stream));
   hipLaunchKernelGGL(k_bin_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_tkeys[1], n,
                      (uint64_t)ctx->dommax, ctx->d_bin_range);
-  ZF_HIP(ctx, hipGetLastError());
-  return ZF_OK;
+  return bin_offsets(ctx);
 }
 
 namespace {
@@ -910,17 +1227,37 @@ int repulsion_impl(zf_ctx *ctx) {
   hipLaunchKernelGGL(k_fill, dim3(nb(3 * NT * 2, 256)), dim3(256), 0, ctx->stream,
                      (int32_t *)ctx->d_vrep, 3 * NT * 2, 0);
   hipLaunchKernelGGL(k_fill, dim3(nb(NT, 256)), dim3(256), 0, ctx->stream, ctx->d_near, NT, 0);
-  if (ctx->have_wall)
-    hipLaunchKernelGGL(k_wall_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
-                       ctx->nn, ctx->ncell, ctx->d_status, ctx->d_windex, ctx->d_wdist,
-                       ctx->d_wnode_tri,
-                       ctx->d_wtri, ctx->d_wxyz, ctx->d_wrad, ctx->d_wtgeo, ctx->d_wnormal, grid_of(ctx), r,
-                       ctx->d_vrep,
-                       ctx->d_near);
-  hipLaunchKernelGGL(k_cell_repulsion, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
+  if (ctx->have_wall) {
+    // the nodes near the wall, then their walk (the list reuses the
+    // overflow list of the cell-cell walk, cleared again below)
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_over, (size_t)1, 0);
+    hipLaunchKernelGGL(k_wall_select, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
+                       ctx->ncell, ctx->d_status, ctx->d_wdist, grid_of(ctx), r, ctx->d_over);
+    hipLaunchKernelGGL(k_wall_repulsion, dim3(std::min<unsigned>(nb(n, 128), 1024u)), dim3(128), 0,
+                       ctx->stream, ctx->d_x, NT, ctx->d_over, ctx->d_windex, ctx->d_wdist,
+                       ctx->d_wnode_tri, ctx->d_wtri, ctx->d_wxyz, ctx->d_wpos, ctx->d_wn_sph,
+                       ctx->d_wn_geo, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep, ctx->d_near);
+  }
+  const int nkeys = ctx->ncell * ctx->nt;
+  hipLaunchKernelGGL(k_cand_records, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_tkeys[1],
+                     nkeys, ctx->d_tgeo, TT, ctx->d_crec);
+  // walk (lists each node's passing triangles), evaluation of the lists,
+  // walk and evaluation again for the nodes whose list overflowed
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_over, (size_t)1, 0);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_rep_lists, (size_t)kRepListHead, 0);
+  hipLaunchKernelGGL(k_cell_repulsion<1>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
-                     ctx->d_tgeo, TT, ctx->d_domid, ctx->d_tkeys[1], ctx->d_bin_range,
-                     ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_vrep);
+                     ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1], ctx->d_bin_off,
+                     ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs, ctx->d_rep_lists,
+                     ctx->d_over, ctx->d_vrep);
+  hipLaunchKernelGGL(k_rep_eval, dim3(std::min<unsigned>(nb(n, 128), 2048u)), dim3(128), 0, ctx->stream, ctx->d_x,
+                     NT, ctx->nn, ctx->nt, ctx->d_tri, ctx->d_npair, ctx->d_pairs, ctx->d_rep_lists, r,
+                     ctx->d_vrep);
+  hipLaunchKernelGGL(k_cell_repulsion<0>, dim3(std::min<unsigned>(nb(n, 128), 256u)), dim3(128), 0, ctx->stream,
+                     ctx->d_x, NT, ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status,
+                     ctx->d_nnormal, ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1],
+                     ctx->d_bin_off, ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs,
+                     ctx->d_rep_lists, ctx->d_over, ctx->d_vrep);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }
@@ -952,9 +1289,16 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
   int r;
   if ((r = dev_alloc(ctx, (void **)&ctx->d_tkeys[0], 8 * TT)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_tkeys[1], 8 * TT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_crec, sizeof(float4) * TT)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_tbin, 4 * TT)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_bin_range, 8 * (size_t)std::max(ctx->dommax, 1))) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_bin_cnt, 4 * ((size_t)ctx->dommax + 1))) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_bin_off, 4 * ((size_t)ctx->dommax + 1))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_near, 4 * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_npair, 4 * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_pairs, 4 * (size_t)kRepPairs * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_over, 4 * (NT + 1))) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_rep_lists, 4 * (kRepListHead + kRepClasses * NT))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_rep_bounds, 2 * sizeof(unsigned long long))))
     return r;
   ctx->bins_gen = ~0ull;
@@ -965,6 +1309,12 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
                                                   (int)TT, 0, 64, ctx->stream));
   ctx->tsort_tmp_bytes = tmp;
   if ((r = dev_alloc(ctx, &ctx->d_tsort_tmp, tmp))) return r;
+  size_t stmp = 0;
+  ZF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, stmp, ctx->d_bin_cnt, ctx->d_bin_off,
+                                               ctx->dommax + 1, ctx->stream));
+  ctx->bin_scan_tmp_bytes = stmp;
+  if ((r = dev_alloc(ctx, &ctx->d_bin_scan_tmp, std::max<size_t>(stmp, 1)))) return r;
+  if ((r = bin_offsets(ctx))) return r;
   ctx->have_rep = true;
   return ZF_OK;
 }
@@ -1012,6 +1362,10 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnormal, 24 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wtri, 12 * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wtgeo, 96 * (size_t)nwt)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wsph, sizeof(float4) * (size_t)nwt)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wn_sph, sizeof(float4) * 12 * (size_t)nwn)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wpos, sizeof(float4) * (size_t)nwn)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wn_geo, 8 * 144 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wrad, 8 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnode_tri, 52 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_windex, 4 * ctx->box)) ||
@@ -1025,7 +1379,11 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   for (double &rv : wrad) rv = rv * (1.0 + 1e-12) + 1e-12;
   ZF_HIP(ctx, hipMemcpy(ctx->d_wrad, wrad.data(), 8 * (size_t)nwn, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_wall_geom, dim3(nb(nwt, 256)), dim3(256), 0, ctx->stream, ctx->d_wxyz,
-                     ctx->d_wtri, nwt, ctx->d_wtgeo);
+                     ctx->d_wtri, nwt, ctx->d_wtgeo, ctx->d_wsph);
+  hipLaunchKernelGGL(k_wall_pos, dim3(nb(nwn, 256)), dim3(256), 0, ctx->stream, ctx->d_wxyz, ctx->d_wrad,
+                     nwn, ctx->d_wpos);
+  hipLaunchKernelGGL(k_wall_node_recs, dim3(nb(12 * (size_t)nwn, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_wnode_tri, nwn, ctx->d_wtgeo, ctx->d_wsph, ctx->d_wn_sph, ctx->d_wn_geo);
   hipLaunchKernelGGL(k_fill_u64, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key, ctx->box);
   hipLaunchKernelGGL(k_windex, dim3(nb(nwn, 64)), dim3(64), 0, ctx->stream, ctx->d_wxyz, nwn,
                      grid_of(ctx), key);

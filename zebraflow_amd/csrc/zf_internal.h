@@ -182,11 +182,20 @@ struct zf_ctx {
   bool have_rep = false, have_wall = false;
   zf_repulsion_config rcfg{};
   uint64_t *d_tkeys[2] = {nullptr, nullptr};  // (bin << 32 | triangle) sort keys
+  float4 *d_crec = nullptr;  // per sorted key: FP32 centroid and the packed normal
   void *d_tsort_tmp = nullptr;
   size_t tsort_tmp_bytes = 0;
   int32_t *d_tbin = nullptr;      // ncap*nt: 1-based DOMID of the triangle's bin or 0
   int32_t *d_bin_range = nullptr; // 2 x dommax: [begin, end) in the sorted keys
+  int32_t *d_bin_cnt = nullptr;   // dommax + 1: keys per bin (scan input)
+  int32_t *d_bin_off = nullptr;   // dommax + 1: first key of each bin, then the key count
+  void *d_bin_scan_tmp = nullptr;
+  size_t bin_scan_tmp_bytes = 0;
   int32_t *d_near = nullptr;      // ncap*nn: chosen wall triangle (1-based) or 0
+  int32_t *d_npair = nullptr;     // ncap*nn: passing triangles of each node (-1: not a live node)
+  int32_t *d_pairs = nullptr;     // ncap*nn x 48: the first of them, in walk order
+  int32_t *d_over = nullptr;      // 1 + ncap*nn: count, then the nodes whose list overflowed
+  int32_t *d_rep_lists = nullptr; // nodes with listed triangles, by count class
   // bounds recorded with the cell list: largest centroid-to-bin and
   // centroid-to-vertex distance (FP64 bits); valid while the triangle
   // geometry is the one the list was built from (geom_gen == bins_gen)
@@ -197,6 +206,10 @@ struct zf_ctx {
   double *d_wxyz = nullptr, *d_wnormal = nullptr;  // 3 x nwn, 3 x nwt (AoS)
   int32_t *d_wtri = nullptr, *d_wnode_tri = nullptr;  // 3 x nwt (1-based), 13 x nwn
   double *d_wtgeo = nullptr;  // 12 x nwt: vertices and centroid of each wall triangle
+  float4 *d_wsph = nullptr;   // nwt: FP32 bounding sphere of each wall triangle
+  float4 *d_wn_sph = nullptr; // 12 x nwn: the spheres of each wall node's triangle slots
+  float4 *d_wpos = nullptr;   // nwn: FP32 position and wrad rounded up
+  double *d_wn_geo = nullptr; // 144 x nwn: the 12-double records of the same slots
   double *d_wrad = nullptr;   // nwn: largest distance from a wall node to its triangles' vertices/centroids
   int32_t *d_windex = nullptr;    // box
   float *d_wdist = nullptr;       // box: FP32 distance to that wall node (or +inf)
