@@ -135,6 +135,19 @@ __device__ __forceinline__ int tile_of(double pos, double origin, double dx, int
   return t < 0 ? 0 : t >= nt ? nt - 1 : t;
 }
 
+// Spreading bin of a position: its tile times 8 plus its octant in the
+// tile, bit 2 (x), 1 (y), 0 (z) set in the upper half along that axis
+// (tile-relative position >= 2 lattice units).  A node within 2 lattice
+// units of a neighbouring tile, the reach of the kernel, lies in the half
+// facing it.
+__device__ __forceinline__ uint64_t spread_bin(double x, double y, double z, const IbmParams &p) {
+  const int tx = tile_of(x, p.x0, p.dx, p.ntx), ty = tile_of(y, p.y0, p.dx, p.nty),
+            tz = tile_of(z, p.z0, p.dx, p.ntz);
+  const int hx = (x - p.x0) / p.dx - kTileW * tx >= 2.0, hy = (y - p.y0) / p.dx - kTileW * ty >= 2.0,
+            hz = (z - p.z0) / p.dx - kTileW * tz >= 2.0;
+  return ((((uint64_t)tx * p.nty + ty) * p.ntz + tz) << 3) | (uint64_t)(hx * 4 + hy * 2 + hz);
+}
+
 // A clone node within its band (band[2c] the plane, band[2c+1] the
 // direction of travel) moves with its parent cell and does not spread
 // (ZebraflowWT.c:11002-11015).
@@ -145,8 +158,8 @@ __device__ __forceinline__ bool clone_band_node(const double *band, int c, int s
 }
 
 // v += v_rep; F_fluid = gamma_p v_rep + F_Total (ZebraflowWT.c:10749-10754);
-// sort key = tile << 32 | node (dead cells and clone band nodes sort past
-// the last tile).  id_base: added to the node index in the key (the
+// sort key = spreading bin << 32 | node (dead cells and clone band nodes
+// sort past the last bin).  id_base: added to the node index in the key (the
 // nucleus nodes follow the membrane's, from id_base = cap * nn).
 template <class U>
 __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V,
@@ -156,11 +169,11 @@ __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V
                             uint32_t id_base, const double *__restrict__ band) {
   const int g = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
-  const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
+  const uint64_t nbins = (uint64_t)p.ntx * p.nty * p.ntz * 8;
   const uint64_t id = (uint64_t)id_base + (uint64_t)g;
   const int st = status[g / nn];
   if (st <= kStatusDead || clone_band_node(band, g / nn, st, X[2 * NT + g])) {
-    keys[g] = (ntiles << 32) | id;
+    keys[g] = (nbins << 32) | id;
     return;
   }
   double fl[3];
@@ -172,10 +185,7 @@ __global__ void k_node_prep(const double *__restrict__ X, double *__restrict__ V
   ffl[g] = (U)fl[0];
   ffl[NT + g] = (U)fl[1];
   ffl[2 * NT + g] = (U)fl[2];
-  const int tx = tile_of(X[g], p.x0, p.dx, p.ntx), ty = tile_of(X[NT + g], p.y0, p.dx, p.nty),
-            tz = tile_of(X[2 * NT + g], p.z0, p.dx, p.ntz);
-  const uint64_t tile = ((uint64_t)tx * p.nty + ty) * p.ntz + tz;
-  keys[g] = (tile << 32) | id;
+  keys[g] = (spread_bin(X[g], X[NT + g], X[2 * NT + g], p) << 32) | id;
 }
 
 __global__ void k_fill_i32(int32_t *a, size_t n, int32_t v) {
@@ -183,22 +193,76 @@ __global__ void k_fill_i32(int32_t *a, size_t n, int32_t v) {
   if (i < n) a[i] = v;
 }
 
-__global__ void k_tile_ranges(const uint64_t *__restrict__ keys, int n, uint64_t ntiles,
-                              int32_t *__restrict__ range) {
+// [begin, end) of every spreading bin in the sorted keys (cleared to empty
+// beforehand), and occ[tile] = 1 for every tile holding a node.
+__global__ void k_bin_ranges(const uint64_t *__restrict__ keys, int n, uint64_t nbins,
+                             int32_t *__restrict__ range, int32_t *__restrict__ occ) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint64_t t = keys[i] >> 32;
-  if (t >= ntiles) return;
-  if (i == 0 || (keys[i - 1] >> 32) != t) range[2 * t] = i;
+  if (t >= nbins) return;
+  if (i == 0 || (keys[i - 1] >> 32) != t) {
+    range[2 * t] = i;
+    occ[t >> 3] = 1;
+  }
   if (i == n - 1 || (keys[i + 1] >> 32) != t) range[2 * t + 1] = i + 1;
 }
+
+// The tiles with a node in one of their 27 neighbours (themselves
+// included), appended to list (count in list[0], tiles from list[1] on, in
+// no particular order): every other tile's force is zero.
+__global__ void k_spread_tile_list(const int32_t *__restrict__ occ, int ntx, int nty, int ntz,
+                                   int32_t *__restrict__ list) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntx * nty * ntz) return;
+  const int tz = t % ntz, ty = (t / ntz) % nty, tx = t / (ntz * nty);
+  bool any = false;
+  for (int a = max(tx - 1, 0); a <= min(tx + 1, ntx - 1); a++)
+    for (int c = max(ty - 1, 0); c <= min(ty + 1, nty - 1); c++)
+      for (int e = max(tz - 1, 0); e <= min(tz + 1, ntz - 1); e++) any |= occ[(a * nty + c) * ntz + e] != 0;
+  if (any) list[1 + atomicAdd(list, 1)] = t;
+}
+
+// The bins a tile gathers from, in gathering order: the 27 neighbouring
+// tiles in (x, y, z) order, and in each the octants in the halves facing
+// the tile (both halves along an axis where the neighbour is level with
+// it), ascending.  8 + 6 x 4 + 12 x 2 + 8 x 1 = 64 entries, one per lane.
+struct SpreadSub {
+  signed char dx, dy, dz, oct;
+};
+struct SpreadSubs {
+  SpreadSub e[64];
+};
+constexpr bool faces(int d, int h) { return d == 0 || (d < 0 ? h == 1 : h == 0); }
+constexpr SpreadSubs make_spread_subs() {
+  SpreadSubs t{};
+  int n = 0;
+  for (int dx = -1; dx <= 1; dx++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dz = -1; dz <= 1; dz++)
+        for (int o = 0; o < 8; o++)
+          if (faces(dx, o >> 2) && faces(dy, (o >> 1) & 1) && faces(dz, o & 1))
+            t.e[n++] = SpreadSub{(signed char)dx, (signed char)dy, (signed char)dz, (signed char)o};
+  return t;
+}
+constexpr int count_spread_subs() {
+  int n = 0;
+  for (int o = 0; o < 27 * 8; o++)
+    n += faces(o / 72 - 1, (o % 8) >> 2) && faces((o / 24) % 3 - 1, ((o % 8) >> 1) & 1) &&
+         faces((o / 8) % 3 - 1, o & 1);
+  return n;
+}
+static_assert(count_spread_subs() == 64, "one bin per lane");
+__device__ constexpr SpreadSubs kSpreadSubs = make_spread_subs();
 
 __device__ __forceinline__ float kcos_pi(float x) { return cospif(x); }
 __device__ __forceinline__ double kcos_pi(double x) { return cospi(x); }
 
-// One wave per tile; lane = lattice node (4x4x4).  Gathers every membrane
-// node of the 27 neighbouring tiles in (tile, node) order, in the field's
-// type F (positions relative to the tile origin, weights and sums).  The
+// One wave per listed tile; lane = lattice node (4x4x4).  Gathers the
+// membrane nodes of the 64 bins of kSpreadSubs in (bin, node) order, in the
+// field's type F (positions relative to the tile origin, weights and
+// sums): the bins' runs of the sorted keys are taken as one list, staged 64
+// nodes at a time.  The
 // kernel weight is a product of one factor per axis, and a lane's offset
 // from a membrane node along an axis takes one of 4 values (its row in the
 // tile), so the lane staging a node evaluates its 12 factors once and the
@@ -217,103 +281,127 @@ struct WrapSpread {
 template <class F, bool WRAP = false>
 __global__ __launch_bounds__(64) void k_spread_gather(
     const uint64_t *__restrict__ keys, const int32_t *__restrict__ range,
-    const double *__restrict__ X, const F *__restrict__ ffl, size_t NT,
-    const double *__restrict__ Xn, const F *__restrict__ ffln, size_t NTn,
+    const int32_t *__restrict__ tlist, const double *__restrict__ X, const F *__restrict__ ffl,
+    size_t NT, const double *__restrict__ Xn, const F *__restrict__ ffln, size_t NTn,
     const int32_t *__restrict__ b, const int32_t *__restrict__ sbox, IbmParams p,
     F *__restrict__ force, size_t stride, WrapSpread ws = WrapSpread{0.0, 0}) {
   __shared__ F sx[64], sy[64], sz[64], sf0[64], sf1[64], sf2[64];
   __shared__ int scode[WRAP ? 64 : 1];
   __shared__ F kx[4][64], ky[4][64], kz[4][64];  // (1 + cos(pi |r - s| / 2)) for tile row r
-  const int tile = blockIdx.x;
-  const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
+  __shared__ int soff[64], sbeg[64];             // each bin's offset in the list, first key
   const int lane = threadIdx.x;
-  // lattice node of this lane (0-based), and its coordinate relative to the
-  // tile origin in lattice units
-  const int li = tx * kTileW + lane / 16, lj = ty * kTileW + (lane / 4) % 4, lk = tz * kTileW + lane % 4;
-  const double ox = p.x0 + (double)(tx * kTileW) * p.dx, oy = p.y0 + (double)(ty * kTileW) * p.dx,
-               oz = p.z0 + (double)(tz * kTileW) * p.dx;
   const int ix = lane / 16, iy = (lane / 4) % 4, iz = lane % 4;
   const F rx = (F)ix, ry = (F)iy, rz = (F)iz;
-  F a0 = 0, a1 = 0, a2 = 0;
-  bool any = false;
-  for (int dxT = -1; dxT <= 1; dxT++)
-    for (int dyT = -1; dyT <= 1; dyT++)
-      for (int dzT = -1; dzT <= 1; dzT++) {
-        const int nx = tx + dxT, ny = ty + dyT, nz = tz + dzT;
-        if (nx < 0 || ny < 0 || nz < 0 || nx >= p.ntx || ny >= p.nty || nz >= p.ntz) continue;
-        const int nt = (nx * p.nty + ny) * p.ntz + nz;
-        const int beg = range[2 * nt], end = range[2 * nt + 1];
-        for (int c0 = beg; c0 < end; c0 += 64) {
-          // staged nodes are compacted in order to those within 2 of this
-          // tile along every axis: any other one is off the support of all
-          // 64 lanes, so the sums are unchanged
-          bool rel = false;
-          F px = 0, py = 0, pz = 0;
-          size_t g = 0;
-          const double *Xs = X;
-          const F *fs = ffl;
-          size_t st = NT;
-          int code = 0;
-          if (lane < end - c0) {
-            g = (size_t)(keys[c0 + lane] & 0xffffffffu);
-            if (WRAP) {
-              code = (int)(g >> 30);
-              g &= 0x3fffffffu;
-            }
-            if (g >= NT) {
-              g -= NT;
-              Xs = Xn;
-              fs = ffln;
-              st = NTn;
-            }
-            double zz = Xs[2 * st + g];
-            if (WRAP) zz += code == 1 ? -ws.L : code == 2 ? ws.L : 0.0;
-            px = (F)((Xs[g] - ox) / p.dx);
-            py = (F)((Xs[st + g] - oy) / p.dx);
-            pz = (F)((zz - oz) / p.dx);
-            rel = px >= (F)-2 && px <= (F)5 && py >= (F)-2 && py <= (F)5 && pz >= (F)-2 && pz <= (F)5;
-          }
-          const unsigned long long relm = __ballot(rel);
-          const int cnt = __popcll(relm);
-          if (rel) {
-            const int sl = __popcll(relm & ((1ull << lane) - 1ull));
-            sx[sl] = px;
-            sy[sl] = py;
-            sz[sl] = pz;
-            sf0[sl] = fs[g];
-            sf1[sl] = fs[st + g];
-            sf2[sl] = fs[2 * st + g];
-            if (WRAP) scode[sl] = code;
+  const int ntl = tlist[0];
+  for (int it = blockIdx.x; it < ntl; it += gridDim.x) {
+    const int tile = tlist[1 + it];
+    const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
+    // lattice node of this lane (0-based), and the tile origin
+    const int li = tx * kTileW + ix, lj = ty * kTileW + iy, lk = tz * kTileW + iz;
+    const double ox = p.x0 + (double)(tx * kTileW) * p.dx, oy = p.y0 + (double)(ty * kTileW) * p.dx,
+                 oz = p.z0 + (double)(tz * kTileW) * p.dx;
+    // this lane's bin, its run of keys, and the run's offset in the list
+    const SpreadSub se = kSpreadSubs.e[lane];
+    const int nx = tx + se.dx, ny = ty + se.dy, nz = tz + se.dz;
+    int beg = 0, len = 0;
+    if (nx >= 0 && ny >= 0 && nz >= 0 && nx < p.ntx && ny < p.nty && nz < p.ntz) {
+      const size_t bin = (((size_t)nx * p.nty + ny) * p.ntz + nz) * 8 + se.oct;
+      beg = range[2 * bin];
+      len = range[2 * bin + 1] - beg;
+    }
+    int incl = len;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-              // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
-              kx[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
-              ky[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
-              kz[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
-            }
-          }
-          __syncthreads();
-          any = true;
-          for (int m = 0; m < cnt; m++) {
-            const F dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
-            if (off_support(dx, dy, dz, p.cube)) continue;
-            if (WRAP && ((scode[m] == 1 && lk == 0) || (scode[m] == 2 && lk == ws.km - 1))) continue;
-            const F w = kx[ix][m] * ky[iy][m] * kz[iz][m] * ((F)1 / (F)64);
-            a0 += w * sf0[m];
-            a1 += w * sf1[m];
-            a2 += w * sf2[m];
-          }
-          __syncthreads();
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d);
+      incl += lane >= d ? v : 0;
+    }
+    const int total = __shfl(incl, 63);
+    soff[lane] = incl - len;
+    sbeg[lane] = beg;
+    __syncthreads();
+    F a0 = 0, a1 = 0, a2 = 0;
+    for (int c0 = 0; c0 < total; c0 += 64) {
+      // staged nodes are compacted in order to those within 2 of this
+      // tile along every axis: any other one is off the support of all 64
+      // lanes, so the sums are unchanged
+      bool rel = false;
+      F px = 0, py = 0, pz = 0;
+      size_t g = 0;
+      const double *Xs = X;
+      const F *fs = ffl;
+      size_t st = NT;
+      int code = 0;
+      const int v = c0 + lane;
+      if (v < total) {
+        // the bin holding list entry v: the last with offset <= v (an
+        // empty bin shares its offset with the next one)
+        int lo = 0, hi = 64;
+#pragma unroll
+        for (int step = 0; step < 6; step++) {
+          const int mid = (lo + hi) >> 1;
+          if (soff[mid] <= v) lo = mid;
+          else hi = mid;
+        }
+        g = (size_t)(keys[sbeg[lo] + (v - soff[lo])] & 0xffffffffu);
+        if (WRAP) {
+          code = (int)(g >> 30);
+          g &= 0x3fffffffu;
+        }
+        if (g >= NT) {
+          g -= NT;
+          Xs = Xn;
+          fs = ffln;
+          st = NTn;
+        }
+        double zz = Xs[2 * st + g];
+        if (WRAP) zz += code == 1 ? -ws.L : code == 2 ? ws.L : 0.0;
+        px = (F)((Xs[g] - ox) / p.dx);
+        py = (F)((Xs[st + g] - oy) / p.dx);
+        pz = (F)((zz - oz) / p.dx);
+        rel = px >= (F)-2 && px <= (F)5 && py >= (F)-2 && py <= (F)5 && pz >= (F)-2 && pz <= (F)5;
+      }
+      const unsigned long long relm = __ballot(rel);
+      const int cnt = __popcll(relm);
+      if (rel) {
+        const int sl = __popcll(relm & ((1ull << lane) - 1ull));
+        sx[sl] = px;
+        sy[sl] = py;
+        sz[sl] = pz;
+        sf0[sl] = fs[g];
+        sf1[sl] = fs[st + g];
+        sf2[sl] = fs[2 * st + g];
+        if (WRAP) scode[sl] = code;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          // (1 + cos(pi |d| / 2)) per axis, d in lattice units (width 2 DX)
+          kx[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - px));
+          ky[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - py));
+          kz[r][sl] = (F)1 + kcos_pi((F)0.5 * fabs((F)r - pz));
         }
       }
-  if (!any || li >= p.im || lj >= p.jm || lk >= p.km) return;
-  const size_t q = ((size_t)li * p.jm + lj) * p.km + lk;
-  if (b[q] < 0) return;
-  const int s = sbox[q];
-  const F fs = (F)p.force_scale;
-  force[s] = a0 * fs;
-  force[stride + s] = a1 * fs;
-  force[2 * stride + s] = a2 * fs;
+      __syncthreads();
+      for (int m = 0; m < cnt; m++) {
+        const F dx = rx - sx[m], dy = ry - sy[m], dz = rz - sz[m];
+        if (off_support(dx, dy, dz, p.cube)) continue;
+        if (WRAP && ((scode[m] == 1 && lk == 0) || (scode[m] == 2 && lk == ws.km - 1))) continue;
+        const F w = kx[ix][m] * ky[iy][m] * kz[iz][m] * ((F)1 / (F)64);
+        a0 += w * sf0[m];
+        a1 += w * sf1[m];
+        a2 += w * sf2[m];
+      }
+      __syncthreads();
+    }
+    if (total > 0 && li < p.im && lj < p.jm && lk < p.km) {
+      const size_t q = ((size_t)li * p.jm + lj) * p.km + lk;
+      if (b[q] >= 0) {
+        const int s = sbox[q];
+        const F fsc = (F)p.force_scale;
+        force[s] = a0 * fsc;
+        force[stride + s] = a1 * fsc;
+        force[2 * stride + s] = a2 * fsc;
+      }
+    }
+  }
 }
 
 // ---- a parent lattice's cells: the periodic window along k (zf_wrap.hip)
@@ -399,11 +487,11 @@ __global__ void k_wrap_node_prep(const double *__restrict__ X, double *__restric
                                  uint32_t id_base) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ncell * nn) return;
-  const uint64_t ntiles = (uint64_t)p.ntx * p.nty * p.ntz;
+  const uint64_t nbins = (uint64_t)p.ntx * p.nty * p.ntz * 8;
   const uint64_t id = (uint64_t)id_base + (uint64_t)g;
-  keys_img[g] = (ntiles << 32) | id;
+  keys_img[g] = (nbins << 32) | id;
   if (status[g / nn] < 2) {
-    keys[g] = (ntiles << 32) | id;
+    keys[g] = (nbins << 32) | id;
     return;
   }
   double fl[3];
@@ -416,15 +504,12 @@ __global__ void k_wrap_node_prep(const double *__restrict__ X, double *__restric
   ffl[NT + g] = (U)fl[1];
   ffl[2 * NT + g] = (U)fl[2];
   const double x = X[g], y = X[NT + g], z = X[2 * NT + g];
-  const int tx = tile_of(x, p.x0, p.dx, p.ntx), ty = tile_of(y, p.y0, p.dx, p.nty);
-  keys[g] = ((((uint64_t)tx * p.nty + ty) * p.ntz + tile_of(z, p.z0, p.dx, p.ntz)) << 32) | id;
+  keys[g] = (spread_bin(x, y, z, p) << 32) | id;
   int code = 0;
   double zi = z;
   if (z > w.zm - 3.0 * p.dx) { code = 1; zi = z - w.L; }
   else if (z < w.z0 + 3.0 * p.dx) { code = 2; zi = z + w.L; }
-  if (code)
-    keys_img[g] = ((((uint64_t)tx * p.nty + ty) * p.ntz + tile_of(zi, p.z0, p.dx, p.ntz)) << 32) |
-                  ((uint64_t)code << 30) | id;
+  if (code) keys_img[g] = (spread_bin(x, y, zi, p) << 32) | ((uint64_t)code << 30) | id;
 }
 
 __global__ void k_speed(const double *__restrict__ V, size_t NT, int nn, int ncell,
@@ -504,7 +589,8 @@ void free_ibm(zf_ctx *ctx) {
   free_wrap(ctx);
   void **bufs[] = {(void **)&ctx->d_umac, (void **)&ctx->d_vrep,
                    (void **)&ctx->d_ffl,   (void **)&ctx->d_keys[0], (void **)&ctx->d_keys[1],
-                   (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag};
+                   (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag,
+                   (void **)&ctx->d_sbin_range, (void **)&ctx->d_tile_occ, (void **)&ctx->d_spread_tiles};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->keys_cap = 0;
   ctx->have_ibm = false;
@@ -576,6 +662,32 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
   return ZF_OK;
 }
 
+// Sorts the nkeys spreading keys (bin << 32 | node) and builds the bins'
+// runs, the occupied tiles and the list of tiles to gather.
+int spread_bins(zf_ctx *ctx, int nkeys, uint64_t ntiles) {
+  const uint64_t nbins = 8 * ntiles;
+  // end bit: bins need ceil(log2(nbins+1)) bits above the 32 node bits
+  int bbits = 1;
+  while ((1ull << bbits) <= nbins) bbits++;
+  size_t tmp = ctx->sort_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
+                                                  nkeys, 0, 32 + bbits, ctx->stream));
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * nbins, 256)), dim3(256), 0, ctx->stream, ctx->d_sbin_range,
+                     (size_t)(2 * nbins), 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb(ntiles, 256)), dim3(256), 0, ctx->stream, ctx->d_tile_occ,
+                     (size_t)ntiles, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_spread_tiles, (size_t)1, 0);
+  if (nkeys > 0)
+    hipLaunchKernelGGL(k_bin_ranges, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1],
+                       nkeys, nbins, ctx->d_sbin_range, ctx->d_tile_occ);
+  hipLaunchKernelGGL(k_spread_tile_list, dim3(nb(ntiles, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_tile_occ, ctx->ntx, ctx->nty, ctx->ntz, ctx->d_spread_tiles);
+  return ZF_OK;
+}
+
+// Workgroups of the gather (one wave each, over the listed tiles).
+unsigned spread_grid(uint64_t ntiles) { return (unsigned)std::min<uint64_t>(ntiles, 16384); }
+
 int ibm_spread_impl(zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
   const int n = ctx->ncell * ctx->nn;
@@ -614,25 +726,17 @@ int ibm_spread_impl(zf_ctx *ctx) {
                          (uint32_t)NT, ctx->d_band);
   }
   const int nkeys = n + nnuc_pts;
-  // end bit: tile ids need ceil(log2(ntiles+1)) bits above the 32 node bits
-  int tbits = 1;
-  while ((1ull << tbits) <= ntiles) tbits++;
-  size_t tmp = ctx->sort_tmp_bytes;
-  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
-                                                  nkeys, 0, 32 + tbits, ctx->stream));
-  hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * ntiles, 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_tile_range, (size_t)(2 * ntiles), 0);
-  hipLaunchKernelGGL(k_tile_ranges, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1],
-                     nkeys, ntiles, ctx->d_tile_range);
+  if ((rc = spread_bins(ctx, nkeys, ntiles))) return rc;
+  const unsigned grid = spread_grid(ntiles);
   if (f64)
-    hipLaunchKernelGGL(k_spread_gather<double>, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
-                       ctx->d_keys[1], ctx->d_tile_range, ctx->d_x,
+    hipLaunchKernelGGL(k_spread_gather<double>, dim3(grid), dim3(64), 0, ctx->stream, ctx->d_keys[1],
+                       ctx->d_sbin_range, ctx->d_spread_tiles, ctx->d_x,
                        static_cast<const double *>(ctx->d_ffl), NT, ctx->d_xn,
                        static_cast<const double *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox, p,
                        static_cast<double *>(ctx->d_force), ctx->stride);
   else
-    hipLaunchKernelGGL(k_spread_gather<float>, dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
-                       ctx->d_keys[1], ctx->d_tile_range, ctx->d_x,
+    hipLaunchKernelGGL(k_spread_gather<float>, dim3(grid), dim3(64), 0, ctx->stream, ctx->d_keys[1],
+                       ctx->d_sbin_range, ctx->d_spread_tiles, ctx->d_x,
                        static_cast<const float *>(ctx->d_ffl), NT, ctx->d_xn,
                        static_cast<const float *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox, p,
                        static_cast<float *>(ctx->d_force), ctx->stride);
@@ -707,19 +811,13 @@ static int wrap_spread_typed(zf_ctx *ctx) {
                        ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell, ctx->d_status, p,
                        w, static_cast<F *>(ctx->d_ffln), k0 + n, k0 + half + n, (uint32_t)NT);
   const int nkeys = 2 * half;
-  int tbits = 1;
-  while ((1ull << tbits) <= ntiles) tbits++;
-  size_t tmp = ctx->sort_tmp_bytes;
-  ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
-                                                  nkeys, 0, 32 + tbits, ctx->stream));
-  hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * ntiles, 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_tile_range, (size_t)(2 * ntiles), 0);
-  hipLaunchKernelGGL(k_tile_ranges, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1],
-                     nkeys, ntiles, ctx->d_tile_range);
-  hipLaunchKernelGGL((k_spread_gather<F, true>), dim3((unsigned)ntiles), dim3(64), 0, ctx->stream,
-                     ctx->d_keys[1], ctx->d_tile_range, ctx->d_x, static_cast<const F *>(ctx->d_ffl),
-                     NT, ctx->d_xn, static_cast<const F *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox,
-                     p, static_cast<F *>(ctx->d_force), ctx->stride, WrapSpread{w.L, ctx->cfg.km});
+  int rc = spread_bins(ctx, nkeys, ntiles);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_spread_gather<F, true>), dim3(spread_grid(ntiles)), dim3(64), 0, ctx->stream,
+                     ctx->d_keys[1], ctx->d_sbin_range, ctx->d_spread_tiles, ctx->d_x,
+                     static_cast<const F *>(ctx->d_ffl), NT, ctx->d_xn,
+                     static_cast<const F *>(ctx->d_ffln), NTn, ctx->d_b, ctx->d_sbox, p,
+                     static_cast<F *>(ctx->d_force), ctx->stride, WrapSpread{w.L, ctx->cfg.km});
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }
@@ -828,6 +926,9 @@ int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
       (rc = dev_alloc(ctx, (void **)&ctx->d_vrep, 24 * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_ffl, 3 * field_elem(ctx) * NT)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_tile_range, 8 * ntiles)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_sbin_range, 64 * ntiles)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_tile_occ, 4 * ntiles)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_spread_tiles, 4 * (ntiles + 1))) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_flag, 4 * (ncap + 1))))
     return rc;
   // sort keys for the membrane nodes and, with a nucleus, its nodes after them

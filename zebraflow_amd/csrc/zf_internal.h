@@ -175,6 +175,9 @@ struct zf_ctx {
   void *d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   int32_t *d_tile_range = nullptr;  // 2 x ntiles: [begin, end) in the sorted keys
+  int32_t *d_sbin_range = nullptr;  // 2 x 8 ntiles: the same per spreading bin (tile octant)
+  int32_t *d_tile_occ = nullptr;    // ntiles: 1 where a tile holds a spreading node
+  int32_t *d_spread_tiles = nullptr;  // 1 + ntiles: count, then the tiles the gather visits
   int32_t *d_flag = nullptr;        // ncap: speed check
   int ntx = 0, nty = 0, ntz = 0;
 
