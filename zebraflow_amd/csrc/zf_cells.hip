@@ -90,18 +90,44 @@ __global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double
           if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
         }
     if (!inside || !(mind <= cmin)) {
-      int lo[3], hi[3];
-      window(g, x, y, z, lo, hi);
+      // Every lattice point outside the 4^3 block ci-1..ci+2 (per axis) is
+      // at least 2 dx from the centroid, so a listed block point nearer
+      // than that (by a margin far above rounding) is the window's nearest,
+      // found first in the same (i,j,k) order; the block's b words are read
+      // a plane at a time (in-bounds addresses, out-of-box points skipped).
       mind = 5000.0;
-      for (int i = lo[0]; i <= hi[0]; i++)
-        for (int j = lo[1]; j <= hi[1]; j++)
-          for (int k = lo[2]; k <= hi[2]; k++) {
-            if (b[bq(g, i, j, k)] < 0) continue;
-            const double dx = g.x0 + (i - 1) * g.dx - x, dy = g.y0 + (j - 1) * g.dx - y,
-                         dz = g.z0 + (k - 1) * g.dx - z;
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
-          }
+#pragma unroll 1
+      for (int a = 0; a < 4; a++) {
+        const int i = ci - 1 + a;
+        int bw[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          bw[u] = b[bq(g, min(max(i, 1), g.im), min(max(cj - 1 + u / 4, 1), g.jm),
+                       min(max(ck - 1 + u % 4, 1), g.km))];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int j = cj - 1 + u / 4, k = ck - 1 + u % 4;
+          if (i < 1 || i > g.im || j < 1 || j > g.jm || k < 1 || k > g.km || bw[u] < 0) continue;
+          const double dx = g.x0 + (i - 1) * g.dx - x, dy = g.y0 + (j - 1) * g.dx - y,
+                       dz = g.z0 + (k - 1) * g.dx - z;
+          const double d = sqrt(dx * dx + dy * dy + dz * dz);
+          if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
+        }
+      }
+      if (!(mind < 2.0 * g.dx * (1.0 - 1e-9))) {
+        int lo[3], hi[3];
+        window(g, x, y, z, lo, hi);
+        mind = 5000.0;
+        for (int i = lo[0]; i <= hi[0]; i++)
+          for (int j = lo[1]; j <= hi[1]; j++)
+            for (int k = lo[2]; k <= hi[2]; k++) {
+              if (b[bq(g, i, j, k)] < 0) continue;
+              const double dx = g.x0 + (i - 1) * g.dx - x, dy = g.y0 + (j - 1) * g.dx - y,
+                           dz = g.z0 + (k - 1) * g.dx - z;
+              const double d = sqrt(dx * dx + dy * dy + dz * dz);
+              if (d < mind) { mind = d; bi = i; bj = j; bk = k; }
+            }
+      }
     }
     if (mind < cutoff) {
       bin = domid[bq(g, bi, bj, bk)];
@@ -125,9 +151,16 @@ __global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double
     bdist = fmax(bdist, __shfl_xor(bdist, off));
     ext = fmax(ext, __shfl_xor(ext, off));
   }
+  // The bounds are read first and the atomic made only by a wave that
+  // raises one (the read may be older than the bound, never above it), so
+  // the many waves of one launch do not queue on the same two words.
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&bounds[0], (unsigned long long)__double_as_longlong(bdist));
-    atomicMax(&bounds[1], (unsigned long long)__double_as_longlong(ext));
+    const unsigned long long bb = (unsigned long long)__double_as_longlong(bdist),
+                             be = (unsigned long long)__double_as_longlong(ext);
+    if (bb > __hip_atomic_load(&bounds[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&bounds[0], bb);
+    if (be > __hip_atomic_load(&bounds[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&bounds[1], be);
   }
 }
 
@@ -480,20 +513,23 @@ __device__ __forceinline__ bool wall_clear(double x, double y, double z, const f
   return lb > 0 && lb * lb >= r.wall_reach2;
 }
 
-// The nodes of released cells that wall_clear does not clear, appended to
-// list (count in list[0], nodes from list[1] on, in no particular order).
+// flag[gq] = 1 for the nodes of released cells that wall_clear does not
+// clear, 0 for the others.
 __global__ __launch_bounds__(256) void k_wall_select(const double *__restrict__ X, size_t NT, int nn, int ncell,
                                                      const int32_t *__restrict__ status,
                                                      const float *__restrict__ wdist, Grid g, RepParams r,
-                                                     int32_t *__restrict__ list) {
+                                                     int32_t *__restrict__ flag) {
   const int gq = blockIdx.x * blockDim.x + threadIdx.x;
   if (gq >= ncell * nn) return;
-  if (status[gq / nn] == 0 || status[gq / nn] <= kStatusDead) return;
-  if (wall_clear(X[gq], X[NT + gq], X[2 * NT + gq], wdist, g, r)) return;
-  list[1 + atomicAdd(list, 1)] = gq;
+  flag[gq] = status[gq / nn] != 0 && status[gq / nn] > kStatusDead &&
+                     !wall_clear(X[gq], X[NT + gq], X[2 * NT + gq], wdist, g, r)
+                 ? 1
+                 : 0;
 }
 
-// Cell-wall repulsion (ZebraflowWT.c:10030-10052), one lane per listed node.
+// Cell-wall repulsion (ZebraflowWT.c:10030-10052), one lane per listed node
+// (the list in node order, a contiguous part per XCD, so that the nodes of
+// one cell, which search the same wall nodes, share an L2).
 // The reference walks the 9^3 window in (i, j, k) order, and for each W_index
 // wall node each of its triangles' three vertices and centroid, keeping the
 // triangle of the first point strictly nearer than all before it, starting
@@ -516,8 +552,10 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
   const float reach_f = (float)sqrt(r.wall_reach2) * (1.f - 1e-6f);
   __shared__ int wq[kWallQueue][128];
   __shared__ float wlb[kWallQueue][128];
-  const int nlist = list[0];
-  for (int li = blockIdx.x * blockDim.x + threadIdx.x; li < nlist; li += gridDim.x * blockDim.x) {
+  int lbeg, lend;
+  xcd_list_span(list[0], lbeg, lend);
+  for (int li = lbeg + (int)(blockIdx.x / 8 * blockDim.x + threadIdx.x); li < lend;
+       li += (int)(gridDim.x / 8 * blockDim.x)) {
     const int gq = list[1 + li];
     const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
     int lo[3], hi[3];
@@ -1114,7 +1152,7 @@ void free_wall(zf_ctx *ctx) {
 void free_cells(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_tkeys[0], (void **)&ctx->d_tkeys[1], (void **)&ctx->d_crec,
                    (void **)&ctx->d_bin_cnt, (void **)&ctx->d_bin_off, &ctx->d_bin_scan_tmp,
-                   (void **)&ctx->d_npair, (void **)&ctx->d_pairs, (void **)&ctx->d_over, (void **)&ctx->d_rep_lists,
+                   (void **)&ctx->d_npair, (void **)&ctx->d_pairs, (void **)&ctx->d_over, (void **)&ctx->d_rep_lists, &ctx->d_rsel_tmp,
                    (void **)&ctx->d_tsort_tmp,
                    (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near,
                    (void **)&ctx->d_rep_bounds};
@@ -1228,12 +1266,15 @@ int repulsion_impl(zf_ctx *ctx) {
                      (int32_t *)ctx->d_vrep, 3 * NT * 2, 0);
   hipLaunchKernelGGL(k_fill, dim3(nb(NT, 256)), dim3(256), 0, ctx->stream, ctx->d_near, NT, 0);
   if (ctx->have_wall) {
-    // the nodes near the wall, then their walk (the list reuses the
-    // overflow list of the cell-cell walk, cleared again below)
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_over, (size_t)1, 0);
+    // the nodes near the wall, listed in node order, then their walk (the
+    // flags and the list reuse the cell-cell walk's counts and overflow
+    // list, both written again below)
     hipLaunchKernelGGL(k_wall_select, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_x, NT, ctx->nn,
-                       ctx->ncell, ctx->d_status, ctx->d_wdist, grid_of(ctx), r, ctx->d_over);
-    hipLaunchKernelGGL(k_wall_repulsion, dim3(std::min<unsigned>(nb(n, 128), 1024u)), dim3(128), 0,
+                       ctx->ncell, ctx->d_status, ctx->d_wdist, grid_of(ctx), r, ctx->d_npair);
+    size_t stmp = ctx->rsel_tmp_bytes;
+    ZF_HIP(ctx, hipcub::DeviceSelect::Flagged(ctx->d_rsel_tmp, stmp, hipcub::CountingInputIterator<int32_t>(0),
+                                              ctx->d_npair, ctx->d_over + 1, ctx->d_over, n, ctx->stream));
+    hipLaunchKernelGGL(k_wall_repulsion, dim3(std::min<unsigned>((nb(n, 128) + 7) / 8 * 8, 1024u)), dim3(128), 0,
                        ctx->stream, ctx->d_x, NT, ctx->d_over, ctx->d_windex, ctx->d_wdist,
                        ctx->d_wnode_tri, ctx->d_wtri, ctx->d_wxyz, ctx->d_wpos, ctx->d_wn_sph,
                        ctx->d_wn_geo, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep, ctx->d_near);
@@ -1314,6 +1355,12 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
                                                ctx->dommax + 1, ctx->stream));
   ctx->bin_scan_tmp_bytes = stmp;
   if ((r = dev_alloc(ctx, &ctx->d_bin_scan_tmp, std::max<size_t>(stmp, 1)))) return r;
+  size_t wtmp = 0;
+  ZF_HIP(ctx, hipcub::DeviceSelect::Flagged(nullptr, wtmp, hipcub::CountingInputIterator<int32_t>(0),
+                                            ctx->d_npair, ctx->d_over + 1, ctx->d_over, (int)NT,
+                                            ctx->stream));
+  ctx->rsel_tmp_bytes = wtmp;
+  if ((r = dev_alloc(ctx, &ctx->d_rsel_tmp, std::max<size_t>(wtmp, 1)))) return r;
   if ((r = bin_offsets(ctx))) return r;
   ctx->have_rep = true;
   return ZF_OK;

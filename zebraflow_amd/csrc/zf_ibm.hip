@@ -75,14 +75,21 @@ __device__ __forceinline__ double kern1(double d, double width) {
   return 1.0 + cos(kPi * fabs(d) / width);
 }
 
-// Interpolation, one lane per membrane node: the nodes within the spherical
-// cut-off |d| <= 2 DX all lie in the window [i-2, i+3] of the reference's
-// (int)((x-X0)/DX + 1.01) index, which is what is scanned.
-// With a wall layer (vslot non-null), window nodes that are not fluid add
-// the same weight times their Uvoid (ZebraflowWT.c:8980-8996), in the
-// window's order like the fluid terms.
+// Interpolation, one lane per membrane node (ZebraflowWT.c:8950-8996):
+// the lattice nodes within the cut-off (|d| <= 2 DX, or the cube |d_a| <
+// 2 DX) of the node, in (i, j, k) order.  They lie within 2 DX along every
+// axis, so among the 5 planes per axis from ceil(u - 2) on (u the node's
+// 1-based lattice coordinate, taken 1e-9 lower to cover rounding), all
+// inside the window the reference scans around its (int)((x-X0)/DX + 1.01)
+// index.  The kernel factor of each plane is evaluated once, and a row's 5
+// points are read in three batches (B, then the storage slots, then the
+// velocities) instead of one dependent chain per point; the terms are the
+// same expressions, summed in the same order.  With a wall layer (vslot
+// non-null), window nodes that are not fluid add the same weight times
+// their Uvoid (ZebraflowWT.c:8980-8996), in the window's order like the
+// fluid terms.
 template <class U>
-__global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int ncell,
+__global__ __launch_bounds__(128) void k_interp(const double *__restrict__ X, size_t NT, int nn, int ncell,
                          const int32_t *__restrict__ status, const int32_t *__restrict__ b,
                          const int32_t *__restrict__ sbox, const U *__restrict__ umac,
                          size_t stride, IbmParams p, const int32_t *__restrict__ vslot,
@@ -95,33 +102,70 @@ __global__ void k_interp(const double *__restrict__ X, size_t NT, int nn, int nc
             k = (int)((zp - p.z0) / p.dx + 1.01);
   const int i1 = max(1, i - 3), i2 = min(p.im, i + 3), j1 = max(1, j - 3), j2 = min(p.jm, j + 3),
             k1 = max(1, k - 3), k2 = min(p.km, k + 3);
+  const int ilo = (int)ceil((xp - p.x0) / p.dx + 1.0 - 2.0 - 1e-9),
+            jlo = (int)ceil((yp - p.y0) / p.dx + 1.0 - 2.0 - 1e-9),
+            klo = (int)ceil((zp - p.z0) / p.dx + 1.0 - 2.0 - 1e-9);
   const double width = 2.0 * p.dx, w2 = width * width;
+  double dyv[5], kyv[5], dzv[5], kzv[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    dyv[t] = p.y0 + (jlo + t - 1) * p.dx - yp;
+    kyv[t] = kern1(dyv[t], width);
+    dzv[t] = p.z0 + (klo + t - 1) * p.dx - zp;
+    kzv[t] = kern1(dzv[t], width);
+  }
   double v0 = 0, v1 = 0, v2 = 0;
-  for (int ii = i1; ii <= i2; ii++) {
+#pragma unroll 1
+  for (int a = 0; a < 5; a++) {
+    const int ii = ilo + a;
     const double dxv = p.x0 + (ii - 1) * p.dx - xp;
-    for (int jj = j1; jj <= j2; jj++) {
-      const double dyv = p.y0 + (jj - 1) * p.dx - yp;
-      for (int kk = k1; kk <= k2; kk++) {
-        const double dzv = p.z0 + (kk - 1) * p.dx - zp;
-        if (p.cube ? (fabs(dxv) >= width || fabs(dyv) >= width || fabs(dzv) >= width)
-                   : dxv * dxv + dyv * dyv + dzv * dzv > w2)
-          continue;
-        const size_t q = ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1);
-        const int bv = b[q];
-        if (!(bv > 0 && bv < 4998)) {
-          const int sl = vslot ? vslot[q] : -1;
-          if (sl < 0) continue;
-          const double w = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
-          v0 += w * p.vel_scale * uvoid[3 * (size_t)sl];
-          v1 += w * p.vel_scale * uvoid[3 * (size_t)sl + 1];
-          v2 += w * p.vel_scale * uvoid[3 * (size_t)sl + 2];
-          continue;
+    const double kxv = kern1(dxv, width);
+    const bool iok = ii >= i1 && ii <= i2;
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const int jj = jlo + c;
+      bool ok[5];
+      size_t q[5];
+      int bv[5];
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        const int kk = klo + t;
+        ok[t] = iok && jj >= j1 && jj <= j2 && kk >= k1 && kk <= k2 &&
+                !(p.cube ? (fabs(dxv) >= width || fabs(dyv[c]) >= width || fabs(dzv[t]) >= width)
+                         : dxv * dxv + dyv[c] * dyv[c] + dzv[t] * dzv[t] > w2);
+        q[t] = ok[t] ? ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1) : 0;
+        if (ok[t]) bv[t] = b[q[t]];
+      }
+      if (!__any(ok[0] || ok[1] || ok[2] || ok[3] || ok[4])) continue;
+      bool fl[5];
+      int sv[5];
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        fl[t] = ok[t] && bv[t] > 0 && bv[t] < 4998;
+        sv[t] = -1;
+        if (fl[t]) sv[t] = sbox[q[t]];
+        else if (ok[t] && vslot) sv[t] = vslot[q[t]];
+      }
+      double u0[5], u1[5], u2[5];
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        if (fl[t]) {
+          u0[t] = (double)umac[sv[t]];
+          u1[t] = (double)umac[stride + sv[t]];
+          u2[t] = (double)umac[2 * stride + sv[t]];
+        } else if (ok[t] && sv[t] >= 0) {
+          u0[t] = uvoid[3 * (size_t)sv[t]];
+          u1[t] = uvoid[3 * (size_t)sv[t] + 1];
+          u2[t] = uvoid[3 * (size_t)sv[t] + 2];
         }
-        const double w = kern1(dxv, width) * kern1(dyv, width) * kern1(dzv, width) / 64.0;
-        const int s = sbox[q];
-        v0 += w * p.vel_scale * (double)umac[s];
-        v1 += w * p.vel_scale * (double)umac[stride + s];
-        v2 += w * p.vel_scale * (double)umac[2 * stride + s];
+      }
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        if (!ok[t] || sv[t] < 0) continue;
+        const double w = kxv * kyv[c] * kzv[t] / 64.0;
+        v0 += w * p.vel_scale * u0[t];
+        v1 += w * p.vel_scale * u1[t];
+        v2 += w * p.vel_scale * u2[t];
       }
     }
   }
@@ -208,11 +252,10 @@ __global__ void k_bin_ranges(const uint64_t *__restrict__ keys, int n, uint64_t 
   if (i == n - 1 || (keys[i + 1] >> 32) != t) range[2 * t + 1] = i + 1;
 }
 
-// The tiles with a node in one of their 27 neighbours (themselves
-// included), appended to list (count in list[0], tiles from list[1] on, in
-// no particular order): every other tile's force is zero.
-__global__ void k_spread_tile_list(const int32_t *__restrict__ occ, int ntx, int nty, int ntz,
-                                   int32_t *__restrict__ list) {
+// act[t] = 1 for the tiles with a node in one of their 27 neighbours
+// (themselves included), 0 for the others, whose force is zero.
+__global__ void k_spread_tile_flags(const int32_t *__restrict__ occ, int ntx, int nty, int ntz,
+                                    int32_t *__restrict__ act) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntx * nty * ntz) return;
   const int tz = t % ntz, ty = (t / ntz) % nty, tx = t / (ntz * nty);
@@ -220,7 +263,7 @@ __global__ void k_spread_tile_list(const int32_t *__restrict__ occ, int ntx, int
   for (int a = max(tx - 1, 0); a <= min(tx + 1, ntx - 1); a++)
     for (int c = max(ty - 1, 0); c <= min(ty + 1, nty - 1); c++)
       for (int e = max(tz - 1, 0); e <= min(tz + 1, ntz - 1); e++) any |= occ[(a * nty + c) * ntz + e] != 0;
-  if (any) list[1 + atomicAdd(list, 1)] = t;
+  act[t] = any ? 1 : 0;
 }
 
 // The bins a tile gathers from, in gathering order: the 27 neighbouring
@@ -258,7 +301,8 @@ __device__ constexpr SpreadSubs kSpreadSubs = make_spread_subs();
 __device__ __forceinline__ float kcos_pi(float x) { return cospif(x); }
 __device__ __forceinline__ double kcos_pi(double x) { return cospi(x); }
 
-// One wave per listed tile; lane = lattice node (4x4x4).  Gathers the
+// One wave per listed tile (the list in tile order, a contiguous part per
+// XCD); lane = lattice node (4x4x4).  Gathers the
 // membrane nodes of the 64 bins of kSpreadSubs in (bin, node) order, in the
 // field's type F (positions relative to the tile origin, weights and
 // sums): the bins' runs of the sorted keys are taken as one list, staged 64
@@ -292,8 +336,9 @@ __global__ __launch_bounds__(64) void k_spread_gather(
   const int lane = threadIdx.x;
   const int ix = lane / 16, iy = (lane / 4) % 4, iz = lane % 4;
   const F rx = (F)ix, ry = (F)iy, rz = (F)iz;
-  const int ntl = tlist[0];
-  for (int it = blockIdx.x; it < ntl; it += gridDim.x) {
+  int tbeg, tend;
+  xcd_list_span(tlist[0], tbeg, tend);
+  for (int it = tbeg + (int)blockIdx.x / 8; it < tend; it += (int)gridDim.x / 8) {
     const int tile = tlist[1 + it];
     const int tz = tile % p.ntz, ty = (tile / p.ntz) % p.nty, tx = tile / (p.ntz * p.nty);
     // lattice node of this lane (0-based), and the tile origin
@@ -590,7 +635,8 @@ void free_ibm(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_umac, (void **)&ctx->d_vrep,
                    (void **)&ctx->d_ffl,   (void **)&ctx->d_keys[0], (void **)&ctx->d_keys[1],
                    (void **)&ctx->d_sort_tmp, (void **)&ctx->d_tile_range, (void **)&ctx->d_flag,
-                   (void **)&ctx->d_sbin_range, (void **)&ctx->d_tile_occ, (void **)&ctx->d_spread_tiles};
+                   (void **)&ctx->d_sbin_range, (void **)&ctx->d_tile_occ, (void **)&ctx->d_spread_tiles,
+                   (void **)&ctx->d_tile_act, &ctx->d_sel_tmp};
   for (auto b : bufs) dev_free(ctx, b);
   ctx->keys_cap = 0;
   ctx->have_ibm = false;
@@ -676,17 +722,22 @@ int spread_bins(zf_ctx *ctx, int nkeys, uint64_t ntiles) {
                      (size_t)(2 * nbins), 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(nb(ntiles, 256)), dim3(256), 0, ctx->stream, ctx->d_tile_occ,
                      (size_t)ntiles, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, ctx->d_spread_tiles, (size_t)1, 0);
   if (nkeys > 0)
     hipLaunchKernelGGL(k_bin_ranges, dim3(nb(nkeys, 256)), dim3(256), 0, ctx->stream, ctx->d_keys[1],
                        nkeys, nbins, ctx->d_sbin_range, ctx->d_tile_occ);
-  hipLaunchKernelGGL(k_spread_tile_list, dim3(nb(ntiles, 256)), dim3(256), 0, ctx->stream,
-                     ctx->d_tile_occ, ctx->ntx, ctx->nty, ctx->ntz, ctx->d_spread_tiles);
+  hipLaunchKernelGGL(k_spread_tile_flags, dim3(nb(ntiles, 256)), dim3(256), 0, ctx->stream,
+                     ctx->d_tile_occ, ctx->ntx, ctx->nty, ctx->ntz, ctx->d_tile_act);
+  // the flagged tiles in tile order: count at [0], tiles from [1] on
+  size_t stmp = ctx->sel_tmp_bytes;
+  ZF_HIP(ctx, hipcub::DeviceSelect::Flagged(ctx->d_sel_tmp, stmp, hipcub::CountingInputIterator<int32_t>(0),
+                                            ctx->d_tile_act, ctx->d_spread_tiles + 1, ctx->d_spread_tiles,
+                                            (int)ntiles, ctx->stream));
   return ZF_OK;
 }
 
-// Workgroups of the gather (one wave each, over the listed tiles).
-unsigned spread_grid(uint64_t ntiles) { return (unsigned)std::min<uint64_t>(ntiles, 16384); }
+// Workgroups of the gather (one wave each, over the listed tiles), a
+// multiple of 8.
+unsigned spread_grid(uint64_t ntiles) { return (unsigned)std::min<uint64_t>((ntiles + 7) / 8 * 8, 16384); }
 
 int ibm_spread_impl(zf_ctx *ctx) {
   const size_t ncap = ctx->mcfg.max_cells, NT = ncap * ctx->nn;
@@ -929,8 +980,15 @@ int zf_ibm_setup(zf_ctx *ctx, const zf_ibm_config *ic) {
       (rc = dev_alloc(ctx, (void **)&ctx->d_sbin_range, 64 * ntiles)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_tile_occ, 4 * ntiles)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_spread_tiles, 4 * (ntiles + 1))) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_tile_act, 4 * ntiles)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_flag, 4 * (ncap + 1))))
     return rc;
+  size_t stmp = 0;
+  ZF_HIP(ctx, hipcub::DeviceSelect::Flagged(nullptr, stmp, hipcub::CountingInputIterator<int32_t>(0),
+                                            ctx->d_tile_act, ctx->d_spread_tiles + 1, ctx->d_spread_tiles,
+                                            (int)ntiles, ctx->stream));
+  ctx->sel_tmp_bytes = stmp;
+  if ((rc = dev_alloc(ctx, &ctx->d_sel_tmp, stmp))) return rc;
   // sort keys for the membrane nodes and, with a nucleus, its nodes after them
   if ((rc = ensure_sort_keys(ctx, NT + (ctx->have_nuc ? ncap * ctx->nnuc : 0)))) return rc;
   hipLaunchKernelGGL(k_zero_f64, dim3(nb(3 * NT, 256)), dim3(256), 0, ctx->stream, ctx->d_vrep, 3 * NT);

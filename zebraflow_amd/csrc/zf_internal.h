@@ -82,6 +82,17 @@ __device__ __forceinline__ int xcd_range_block(int b, int nblk) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Entries [beg, end) of a list of n that the workgroups on XCD blockIdx.x % 8
+// share: 8 contiguous ranges, one per XCD, so that entries next to each
+// other in the list (neighbouring nodes or tiles) meet in one L2.  The grid
+// must be a multiple of 8; workgroup b takes every (gridDim.x / 8)-th part
+// of its range from part b / 8 on.
+__device__ __forceinline__ void xcd_list_span(int n, int &beg, int &end) {
+  const int x = blockIdx.x % 8;
+  beg = (int)((long long)n * x / 8);
+  end = (int)((long long)n * (x + 1) / 8);
+}
+
 struct PipeState;  // two-step launches (zf_pipe.hip)
 
 }  // namespace zf
@@ -178,6 +189,9 @@ struct zf_ctx {
   int32_t *d_sbin_range = nullptr;  // 2 x 8 ntiles: the same per spreading bin (tile octant)
   int32_t *d_tile_occ = nullptr;    // ntiles: 1 where a tile holds a spreading node
   int32_t *d_spread_tiles = nullptr;  // 1 + ntiles: count, then the tiles the gather visits
+  int32_t *d_tile_act = nullptr;    // ntiles: 1 where the gather visits a tile
+  void *d_sel_tmp = nullptr;        // temporary storage of the ordered list selections
+  size_t sel_tmp_bytes = 0;
   int32_t *d_flag = nullptr;        // ncap: speed check
   int ntx = 0, nty = 0, ntz = 0;
 
@@ -199,6 +213,8 @@ struct zf_ctx {
   int32_t *d_pairs = nullptr;     // ncap*nn x 48: the first of them, in walk order
   int32_t *d_over = nullptr;      // 1 + ncap*nn: count, then the nodes whose list overflowed
   int32_t *d_rep_lists = nullptr; // nodes with listed triangles, by count class
+  void *d_rsel_tmp = nullptr;     // temporary storage of the near-wall node selection
+  size_t rsel_tmp_bytes = 0;
   // bounds recorded with the cell list: largest centroid-to-bin and
   // centroid-to-vertex distance (FP64 bits); valid while the triangle
   // geometry is the one the list was built from (geom_gen == bins_gen)
