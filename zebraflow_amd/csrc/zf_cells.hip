@@ -199,9 +199,6 @@ constexpr double kReachSlack = 1e-9;
 
 // entries of a lane's queue of passing triangles in the repulsion kernel
 constexpr int kRepQueue = 32;
-// entries of a lane's queue of wall nodes in the wall-repulsion kernel (a
-// walking node queues 27 on average in the 15-segment vessel pair)
-constexpr int kWallQueue = 40;
 
 // Sub-triangle centroids of a triangle refined four times
 // (ZebraflowWT.c:9990-10025): the reference forms them through chained edge
@@ -527,76 +524,168 @@ __global__ __launch_bounds__(256) void k_wall_select(const double *__restrict__ 
                  : 0;
 }
 
-// Cell-wall repulsion (ZebraflowWT.c:10030-10052), one lane per listed node
+// Cell-wall repulsion (ZebraflowWT.c:10030-10052), one wave per listed node
 // (the list in node order, a contiguous part per XCD, so that the nodes of
-// one cell, which search the same wall nodes, share an L2).
-// The reference walks the 9^3 window in (i, j, k) order, and for each W_index
-// wall node each of its triangles' three vertices and centroid, keeping the
-// triangle of the first point strictly nearer than all before it, starting
-// from wall_search: the winner is the point of least distance, the earliest
-// in that order among equal distances.  Here every point carries its order
-// (the visit number of its wall node, then its triangle and point), and the
+// one cell, which search the same wall nodes, share an L2).  The reference
+// walks the 9^3 window in (i, j, k) order, and for each W_index wall node
+// each of its triangles' three vertices and centroid, keeping the triangle
+// of the first point strictly nearer than all before it, starting from
+// wall_search: the winner is the point of least distance, the earliest in
+// that order among equal distances.  Here every point carries its order
+// (its window position, then its triangle slot and point) and the
 // lexicographically least (distance, order) wins, whatever the order of
-// evaluation: the same triangle.
-__global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict__ X, size_t NT,
+// evaluation: the same triangle.  The wave's lanes take the window's rows
+// (lane l rows l and l + 64), put the wall nodes that pass the bounds below
+// in a list in LDS, examine the list's entries in parallel and reduce.
+// A wall node repeated along the walk is listed again; its points come
+// later with the same distances and never win.
+constexpr int kWallList = 729;  // at most 9 entries per row of the 9 x 9 rows
+__global__ __launch_bounds__(64) void k_wall_repulsion(const double *__restrict__ X, size_t NT,
                                  const int32_t *__restrict__ list, const int32_t *__restrict__ windex,
                                  const float *__restrict__ wdist,
                                  const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
                                  const double *__restrict__ wxyz, const float4 *__restrict__ wpos,
+                                 const float4 *__restrict__ wnrm,
                                  const float4 *__restrict__ wn_sph, const double *__restrict__ wn_geo,
                                  const double *__restrict__ wnormal,
                                  Grid g, RepParams r, double *__restrict__ vrep,
                                  int32_t *__restrict__ nearest) {
+  __shared__ int ewn[kWallList], eord[kWallList];
+  __shared__ float elb[kWallList];
+  const int lane = threadIdx.x;
   // the reach in FP32, rounded down: the tests against it below skip only
   // wall nodes beyond the reach
   const float reach_f = (float)sqrt(r.wall_reach2) * (1.f - 1e-6f);
-  __shared__ int wq[kWallQueue][128];
-  __shared__ float wlb[kWallQueue][128];
+  // wall_search in FP32, rounded up
+  float search_f = (float)r.wall_search;
+  if ((double)search_f < r.wall_search) search_f = nextafterf(search_f, INFINITY);
   int lbeg, lend;
   xcd_list_span(list[0], lbeg, lend);
-  for (int li = lbeg + (int)(blockIdx.x / 8 * blockDim.x + threadIdx.x); li < lend;
-       li += (int)(gridDim.x / 8 * blockDim.x)) {
+  for (int li = lbeg + (int)blockIdx.x / 8; li < lend; li += (int)gridDim.x / 8) {
     const int gq = list[1 + li];
     const double x = X[gq], y = X[NT + gq], z = X[2 * NT + gq];
     int lo[3], hi[3];
     window(g, x, y, z, lo, hi);
-    double mind = r.wall_search;
     const float xf = (float)x, yf = (float)y, zf = (float)z;
-    // A coordinate rounded to FP32 is off by at most |a| 2^-24, so the FP32
-    // distance from x to a sphere centre within reach is off by less than
+    // A coordinate rounded to FP32 is off by at most |a| 2^-24, so an FP32
+    // distance from x to a point within reach is off by less than
     // 2^-22 (|x| + |y| + |z| + 3 reach); the margin is four times that
     const float smargin =
         (float)std::ldexp(fabs(x) + fabs(y) + fabs(z) + 4.0 * sqrt(r.wall_reach2), -20) + 1e-6f;
-    // order of the current winner: -1 while none (the first point must be
-    // strictly nearer than wall_search)
-    int best = -1, wbest = 0, last = 0, seq = 0;
-    // Wall nodes that pass the distance tests go to the lane's queue in LDS
-    // with their lower bound (rounded down to float); their triangles are
-    // examined when any active lane's queue could overflow in the next row,
-    // and once at the end.  The queued node of least bound goes first: its
-    // triangles usually bring mindis near its final value, so that most of
-    // the others are then skipped by their bound.  A node is skipped when
-    // its bound exceeds mindis, or equals it and the node comes later than
-    // the current winner (it can then neither be nearer nor tie earlier).
-    int qlen = 0;
-    auto examine = [&](int k) {
-      const int wn = wq[k][threadIdx.x];
-      // the bounding spheres of the node's triangles in one batch: a
-      // triangle whose sphere lies beyond mindis (by more than the FP32
-      // rounding of the test) holds no point that can win (unused slots
-      // hold a sphere at infinity)
+    const int nj = hi[1] - lo[1] + 1, nrow = (hi[0] - lo[0] + 1) * nj;
+    // ub: an upper bound on the winning distance, the distance (rounded up)
+    // to the nearest listed wall node, itself a vertex of its triangles
+    float ub = INFINITY;
+    int ne = 0;
+    for (int r0 = 0; r0 < nrow; r0 += 64) {
+      const int rw = r0 + lane;
+      int cw[9], co[9];
+      float cl[9];
+      int cnt = 0;
+      if (rw < nrow) {
+        const int i = lo[0] + rw / nj, j = lo[1] + rw % nj;
+        const double qx = g.x0 + (i - 1) * g.dx - x, qy = g.y0 + (j - 1) * g.dx - y;
+        const double rij = qx * qx + qy * qy;
+        // the row's 9 W_index entries and their distances in one batch
+        // (in-bounds addresses; entries past hi[2] are not used)
+        const size_t row = bq(g, i, j, 1);
+        int wv[9];
+        float wd[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const size_t at = row + (size_t)(min(lo[2] + t, hi[2]) - 1);
+          wv[t] = windex[at];
+          wd[t] = wdist[at];
+        }
+        bool cand[9];
+        int last = 0;
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const int wn = wv[t];
+          // a wall node just seen in the row is not listed again
+          cand[t] = lo[2] + t <= hi[2] && wn > 0 && wn != last;
+          last = cand[t] ? wn : last;
+          // wn lies wd of lattice node p = (i, j, lo[2] + t), so at least
+          // |x - p| - wd from x: beyond reach (tested below exactly) for
+          // certain if that exceeds the reach lowered by a margin
+          const double qz = g.z0 + (lo[2] + t - 1) * g.dx - z;
+          const float dp = sqrtf((float)(rij + qz * qz));
+          cand[t] = cand[t] && dp - wd[t] - 1e-4f < reach_f;
+        }
+        float4 wp[9], wnr[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+          if (cand[t]) {
+            wp[t] = wpos[wv[t] - 1];
+            wnr[t] = wnrm[wv[t] - 1];
+          }
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          if (!cand[t]) continue;
+          const float ex = xf - wp[t].x, ey = yf - wp[t].y, ez = zf - wp[t].z;
+          const float dist = sqrtf(ex * ex + ey * ey + ez * ez);
+          // a lower bound on the distance from x to wn
+          const float e = dist - smargin;
+          // every vertex and the centroid of a triangle of wall node wn lie
+          // within wall_lmax of it: if wn is farther than wall_search +
+          // wall_lmax, none of them can beat wall_search
+          if (e >= reach_f) continue;
+          // they lie within wrad (rounded up) of wn, and within h of its
+          // tangent plane along n: none is nearer than wall_search if the
+          // larger of the two bounds reaches it
+          const float dn = fabsf(wnr[t].x * ex + wnr[t].y * ey + wnr[t].z * ez) - smargin;
+          const float lbf = fmaxf(e - wp[t].w, dn - wnr[t].w);
+          if (lbf >= search_f) continue;
+          // a node with triangles (h >= 0) is one of their vertices
+          if (wnr[t].w >= 0.f) ub = fminf(ub, dist + smargin);
+          cw[cnt] = wv[t];
+          cl[cnt] = lbf;
+          co[cnt] = (rw * 9 + t) * 64;  // window position, then 4 (slot + 1) + point
+          cnt++;
+        }
+      }
+      // the entries go to the list in row order
+      int incl = cnt;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        incl += lane >= d ? v : 0;
+      }
+      const int at = ne + incl - cnt;
+#pragma unroll
+      for (int c = 0; c < 9; c++)
+        if (c < cnt) {
+          ewn[at + c] = cw[c];
+          elb[at + c] = cl[c];
+          eord[at + c] = co[c];
+        }
+      ne += __shfl(incl, 63);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ub = fminf(ub, __shfl_xor(ub, off));
+    __syncthreads();
+    // bound of the examination: every point nearer than the winner's
+    // distance is at least as near as ub, and a winning point is nearer
+    // than wall_search
+    const float bnd = fminf(ub, search_f);
+    double bd = r.wall_search;
+    int bo = -1, bw = 0;
+    for (int e = lane; e < ne; e += 64) {
+      if (elb[e] > bnd) continue;
+      const int wn = ewn[e];
+      // the bounding spheres of the node's triangles in one batch (unused
+      // slots hold a sphere at infinity)
       const float4 *sph = &wn_sph[12 * (size_t)(wn - 1)];
       float4 sp[12];
 #pragma unroll
       for (int u = 0; u < 12; u++) sp[u] = sph[u];
-      const float lim = (float)mind + smargin;
       unsigned live = 0;
 #pragma unroll
       for (int u = 0; u < 12; u++) {
         const float fx = xf - sp[u].x, fy = yf - sp[u].y, fz = zf - sp[u].z;
-        live |= sqrtf(fx * fx + fy * fy + fz * fz) - sp[u].w <= lim ? 1u << u : 0u;
+        live |= sqrtf(fx * fx + fy * fy + fz * fz) - smargin - sp[u].w <= bnd ? 1u << u : 0u;
       }
-      const int order0 = (seq - qlen + k) * 64;  // visit number, then 4 l + point
+      const int order0 = eord[e];
 #pragma unroll 1
       while (live) {
         const int u = __builtin_ctz(live);
@@ -611,111 +700,41 @@ __global__ __launch_bounds__(128) void k_wall_repulsion(const double *__restrict
           const double wx = gv[3 * sv], wy = gv[3 * sv + 1], wz = gv[3 * sv + 2];
           const double d = sqrt((x - wx) * (x - wx) + (y - wy) * (y - wy) + (z - wz) * (z - wz));
           const int o = order0 + 4 * (u + 1) + sv;
-          if (d < mind || (d == mind && best >= 0 && o < best)) {
-            mind = d;
-            best = o;
-            wbest = wn;
+          if (d < bd || (d == bd && bo >= 0 && o < bo)) {
+            bd = d;
+            bo = o;
+            bw = wn;
           }
         }
       }
-    };
-    auto skip = [&](int k) {
-      const double b = (double)wlb[k][threadIdx.x];
-      return b > mind || (b == mind && (best < 0 || (seq - qlen + k) * 64 > best));
-    };
-    auto drain = [&]() {
-      int kb = 0;
-      for (int k = 1; k < qlen; k++) kb = wlb[k][threadIdx.x] < wlb[kb][threadIdx.x] ? k : kb;
-      // kb, then the others in queue order (one call site of examine)
-#pragma unroll 1
-      for (int t = 0; t < qlen; t++) {
-        const int k = t == 0 ? kb : t <= kb ? t - 1 : t;
-        if (!skip(k)) examine(k);
-      }
-      qlen = 0;
-    };
-    // rows (i, j) of the window in order, then one more pass that only
-    // drains (a single call site keeps the drain's code and registers once)
-    const int nj = hi[1] - lo[1] + 1, nrow = (hi[0] - lo[0] + 1) * nj;
-    for (int rw = 0; rw <= nrow; rw++) {
-      // room for this row's up to 9 wall nodes in every active lane's queue
-      if (rw == nrow || __ballot(qlen > kWallQueue - 9)) drain();
-      if (rw == nrow) break;
-      {
-        const int i = lo[0] + rw / nj, j = lo[1] + rw % nj;
-        const double qx = g.x0 + (i - 1) * g.dx - x, qy = g.y0 + (j - 1) * g.dx - y;
-        const double rij = qx * qx + qy * qy;
-        // the row's 9 W_index entries and their distances in one batch of
-        // loads (in-bounds addresses; entries past hi[2] are not used)
-        const size_t row = bq(g, i, j, 1);
-        int wv[9];
-        float wd[9];
+    }
+    // the wave's least (distance, order)
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
-          const size_t at = row + (size_t)(min(lo[2] + t, hi[2]) - 1);
-          wv[t] = windex[at];
-          wd[t] = wdist[at];
-        }
-        // the wall nodes worth a closer look, and their positions and
-        // radii fetched in one batch
-        bool cand[9];
-#pragma unroll
-        for (int t = 0; t < 9; t++) {
-          const int wn = wv[t];
-          // a wall node just examined cannot win again (it would come
-          // later with the same distances), and neighbouring lattice nodes
-          // mostly share their nearest wall node
-          cand[t] = lo[2] + t <= hi[2] && wn > 0 && wn != last;
-          last = cand[t] ? wn : last;
-          // wn lies wd of lattice node p = (i, j, lo[2] + t), so at least
-          // |x - p| - wd from x: beyond reach (tested below exactly) for
-          // certain if that exceeds the reach lowered by a margin
-          const double qz = g.z0 + (lo[2] + t - 1) * g.dx - z;
-          const float dp = sqrtf((float)(rij + qz * qz));
-          cand[t] = cand[t] && dp - wd[t] - 1e-4f < reach_f;
-        }
-        float4 wp[9];
-#pragma unroll
-        for (int t = 0; t < 9; t++)
-          if (cand[t]) wp[t] = wpos[wv[t] - 1];
-#pragma unroll
-        for (int t = 0; t < 9; t++) {
-          if (!cand[t]) continue;
-          // a lower bound on the distance from x to wn (FP32 distance less
-          // its rounding margin)
-          const float ex = xf - wp[t].x, ey = yf - wp[t].y, ez = zf - wp[t].z;
-          const float e = sqrtf(ex * ex + ey * ey + ez * ez) - smargin;
-          // every vertex and the centroid of a triangle of wall node wn lie
-          // within wall_lmax of it: if wn is farther than wall_search +
-          // wall_lmax, none of them can beat mindis <= wall_search
-          if (e >= reach_f) continue;
-          // they lie within wrad (rounded up) of wn: none can beat the
-          // current mindis if wn is farther than mindis + wrad (branch and
-          // bound); one at the bound comes later than the winner and cannot
-          // win a tie
-          const float lbf = e - wp[t].w;
-          if ((double)lbf >= mind) continue;
-          wq[qlen][threadIdx.x] = wv[t];
-          wlb[qlen][threadIdx.x] = lbf;
-          qlen++;
-          seq++;
-        }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double od = __shfl_xor(bd, off);
+      const int oo = __shfl_xor(bo, off), ow = __shfl_xor(bw, off);
+      if (oo >= 0 && (bo < 0 || od < bd || (od == bd && oo < bo))) {
+        bd = od;
+        bo = oo;
+        bw = ow;
       }
     }
-    if (best < 0) continue;
-    // the winner's triangle: slot (best % 64) / 4 - 1 of wall node wbest
-    const int triwall = wnode_tri[13 * (size_t)(wbest - 1) + (best & 63) / 4];
-    nearest[gq] = triwall;
-    const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
-    const double *n = &wnormal[3 * (size_t)(triwall - 1)];
-    const double d = (x - w1[0]) * n[0] + (y - w1[1]) * n[1] + (z - w1[2]) * n[2];
-    if (d < r.wall_d0 && fabs(d) < r.wall_dmax) {
-      double f = (r.wall_d0 - d) / r.wall_dmax;
-      if (d < r.wall_d0 - r.wall_dmax) f = 1.0;
-      vrep[gq] += f * r.k_wall * n[0];
-      vrep[NT + gq] += f * r.k_wall * n[1];
-      vrep[2 * NT + gq] += f * r.k_wall * n[2];
+    if (lane == 0 && bo >= 0) {
+      // the winner's triangle: slot (bo % 64) / 4 - 1 of wall node bw
+      const int triwall = wnode_tri[13 * (size_t)(bw - 1) + (bo & 63) / 4];
+      nearest[gq] = triwall;
+      const double *w1 = &wxyz[3 * (size_t)(wtri[3 * (size_t)(triwall - 1)] - 1)];
+      const double *n = &wnormal[3 * (size_t)(triwall - 1)];
+      const double d = (x - w1[0]) * n[0] + (y - w1[1]) * n[1] + (z - w1[2]) * n[2];
+      if (d < r.wall_d0 && fabs(d) < r.wall_dmax) {
+        double f = (r.wall_d0 - d) / r.wall_dmax;
+        if (d < r.wall_d0 - r.wall_dmax) f = 1.0;
+        vrep[gq] += f * r.k_wall * n[0];
+        vrep[NT + gq] += f * r.k_wall * n[1];
+        vrep[2 * NT + gq] += f * r.k_wall * n[2];
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -948,9 +967,16 @@ __global__ void k_wall_geom(const double *__restrict__ wxyz, const int32_t *__re
 
 // Per wall node, its position in FP32 and, rounded up, the largest distance
 // from it to its triangles' vertices and centroids (wrad) plus the distance
-// from its FP32 position to its exact one.
+// from its FP32 position to its exact one; and a unit vector n in FP32 (the
+// normalised sum of its triangles' normals) with, rounded up, the largest
+// |n . (p - w)| over those points p (w its FP32 position).  For any unit n,
+// |x - p| >= |n . (x - w)| - |n . (p - w)|: on a wall that is nearly flat
+// around the node this bounds the distance to its triangles far more
+// tightly than |x - w| - wrad.
 __global__ void k_wall_pos(const double *__restrict__ wxyz, const double *__restrict__ wrad, int nwn,
-                           float4 *__restrict__ wpos) {
+                           const int32_t *__restrict__ wnode_tri, const double *__restrict__ wnormal,
+                           const double *__restrict__ wtgeo, float4 *__restrict__ wpos,
+                           float4 *__restrict__ wnrm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nwn) return;
   const double x = wxyz[3 * (size_t)i], y = wxyz[3 * (size_t)i + 1], z = wxyz[3 * (size_t)i + 2];
@@ -960,6 +986,35 @@ __global__ void k_wall_pos(const double *__restrict__ wxyz, const double *__rest
   float rf = (float)rad;
   if ((double)rf < rad) rf = nextafterf(rf, INFINITY);
   wpos[i] = make_float4(fx, fy, fz, rf);
+  const int32_t *row = &wnode_tri[13 * (size_t)i];
+  const int cnt = min(row[0], 12);
+  double sx = 0, sy = 0, sz = 0;
+  for (int u = 0; u < cnt; u++) {
+    const double *n = &wnormal[3 * (size_t)(row[1 + u] - 1)];
+    sx += n[0];
+    sy += n[1];
+    sz += n[2];
+  }
+  const double len = sqrt(sx * sx + sy * sy + sz * sz);
+  if (!(len > 1e-6)) {  // no usable normal: fall back to |n . (p - w)| <= |p - w| <= wrad
+    wnrm[i] = make_float4(1.f, 0.f, 0.f, rf);
+    return;
+  }
+  // the FP32 unit vector, shortened by a hair so that |n| <= 1 however it
+  // rounded (the bound needs |n| <= 1)
+  const float nx = (float)(sx / len * (1.0 - 1e-6)), ny = (float)(sy / len * (1.0 - 1e-6)),
+              nz = (float)(sz / len * (1.0 - 1e-6));
+  double h = 0;
+  for (int u = 0; u < cnt; u++) {
+    const double *tgp = &wtgeo[12 * (size_t)(row[1 + u] - 1)];
+    for (int v = 0; v < 4; v++)
+      h = fmax(h, fabs((double)nx * (tgp[3 * v] - fx) + (double)ny * (tgp[3 * v + 1] - fy) +
+                       (double)nz * (tgp[3 * v + 2] - fz)));
+  }
+  h = h * (1.0 + 1e-9) + 1e-9;
+  float hf = (float)h;
+  if ((double)hf < h) hf = nextafterf(hf, INFINITY);
+  wnrm[i] = make_float4(nx, ny, nz, hf);
 }
 
 // Per wall node, for each of its (up to 12) triangle slots in row order, the
@@ -1141,7 +1196,7 @@ void free_wall(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_wxyz, (void **)&ctx->d_wnormal, (void **)&ctx->d_wtri,
                    (void **)&ctx->d_wnode_tri, (void **)&ctx->d_windex, (void **)&ctx->d_wdist,
                    (void **)&ctx->d_wtgeo, (void **)&ctx->d_wsph, (void **)&ctx->d_wn_sph,
-                   (void **)&ctx->d_wn_geo, (void **)&ctx->d_wpos, (void **)&ctx->d_wrad};
+                   (void **)&ctx->d_wn_geo, (void **)&ctx->d_wpos, (void **)&ctx->d_wnrm, (void **)&ctx->d_wrad};
   for (auto b : bufs) dev_free(ctx, b);
   free_void(ctx);
   ctx->have_wall = false;
@@ -1274,9 +1329,9 @@ int repulsion_impl(zf_ctx *ctx) {
     size_t stmp = ctx->rsel_tmp_bytes;
     ZF_HIP(ctx, hipcub::DeviceSelect::Flagged(ctx->d_rsel_tmp, stmp, hipcub::CountingInputIterator<int32_t>(0),
                                               ctx->d_npair, ctx->d_over + 1, ctx->d_over, n, ctx->stream));
-    hipLaunchKernelGGL(k_wall_repulsion, dim3(std::min<unsigned>((nb(n, 128) + 7) / 8 * 8, 1024u)), dim3(128), 0,
+    hipLaunchKernelGGL(k_wall_repulsion, dim3(std::min<unsigned>((nb(n, 64) + 7) / 8 * 8, 16384u)), dim3(64), 0,
                        ctx->stream, ctx->d_x, NT, ctx->d_over, ctx->d_windex, ctx->d_wdist,
-                       ctx->d_wnode_tri, ctx->d_wtri, ctx->d_wxyz, ctx->d_wpos, ctx->d_wn_sph,
+                       ctx->d_wnode_tri, ctx->d_wtri, ctx->d_wxyz, ctx->d_wpos, ctx->d_wnrm, ctx->d_wn_sph,
                        ctx->d_wn_geo, ctx->d_wnormal, grid_of(ctx), r, ctx->d_vrep, ctx->d_near);
   }
   const int nkeys = ctx->ncell * ctx->nt;
@@ -1412,6 +1467,7 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
       (rc = dev_alloc(ctx, (void **)&ctx->d_wsph, sizeof(float4) * (size_t)nwt)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wn_sph, sizeof(float4) * 12 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wpos, sizeof(float4) * (size_t)nwn)) ||
+      (rc = dev_alloc(ctx, (void **)&ctx->d_wnrm, sizeof(float4) * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wn_geo, 8 * 144 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wrad, 8 * (size_t)nwn)) ||
       (rc = dev_alloc(ctx, (void **)&ctx->d_wnode_tri, 52 * (size_t)nwn)) ||
@@ -1428,7 +1484,7 @@ int zf_wall_setup(zf_ctx *ctx, int32_t nwn, const double *wall_xyz, int32_t nwt,
   hipLaunchKernelGGL(k_wall_geom, dim3(nb(nwt, 256)), dim3(256), 0, ctx->stream, ctx->d_wxyz,
                      ctx->d_wtri, nwt, ctx->d_wtgeo, ctx->d_wsph);
   hipLaunchKernelGGL(k_wall_pos, dim3(nb(nwn, 256)), dim3(256), 0, ctx->stream, ctx->d_wxyz, ctx->d_wrad,
-                     nwn, ctx->d_wpos);
+                     nwn, ctx->d_wnode_tri, ctx->d_wnormal, ctx->d_wtgeo, ctx->d_wpos, ctx->d_wnrm);
   hipLaunchKernelGGL(k_wall_node_recs, dim3(nb(12 * (size_t)nwn, 256)), dim3(256), 0, ctx->stream,
                      ctx->d_wnode_tri, nwn, ctx->d_wtgeo, ctx->d_wsph, ctx->d_wn_sph, ctx->d_wn_geo);
   hipLaunchKernelGGL(k_fill_u64, dim3(nb(ctx->box, 256)), dim3(256), 0, ctx->stream, key, ctx->box);

@@ -228,6 +228,7 @@ struct zf_ctx {
   float4 *d_wsph = nullptr;   // nwt: FP32 bounding sphere of each wall triangle
   float4 *d_wn_sph = nullptr; // 12 x nwn: the spheres of each wall node's triangle slots
   float4 *d_wpos = nullptr;   // nwn: FP32 position and wrad rounded up
+  float4 *d_wnrm = nullptr;   // nwn: FP32 unit normal and the largest offset of its points along it
   double *d_wn_geo = nullptr; // 144 x nwn: the 12-double records of the same slots
   double *d_wrad = nullptr;   // nwn: largest distance from a wall node to its triangles' vertices/centroids
   int32_t *d_windex = nullptr;    // box
