@@ -46,9 +46,12 @@ def engine(b, cells, tris, xyz_rest, periodic=(1, 0, 0), wall=None, store=STORE_
     return eng
 
 
-def test_cell_list_and_cell_repulsion_match_oracle():
-    print("case cell-cell, two cells 0.3 apart", flush=True)
-    b, cells, tris = two_cells(0.3, seed=2)
+@pytest.mark.parametrize("gap", [0.3, 0.1])
+def test_cell_list_and_cell_repulsion_match_oracle(gap):
+    """Two cells pressed together: the nearer, the more facing triangles a
+    node lists (past its 16 own slots into pool chunks at 0.1)."""
+    print(f"case cell-cell, two cells {gap} apart", flush=True)
+    b, cells, tris = two_cells(gap, seed=2)
     xyz_rest = geometry.rbc_mesh(radius=3.0, level=2)[0]
     eng = engine(b, cells, tris, xyz_rest)
     eng.membrane_forces()

@@ -277,13 +277,19 @@ __device__ __forceinline__ void rep_triangle(int q, int nt, int nn, size_t NT, c
   }
 }
 
-// Triangles listed per node by the walk (kernel k_cell_repulsion<1>); a
-// node with more goes through the whole walk again with the evaluation
-// (k_cell_repulsion<0>).  At 1200 cells of 642 nodes in the 15-segment
-// vessel pair (haematocrit 0.094) the most any node lists is 47.
-constexpr int kRepPairs = 48;
+// Triangles listed per node by the walk (kernel k_cell_repulsion<1>), in
+// walk order: the first kRepSlots in the node's own slots, the rest in
+// chunks of kRepChunk words taken from a pool (one atomic per chunk), each
+// holding kRepChunk - 1 triangles and, last, the next chunk's number.  A
+// node whose chunk the pool cannot supply goes through the whole walk
+// again with the evaluation (k_cell_repulsion<0>).  At 1200 cells of 642
+// nodes in the 15-segment vessel pair (haematocrit 0.094) 92 % of the nodes
+// list none and 1.4 % more than 16.
+constexpr int kRepSlots = 16;
+constexpr int kRepChunk = 32;
+constexpr int kRepPoolHead = 64;  // pool words: the chunk counter, padded
 // Nodes with listed triangles are appended to one of kRepClasses lists by
-// their count (1-4, 5-8, 9-16, 17-32, 33-48), so an evaluation wave holds
+// their count (1-4, 5-8, 9-16, 17-32, more), so an evaluation wave holds
 // nodes of similar cost.  Layout of the list buffer: kRepClasses counts,
 // padded to 64 words, then one region of NT entries per class.
 constexpr int kRepClasses = 5;
@@ -291,6 +297,9 @@ constexpr int kRepListHead = 64;
 __device__ __forceinline__ int rep_class(int np) {
   return np <= 4 ? 0 : np <= 8 ? 1 : np <= 16 ? 2 : np <= 32 ? 3 : 4;
 }
+// Chunks in the pool: one per 8 nodes (room for about 4 extra triangles per
+// node on average; 1.4 % of the nodes need one or two chunks at 1200 cells).
+inline int rep_pool_chunks(size_t NT) { return (int)std::max<size_t>(4096, NT / 8); }
 
 // Cell-cell repulsion, one lane per node of a released cell (status > 0).
 // The window is walked row by row (fixed i, j): DOMID is k-fastest and the
@@ -307,8 +316,9 @@ __device__ __forceinline__ void rep_walk(
     const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
     const int32_t *__restrict__ domid, const float4 *__restrict__ crec, const uint64_t *__restrict__ keys,
     const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds, Grid g,
-    RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ lists,
-    int32_t *__restrict__ over, double *__restrict__ vrep) {
+    RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ ochunk,
+    int32_t *__restrict__ pool, int pool_cap, int32_t *__restrict__ lists, int32_t *__restrict__ over,
+    double *__restrict__ vrep) {
   const int c = gq / nn, node = gq % nn;
   // released cells and clones (the reference's :9920-9921, :10055)
   if (status[c] == 0 || status[c] <= kStatusDead) {
@@ -360,7 +370,9 @@ __device__ __forceinline__ void rep_walk(
     }
     qlen = 0;
   };
-  int np = 0;  // mode 1: triangles listed
+  int np = 0;       // mode 1: triangles listed
+  int chunk = -1;   // mode 1: the pool chunk being filled
+  bool lost = false;  // mode 1: the pool ran out
   for (int i = lo[0]; i <= hi[0]; i++)
     for (int j = lo[1]; j <= hi[1]; j++) {
       const double qx = g.x0 + (i - 1) * g.dx - xc, qy = g.y0 + (j - 1) * g.dx - yc;
@@ -418,7 +430,22 @@ __device__ __forceinline__ void rep_walk(
             queue[qlen][threadIdx.x] = q;
             qlen++;
           } else {
-            if (np < kRepPairs) pairs[(size_t)gq * kRepPairs + np] = q;
+            if (np < kRepSlots) {
+              pairs[(size_t)gq * kRepSlots + np] = q;
+            } else {
+              const int at = (np - kRepSlots) % (kRepChunk - 1);
+              if (at == 0 && !lost) {
+                const int cn = atomicAdd(pool, 1);
+                if (cn >= pool_cap) {
+                  lost = true;
+                } else {
+                  if (chunk < 0) ochunk[gq] = cn;
+                  else pool[kRepPoolHead + (size_t)chunk * kRepChunk + kRepChunk - 1] = cn;
+                  chunk = cn;
+                }
+              }
+              if (!lost) pool[kRepPoolHead + (size_t)chunk * kRepChunk + at] = q;
+            }
             np++;
           }
         }
@@ -426,8 +453,8 @@ __device__ __forceinline__ void rep_walk(
     }
   if constexpr (MODE == 1) {
     npair[gq] = np;
-    if (np > kRepPairs) over[1 + atomicAdd(over, 1)] = gq;
-    const int cls = np >= 1 && np <= kRepPairs ? rep_class(np) : -1;
+    if (lost) over[1 + atomicAdd(over, 1)] = gq;
+    const int cls = np >= 1 && !lost ? rep_class(np) : -1;
     // one class per pass, so each atomic has one address across the wave
 #pragma unroll
     for (int k = 0; k < kRepClasses; k++)
@@ -447,30 +474,33 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
     const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
     const int32_t *__restrict__ domid, const float4 *__restrict__ crec, const uint64_t *__restrict__ keys,
     const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds, Grid g,
-    RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ lists,
-    int32_t *__restrict__ over, double *__restrict__ vrep) {
+    RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ ochunk,
+    int32_t *__restrict__ pool, int pool_cap, int32_t *__restrict__ lists, int32_t *__restrict__ over,
+    double *__restrict__ vrep) {
   if constexpr (MODE == 1) {
     const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (gq >= ncell * nn) return;
     rep_walk<MODE>(gq, X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off, bounds, g, r,
-                   npair, pairs, lists, over, vrep);
+                   npair, pairs, ochunk, pool, pool_cap, lists, over, vrep);
   } else {
     // the nodes whose passing triangles did not fit their list (over[0]
     // of them, listed from over[1] on)
     const int nover = over[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nover; i += gridDim.x * blockDim.x)
       rep_walk<MODE>(over[1 + i], X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off,
-                     bounds, g, r, npair, pairs, lists, over, vrep);
+                     bounds, g, r, npair, pairs, ochunk, pool, pool_cap, lists, over, vrep);
   }
 }
 
-// The listed triangles of every node that had 1 to kRepPairs, in the walk's
+// The listed triangles of every node in the class lists, in the walk's
 // order (the same sums as the walk's own evaluation).  Entry i of the class
 // lists taken heaviest class first, grid-stride.
 __global__ __launch_bounds__(128) void k_rep_eval(const double *__restrict__ X, size_t NT, int nn, int nt,
                                                   const int32_t *__restrict__ tri,
                                                   const int32_t *__restrict__ npair,
                                                   const int32_t *__restrict__ pairs,
+                                                  const int32_t *__restrict__ ochunk,
+                                                  const int32_t *__restrict__ pool,
                                                   const int32_t *__restrict__ lists, RepParams r,
                                                   double *__restrict__ vrep) {
   int cnt[kRepClasses], total = 0;
@@ -483,8 +513,19 @@ __global__ __launch_bounds__(128) void k_rep_eval(const double *__restrict__ X, 
     const int np = npair[gq];
     const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
     double v0 = 0, v1 = 0, v2 = 0;
-    for (int p = 0; p < np; p++)
-      rep_triangle(pairs[(size_t)gq * kRepPairs + p], nt, nn, NT, tri, X, xc, yc, zc, r, v0, v1, v2);
+    int chunk = 0;
+    for (int p = 0; p < np; p++) {
+      int q;
+      if (p < kRepSlots) {
+        q = pairs[(size_t)gq * kRepSlots + p];
+      } else {
+        const int at = (p - kRepSlots) % (kRepChunk - 1);
+        if (at == 0)
+          chunk = p == kRepSlots ? ochunk[gq] : pool[kRepPoolHead + (size_t)chunk * kRepChunk + kRepChunk - 1];
+        q = pool[kRepPoolHead + (size_t)chunk * kRepChunk + at];
+      }
+      rep_triangle(q, nt, nn, NT, tri, X, xc, yc, zc, r, v0, v1, v2);
+    }
     vrep[gq] += v0;
     vrep[NT + gq] += v1;
     vrep[2 * NT + gq] += v2;
@@ -1208,6 +1249,7 @@ void free_cells(zf_ctx *ctx) {
   void **bufs[] = {(void **)&ctx->d_tkeys[0], (void **)&ctx->d_tkeys[1], (void **)&ctx->d_crec,
                    (void **)&ctx->d_bin_cnt, (void **)&ctx->d_bin_off, &ctx->d_bin_scan_tmp,
                    (void **)&ctx->d_npair, (void **)&ctx->d_pairs, (void **)&ctx->d_over, (void **)&ctx->d_rep_lists, &ctx->d_rsel_tmp,
+                   (void **)&ctx->d_ochunk, (void **)&ctx->d_rep_pool,
                    (void **)&ctx->d_tsort_tmp,
                    (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near,
                    (void **)&ctx->d_rep_bounds};
@@ -1341,19 +1383,22 @@ int repulsion_impl(zf_ctx *ctx) {
   // walk and evaluation again for the nodes whose list overflowed
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_over, (size_t)1, 0);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_rep_lists, (size_t)kRepListHead, 0);
+  hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, ctx->d_rep_pool, (size_t)1, 0);
+  const int pool_cap = rep_pool_chunks(NT);
   hipLaunchKernelGGL(k_cell_repulsion<1>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
                      ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1], ctx->d_bin_off,
-                     ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs, ctx->d_rep_lists,
-                     ctx->d_over, ctx->d_vrep);
+                     ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
+                     ctx->d_rep_pool, pool_cap, ctx->d_rep_lists, ctx->d_over, ctx->d_vrep);
   hipLaunchKernelGGL(k_rep_eval, dim3(std::min<unsigned>(nb(n, 128), 2048u)), dim3(128), 0, ctx->stream, ctx->d_x,
-                     NT, ctx->nn, ctx->nt, ctx->d_tri, ctx->d_npair, ctx->d_pairs, ctx->d_rep_lists, r,
+                     NT, ctx->nn, ctx->nt, ctx->d_tri, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
+                     ctx->d_rep_pool, ctx->d_rep_lists, r,
                      ctx->d_vrep);
   hipLaunchKernelGGL(k_cell_repulsion<0>, dim3(std::min<unsigned>(nb(n, 128), 256u)), dim3(128), 0, ctx->stream,
                      ctx->d_x, NT, ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status,
                      ctx->d_nnormal, ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1],
                      ctx->d_bin_off, ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs,
-                     ctx->d_rep_lists, ctx->d_over, ctx->d_vrep);
+                     ctx->d_ochunk, ctx->d_rep_pool, pool_cap, ctx->d_rep_lists, ctx->d_over, ctx->d_vrep);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
 }
@@ -1392,7 +1437,10 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
       (r = dev_alloc(ctx, (void **)&ctx->d_bin_off, 4 * ((size_t)ctx->dommax + 1))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_near, 4 * NT)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_npair, 4 * NT)) ||
-      (r = dev_alloc(ctx, (void **)&ctx->d_pairs, 4 * (size_t)kRepPairs * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_pairs, 4 * (size_t)kRepSlots * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_ochunk, 4 * NT)) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_rep_pool,
+                     4 * (kRepPoolHead + (size_t)kRepChunk * rep_pool_chunks(NT)))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_over, 4 * (NT + 1))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_rep_lists, 4 * (kRepListHead + kRepClasses * NT))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_rep_bounds, 2 * sizeof(unsigned long long))))

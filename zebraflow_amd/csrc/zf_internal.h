@@ -210,7 +210,9 @@ struct zf_ctx {
   size_t bin_scan_tmp_bytes = 0;
   int32_t *d_near = nullptr;      // ncap*nn: chosen wall triangle (1-based) or 0
   int32_t *d_npair = nullptr;     // ncap*nn: passing triangles of each node (-1: not a live node)
-  int32_t *d_pairs = nullptr;     // ncap*nn x 48: the first of them, in walk order
+  int32_t *d_pairs = nullptr;     // ncap*nn x 16: the first of them, in walk order
+  int32_t *d_ochunk = nullptr;    // ncap*nn: first pool chunk of the rest
+  int32_t *d_rep_pool = nullptr;  // chunk counter, then chunks of 31 triangles + next chunk
   int32_t *d_over = nullptr;      // 1 + ncap*nn: count, then the nodes whose list overflowed
   int32_t *d_rep_lists = nullptr; // nodes with listed triangles, by count class
   void *d_rsel_tmp = nullptr;     // temporary storage of the near-wall node selection
