@@ -63,7 +63,7 @@ __global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double
                           const int32_t *__restrict__ status, const int32_t *__restrict__ b,
                           const int32_t *__restrict__ domid, Grid g, double cutoff, int dommax,
                           int32_t *__restrict__ tbin, uint64_t *__restrict__ keys,
-                          unsigned long long *__restrict__ bounds) {
+                          unsigned long long *__restrict__ bounds, uint32_t *__restrict__ row_bd) {
   const int q = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool live = q < ncell * nt;
   int bin = 0;
@@ -132,6 +132,14 @@ __global__ void k_tri_bin(const double *__restrict__ tg, size_t TT, const double
     if (mind < cutoff) {
       bin = domid[bq(g, bi, bj, bk)];
       bdist = mind;
+      // the largest centroid-to-bin distance of the bins of row (bi, bj),
+      // FP32 rounded up (the bits of non-negative floats order like them);
+      // read first, the atomic only when it raises the bound
+      float bf = (float)mind;
+      if ((double)bf < mind) bf = nextafterf(bf, INFINITY);
+      uint32_t *rb = &row_bd[(size_t)(bi - 1) * g.jm + (bj - 1)];
+      if (__float_as_uint(bf) > __hip_atomic_load(rb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(rb, __float_as_uint(bf));
     }
     const size_t b0 = (size_t)(q / nt) * nn;
     const int t = q % nt;
@@ -315,7 +323,8 @@ __device__ __forceinline__ void rep_walk(
     const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
     const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
     const int32_t *__restrict__ domid, const float4 *__restrict__ crec, const uint64_t *__restrict__ keys,
-    const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds, Grid g,
+    const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds,
+    const uint32_t *__restrict__ row_bd, Grid g,
     RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ ochunk,
     int32_t *__restrict__ pool, int pool_cap, int32_t *__restrict__ lists, int32_t *__restrict__ over,
     double *__restrict__ vrep) {
@@ -382,6 +391,13 @@ __device__ __forceinline__ void rep_walk(
       int ov[9];
 #pragma unroll
       for (int t = 0; t < 9; t++) ov[t] = domid[row + (size_t)(min(lo[2] + t, hi[2]) - 1)];
+      // the bins of this row hold triangles within rbd of them (the row's
+      // own bound, at most the list's): the reach of this row's bins
+      double rbreach2 = breach2;
+      if (r.use_bounds) {
+        const double rb = treach + (double)__uint_as_float(row_bd[(size_t)(i - 1) * g.jm + (j - 1)]) + kReachSlack;
+        rbreach2 = fmin(breach2, rb * rb);
+      }
       // the bins in reach are a k-interval of the row; its listed nodes
       // have consecutive DOMIDs (k is the fastest index of the list), so
       // their triangles are the sorted keys from the first one's offset to
@@ -391,7 +407,7 @@ __device__ __forceinline__ void rep_walk(
       for (int t = 0; t < 9; t++) {
         const int k = lo[2] + t;
         const double qz = g.z0 + (k - 1) * g.dx - zc;
-        const bool ok = k <= hi[2] && rij + qz * qz < breach2 && ov[t] > 0;
+        const bool ok = k <= hi[2] && rij + qz * qz < rbreach2 && ov[t] > 0;
         dfirst = (ok && dlast < 0) ? ov[t] : dfirst;
         dlast = ok ? ov[t] : dlast;
       }
@@ -473,14 +489,15 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
     const int32_t *__restrict__ tri, const int32_t *__restrict__ status,
     const double *__restrict__ nnormal, const double *__restrict__ tg, size_t TT,
     const int32_t *__restrict__ domid, const float4 *__restrict__ crec, const uint64_t *__restrict__ keys,
-    const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds, Grid g,
+    const int32_t *__restrict__ bin_off, const unsigned long long *__restrict__ bounds,
+    const uint32_t *__restrict__ row_bd, Grid g,
     RepParams r, int32_t *__restrict__ npair, int32_t *__restrict__ pairs, int32_t *__restrict__ ochunk,
     int32_t *__restrict__ pool, int pool_cap, int32_t *__restrict__ lists, int32_t *__restrict__ over,
     double *__restrict__ vrep) {
   if constexpr (MODE == 1) {
     const int gq = xcd_range_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (gq >= ncell * nn) return;
-    rep_walk<MODE>(gq, X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off, bounds, g, r,
+    rep_walk<MODE>(gq, X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off, bounds, row_bd, g, r,
                    npair, pairs, ochunk, pool, pool_cap, lists, over, vrep);
   } else {
     // the nodes whose passing triangles did not fit their list (over[0]
@@ -488,7 +505,7 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
     const int nover = over[0];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nover; i += gridDim.x * blockDim.x)
       rep_walk<MODE>(over[1 + i], X, NT, nn, nt, tri, status, nnormal, tg, TT, domid, crec, keys, bin_off,
-                     bounds, g, r, npair, pairs, ochunk, pool, pool_cap, lists, over, vrep);
+                     bounds, row_bd, g, r, npair, pairs, ochunk, pool, pool_cap, lists, over, vrep);
   }
 }
 
@@ -1252,7 +1269,7 @@ void free_cells(zf_ctx *ctx) {
                    (void **)&ctx->d_ochunk, (void **)&ctx->d_rep_pool,
                    (void **)&ctx->d_tsort_tmp,
                    (void **)&ctx->d_tbin, (void **)&ctx->d_bin_range, (void **)&ctx->d_near,
-                   (void **)&ctx->d_rep_bounds};
+                   (void **)&ctx->d_rep_bounds, (void **)&ctx->d_row_bd};
   for (auto b : bufs) dev_free(ctx, b);
   free_wall(ctx);
   ctx->have_rep = false;
@@ -1279,11 +1296,14 @@ int cell_list_impl(zf_ctx *ctx) {
   if (n == 0) return bin_offsets(ctx);
   hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)ctx->d_rep_bounds,
                      (size_t)4, 0);
+  const size_t nrows = (size_t)ctx->cfg.im * ctx->cfg.jm;
+  hipLaunchKernelGGL(k_fill, dim3(nb(nrows, 256)), dim3(256), 0, ctx->stream, (int32_t *)ctx->d_row_bd,
+                     nrows, 0);
   const size_t NT = ncap * ctx->nn;
   hipLaunchKernelGGL(k_tri_bin, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_tgeo, TT,
                      ctx->d_x, NT, ctx->nn, ctx->d_tri, ctx->nt, ctx->ncell, ctx->d_status,
                      ctx->d_b, ctx->d_domid, grid_of(ctx), ctx->rcfg.bin_cutoff, ctx->dommax,
-                     ctx->d_tbin, ctx->d_tkeys[0], ctx->d_rep_bounds);
+                     ctx->d_tbin, ctx->d_tkeys[0], ctx->d_rep_bounds, ctx->d_row_bd);
   ctx->bins_gen = ctx->geom_gen;
   int bits = 1;
   while ((1ull << bits) <= (uint64_t)ctx->dommax) bits++;
@@ -1388,7 +1408,7 @@ int repulsion_impl(zf_ctx *ctx) {
   hipLaunchKernelGGL(k_cell_repulsion<1>, dim3(nb(n, 128)), dim3(128), 0, ctx->stream, ctx->d_x, NT,
                      ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status, ctx->d_nnormal,
                      ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1], ctx->d_bin_off,
-                     ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
+                     ctx->d_rep_bounds, ctx->d_row_bd, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
                      ctx->d_rep_pool, pool_cap, ctx->d_rep_lists, ctx->d_over, ctx->d_vrep);
   hipLaunchKernelGGL(k_rep_eval, dim3(std::min<unsigned>(nb(n, 128), 2048u)), dim3(128), 0, ctx->stream, ctx->d_x,
                      NT, ctx->nn, ctx->nt, ctx->d_tri, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
@@ -1397,7 +1417,7 @@ int repulsion_impl(zf_ctx *ctx) {
   hipLaunchKernelGGL(k_cell_repulsion<0>, dim3(std::min<unsigned>(nb(n, 128), 256u)), dim3(128), 0, ctx->stream,
                      ctx->d_x, NT, ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status,
                      ctx->d_nnormal, ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1],
-                     ctx->d_bin_off, ctx->d_rep_bounds, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs,
+                     ctx->d_bin_off, ctx->d_rep_bounds, ctx->d_row_bd, grid_of(ctx), r, ctx->d_npair, ctx->d_pairs,
                      ctx->d_ochunk, ctx->d_rep_pool, pool_cap, ctx->d_rep_lists, ctx->d_over, ctx->d_vrep);
   ZF_HIP(ctx, hipGetLastError());
   return ZF_OK;
@@ -1443,6 +1463,7 @@ int zf_repulsion_setup(zf_ctx *ctx, const zf_repulsion_config *rc) {
                      4 * (kRepPoolHead + (size_t)kRepChunk * rep_pool_chunks(NT)))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_over, 4 * (NT + 1))) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_rep_lists, 4 * (kRepListHead + kRepClasses * NT))) ||
+      (r = dev_alloc(ctx, (void **)&ctx->d_row_bd, 4 * (size_t)ctx->cfg.im * ctx->cfg.jm)) ||
       (r = dev_alloc(ctx, (void **)&ctx->d_rep_bounds, 2 * sizeof(unsigned long long))))
     return r;
   ctx->bins_gen = ~0ull;

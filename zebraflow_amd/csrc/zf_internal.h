@@ -206,6 +206,7 @@ struct zf_ctx {
   int32_t *d_bin_range = nullptr; // 2 x dommax: [begin, end) in the sorted keys
   int32_t *d_bin_cnt = nullptr;   // dommax + 1: keys per bin (scan input)
   int32_t *d_bin_off = nullptr;   // dommax + 1: first key of each bin, then the key count
+  uint32_t *d_row_bd = nullptr;   // im*jm: largest centroid-to-bin distance per (i, j) row, FP32 bits
   void *d_bin_scan_tmp = nullptr;
   size_t bin_scan_tmp_bytes = 0;
   int32_t *d_near = nullptr;      // ncap*nn: chosen wall triangle (1-based) or 0
