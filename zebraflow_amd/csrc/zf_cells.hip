@@ -285,6 +285,39 @@ __device__ __forceinline__ void rep_triangle(int q, int nt, int nn, size_t NT, c
   }
 }
 
+// The terms of rep_triangle, unsummed: term sc of each component, +0 where
+// sub-centroid sc is beyond the cut-off.
+__device__ __forceinline__ void rep_terms(int q, int nt, int nn, size_t NT, const int32_t *__restrict__ tri,
+                                              const double *__restrict__ X, double xc, double yc, double zc,
+                                              const RepParams &r, double *t0, double *t1, double *t2) {
+  const int m = q / nt, t = q % nt;
+  const int n1 = tri[3 * t], n2 = tri[3 * t + 1], n3 = tri[3 * t + 2];
+  const size_t b0 = (size_t)m * nn;
+  const double ax = X[b0 + n1], ay = X[NT + b0 + n1], az = X[2 * NT + b0 + n1];
+  const double bx = X[b0 + n2], by = X[NT + b0 + n2], bz = X[2 * NT + b0 + n2];
+  const double cx = X[b0 + n3], cy = X[NT + b0 + n3], cz = X[2 * NT + b0 + n3];
+#pragma unroll 2
+  for (int sc = 0; sc < 16; sc++) {
+    const double wa = (double)((kSubWA >> (4 * sc)) & 15u);
+    const double wb = (double)((kSubWB >> (4 * sc)) & 15u);
+    const double wc = (double)((kSubWC >> (4 * sc)) & 15u);
+    const double px = (wa * ax + wb * bx + wc * cx) * (1.0 / 12.0) - xc;
+    const double py = (wa * ay + wb * by + wc * cy) * (1.0 / 12.0) - yc;
+    const double pz = (wa * az + wb * bz + wc * cz) * (1.0 / 12.0) - zc;
+    const double d = sqrt(px * px + py * py + pz * pz);
+    double a = 0, b = 0, c = 0;
+    if (d < r.cutoff) {
+      const double f = -r.k_cell * (r.cutoff - d) / r.cutoff;
+      a = f * px / d;
+      b = f * py / d;
+      c = f * pz / d;
+    }
+    t0[sc * 64] = a;
+    t1[sc * 64] = b;
+    t2[sc * 64] = c;
+  }
+}
+
 // Triangles listed per node by the walk (kernel k_cell_repulsion<1>), in
 // walk order: the first kRepSlots in the node's own slots, the rest in
 // chunks of kRepChunk words taken from a pool (one atomic per chunk), each
@@ -509,9 +542,73 @@ __global__ __launch_bounds__(128) void k_cell_repulsion(
   }
 }
 
-// The listed triangles of every node in the class lists, in the walk's
-// order (the same sums as the walk's own evaluation).  Entry i of the class
-// lists taken heaviest class first, grid-stride.
+// The listed triangle p of node gq (its slots, then its pool chunks).
+__device__ __forceinline__ int rep_pair_at(int gq, int p, const int32_t *__restrict__ pairs,
+                                           const int32_t *__restrict__ ochunk, const int32_t *__restrict__ pool) {
+  if (p < kRepSlots) return pairs[(size_t)gq * kRepSlots + p];
+  int chunk = ochunk[gq];
+  for (int h = (p - kRepSlots) / (kRepChunk - 1); h > 0; h--)
+    chunk = pool[kRepPoolHead + (size_t)chunk * kRepChunk + kRepChunk - 1];
+  return pool[kRepPoolHead + (size_t)chunk * kRepChunk + (p - kRepSlots) % (kRepChunk - 1)];
+}
+
+// The nodes of the two heaviest classes (more than 16 listed triangles),
+// one wave per node: lane p forms the 16 terms of triangle p (64 at a
+// time) in LDS, and lanes 0, 1, 2 add those of one component each in walk
+// order, term by term, as the per-node evaluation does, so that a node's
+// triangles are gathered together instead of one after another.  A term
+// beyond the cut-off is stored as +0 and added: the sums start at +0 and a
+// sum of doubles is -0 only if both addends are, so adding +0 leaves every
+// partial sum as it is and the sums are the per-node evaluation's, bit for
+// bit.
+constexpr int kRepWideClass = 3;
+__global__ __launch_bounds__(64) void k_rep_eval_wide(const double *__restrict__ X, size_t NT, int nn, int nt,
+                                                      const int32_t *__restrict__ tri,
+                                                      const int32_t *__restrict__ npair,
+                                                      const int32_t *__restrict__ pairs,
+                                                      const int32_t *__restrict__ ochunk,
+                                                      const int32_t *__restrict__ pool,
+                                                      const int32_t *__restrict__ lists, RepParams r,
+                                                      double *__restrict__ vrep) {
+  __shared__ double t0[16 * 64], t1[16 * 64], t2[16 * 64];  // [term][lane]
+  const int lane = threadIdx.x;
+  int cnt[kRepClasses], total = 0;
+#pragma unroll
+  for (int k = kRepWideClass; k < kRepClasses; k++) total += (cnt[k] = lists[k]);
+  for (int i = blockIdx.x; i < total; i += gridDim.x) {
+    int k = kRepClasses - 1, at = i;
+    while (at >= cnt[k]) at -= cnt[k--];
+    const int gq = lists[kRepListHead + (size_t)k * NT + at];
+    const int np = npair[gq];
+    const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
+    double v0 = 0;
+    for (int p0 = 0; p0 < np; p0 += 64) {
+      const int p = p0 + lane;
+      if (p < np)
+        rep_terms(rep_pair_at(gq, p, pairs, ochunk, pool), nt, nn, NT, tri, X, xc, yc, zc, r, t0 + lane,
+                  t1 + lane, t2 + lane);
+      __syncthreads();
+      if (lane < 3) {
+        const double *tc = lane == 0 ? t0 : lane == 1 ? t1 : t2;
+        const int n = min(64, np - p0);
+        for (int e = 0; e < n; e++) {
+          double tv[16];
+#pragma unroll
+          for (int sc = 0; sc < 16; sc++) tv[sc] = tc[sc * 64 + e];
+#pragma unroll
+          for (int sc = 0; sc < 16; sc++) v0 += tv[sc];
+        }
+      }
+      __syncthreads();
+    }
+    // lane c holds component c
+    if (lane < 3) vrep[(size_t)lane * NT + gq] += v0;
+  }
+}
+
+// The listed triangles of every node in the lighter class lists, in the
+// walk's order (the same sums as the walk's own evaluation).  Entry i of
+// the class lists taken heaviest class first, grid-stride.
 __global__ __launch_bounds__(128) void k_rep_eval(const double *__restrict__ X, size_t NT, int nn, int nt,
                                                   const int32_t *__restrict__ tri,
                                                   const int32_t *__restrict__ npair,
@@ -520,11 +617,11 @@ __global__ __launch_bounds__(128) void k_rep_eval(const double *__restrict__ X, 
                                                   const int32_t *__restrict__ pool,
                                                   const int32_t *__restrict__ lists, RepParams r,
                                                   double *__restrict__ vrep) {
-  int cnt[kRepClasses], total = 0;
+  int cnt[kRepWideClass], total = 0;
 #pragma unroll
-  for (int k = 0; k < kRepClasses; k++) total += (cnt[k] = lists[k]);
+  for (int k = 0; k < kRepWideClass; k++) total += (cnt[k] = lists[k]);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-    int k = kRepClasses - 1, at = i;
+    int k = kRepWideClass - 1, at = i;
     while (at >= cnt[k]) at -= cnt[k--];
     const int gq = lists[kRepListHead + (size_t)k * NT + at];
     const int np = npair[gq];
@@ -1414,6 +1511,9 @@ int repulsion_impl(zf_ctx *ctx) {
                      NT, ctx->nn, ctx->nt, ctx->d_tri, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
                      ctx->d_rep_pool, ctx->d_rep_lists, r,
                      ctx->d_vrep);
+  hipLaunchKernelGGL(k_rep_eval_wide, dim3(std::min<unsigned>(nb(n, 64), 4096u)), dim3(64), 0, ctx->stream,
+                     ctx->d_x, NT, ctx->nn, ctx->nt, ctx->d_tri, ctx->d_npair, ctx->d_pairs, ctx->d_ochunk,
+                     ctx->d_rep_pool, ctx->d_rep_lists, r, ctx->d_vrep);
   hipLaunchKernelGGL(k_cell_repulsion<0>, dim3(std::min<unsigned>(nb(n, 128), 256u)), dim3(128), 0, ctx->stream,
                      ctx->d_x, NT, ctx->nn, ctx->nt, ctx->ncell, ctx->d_tri, ctx->d_status,
                      ctx->d_nnormal, ctx->d_tgeo, TT, ctx->d_domid, ctx->d_crec, ctx->d_tkeys[1],
