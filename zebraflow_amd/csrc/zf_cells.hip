@@ -1405,8 +1405,11 @@ int cell_list_impl(zf_ctx *ctx) {
   int bits = 1;
   while ((1ull << bits) <= (uint64_t)ctx->dommax) bits++;
   size_t tmp = ctx->tsort_tmp_bytes;
+  // k_tri_bin writes the key of triangle q at position q, so the keys come
+  // in triangle order and a (stable) radix sort of the bin bits alone gives
+  // the (bin, triangle) order
   ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_tsort_tmp, tmp, ctx->d_tkeys[0],
-                                                  ctx->d_tkeys[1], n, 0, 32 + bits, ctx->stream));
+                                                  ctx->d_tkeys[1], n, 32, 32 + bits, ctx->stream));
   hipLaunchKernelGGL(k_bin_ranges, dim3(nb(n, 256)), dim3(256), 0, ctx->stream, ctx->d_tkeys[1], n,
                      (uint64_t)ctx->dommax, ctx->d_bin_range);
   return bin_offsets(ctx);

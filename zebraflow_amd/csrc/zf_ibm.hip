@@ -709,15 +709,17 @@ int ibm_interpolate_impl(zf_ctx *ctx) {
 }
 
 // Sorts the nkeys spreading keys (bin << 32 | node) and builds the bins'
-// runs, the occupied tiles and the list of tiles to gather.
-int spread_bins(zf_ctx *ctx, int nkeys, uint64_t ntiles) {
+// runs, the occupied tiles and the list of tiles to gather.  in_order: the
+// keys come in node order, so a (stable) radix sort of the bin bits alone
+// gives the (bin, node) order.
+int spread_bins(zf_ctx *ctx, int nkeys, uint64_t ntiles, bool in_order) {
   const uint64_t nbins = 8 * ntiles;
   // end bit: bins need ceil(log2(nbins+1)) bits above the 32 node bits
   int bbits = 1;
   while ((1ull << bbits) <= nbins) bbits++;
   size_t tmp = ctx->sort_tmp_bytes;
   ZF_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(ctx->d_sort_tmp, tmp, ctx->d_keys[0], ctx->d_keys[1],
-                                                  nkeys, 0, 32 + bbits, ctx->stream));
+                                                  nkeys, in_order ? 32 : 0, 32 + bbits, ctx->stream));
   hipLaunchKernelGGL(k_fill_i32, dim3(nb(2 * nbins, 256)), dim3(256), 0, ctx->stream, ctx->d_sbin_range,
                      (size_t)(2 * nbins), 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(nb(ntiles, 256)), dim3(256), 0, ctx->stream, ctx->d_tile_occ,
@@ -777,7 +779,8 @@ int ibm_spread_impl(zf_ctx *ctx) {
                          (uint32_t)NT, ctx->d_band);
   }
   const int nkeys = n + nnuc_pts;
-  if ((rc = spread_bins(ctx, nkeys, ntiles))) return rc;
+  // membrane keys by node, then the nucleus keys by node from NT on
+  if ((rc = spread_bins(ctx, nkeys, ntiles, true))) return rc;
   const unsigned grid = spread_grid(ntiles);
   if (f64)
     hipLaunchKernelGGL(k_spread_gather<double>, dim3(grid), dim3(64), 0, ctx->stream, ctx->d_keys[1],
@@ -862,7 +865,7 @@ static int wrap_spread_typed(zf_ctx *ctx) {
                        ctx->d_vn, ctx->d_vrepn, ctx->d_fn, NTn, ctx->nnuc, ctx->ncell, ctx->d_status, p,
                        w, static_cast<F *>(ctx->d_ffln), k0 + n, k0 + half + n, (uint32_t)NT);
   const int nkeys = 2 * half;
-  int rc = spread_bins(ctx, nkeys, ntiles);
+  int rc = spread_bins(ctx, nkeys, ntiles, false);  // images after the originals: full sort
   if (rc) return rc;
   hipLaunchKernelGGL((k_spread_gather<F, true>), dim3(spread_grid(ntiles)), dim3(64), 0, ctx->stream,
                      ctx->d_keys[1], ctx->d_sbin_range, ctx->d_spread_tiles, ctx->d_x,
