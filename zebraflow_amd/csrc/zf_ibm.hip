@@ -106,11 +106,9 @@ __global__ __launch_bounds__(128) void k_interp(const double *__restrict__ X, si
             jlo = (int)ceil((yp - p.y0) / p.dx + 1.0 - 2.0 - 1e-9),
             klo = (int)ceil((zp - p.z0) / p.dx + 1.0 - 2.0 - 1e-9);
   const double width = 2.0 * p.dx, w2 = width * width;
-  double dyv[5], kyv[5], dzv[5], kzv[5];
+  double dzv[5], kzv[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) {
-    dyv[t] = p.y0 + (jlo + t - 1) * p.dx - yp;
-    kyv[t] = kern1(dyv[t], width);
     dzv[t] = p.z0 + (klo + t - 1) * p.dx - zp;
     kzv[t] = kern1(dzv[t], width);
   }
@@ -121,9 +119,10 @@ __global__ __launch_bounds__(128) void k_interp(const double *__restrict__ X, si
     const double dxv = p.x0 + (ii - 1) * p.dx - xp;
     const double kxv = kern1(dxv, width);
     const bool iok = ii >= i1 && ii <= i2;
-#pragma unroll
+#pragma unroll 1
     for (int c = 0; c < 5; c++) {
       const int jj = jlo + c;
+      const double dyc = p.y0 + (jj - 1) * p.dx - yp;
       bool ok[5];
       size_t q[5];
       int bv[5];
@@ -131,12 +130,13 @@ __global__ __launch_bounds__(128) void k_interp(const double *__restrict__ X, si
       for (int t = 0; t < 5; t++) {
         const int kk = klo + t;
         ok[t] = iok && jj >= j1 && jj <= j2 && kk >= k1 && kk <= k2 &&
-                !(p.cube ? (fabs(dxv) >= width || fabs(dyv[c]) >= width || fabs(dzv[t]) >= width)
-                         : dxv * dxv + dyv[c] * dyv[c] + dzv[t] * dzv[t] > w2);
+                !(p.cube ? (fabs(dxv) >= width || fabs(dyc) >= width || fabs(dzv[t]) >= width)
+                         : dxv * dxv + dyc * dyc + dzv[t] * dzv[t] > w2);
         q[t] = ok[t] ? ((size_t)(ii - 1) * p.jm + (jj - 1)) * p.km + (kk - 1) : 0;
         if (ok[t]) bv[t] = b[q[t]];
       }
       if (!__any(ok[0] || ok[1] || ok[2] || ok[3] || ok[4])) continue;
+      const double kyc = kern1(dyc, width);
       bool fl[5];
       int sv[5];
 #pragma unroll
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(128) void k_interp(const double *__restrict__ X, si
 #pragma unroll
       for (int t = 0; t < 5; t++) {
         if (!ok[t] || sv[t] < 0) continue;
-        const double w = kxv * kyv[c] * kzv[t] / 64.0;
+        const double w = kxv * kyc * kzv[t] / 64.0;
         v0 += w * p.vel_scale * u0[t];
         v1 += w * p.vel_scale * u1[t];
         v2 += w * p.vel_scale * u2[t];
