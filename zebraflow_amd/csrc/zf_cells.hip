@@ -695,7 +695,7 @@ __global__ __launch_bounds__(256) void k_wall_select(const double *__restrict__ 
 // A wall node repeated along the walk is listed again; its points come
 // later with the same distances and never win.
 constexpr int kWallList = 729;  // at most 9 entries per row of the 9 x 9 rows
-__global__ __launch_bounds__(64) void k_wall_repulsion(const double *__restrict__ X, size_t NT,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_wall_repulsion(const double *__restrict__ X, size_t NT,
                                  const int32_t *__restrict__ list, const int32_t *__restrict__ windex,
                                  const float *__restrict__ wdist,
                                  const int32_t *__restrict__ wnode_tri, const int32_t *__restrict__ wtri,
@@ -734,9 +734,9 @@ __global__ __launch_bounds__(64) void k_wall_repulsion(const double *__restrict_
     int ne = 0;
     for (int r0 = 0; r0 < nrow; r0 += 64) {
       const int rw = r0 + lane;
-      int cw[9], co[9];
-      float cl[9];
-      int cnt = 0;
+      int wv[9];
+      float lbt[9];
+      unsigned keep = 0;  // the row's slots listed
       if (rw < nrow) {
         const int i = lo[0] + rw / nj, j = lo[1] + rw % nj;
         const double qx = g.x0 + (i - 1) * g.dx - x, qy = g.y0 + (j - 1) * g.dx - y;
@@ -744,7 +744,6 @@ __global__ __launch_bounds__(64) void k_wall_repulsion(const double *__restrict_
         // the row's 9 W_index entries and their distances in one batch
         // (in-bounds addresses; entries past hi[2] are not used)
         const size_t row = bq(g, i, j, 1);
-        int wv[9];
         float wd[9];
 #pragma unroll
         for (int t = 0; t < 9; t++) {
@@ -793,26 +792,26 @@ __global__ __launch_bounds__(64) void k_wall_repulsion(const double *__restrict_
           if (lbf >= search_f) continue;
           // a node with triangles (h >= 0) is one of their vertices
           if (wnr[t].w >= 0.f) ub = fminf(ub, dist + smargin);
-          cw[cnt] = wv[t];
-          cl[cnt] = lbf;
-          co[cnt] = (rw * 9 + t) * 64;  // window position, then 4 (slot + 1) + point
-          cnt++;
+          lbt[t] = lbf;
+          keep |= 1u << t;
         }
       }
       // the entries go to the list in row order
+      const int cnt = __popc(keep);
       int incl = cnt;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const int v = __shfl_up(incl, d);
         incl += lane >= d ? v : 0;
       }
-      const int at = ne + incl - cnt;
+      int at = ne + incl - cnt;
 #pragma unroll
-      for (int c = 0; c < 9; c++)
-        if (c < cnt) {
-          ewn[at + c] = cw[c];
-          elb[at + c] = cl[c];
-          eord[at + c] = co[c];
+      for (int t = 0; t < 9; t++)
+        if (keep >> t & 1u) {
+          ewn[at] = wv[t];
+          elb[at] = lbt[t];
+          eord[at] = (rw * 9 + t) * 64;  // window position, then 4 (slot + 1) + point
+          at++;
         }
       ne += __shfl(incl, 63);
     }
