@@ -69,3 +69,43 @@ def test_points_outside_the_block_are_at_least_two_dx_away():
             if all(pi in b for pi, b in zip(p, block)):
                 continue
             assert np.linalg.norm(np.array(p) - x) >= 2.0 - 1e-12
+
+
+def test_lexicographic_least_equals_the_in_order_strict_scan():
+    """The wall search (k_wall_repulsion) keeps the least (distance, visit
+    order) of the points of the listed wall nodes, evaluated in any order,
+    with a wall node repeated along the walk listed again; the reference
+    scans the walk in order, keeping a point only when strictly nearer and
+    skipping a wall node equal to the one just seen.  Both pick the same
+    triangle, ties of equal distances included."""
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        nwall = 12
+        # each wall node: up to 12 triangles of 4 points, distances drawn
+        # from a small set so that ties are frequent
+        tris = {w: [(int(rng.integers(1, 50)), rng.choice([0.5, 0.75, 1.0, 1.25, 2.0], 4))
+                    for _ in range(int(rng.integers(1, 13)))] for w in range(1, nwall + 1)}
+        walk = rng.integers(0, nwall + 1, 60)  # W_index entries along the window (0: none)
+        search = 1.5
+        # reference: in order, strict <, consecutive repeats skipped
+        mind, win, last = search, 0, 0
+        for wn in walk:
+            if wn <= 0 or wn == last:
+                continue
+            last = wn
+            for tw, ds in tris[wn]:
+                for d in ds:
+                    if d < mind:
+                        mind, win = d, tw
+        # lexicographic least over every listed occurrence, shuffled
+        cands = []
+        for pos, wn in enumerate(walk):
+            if wn <= 0:
+                continue
+            for l, (tw, ds) in enumerate(tris[wn]):
+                for sv, d in enumerate(ds):
+                    if d < search:
+                        cands.append((d, pos * 64 + 4 * (l + 1) + sv, tw))
+        rng.shuffle(cands)
+        best = min(cands, key=lambda c: (c[0], c[1])) if cands else None
+        assert (best[2] if best else 0) == win
