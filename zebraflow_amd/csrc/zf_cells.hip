@@ -285,8 +285,9 @@ __device__ __forceinline__ void rep_triangle(int q, int nt, int nn, size_t NT, c
   }
 }
 
-// The terms of rep_triangle, unsummed: term sc of each component, +0 where
-// sub-centroid sc is beyond the cut-off.
+// The terms of rep_triangle, unsummed: term sc of each component at
+// t[sc * STRIDE], +0 where sub-centroid sc is beyond the cut-off.
+template <int STRIDE>
 __device__ __forceinline__ void rep_terms(int q, int nt, int nn, size_t NT, const int32_t *__restrict__ tri,
                                               const double *__restrict__ X, double xc, double yc, double zc,
                                               const RepParams &r, double *t0, double *t1, double *t2) {
@@ -312,9 +313,9 @@ __device__ __forceinline__ void rep_terms(int q, int nt, int nn, size_t NT, cons
       b = f * py / d;
       c = f * pz / d;
     }
-    t0[sc * 64] = a;
-    t1[sc * 64] = b;
-    t2[sc * 64] = c;
+    t0[sc * STRIDE] = a;
+    t1[sc * STRIDE] = b;
+    t2[sc * STRIDE] = c;
   }
 }
 
@@ -553,7 +554,7 @@ __device__ __forceinline__ int rep_pair_at(int gq, int p, const int32_t *__restr
 }
 
 // The nodes of the two heaviest classes (more than 16 listed triangles),
-// one wave per node: lane p forms the 16 terms of triangle p (64 at a
+// one wave per node: lane p forms the 16 terms of triangle p (32 at a
 // time) in LDS, and lanes 0, 1, 2 add those of one component each in walk
 // order, term by term, as the per-node evaluation does, so that a node's
 // triangles are gathered together instead of one after another.  A term
@@ -570,7 +571,8 @@ __global__ __launch_bounds__(64) void k_rep_eval_wide(const double *__restrict__
                                                       const int32_t *__restrict__ pool,
                                                       const int32_t *__restrict__ lists, RepParams r,
                                                       double *__restrict__ vrep) {
-  __shared__ double t0[16 * 64], t1[16 * 64], t2[16 * 64];  // [term][lane]
+  constexpr int B = 32;  // triangles per batch (LDS 12 KB per wave)
+  __shared__ double t0[16 * B], t1[16 * B], t2[16 * B];  // [term][lane]
   const int lane = threadIdx.x;
   int cnt[kRepClasses], total = 0;
 #pragma unroll
@@ -582,19 +584,19 @@ __global__ __launch_bounds__(64) void k_rep_eval_wide(const double *__restrict__
     const int np = npair[gq];
     const double xc = X[gq], yc = X[NT + gq], zc = X[2 * NT + gq];
     double v0 = 0;
-    for (int p0 = 0; p0 < np; p0 += 64) {
+    for (int p0 = 0; p0 < np; p0 += B) {
       const int p = p0 + lane;
-      if (p < np)
-        rep_terms(rep_pair_at(gq, p, pairs, ochunk, pool), nt, nn, NT, tri, X, xc, yc, zc, r, t0 + lane,
-                  t1 + lane, t2 + lane);
+      if (lane < B && p < np)
+        rep_terms<B>(rep_pair_at(gq, p, pairs, ochunk, pool), nt, nn, NT, tri, X, xc, yc, zc, r, t0 + lane,
+                     t1 + lane, t2 + lane);
       __syncthreads();
       if (lane < 3) {
         const double *tc = lane == 0 ? t0 : lane == 1 ? t1 : t2;
-        const int n = min(64, np - p0);
+        const int n = min(B, np - p0);
         for (int e = 0; e < n; e++) {
           double tv[16];
 #pragma unroll
-          for (int sc = 0; sc < 16; sc++) tv[sc] = tc[sc * 64 + e];
+          for (int sc = 0; sc < 16; sc++) tv[sc] = tc[sc * B + e];
 #pragma unroll
           for (int sc = 0; sc < 16; sc++) v0 += tv[sc];
         }
